@@ -1,0 +1,157 @@
+/*
+ * petar_hip.h -- C ABI of libpetar_hip.so, the MI355X (gfx950) soft-force
+ * path of PeTar: FDPS multi-walk tree force with linear cutoff, i.e.
+ * particle-particle EPJ terms, quadrupole SPJ terms and the neighbour count
+ * within r_search.
+ *
+ * Entry points and the reference interface each one replaces:
+ *
+ *   pth_create / pth_destroy
+ *       first-call initialisation of CalcForceWithLinearCutoffCUDAMultiWalk
+ *       (src/force_gpu_cuda.cu:547-571: device = rank % n_devices, fixed
+ *       buffers) and the file-static buffers of src/force_gpu_cuda.cu:513-533.
+ *   pth_send_j
+ *       CalcForceWithLinearCutoffCUDAMultiWalk::operator() with
+ *       send_flag == true (src/force_gpu_cuda.cu:577-620; declared at
+ *       src/force_gpu_cuda.hpp:120-132): all EPJ and SPJ of the force call.
+ *   pth_dispatch
+ *       the same operator() with send_flag == false
+ *       (src/force_gpu_cuda.cu:621-699): one batch of walks with their
+ *       index lists; returns before the kernels finish.
+ *   pth_retrieve
+ *       RetrieveForceCUDA (src/force_gpu_cuda.cu:831-880; declared at
+ *       src/force_gpu_cuda.hpp:165-168): waits, then ASSIGNS acc, pot and
+ *       n_ngb of every i of the batch in walk order.
+ *   pth_last_error
+ *       no counterpart (the reference asserts and ignores CUDA errors,
+ *       src/cuda_pointer.h:7); text of the last failure of a context.
+ *
+ * Records are read and written in place through "views" (stride and byte
+ * offsets of the members FDPS's EPISoft / EPJSoft / SPJSoft / ForceSoft carry,
+ * src/soft_ptcl.hpp:4-24, 271-376), so the caller's arrays are used as they
+ * are; host pointers are only read during the call (as FDPS guarantees).
+ *
+ * Results follow the reference GPU kernels (src/force_gpu_cuda.cu:50-77,
+ * 276-323): acc = G sum m_j (x_j - x_i) / max(r^2+eps^2, r_out^2)^(3/2) over
+ * EPJ plus the quadrupole SPJ terms, pot likewise (including the self term
+ * -G m_i / r_out), n_ngb counts j (self included) with
+ * r^2 + eps^2 < max(rs_i, rs_j)^2 (nnb_mode 0) or <= (nnb_mode 1, the x86
+ * kernel's convention).  Pair arithmetic is FP32 on coordinates relative to
+ * the walk's centre (computed in FP64); sums are FP64.
+ *
+ * Every function returns PTH_OK (0) or a negative PTH_ERR_* code.
+ */
+#ifndef PETAR_HIP_H
+#define PETAR_HIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTH_OK 0
+#define PTH_ERR_ARG (-1)
+#define PTH_ERR_CAPACITY (-2)
+#define PTH_ERR_HIP (-3)
+#define PTH_ERR_STATE (-4)
+#define PTH_ERR_NOMEM (-5)
+
+/* execution forms */
+#define PTH_EXEC_FUSED 0 /* work items over all walks, long lists split, XCD-aware order */
+#define PTH_EXEC_PLAIN 1 /* one workgroup per walk, one launch per batch, synchronous copies */
+
+typedef struct pth_ctx pth_ctx;
+
+typedef struct pth_params {
+    double eps2;   /* softening^2 (EPISoft::eps^2) */
+    double rcut2;  /* r_out^2 of the linear cutoff (EPISoft::r_out^2) */
+    double G;      /* ForceSoft::grav_const */
+    int nnb_mode;  /* 0: r2+eps2 < rs^2 (GPU reference), 1: <= (x86 reference) */
+    int exec_mode; /* PTH_EXEC_FUSED or PTH_EXEC_PLAIN */
+    int chunk_j;   /* fused form: max j-entries per work item (0 = default) */
+    int max_walk;  /* largest n_walk accepted by pth_dispatch (0 = 1000, the
+                      reference's N_WALK_LIMIT, src/force_gpu_cuda.cu:14) */
+} pth_params;
+
+/* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0. */
+void pth_params_default(pth_params *p);
+
+/* Creates a context on device (rank % number of visible devices). */
+int pth_create(pth_ctx **ctx, int rank, const pth_params *params);
+/* Text of the last pth_create failure of the calling thread. */
+const char *pth_create_error(void);
+void pth_destroy(pth_ctx *ctx);
+const char *pth_last_error(const pth_ctx *ctx);
+/* Device index the context runs on, or -1. */
+int pth_device(const pth_ctx *ctx);
+
+/* ---- record views (byte offsets into the caller's structs) ---------------- */
+typedef struct pth_epj_view { /* EPJSoft */
+    size_t stride, off_pos, off_mass, off_rsearch;
+} pth_epj_view;
+typedef struct pth_spj_view { /* SPJQuadrupoleInAndOut */
+    size_t stride, off_pos, off_mass, off_qxx, off_qyy, off_qzz, off_qxy, off_qxz, off_qyz;
+} pth_spj_view;
+typedef struct pth_epi_view { /* EPISoft */
+    size_t stride, off_pos, off_rsearch;
+} pth_epi_view;
+typedef struct pth_force_view { /* ForceSoft */
+    size_t stride, off_acc, off_pot, off_nngb;
+    int nngb_bytes; /* 4 or 8 */
+} pth_force_view;
+/* pos members are three consecutive doubles (x, y, z), mass/rsearch/q* are
+ * doubles. */
+
+/* ---- FDPS multi-walk hooks ------------------------------------------------- */
+int pth_send_j(pth_ctx *ctx, const void *epj, int n_epj_tot, const pth_epj_view *ev,
+               const void *spj, int n_spj_tot, const pth_spj_view *sv);
+
+int pth_dispatch(pth_ctx *ctx, int tag, int n_walk, const void *const *epi, const int *n_epi,
+                 const pth_epi_view *iv, const int *const *id_epj, const int *n_epj,
+                 const int *const *id_spj, const int *n_spj);
+
+int pth_retrieve(pth_ctx *ctx, int tag, int n_walk, const int *ni, void *const *force,
+                 const pth_force_view *fv);
+
+/* ---- device-resident evaluation (inputs kept in HBM across calls) --------- */
+typedef struct pth_resident pth_resident;
+
+/* Uploads one complete soft-force evaluation: EPI (concatenated per walk),
+ * walk sizes, concatenated index lists and the EPJ/SPJ arrays they index.
+ * Arrays are plain FP64 ([n][3] positions, [n][6] quadrupoles xx yy zz xy
+ * xz yz). */
+int pth_resident_create(pth_ctx *ctx, pth_resident **res, int n_walk, const int *n_epi,
+                        const int *n_epj, const int *n_spj, const double *epi_pos,
+                        const double *epi_rs, const int *id_epj, const int *id_spj,
+                        int n_epj_tot, const double *epj_pos, const double *epj_mass,
+                        const double *epj_rs, int n_spj_tot, const double *spj_pos,
+                        const double *spj_mass, const double *spj_quad);
+/* Enqueues one evaluation on the context's stream (asynchronous). */
+int pth_resident_run(pth_resident *res);
+/* Waits for the last evaluation. */
+int pth_resident_sync(pth_resident *res);
+/* Copies results (waits first): acc [ni][3], pot [ni], nnb [ni]. */
+int pth_resident_results(pth_resident *res, double *acc, double *pot, long long *nnb);
+/* Per-kernel GPU times of the last evaluation (ms), from HIP events on the
+ * context's stream: stage (centres + gather), force, reduce. */
+int pth_resident_timing(pth_resident *res, float *ms_stage, float *ms_force, float *ms_reduce);
+/* Sizes: interactions N_e, N_s; work items; staged bytes. */
+int pth_resident_stats(const pth_resident *res, long long *n_epep, long long *n_epsp,
+                       long long *n_items, long long *staged_bytes);
+void pth_resident_free(pth_resident *res);
+
+/* ---- diagnostics (host only, no device needed) ----------------------------- */
+/* The work decomposition pth_dispatch / pth_resident_create use for these walk
+ * sizes: returns the number of work items in launch order (or a negative
+ * code).  items (8 ints each: walk, i_begin, n_i, e_blk_begin, e_blk_end,
+ * s_begin, s_end, part; walk < 0 marks an empty slot), the staged offsets
+ * es_off / ss_off (n_walk + 1 each) and the reduction items (4 ints each:
+ * out, n_i, part_first, n_part) are written when the pointers are not NULL. */
+long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
+                         int exec_mode, int chunk_j, int *items, long long max_items,
+                         long long *es_off, long long *ss_off, int *ritems,
+                         long long max_ritems, long long *n_ritems);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

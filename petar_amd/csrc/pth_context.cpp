@@ -1,0 +1,775 @@
+// pth_context.cpp -- host side of libpetar_hip.so: contexts, the FDPS
+// multi-walk hooks (send / dispatch / retrieve) and device-resident
+// evaluations.  See include/petar_hip.h for the interface and the reference
+// functions each entry point replaces.
+#include <hip/hip_runtime.h>
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/petar_hip.h"
+#include "pth_device.h"
+#include "pth_plan.h"
+
+using namespace pth;
+
+namespace {
+
+constexpr int MAX_TAG = 8;
+
+// Test hook: PTH_FAIL_INJECT=alloc makes every device allocation fail, so
+// the error paths can be exercised from the host.
+bool inject_alloc_failure()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = std::getenv("PTH_FAIL_INJECT");
+        v = (e && std::strcmp(e, "alloc") == 0) ? 1 : 0;
+    }
+    return v == 1;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    // grows (never shrinks); the caller has synchronised the stream
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (inject_alloc_failure()) return hipErrorOutOfMemory;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (inject_alloc_failure()) return hipErrorOutOfMemory;
+        if (p) {
+            (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Layout of one upload arena (pinned host mirror and device copy).
+struct Arena {
+    size_t epi, walks, es, ss, ide, ids, items, ritems, total;
+    void layout(const Plan &P, long long n_ide, long long n_ids)
+    {
+        size_t o = 0;
+        epi = o; o = align256(o + sizeof(EpiDev) * (size_t)P.ni_tot);
+        walks = o; o = align256(o + sizeof(WalkDev) * P.walks.size());
+        es = o; o = align256(o + sizeof(long long) * P.es_off.size());
+        ss = o; o = align256(o + sizeof(long long) * P.ss_off.size());
+        ide = o; o = align256(o + sizeof(int) * (size_t)n_ide);
+        ids = o; o = align256(o + sizeof(int) * (size_t)n_ids);
+        items = o; o = align256(o + sizeof(Item) * P.items.size());
+        ritems = o; o = align256(o + sizeof(RItem) * P.ritems.size());
+        total = o;
+    }
+};
+
+// Device work buffers of one batch (staging, partials, outputs).
+struct Work {
+    DevBuf centre, staged_i, staged_e, staged_s, part_accp, part_nnb;
+    hipError_t ensure(const Plan &P)
+    {
+        hipError_t e;
+        if ((e = centre.ensure(sizeof(double4) * std::max<size_t>(P.walks.size(), 1)))) return e;
+        if ((e = staged_i.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
+        if ((e = staged_e.ensure(sizeof(float) * 5 * std::max<long long>(P.n_e_slots, 8)))) return e;
+        if ((e = staged_s.ensure(sizeof(float) * 12 * std::max<long long>(P.n_s_rec, 1)))) return e;
+        if ((e = part_accp.ensure(sizeof(double4) * 64 * std::max(P.n_part, 1)))) return e;
+        if ((e = part_nnb.ensure(sizeof(int) * 64 * std::max(P.n_part, 1)))) return e;
+        return hipSuccess;
+    }
+    void release()
+    {
+        centre.release(); staged_i.release(); staged_e.release(); staged_s.release();
+        part_accp.release(); part_nnb.release();
+    }
+};
+
+void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf &arena,
+                const Arena &A, Work &W, const EpjDev *epj, const SpjDev *spj, double4 *out_accp,
+                int *out_nnb)
+{
+    b.n_walk = (int)P.walks.size();
+    b.ni_tot = P.ni_tot;
+    b.n_e_slots = P.n_e_slots;
+    b.n_s_rec = P.n_s_rec;
+    b.n_items = (int)P.items.size();
+    b.n_ritems = (int)P.ritems.size();
+    b.plain = (par.exec_mode == PTH_EXEC_PLAIN);
+    b.nnb_le = (par.nnb_mode == 1);
+    b.eps2 = (float)par.eps2;
+    b.rout2 = (float)par.rcut2;
+    b.G = par.G;
+    char *base = arena.as<char>();
+    b.epi = reinterpret_cast<const EpiDev *>(base + A.epi);
+    b.walks = reinterpret_cast<const WalkDev *>(base + A.walks);
+    b.es_off = reinterpret_cast<const long long *>(base + A.es);
+    b.ss_off = reinterpret_cast<const long long *>(base + A.ss);
+    b.id_epj = reinterpret_cast<const int *>(base + A.ide);
+    b.id_spj = reinterpret_cast<const int *>(base + A.ids);
+    b.items = reinterpret_cast<const Item *>(base + A.items);
+    b.ritems = reinterpret_cast<const RItem *>(base + A.ritems);
+    b.epj = epj;
+    b.spj = spj;
+    b.centre = W.centre.as<double4>();
+    b.staged_i = W.staged_i.as<float4>();
+    b.staged_e = W.staged_e.as<float>();
+    b.staged_s = W.staged_s.as<float>();
+    b.part_accp = W.part_accp.as<double4>();
+    b.part_nnb = W.part_nnb.as<int>();
+    b.out_accp = out_accp;
+    b.out_nnb = out_nnb;
+}
+
+void copy_plan_to_arena(char *h, const Plan &P, const Arena &A)
+{
+    std::memcpy(h + A.walks, P.walks.data(), sizeof(WalkDev) * P.walks.size());
+    std::memcpy(h + A.es, P.es_off.data(), sizeof(long long) * P.es_off.size());
+    std::memcpy(h + A.ss, P.ss_off.data(), sizeof(long long) * P.ss_off.size());
+    if (!P.items.empty()) std::memcpy(h + A.items, P.items.data(), sizeof(Item) * P.items.size());
+    if (!P.ritems.empty()) std::memcpy(h + A.ritems, P.ritems.data(), sizeof(RItem) * P.ritems.size());
+}
+
+}  // namespace
+
+struct TagState {
+    DevBuf out_accp, out_nnb;
+    PinBuf h_accp, h_nnb;
+    std::vector<int> n_i;
+    long long ni_tot = 0;
+    bool pending = false;
+    hipEvent_t done = nullptr;
+};
+
+struct pth_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    pth_params par{};
+    std::string err;
+    // resident j arrays of the current force call
+    DevBuf d_epj, d_spj;
+    PinBuf h_epj, h_spj;
+    int n_epj = 0, n_spj = 0;
+    bool j_sent = false;
+    hipEvent_t ev_send = nullptr;
+    // batch upload
+    PinBuf h_arena;
+    DevBuf d_arena;
+    hipEvent_t ev_upload = nullptr;
+    Work work;
+    TagState tag[MAX_TAG];
+
+    int fail(int code, const std::string &msg)
+    {
+        err = msg;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char *what)
+    {
+        err = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+        return PTH_ERR_HIP;
+    }
+};
+
+struct pth_resident {
+    pth_ctx *ctx = nullptr;
+    Plan plan;
+    Arena A{};
+    DevBuf arena, epj, spj, out_accp, out_nnb;
+    Work work;
+    BatchDev b;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ran = false;
+    long long staged_bytes = 0;
+};
+
+#define HIPCHK(ctx, call, what)                      \
+    do {                                             \
+        hipError_t e_ = (call);                      \
+        if (e_ != hipSuccess) return (ctx)->hip_fail(e_, what); \
+    } while (0)
+
+extern "C" {
+
+void pth_params_default(pth_params *p)
+{
+    if (!p) return;
+    p->eps2 = 0.0;
+    p->rcut2 = 0.0;
+    p->G = 1.0;
+    p->nnb_mode = 0;
+    p->exec_mode = PTH_EXEC_FUSED;
+    p->chunk_j = 0;
+    p->max_walk = 0;
+}
+
+const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pth_device(const pth_ctx *ctx) { return ctx ? ctx->device : -1; }
+
+static thread_local std::string g_create_err;
+const char *pth_create_error(void) { return g_create_err.c_str(); }
+
+int pth_create(pth_ctx **out, int rank, const pth_params *params)
+{
+    if (!out) {
+        g_create_err = "pth_create: null output pointer";
+        return PTH_ERR_ARG;
+    }
+    *out = nullptr;
+    pth_params par;
+    if (params) par = *params;
+    else pth_params_default(&par);
+    if (par.nnb_mode != 0 && par.nnb_mode != 1) {
+        g_create_err = "pth_create: nnb_mode must be 0 or 1";
+        return PTH_ERR_ARG;
+    }
+    if (par.exec_mode != PTH_EXEC_FUSED && par.exec_mode != PTH_EXEC_PLAIN) {
+        g_create_err = "pth_create: unknown exec_mode";
+        return PTH_ERR_ARG;
+    }
+    if (!(par.rcut2 >= 0.0) || !(par.eps2 >= 0.0)) {
+        g_create_err = "pth_create: eps2 and rcut2 must be >= 0";
+        return PTH_ERR_ARG;
+    }
+    if (par.max_walk <= 0) par.max_walk = 1000;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) {
+        g_create_err = std::string("pth_create: no HIP device (") +
+                       (e != hipSuccess ? hipGetErrorName(e) : "count 0") + ")";
+        return PTH_ERR_HIP;
+    }
+    pth_ctx *c = new pth_ctx;
+    c->par = par;
+    c->device = (rank >= 0 ? rank : -rank) % ndev;  // src/force_gpu_cuda.cu:550-553
+    if ((e = hipSetDevice(c->device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_send, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_upload, hipEventDisableTiming)) != hipSuccess) {
+        g_create_err = std::string("pth_create: ") + hipGetErrorName(e);
+        pth_destroy(c);
+        return PTH_ERR_HIP;
+    }
+    for (int t = 0; t < MAX_TAG; t++) {
+        if ((e = hipEventCreateWithFlags(&c->tag[t].done, hipEventDisableTiming)) != hipSuccess) {
+            g_create_err = std::string("pth_create: ") + hipGetErrorName(e);
+            pth_destroy(c);
+            return PTH_ERR_HIP;
+        }
+    }
+    *out = c;
+    return PTH_OK;
+}
+
+void pth_destroy(pth_ctx *c)
+{
+    if (!c) return;
+    if (c->device >= 0) (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->d_epj.release();
+    c->d_spj.release();
+    c->h_epj.release();
+    c->h_spj.release();
+    c->h_arena.release();
+    c->d_arena.release();
+    c->work.release();
+    for (int t = 0; t < MAX_TAG; t++) {
+        c->tag[t].out_accp.release();
+        c->tag[t].out_nnb.release();
+        c->tag[t].h_accp.release();
+        c->tag[t].h_nnb.release();
+        if (c->tag[t].done) (void)hipEventDestroy(c->tag[t].done);
+    }
+    if (c->ev_send) (void)hipEventDestroy(c->ev_send);
+    if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// ---- send_flag == true: src/force_gpu_cuda.cu:577-620 -----------------------
+int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *ev, const void *spj,
+               int n_spj_tot, const pth_spj_view *sv)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (n_epj_tot < 0 || n_spj_tot < 0) return c->fail(PTH_ERR_ARG, "pth_send_j: negative count");
+    if ((n_epj_tot > 0 && (!epj || !ev)) || (n_spj_tot > 0 && (!spj || !sv)))
+        return c->fail(PTH_ERR_ARG, "pth_send_j: null records or view");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    // the pinned staging and the resident arrays may still be in use by the
+    // previous force call's copies and kernels
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const size_t be = sizeof(EpjDev) * (size_t)std::max(n_epj_tot, 1);
+    const size_t bs = sizeof(SpjDev) * (size_t)std::max(n_spj_tot, 1);
+    HIPCHK(c, c->h_epj.ensure(be), "pinned EPJ allocation");
+    HIPCHK(c, c->h_spj.ensure(bs), "pinned SPJ allocation");
+    HIPCHK(c, c->d_epj.ensure(be), "device EPJ allocation");
+    HIPCHK(c, c->d_spj.ensure(bs), "device SPJ allocation");
+    EpjDev *he = c->h_epj.as<EpjDev>();
+    SpjDev *hs = c->h_spj.as<SpjDev>();
+    const char *eb = static_cast<const char *>(epj);
+    const char *sb = static_cast<const char *>(spj);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n_epj_tot; i++) {
+        const char *r = eb + (size_t)i * ev->stride;
+        const double *p = reinterpret_cast<const double *>(r + ev->off_pos);
+        EpjDev d;
+        d.x = p[0];
+        d.y = p[1];
+        d.z = p[2];
+        d.m = (float)*reinterpret_cast<const double *>(r + ev->off_mass);
+        d.rs = (float)*reinterpret_cast<const double *>(r + ev->off_rsearch);
+        he[i] = d;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n_spj_tot; i++) {
+        const char *r = sb + (size_t)i * sv->stride;
+        const double *p = reinterpret_cast<const double *>(r + sv->off_pos);
+        SpjDev d;
+        d.x = p[0];
+        d.y = p[1];
+        d.z = p[2];
+        d.m = (float)*reinterpret_cast<const double *>(r + sv->off_mass);
+        d.qxx = (float)*reinterpret_cast<const double *>(r + sv->off_qxx);
+        d.qyy = (float)*reinterpret_cast<const double *>(r + sv->off_qyy);
+        d.qzz = (float)*reinterpret_cast<const double *>(r + sv->off_qzz);
+        d.qxy = (float)*reinterpret_cast<const double *>(r + sv->off_qxy);
+        d.qxz = (float)*reinterpret_cast<const double *>(r + sv->off_qxz);
+        d.qyz = (float)*reinterpret_cast<const double *>(r + sv->off_qyz);
+        d.pad[0] = d.pad[1] = d.pad[2] = 0.f;
+        hs[i] = d;
+    }
+    if (n_epj_tot > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_epj.p, he, sizeof(EpjDev) * n_epj_tot, hipMemcpyHostToDevice, c->stream),
+               "EPJ upload");
+    if (n_spj_tot > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_spj.p, hs, sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice, c->stream),
+               "SPJ upload");
+    HIPCHK(c, hipEventRecord(c->ev_send, c->stream), "hipEventRecord");
+    if (c->par.exec_mode == PTH_EXEC_PLAIN) HIPCHK(c, hipStreamSynchronize(c->stream), "send sync");
+    c->n_epj = n_epj_tot;
+    c->n_spj = n_spj_tot;
+    c->j_sent = true;
+    return PTH_OK;
+}
+
+// ---- send_flag == false: src/force_gpu_cuda.cu:621-699 -----------------------
+int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const int *n_epi,
+                 const pth_epi_view *iv, const int *const *id_epj, const int *n_epj,
+                 const int *const *id_spj, const int *n_spj)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (tag < 0 || tag >= MAX_TAG) return c->fail(PTH_ERR_ARG, "pth_dispatch: tag out of range [0,8)");
+    if (n_walk < 0 || n_walk > c->par.max_walk)
+        return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: n_walk " + std::to_string(n_walk) +
+                                             " outside [0, " + std::to_string(c->par.max_walk) + "]");
+    if (!c->j_sent) return c->fail(PTH_ERR_STATE, "pth_dispatch: no j arrays sent (send_flag=true call missing)");
+    if (n_walk > 0 && (!epi || !n_epi || !iv || !id_epj || !n_epj || !id_spj || !n_spj))
+        return c->fail(PTH_ERR_ARG, "pth_dispatch: null argument");
+    TagState &T = c->tag[tag];
+    if (T.pending) return c->fail(PTH_ERR_STATE, "pth_dispatch: previous batch of this tag not retrieved");
+    long long n_ide = 0, n_ids = 0;
+    for (int w = 0; w < n_walk; w++) {
+        if (n_epi[w] < 0 || n_epj[w] < 0 || n_spj[w] < 0)
+            return c->fail(PTH_ERR_ARG, "pth_dispatch: negative size in walk " + std::to_string(w));
+        n_ide += n_epj[w];
+        n_ids += n_spj[w];
+    }
+    Plan P;
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, P))
+        return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: batch exceeds the int32 index range");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    Arena A;
+    A.layout(P, n_ide, n_ids);
+    // the pinned arena may still feed the previous batch's copy
+    HIPCHK(c, hipEventSynchronize(c->ev_upload), "hipEventSynchronize");
+    HIPCHK(c, c->h_arena.ensure(A.total), "pinned arena allocation");
+    const bool grow = A.total > c->d_arena.cap;
+    if (grow) HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    HIPCHK(c, c->d_arena.ensure(A.total), "device arena allocation");
+    {
+        // the work buffers are shared by all tags: reallocation waits for the stream
+        bool need = false;
+        need |= sizeof(double4) * P.walks.size() > c->work.centre.cap;
+        need |= sizeof(float4) * (size_t)P.ni_tot > c->work.staged_i.cap;
+        need |= sizeof(float) * 5 * (size_t)P.n_e_slots > c->work.staged_e.cap;
+        need |= sizeof(float) * 12 * (size_t)P.n_s_rec > c->work.staged_s.cap;
+        need |= sizeof(double4) * 64 * (size_t)P.n_part > c->work.part_accp.cap;
+        if (need) HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        HIPCHK(c, c->work.ensure(P), "work buffer allocation");
+    }
+    HIPCHK(c, T.out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "output allocation");
+    HIPCHK(c, T.out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "output allocation");
+
+    char *h = c->h_arena.as<char>();
+    EpiDev *he = reinterpret_cast<EpiDev *>(h + A.epi);
+    int *hide = reinterpret_cast<int *>(h + A.ide);
+    int *hids = reinterpret_cast<int *>(h + A.ids);
+    const int nj_lim = c->n_epj, ns_lim = c->n_spj;
+    int bad = -1;
+#pragma omp parallel for schedule(dynamic)
+    for (int w = 0; w < n_walk; w++) {
+        const WalkDev &W = P.walks[w];
+        const char *base = static_cast<const char *>(epi[w]);
+        for (int i = 0; i < W.n_i; i++) {
+            const char *r = base + (size_t)i * iv->stride;
+            const double *p = reinterpret_cast<const double *>(r + iv->off_pos);
+            EpiDev d;
+            d.x = p[0];
+            d.y = p[1];
+            d.z = p[2];
+            d.rs = (float)*reinterpret_cast<const double *>(r + iv->off_rsearch);
+            d.pad = 0;
+            he[W.i_off + i] = d;
+        }
+        for (int j = 0; j < W.n_e; j++) {
+            const int id = id_epj[w][j];
+            if (id < 0 || id >= nj_lim) {
+#pragma omp atomic write
+                bad = w;
+            }
+            hide[W.e_off + j] = id;
+        }
+        for (int j = 0; j < W.n_s; j++) {
+            const int id = id_spj[w][j];
+            if (id < 0 || id >= ns_lim) {
+#pragma omp atomic write
+                bad = w;
+            }
+            hids[W.s_off + j] = id;
+        }
+    }
+    if (bad >= 0)
+        return c->fail(PTH_ERR_ARG, "pth_dispatch: list index out of range in walk " + std::to_string(bad));
+    copy_plan_to_arena(h, P, A);
+    HIPCHK(c, hipMemcpyAsync(c->d_arena.p, h, A.total, hipMemcpyHostToDevice, c->stream), "batch upload");
+    HIPCHK(c, hipEventRecord(c->ev_upload, c->stream), "hipEventRecord");
+    if (c->par.exec_mode == PTH_EXEC_PLAIN) HIPCHK(c, hipStreamSynchronize(c->stream), "upload sync");
+    BatchDev b;
+    fill_batch(b, P, c->par, c->d_arena, A, c->work, c->d_epj.as<EpjDev>(), c->d_spj.as<SpjDev>(),
+               T.out_accp.as<double4>(), T.out_nnb.as<int>());
+    HIPCHK(c, launch_stage(b, c->stream), "stage kernels");
+    HIPCHK(c, launch_force(b, c->stream), "force kernel");
+    HIPCHK(c, launch_reduce(b, c->stream), "reduce kernel");
+    HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
+    HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");
+    if (P.ni_tot > 0) {
+        HIPCHK(c, hipMemcpyAsync(T.h_accp.p, T.out_accp.p, sizeof(double4) * P.ni_tot, hipMemcpyDeviceToHost,
+                                 c->stream), "result download");
+        HIPCHK(c, hipMemcpyAsync(T.h_nnb.p, T.out_nnb.p, sizeof(int) * P.ni_tot, hipMemcpyDeviceToHost,
+                                 c->stream), "result download");
+    }
+    HIPCHK(c, hipEventRecord(T.done, c->stream), "hipEventRecord");
+    T.n_i.assign(n_epi, n_epi + n_walk);
+    T.ni_tot = P.ni_tot;
+    T.pending = true;
+    return PTH_OK;
+}
+
+// ---- RetrieveForceCUDA: src/force_gpu_cuda.cu:831-880 -------------------------
+int pth_retrieve(pth_ctx *c, int tag, int n_walk, const int *ni, void *const *force,
+                 const pth_force_view *fv)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (tag < 0 || tag >= MAX_TAG) return c->fail(PTH_ERR_ARG, "pth_retrieve: tag out of range [0,8)");
+    TagState &T = c->tag[tag];
+    if (!T.pending) return c->fail(PTH_ERR_STATE, "pth_retrieve: nothing dispatched for this tag");
+    if (n_walk != (int)T.n_i.size())
+        return c->fail(PTH_ERR_ARG, "pth_retrieve: n_walk differs from the dispatched batch");
+    for (int w = 0; w < n_walk; w++)
+        if (!ni || ni[w] != T.n_i[w])
+            return c->fail(PTH_ERR_ARG, "pth_retrieve: ni differs from the dispatched batch in walk " +
+                                            std::to_string(w));
+    if (n_walk > 0 && (!force || !fv)) return c->fail(PTH_ERR_ARG, "pth_retrieve: null force records");
+    if (fv && fv->nngb_bytes != 4 && fv->nngb_bytes != 8)
+        return c->fail(PTH_ERR_ARG, "pth_retrieve: nngb_bytes must be 4 or 8");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    T.pending = false;
+    HIPCHK(c, hipEventSynchronize(T.done), "hipEventSynchronize");
+    HIPCHK(c, hipGetLastError(), "kernel execution");
+    const double4 *a = T.h_accp.as<double4>();
+    const int *nb = T.h_nnb.as<int>();
+    std::vector<long long> off(n_walk + 1, 0);
+    for (int w = 0; w < n_walk; w++) off[w + 1] = off[w] + ni[w];
+#pragma omp parallel for schedule(dynamic)
+    for (int w = 0; w < n_walk; w++) {
+        char *base = static_cast<char *>(force[w]);
+        for (int i = 0; i < ni[w]; i++) {
+            char *r = base + (size_t)i * fv->stride;
+            const double4 v = a[off[w] + i];
+            double *acc = reinterpret_cast<double *>(r + fv->off_acc);
+            acc[0] = v.x;
+            acc[1] = v.y;
+            acc[2] = v.z;
+            *reinterpret_cast<double *>(r + fv->off_pot) = v.w;
+            if (fv->nngb_bytes == 8) *reinterpret_cast<long long *>(r + fv->off_nngb) = nb[off[w] + i];
+            else *reinterpret_cast<int *>(r + fv->off_nngb) = nb[off[w] + i];
+        }
+    }
+    return PTH_OK;
+}
+
+// ---- device-resident evaluation ------------------------------------------------
+int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n_epi,
+                        const int *n_epj, const int *n_spj, const double *epi_pos,
+                        const double *epi_rs, const int *id_epj, const int *id_spj,
+                        int n_epj_tot, const double *epj_pos, const double *epj_mass,
+                        const double *epj_rs, int n_spj_tot, const double *spj_pos,
+                        const double *spj_mass, const double *spj_quad)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (!out) return c->fail(PTH_ERR_ARG, "pth_resident_create: null output");
+    *out = nullptr;
+    if (n_walk < 0 || n_epj_tot < 0 || n_spj_tot < 0)
+        return c->fail(PTH_ERR_ARG, "pth_resident_create: negative count");
+    long long n_ide = 0, n_ids = 0;
+    for (int w = 0; w < n_walk; w++) {
+        if (n_epi[w] < 0 || n_epj[w] < 0 || n_spj[w] < 0)
+            return c->fail(PTH_ERR_ARG, "pth_resident_create: negative size in walk " + std::to_string(w));
+        n_ide += n_epj[w];
+        n_ids += n_spj[w];
+    }
+    for (long long k = 0; k < n_ide; k++)
+        if (id_epj[k] < 0 || id_epj[k] >= n_epj_tot)
+            return c->fail(PTH_ERR_ARG, "pth_resident_create: EPJ index out of range");
+    for (long long k = 0; k < n_ids; k++)
+        if (id_spj[k] < 0 || id_spj[k] >= n_spj_tot)
+            return c->fail(PTH_ERR_ARG, "pth_resident_create: SPJ index out of range");
+    pth_resident *r = new pth_resident;
+    r->ctx = c;
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, r->plan)) {
+        delete r;
+        return c->fail(PTH_ERR_CAPACITY, "pth_resident_create: sizes exceed the int32 index range");
+    }
+    const Plan &P = r->plan;
+    r->A.layout(P, n_ide, n_ids);
+    auto bail = [&](hipError_t e, const char *what) {
+        int rc = c->hip_fail(e, what);
+        pth_resident_free(r);
+        return rc;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(c->device))) return bail(e, "hipSetDevice");
+    if ((e = r->arena.ensure(r->A.total))) return bail(e, "resident arena");
+    if ((e = r->epj.ensure(sizeof(EpjDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
+    if ((e = r->spj.ensure(sizeof(SpjDev) * std::max(n_spj_tot, 1)))) return bail(e, "resident SPJ");
+    if ((e = r->out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
+    if ((e = r->out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
+    if ((e = r->work.ensure(P))) return bail(e, "resident work buffers");
+    for (int k = 0; k < 4; k++)
+        if ((e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
+    // host mirrors (pageable is fine: uploaded once)
+    std::vector<char> h(r->A.total);
+    EpiDev *he = reinterpret_cast<EpiDev *>(h.data() + r->A.epi);
+    for (long long i = 0; i < P.ni_tot; i++) {
+        he[i].x = epi_pos[3 * i];
+        he[i].y = epi_pos[3 * i + 1];
+        he[i].z = epi_pos[3 * i + 2];
+        he[i].rs = (float)epi_rs[i];
+        he[i].pad = 0;
+    }
+    if (n_ide) std::memcpy(h.data() + r->A.ide, id_epj, sizeof(int) * n_ide);
+    if (n_ids) std::memcpy(h.data() + r->A.ids, id_spj, sizeof(int) * n_ids);
+    copy_plan_to_arena(h.data(), P, r->A);
+    std::vector<EpjDev> ej(std::max(n_epj_tot, 0));
+    for (int i = 0; i < n_epj_tot; i++) {
+        ej[i].x = epj_pos[3 * (size_t)i];
+        ej[i].y = epj_pos[3 * (size_t)i + 1];
+        ej[i].z = epj_pos[3 * (size_t)i + 2];
+        ej[i].m = (float)epj_mass[i];
+        ej[i].rs = (float)epj_rs[i];
+    }
+    std::vector<SpjDev> sj(std::max(n_spj_tot, 0));
+    for (int i = 0; i < n_spj_tot; i++) {
+        const double *q = spj_quad + 6 * (size_t)i;
+        sj[i].x = spj_pos[3 * (size_t)i];
+        sj[i].y = spj_pos[3 * (size_t)i + 1];
+        sj[i].z = spj_pos[3 * (size_t)i + 2];
+        sj[i].m = (float)spj_mass[i];
+        sj[i].qxx = (float)q[0];
+        sj[i].qyy = (float)q[1];
+        sj[i].qzz = (float)q[2];
+        sj[i].qxy = (float)q[3];
+        sj[i].qxz = (float)q[4];
+        sj[i].qyz = (float)q[5];
+        sj[i].pad[0] = sj[i].pad[1] = sj[i].pad[2] = 0.f;
+    }
+    if ((e = hipMemcpy(r->arena.p, h.data(), r->A.total, hipMemcpyHostToDevice))) return bail(e, "resident upload");
+    if (n_epj_tot && (e = hipMemcpy(r->epj.p, ej.data(), sizeof(EpjDev) * n_epj_tot, hipMemcpyHostToDevice)))
+        return bail(e, "resident upload");
+    if (n_spj_tot && (e = hipMemcpy(r->spj.p, sj.data(), sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice)))
+        return bail(e, "resident upload");
+    fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
+               r->out_accp.as<double4>(), r->out_nnb.as<int>());
+    r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 48 * P.n_s_rec);
+    *out = r;
+    return PTH_OK;
+}
+
+int pth_resident_run(pth_resident *r)
+{
+    if (!r) return PTH_ERR_ARG;
+    pth_ctx *c = r->ctx;
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    HIPCHK(c, hipEventRecord(r->ev[0], c->stream), "hipEventRecord");
+    HIPCHK(c, launch_stage(r->b, c->stream), "stage kernels");
+    HIPCHK(c, hipEventRecord(r->ev[1], c->stream), "hipEventRecord");
+    HIPCHK(c, launch_force(r->b, c->stream), "force kernel");
+    HIPCHK(c, hipEventRecord(r->ev[2], c->stream), "hipEventRecord");
+    HIPCHK(c, launch_reduce(r->b, c->stream), "reduce kernel");
+    HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
+    r->ran = true;
+    return PTH_OK;
+}
+
+int pth_resident_sync(pth_resident *r)
+{
+    if (!r) return PTH_ERR_ARG;
+    pth_ctx *c = r->ctx;
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    HIPCHK(c, hipGetLastError(), "kernel execution");
+    return PTH_OK;
+}
+
+int pth_resident_results(pth_resident *r, double *acc, double *pot, long long *nnb)
+{
+    if (!r) return PTH_ERR_ARG;
+    pth_ctx *c = r->ctx;
+    if (!r->ran) return c->fail(PTH_ERR_STATE, "pth_resident_results: no evaluation run");
+    int rc = pth_resident_sync(r);
+    if (rc) return rc;
+    const long long n = r->plan.ni_tot;
+    std::vector<double4> a(std::max<long long>(n, 1));
+    std::vector<int> nb(std::max<long long>(n, 1));
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(a.data(), r->out_accp.p, sizeof(double4) * n, hipMemcpyDeviceToHost), "result download");
+        HIPCHK(c, hipMemcpy(nb.data(), r->out_nnb.p, sizeof(int) * n, hipMemcpyDeviceToHost), "result download");
+    }
+    for (long long i = 0; i < n; i++) {
+        if (acc) {
+            acc[3 * i] = a[i].x;
+            acc[3 * i + 1] = a[i].y;
+            acc[3 * i + 2] = a[i].z;
+        }
+        if (pot) pot[i] = a[i].w;
+        if (nnb) nnb[i] = nb[i];
+    }
+    return PTH_OK;
+}
+
+int pth_resident_timing(pth_resident *r, float *ms_stage, float *ms_force, float *ms_reduce)
+{
+    if (!r) return PTH_ERR_ARG;
+    pth_ctx *c = r->ctx;
+    if (!r->ran) return c->fail(PTH_ERR_STATE, "pth_resident_timing: no evaluation run");
+    HIPCHK(c, hipEventSynchronize(r->ev[3]), "hipEventSynchronize");
+    float t = 0;
+    if (ms_stage) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[0], r->ev[1]), "hipEventElapsedTime"); *ms_stage = t; }
+    if (ms_force) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[1], r->ev[2]), "hipEventElapsedTime"); *ms_force = t; }
+    if (ms_reduce) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[2], r->ev[3]), "hipEventElapsedTime"); *ms_reduce = t; }
+    return PTH_OK;
+}
+
+int pth_resident_stats(const pth_resident *r, long long *n_epep, long long *n_epsp, long long *n_items,
+                       long long *staged_bytes)
+{
+    if (!r) return PTH_ERR_ARG;
+    if (n_epep) *n_epep = r->plan.n_epep;
+    if (n_epsp) *n_epsp = r->plan.n_epsp;
+    if (n_items) *n_items = r->plan.n_real_items;
+    if (staged_bytes) *staged_bytes = r->staged_bytes;
+    return PTH_OK;
+}
+
+void pth_resident_free(pth_resident *r)
+{
+    if (!r) return;
+    if (r->ctx && r->ctx->device >= 0) {
+        (void)hipSetDevice(r->ctx->device);
+        if (r->ctx->stream) (void)hipStreamSynchronize(r->ctx->stream);
+    }
+    r->arena.release();
+    r->epj.release();
+    r->spj.release();
+    r->out_accp.release();
+    r->out_nnb.release();
+    r->work.release();
+    for (int k = 0; k < 4; k++)
+        if (r->ev[k]) (void)hipEventDestroy(r->ev[k]);
+    delete r;
+}
+
+long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
+                         int chunk_j, int *items, long long max_items, long long *es_off, long long *ss_off,
+                         int *ritems, long long max_ritems, long long *n_ritems)
+{
+    if (n_walk < 0 || (n_walk > 0 && (!n_epi || !n_epj || !n_spj))) return PTH_ERR_ARG;
+    Plan P;
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, exec_mode, chunk_j, P)) return PTH_ERR_CAPACITY;
+    if (es_off) std::copy(P.es_off.begin(), P.es_off.end(), es_off);
+    if (ss_off) std::copy(P.ss_off.begin(), P.ss_off.end(), ss_off);
+    if (n_ritems) *n_ritems = (long long)P.ritems.size();
+    if (ritems) {
+        if ((long long)P.ritems.size() > max_ritems) return PTH_ERR_CAPACITY;
+        std::memcpy(ritems, P.ritems.data(), sizeof(RItem) * P.ritems.size());
+    }
+    if (items) {
+        if ((long long)P.items.size() > max_items) return PTH_ERR_CAPACITY;
+        std::memcpy(items, P.items.data(), sizeof(Item) * P.items.size());
+    }
+    return (long long)P.items.size();
+}
+
+}  // extern "C"

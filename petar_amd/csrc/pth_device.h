@@ -1,0 +1,88 @@
+// pth_device.h -- device record layouts and launch descriptors shared by the
+// kernels (pth_kernels.hip) and the host side (pth_context.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+// FP32 partial sums are flushed into FP64 every PTH_FLUSH_BLOCKS EPJ blocks
+// of 8 (64 pairs) and every PTH_FLUSH_SPJ SPJ records.
+#define PTH_FLUSH_BLOCKS 8
+#define PTH_FLUSH_SPJ 32
+
+namespace pth {
+
+// Resident EPJ record (32 B): FP64 position, mass, r_search.
+struct __align__(32) EpjDev {
+    double x, y, z;
+    float m, rs;
+};
+
+// Resident SPJ record (64 B): FP64 centre of mass, mass, raw second moment.
+struct __align__(64) SpjDev {
+    double x, y, z;
+    float m, qxx, qyy, qzz, qxy, qxz, qyz;
+    float pad[3];
+};
+
+// Batch i-particle (32 B): FP64 position, r_search.
+struct __align__(32) EpiDev {
+    double x, y, z;
+    float rs;
+    int pad;
+};
+
+// One walk (i-group) of a batch: i range, list ranges (in the concatenated
+// id lists) and sizes.
+struct WalkDev {
+    int i_off, n_i;
+    int e_off, n_e;
+    int s_off, n_s;
+    int pad[2];
+};
+
+// One work item (one wave): 64-i tile of a walk against a block range of
+// the walk's staged EPJ and a record range of its staged SPJ.  walk < 0
+// marks an empty slot.  part < 0: results go to the output, else to
+// partial slot `part` (64 entries).
+struct __align__(16) Item {
+    int walk, i_begin, n_i, e_blk_begin, e_blk_end, s_begin, s_end, part;
+};
+
+// Sum of n_part consecutive partial slots into outputs [out, out + n_i).
+struct RItem {
+    int out, n_i, part_first, n_part;
+};
+
+struct BatchDev {
+    int n_walk = 0;
+    long long ni_tot = 0;
+    long long n_e_slots = 0;  // staged EPJ slots (multiple of 8 per walk)
+    long long n_s_rec = 0;    // staged SPJ records
+    int n_items = 0, n_ritems = 0;
+    int plain = 0, nnb_le = 0;
+    float eps2 = 0.f, rout2 = 0.f;
+    double G = 1.0;
+    const EpiDev *epi = nullptr;
+    const WalkDev *walks = nullptr;
+    const long long *es_off = nullptr;  // n_walk+1 staged EPJ slot offsets
+    const long long *ss_off = nullptr;  // n_walk+1 staged SPJ record offsets
+    const int *id_epj = nullptr;
+    const int *id_spj = nullptr;
+    const EpjDev *epj = nullptr;
+    const SpjDev *spj = nullptr;
+    const Item *items = nullptr;
+    const RItem *ritems = nullptr;
+    double4 *centre = nullptr;
+    float4 *staged_i = nullptr;
+    float *staged_e = nullptr;
+    float *staged_s = nullptr;
+    double4 *out_accp = nullptr;
+    int *out_nnb = nullptr;
+    double4 *part_accp = nullptr;
+    int *part_nnb = nullptr;
+};
+
+hipError_t launch_stage(const BatchDev &b, hipStream_t st);
+hipError_t launch_force(const BatchDev &b, hipStream_t st);
+hipError_t launch_reduce(const BatchDev &b, hipStream_t st);
+
+}  // namespace pth

@@ -1,0 +1,354 @@
+"""Python mirror of PeTar's GPU soft-force hooks over libpetar_hip.so.
+
+The FDPS multi-walk interface (src/force_gpu_cuda.hpp:101-168) is kept:
+
+    f = CalcForceWithLinearCutoffMultiWalk(my_rank, eps2, rcut2, G)
+    f(tag, n_walk, epi, n_epi, id_epj, n_epj, id_spj, n_spj,
+      epj, n_epj_tot, spj, n_spj_tot, send_flag)          # dispatch
+    RetrieveForce(tag, n_walk, ni, force)                   # retrieve
+
+with epi/force lists of per-walk numpy record arrays of dtype EPI_SOFT /
+FORCE_SOFT and epj/spj record arrays of dtype EPJ_SOFT / SPJ_SOFT, whose
+layouts are those of EPISoft, EPJSoft and ForceSoft (src/soft_ptcl.hpp:4-24,
+271-376) and of the SPJQuadrupoleInAndOut members PeTar reads
+(src/force_gpu_cuda.cu:596-608).  Results are assigned (not accumulated), as
+RetrieveForceCUDA does (src/force_gpu_cuda.cu:866-875).
+
+Resident evaluates a whole soft-force call that is kept in HBM (the form
+bench.py times and the serving loop uses).
+
+The HIP library is required: every entry point raises if it is missing or if
+no GPU is visible; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libpetar_hip.so")
+_lib = None
+
+PTH_OK, PTH_ERR_ARG, PTH_ERR_CAPACITY, PTH_ERR_HIP, PTH_ERR_STATE, PTH_ERR_NOMEM = 0, -1, -2, -3, -4, -5
+EXEC_FUSED, EXEC_PLAIN = 0, 1
+
+# record layouts of the FDPS particle types (no KDKDK_4TH / SAVE_NEIGHBOR_ID)
+EPI_SOFT = np.dtype([("id", "<i8"), ("pos", "<f8", 3), ("r_search", "<f8"), ("rank_org", "<i4"),
+                     ("type", "<i4")])
+EPJ_SOFT = np.dtype([("id", "<i8"), ("mass", "<f8"), ("pos", "<f8", 3), ("vel", "<f8", 3), ("r_in", "<f8"),
+                     ("r_out", "<f8"), ("r_search", "<f8"), ("r_scale_next", "<f8"), ("group_data", "<f8", 2),
+                     ("rank_org", "<i4"), ("adr_org", "<i4")])
+SPJ_SOFT = np.dtype([("mass", "<f8"), ("pos", "<f8", 3),
+                     ("quad", [("xx", "<f8"), ("yy", "<f8"), ("zz", "<f8"), ("xy", "<f8"), ("xz", "<f8"),
+                               ("yz", "<f8")])])
+FORCE_SOFT = np.dtype([("acc", "<f8", 3), ("pot", "<f8"), ("n_ngb", "<i8")])
+assert EPI_SOFT.itemsize == 48 and EPJ_SOFT.itemsize == 120 and FORCE_SOFT.itemsize == 40
+
+
+class Params(C.Structure):
+    _fields_ = [("eps2", C.c_double), ("rcut2", C.c_double), ("G", C.c_double), ("nnb_mode", C.c_int),
+                ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int)]
+
+
+class EpjView(C.Structure):
+    _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_mass", "off_rsearch")]
+
+
+class SpjView(C.Structure):
+    _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_mass", "off_qxx", "off_qyy", "off_qzz",
+                                          "off_qxy", "off_qxz", "off_qyz")]
+
+
+class EpiView(C.Structure):
+    _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_rsearch")]
+
+
+class ForceView(C.Structure):
+    _fields_ = [("stride", C.c_size_t), ("off_acc", C.c_size_t), ("off_pot", C.c_size_t),
+                ("off_nngb", C.c_size_t), ("nngb_bytes", C.c_int)]
+
+
+EXPORTED = ("pth_params_default", "pth_create", "pth_create_error", "pth_destroy", "pth_last_error",
+            "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve", "pth_resident_create",
+            "pth_resident_run", "pth_resident_sync", "pth_resident_results", "pth_resident_timing",
+            "pth_resident_stats", "pth_resident_free", "pth_plan_debug")
+
+
+def lib():
+    """Loads libpetar_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        raise RuntimeError(f"{_LIB} is missing: the HIP extension must be built (__graft_entry__.build())")
+    L = C.CDLL(_LIB)
+    vp = C.c_void_p
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    L.pth_params_default.argtypes = [C.POINTER(Params)]
+    L.pth_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(Params)]
+    L.pth_create_error.restype = C.c_char_p
+    L.pth_destroy.argtypes = [vp]
+    L.pth_last_error.argtypes = [vp]
+    L.pth_last_error.restype = C.c_char_p
+    L.pth_device.argtypes = [vp]
+    L.pth_send_j.argtypes = [vp, vp, C.c_int, C.POINTER(EpjView), vp, C.c_int, C.POINTER(SpjView)]
+    L.pth_dispatch.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), ip, C.POINTER(EpiView), C.POINTER(vp), ip,
+                               C.POINTER(vp), ip]
+    L.pth_retrieve.argtypes = [vp, C.c_int, C.c_int, ip, C.POINTER(vp), C.POINTER(ForceView)]
+    L.pth_resident_create.argtypes = [vp, C.POINTER(vp), C.c_int, ip, ip, ip, dp, dp, ip, ip, C.c_int, dp, dp, dp,
+                                      C.c_int, dp, dp, dp]
+    L.pth_resident_run.argtypes = [vp]
+    L.pth_resident_sync.argtypes = [vp]
+    L.pth_resident_results.argtypes = [vp, dp, dp, C.POINTER(C.c_longlong)]
+    L.pth_resident_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.pth_resident_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 4
+    L.pth_resident_free.argtypes = [vp]
+    L.pth_plan_debug.restype = C.c_longlong
+    L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, ip, C.c_longlong, C.POINTER(C.c_longlong),
+                                 C.POINTER(C.c_longlong), ip, C.c_longlong, C.POINTER(C.c_longlong)]
+    _lib = L
+    return L
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class PthError(RuntimeError):
+    pass
+
+
+def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=0):
+    p = Params()
+    lib().pth_params_default(C.byref(p))
+    p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk = (
+        eps2, rcut2, G, nnb_mode, exec_mode, chunk_j, max_walk)
+    return p
+
+
+class Context:
+    """One device context (pth_create); device = rank % visible devices."""
+
+    def __init__(self, rank=0, **params):
+        L = lib()
+        self._h = C.c_void_p()
+        self.params = default_params(**params)
+        rc = L.pth_create(C.byref(self._h), rank, C.byref(self.params))
+        if rc != PTH_OK:
+            raise PthError(f"pth_create failed ({rc}): {L.pth_create_error().decode()}")
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def device(self) -> int:
+        return lib().pth_device(self._h)
+
+    def check(self, rc, what):
+        if rc != PTH_OK:
+            raise PthError(f"{what} failed ({rc}): {lib().pth_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            lib().pth_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _views():
+    f = EPJ_SOFT.fields
+    ev = EpjView(EPJ_SOFT.itemsize, f["pos"][1], f["mass"][1], f["r_search"][1])
+    sf = SPJ_SOFT.fields
+    qf = SPJ_SOFT.fields["quad"][0].fields
+    qo = sf["quad"][1]
+    sv = SpjView(SPJ_SOFT.itemsize, sf["pos"][1], sf["mass"][1], qo + qf["xx"][1], qo + qf["yy"][1],
+                 qo + qf["zz"][1], qo + qf["xy"][1], qo + qf["xz"][1], qo + qf["yz"][1])
+    fi = EPI_SOFT.fields
+    iv = EpiView(EPI_SOFT.itemsize, fi["pos"][1], fi["r_search"][1])
+    ff = FORCE_SOFT.fields
+    fv = ForceView(FORCE_SOFT.itemsize, ff["acc"][1], ff["pot"][1], ff["n_ngb"][1], 8)
+    return ev, sv, iv, fv
+
+
+class CalcForceWithLinearCutoffMultiWalk:
+    """The dispatch functor of src/force_gpu_cuda.hpp:101-133 over the C ABI.
+
+    One process-wide context per rank, created at the first call (as the
+    reference's first-call initialisation, src/force_gpu_cuda.cu:547-571).
+    """
+
+    _ctx = {}
+
+    def __init__(self, my_rank=0, eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0):
+        self.my_rank, self.eps2, self.rcut2, self.G = my_rank, eps2, rcut2, G
+        self.nnb_mode, self.exec_mode, self.chunk_j = nnb_mode, exec_mode, chunk_j
+
+    def context(self) -> Context:
+        key = (self.my_rank, self.eps2, self.rcut2, self.G, self.nnb_mode, self.exec_mode, self.chunk_j)
+        c = self._ctx.get(key)
+        if c is None:
+            c = Context(self.my_rank, eps2=self.eps2, rcut2=self.rcut2, G=self.G, nnb_mode=self.nnb_mode,
+                        exec_mode=self.exec_mode, chunk_j=self.chunk_j)
+            self._ctx[key] = c
+        return c
+
+    def __call__(self, tag, n_walk, epi, n_epi, id_epj, n_epj, id_spj, n_spj, epj, n_epj_tot, spj, n_spj_tot,
+                 send_flag):
+        c = self.context()
+        ev, sv, iv, _ = _views()
+        L = lib()
+        if send_flag:
+            epj = np.ascontiguousarray(epj, dtype=EPJ_SOFT)
+            spj = np.ascontiguousarray(spj, dtype=SPJ_SOFT)
+            c.check(L.pth_send_j(c.handle, epj.ctypes.data, int(n_epj_tot), C.byref(ev), spj.ctypes.data,
+                                 int(n_spj_tot), C.byref(sv)), "pth_send_j")
+            return 0
+        epi = [np.ascontiguousarray(e, dtype=EPI_SOFT) for e in epi[:n_walk]]
+        ide = [_i32(a) for a in id_epj[:n_walk]]
+        ids = [_i32(a) for a in id_spj[:n_walk]]
+        pe = (C.c_void_p * max(n_walk, 1))(*[e.ctypes.data for e in epi])
+        pj = (C.c_void_p * max(n_walk, 1))(*[a.ctypes.data for a in ide])
+        ps = (C.c_void_p * max(n_walk, 1))(*[a.ctypes.data for a in ids])
+        ne, nj, ns = _i32(n_epi), _i32(n_epj), _i32(n_spj)
+        c.check(L.pth_dispatch(c.handle, tag, n_walk, pe, _ptr(ne, C.c_int), C.byref(iv), pj, _ptr(nj, C.c_int),
+                               ps, _ptr(ns, C.c_int)), "pth_dispatch")
+        return 0
+
+    def retrieve(self, tag, n_walk, ni, force):
+        c = self.context()
+        _, _, _, fv = _views()
+        for f in force[:n_walk]:
+            if f.dtype != FORCE_SOFT or not f.flags.c_contiguous:
+                raise TypeError("force records must be contiguous FORCE_SOFT arrays")
+        pf = (C.c_void_p * max(n_walk, 1))(*[f.ctypes.data for f in force[:n_walk]])
+        n = _i32(ni)
+        c.check(lib().pth_retrieve(c.handle, tag, n_walk, _ptr(n, C.c_int), pf, C.byref(fv)), "pth_retrieve")
+        return 0
+
+
+_last_functor = [None]
+
+
+def RetrieveForce(tag, n_walk, ni, force, functor=None):
+    """RetrieveForceCUDA (src/force_gpu_cuda.cu:831-880): waits and assigns
+    acc/pot/n_ngb of the batch dispatched by `functor` (default: the last one
+    constructed through make_functor)."""
+    f = functor or _last_functor[0]
+    if f is None:
+        raise PthError("RetrieveForce: no functor has dispatched")
+    return f.retrieve(tag, n_walk, ni, force)
+
+
+def make_functor(*args, **kw):
+    f = CalcForceWithLinearCutoffMultiWalk(*args, **kw)
+    _last_functor[0] = f
+    return f
+
+
+class Resident:
+    """A whole soft-force evaluation kept in HBM (pth_resident_*)."""
+
+    def __init__(self, ctx: Context, n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
+                 spj_pos, spj_mass, spj_quad):
+        self.ctx = ctx
+        L = lib()
+        self._keep = [_i32(n_epi), _i32(n_epj), _i32(n_spj), _f64(epi_pos).reshape(-1, 3), _f64(epi_rs),
+                      _i32(id_epj) if len(id_epj) else np.zeros(1, np.int32),
+                      _i32(id_spj) if len(id_spj) else np.zeros(1, np.int32),
+                      _f64(epj_pos).reshape(-1, 3) if len(epj_pos) else np.zeros((1, 3)),
+                      _f64(epj_mass) if len(epj_mass) else np.zeros(1),
+                      _f64(epj_rs) if len(epj_rs) else np.zeros(1),
+                      _f64(spj_pos).reshape(-1, 3) if len(spj_pos) else np.zeros((1, 3)),
+                      _f64(spj_mass) if len(spj_mass) else np.zeros(1),
+                      _f64(spj_quad).reshape(-1, 6) if len(spj_quad) else np.zeros((1, 6))]
+        k = self._keep
+        self.n_walk = k[0].size
+        self.ni = int(k[0].sum())
+        self._h = C.c_void_p()
+        ctx.check(L.pth_resident_create(ctx.handle, C.byref(self._h), self.n_walk, _ptr(k[0], C.c_int),
+                                        _ptr(k[1], C.c_int), _ptr(k[2], C.c_int), _ptr(k[3], C.c_double),
+                                        _ptr(k[4], C.c_double), _ptr(k[5], C.c_int), _ptr(k[6], C.c_int),
+                                        len(epj_mass), _ptr(k[7], C.c_double), _ptr(k[8], C.c_double),
+                                        _ptr(k[9], C.c_double), len(spj_mass), _ptr(k[10], C.c_double),
+                                        _ptr(k[11], C.c_double), _ptr(k[12], C.c_double)), "pth_resident_create")
+        self._keep = None
+
+    @classmethod
+    def from_lists(cls, ctx, lists):
+        idx = np.concatenate([np.arange(b, b + n) for b, n in zip(lists.epi_begin, lists.n_epi)])
+        return cls(ctx, lists.n_epi, lists.n_epj, lists.n_spj, lists.epj_pos[idx], lists.epj_rs[idx], lists.id_epj,
+                   lists.id_spj, lists.epj_pos, lists.epj_mass, lists.epj_rs, lists.spj_pos, lists.spj_mass,
+                   lists.spj_quad)
+
+    def run(self):
+        self.ctx.check(lib().pth_resident_run(self._h), "pth_resident_run")
+
+    def sync(self):
+        self.ctx.check(lib().pth_resident_sync(self._h), "pth_resident_sync")
+
+    def results(self):
+        acc = np.zeros((self.ni, 3))
+        pot = np.zeros(self.ni)
+        nnb = np.zeros(self.ni, np.int64)
+        self.ctx.check(lib().pth_resident_results(self._h, _ptr(acc, C.c_double), _ptr(pot, C.c_double),
+                                                  _ptr(nnb, C.c_longlong)), "pth_resident_results")
+        return acc, pot, nnb
+
+    def timing(self):
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        self.ctx.check(lib().pth_resident_timing(self._h, C.byref(a), C.byref(b), C.byref(c)), "pth_resident_timing")
+        return a.value, b.value, c.value
+
+    def stats(self):
+        v = [C.c_longlong() for _ in range(4)]
+        self.ctx.check(lib().pth_resident_stats(self._h, *[C.byref(x) for x in v]), "pth_resident_stats")
+        return dict(n_epep=v[0].value, n_epsp=v[1].value, n_items=v[2].value, staged_bytes=v[3].value)
+
+    def close(self):
+        if self._h:
+            lib().pth_resident_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_debug(n_epi, n_epj, n_spj, exec_mode=EXEC_FUSED, chunk_j=0):
+    """The work decomposition (host only): items [k, 8], es_off, ss_off, ritems [k, 4]."""
+    L = lib()
+    ne, nj, ns = _i32(n_epi), _i32(n_epj), _i32(n_spj)
+    nw = ne.size
+    nr = C.c_longlong()
+    n = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j, None, 0,
+                         None, None, None, 0, C.byref(nr))
+    if n < 0:
+        raise PthError(f"pth_plan_debug failed ({n})")
+    items = np.zeros((max(n, 1), 8), np.int32)
+    rit = np.zeros((max(nr.value, 1), 4), np.int32)
+    es = np.zeros(nw + 1, np.int64)
+    ss = np.zeros(nw + 1, np.int64)
+    n2 = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j,
+                          _ptr(items, C.c_int), items.shape[0], _ptr(es, C.c_longlong), _ptr(ss, C.c_longlong),
+                          _ptr(rit, C.c_int), rit.shape[0], C.byref(nr))
+    if n2 < 0:
+        raise PthError(f"pth_plan_debug failed ({n2})")
+    return items[:n], es, ss, rit[:nr.value]

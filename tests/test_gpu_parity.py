@@ -1,0 +1,229 @@
+"""GPU parity of the soft-force path against the FP64 oracle (same lists).
+
+Tolerances (FP32 pair arithmetic on walk-relative coordinates, FP64 sums):
+median relative error of acc and pot <= 1e-6, 99th percentile <= 1e-5,
+maximum <= 1e-3 (relative errors of tiny net accelerations); neighbour
+counts exact except for pairs inside the FP32 rounding band of r_search,
+where they may differ by at most the number of such pairs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from petar_amd import soft_force as sf
+from tests._common import R_OUT, c1_lists, nnb_check, rel_err, walk_centres
+
+pytestmark = pytest.mark.gpu
+
+MED, P99, MAX = 1e-6, 1e-5, 1e-3
+
+
+@pytest.fixture(scope="module")
+def c1():
+    L = c1_lists(16384, seed=1)
+    a = O.lists_args(L)
+    ref = O.soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2, G=1.0, nnb_mode=0)
+    amb = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=0.0)
+    return L, a, ref, amb
+
+
+def _check(res, ref, amb, label):
+    acc, pot, nnb = res
+    racc, rpot, rnnb = ref
+    ea, ep = rel_err(acc, racc), rel_err(pot, rpot)
+    print(f"[{label}] acc err med {np.median(ea):.3e} p99 {np.percentile(ea, 99):.3e} max {ea.max():.3e}; "
+          f"pot err med {np.median(ep):.3e} p99 {np.percentile(ep, 99):.3e} max {ep.max():.3e}", flush=True)
+    assert np.median(ea) <= MED and np.median(ep) <= MED
+    assert np.percentile(ea, 99) <= P99 and np.percentile(ep, 99) <= P99
+    assert ea.max() <= MAX and ep.max() <= MAX
+    bad_exact, bad_band, n_amb_i = nnb_check(nnb, rnnb, amb)
+    print(f"[{label}] nnb: mismatches outside band {bad_exact}, beyond band {bad_band}, i with ambiguous pairs "
+          f"{n_amb_i}", flush=True)
+    assert bad_exact == 0 and bad_band == 0
+
+
+@pytest.mark.parametrize("exec_mode", [sf.EXEC_FUSED, sf.EXEC_PLAIN])
+def test_resident_c1(c1, exec_mode):
+    L, a, ref, amb = c1
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode)
+    r = sf.Resident.from_lists(ctx, L)
+    r.run()
+    res = r.results()
+    _check(res, ref, amb, f"resident exec={exec_mode}")
+    # a second evaluation gives bit-identical results
+    r.run()
+    res2 = r.results()
+    assert np.array_equal(res[0], res2[0]) and np.array_equal(res[1], res2[1]) and np.array_equal(res[2], res2[2])
+    r.close()
+    ctx.close()
+
+
+def test_fused_split_chunks(c1):
+    """Small chunk size forces every walk's list over many items + reduction."""
+    L, a, ref, amb = c1
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, chunk_j=256)
+    r = sf.Resident.from_lists(ctx, L)
+    assert r.stats()["n_items"] > L.n_walk * 4
+    r.run()
+    _check(r.results(), ref, amb, "fused chunk=256")
+    r.close()
+    ctx.close()
+
+
+def _fdps_replay(L, functor, n_walk_limit=200):
+    """Replays FDPS's multi-walk call order (send once, then dispatch/retrieve
+    per batch of <= n_walk_limit walks) with FDPS-layout records."""
+    n = L.epj_pos.shape[0]
+    epj = np.zeros(n, sf.EPJ_SOFT)
+    epj["id"] = np.arange(n)
+    epj["mass"] = L.epj_mass
+    epj["pos"] = L.epj_pos
+    epj["r_search"] = L.epj_rs
+    epj["r_out"] = R_OUT
+    ns = L.spj_mass.size
+    spj = np.zeros(ns, sf.SPJ_SOFT)
+    spj["mass"] = L.spj_mass
+    spj["pos"] = L.spj_pos
+    for k, q in enumerate(("xx", "yy", "zz", "xy", "xz", "yz")):
+        spj["quad"][q] = L.spj_quad[:, k]
+    eoff, soff = L.walk_offsets()
+    functor(0, 0, [], [], [], [], [], [], epj, n, spj, ns, True)
+    out = []
+    for b0 in range(0, L.n_walk, n_walk_limit):
+        ws = range(b0, min(b0 + n_walk_limit, L.n_walk))
+        epi, force = [], []
+        for w in ws:
+            e = np.zeros(L.n_epi[w], sf.EPI_SOFT)
+            sl = slice(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w])
+            e["pos"] = L.epj_pos[sl]
+            e["r_search"] = L.epj_rs[sl]
+            e["type"] = 1
+            epi.append(e)
+            f = np.zeros(L.n_epi[w], sf.FORCE_SOFT)
+            f["pot"] = 123.0  # must be overwritten (assignment semantics)
+            force.append(f)
+        ide = [L.id_epj[eoff[w]:eoff[w + 1]] for w in ws]
+        ids = [L.id_spj[soff[w]:soff[w + 1]] for w in ws]
+        nw = len(ws)
+        functor(0, nw, epi, L.n_epi[list(ws)], ide, L.n_epj[list(ws)], ids, L.n_spj[list(ws)], epj, n, spj, ns,
+                False)
+        sf.RetrieveForce(0, nw, L.n_epi[list(ws)], force, functor)
+        out.extend(force)
+    f = np.concatenate(out)
+    return f["acc"], f["pot"], f["n_ngb"]
+
+
+@pytest.mark.parametrize("exec_mode", [sf.EXEC_FUSED, sf.EXEC_PLAIN])
+def test_fdps_multiwalk_replay(c1, exec_mode):
+    L, a, ref, amb = c1
+    fn = sf.make_functor(0, 0.0, R_OUT ** 2, 1.0, exec_mode=exec_mode)
+    res = _fdps_replay(L, fn)
+    _check(res, ref, amb, f"fdps replay exec={exec_mode}")
+    # the drop-in path and the resident path compute every walk identically
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, exec_mode=exec_mode)
+    r = sf.Resident.from_lists(ctx, L)
+    r.run()
+    racc, rpot, rnnb = r.results()
+    assert np.array_equal(res[0], racc) and np.array_equal(res[1], rpot) and np.array_equal(res[2], rnnb)
+    r.close()
+    ctx.close()
+
+
+def test_nnb_mode_le_matches_reference_kernel(c1):
+    """nnb_mode 1 is the x86 kernel's r2 <= rs^2: compare with the reference
+    phantomquad outputs (built from source) outside the rounding band of
+    both FP32 evaluations."""
+    L, a, _, _ = c1
+    if not O.ref_available():
+        pytest.skip("reference kernel not built")
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, nnb_mode=1)
+    r = sf.Resident.from_lists(ctx, L)
+    r.run()
+    acc, pot, nnb = r.results()
+    racc, rpot, rnnb = O.ref_soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2)
+    amb_rel = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=0.0)
+    amb_abs = O.nnb_ambiguous(**a, centres=None, eps2=0.0)
+    bad_exact, bad_band, _ = nnb_check(nnb, rnnb, amb_rel + amb_abs)
+    assert bad_exact == 0 and bad_band == 0
+    # forces agree to the reference kernel's own rsqrt accuracy
+    # (SPJ terms use the raw hardware estimate, src/petar.hpp:12-15)
+    ea = rel_err(acc, racc)
+    assert np.median(ea) < 1e-4
+    r.close()
+    ctx.close()
+
+
+def test_edge_cases():
+    """Empty walks, walks with empty lists, the self pair, zero-mass j and
+    a list longer than one chunk."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    pos = rng.normal(size=(n, 3)) * 0.02
+    mass = rng.random(n) / n
+    mass[::17] = 0.0
+    rs = np.full(n, 0.015)
+    n_epi = np.array([0, 1, 5, 64, 65, 200, 3, 0], np.int32)
+    n_epj = np.array([10, 1, 0, 2999, 700, 3000, 5, 0], np.int32)
+    n_spj = np.array([0, 0, 4, 0, 33, 100, 0, 7], np.int32)
+    ni = int(n_epi.sum())
+    ipick = rng.integers(0, n, ni)
+    # walk 1 is the self pair only
+    off = np.concatenate([[0], np.cumsum(n_epi)])
+    ide = [rng.permutation(n)[:k] for k in n_epj]
+    ide[1] = ipick[off[1]:off[1] + 1]
+    id_epj = np.concatenate(ide).astype(np.int32)
+    nsp = 50
+    spos = rng.normal(size=(nsp, 3)) * 3 + 5
+    smass = rng.random(nsp) * 1e-3
+    squad = rng.random((nsp, 6)) * 1e-4
+    id_spj = np.concatenate([rng.integers(0, nsp, k) for k in n_spj]).astype(np.int32)
+    args = dict(n_epi=n_epi, n_epj=n_epj, n_spj=n_spj, epi_pos=pos[ipick], epi_rs=rs[ipick], id_epj=id_epj,
+                id_spj=id_spj, epj_pos=pos, epj_mass=mass, epj_rs=rs, spj_pos=spos, spj_mass=smass, spj_quad=squad)
+    G = 2.5
+    ref = O.soft_force_walks(**args, eps2=1e-8, rout2=R_OUT ** 2, G=G)
+    amb = O.nnb_ambiguous(**args, centres=walk_centres(args), eps2=1e-8)
+    for mode in (sf.EXEC_FUSED, sf.EXEC_PLAIN):
+        for cj in (0, 64):
+            ctx = sf.Context(0, eps2=1e-8, rcut2=R_OUT ** 2, G=G, exec_mode=mode, chunk_j=cj)
+            r = sf.Resident(ctx, **args)
+            r.run()
+            acc, pot, nnb = r.results()
+            nz = np.linalg.norm(ref[0], axis=1) > 0
+            assert np.all(rel_err(acc[nz], ref[0][nz]) < 1e-5)
+            assert np.all(acc[~nz] == 0)
+            pz = ref[1] != 0
+            assert np.all(rel_err(pot[pz], ref[1][pz]) < 1e-5)
+            bad_exact, bad_band, _ = nnb_check(nnb, ref[2], amb)
+            assert bad_exact == 0 and bad_band == 0
+            # self pair: pot = -G m / r_out (softened by eps), acc = 0, nnb = 1
+            i1 = off[1]
+            m1 = mass[ipick[i1]]
+            assert abs(pot[i1] - (-G * m1 / np.sqrt(max(1e-8, R_OUT ** 2)))) <= 1e-6 * abs(pot[i1]) + 1e-300
+            assert nnb[i1] == 1 and np.all(acc[i1] == 0)
+            r.close()
+            ctx.close()
+
+
+def test_error_paths():
+    ctx = sf.Context(0, rcut2=1e-4)
+    fn = sf.CalcForceWithLinearCutoffMultiWalk(0, 0.0, 1e-4, 1.0)
+    # retrieve before any dispatch
+    with pytest.raises(sf.PthError):
+        fn.retrieve(0, 1, [1], [np.zeros(1, sf.FORCE_SOFT)])
+    # dispatch before send
+    fresh = sf.CalcForceWithLinearCutoffMultiWalk(0, 0.0, 2e-4, 1.0)
+    with pytest.raises(sf.PthError, match="no j arrays"):
+        fresh(0, 1, [np.zeros(1, sf.EPI_SOFT)], [1], [np.zeros(1, np.int32)], [1], [np.zeros(0, np.int32)], [0],
+              None, 0, None, 0, False)
+    # list index out of range is rejected before any kernel runs
+    epj = np.zeros(4, sf.EPJ_SOFT)
+    spj = np.zeros(0, sf.SPJ_SOFT)
+    fn(0, 0, [], [], [], [], [], [], epj, 4, spj, 0, True)
+    with pytest.raises(sf.PthError, match="out of range"):
+        fn(0, 1, [np.zeros(2, sf.EPI_SOFT)], [2], [np.array([0, 9], np.int32)], [2], [np.zeros(0, np.int32)], [0],
+           epj, 4, spj, 0, False)
+    # too many walks for one dispatch
+    with pytest.raises(sf.PthError, match="n_walk"):
+        fn(0, 1001, [np.zeros(1, sf.EPI_SOFT)] * 1001, [1] * 1001, [np.zeros(1, np.int32)] * 1001, [1] * 1001,
+           [np.zeros(0, np.int32)] * 1001, [0] * 1001, epj, 4, spj, 0, False)
+    ctx.close()
