@@ -1,0 +1,232 @@
+"""Benchmark of the soft-force path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+
+A step is one complete soft-force evaluation (all walks of the force call:
+staging of the lists + force kernel + reduction of split walks) over inputs
+already resident in HBM.  The default workload is the precision
+configuration: Plummer N = 1,048,576 with 50% primordial binaries (Kroupa
+masses, log-uniform a in [1e-5, 1e-2], thermal e), theta = 0.3,
+n_leaf_limit 20, n_group_limit 512, quadrupole SPJ, r_out = 0.01,
+r_in = 0.001, r_search = 0.015, G = 1, eps = 0.
+
+Metric: pair interactions per second, I = N_e + N_s = sum over walks of
+n_i (n_epj + n_spj).  Flop convention (reference formulas, FMA = 2, rsqrt =
+max = compare = 1): F = 26 N_e + 65 N_s.
+
+Prints one JSON line (rank 0).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+P32_PEAK_TFLOPS = 157.3   # MI355X FP32 peak (vector = matrix)
+HBM_PEAK_GBS = 8000.0
+FLOP_EPJ, FLOP_SPJ = 26, 65
+
+WORKLOADS = {
+    # name: (description, generator kwargs, r_out)
+    "c1": ("Plummer N=16384, Kroupa IMF, no binaries, theta=0.3, quad SPJ, r_out=0.01",
+           dict(n=16384, model="plummer", f_bin=0.0, seed=1), 0.01),
+    "c2": ("Plummer N=1048576, Kroupa IMF, 50% primordial binaries, theta=0.3, quad SPJ, r_out=0.01",
+           dict(n=1048576, model="plummer", f_bin=0.5, seed=2), 0.01),
+    "c3": ("King W0=7 N=4194304, Kroupa IMF, mass-segregated, theta=0.3, quad SPJ, r_out=0.01",
+           dict(n=4194304, model="king", f_bin=0.0, seed=3, segregate=True), 0.01),
+    "c5": ("Plummer N=8388608, Kroupa IMF, 100% primordial binaries, theta=0.3, quad SPJ, r_out=0.003",
+           dict(n=8388608, model="plummer", f_bin=1.0, seed=5), 0.003),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name, nthreads=0):
+    from petar_amd import host
+    desc, kw, r_out = WORKLOADS[name]
+    t0 = time.time()
+    kw = dict(kw)
+    n = kw.pop("n")
+    model = kw.pop("model")
+    m, x, _ = host.cluster(n, model, kroupa=True, **kw)
+    t1 = time.time()
+    L = host.build_lists(x, m, 1.5 * r_out, theta=0.3, n_leaf_limit=20, n_group_limit=512, nthreads=nthreads)
+    t2 = time.time()
+    log(f"[bench] workload {name}: {desc}; ic {t1 - t0:.1f}s, tree+walk {t2 - t1:.1f}s, {L.n_walk} walks")
+    return L, r_out, desc, dict(ic_s=round(t1 - t0, 2), tree_walk_s=round(t2 - t1, 2))
+
+
+def read_traffic(name):
+    """HBM bytes per force-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", f"pmc_force_{name}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                try:
+                    return float(line.split(":", 1)[1])
+                except ValueError:
+                    return None
+    return None
+
+
+def cpu_baseline(args_, r_out, target_s):
+    """Reference x86 kernel (oracle/_ref, built from the reference source) on
+    the host cores, same lists; repeated until target_s of CPU work."""
+    from oracle import oracle_py as O
+    if not O.ref_available():
+        return None
+    nt = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    isa = "avx512" if O.cpu_has_avx512() else "avx2"
+    n_int = int(np.dot(args_["n_epi"].astype(np.int64), (args_["n_epj"] + args_["n_spj"]).astype(np.int64)))
+    O.ref_soft_force_walks(**args_, eps2=0.0, rout2=r_out ** 2, nthreads=nt, isa=isa)  # warm-up
+    ts = []
+    t_start = time.time()
+    while time.time() - t_start < target_s or len(ts) < 3:
+        t = time.time()
+        O.ref_soft_force_walks(**args_, eps2=0.0, rout2=r_out ** 2, nthreads=nt, isa=isa)
+        ts.append(time.time() - t)
+    med = float(np.median(ts))
+    return dict(value=round(n_int / med / 1e9, 3), unit="Gint/s", cores=nt, kind="reference",
+                sample=f"full workload lists ({n_int:.3e} interactions), median of {len(ts)} runs, "
+                       f"phantomquad {isa} -DRSQRT_NR_EPJ_X2, {nt} OpenMP threads")
+
+
+def accuracy(res_acc, res_pot, res_nnb, L, r_out, frac=0.05, seed=0):
+    """Error vs the FP64 oracle on the same lists, on a sample of walks."""
+    from oracle import oracle_py as O
+    rng = np.random.default_rng(seed)
+    w_sel = np.sort(rng.choice(L.n_walk, max(1, int(frac * L.n_walk)), replace=False))
+    eoff, soff = L.walk_offsets()
+    ioff = np.concatenate([[0], np.cumsum(L.n_epi)])
+    idx = np.concatenate([np.arange(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w]) for w in w_sel])
+    rows = np.concatenate([np.arange(ioff[w], ioff[w + 1]) for w in w_sel])
+    a = dict(n_epi=L.n_epi[w_sel], n_epj=L.n_epj[w_sel], n_spj=L.n_spj[w_sel], epi_pos=L.epj_pos[idx],
+             epi_rs=L.epj_rs[idx],
+             id_epj=np.concatenate([L.id_epj[eoff[w]:eoff[w + 1]] for w in w_sel]),
+             id_spj=np.concatenate([L.id_spj[soff[w]:soff[w + 1]] for w in w_sel]),
+             epj_pos=L.epj_pos, epj_mass=L.epj_mass, epj_rs=L.epj_rs, spj_pos=L.spj_pos, spj_mass=L.spj_mass,
+             spj_quad=L.spj_quad)
+    racc, rpot, rnnb = O.soft_force_walks(**a, eps2=0.0, rout2=r_out ** 2)
+    ea = np.linalg.norm(res_acc[rows] - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    ep = np.abs(res_pot[rows] - rpot) / np.abs(rpot)
+    return dict(sample_i=int(rows.size), acc_rel_median=float(np.median(ea)), acc_rel_p99=float(np.percentile(ea, 99)),
+                acc_rel_max=float(ea.max()), pot_rel_median=float(np.median(ep)),
+                pot_rel_p99=float(np.percentile(ep, 99)), nnb_mismatch=int((res_nnb[rows] != rnnb).sum()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--exec", default="fused", choices=["fused", "plain"])
+    ap.add_argument("--chunk-j", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the soft-force path is "
+              f"single-GPU here and multi-GPU sharding is design-only", file=sys.stderr)
+        sys.exit(2)
+
+    from petar_amd import soft_force as sf
+    L, r_out, desc, prep = build_workload(a.workload)
+    ne, ns = L.interactions()
+    n_int = ne + ns
+    flops = FLOP_EPJ * ne + FLOP_SPJ * ns
+
+    import torch  # device-wide synchronisation around the timed region
+    exec_mode = sf.EXEC_FUSED if a.exec == "fused" else sf.EXEC_PLAIN
+    ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=a.chunk_j)
+    res = sf.Resident.from_lists(ctx, L)
+    st = res.stats()
+    log(f"[bench] resident: N_e {ne:.3e} N_s {ns:.3e} items {st['n_items']} staged {st['staged_bytes'] / 2**20:.0f} MiB")
+    for _ in range(a.warmup):
+        res.run()
+    res.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res.run()
+    res.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    log(f"[bench] timed {a.steps} steps: {ms_step:.3f} ms/step")
+
+    # per-step GPU times from HIP events on the path's stream
+    k_force, k_stage, k_red, lat = [], [], [], []
+    for _ in range(min(a.steps, 20)):
+        res.run()
+        s, f, r = res.timing()
+        k_stage.append(s)
+        k_force.append(f)
+        k_red.append(r)
+        lat.append(s + f + r)
+    force_ms = float(np.mean(k_force))
+    achieved = flops / (force_ms * 1e-3) / 1e12
+    traffic = read_traffic(a.workload)
+
+    out = {
+        "metric": "pair interactions/s (Gint/s, % FP32 peak) + p50/p99 latency at fixed force error",
+        "value": round(n_int / (ms_step * 1e-3) / 1e9, 2),
+        "unit": "Gint/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 accumulation",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "description": desc, "exec": a.exec,
+                   "interactions_per_step": n_int, "n_epep": ne, "n_epsp": ns, "walks": L.n_walk,
+                   "flops_per_step": flops, "flop_convention": "26/EPJ pair + 65/SPJ pair"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": P32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / P32_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "k_force (FP32 vector ALU, no MFMA)", "kernel_ms": round(force_ms, 4)},
+        "pct_fp32_peak_step": round(flops / (ms_step * 1e-3) / 1e12 / P32_PEAK_TFLOPS * 100, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                       "stage_ms": round(float(np.mean(k_stage)), 4), "reduce_ms": round(float(np.mean(k_red)), 4)},
+        "prep": prep,
+    }
+    if not a.no_accuracy:
+        acc, pot, nnb = res.results()
+        out["accuracy_vs_fp64_same_lists"] = accuracy(acc, pot, nnb, L, r_out)
+    res.close()
+    if not a.no_plain and a.exec == "fused":
+        ctxp = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, exec_mode=sf.EXEC_PLAIN)
+        rp = sf.Resident.from_lists(ctxp, L)
+        rp.run()
+        rp.sync()
+        tp = []
+        for _ in range(5):
+            rp.run()
+            s, f, r = rp.timing()
+            tp.append(s + f + r)
+        out["plain_form_gint_s"] = round(n_int / (np.median(tp) * 1e-3) / 1e9, 2)
+        rp.close()
+        ctxp.close()
+    ctx.close()
+    if not a.no_cpu:
+        from oracle import oracle_py as O
+        out["cpu_baseline"] = cpu_baseline(O.lists_args(L), r_out, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

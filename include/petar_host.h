@@ -40,7 +40,8 @@ int pth_ic_plummer_equal(long n, double mass_glb, double eng, unsigned int seed,
                          double *mass, double *pos, double *vel);
 
 /* Kroupa (2001) IMF masses in [m_lo, m_hi] (solar units, slopes 1.3 below
- * 0.5 Msun and 2.3 above), scaled so that they sum to m_tot. */
+ * 0.5 Msun and 2.3 above), scaled so that they sum to m_tot (m_tot <= 0:
+ * left in solar units). */
 int pth_ic_kroupa_masses(long n, double m_lo, double m_hi, double m_tot, unsigned int seed,
                          double *mass);
 

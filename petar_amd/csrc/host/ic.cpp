@@ -235,7 +235,8 @@ extern "C" int pth_ic_kroupa_masses(long n, double m_lo, double m_hi, double m_t
         mass[i] = m;
         sum += m;
     }
-    for (long i = 0; i < n; i++) mass[i] *= m_tot / sum;
+    if (m_tot > 0)
+        for (long i = 0; i < n; i++) mass[i] *= m_tot / sum;
     return 0;
 }
 
