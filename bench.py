@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--exec", default="fused", choices=["fused", "plain"])
     ap.add_argument("--chunk-j", type=int, default=0)
+    ap.add_argument("--max-ipl", type=int, default=2, choices=[1, 2, 4])
+    ap.add_argument("--j-vgpr", type=int, default=0, choices=[0, 1])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -150,7 +152,8 @@ def main():
 
     import torch  # device-wide synchronisation around the timed region
     exec_mode = sf.EXEC_FUSED if a.exec == "fused" else sf.EXEC_PLAIN
-    ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=a.chunk_j)
+    ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=a.chunk_j,
+                     max_ipl=a.max_ipl, j_vgpr=a.j_vgpr)
     res = sf.Resident.from_lists(ctx, L)
     st = res.stats()
     log(f"[bench] resident: N_e {ne:.3e} N_s {ns:.3e} items {st['n_items']} staged {st['staged_bytes'] / 2**20:.0f} MiB")

@@ -70,13 +70,21 @@ typedef struct pth_params {
     int chunk_j;   /* fused form: max j-entries per work item (0 = default) */
     int max_walk;  /* largest n_walk accepted by pth_dispatch (0 = 1000, the
                       reference's N_WALK_LIMIT, src/force_gpu_cuda.cu:14) */
+    int max_ipl;   /* fused form: most i-particles per lane, 1, 2 or 4 (0 = 4) */
+    int j_vgpr;    /* fused form: 1 copies each j record into VGPRs before use,
+                      0 uses it as SGPR operands */
 } pth_params;
 
-/* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0. */
+/* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
+ * max_walk 1000, max_ipl 2, j_vgpr 0. */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */
 int pth_create(pth_ctx **ctx, int rank, const pth_params *params);
+/* Changes eps2, rcut2, G and nnb_mode of a context between force calls (the
+ * reference functor receives them at every construction,
+ * src/force_gpu_cuda.hpp:108-118); fails while a batch awaits retrieval. */
+int pth_set_params(pth_ctx *ctx, const pth_params *params);
 /* Text of the last pth_create failure of the calling thread. */
 const char *pth_create_error(void);
 void pth_destroy(pth_ctx *ctx);
@@ -142,12 +150,13 @@ void pth_resident_free(pth_resident *res);
 /* ---- diagnostics (host only, no device needed) ----------------------------- */
 /* The work decomposition pth_dispatch / pth_resident_create use for these walk
  * sizes: returns the number of work items in launch order (or a negative
- * code).  items (8 ints each: walk, i_begin, n_i, e_blk_begin, e_blk_end,
- * s_begin, s_end, part; walk < 0 marks an empty slot), the staged offsets
- * es_off / ss_off (n_walk + 1 each) and the reduction items (4 ints each:
- * out, n_i, part_first, n_part) are written when the pointers are not NULL. */
+ * code).  items (8 ints each: walk, i_begin, n_i | ipl << 16, e_blk_begin,
+ * e_blk_end, s_begin, s_end, part; walk < 0 marks an empty slot), the staged
+ * offsets es_off / ss_off (n_walk + 1 each) and the reduction items (8 ints
+ * each: out, n_i, part_first, n_part, ipl, 0, 0, 0) are written when the
+ * pointers are not NULL. */
 long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
-                         int exec_mode, int chunk_j, int *items, long long max_items,
+                         int exec_mode, int chunk_j, int max_ipl, int *items, long long max_items,
                          long long *es_off, long long *ss_off, int *ritems,
                          long long max_ritems, long long *n_ritems);
 

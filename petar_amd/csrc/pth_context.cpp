@@ -117,11 +117,12 @@ struct Arena {
 
 // Device work buffers of one batch (staging, partials, outputs).
 struct Work {
-    DevBuf centre, staged_i, staged_e, staged_s, part_accp, part_nnb;
+    DevBuf centre, near_blk, staged_i, staged_e, staged_s, part_accp, part_nnb;
     hipError_t ensure(const Plan &P)
     {
         hipError_t e;
         if ((e = centre.ensure(sizeof(double4) * std::max<size_t>(P.walks.size(), 1)))) return e;
+        if ((e = near_blk.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)))) return e;
         if ((e = staged_i.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
         if ((e = staged_e.ensure(sizeof(float) * 5 * std::max<long long>(P.n_e_slots, 8)))) return e;
         if ((e = staged_s.ensure(sizeof(float) * 12 * std::max<long long>(P.n_s_rec, 1)))) return e;
@@ -131,7 +132,7 @@ struct Work {
     }
     void release()
     {
-        centre.release(); staged_i.release(); staged_e.release(); staged_s.release();
+        centre.release(); near_blk.release(); staged_i.release(); staged_e.release(); staged_s.release();
         part_accp.release(); part_nnb.release();
     }
 };
@@ -146,6 +147,11 @@ void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf 
     b.n_s_rec = P.n_s_rec;
     b.n_items = (int)P.items.size();
     b.n_ritems = (int)P.ritems.size();
+    for (int c = 0; c < 3; c++) {
+        b.seg_begin[c] = P.seg_begin[c];
+        b.seg_count[c] = P.seg_count[c];
+    }
+    b.j_vgpr = par.j_vgpr ? 1 : 0;
     b.plain = (par.exec_mode == PTH_EXEC_PLAIN);
     b.nnb_le = (par.nnb_mode == 1);
     b.eps2 = (float)par.eps2;
@@ -163,6 +169,7 @@ void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf 
     b.epj = epj;
     b.spj = spj;
     b.centre = W.centre.as<double4>();
+    b.near_blk = W.near_blk.as<int>();
     b.staged_i = W.staged_i.as<float4>();
     b.staged_e = W.staged_e.as<float>();
     b.staged_s = W.staged_s.as<float>();
@@ -251,7 +258,9 @@ void pth_params_default(pth_params *p)
     p->nnb_mode = 0;
     p->exec_mode = PTH_EXEC_FUSED;
     p->chunk_j = 0;
-    p->max_walk = 0;
+    p->max_walk = 1000;
+    p->max_ipl = 2;  // measured best on the precision workload (1, 2, 4: 11.2, 10.6, 11.8 ms)
+    p->j_vgpr = 0;   // SGPR operands beat VGPR copies at every max_ipl
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -310,6 +319,22 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
         }
     }
     *out = c;
+    return PTH_OK;
+}
+
+int pth_set_params(pth_ctx *c, const pth_params *p)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (!p) return c->fail(PTH_ERR_ARG, "pth_set_params: null params");
+    if (p->nnb_mode != 0 && p->nnb_mode != 1) return c->fail(PTH_ERR_ARG, "pth_set_params: nnb_mode must be 0 or 1");
+    if (!(p->rcut2 >= 0.0) || !(p->eps2 >= 0.0))
+        return c->fail(PTH_ERR_ARG, "pth_set_params: eps2 and rcut2 must be >= 0");
+    for (int t = 0; t < MAX_TAG; t++)
+        if (c->tag[t].pending) return c->fail(PTH_ERR_STATE, "pth_set_params: a batch awaits retrieval");
+    c->par.eps2 = p->eps2;
+    c->par.rcut2 = p->rcut2;
+    c->par.G = p->G;
+    c->par.nnb_mode = p->nnb_mode;
     return PTH_OK;
 }
 
@@ -427,7 +452,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         n_ids += n_spj[w];
     }
     Plan P;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, P))
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, P))
         return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: batch exceeds the int32 index range");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     Arena A;
@@ -588,7 +613,7 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
             return c->fail(PTH_ERR_ARG, "pth_resident_create: SPJ index out of range");
     pth_resident *r = new pth_resident;
     r->ctx = c;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, r->plan)) {
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan)) {
         delete r;
         return c->fail(PTH_ERR_CAPACITY, "pth_resident_create: sizes exceed the int32 index range");
     }
@@ -752,12 +777,12 @@ void pth_resident_free(pth_resident *r)
 }
 
 long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-                         int chunk_j, int *items, long long max_items, long long *es_off, long long *ss_off,
+                         int chunk_j, int max_ipl, int *items, long long max_items, long long *es_off, long long *ss_off,
                          int *ritems, long long max_ritems, long long *n_ritems)
 {
     if (n_walk < 0 || (n_walk > 0 && (!n_epi || !n_epj || !n_spj))) return PTH_ERR_ARG;
     Plan P;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, exec_mode, chunk_j, P)) return PTH_ERR_CAPACITY;
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, exec_mode, chunk_j, max_ipl, P)) return PTH_ERR_CAPACITY;
     if (es_off) std::copy(P.es_off.begin(), P.es_off.end(), es_off);
     if (ss_off) std::copy(P.ss_off.begin(), P.ss_off.end(), ss_off);
     if (n_ritems) *n_ritems = (long long)P.ritems.size();

@@ -1,5 +1,5 @@
 // pth_device.h -- device record layouts and launch descriptors shared by the
-// kernels (pth_kernels.hip) and the host side (pth_context.cpp).
+// kernels (pth_kernels.hip) and the host side (pth_context.cpp, pth_plan.cpp).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -39,26 +39,32 @@ struct WalkDev {
     int pad[2];
 };
 
-// One work item (one wave): 64-i tile of a walk against a block range of
-// the walk's staged EPJ and a record range of its staged SPJ.  walk < 0
-// marks an empty slot.  part < 0: results go to the output, else to
-// partial slot `part` (64 entries).
+// One work item (one wave): a tile of n_i <= 64*ipl i-particles of a walk
+// (lane l holds i = i_begin + l + 64 k, k < ipl) against a block range of the
+// walk's staged EPJ region and a record range of its staged SPJ.  walk < 0
+// marks an empty slot.  part < 0: results go to the output, else to the
+// partial slots part .. part+ipl-1 (64 entries each).
 struct __align__(16) Item {
-    int walk, i_begin, n_i, e_blk_begin, e_blk_end, s_begin, s_end, part;
+    int walk, i_begin, n_i_ipl, e_blk_begin, e_blk_end, s_begin, s_end, part;
 };
+__host__ __device__ inline int item_n_i(int v) { return v & 0xffff; }
+__host__ __device__ inline int item_ipl(int v) { return v >> 16; }
+__host__ __device__ inline int item_pack(int n_i, int ipl) { return n_i | (ipl << 16); }
 
-// Sum of n_part consecutive partial slots into outputs [out, out + n_i).
+// Sum of n_part chunks of partial slots into outputs [out, out + n_i):
+// i = 64 k + lane of the tile is in slot part_first + c*ipl + k of chunk c.
 struct RItem {
-    int out, n_i, part_first, n_part;
+    int out, n_i, part_first, n_part, ipl, pad[3];
 };
 
 struct BatchDev {
     int n_walk = 0;
     long long ni_tot = 0;
-    long long n_e_slots = 0;  // staged EPJ slots (multiple of 8 per walk)
+    long long n_e_slots = 0;  // staged EPJ slots (per walk: 8*ceil(n_e/8) + 8)
     long long n_s_rec = 0;    // staged SPJ records
     int n_items = 0, n_ritems = 0;
-    int plain = 0, nnb_le = 0;
+    int seg_begin[3] = {0, 0, 0}, seg_count[3] = {0, 0, 0};  // items of ipl 1, 2, 4
+    int plain = 0, nnb_le = 0, j_vgpr = 0;
     float eps2 = 0.f, rout2 = 0.f;
     double G = 1.0;
     const EpiDev *epi = nullptr;
@@ -72,6 +78,7 @@ struct BatchDev {
     const Item *items = nullptr;
     const RItem *ritems = nullptr;
     double4 *centre = nullptr;
+    int *near_blk = nullptr;  // per walk: first block of the far region (absolute)
     float4 *staged_i = nullptr;
     float *staged_e = nullptr;
     float *staged_s = nullptr;

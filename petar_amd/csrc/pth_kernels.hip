@@ -1,22 +1,29 @@
 // pth_kernels.hip -- gfx950 kernels of the soft-force path.
 //
 // Pipeline for one batch of walks (i-groups with their interaction lists):
-//   k_stage_i   one workgroup per walk: FP64 bounding-box centre of the
-//               walk's i-particles, then every i as FP32 (x-c, y-c, z-c, rs^2)
-//   k_stage_j   gather of the index lists (id_epj / id_spj) into per-walk
-//               contiguous FP32 records relative to the walk centre:
-//               EPJ as SoA blocks of 8 {x[8] y[8] z[8] m[8] rs2[8]},
-//               SPJ as 12-float records {x y z m q'xx q'yy q'zz q'xy q'yz
-//               q'zx -eps^2 tr pad} with q' = 3Q - tr I
-//   k_force     one wave per work item (walk, 64-i tile, j-range): i in
-//               registers (one per lane), j records read through the
-//               constant address space so that they arrive in SGPRs with
-//               one scalar fetch per wave; FP32 pair arithmetic, FP32
-//               partial sums per chunk, FP64 running sums
-//   k_reduce    fixed-order FP64 sum of the partial results of walks whose
-//               j-range was split over several items
-//   k_force_plain  the plain per-walk form: one workgroup per walk, every
-//               wave takes 64-i tiles of the walk over the whole list
+//   k_stage_walk  one workgroup per walk: FP64 bounding box of the walk's i
+//                 (centre c, half extents, largest r_search), every i as FP32
+//                 (x-c, y-c, z-c, rs^2), and the walk's EPJ list gathered
+//                 into FP32 records relative to c, split stably into a "near"
+//                 region (entries that may lie within max(rs_i, rs_j, r_out)
+//                 of some i of the walk) at the front and a "far" region at
+//                 the back; records are SoA blocks of 8 {x[8] y[8] z[8] m[8]
+//                 rs2[8]}, gaps padded with far massless entries
+//   k_stage_spj   gather of the SPJ lists into 12-float records relative to
+//                 the walk centre {x y z m q'xx q'yy q'zz q'xy q'yz q'zx
+//                 -eps^2 tr pad}, q' = 3Q - tr I
+//   k_force       one wave per work item (walk, tile of 64/128/256 i with
+//                 1/2/4 i per lane, j-range): j records are read through the
+//                 constant address space (uniform addresses, scalar loads into
+//                 SGPRs) and copied once into VGPRs, then used by every i of
+//                 the lane: on gfx950 FP32 add/mul/fma with only VGPR operands
+//                 issue at twice the rate of the same instruction with an
+//                 SGPR operand.  Far EPJ blocks skip the neighbour count and
+//                 the cutoff (both are known to be no-ops there).  FP32 pair
+//                 arithmetic, FP32 partial sums per chunk, FP64 running sums.
+//   k_reduce      fixed-order FP64 sum of the partial results of split walks
+//   k_force_plain the plain per-walk form (first kernel of this build): one
+//                 workgroup per walk, 64-i tiles, SGPR operands, whole list
 //
 // Pair formulas (the physics of src/force_gpu_cuda.cu:50-77 and 276-323):
 //   EPJ  dx = xj - xi, r2 = |dx|^2 + eps^2, nnb += r2 < max(rs_i,rs_j)^2,
@@ -38,44 +45,75 @@ typedef const __attribute__((address_space(4))) float *cfp;
 typedef const __attribute__((address_space(4))) int *cip;
 
 __device__ __forceinline__ cfp as_const(const float *p) { return (cfp)(p); }
-
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// With VC, copies a wave-uniform (SGPR) value into a VGPR, so that the
+// arithmetic that uses it has VGPR operands only; otherwise the SGPR value
+// is used as an operand directly.
+template <bool VC> __device__ __forceinline__ float jv(float s)
+{
+    if (!VC) return s;
+    float v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+__device__ __forceinline__ float vcopy(float s) { return jv<true>(s); }
 
 // ---------------------------------------------------------------------------
 // staging
 
-__global__ __launch_bounds__(256) void k_stage_i(const EpiDev *__restrict__ epi,
-                                                 const WalkDev *__restrict__ walks,
-                                                 double4 *__restrict__ centre,
-                                                 float4 *__restrict__ staged_i)
+__device__ __forceinline__ void put_epj(float *se, long long slot, float x, float y, float z, float m,
+                                        float r2)
+{
+    float *blk = se + (slot >> 3) * 40;
+    const int l = (int)(slot & 7);
+    blk[l] = x;
+    blk[8 + l] = y;
+    blk[16 + l] = z;
+    blk[24 + l] = m;
+    blk[32 + l] = r2;
+}
+
+__global__ __launch_bounds__(256) void k_stage_walk(
+    const EpiDev *__restrict__ epi, const WalkDev *__restrict__ walks,
+    const long long *__restrict__ es_off, const int *__restrict__ id_epj,
+    const EpjDev *__restrict__ epj, double r_out, double4 *__restrict__ centre,
+    int *__restrict__ near_blk, float4 *__restrict__ staged_i, float *__restrict__ staged_e)
 {
     const int w = blockIdx.x;
     const WalkDev wk = walks[w];
-    const int tid = threadIdx.x;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    __shared__ double red[4][7];
+    __shared__ int cnt[2][4];
+    // -- bounding box and largest r_search of the walk's i
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rsm = 0.0;
     for (int k = tid; k < wk.n_i; k += 256) {
         const EpiDev e = epi[wk.i_off + k];
         lo[0] = fmin(lo[0], e.x); hi[0] = fmax(hi[0], e.x);
         lo[1] = fmin(lo[1], e.y); hi[1] = fmax(hi[1], e.y);
         lo[2] = fmin(lo[2], e.z); hi[2] = fmax(hi[2], e.z);
+        rsm = fmax(rsm, (double)e.rs);
     }
     for (int off = 32; off > 0; off >>= 1) {
         for (int d = 0; d < 3; d++) {
             lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
             hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
         }
+        rsm = fmax(rsm, __shfl_xor(rsm, off, 64));
     }
-    __shared__ double red[4][6];
-    const int wave = tid >> 6, lane = tid & 63;
-    if (lane == 0)
+    if (lane == 0) {
         for (int d = 0; d < 3; d++) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
-    __syncthreads();
-    double c[3];
-    for (int d = 0; d < 3; d++) {
-        double l = fmin(fmin(red[0][d], red[1][d]), fmin(red[2][d], red[3][d]));
-        double h = fmax(fmax(red[0][3 + d], red[1][3 + d]), fmax(red[2][3 + d], red[3][3 + d]));
-        c[d] = (wk.n_i > 0) ? 0.5 * (l + h) : 0.0;
+        red[wave][6] = rsm;
     }
+    __syncthreads();
+    double c[3], h[3];
+    for (int d = 0; d < 3; d++) {
+        const double l = fmin(fmin(red[0][d], red[1][d]), fmin(red[2][d], red[3][d]));
+        const double u = fmax(fmax(red[0][3 + d], red[1][3 + d]), fmax(red[2][3 + d], red[3][3 + d]));
+        c[d] = (wk.n_i > 0) ? 0.5 * (l + u) : 0.0;
+        h[d] = (wk.n_i > 0) ? 0.5 * (u - l) : 0.0;
+    }
+    rsm = fmax(fmax(red[0][6], red[1][6]), fmax(red[2][6], red[3][6]));
     if (tid == 0) centre[w] = make_double4(c[0], c[1], c[2], 0.0);
     for (int k = tid; k < wk.n_i; k += 256) {
         const EpiDev e = epi[wk.i_off + k];
@@ -83,6 +121,60 @@ __global__ __launch_bounds__(256) void k_stage_i(const EpiDev *__restrict__ epi,
         staged_i[wk.i_off + k] = make_float4((float)(e.x - c[0]), (float)(e.y - c[1]),
                                              (float)(e.z - c[2]), (float)(rs * rs));
     }
+    // -- EPJ list: stable split into near (front) and far (back, reversed)
+    const long long base = es_off[w];
+    const long long cap = es_off[w + 1] - base;
+    const double hmax = fmax(h[0], fmax(h[1], h[2]));
+    int n_near = 0, n_far = 0;
+    for (int t0 = 0; t0 < wk.n_e; t0 += 256) {
+        const int t = t0 + tid;
+        const bool valid = t < wk.n_e;
+        bool nearj = false;
+        float x = 0.f, y = 0.f, z = 0.f, m = 0.f, r2 = 0.f;
+        if (valid) {
+            const EpjDev e = epj[id_epj[(long long)wk.e_off + t]];
+            const double dx = e.x - c[0], dy = e.y - c[1], dz = e.z - c[2];
+            // near: inside the walk's box widened by max(rs_i, rs_j, r_out),
+            // with a margin that covers the FP32 rounding of the pair arithmetic
+            const double R = fmax(fmax(rsm, (double)e.rs), r_out);
+            const double dm = fmax(fabs(dx), fmax(fabs(dy), fabs(dz)));
+            const double Rm = R * 1.00001 + 1e-5 * (dm + hmax);
+            nearj = fabs(dx) <= h[0] + Rm && fabs(dy) <= h[1] + Rm && fabs(dz) <= h[2] + Rm;
+            x = (float)dx;
+            y = (float)dy;
+            z = (float)dz;
+            m = e.m;
+            const double rs = (double)e.rs;
+            r2 = (float)(rs * rs);
+        }
+        const unsigned long long bn = __ballot(valid && nearj);
+        const unsigned long long bf = __ballot(valid && !nearj);
+        const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (lane == 0) {
+            cnt[0][wave] = __popcll(bn);
+            cnt[1][wave] = __popcll(bf);
+        }
+        __syncthreads();
+        int on = n_near, of = n_far;
+        for (int q = 0; q < wave; q++) {
+            on += cnt[0][q];
+            of += cnt[1][q];
+        }
+        const int tn = cnt[0][0] + cnt[0][1] + cnt[0][2] + cnt[0][3];
+        const int tf = cnt[1][0] + cnt[1][1] + cnt[1][2] + cnt[1][3];
+        if (valid) {
+            const long long slot = nearj ? base + on + __popcll(bn & below)
+                                         : base + cap - 1 - (of + __popcll(bf & below));
+            put_epj(staged_e, slot, x, y, z, m, r2);
+        }
+        n_near += tn;
+        n_far += tf;
+        __syncthreads();
+    }
+    // pad the gap between the two regions (far, massless)
+    for (long long s = base + n_near + tid; s < base + cap - n_far; s += 256)
+        put_epj(staged_e, s, 1e18f, 1e18f, 1e18f, 0.f, 0.f);
+    if (tid == 0) near_blk[w] = (int)((base + ((n_near + 7) & ~7)) >> 3);
 }
 
 __device__ __forceinline__ int walk_of(const long long *__restrict__ off, int n_walk, long long s)
@@ -90,176 +182,192 @@ __device__ __forceinline__ int walk_of(const long long *__restrict__ off, int n_
     // largest w with off[w] <= s (off is non-decreasing, off[0] = 0)
     int lo = 0, hi = n_walk;
     while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
+        const int mid = (lo + hi) >> 1;
         if (off[mid] <= s) lo = mid; else hi = mid;
     }
     return lo;
 }
 
-// Blocks [0, nb_e) gather EPJ slots, blocks [nb_e, nb_e + nb_s) SPJ records.
-__global__ __launch_bounds__(256) void k_stage_j(
-    int n_walk, const WalkDev *__restrict__ walks, const long long *__restrict__ es_off,
-    const long long *__restrict__ ss_off, const double4 *__restrict__ centre,
-    const int *__restrict__ id_epj, const int *__restrict__ id_spj,
-    const EpjDev *__restrict__ epj, const SpjDev *__restrict__ spj,
-    long long n_e_slots, long long n_s_rec, int nb_e, float eps2,
-    float *__restrict__ staged_e, float *__restrict__ staged_s)
+__global__ __launch_bounds__(256) void k_stage_spj(
+    int n_walk, const WalkDev *__restrict__ walks, const long long *__restrict__ ss_off,
+    const double4 *__restrict__ centre, const int *__restrict__ id_spj,
+    const SpjDev *__restrict__ spj, long long n_s_rec, double eps2, float *__restrict__ staged_s)
 {
-    if ((int)blockIdx.x < nb_e) {
-        const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
-        if (s >= n_e_slots) return;
-        const int w = walk_of(es_off, n_walk, s);
-        const WalkDev wk = walks[w];
-        const long long k = s - es_off[w];
-        float x = 1e18f, y = 1e18f, z = 1e18f, m = 0.f, r2 = 0.f;  // padding: far, massless
-        if (k < wk.n_e) {
-            const EpjDev e = epj[id_epj[(long long)wk.e_off + k]];
-            const double4 c = centre[w];
-            x = (float)(e.x - c.x);
-            y = (float)(e.y - c.y);
-            z = (float)(e.z - c.z);
-            m = e.m;
-            const double rs = (double)e.rs;
-            r2 = (float)(rs * rs);
-        }
-        float *blk = staged_e + (s >> 3) * 40;
-        const int l = (int)(s & 7);
-        blk[l] = x;
-        blk[8 + l] = y;
-        blk[16 + l] = z;
-        blk[24 + l] = m;
-        blk[32 + l] = r2;
-    } else {
-        const long long s = (long long)(blockIdx.x - nb_e) * 256 + threadIdx.x;
-        if (s >= n_s_rec) return;
-        const int w = walk_of(ss_off, n_walk, s);
-        const WalkDev wk = walks[w];
-        const long long k = s - ss_off[w];
-        if (k >= wk.n_s) return;
-        const SpjDev p = spj[id_spj[(long long)wk.s_off + k]];
-        const double4 c = centre[w];
-        const double qxx = p.qxx, qyy = p.qyy, qzz = p.qzz;
-        const double tr = qxx + qyy + qzz;
-        float4 *o = reinterpret_cast<float4 *>(staged_s + s * 12);
-        o[0] = make_float4((float)(p.x - c.x), (float)(p.y - c.y), (float)(p.z - c.z), p.m);
-        o[1] = make_float4((float)(3.0 * qxx - tr), (float)(3.0 * qyy - tr), (float)(3.0 * qzz - tr),
-                           (float)(3.0 * (double)p.qxy));
-        o[2] = make_float4((float)(3.0 * (double)p.qyz), (float)(3.0 * (double)p.qxz),
-                           (float)(-(double)eps2 * tr), 0.f);
-    }
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n_s_rec) return;
+    const int w = walk_of(ss_off, n_walk, s);
+    const WalkDev wk = walks[w];
+    const long long k = s - ss_off[w];
+    if (k >= wk.n_s) return;
+    const SpjDev p = spj[id_spj[(long long)wk.s_off + k]];
+    const double4 c = centre[w];
+    const double qxx = p.qxx, qyy = p.qyy, qzz = p.qzz;
+    const double tr = qxx + qyy + qzz;
+    float4 *o = reinterpret_cast<float4 *>(staged_s + s * 12);
+    o[0] = make_float4((float)(p.x - c.x), (float)(p.y - c.y), (float)(p.z - c.z), p.m);
+    o[1] = make_float4((float)(3.0 * qxx - tr), (float)(3.0 * qyy - tr), (float)(3.0 * qzz - tr),
+                       (float)(3.0 * (double)p.qxy));
+    o[2] = make_float4((float)(3.0 * (double)p.qyz), (float)(3.0 * (double)p.qxz), (float)(-eps2 * tr), 0.f);
 }
 
 // ---------------------------------------------------------------------------
-// force
+// force, fused form
 
-struct Acc {
-    double ax, ay, az, p;
-    int nnb;
+struct Consts {
+    float eps2, rout2, half, two_half;
 };
 
-template <int NNB_LE>
-__device__ __forceinline__ void epj_range(cfp se, int b_begin, int b_end, float xi, float yi,
-                                          float zi, float rsi2, float eps2, float rout2, Acc &A)
+template <int IPL, int NNB_LE, bool NEAR, bool VC>
+__device__ __forceinline__ void epj_blocks(cfp se, int b_begin, int b_end, const float (&xi)[IPL],
+                                           const float (&yi)[IPL], const float (&zi)[IPL],
+                                           const float (&rsi2)[IPL], const Consts &K, double (&A)[IPL][4],
+                                           int (&nnb)[IPL])
 {
     for (int b0 = b_begin; b0 < b_end; b0 += PTH_FLUSH_BLOCKS) {
         const int b1 = min(b0 + PTH_FLUSH_BLOCKS, b_end);
-        float ax = 0.f, ay = 0.f, az = 0.f, p = 0.f;
-        int nb = 0;
+        float ax[IPL], ay[IPL], az[IPL], p[IPL];
+        int nb[IPL];
+#pragma unroll
+        for (int q = 0; q < IPL; q++) { ax[q] = ay[q] = az[q] = p[q] = 0.f; nb[q] = 0; }
         for (int b = b0; b < b1; b++) {
             const cfp B = se + (size_t)b * 40;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const float dx = B[k] - xi;
-                const float dy = B[8 + k] - yi;
-                const float dz = B[16 + k] - zi;
-                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-                // rs^2 values are >= 0 (or -1 on idle lanes), so the signed
-                // integer maximum of their bit patterns is the float maximum
-                const float rs2 = __int_as_float(max(__float_as_int(rsi2), __float_as_int(B[32 + k])));
-                if (NNB_LE) nb += (r2 <= rs2) ? 1 : 0;
-                else nb += (r2 < rs2) ? 1 : 0;
-                const float r2c = __builtin_fmaxf(r2, rout2);
-                const float ri = __builtin_amdgcn_rsqf(r2c);
-                const float mri = B[24 + k] * ri;
-                const float ri2 = ri * ri;
-                const float mri3 = mri * ri2;
-                ax = __builtin_fmaf(mri3, dx, ax);
-                ay = __builtin_fmaf(mri3, dy, ay);
-                az = __builtin_fmaf(mri3, dz, az);
-                p -= mri;
+                const float xj = jv<VC>(B[k]), yj = jv<VC>(B[8 + k]), zj = jv<VC>(B[16 + k]);
+                const float mj = jv<VC>(B[24 + k]);
+                float rj = 0.f;
+                if (NEAR) rj = jv<VC>(B[32 + k]);
+#pragma unroll
+                for (int q = 0; q < IPL; q++) {
+                    const float dx = xj - xi[q];
+                    const float dy = yj - yi[q];
+                    const float dz = zj - zi[q];
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
+                    float ri;
+                    if (NEAR) {
+                        // rs^2 >= 0 (or -1 on idle lanes): integer max of the bit patterns
+                        const float rs2 = __int_as_float(max(__float_as_int(rsi2[q]), __float_as_int(rj)));
+                        if (NNB_LE) nb[q] += (r2 <= rs2) ? 1 : 0;
+                        else nb[q] += (r2 < rs2) ? 1 : 0;
+                        ri = __builtin_amdgcn_rsqf(__builtin_fmaxf(r2, K.rout2));
+                    } else {
+                        ri = __builtin_amdgcn_rsqf(r2);
+                    }
+                    const float mri = mj * ri;
+                    const float ri2 = ri * ri;
+                    const float mri3 = mri * ri2;
+                    ax[q] = __builtin_fmaf(mri3, dx, ax[q]);
+                    ay[q] = __builtin_fmaf(mri3, dy, ay[q]);
+                    az[q] = __builtin_fmaf(mri3, dz, az[q]);
+                    p[q] -= mri;
+                }
             }
         }
-        A.ax += (double)ax;
-        A.ay += (double)ay;
-        A.az += (double)az;
-        A.p += (double)p;
-        A.nnb += nb;
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            A[q][0] += (double)ax[q];
+            A[q][1] += (double)ay[q];
+            A[q][2] += (double)az[q];
+            A[q][3] += (double)p[q];
+            nnb[q] += nb[q];
+        }
     }
 }
 
-__device__ __forceinline__ void spj_range(cfp ss, int s_begin, int s_end, float xi, float yi,
-                                          float zi, float eps2, Acc &A)
+template <int IPL, bool VC>
+__device__ __forceinline__ void spj_records(cfp ss, int s_begin, int s_end, const float (&xi)[IPL],
+                                            const float (&yi)[IPL], const float (&zi)[IPL], const Consts &K,
+                                            double (&A)[IPL][4])
 {
     for (int s0 = s_begin; s0 < s_end; s0 += PTH_FLUSH_SPJ) {
         const int s1 = min(s0 + PTH_FLUSH_SPJ, s_end);
-        float ax = 0.f, ay = 0.f, az = 0.f, p = 0.f;
+        float ax[IPL], ay[IPL], az[IPL], p[IPL];
+#pragma unroll
+        for (int q = 0; q < IPL; q++) ax[q] = ay[q] = az[q] = p[q] = 0.f;
         for (int s = s0; s < s1; s++) {
             const cfp R = ss + (size_t)s * 12;
-            const float dx = xi - R[0];
-            const float dy = yi - R[1];
-            const float dz = zi - R[2];
-            const float m = R[3];
-            const float qxx = R[4], qyy = R[5], qzz = R[6], qxy = R[7], qyz = R[8], qzx = R[9];
-            const float mtr = R[10];
-            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-            const float ri = __builtin_amdgcn_rsqf(r2);
-            const float ri2 = ri * ri;
-            const float ri3 = ri * ri2;
-            const float ri4 = ri2 * ri2;
-            const float ri5 = ri2 * ri3;
-            const float qrx = __builtin_fmaf(qzx, dz, __builtin_fmaf(qxy, dy, qxx * dx));
-            const float qry = __builtin_fmaf(qyz, dz, __builtin_fmaf(qxy, dx, qyy * dy));
-            const float qrz = __builtin_fmaf(qyz, dy, __builtin_fmaf(qzx, dx, qzz * dz));
-            const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, mtr)));
-            const float rqr_ri4 = rqr * ri4;
-            const float meff = __builtin_fmaf(rqr_ri4, 0.5f, m);
-            const float meff3 = __builtin_fmaf(rqr_ri4, 2.5f, m) * ri3;
-            p = __builtin_fmaf(-meff, ri, p);
-            ax = __builtin_fmaf(-meff3, dx, __builtin_fmaf(ri5, qrx, ax));
-            ay = __builtin_fmaf(-meff3, dy, __builtin_fmaf(ri5, qry, ay));
-            az = __builtin_fmaf(-meff3, dz, __builtin_fmaf(ri5, qrz, az));
+            const float xs = jv<VC>(R[0]), ys = jv<VC>(R[1]), zs = jv<VC>(R[2]), m = jv<VC>(R[3]);
+            const float qxx = jv<VC>(R[4]), qyy = jv<VC>(R[5]), qzz = jv<VC>(R[6]);
+            const float qxy = jv<VC>(R[7]), qyz = jv<VC>(R[8]), qzx = jv<VC>(R[9]);
+            const float mtr = jv<VC>(R[10]);
+#pragma unroll
+            for (int q = 0; q < IPL; q++) {
+                const float dx = xi[q] - xs;
+                const float dy = yi[q] - ys;
+                const float dz = zi[q] - zs;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
+                const float ri = __builtin_amdgcn_rsqf(r2);
+                const float ri2 = ri * ri;
+                const float ri3 = ri * ri2;
+                const float ri4 = ri2 * ri2;
+                const float ri5 = ri2 * ri3;
+                const float qrx = __builtin_fmaf(qzx, dz, __builtin_fmaf(qxy, dy, qxx * dx));
+                const float qry = __builtin_fmaf(qyz, dz, __builtin_fmaf(qxy, dx, qyy * dy));
+                const float qrz = __builtin_fmaf(qyz, dy, __builtin_fmaf(qzx, dx, qzz * dz));
+                const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, mtr)));
+                const float rqr_ri4 = rqr * ri4;
+                const float meff = __builtin_fmaf(rqr_ri4, K.half, m);
+                const float meff3 = __builtin_fmaf(rqr_ri4, K.two_half, m) * ri3;
+                p[q] = __builtin_fmaf(-meff, ri, p[q]);
+                ax[q] = __builtin_fmaf(-meff3, dx, __builtin_fmaf(ri5, qrx, ax[q]));
+                ay[q] = __builtin_fmaf(-meff3, dy, __builtin_fmaf(ri5, qry, ay[q]));
+                az[q] = __builtin_fmaf(-meff3, dz, __builtin_fmaf(ri5, qrz, az[q]));
+            }
         }
-        A.ax += (double)ax;
-        A.ay += (double)ay;
-        A.az += (double)az;
-        A.p += (double)p;
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            A[q][0] += (double)ax[q];
+            A[q][1] += (double)ay[q];
+            A[q][2] += (double)az[q];
+            A[q][3] += (double)p[q];
+        }
     }
 }
 
-template <int NNB_LE>
-__device__ __forceinline__ void run_item(const Item &it, int lane, const float4 *__restrict__ staged_i,
-                                         cfp se, cfp ss, float eps2, float rout2, double G,
-                                         double4 *__restrict__ out_accp, int *__restrict__ out_nnb,
+template <int IPL, int NNB_LE, bool VC>
+__device__ __forceinline__ void run_tile(const Item &it, int near_end, int lane,
+                                         const float4 *__restrict__ staged_i, cfp se, cfp ss, const Consts &K,
+                                         double G, double4 *__restrict__ out_accp, int *__restrict__ out_nnb,
                                          double4 *__restrict__ part_accp, int *__restrict__ part_nnb)
 {
-    const bool valid = lane < it.n_i;
-    float4 xi = make_float4(0.f, 0.f, 0.f, -1.f);
-    if (valid) xi = staged_i[it.i_begin + lane];
-    Acc A{0.0, 0.0, 0.0, 0.0, 0};
-    epj_range<NNB_LE>(se, it.e_blk_begin, it.e_blk_end, xi.x, xi.y, xi.z, xi.w, eps2, rout2, A);
-    spj_range(ss, it.s_begin, it.s_end, xi.x, xi.y, xi.z, eps2, A);
-    if (!valid) return;
-    if (it.part < 0) {
-        out_accp[it.i_begin + lane] = make_double4(G * A.ax, G * A.ay, G * A.az, G * A.p);
-        out_nnb[it.i_begin + lane] = A.nnb;
-    } else {
-        part_accp[(size_t)it.part * 64 + lane] = make_double4(A.ax, A.ay, A.az, A.p);
-        part_nnb[(size_t)it.part * 64 + lane] = A.nnb;
+    const int n_i = item_n_i(it.n_i_ipl);
+    float xi[IPL], yi[IPL], zi[IPL], rsi2[IPL];
+    double A[IPL][4];
+    int nnb[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
+        if (lane + 64 * q < n_i) v = staged_i[it.i_begin + lane + 64 * q];
+        xi[q] = v.x;
+        yi[q] = v.y;
+        zi[q] = v.z;
+        rsi2[q] = v.w;
+        A[q][0] = A[q][1] = A[q][2] = A[q][3] = 0.0;
+        nnb[q] = 0;
+    }
+    const int e_mid = min(max(near_end, it.e_blk_begin), it.e_blk_end);
+    epj_blocks<IPL, NNB_LE, true, VC>(se, it.e_blk_begin, e_mid, xi, yi, zi, rsi2, K, A, nnb);
+    epj_blocks<IPL, NNB_LE, false, VC>(se, e_mid, it.e_blk_end, xi, yi, zi, rsi2, K, A, nnb);
+    spj_records<IPL, VC>(ss, it.s_begin, it.s_end, xi, yi, zi, K, A);
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const int i = lane + 64 * q;
+        if (i >= n_i) continue;
+        if (it.part < 0) {
+            out_accp[it.i_begin + i] = make_double4(G * A[q][0], G * A[q][1], G * A[q][2], G * A[q][3]);
+            out_nnb[it.i_begin + i] = nnb[q];
+        } else {
+            const size_t slot = (size_t)(it.part + q) * 64 + lane;
+            part_accp[slot] = make_double4(A[q][0], A[q][1], A[q][2], A[q][3]);
+            part_nnb[slot] = nnb[q];
+        }
     }
 }
 
-template <int NNB_LE>
+// One launch per i-per-lane class: items [0, n_items) of this class.
+template <int NNB_LE, int IPL, bool VC>
 __global__ __launch_bounds__(256) void k_force(const Item *__restrict__ items, int n_items,
+                                               const int *__restrict__ near_blk,
                                                const float4 *__restrict__ staged_i,
                                                const float *__restrict__ staged_e,
                                                const float *__restrict__ staged_s, float eps2,
@@ -274,18 +382,90 @@ __global__ __launch_bounds__(256) void k_force(const Item *__restrict__ items, i
     Item it;
     it.walk = ci[0];
     it.i_begin = ci[1];
-    it.n_i = ci[2];
+    it.n_i_ipl = ci[2];
     it.e_blk_begin = ci[3];
     it.e_blk_end = ci[4];
     it.s_begin = ci[5];
     it.s_end = ci[6];
     it.part = ci[7];
     if (it.walk < 0) return;
-    run_item<NNB_LE>(it, threadIdx.x & 63, staged_i, as_const(staged_e), as_const(staged_s), eps2,
-                     rout2, G, out_accp, out_nnb, part_accp, part_nnb);
+    const int near_end = ((cip)(near_blk))[it.walk];
+    const Consts K{jv<VC>(eps2), jv<VC>(rout2), jv<VC>(0.5f), jv<VC>(2.5f)};
+    run_tile<IPL, NNB_LE, VC>(it, near_end, threadIdx.x & 63, staged_i, as_const(staged_e), as_const(staged_s),
+                              K, G, out_accp, out_nnb, part_accp, part_nnb);
 }
 
-// One workgroup per walk, waves take the walk's 64-i tiles round robin.
+// ---------------------------------------------------------------------------
+// force, plain per-walk form: SGPR operands, one i per lane, whole list
+
+template <int NNB_LE>
+__device__ __forceinline__ void plain_epj(cfp se, int b_begin, int b_end, float xi, float yi, float zi,
+                                          float rsi2, float eps2, float rout2, double (&A)[4], int &nnb)
+{
+    for (int b0 = b_begin; b0 < b_end; b0 += PTH_FLUSH_BLOCKS) {
+        const int b1 = min(b0 + PTH_FLUSH_BLOCKS, b_end);
+        float ax = 0.f, ay = 0.f, az = 0.f, p = 0.f;
+        int nb = 0;
+        for (int b = b0; b < b1; b++) {
+            const cfp B = se + (size_t)b * 40;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float dx = B[k] - xi;
+                const float dy = B[8 + k] - yi;
+                const float dz = B[16 + k] - zi;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+                const float rs2 = __int_as_float(max(__float_as_int(rsi2), __float_as_int(B[32 + k])));
+                if (NNB_LE) nb += (r2 <= rs2) ? 1 : 0;
+                else nb += (r2 < rs2) ? 1 : 0;
+                const float ri = __builtin_amdgcn_rsqf(__builtin_fmaxf(r2, rout2));
+                const float mri = B[24 + k] * ri;
+                const float mri3 = mri * (ri * ri);
+                ax = __builtin_fmaf(mri3, dx, ax);
+                ay = __builtin_fmaf(mri3, dy, ay);
+                az = __builtin_fmaf(mri3, dz, az);
+                p -= mri;
+            }
+        }
+        A[0] += (double)ax;
+        A[1] += (double)ay;
+        A[2] += (double)az;
+        A[3] += (double)p;
+        nnb += nb;
+    }
+}
+
+__device__ __forceinline__ void plain_spj(cfp ss, int s_begin, int s_end, float xi, float yi, float zi,
+                                          float eps2, double (&A)[4])
+{
+    for (int s0 = s_begin; s0 < s_end; s0 += PTH_FLUSH_SPJ) {
+        const int s1 = min(s0 + PTH_FLUSH_SPJ, s_end);
+        float ax = 0.f, ay = 0.f, az = 0.f, p = 0.f;
+        for (int s = s0; s < s1; s++) {
+            const cfp R = ss + (size_t)s * 12;
+            const float dx = xi - R[0], dy = yi - R[1], dz = zi - R[2];
+            const float m = R[3];
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            const float ri2 = ri * ri, ri3 = ri * ri2, ri4 = ri2 * ri2, ri5 = ri2 * ri3;
+            const float qrx = __builtin_fmaf(R[9], dz, __builtin_fmaf(R[7], dy, R[4] * dx));
+            const float qry = __builtin_fmaf(R[8], dz, __builtin_fmaf(R[7], dx, R[5] * dy));
+            const float qrz = __builtin_fmaf(R[8], dy, __builtin_fmaf(R[9], dx, R[6] * dz));
+            const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, R[10])));
+            const float rqr_ri4 = rqr * ri4;
+            const float meff = __builtin_fmaf(rqr_ri4, 0.5f, m);
+            const float meff3 = __builtin_fmaf(rqr_ri4, 2.5f, m) * ri3;
+            p = __builtin_fmaf(-meff, ri, p);
+            ax = __builtin_fmaf(-meff3, dx, __builtin_fmaf(ri5, qrx, ax));
+            ay = __builtin_fmaf(-meff3, dy, __builtin_fmaf(ri5, qry, ay));
+            az = __builtin_fmaf(-meff3, dz, __builtin_fmaf(ri5, qrz, az));
+        }
+        A[0] += (double)ax;
+        A[1] += (double)ay;
+        A[2] += (double)az;
+        A[3] += (double)p;
+    }
+}
+
 template <int NNB_LE>
 __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__ walks,
                                                      const float4 *__restrict__ staged_i,
@@ -299,46 +479,49 @@ __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__
 {
     const int w = blockIdx.x;
     const WalkDev wk = walks[w];
-    const int wave = uniform((int)threadIdx.x >> 6);
+    const int wave = uniform((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int n_tiles = (wk.n_i + 63) >> 6;
-    Item it;
-    it.walk = w;
-    it.e_blk_begin = uniform((int)(es_off[w] >> 3));
-    it.e_blk_end = uniform((int)(es_off[w + 1] >> 3));
-    it.s_begin = uniform((int)ss_off[w]);
-    it.s_end = uniform((int)ss_off[w] + wk.n_s);
-    it.part = -1;
+    const int eb = uniform((int)(es_off[w] >> 3)), ee = uniform((int)(es_off[w + 1] >> 3));
+    const int sb = uniform((int)ss_off[w]), se_ = uniform((int)ss_off[w] + wk.n_s);
     for (int t = wave; t < n_tiles; t += 4) {
-        it.i_begin = wk.i_off + 64 * t;
-        it.n_i = min(64, wk.n_i - 64 * t);
-        run_item<NNB_LE>(it, threadIdx.x & 63, staged_i, as_const(staged_e), as_const(staged_s), eps2,
-                         rout2, G, out_accp, out_nnb, nullptr, nullptr);
+        const int i = 64 * t + lane;
+        float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
+        if (i < wk.n_i) v = staged_i[wk.i_off + i];
+        double A[4] = {0.0, 0.0, 0.0, 0.0};
+        int nnb = 0;
+        plain_epj<NNB_LE>(as_const(staged_e), eb, ee, v.x, v.y, v.z, v.w, eps2, rout2, A, nnb);
+        plain_spj(as_const(staged_s), sb, se_, v.x, v.y, v.z, eps2, A);
+        if (i < wk.n_i) {
+            out_accp[wk.i_off + i] = make_double4(G * A[0], G * A[1], G * A[2], G * A[3]);
+            out_nnb[wk.i_off + i] = nnb;
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems, int n_ritems,
+// One 256-thread workgroup per split tile.
+__global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems,
                                                 const double4 *__restrict__ part_accp,
                                                 const int *__restrict__ part_nnb, double G,
                                                 double4 *__restrict__ out_accp,
                                                 int *__restrict__ out_nnb)
 {
-    const int idx = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
-    if (idx >= n_ritems) return;
-    const RItem r = ritems[idx];
-    const int lane = threadIdx.x & 63;
-    if (lane >= r.n_i) return;
+    const RItem r = ritems[blockIdx.x];
+    const int t = threadIdx.x;
+    if (t >= r.n_i) return;
+    const int sub = t >> 6, lane = t & 63;
     double ax = 0, ay = 0, az = 0, p = 0;
     int nb = 0;
     for (int k = 0; k < r.n_part; k++) {
-        const double4 a = part_accp[(size_t)(r.part_first + k) * 64 + lane];
+        const size_t slot = (size_t)(r.part_first + k * r.ipl + sub) * 64 + lane;
+        const double4 a = part_accp[slot];
         ax += a.x;
         ay += a.y;
         az += a.z;
         p += a.w;
-        nb += part_nnb[(size_t)(r.part_first + k) * 64 + lane];
+        nb += part_nnb[slot];
     }
-    out_accp[r.out + lane] = make_double4(G * ax, G * ay, G * az, G * p);
-    out_nnb[r.out + lane] = nb;
+    out_accp[r.out + t] = make_double4(G * ax, G * ay, G * az, G * p);
+    out_nnb[r.out + t] = nb;
 }
 
 // ---------------------------------------------------------------------------
@@ -347,13 +530,12 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 hipError_t launch_stage(const BatchDev &b, hipStream_t st)
 {
     if (b.n_walk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stage_i, dim3(b.n_walk), dim3(256), 0, st, b.epi, b.walks, b.centre,
-                       b.staged_i);
-    const long long nbe = (b.n_e_slots + 255) / 256, nbs = (b.n_s_rec + 255) / 256;
-    if (nbe + nbs > 0)
-        hipLaunchKernelGGL(k_stage_j, dim3((unsigned)(nbe + nbs)), dim3(256), 0, st, b.n_walk,
-                           b.walks, b.es_off, b.ss_off, b.centre, b.id_epj, b.id_spj, b.epj, b.spj,
-                           b.n_e_slots, b.n_s_rec, (int)nbe, b.eps2, b.staged_e, b.staged_s);
+    hipLaunchKernelGGL(k_stage_walk, dim3(b.n_walk), dim3(256), 0, st, b.epi, b.walks, b.es_off, b.id_epj,
+                       b.epj, std::sqrt((double)b.rout2), b.centre, b.near_blk, b.staged_i, b.staged_e);
+    const long long nbs = (b.n_s_rec + 255) / 256;
+    if (nbs > 0)
+        hipLaunchKernelGGL(k_stage_spj, dim3((unsigned)nbs), dim3(256), 0, st, b.n_walk, b.walks, b.ss_off,
+                           b.centre, b.id_spj, b.spj, b.n_s_rec, (double)b.eps2, b.staged_s);
     return hipGetLastError();
 }
 
@@ -371,24 +553,42 @@ hipError_t launch_force(const BatchDev &b, hipStream_t st)
                                b.out_accp, b.out_nnb);
         return hipGetLastError();
     }
-    if (b.n_items <= 0) return hipSuccess;
-    const unsigned nblk = (unsigned)((b.n_items + 3) / 4);
-    if (b.nnb_le)
-        hipLaunchKernelGGL(k_force<1>, dim3(nblk), dim3(256), 0, st, b.items, b.n_items, b.staged_i,
-                           b.staged_e, b.staged_s, b.eps2, b.rout2, b.G, b.out_accp, b.out_nnb,
-                           b.part_accp, b.part_nnb);
-    else
-        hipLaunchKernelGGL(k_force<0>, dim3(nblk), dim3(256), 0, st, b.items, b.n_items, b.staged_i,
-                           b.staged_e, b.staged_s, b.eps2, b.rout2, b.G, b.out_accp, b.out_nnb,
-                           b.part_accp, b.part_nnb);
-    return hipGetLastError();
+    static const int ipls[3] = {1, 2, 4};
+    for (int c = 0; c < 3; c++) {
+        const int n = b.seg_count[c];
+        if (n <= 0) continue;
+        const Item *it = b.items + b.seg_begin[c];
+        const unsigned nblk = (unsigned)((n + 3) / 4);
+#define PTH_LAUNCH(NNB, IPL, VC)                                                                           \
+    hipLaunchKernelGGL((k_force<NNB, IPL, VC>), dim3(nblk), dim3(256), 0, st, it, n, b.near_blk, b.staged_i, \
+                       b.staged_e, b.staged_s, b.eps2, b.rout2, b.G, b.out_accp, b.out_nnb, b.part_accp,     \
+                       b.part_nnb)
+#define PTH_LAUNCH_IPL(NNB, VC)                  \
+    do {                                         \
+        if (ipls[c] == 1) PTH_LAUNCH(NNB, 1, VC); \
+        else if (ipls[c] == 2) PTH_LAUNCH(NNB, 2, VC); \
+        else PTH_LAUNCH(NNB, 4, VC);             \
+    } while (0)
+        if (b.nnb_le) {
+            if (b.j_vgpr) PTH_LAUNCH_IPL(1, true);
+            else PTH_LAUNCH_IPL(1, false);
+        } else {
+            if (b.j_vgpr) PTH_LAUNCH_IPL(0, true);
+            else PTH_LAUNCH_IPL(0, false);
+        }
+#undef PTH_LAUNCH_IPL
+#undef PTH_LAUNCH
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_reduce(const BatchDev &b, hipStream_t st)
 {
     if (b.plain || b.n_ritems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)((b.n_ritems + 3) / 4)), dim3(256), 0, st, b.ritems,
-                       b.n_ritems, b.part_accp, b.part_nnb, b.G, b.out_accp, b.out_nnb);
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)b.n_ritems), dim3(256), 0, st, b.ritems, b.part_accp,
+                       b.part_nnb, b.G, b.out_accp, b.out_nnb);
     return hipGetLastError();
 }
 

@@ -1,14 +1,18 @@
 // pth_plan.cpp -- work decomposition of a batch of walks (see pth_plan.h).
 //
-// The fused form cuts every walk into 64-i tiles and, when the walk's list
-// is long, its staged list into chunks of at most chunk_j cost units (EPJ
-// blocks first, then SPJ records).  Chunk boundaries depend only on the
-// walk's own sizes, so a walk gives bit-identical results whatever batch it
-// is part of.  Items are then dealt to the 8 XCDs as 8 contiguous runs of
-// equal total cost (neighbouring items share walks, hence staged lists, so
-// each list is fetched into one XCD's L2), most expensive first inside each
-// run, and written in the order the hardware hands workgroups to XCDs
-// (round robin), padding short runs with empty items.
+// The fused form cuts every walk into tiles of 64, 128 or 256 i-particles
+// (1, 2 or 4 per lane), choosing for each walk the tiling of least issue
+// cost: more i per lane share the per-wave copy of each j record, fewer
+// leave fewer lanes idle on small walks.  When the walk's list is long it
+// is cut into chunks of at most chunk_j list entries (an SPJ counts twice),
+// EPJ blocks first, then SPJ records.  Tiling and chunk boundaries depend
+// only on the walk's own sizes, so a walk gives bit-identical results
+// whatever batch it is part of.  Items are then dealt to the 8 XCDs as 8
+// contiguous runs of equal total cost (neighbouring items share walks,
+// hence staged lists, so each list is fetched into one XCD's L2), most
+// expensive first inside each run, and written in the order the hardware
+// hands workgroups to XCDs (round robin), padding short runs with empty
+// items.
 #include "pth_plan.h"
 
 #include <algorithm>
@@ -18,8 +22,54 @@
 
 namespace pth {
 
+namespace {
+
+struct Tile {
+    int begin, n, ipl;
+};
+
+// Least-cost cover of n i-particles by tiles of 256 (ipl 4), 128 (ipl 2)
+// and 64 (ipl 1) lanes x i, for a walk with n_e EPJ and n_s SPJ.
+std::vector<Tile> tile_walk(int n, long long n_e, long long n_s, int max_ipl)
+{
+    std::vector<Tile> out;
+    if (n <= 0) return out;
+    auto cost = [&](int ipl) { return 64.0 * ipl * (epj_unit_cost(ipl) * n_e + spj_unit_cost(ipl) * n_s); };
+    const int amax = (max_ipl >= 4) ? (n + 255) / 256 : 0;
+    const int bmax = (max_ipl >= 2) ? 3 : 0;
+    double best = -1;
+    int ba = 0, bb = 0, bc = 0;
+    for (int a = 0; a <= amax; a++)
+        for (int b = 0; b <= bmax; b++) {
+            const long long covered = 256LL * a + 128LL * b;
+            const int c = covered >= n ? 0 : (int)((n - covered + 63) / 64);
+            if (max_ipl < 2 && b > 0) continue;
+            const double t = a * cost(4) + b * cost(2) + c * cost(1);
+            if (best < 0 || t < best - 1e-9) {
+                best = t;
+                ba = a;
+                bb = b;
+                bc = c;
+            }
+        }
+    int pos = 0;
+    auto emit = [&](int count, int ipl) {
+        for (int k = 0; k < count && pos < n; k++) {
+            const int m = std::min(64 * ipl, n - pos);
+            out.push_back(Tile{pos, m, ipl});
+            pos += m;
+        }
+    };
+    emit(ba, 4);
+    emit(bb, 2);
+    emit(bc, 1);
+    return out;
+}
+
+}  // namespace
+
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, Plan &P)
+               int chunk_j, int max_ipl, Plan &P)
 {
     P = Plan();
     P.walks.resize(n_walk);
@@ -38,105 +88,121 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         io += n_epi[w];
         eo += n_epj[w];
         so += n_spj[w];
-        P.es_off[w + 1] = P.es_off[w] + ((long long)n_epj[w] + 7) / 8 * 8;
+        P.es_off[w + 1] = P.es_off[w] + epj_region_slots(n_epj[w]);
         P.ss_off[w + 1] = P.ss_off[w] + n_spj[w];
         P.n_epep += (long long)n_epi[w] * n_epj[w];
         P.n_epsp += (long long)n_epi[w] * n_spj[w];
+        if (io > INT_MAX || eo > INT_MAX || so > INT_MAX) return false;
     }
     P.ni_tot = io;
     P.n_e_slots = P.es_off[n_walk];
     P.n_s_rec = P.ss_off[n_walk];
-    if (io > INT_MAX || eo > INT_MAX || so > INT_MAX || P.n_e_slots / 8 > INT_MAX ||
-        P.n_s_rec > INT_MAX)
-        return false;
+    if (P.n_e_slots / 8 > INT_MAX || P.n_s_rec > INT_MAX) return false;
     if (exec_mode == PTH_EXEC_PLAIN) return true;
+    if (max_ipl != 1 && max_ipl != 2) max_ipl = 4;
 
     const long long cj = (chunk_j > 0) ? chunk_j : DEFAULT_CHUNK_J;
     struct Tmp {
         Item it;
-        long long cost;
+        double cost;
     };
     std::vector<Tmp> seq;
     for (int w = 0; w < n_walk; w++) {
         const WalkDev &W = P.walks[w];
         if (W.n_i <= 0) continue;
         const long long nb = (P.es_off[w + 1] - P.es_off[w]) / 8;
-        const long long units = nb * 8 + (long long)SPJ_COST * W.n_s;
+        const long long units = nb * 8 + 2LL * W.n_s;
         long long nch = (units + cj - 1) / cj;
         if (nch < 1) nch = 1;
-        // chunk k covers cost units [k*units/nch, (k+1)*units/nch); EPJ
+        // chunk k covers list units [k*units/nch, (k+1)*units/nch); EPJ
         // boundaries are rounded to whole blocks of 8
         std::vector<long long> eb(nch + 1), sb(nch + 1);
         for (long long k = 0; k <= nch; k++) {
             const long long u = (k == nch) ? units : (k * units) / nch;
-            long long e = std::min(u, nb * 8);
-            e = (e + 7) / 8;  // blocks
-            const long long su = std::max(0LL, u - nb * 8);
-            eb[k] = std::min(e, nb);
-            sb[k] = std::min((su + SPJ_COST - 1) / SPJ_COST, (long long)W.n_s);
+            eb[k] = std::min((std::min(u, nb * 8) + 7) / 8, nb);
+            sb[k] = std::min((std::max(0LL, u - nb * 8) + 1) / 2, (long long)W.n_s);
         }
         eb[nch] = nb;
         sb[nch] = W.n_s;
-        const int n_tiles = (W.n_i + 63) / 64;
-        for (int t = 0; t < n_tiles; t++) {
+        for (const Tile &t : tile_walk(W.n_i, W.n_e, W.n_s, max_ipl)) {
             const int part_first = P.n_part;
             for (long long k = 0; k < nch; k++) {
                 Item it;
                 it.walk = w;
-                it.i_begin = W.i_off + 64 * t;
-                it.n_i = std::min(64, W.n_i - 64 * t);
+                it.i_begin = W.i_off + t.begin;
+                it.n_i_ipl = item_pack(t.n, t.ipl);
                 it.e_blk_begin = (int)(P.es_off[w] / 8 + eb[k]);
                 it.e_blk_end = (int)(P.es_off[w] / 8 + eb[k + 1]);
                 it.s_begin = (int)(P.ss_off[w] + sb[k]);
                 it.s_end = (int)(P.ss_off[w] + sb[k + 1]);
-                it.part = (nch > 1) ? P.n_part++ : -1;
-                const long long c = 8LL * (it.e_blk_end - it.e_blk_begin) +
-                                    (long long)SPJ_COST * (it.s_end - it.s_begin) + 8;
+                it.part = -1;
+                if (nch > 1) {
+                    it.part = P.n_part;
+                    P.n_part += t.ipl;
+                }
+                const double c = 64.0 * t.ipl *
+                                 (epj_unit_cost(t.ipl) * 8.0 * (it.e_blk_end - it.e_blk_begin) +
+                                  spj_unit_cost(t.ipl) * (it.s_end - it.s_begin) + 50.0);
                 seq.push_back({it, c});
             }
-            if (nch > 1) P.ritems.push_back(RItem{W.i_off + 64 * t, std::min(64, W.n_i - 64 * t),
-                                                  part_first, (int)nch});
+            if (nch > 1) {
+                RItem r;
+                r.out = W.i_off + t.begin;
+                r.n_i = t.n;
+                r.part_first = part_first;
+                r.n_part = (int)nch;
+                r.ipl = t.ipl;
+                r.pad[0] = r.pad[1] = r.pad[2] = 0;
+                P.ritems.push_back(r);
+            }
         }
     }
     P.n_real_items = (int)seq.size();
-    // 8 contiguous runs of equal cost
-    long long total = 0;
-    for (auto &s : seq) total += s.cost;
-    std::vector<std::vector<Item>> run(XCDS);
-    {
-        long long acc = 0;
+    // one launch per ipl class (each kernel has its own register budget);
+    // inside a class, 8 contiguous runs of equal cost, one per XCD
+    Item dummy;
+    dummy.walk = -1;
+    dummy.i_begin = dummy.n_i_ipl = dummy.e_blk_begin = dummy.e_blk_end = dummy.s_begin = dummy.s_end = 0;
+    dummy.part = -1;
+    static const int class_ipl[3] = {1, 2, 4};
+    for (int cls = 0; cls < 3; cls++) {
+        std::vector<Tmp> sub;
+        for (auto &s : seq)
+            if (item_ipl(s.it.n_i_ipl) == class_ipl[cls]) sub.push_back(s);
+        P.seg_begin[cls] = (int)P.items.size();
+        if (sub.empty()) continue;
+        double total = 0;
+        for (auto &s : sub) total += s.cost;
+        std::vector<std::vector<Item>> run(XCDS);
+        double acc = 0;
         int x = 0;
-        size_t k = 0;
         std::vector<Tmp> cur;
         auto flush = [&](int xx) {
-            std::stable_sort(cur.begin(), cur.end(),
-                             [](const Tmp &a, const Tmp &b) { return a.cost > b.cost; });
+            std::stable_sort(cur.begin(), cur.end(), [](const Tmp &a, const Tmp &b) { return a.cost > b.cost; });
             for (auto &c : cur) run[xx].push_back(c.it);
             cur.clear();
         };
-        for (; k < seq.size(); k++) {
-            // item goes to run x while the running cost is below (x+1)/8 of the total
-            while (x < XCDS - 1 && acc >= (total * (x + 1)) / XCDS) {
+        for (size_t k = 0; k < sub.size(); k++) {
+            // an item goes to run x while the running cost is below (x+1)/8 of the total
+            while (x < XCDS - 1 && acc >= total * (x + 1) / XCDS) {
                 flush(x);
                 x++;
             }
-            cur.push_back(seq[k]);
-            acc += seq[k].cost;
+            cur.push_back(sub[k]);
+            acc += sub[k].cost;
         }
         flush(x);
+        size_t max_blocks = 0;
+        for (auto &r : run) max_blocks = std::max(max_blocks, (r.size() + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
+        const size_t base = P.items.size();
+        P.items.resize(base + max_blocks * XCDS * ITEMS_PER_BLOCK, dummy);
+        for (int xx = 0; xx < XCDS; xx++)
+            for (size_t k = 0; k < run[xx].size(); k++) {
+                const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
+                P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = run[xx][k];
+            }
+        P.seg_count[cls] = (int)(P.items.size() - base);
     }
-    size_t max_blocks = 0;
-    for (auto &r : run) max_blocks = std::max(max_blocks, (r.size() + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
-    Item dummy;
-    dummy.walk = -1;
-    dummy.i_begin = dummy.n_i = dummy.e_blk_begin = dummy.e_blk_end = dummy.s_begin = dummy.s_end = 0;
-    dummy.part = -1;
-    P.items.assign(max_blocks * XCDS * ITEMS_PER_BLOCK, dummy);
-    for (int x = 0; x < XCDS; x++)
-        for (size_t k = 0; k < run[x].size(); k++) {
-            const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
-            P.items[(r * XCDS + x) * ITEMS_PER_BLOCK + q] = run[x][k];
-        }
     return true;
 }
 

@@ -9,28 +9,39 @@
 
 namespace pth {
 
-// Relative cost of one staged SPJ record against one EPJ slot (VALU
-// instructions per pair of the two inner loops, 33 vs 18, rounded).
-constexpr int SPJ_COST = 2;
-constexpr int XCDS = 8;          // MI355X: 8 XCDs, workgroup b runs on XCD b % 8
+constexpr int XCDS = 8;             // MI355X: 8 XCDs, workgroup b runs on XCD b % 8
 constexpr int ITEMS_PER_BLOCK = 4;  // one item per wave, 256-thread workgroups
 constexpr int DEFAULT_CHUNK_J = 2048;
+
+// Issue cost of one pair per lane, in units of one full-rate (4-cycle)
+// wave64 VALU instruction, for a tile holding ipl i-particles per lane:
+// EPJ (mix of near and far blocks) and SPJ, including the per-wave copy of
+// the j record into VGPRs shared by the ipl i-particles.
+inline double epj_unit_cost(int ipl) { return 9.5 + 2.25 / ipl; }
+inline double spj_unit_cost(int ipl) { return 18.5 + 5.5 / ipl; }
 
 struct Plan {
     std::vector<WalkDev> walks;
     std::vector<long long> es_off, ss_off;  // n_walk + 1
-    std::vector<Item> items;                // physical (launch) order, dummies have walk < 0
+    std::vector<Item> items;                // launch order, empty slots have walk < 0
+    int seg_begin[3] = {0, 0, 0};           // items of ipl 1, 2, 4: one launch each
+    int seg_count[3] = {0, 0, 0};
     std::vector<RItem> ritems;
-    int n_part = 0;
+    int n_part = 0;                         // partial slots of 64 entries
     int n_real_items = 0;
     long long ni_tot = 0, n_e_slots = 0, n_s_rec = 0;
     long long n_epep = 0, n_epsp = 0;       // sum n_i n_e, sum n_i n_s
 };
 
+// Staged EPJ slots of a walk: near and far regions with at least 8 slots of
+// padding between them.
+inline long long epj_region_slots(long long n_e) { return n_e > 0 ? (n_e + 7) / 8 * 8 + 8 : 0; }
+
 // Builds the plan; exec_mode PTH_EXEC_FUSED or PTH_EXEC_PLAIN, chunk_j the
-// largest cost (in EPJ-slot units) of one item (<= 0: default).
+// largest list length (EPJ entries + 2 x SPJ entries) of one item (<= 0:
+// default), max_ipl the largest number of i per lane (1, 2 or 4).
 // Returns false if a size overflows the int32 ranges of the device records.
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, Plan &P);
+               int chunk_j, int max_ipl, Plan &P);
 
 }  // namespace pth

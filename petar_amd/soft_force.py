@@ -48,7 +48,8 @@ assert EPI_SOFT.itemsize == 48 and EPJ_SOFT.itemsize == 120 and FORCE_SOFT.items
 
 class Params(C.Structure):
     _fields_ = [("eps2", C.c_double), ("rcut2", C.c_double), ("G", C.c_double), ("nnb_mode", C.c_int),
-                ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int)]
+                ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int), ("max_ipl", C.c_int),
+                ("j_vgpr", C.c_int)]
 
 
 class EpjView(C.Structure):
@@ -69,7 +70,7 @@ class ForceView(C.Structure):
                 ("off_nngb", C.c_size_t), ("nngb_bytes", C.c_int)]
 
 
-EXPORTED = ("pth_params_default", "pth_create", "pth_create_error", "pth_destroy", "pth_last_error",
+EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_error", "pth_destroy", "pth_last_error",
             "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve", "pth_resident_create",
             "pth_resident_run", "pth_resident_sync", "pth_resident_results", "pth_resident_timing",
             "pth_resident_stats", "pth_resident_free", "pth_plan_debug")
@@ -88,6 +89,7 @@ def lib():
     dp = C.POINTER(C.c_double)
     L.pth_params_default.argtypes = [C.POINTER(Params)]
     L.pth_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(Params)]
+    L.pth_set_params.argtypes = [vp, C.POINTER(Params)]
     L.pth_create_error.restype = C.c_char_p
     L.pth_destroy.argtypes = [vp]
     L.pth_last_error.argtypes = [vp]
@@ -106,8 +108,9 @@ def lib():
     L.pth_resident_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 4
     L.pth_resident_free.argtypes = [vp]
     L.pth_plan_debug.restype = C.c_longlong
-    L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, ip, C.c_longlong, C.POINTER(C.c_longlong),
-                                 C.POINTER(C.c_longlong), ip, C.c_longlong, C.POINTER(C.c_longlong)]
+    L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
+                                 C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip, C.c_longlong,
+                                 C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -128,11 +131,16 @@ class PthError(RuntimeError):
     pass
 
 
-def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=0):
+def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=1000,
+                   max_ipl=None, j_vgpr=None):
     p = Params()
     lib().pth_params_default(C.byref(p))
     p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk = (
         eps2, rcut2, G, nnb_mode, exec_mode, chunk_j, max_walk)
+    if max_ipl is not None:
+        p.max_ipl = max_ipl
+    if j_vgpr is not None:
+        p.j_vgpr = j_vgpr
     return p
 
 
@@ -332,21 +340,22 @@ class Resident:
             pass
 
 
-def plan_debug(n_epi, n_epj, n_spj, exec_mode=EXEC_FUSED, chunk_j=0):
-    """The work decomposition (host only): items [k, 8], es_off, ss_off, ritems [k, 4]."""
+def plan_debug(n_epi, n_epj, n_spj, exec_mode=EXEC_FUSED, chunk_j=0, max_ipl=4):
+    """The work decomposition (host only): items [k, 8] (column 2 packs
+    n_i | ipl << 16), es_off, ss_off, ritems [k, 8]."""
     L = lib()
     ne, nj, ns = _i32(n_epi), _i32(n_epj), _i32(n_spj)
     nw = ne.size
     nr = C.c_longlong()
-    n = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j, None, 0,
-                         None, None, None, 0, C.byref(nr))
+    n = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j, max_ipl,
+                         None, 0, None, None, None, 0, C.byref(nr))
     if n < 0:
         raise PthError(f"pth_plan_debug failed ({n})")
     items = np.zeros((max(n, 1), 8), np.int32)
-    rit = np.zeros((max(nr.value, 1), 4), np.int32)
+    rit = np.zeros((max(nr.value, 1), 8), np.int32)
     es = np.zeros(nw + 1, np.int64)
     ss = np.zeros(nw + 1, np.int64)
-    n2 = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j,
+    n2 = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j, max_ipl,
                           _ptr(items, C.c_int), items.shape[0], _ptr(es, C.c_longlong), _ptr(ss, C.c_longlong),
                           _ptr(rit, C.c_int), rit.shape[0], C.byref(nr))
     if n2 < 0:

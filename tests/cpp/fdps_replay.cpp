@@ -1,0 +1,163 @@
+// fdps_replay.cpp -- drives include/petar_hip_fdps.hpp the way FDPS's
+// calcForceAllAndWriteBackMultiWalkIndex drives the reference GPU functor
+// (src/petar.hpp:887-899): one send_flag == true call with every EPJ and SPJ,
+// then batches of at most n_walk_limit walks, each dispatched (send_flag ==
+// false) and retrieved into ForceSoft records, with the names petar.hpp uses
+// (CalcForceWithLinearCutoffCUDAMultiWalk, RetrieveForceCUDA).
+//
+// The walks come from the repository's own tree (libpetar_host).  The
+// program writes the inputs it used and the retrieved forces to a binary
+// file that tests/test_gpu_dropin.py checks against the FP64 oracle.
+//
+// usage: fdps_replay N seed theta n_walk_limit r_out out_file
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "particle_simulator.hpp"
+#include "soft_types.hpp"
+#include "petar_hip_fdps.hpp"
+#include "petar_host.h"
+
+static void die(const char *what)
+{
+    std::fprintf(stderr, "fdps_replay: %s: %s\n", what, pth_host_last_error());
+    std::exit(2);
+}
+
+template <class T> static void put(FILE *f, const std::vector<T> &v)
+{
+    if (!v.empty() && std::fwrite(v.data(), sizeof(T), v.size(), f) != v.size()) die("write");
+}
+
+int main(int argc, char **argv)
+{
+    if (argc != 7) {
+        std::fprintf(stderr, "usage: %s N seed theta n_walk_limit r_out out_file\n", argv[0]);
+        return 1;
+    }
+    const long n = std::atol(argv[1]);
+    const unsigned seed = (unsigned)std::atoi(argv[2]);
+    const double theta = std::atof(argv[3]);
+    const int n_walk_limit = std::atoi(argv[4]);
+    const double r_out = std::atof(argv[5]);
+    const double r_search = 1.5 * r_out;
+
+    std::vector<double> mass(n), pos(3 * n);
+    if (pth_ic_cluster(n, 0, 0.0, 1, 0.3, 1e-5, 1e-2, 0, seed, mass.data(), pos.data(), nullptr)) die("ic");
+    std::vector<double> rs(n, r_search);
+    pth_tree *tree = nullptr;
+    if (pth_tree_build(&tree, n, pos.data(), mass.data(), rs.data(), 20, 512, 0)) die("tree");
+    if (pth_tree_walk(tree, theta)) die("walk");
+    long n_node, n_group, n_le, n_ls;
+    pth_tree_sizes(tree, &n_node, &n_group, &n_le, &n_ls);
+    std::vector<long> order(n), nb(n_node), ne(n_node), epi_begin(n_group);
+    std::vector<int> lvl(n_node), n_epi(n_group), n_epj(n_group), n_spj(n_group);
+    std::vector<double> smass(n_node), scom(3 * n_node), squad(6 * n_node);
+    std::vector<int> id_epj(n_le), id_spj(n_ls);
+    pth_tree_order(tree, order.data());
+    pth_tree_nodes(tree, smass.data(), scom.data(), squad.data(), nb.data(), ne.data(), lvl.data());
+    pth_tree_groups(tree, epi_begin.data(), n_epi.data(), n_epj.data(), n_spj.data());
+    pth_tree_lists(tree, id_epj.data(), id_spj.data());
+    pth_tree_free(tree);
+
+    // FDPS-side records (tree order)
+    std::vector<EPJSoft> epj(n);
+    std::vector<double> epj_pos(3 * n), epj_mass(n), epj_rs(n);
+    for (long k = 0; k < n; k++) {
+        const long i = order[k];
+        std::memset(&epj[k], 0, sizeof(EPJSoft));
+        epj[k].id = i + 1;
+        epj[k].mass = mass[i];
+        epj[k].pos = PS::F64vec(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        epj[k].r_in = 0.1 * r_out;
+        epj[k].r_out = r_out;
+        epj[k].r_search = r_search;
+        epj[k].r_scale_next = 1.0;
+        for (int d = 0; d < 3; d++) epj_pos[3 * k + d] = pos[3 * i + d];
+        epj_mass[k] = mass[i];
+        epj_rs[k] = r_search;
+    }
+    std::vector<PS::SPJQuadrupoleInAndOut> spj(n_node);
+    for (long s = 0; s < n_node; s++) {
+        spj[s].mass = smass[s];
+        spj[s].pos = PS::F64vec(scom[3 * s], scom[3 * s + 1], scom[3 * s + 2]);
+        const double *q = &squad[6 * s];
+        spj[s].quad = PS::F64mat{q[0], q[1], q[2], q[3], q[4], q[5]};
+    }
+    std::vector<std::vector<EPISoft>> epi(n_group);
+    std::vector<std::vector<ForceSoft>> force(n_group);
+    std::vector<long> eoff(n_group + 1, 0), soff(n_group + 1, 0);
+    for (long g = 0; g < n_group; g++) {
+        eoff[g + 1] = eoff[g] + n_epj[g];
+        soff[g + 1] = soff[g] + n_spj[g];
+        epi[g].resize(n_epi[g]);
+        force[g].resize(n_epi[g]);
+        for (int i = 0; i < n_epi[g]; i++) {
+            const EPJSoft &p = epj[epi_begin[g] + i];
+            epi[g][i].id = p.id;
+            epi[g][i].pos = p.pos;
+            epi[g][i].r_search = p.r_search;
+            epi[g][i].rank_org = 0;
+            epi[g][i].type = 1;
+            force[g][i].pot = 99.0;  // retrieve must overwrite
+        }
+    }
+
+    // FDPS multi-walk call sequence
+    const PS::F64 eps2 = 0.0, rout2 = r_out * r_out, G = 1.0;
+    CalcForceWithLinearCutoffCUDAMultiWalk f(0, eps2, rout2, G);
+    f(0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, epj.data(), (PS::S32)n, spj.data(),
+      (PS::S32)n_node, true);
+    for (long g0 = 0; g0 < n_group; g0 += n_walk_limit) {
+        const int nw = (int)std::min<long>(n_walk_limit, n_group - g0);
+        std::vector<const EPISoft *> pe(nw);
+        std::vector<const PS::S32 *> pj(nw), ps(nw);
+        std::vector<ForceSoft *> pf(nw);
+        for (int w = 0; w < nw; w++) {
+            pe[w] = epi[g0 + w].data();
+            pj[w] = id_epj.data() + eoff[g0 + w];
+            ps[w] = id_spj.data() + soff[g0 + w];
+            pf[w] = force[g0 + w].data();
+        }
+        CalcForceWithLinearCutoffCUDAMultiWalk fw(0, eps2, rout2, G);
+        fw(0, nw, pe.data(), n_epi.data() + g0, pj.data(), n_epj.data() + g0, ps.data(), n_spj.data() + g0,
+           epj.data(), (PS::S32)n, spj.data(), (PS::S32)n_node, false);
+        RetrieveForceCUDA(0, nw, n_epi.data() + g0, pf.data());
+    }
+
+    std::vector<double> acc, pot;
+    std::vector<long long> nngb;
+    for (long g = 0; g < n_group; g++)
+        for (auto &fr : force[g]) {
+            acc.push_back(fr.acc.x);
+            acc.push_back(fr.acc.y);
+            acc.push_back(fr.acc.z);
+            pot.push_back(fr.pot);
+            nngb.push_back(fr.n_ngb);
+        }
+    FILE *out = std::fopen(argv[6], "wb");
+    if (!out) die("open output");
+    std::vector<long long> hdr{n, n_node, n_group, n_le, n_ls};
+    put(out, hdr);
+    put(out, epj_pos);
+    put(out, epj_mass);
+    put(out, epj_rs);
+    put(out, scom);
+    put(out, smass);
+    put(out, squad);
+    put(out, epi_begin);
+    put(out, n_epi);
+    put(out, n_epj);
+    put(out, n_spj);
+    put(out, id_epj);
+    put(out, id_spj);
+    put(out, acc);
+    put(out, pot);
+    put(out, nngb);
+    std::fclose(out);
+    std::printf("fdps_replay: N=%ld walks=%ld batches=%ld written %s\n", n, n_group,
+                (n_group + n_walk_limit - 1) / n_walk_limit, argv[6]);
+    return 0;
+}

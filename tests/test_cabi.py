@@ -36,7 +36,8 @@ def test_python_binding_covers_header():
 def test_params_default():
     p = sf.Params()
     sf.lib().pth_params_default(C.byref(p))
-    assert (p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk) == (0.0, 0.0, 1.0, 0, 0, 0, 0)
+    assert (p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk, p.max_ipl, p.j_vgpr) == \
+        (0.0, 0.0, 1.0, 0, 0, 0, 1000, 2, 0)
 
 
 def test_create_without_device_fails_cleanly():
@@ -68,31 +69,42 @@ def test_null_context_calls():
     assert L.pth_last_error(None) == b"null context"
 
 
-def _coverage(ne, nj, ns, mode, cj):
-    items, es, ss, rit = sf.plan_debug(ne, nj, ns, mode, cj)
+def _coverage(ne, nj, ns, mode, cj, max_ipl=4):
+    items, es, ss, rit = sf.plan_debug(ne, nj, ns, mode, cj, max_ipl)
     ioff = np.concatenate([[0], np.cumsum(ne)])
-    cov_e, cov_s, parts = {}, {}, []
+    cov_e, cov_s, slots, tiles = {}, {}, [], {}
     for it in items:
-        w, ib, ni, eb0, eb1, s0, s1, part = it
+        w, ib, nipl, eb0, eb1, s0, s1, part = it
         if w < 0:
             continue
-        assert 0 < ni <= 64 and ioff[w] <= ib and ib + ni <= ioff[w + 1]
+        ni, ipl = nipl & 0xFFFF, nipl >> 16
+        assert ipl in (1, 2, 4) and ipl <= max_ipl
+        assert 0 < ni <= 64 * ipl and ioff[w] <= ib and ib + ni <= ioff[w + 1]
         assert es[w] // 8 <= eb0 <= eb1 <= es[w + 1] // 8 and ss[w] <= s0 <= s1 <= ss[w + 1]
+        assert tiles.setdefault((w, ib), (ni, ipl)) == (ni, ipl)
         cov_e.setdefault((w, ib), []).append((eb0, eb1))
         cov_s.setdefault((w, ib), []).append((s0, s1))
         if part >= 0:
-            parts.append(part)
-    assert len(parts) == len(set(parts))
+            slots.extend(range(part, part + ipl))
+    assert len(slots) == len(set(slots))
     for w in range(len(ne)):
-        assert es[w + 1] - es[w] == (nj[w] + 7) // 8 * 8 and ss[w + 1] - ss[w] == ns[w]
-        for t in range((ne[w] + 63) // 64):
+        assert es[w + 1] - es[w] == ((nj[w] + 7) // 8 * 8 + 8 if nj[w] else 0) and ss[w + 1] - ss[w] == ns[w]
+        # the walk's tiles cover its i exactly once
+        mine = sorted((ib, ni) for (ww, ib), (ni, _) in tiles.items() if ww == w)
+        pos = ioff[w]
+        for ib, ni in mine:
+            assert ib == pos
+            pos += ni
+        assert pos == ioff[w + 1]
+        for ib, _ in mine:
             for cov, lo, hi in ((cov_e, es[w] // 8, es[w + 1] // 8), (cov_s, ss[w], ss[w + 1])):
-                r = sorted(cov[(w, ioff[w] + 64 * t)])
+                r = sorted(cov[(w, ib)])
                 assert r[0][0] == lo and r[-1][1] == hi
                 assert all(r[k][1] == r[k + 1][0] for k in range(len(r) - 1))
-    # reduction items: one per split tile, covering its partial slots
-    for out, ni, pf, npart in rit:
-        assert npart > 1 and ni > 0
+    # reduction items: one per split tile
+    for out, ni, pf, npart, ipl, *_ in rit:
+        assert npart > 1 and 0 < ni <= 64 * ipl
+        assert tiles[(np.searchsorted(ioff, out, side="right") - 1, out)] == (ni, ipl)
     return items
 
 
@@ -105,7 +117,7 @@ def test_plan_covers_every_pair_once():
         nj = rng.integers(0, 5000, nw)
         ns = rng.integers(0, 1500, nw)
         cj = int(rng.choice([0, 64, 500, 3000]))
-        items = _coverage(ne, nj, ns, sf.EXEC_FUSED, cj)
+        items = _coverage(ne, nj, ns, sf.EXEC_FUSED, cj, int(rng.choice([1, 2, 4])))
         # launch order: 8 XCD runs interleaved in groups of 4 items
         assert items.shape[0] % 32 == 0 or items.shape[0] == 0
 
@@ -143,19 +155,21 @@ def test_plan_xcd_runs_balanced():
     for k, it in enumerate(items):
         if it[0] >= 0:
             x = (k // 4) % 8
-            cost[x] += 8 * (it[4] - it[3]) + 2 * (it[6] - it[5]) + 8
+            ipl = it[2] >> 16
+            ce, cs = 9.5 + 2.25 / ipl, 18.5 + 5.5 / ipl
+            cost[x] += 64 * ipl * (ce * 8 * (it[4] - it[3]) + cs * (it[6] - it[5]) + 50)
     assert cost.max() / cost.mean() < 1.05
 
 
 def test_plan_plain_has_no_items():
     items, es, ss, rit = sf.plan_debug([10, 70], [5, 9], [1, 0], sf.EXEC_PLAIN, 0)
-    assert items.shape[0] == 0 and rit.shape[0] == 0 and list(es) == [0, 8, 24]
+    assert items.shape[0] == 0 and rit.shape[0] == 0 and list(es) == [0, 16, 40]
 
 
 def test_plan_rejects_bad_input():
     L = sf.lib()
-    assert L.pth_plan_debug(-1, None, None, None, 0, 0, None, 0, None, None, None, 0, None) == sf.PTH_ERR_ARG
-    assert L.pth_plan_debug(2, None, None, None, 0, 0, None, 0, None, None, None, 0, None) == sf.PTH_ERR_ARG
+    assert L.pth_plan_debug(-1, None, None, None, 0, 0, 4, None, 0, None, None, None, 0, None) == sf.PTH_ERR_ARG
+    assert L.pth_plan_debug(2, None, None, None, 0, 0, 4, None, 0, None, None, None, 0, None) == sf.PTH_ERR_ARG
 
 
 def test_fdps_record_layouts():

@@ -58,14 +58,28 @@ def test_resident_c1(c1, exec_mode):
     ctx.close()
 
 
-def test_fused_split_chunks(c1):
-    """Small chunk size forces every walk's list over many items + reduction."""
+@pytest.mark.parametrize("max_ipl", [1, 2, 4])
+def test_fused_split_chunks(c1, max_ipl):
+    """Small chunk size forces every walk's list over many items + reduction,
+    for each number of i-particles per lane."""
     L, a, ref, amb = c1
-    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, chunk_j=256)
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, chunk_j=256, max_ipl=max_ipl)
     r = sf.Resident.from_lists(ctx, L)
-    assert r.stats()["n_items"] > L.n_walk * 4
+    assert r.stats()["n_items"] > L.n_walk * 2
     r.run()
-    _check(r.results(), ref, amb, "fused chunk=256")
+    _check(r.results(), ref, amb, f"fused chunk=256 max_ipl={max_ipl}")
+    r.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("max_ipl", [1, 2, 4])
+def test_ipl_variants_agree(c1, max_ipl):
+    """Unsplit walks: every tiling gives results within FP32 rounding of each other."""
+    L, a, ref, amb = c1
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, chunk_j=1 << 30, max_ipl=max_ipl)
+    r = sf.Resident.from_lists(ctx, L)
+    r.run()
+    _check(r.results(), ref, amb, f"unsplit max_ipl={max_ipl}")
     r.close()
     ctx.close()
 
