@@ -261,6 +261,9 @@ __device__ __forceinline__ void epj_blocks(cfp se, int b_begin, int b_end, const
                     az[q] = __builtin_fmaf(mri3, dz, az[q]);
                     p[q] -= mri;
                 }
+                // keep each j's VGPR copies next to their use (no hoisting of
+                // the whole block's copies, which costs occupancy)
+                if (VC) __builtin_amdgcn_sched_barrier(0);
             }
         }
 #pragma unroll
