@@ -8,19 +8,20 @@
 //                 region (entries that may lie within max(rs_i, rs_j, r_out)
 //                 of some i of the walk) at the front and a "far" region at
 //                 the back; records are SoA blocks of 8 {x[8] y[8] z[8] m[8]
-//                 rs2[8]}, gaps padded with far massless entries
-//   k_stage_spj   gather of the SPJ lists into 12-float records relative to
-//                 the walk centre {x y z m q'xx q'yy q'zz q'xy q'yz q'zx
+//                 rs2[8]}, gaps padded with far massless entries; then the
+//                 walk's SPJ list gathered into 12-float records relative to
+//                 the centre {x y z m q'xx q'yy q'zz q'xy q'yz q'zx
 //                 -eps^2 tr pad}, q' = 3Q - tr I
 //   k_force       one wave per work item (walk, tile of 64/128/256 i with
-//                 1/2/4 i per lane, j-range): j records are read through the
-//                 constant address space (uniform addresses, scalar loads into
-//                 SGPRs) and copied once into VGPRs, then used by every i of
-//                 the lane: on gfx950 FP32 add/mul/fma with only VGPR operands
-//                 issue at twice the rate of the same instruction with an
-//                 SGPR operand.  Far EPJ blocks skip the neighbour count and
-//                 the cutoff (both are known to be no-ops there).  FP32 pair
-//                 arithmetic, FP32 partial sums per chunk, FP64 running sums.
+//                 1/2/4 i per lane, j-range), one launch per i-per-lane
+//                 class: j records are read through the constant address
+//                 space (uniform addresses, scalar loads into SGPRs) and used
+//                 as SGPR operands by every i of the lane (the VC variant
+//                 copies them into VGPRs first; on gfx950 that copy issues at
+//                 single rate and costs more than it saves).  Far EPJ blocks
+//                 skip the neighbour count and the cutoff (both are known to
+//                 be no-ops there).  FP32 pair arithmetic, FP32 partial sums
+//                 per chunk, FP64 running sums.
 //   k_reduce      fixed-order FP64 sum of the partial results of split walks
 //   k_force_plain the plain per-walk form (first kernel of this build): one
 //                 workgroup per walk, 64-i tiles, SGPR operands, whole list
@@ -74,11 +75,26 @@ __device__ __forceinline__ void put_epj(float *se, long long slot, float x, floa
     blk[32 + l] = r2;
 }
 
+// SPJ record {x-c, y-c, z-c, m, q'xx, q'yy, q'zz, q'xy, q'yz, q'zx, -eps^2 tr, 0},
+// q' = 3Q - tr I (the traceless rearrangement of
+// src/phantomquad_for_p3t_x86.hpp:144-163)
+__device__ __forceinline__ void put_spj(float4 *o, const SpjDev &p, const double (&c)[3], double eps2)
+{
+    const double qxx = p.qxx, qyy = p.qyy, qzz = p.qzz;
+    const double tr = qxx + qyy + qzz;
+    o[0] = make_float4((float)(p.x - c[0]), (float)(p.y - c[1]), (float)(p.z - c[2]), p.m);
+    o[1] = make_float4((float)(3.0 * qxx - tr), (float)(3.0 * qyy - tr), (float)(3.0 * qzz - tr),
+                       (float)(3.0 * (double)p.qxy));
+    o[2] = make_float4((float)(3.0 * (double)p.qyz), (float)(3.0 * (double)p.qxz), (float)(-eps2 * tr), 0.f);
+}
+
 __global__ __launch_bounds__(256) void k_stage_walk(
     const EpiDev *__restrict__ epi, const WalkDev *__restrict__ walks,
     const long long *__restrict__ es_off, const int *__restrict__ id_epj,
-    const EpjDev *__restrict__ epj, double r_out, double4 *__restrict__ centre,
-    int *__restrict__ near_blk, float4 *__restrict__ staged_i, float *__restrict__ staged_e)
+    const EpjDev *__restrict__ epj, const long long *__restrict__ ss_off,
+    const int *__restrict__ id_spj, const SpjDev *__restrict__ spj, double r_out, double eps2,
+    double4 *__restrict__ centre, int *__restrict__ near_blk, float4 *__restrict__ staged_i,
+    float *__restrict__ staged_e, float *__restrict__ staged_s)
 {
     const int w = blockIdx.x;
     const WalkDev wk = walks[w];
@@ -175,39 +191,19 @@ __global__ __launch_bounds__(256) void k_stage_walk(
     for (long long s = base + n_near + tid; s < base + cap - n_far; s += 256)
         put_epj(staged_e, s, 1e18f, 1e18f, 1e18f, 0.f, 0.f);
     if (tid == 0) near_blk[w] = (int)((base + ((n_near + 7) & ~7)) >> 3);
-}
-
-__device__ __forceinline__ int walk_of(const long long *__restrict__ off, int n_walk, long long s)
-{
-    // largest w with off[w] <= s (off is non-decreasing, off[0] = 0)
-    int lo = 0, hi = n_walk;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= s) lo = mid; else hi = mid;
+    // -- SPJ list: 12-float records relative to the centre, two per thread
+    //    and pass so that two gathers are in flight
+    float4 *out_s = reinterpret_cast<float4 *>(staged_s) + ss_off[w] * 3;
+    const int *ids = id_spj + wk.s_off;
+    for (int k0 = 0; k0 < wk.n_s; k0 += 512) {
+        const int ka = k0 + tid, kb = ka + 256;
+        const bool va = ka < wk.n_s, vb = kb < wk.n_s;
+        SpjDev pa, pb;
+        if (va) pa = spj[ids[ka]];
+        if (vb) pb = spj[ids[kb]];
+        if (va) put_spj(out_s + 3 * ka, pa, c, eps2);
+        if (vb) put_spj(out_s + 3 * kb, pb, c, eps2);
     }
-    return lo;
-}
-
-__global__ __launch_bounds__(256) void k_stage_spj(
-    int n_walk, const WalkDev *__restrict__ walks, const long long *__restrict__ ss_off,
-    const double4 *__restrict__ centre, const int *__restrict__ id_spj,
-    const SpjDev *__restrict__ spj, long long n_s_rec, double eps2, float *__restrict__ staged_s)
-{
-    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (s >= n_s_rec) return;
-    const int w = walk_of(ss_off, n_walk, s);
-    const WalkDev wk = walks[w];
-    const long long k = s - ss_off[w];
-    if (k >= wk.n_s) return;
-    const SpjDev p = spj[id_spj[(long long)wk.s_off + k]];
-    const double4 c = centre[w];
-    const double qxx = p.qxx, qyy = p.qyy, qzz = p.qzz;
-    const double tr = qxx + qyy + qzz;
-    float4 *o = reinterpret_cast<float4 *>(staged_s + s * 12);
-    o[0] = make_float4((float)(p.x - c.x), (float)(p.y - c.y), (float)(p.z - c.z), p.m);
-    o[1] = make_float4((float)(3.0 * qxx - tr), (float)(3.0 * qyy - tr), (float)(3.0 * qzz - tr),
-                       (float)(3.0 * (double)p.qxy));
-    o[2] = make_float4((float)(3.0 * (double)p.qyz), (float)(3.0 * (double)p.qxz), (float)(-eps2 * tr), 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -534,11 +530,8 @@ hipError_t launch_stage(const BatchDev &b, hipStream_t st)
 {
     if (b.n_walk <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_stage_walk, dim3(b.n_walk), dim3(256), 0, st, b.epi, b.walks, b.es_off, b.id_epj,
-                       b.epj, std::sqrt((double)b.rout2), b.centre, b.near_blk, b.staged_i, b.staged_e);
-    const long long nbs = (b.n_s_rec + 255) / 256;
-    if (nbs > 0)
-        hipLaunchKernelGGL(k_stage_spj, dim3((unsigned)nbs), dim3(256), 0, st, b.n_walk, b.walks, b.ss_off,
-                           b.centre, b.id_spj, b.spj, b.n_s_rec, (double)b.eps2, b.staged_s);
+                       b.epj, b.ss_off, b.id_spj, b.spj, std::sqrt((double)b.rout2), (double)b.eps2, b.centre,
+                       b.near_blk, b.staged_i, b.staged_e, b.staged_s);
     return hipGetLastError();
 }
 
