@@ -73,10 +73,28 @@ typedef struct pth_params {
     int max_ipl;   /* fused form: most i-particles per lane, 1, 2 or 4 (0 = 4) */
     int j_vgpr;    /* fused form: 1 copies each j record into VGPRs before use,
                       0 uses it as SGPR operands */
+    int changeover; /* 0: linear cutoff only (the reference GPU kernels);
+                       1: changeover mode (fused form only), below */
+    double ngb_factor; /* changeover mode: neighbour radius factor on r_search
+                          (0 = 0.99, SAFTY_FACTOR_FOR_SEARCH of src/ptcl.hpp:8) */
 } pth_params;
 
+/* Changeover mode folds PeTar's changeover correction of the soft force
+ * (src/hard.hpp:1404-1476 per neighbour, 1655-1691 per particle, with the
+ * changeover functions of src/changeover.hpp:294-375) into the force kernel:
+ * for every EPJ pair that is a neighbour pair, r^2 < (ngb_factor *
+ * max(rs_i, rs_j))^2 (FDPS's symmetric neighbour search on getRSearch(),
+ * src/soft_ptcl.hpp:97-101,349-353), j != i, the linear-cutoff terms are
+ * replaced by G m (1 - W_acc(r)) dx / r^3 and -G m (1 - W_pot(r)) / r (pot)
+ * and -G m / r (pot_tot), W taken from the pair member with the larger r_out;
+ * the self term -G m_i / r_out is dropped.  Every j is treated as a single
+ * (group_data is not read).  Requires EPJ ids and r_in/r_out (epj view),
+ * EPI ids (epi view), each i's own EPJ in its walk's list, and at most 1024
+ * i per walk.  pot receives pot_soft; pot_tot goes to force view member
+ * off_pot_tot when that is >= 0. */
+
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
- * max_walk 1000, max_ipl 2, j_vgpr 0. */
+ * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99. */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */
@@ -95,19 +113,22 @@ int pth_device(const pth_ctx *ctx);
 /* ---- record views (byte offsets into the caller's structs) ---------------- */
 typedef struct pth_epj_view { /* EPJSoft */
     size_t stride, off_pos, off_mass, off_rsearch;
+    size_t off_id, off_rin, off_rout; /* changeover mode only */
 } pth_epj_view;
 typedef struct pth_spj_view { /* SPJQuadrupoleInAndOut */
     size_t stride, off_pos, off_mass, off_qxx, off_qyy, off_qzz, off_qxy, off_qxz, off_qyz;
 } pth_spj_view;
 typedef struct pth_epi_view { /* EPISoft */
     size_t stride, off_pos, off_rsearch;
+    size_t off_id; /* changeover mode only */
 } pth_epi_view;
 typedef struct pth_force_view { /* ForceSoft */
     size_t stride, off_acc, off_pot, off_nngb;
-    int nngb_bytes; /* 4 or 8 */
+    int nngb_bytes;       /* 4 or 8 */
+    long long off_pot_tot; /* changeover mode: pot_tot member, or -1 */
 } pth_force_view;
-/* pos members are three consecutive doubles (x, y, z), mass/rsearch/q* are
- * doubles. */
+/* pos members are three consecutive doubles (x, y, z), mass/rsearch/r_in/
+ * r_out/q* are doubles, ids are 64-bit integers. */
 
 /* ---- FDPS multi-walk hooks ------------------------------------------------- */
 int pth_send_j(pth_ctx *ctx, const void *epj, int n_epj_tot, const pth_epj_view *ev,
@@ -137,8 +158,13 @@ int pth_resident_create(pth_ctx *ctx, pth_resident **res, int n_walk, const int 
 int pth_resident_run(pth_resident *res);
 /* Waits for the last evaluation. */
 int pth_resident_sync(pth_resident *res);
-/* Copies results (waits first): acc [ni][3], pot [ni], nnb [ni]. */
-int pth_resident_results(pth_resident *res, double *acc, double *pot, long long *nnb);
+/* Changeover mode: the ids of the EPI (concatenated per walk) and the ids
+ * and changeover radii of the EPJ; must be given before the first run. */
+int pth_resident_changeover(pth_resident *res, const long long *epi_id, const long long *epj_id,
+                            const double *epj_rin, const double *epj_rout);
+/* Copies results (waits first): acc [ni][3], pot [ni], nnb [ni]; pot_tot
+ * [ni] in changeover mode (else left alone).  Any pointer may be NULL. */
+int pth_resident_results(pth_resident *res, double *acc, double *pot, long long *nnb, double *pot_tot);
 /* Per-kernel GPU times of the last evaluation (ms), from HIP events on the
  * context's stream: stage (centres + gather), force, reduce. */
 int pth_resident_timing(pth_resident *res, float *ms_stage, float *ms_force, float *ms_reduce);

@@ -18,6 +18,12 @@
 // dispatch, retrieve assigns acc/pot/n_ngb.  The reference asserts on bad
 // input and ignores device errors; here every failure stops the program with
 // the library's error text (PeTar's fail-stop behaviour).
+//
+// With PETAR_HIP_CHANGEOVER defined, the context runs in changeover mode
+// (see petar_hip.h): the kernel also applies the changeover weight to the
+// neighbour pairs, which replaces PeTar's CPU correction pass
+// (treeForceCorrectChangeover, src/petar.hpp:965-989); ForceSoft must then
+// have a PS::F64 pot_tot member, and pot receives the soft potential.
 #pragma once
 #include <cstddef>
 #include <cstdio>
@@ -52,6 +58,9 @@ inline pth_ctx *context(PS::S32 rank, PS::F64 eps2, PS::F64 rcut2, PS::F64 G)
     p.eps2 = eps2;
     p.rcut2 = rcut2;
     p.G = G;
+#ifdef PETAR_HIP_CHANGEOVER
+    p.changeover = 1;
+#endif
     if (!c) {
         if (pth_create(&c, rank, &p) != PTH_OK) fail("pth_create", nullptr);
     } else if (pth_set_params(c, &p) != PTH_OK) {
@@ -63,7 +72,8 @@ inline pth_ctx *context(PS::S32 rank, PS::F64 eps2, PS::F64 rcut2, PS::F64 G)
 inline pth_epj_view epj_view()
 {
     return pth_epj_view{sizeof(EPJSoft), offsetof(EPJSoft, pos), offsetof(EPJSoft, mass),
-                        offsetof(EPJSoft, r_search)};
+                        offsetof(EPJSoft, r_search), offsetof(EPJSoft, id), offsetof(EPJSoft, r_in),
+                        offsetof(EPJSoft, r_out)};
 }
 
 inline pth_spj_view spj_view()
@@ -82,13 +92,20 @@ inline pth_spj_view spj_view()
 
 inline pth_epi_view epi_view()
 {
-    return pth_epi_view{sizeof(EPISoft), offsetof(EPISoft, pos), offsetof(EPISoft, r_search)};
+    return pth_epi_view{sizeof(EPISoft), offsetof(EPISoft, pos), offsetof(EPISoft, r_search),
+                        offsetof(EPISoft, id)};
 }
 
 inline pth_force_view force_view()
 {
+#ifdef PETAR_HIP_CHANGEOVER
+    // ForceSoft then carries a pot_tot member (pot receives pot_soft)
+    const long long pot_tot = (long long)offsetof(ForceSoft, pot_tot);
+#else
+    const long long pot_tot = -1;
+#endif
     return pth_force_view{sizeof(ForceSoft), offsetof(ForceSoft, acc), offsetof(ForceSoft, pot),
-                          offsetof(ForceSoft, n_ngb), (int)sizeof(((ForceSoft *)nullptr)->n_ngb)};
+                          offsetof(ForceSoft, n_ngb), (int)sizeof(((ForceSoft *)nullptr)->n_ngb), pot_tot};
 }
 
 }  // namespace petar_hip

@@ -24,6 +24,13 @@
  *                            W_acc1 = dW_acc/dt by finite differences.
  *   orc_changeover_correct_one  pinned by the pairwise identity
  *                            linear-cutoff + correction = (1-W) Newton.
+ *   orc_soft_force_walks_co  two-body closed forms inside r_in, between r_in
+ *                            and r_out and beyond the neighbour radius, the
+ *                            larger-r_out rule, the self term, and pot_soft
+ *                            as the potential of acc (finite differences).
+ *                            Which pairs FDPS's neighbour search returns at
+ *                            the search radius is parity unpinned (FDPS is
+ *                            not available).
  */
 #include "oracle.h"
 #include <math.h>
@@ -410,4 +417,91 @@ void orc_changeover_correct_one(const double *pos_i, double mass_i, double rin_i
         *pot_soft -= gmor * kpot - gmor_max;
         *pot_tot -= gmor - gmor_max;
     }
+}
+
+/* Changeover mode over walks: the linear-cutoff result of
+   orc_soft_force_walks (nnb_mode 0), then for every i the FP64 correction
+   of src/hard.hpp:1655-1691 (self-potential term, then
+   src/hard.hpp:1404-1476 per neighbour, all j singles) over the neighbour set
+   taken from the walk's own EPJ list: j with |x_i - x_j|^2 <
+   (f max(rs_i, rs_j))^2 (FDPS's symmetric search on getRSearch() = f r_search,
+   src/soft_ptcl.hpp:97-101,349-353), the correction itself skipping id_j ==
+   id_i.  The i's mass and changeover radii come from its own EPJ entry (same
+   id) in the list.  n_amb[i] counts pairs with |r^2 - (f rs)^2| <= amb_rel
+   (f rs)^2 (membership undecided at FP32 rounding).  Returns -1 if some i has
+   no own entry, else 0.  pot_soft and pot_tot are both assigned. */
+int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
+                            const double *epi_pos, const double *epi_rs, const long long *epi_id,
+                            const int *id_epj, const int *id_spj,
+                            const double *epj_pos, const double *epj_mass, const double *epj_rs,
+                            const long long *epj_id, const double *epj_rin, const double *epj_rout,
+                            const double *spj_pos, const double *spj_mass, const double *spj_quad,
+                            double eps2, double rout2, double G, double ngb_factor, double amb_rel,
+                            double *acc, double *pot_soft, double *pot_tot, long long *nnb,
+                            long long *n_amb)
+{
+    int rc = orc_soft_force_walks(n_walk, n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos,
+                                  epj_mass, epj_rs, spj_pos, spj_mass, spj_quad, eps2, rout2, G, 0, acc,
+                                  pot_soft, nnb);
+    if (rc) return rc;
+    long long *io = malloc(sizeof(long long) * (n_walk + 1));
+    long long *eo = malloc(sizeof(long long) * (n_walk + 1));
+    prefix(n_walk, n_epi, io);
+    prefix(n_walk, n_epj, eo);
+    const double r_out = sqrt(rout2);
+    int missing = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : missing)
+    for (int w = 0; w < n_walk; w++) {
+        const int ne = n_epj[w];
+        double *pj = malloc(sizeof(double) * 3 * (ne > 0 ? ne : 1));
+        double *mj = malloc(sizeof(double) * (ne > 0 ? ne : 1));
+        double *ri = malloc(sizeof(double) * (ne > 0 ? ne : 1));
+        double *ro = malloc(sizeof(double) * (ne > 0 ? ne : 1));
+        long long *idj = malloc(sizeof(long long) * (ne > 0 ? ne : 1));
+        const int *lst = id_epj + eo[w];
+        for (long long i = io[w]; i < io[w + 1]; i++) {
+            const double *xi = epi_pos + 3 * i;
+            pot_tot[i] = pot_soft[i];
+            n_amb[i] = 0;
+            int self = -1;
+            for (int k = 0; k < ne; k++)
+                if (epj_id[lst[k]] == epi_id[i]) {
+                    self = lst[k];
+                    break;
+                }
+            if (self < 0) {
+                missing = 1;
+                continue;
+            }
+            int nn = 0;
+            for (int k = 0; k < ne; k++) {
+                const int j = lst[k];
+                const double dx = xi[0] - epj_pos[3 * j], dy = xi[1] - epj_pos[3 * j + 1],
+                             dz = xi[2] - epj_pos[3 * j + 2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double rs = ngb_factor * dmax(epi_rs[i], epj_rs[j]);
+                const double rs2 = rs * rs;
+                if (fabs(r2 - rs2) <= amb_rel * rs2 && epj_id[j] != epi_id[i]) n_amb[i]++;
+                if (!(r2 < rs2)) continue;
+                pj[3 * nn] = epj_pos[3 * j];
+                pj[3 * nn + 1] = epj_pos[3 * j + 1];
+                pj[3 * nn + 2] = epj_pos[3 * j + 2];
+                mj[nn] = epj_mass[j];
+                ri[nn] = epj_rin[j];
+                ro[nn] = epj_rout[j];
+                idj[nn] = epj_id[j];
+                nn++;
+            }
+            orc_changeover_correct_one(xi, epj_mass[self], epj_rin[self], epj_rout[self], epi_id[i], nn, pj, mj,
+                                       ri, ro, idj, eps2, r_out, G, acc + 3 * i, pot_soft + i, pot_tot + i);
+        }
+        free(pj);
+        free(mj);
+        free(ri);
+        free(ro);
+        free(idj);
+    }
+    free(io);
+    free(eo);
+    return missing ? -1 : 0;
 }

@@ -95,6 +95,18 @@ void orc_changeover_correct_one(const double *pos_i, double mass_i, double rin_i
                                 const long long *id_j, double eps2, double r_out_glb, double G,
                                 double *acc, double *pot_soft, double *pot_tot);
 
+/* Changeover mode (the composition src/hard.hpp:1404-1476,1655-1691 applies
+ * on top of the linear-cutoff force) over walks; see oracle.c. */
+int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
+                            const double *epi_pos, const double *epi_rs, const long long *epi_id,
+                            const int *id_epj, const int *id_spj,
+                            const double *epj_pos, const double *epj_mass, const double *epj_rs,
+                            const long long *epj_id, const double *epj_rin, const double *epj_rout,
+                            const double *spj_pos, const double *spj_mass, const double *spj_quad,
+                            double eps2, double rout2, double G, double ngb_factor, double amb_rel,
+                            double *acc, double *pot_soft, double *pot_tot, long long *nnb,
+                            long long *n_amb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,9 @@ def orc():
         L.orc_direct_newton.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_void_p, C.c_double, C.c_double,
                                         _dp, _dp]
         L.orc_moments.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        _llp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        L.orc_soft_force_walks_co.argtypes = wl + [_dp, _dp, _llp, _ip, _ip, _dp, _dp, _dp, _llp, _dp, _dp,
+                                                   _dp, _dp, _dp] + [C.c_double] * 5 + [_dp, _dp, _dp, _lp, _lp]
         L.orc_changeover_setr.argtypes = [C.c_double] * 3 + [C.POINTER(C.c_double)] * 2
         for f in ("orc_changeover_wpot", "orc_changeover_wacc"):
             getattr(L, f).argtypes = [C.c_double] * 3
@@ -168,6 +171,32 @@ def ref_soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, e
     if rc != 0:
         raise RuntimeError(f"reference kernel failed ({rc})")
     return acc, pot, nnb
+
+
+def soft_force_walks_co(n_epi, n_epj, n_spj, epi_pos, epi_rs, epi_id, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
+                        epj_id, epj_rin, epj_rout, spj_pos, spj_mass, spj_quad, eps2, rout2, G=1.0, ngb_factor=0.99,
+                        amb_rel=1e-5):
+    """FP64 oracle of the changeover mode: acc, pot_soft, pot_tot, nnb and the
+    per-i count of neighbour pairs at the search radius within amb_rel."""
+    n_epi, n_epj, n_spj = _i32(n_epi), _i32(n_epj), _i32(n_spj)
+    ni = int(n_epi.sum())
+    acc, ps, pt = np.zeros((ni, 3)), np.zeros(ni), np.zeros(ni)
+    nnb, amb = np.zeros(ni, np.int64), np.zeros(ni, np.int64)
+    if ni == 0:
+        return acc, ps, pt, nnb, amb
+
+    def arr(a, dt=np.float64):
+        # flat contiguous copy; a one-element dummy stands in for an empty array
+        a = np.ascontiguousarray(a, dtype=dt).ravel()
+        return a if a.size else np.zeros(1, dt)
+    rc = orc().orc_soft_force_walks_co(
+        n_epi.size, n_epi, n_epj, n_spj, arr(epi_pos), arr(epi_rs), arr(epi_id, np.int64),
+        arr(id_epj, np.int32), arr(id_spj, np.int32), arr(epj_pos), arr(epj_mass), arr(epj_rs),
+        arr(epj_id, np.int64), arr(epj_rin), arr(epj_rout), arr(spj_pos), arr(spj_mass), arr(spj_quad),
+        eps2, rout2, G, ngb_factor, amb_rel, acc.ravel(), ps, pt, nnb, amb)
+    if rc != 0:
+        raise RuntimeError("orc_soft_force_walks_co: an i-particle has no own EPJ entry in its walk")
+    return acc, ps, pt, nnb, amb
 
 
 def lists_args(lists):

@@ -99,11 +99,12 @@ inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Layout of one upload arena (pinned host mirror and device copy).
 struct Arena {
-    size_t epi, walks, es, ss, ide, ids, items, ritems, total;
-    void layout(const Plan &P, long long n_ide, long long n_ids)
+    size_t epi, epi_id, walks, es, ss, ide, ids, items, ritems, total;
+    void layout(const Plan &P, long long n_ide, long long n_ids, bool co)
     {
         size_t o = 0;
         epi = o; o = align256(o + sizeof(EpiDev) * (size_t)P.ni_tot);
+        epi_id = o; o = align256(o + (co ? sizeof(long long) * (size_t)P.ni_tot : 0));
         walks = o; o = align256(o + sizeof(WalkDev) * P.walks.size());
         es = o; o = align256(o + sizeof(long long) * P.es_off.size());
         ss = o; o = align256(o + sizeof(long long) * P.ss_off.size());
@@ -118,9 +119,18 @@ struct Arena {
 // Device work buffers of one batch (staging, partials, outputs).
 struct Work {
     DevBuf centre, near_blk, staged_i, staged_e, staged_s, part_accp, part_nnb;
-    hipError_t ensure(const Plan &P)
+    DevBuf staged_ic, staged_il, staged_ec, part_pott, walk_err;  // changeover mode
+    hipError_t ensure(const Plan &P, bool co)
     {
         hipError_t e;
+        if (co) {
+            if ((e = staged_ic.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
+            if ((e = staged_il.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
+            if ((e = staged_ec.ensure(sizeof(float) * (PTH_CO_EXTRA / 8) * std::max<long long>(P.n_e_slots, 8))))
+                return e;
+            if ((e = part_pott.ensure(sizeof(double) * 64 * std::max(P.n_part, 1)))) return e;
+            if ((e = walk_err.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)))) return e;
+        }
         if ((e = centre.ensure(sizeof(double4) * std::max<size_t>(P.walks.size(), 1)))) return e;
         if ((e = near_blk.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)))) return e;
         if ((e = staged_i.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
@@ -134,12 +144,13 @@ struct Work {
     {
         centre.release(); near_blk.release(); staged_i.release(); staged_e.release(); staged_s.release();
         part_accp.release(); part_nnb.release();
+        staged_ic.release(); staged_il.release(); staged_ec.release(); part_pott.release(); walk_err.release();
     }
 };
 
 void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf &arena,
                 const Arena &A, Work &W, const EpjDev *epj, const SpjDev *spj, double4 *out_accp,
-                int *out_nnb)
+                int *out_nnb, const EpjCoDev *epj_co, double *out_pott)
 {
     b.n_walk = (int)P.walks.size();
     b.ni_tot = P.ni_tot;
@@ -157,6 +168,11 @@ void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf 
     b.eps2 = (float)par.eps2;
     b.rout2 = (float)par.rcut2;
     b.G = par.G;
+    b.co = par.changeover ? 1 : 0;
+    {
+        const double f = par.ngb_factor > 0.0 ? par.ngb_factor : 0.99;
+        b.ngb2 = (float)(f * f);
+    }
     char *base = arena.as<char>();
     b.epi = reinterpret_cast<const EpiDev *>(base + A.epi);
     b.walks = reinterpret_cast<const WalkDev *>(base + A.walks);
@@ -177,6 +193,25 @@ void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf 
     b.part_nnb = W.part_nnb.as<int>();
     b.out_accp = out_accp;
     b.out_nnb = out_nnb;
+    if (b.co) {
+        b.epi_id = reinterpret_cast<const long long *>(base + A.epi_id);
+        b.epj_co = epj_co;
+        b.staged_ic = W.staged_ic.as<float4>();
+        b.staged_il = W.staged_il.as<float4>();
+        b.staged_ec = W.staged_ec.as<float>();
+        b.walk_err = W.walk_err.as<int>();
+        b.part_pott = W.part_pott.as<double>();
+        b.out_pott = out_pott;
+    }
+}
+
+// Checks of the changeover-mode preconditions on a plan.
+const char *co_plan_error(const Plan &P)
+{
+    for (const WalkDev &w : P.walks)
+        if (w.n_i > PTH_CO_MAX_NI) return "changeover mode: a walk has more than 1024 i-particles";
+    if (P.n_e_slots >= (1ll << 31)) return "changeover mode: staged EPJ slots exceed the int32 range";
+    return nullptr;
 }
 
 void copy_plan_to_arena(char *h, const Plan &P, const Arena &A)
@@ -191,8 +226,9 @@ void copy_plan_to_arena(char *h, const Plan &P, const Arena &A)
 }  // namespace
 
 struct TagState {
-    DevBuf out_accp, out_nnb;
-    PinBuf h_accp, h_nnb;
+    DevBuf out_accp, out_nnb, out_pott;
+    PinBuf h_accp, h_nnb, h_pott, h_err;
+    bool co = false;
     std::vector<int> n_i;
     long long ni_tot = 0;
     bool pending = false;
@@ -205,10 +241,11 @@ struct pth_ctx {
     pth_params par{};
     std::string err;
     // resident j arrays of the current force call
-    DevBuf d_epj, d_spj;
-    PinBuf h_epj, h_spj;
+    DevBuf d_epj, d_spj, d_epj_co;
+    PinBuf h_epj, h_spj, h_epj_co;
     int n_epj = 0, n_spj = 0;
     bool j_sent = false;
+    bool co_sent = false;  // the last send carried the changeover data
     hipEvent_t ev_send = nullptr;
     // batch upload
     PinBuf h_arena;
@@ -234,6 +271,9 @@ struct pth_resident {
     Plan plan;
     Arena A{};
     DevBuf arena, epj, spj, out_accp, out_nnb;
+    DevBuf epj_co, out_pott;  // changeover mode
+    bool co = false, co_ready = false;
+    int n_epj_tot = 0;
     Work work;
     BatchDev b;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -261,6 +301,8 @@ void pth_params_default(pth_params *p)
     p->max_walk = 1000;
     p->max_ipl = 2;  // measured best on the precision workload (1, 2, 4: 11.2, 10.6, 11.8 ms)
     p->j_vgpr = 0;   // SGPR operands beat VGPR copies at every max_ipl
+    p->changeover = 0;
+    p->ngb_factor = 0.99;
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -290,6 +332,18 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
     }
     if (!(par.rcut2 >= 0.0) || !(par.eps2 >= 0.0)) {
         g_create_err = "pth_create: eps2 and rcut2 must be >= 0";
+        return PTH_ERR_ARG;
+    }
+    if (par.changeover != 0 && par.changeover != 1) {
+        g_create_err = "pth_create: changeover must be 0 or 1";
+        return PTH_ERR_ARG;
+    }
+    if (par.changeover && par.exec_mode != PTH_EXEC_FUSED) {
+        g_create_err = "pth_create: changeover mode needs the fused form";
+        return PTH_ERR_ARG;
+    }
+    if (!(par.ngb_factor >= 0.0)) {
+        g_create_err = "pth_create: ngb_factor must be >= 0";
         return PTH_ERR_ARG;
     }
     if (par.max_walk <= 0) par.max_walk = 1000;
@@ -329,12 +383,19 @@ int pth_set_params(pth_ctx *c, const pth_params *p)
     if (p->nnb_mode != 0 && p->nnb_mode != 1) return c->fail(PTH_ERR_ARG, "pth_set_params: nnb_mode must be 0 or 1");
     if (!(p->rcut2 >= 0.0) || !(p->eps2 >= 0.0))
         return c->fail(PTH_ERR_ARG, "pth_set_params: eps2 and rcut2 must be >= 0");
+    if (p->changeover != 0 && p->changeover != 1)
+        return c->fail(PTH_ERR_ARG, "pth_set_params: changeover must be 0 or 1");
+    if (p->changeover && c->par.exec_mode != PTH_EXEC_FUSED)
+        return c->fail(PTH_ERR_ARG, "pth_set_params: changeover mode needs the fused form");
+    if (!(p->ngb_factor >= 0.0)) return c->fail(PTH_ERR_ARG, "pth_set_params: ngb_factor must be >= 0");
     for (int t = 0; t < MAX_TAG; t++)
         if (c->tag[t].pending) return c->fail(PTH_ERR_STATE, "pth_set_params: a batch awaits retrieval");
     c->par.eps2 = p->eps2;
     c->par.rcut2 = p->rcut2;
     c->par.G = p->G;
     c->par.nnb_mode = p->nnb_mode;
+    c->par.changeover = p->changeover;
+    c->par.ngb_factor = p->ngb_factor;
     return PTH_OK;
 }
 
@@ -345,16 +406,21 @@ void pth_destroy(pth_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->d_epj.release();
     c->d_spj.release();
+    c->d_epj_co.release();
     c->h_epj.release();
     c->h_spj.release();
+    c->h_epj_co.release();
     c->h_arena.release();
     c->d_arena.release();
     c->work.release();
     for (int t = 0; t < MAX_TAG; t++) {
         c->tag[t].out_accp.release();
         c->tag[t].out_nnb.release();
+        c->tag[t].out_pott.release();
         c->tag[t].h_accp.release();
         c->tag[t].h_nnb.release();
+        c->tag[t].h_pott.release();
+        c->tag[t].h_err.release();
         if (c->tag[t].done) (void)hipEventDestroy(c->tag[t].done);
     }
     if (c->ev_send) (void)hipEventDestroy(c->ev_send);
@@ -415,6 +481,25 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
         d.pad[0] = d.pad[1] = d.pad[2] = 0.f;
         hs[i] = d;
     }
+    if (c->par.changeover) {
+        const size_t bc = sizeof(EpjCoDev) * (size_t)std::max(n_epj_tot, 1);
+        HIPCHK(c, c->h_epj_co.ensure(bc), "pinned EPJ allocation");
+        HIPCHK(c, c->d_epj_co.ensure(bc), "device EPJ allocation");
+        EpjCoDev *hc = c->h_epj_co.as<EpjCoDev>();
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n_epj_tot; i++) {
+            const char *r = eb + (size_t)i * ev->stride;
+            EpjCoDev d;
+            d.rin = (float)*reinterpret_cast<const double *>(r + ev->off_rin);
+            d.rout = (float)*reinterpret_cast<const double *>(r + ev->off_rout);
+            d.id = *reinterpret_cast<const long long *>(r + ev->off_id);
+            hc[i] = d;
+        }
+        if (n_epj_tot > 0)
+            HIPCHK(c, hipMemcpyAsync(c->d_epj_co.p, hc, sizeof(EpjCoDev) * n_epj_tot, hipMemcpyHostToDevice,
+                                     c->stream), "EPJ upload");
+    }
+    c->co_sent = c->par.changeover != 0;
     if (n_epj_tot > 0)
         HIPCHK(c, hipMemcpyAsync(c->d_epj.p, he, sizeof(EpjDev) * n_epj_tot, hipMemcpyHostToDevice, c->stream),
                "EPJ upload");
@@ -451,12 +536,19 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         n_ide += n_epj[w];
         n_ids += n_spj[w];
     }
+    const bool co = c->par.changeover != 0;
+    if (co && !c->co_sent)
+        return c->fail(PTH_ERR_STATE, "pth_dispatch: changeover mode, but the j arrays were sent without it");
     Plan P;
     if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, P))
         return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: batch exceeds the int32 index range");
+    if (co) {
+        const char *m = co_plan_error(P);
+        if (m) return c->fail(PTH_ERR_CAPACITY, std::string("pth_dispatch: ") + m);
+    }
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     Arena A;
-    A.layout(P, n_ide, n_ids);
+    A.layout(P, n_ide, n_ids, co);
     // the pinned arena may still feed the previous batch's copy
     HIPCHK(c, hipEventSynchronize(c->ev_upload), "hipEventSynchronize");
     HIPCHK(c, c->h_arena.ensure(A.total), "pinned arena allocation");
@@ -471,14 +563,23 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         need |= sizeof(float) * 5 * (size_t)P.n_e_slots > c->work.staged_e.cap;
         need |= sizeof(float) * 12 * (size_t)P.n_s_rec > c->work.staged_s.cap;
         need |= sizeof(double4) * 64 * (size_t)P.n_part > c->work.part_accp.cap;
+        if (co) {
+            need |= sizeof(float4) * (size_t)P.ni_tot > c->work.staged_ic.cap;
+            need |= sizeof(float4) * (size_t)P.ni_tot > c->work.staged_il.cap;
+            need |= sizeof(float) * (PTH_CO_EXTRA / 8) * (size_t)P.n_e_slots > c->work.staged_ec.cap;
+            need |= sizeof(double) * 64 * (size_t)P.n_part > c->work.part_pott.cap;
+            need |= sizeof(int) * P.walks.size() > c->work.walk_err.cap;
+        }
         if (need) HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        HIPCHK(c, c->work.ensure(P), "work buffer allocation");
+        HIPCHK(c, c->work.ensure(P, co), "work buffer allocation");
     }
     HIPCHK(c, T.out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "output allocation");
     HIPCHK(c, T.out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "output allocation");
+    if (co) HIPCHK(c, T.out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "output allocation");
 
     char *h = c->h_arena.as<char>();
     EpiDev *he = reinterpret_cast<EpiDev *>(h + A.epi);
+    long long *hid = reinterpret_cast<long long *>(h + A.epi_id);
     int *hide = reinterpret_cast<int *>(h + A.ide);
     int *hids = reinterpret_cast<int *>(h + A.ids);
     const int nj_lim = c->n_epj, ns_lim = c->n_spj;
@@ -497,6 +598,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
             d.rs = (float)*reinterpret_cast<const double *>(r + iv->off_rsearch);
             d.pad = 0;
             he[W.i_off + i] = d;
+            if (co) hid[W.i_off + i] = *reinterpret_cast<const long long *>(r + iv->off_id);
         }
         for (int j = 0; j < W.n_e; j++) {
             const int id = id_epj[w][j];
@@ -523,18 +625,30 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     if (c->par.exec_mode == PTH_EXEC_PLAIN) HIPCHK(c, hipStreamSynchronize(c->stream), "upload sync");
     BatchDev b;
     fill_batch(b, P, c->par, c->d_arena, A, c->work, c->d_epj.as<EpjDev>(), c->d_spj.as<SpjDev>(),
-               T.out_accp.as<double4>(), T.out_nnb.as<int>());
+               T.out_accp.as<double4>(), T.out_nnb.as<int>(), c->d_epj_co.as<EpjCoDev>(),
+               T.out_pott.as<double>());
     HIPCHK(c, launch_stage(b, c->stream), "stage kernels");
     HIPCHK(c, launch_force(b, c->stream), "force kernel");
     HIPCHK(c, launch_reduce(b, c->stream), "reduce kernel");
     HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");
+    if (co) {
+        HIPCHK(c, T.h_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "pinned output");
+        HIPCHK(c, T.h_err.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)), "pinned output");
+    }
     if (P.ni_tot > 0) {
         HIPCHK(c, hipMemcpyAsync(T.h_accp.p, T.out_accp.p, sizeof(double4) * P.ni_tot, hipMemcpyDeviceToHost,
                                  c->stream), "result download");
         HIPCHK(c, hipMemcpyAsync(T.h_nnb.p, T.out_nnb.p, sizeof(int) * P.ni_tot, hipMemcpyDeviceToHost,
                                  c->stream), "result download");
+        if (co)
+            HIPCHK(c, hipMemcpyAsync(T.h_pott.p, T.out_pott.p, sizeof(double) * P.ni_tot, hipMemcpyDeviceToHost,
+                                     c->stream), "result download");
     }
+    if (co && n_walk > 0)
+        HIPCHK(c, hipMemcpyAsync(T.h_err.p, c->work.walk_err.p, sizeof(int) * n_walk, hipMemcpyDeviceToHost,
+                                 c->stream), "result download");
+    T.co = co;
     HIPCHK(c, hipEventRecord(T.done, c->stream), "hipEventRecord");
     T.n_i.assign(n_epi, n_epi + n_walk);
     T.ni_tot = P.ni_tot;
@@ -565,6 +679,13 @@ int pth_retrieve(pth_ctx *c, int tag, int n_walk, const int *ni, void *const *fo
     HIPCHK(c, hipGetLastError(), "kernel execution");
     const double4 *a = T.h_accp.as<double4>();
     const int *nb = T.h_nnb.as<int>();
+    const double *pt = T.h_pott.as<double>();
+    if (T.co)
+        for (int w = 0; w < n_walk; w++)
+            if (T.h_err.as<int>()[w] != 0)
+                return c->fail(PTH_ERR_ARG, "pth_retrieve: changeover mode: an i-particle of walk " +
+                                                std::to_string(w) + " has no EPJ with its id in the walk's list");
+    const bool write_pt = T.co && fv && fv->off_pot_tot >= 0;
     std::vector<long long> off(n_walk + 1, 0);
     for (int w = 0; w < n_walk; w++) off[w + 1] = off[w] + ni[w];
 #pragma omp parallel for schedule(dynamic)
@@ -578,6 +699,7 @@ int pth_retrieve(pth_ctx *c, int tag, int n_walk, const int *ni, void *const *fo
             acc[1] = v.y;
             acc[2] = v.z;
             *reinterpret_cast<double *>(r + fv->off_pot) = v.w;
+            if (write_pt) *reinterpret_cast<double *>(r + fv->off_pot_tot) = pt[off[w] + i];
             if (fv->nngb_bytes == 8) *reinterpret_cast<long long *>(r + fv->off_nngb) = nb[off[w] + i];
             else *reinterpret_cast<int *>(r + fv->off_nngb) = nb[off[w] + i];
         }
@@ -618,7 +740,15 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
         return c->fail(PTH_ERR_CAPACITY, "pth_resident_create: sizes exceed the int32 index range");
     }
     const Plan &P = r->plan;
-    r->A.layout(P, n_ide, n_ids);
+    r->co = c->par.changeover != 0;
+    if (r->co) {
+        const char *m = co_plan_error(P);
+        if (m) {
+            delete r;
+            return c->fail(PTH_ERR_CAPACITY, std::string("pth_resident_create: ") + m);
+        }
+    }
+    r->A.layout(P, n_ide, n_ids, r->co);
     auto bail = [&](hipError_t e, const char *what) {
         int rc = c->hip_fail(e, what);
         pth_resident_free(r);
@@ -631,7 +761,11 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
     if ((e = r->spj.ensure(sizeof(SpjDev) * std::max(n_spj_tot, 1)))) return bail(e, "resident SPJ");
     if ((e = r->out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
     if ((e = r->out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
-    if ((e = r->work.ensure(P))) return bail(e, "resident work buffers");
+    if ((e = r->work.ensure(P, r->co))) return bail(e, "resident work buffers");
+    if (r->co) {
+        if ((e = r->epj_co.ensure(sizeof(EpjCoDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
+        if ((e = r->out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
+    }
     for (int k = 0; k < 4; k++)
         if ((e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
     // host mirrors (pageable is fine: uploaded once)
@@ -676,9 +810,38 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
     if (n_spj_tot && (e = hipMemcpy(r->spj.p, sj.data(), sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice)))
         return bail(e, "resident upload");
     fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
-               r->out_accp.as<double4>(), r->out_nnb.as<int>());
+               r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(),
+               r->out_pott.as<double>());
+    r->n_epj_tot = n_epj_tot;
     r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 48 * P.n_s_rec);
     *out = r;
+    return PTH_OK;
+}
+
+int pth_resident_changeover(pth_resident *r, const long long *epi_id, const long long *epj_id,
+                            const double *epj_rin, const double *epj_rout)
+{
+    if (!r) return PTH_ERR_ARG;
+    pth_ctx *c = r->ctx;
+    if (!r->co) return c->fail(PTH_ERR_STATE, "pth_resident_changeover: the context is not in changeover mode");
+    const long long ni = r->plan.ni_tot;
+    if ((ni > 0 && !epi_id) || (r->n_epj_tot > 0 && (!epj_id || !epj_rin || !epj_rout)))
+        return c->fail(PTH_ERR_ARG, "pth_resident_changeover: null array");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    std::vector<EpjCoDev> co(std::max(r->n_epj_tot, 1));
+    for (int i = 0; i < r->n_epj_tot; i++) {
+        co[i].rin = (float)epj_rin[i];
+        co[i].rout = (float)epj_rout[i];
+        co[i].id = epj_id[i];
+    }
+    if (r->n_epj_tot > 0)
+        HIPCHK(c, hipMemcpy(r->epj_co.p, co.data(), sizeof(EpjCoDev) * r->n_epj_tot, hipMemcpyHostToDevice),
+               "resident upload");
+    if (ni > 0)
+        HIPCHK(c, hipMemcpy(r->arena.as<char>() + r->A.epi_id, epi_id, sizeof(long long) * ni,
+                            hipMemcpyHostToDevice), "resident upload");
+    r->co_ready = true;
     return PTH_OK;
 }
 
@@ -686,6 +849,8 @@ int pth_resident_run(pth_resident *r)
 {
     if (!r) return PTH_ERR_ARG;
     pth_ctx *c = r->ctx;
+    if (r->co && !r->co_ready)
+        return c->fail(PTH_ERR_STATE, "pth_resident_run: changeover mode needs pth_resident_changeover first");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     HIPCHK(c, hipEventRecord(r->ev[0], c->stream), "hipEventRecord");
     HIPCHK(c, launch_stage(r->b, c->stream), "stage kernels");
@@ -708,7 +873,7 @@ int pth_resident_sync(pth_resident *r)
     return PTH_OK;
 }
 
-int pth_resident_results(pth_resident *r, double *acc, double *pot, long long *nnb)
+int pth_resident_results(pth_resident *r, double *acc, double *pot, long long *nnb, double *pot_tot)
 {
     if (!r) return PTH_ERR_ARG;
     pth_ctx *c = r->ctx;
@@ -721,6 +886,20 @@ int pth_resident_results(pth_resident *r, double *acc, double *pot, long long *n
     if (n > 0) {
         HIPCHK(c, hipMemcpy(a.data(), r->out_accp.p, sizeof(double4) * n, hipMemcpyDeviceToHost), "result download");
         HIPCHK(c, hipMemcpy(nb.data(), r->out_nnb.p, sizeof(int) * n, hipMemcpyDeviceToHost), "result download");
+    }
+    if (r->co) {
+        const int nw = (int)r->plan.walks.size();
+        std::vector<int> err(std::max(nw, 1));
+        if (nw > 0)
+            HIPCHK(c, hipMemcpy(err.data(), r->work.walk_err.p, sizeof(int) * nw, hipMemcpyDeviceToHost),
+                   "result download");
+        for (int w = 0; w < nw; w++)
+            if (err[w])
+                return c->fail(PTH_ERR_ARG, "pth_resident_results: changeover mode: an i-particle of walk " +
+                                                std::to_string(w) + " has no EPJ with its id in the walk's list");
+        if (pot_tot && n > 0)
+            HIPCHK(c, hipMemcpy(pot_tot, r->out_pott.p, sizeof(double) * n, hipMemcpyDeviceToHost),
+                   "result download");
     }
     for (long long i = 0; i < n; i++) {
         if (acc) {
@@ -770,6 +949,8 @@ void pth_resident_free(pth_resident *r)
     r->spj.release();
     r->out_accp.release();
     r->out_nnb.release();
+    r->epj_co.release();
+    r->out_pott.release();
     r->work.release();
     for (int k = 0; k < 4; k++)
         if (r->ev[k]) (void)hipEventDestroy(r->ev[k]);

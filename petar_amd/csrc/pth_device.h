@@ -23,12 +23,26 @@ struct __align__(64) SpjDev {
     float pad[3];
 };
 
+// Changeover data of an EPJ (16 B, changeover mode only): the particle's
+// changeover radii (EPJSoft::r_in, r_out) and id.
+struct __align__(16) EpjCoDev {
+    float rin, rout;
+    long long id;
+};
+
 // Batch i-particle (32 B): FP64 position, r_search.
 struct __align__(32) EpiDev {
     double x, y, z;
     float rs;
     int pad;
 };
+
+// Changeover mode: an i-group may hold at most this many i (the per-walk
+// id table lives in LDS).
+#define PTH_CO_MAX_NI 1024
+#define PTH_CO_HASH 2048
+// floats per staged EPJ block of 8 in the changeover extras
+#define PTH_CO_EXTRA 40
 
 // One walk (i-group) of a batch: i range, list ranges (in the concatenated
 // id lists) and sizes.
@@ -65,8 +79,19 @@ struct BatchDev {
     int n_items = 0, n_ritems = 0;
     int seg_begin[3] = {0, 0, 0}, seg_count[3] = {0, 0, 0};  // items of ipl 1, 2, 4
     int plain = 0, nnb_le = 0, j_vgpr = 0;
+    int co = 0;        // changeover mode: (1 - K) weight for neighbours, no self term
     float eps2 = 0.f, rout2 = 0.f;
+    float ngb2 = 0.f;  // changeover mode: squared neighbour-radius factor
     double G = 1.0;
+    // changeover mode only
+    const long long *epi_id = nullptr;
+    const EpjCoDev *epj_co = nullptr;
+    float4 *staged_ic = nullptr;  // per i: {self slot (int bits), r_in, r_out, m}
+    float4 *staged_il = nullptr;  // per i: low parts of x-c, y-c, z-c (double-float), 0
+    float *staged_ec = nullptr;   // per staged EPJ block of 8: {r_in[8], r_out[8], x, y, z low parts[8]}
+    int *walk_err = nullptr;      // per walk: number of i without their own EPJ
+    double *out_pott = nullptr;
+    double *part_pott = nullptr;
     const EpiDev *epi = nullptr;
     const WalkDev *walks = nullptr;
     const long long *es_off = nullptr;  // n_walk+1 staged EPJ slot offsets

@@ -88,19 +88,78 @@ __device__ __forceinline__ void put_spj(float4 *o, const SpjDev &p, const double
     o[2] = make_float4((float)(3.0 * (double)p.qyz), (float)(3.0 * (double)p.qxz), (float)(-eps2 * tr), 0.f);
 }
 
-__global__ __launch_bounds__(256) void k_stage_walk(
-    const EpiDev *__restrict__ epi, const WalkDev *__restrict__ walks,
-    const long long *__restrict__ es_off, const int *__restrict__ id_epj,
-    const EpjDev *__restrict__ epj, const long long *__restrict__ ss_off,
-    const int *__restrict__ id_spj, const SpjDev *__restrict__ spj, double r_out, double eps2,
-    double4 *__restrict__ centre, int *__restrict__ near_blk, float4 *__restrict__ staged_i,
-    float *__restrict__ staged_e, float *__restrict__ staged_s)
+// bucket of a particle id in the per-walk LDS table (PTH_CO_HASH buckets)
+__device__ __forceinline__ unsigned co_hash(long long id)
+{
+    const unsigned long long u = (unsigned long long)id;
+    return ((unsigned)(u ^ (u >> 32)) * 2654435761u) >> 21;
+}
+
+struct StageArgs {
+    const EpiDev *epi;
+    const WalkDev *walks;
+    const long long *es_off, *ss_off;
+    const int *id_epj, *id_spj;
+    const EpjDev *epj;
+    const SpjDev *spj;
+    double r_out, eps2;
+    double4 *centre;
+    int *near_blk;
+    float4 *staged_i;
+    float *staged_e, *staged_s;
+    // changeover mode
+    const long long *epi_id;
+    const EpjCoDev *epj_co;
+    float4 *staged_ic;
+    float4 *staged_il;
+    float *staged_ec;
+    int *walk_err;
+};
+
+// CO (changeover mode): every near EPJ also gets its changeover radii and the
+// low-order parts of its FP32 relative coordinates staged (per block of 8:
+// r_in[8], r_out[8], xlo[8], ylo[8], zlo[8]; x - c = x_hi + x_lo as a
+// double-float), every i the low-order parts of its coordinates, and each i
+// finds its own EPJ entry
+// (same id) among the walk's near entries through an LDS hash table of the
+// walk's i ids; the i's staged slot of that entry and its radii go to
+// staged_ic (the force kernel drops the self pair by slot and takes the i's
+// changeover radii from there).  walk_err counts i with no such entry.
+template <bool CO>
+__global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
 {
     const int w = blockIdx.x;
-    const WalkDev wk = walks[w];
+    const WalkDev wk = a.walks[w];
+    const EpiDev *__restrict__ epi = a.epi;
+    const EpjDev *__restrict__ epj = a.epj;
+    const int *__restrict__ id_epj = a.id_epj;
+    const double r_out = a.r_out;
+    float4 *__restrict__ staged_i = a.staged_i;
+    float *__restrict__ staged_e = a.staged_e;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     __shared__ double red[4][7];
     __shared__ int cnt[2][4];
+    __shared__ long long co_key[CO ? PTH_CO_HASH : 1];
+    __shared__ int co_idx[CO ? PTH_CO_HASH : 1];
+    __shared__ int co_self[CO ? PTH_CO_MAX_NI : 1];
+    __shared__ float co_rin[CO ? PTH_CO_MAX_NI : 1], co_rout[CO ? PTH_CO_MAX_NI : 1], co_m[CO ? PTH_CO_MAX_NI : 1];
+    const int n_co = CO ? min(wk.n_i, PTH_CO_MAX_NI) : 0;
+    if constexpr (CO) {
+        for (int k = tid; k < PTH_CO_HASH; k += 256) co_idx[k] = -1;
+        __syncthreads();
+        for (int k = tid; k < n_co; k += 256) {
+            const long long id = a.epi_id[wk.i_off + k];
+            unsigned h = co_hash(id);
+            // at most PTH_CO_MAX_NI keys in PTH_CO_HASH buckets: a free one exists
+            while (atomicCAS(&co_idx[h], -1, k) != -1) h = (h + 1) & (PTH_CO_HASH - 1);
+            co_key[h] = id;
+            co_self[k] = -1;
+            co_rin[k] = 0.f;
+            co_rout[k] = 0.f;
+            co_m[k] = 0.f;
+        }
+        __syncthreads();
+    }
     // -- bounding box and largest r_search of the walk's i
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rsm = 0.0;
     for (int k = tid; k < wk.n_i; k += 256) {
@@ -130,16 +189,20 @@ __global__ __launch_bounds__(256) void k_stage_walk(
         h[d] = (wk.n_i > 0) ? 0.5 * (u - l) : 0.0;
     }
     rsm = fmax(fmax(red[0][6], red[1][6]), fmax(red[2][6], red[3][6]));
-    if (tid == 0) centre[w] = make_double4(c[0], c[1], c[2], 0.0);
+    if (tid == 0) a.centre[w] = make_double4(c[0], c[1], c[2], 0.0);
     for (int k = tid; k < wk.n_i; k += 256) {
         const EpiDev e = epi[wk.i_off + k];
         const double rs = (double)e.rs;
-        staged_i[wk.i_off + k] = make_float4((float)(e.x - c[0]), (float)(e.y - c[1]),
-                                             (float)(e.z - c[2]), (float)(rs * rs));
+        const double dx = e.x - c[0], dy = e.y - c[1], dz = e.z - c[2];
+        const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+        staged_i[wk.i_off + k] = make_float4(hx, hy, hz, (float)(rs * rs));
+        if constexpr (CO)
+            a.staged_il[wk.i_off + k] =
+                make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.f);
     }
     // -- EPJ list: stable split into near (front) and far (back, reversed)
-    const long long base = es_off[w];
-    const long long cap = es_off[w + 1] - base;
+    const long long base = a.es_off[w];
+    const long long cap = a.es_off[w + 1] - base;
     const double hmax = fmax(h[0], fmax(h[1], h[2]));
     int n_near = 0, n_far = 0;
     for (int t0 = 0; t0 < wk.n_e; t0 += 256) {
@@ -147,8 +210,11 @@ __global__ __launch_bounds__(256) void k_stage_walk(
         const bool valid = t < wk.n_e;
         bool nearj = false;
         float x = 0.f, y = 0.f, z = 0.f, m = 0.f, r2 = 0.f;
+        float xl = 0.f, yl = 0.f, zl = 0.f;
+        int jx = 0;
         if (valid) {
-            const EpjDev e = epj[id_epj[(long long)wk.e_off + t]];
+            jx = id_epj[(long long)wk.e_off + t];
+            const EpjDev e = epj[jx];
             const double dx = e.x - c[0], dy = e.y - c[1], dz = e.z - c[2];
             // near: inside the walk's box widened by max(rs_i, rs_j, r_out),
             // with a margin that covers the FP32 rounding of the pair arithmetic
@@ -159,6 +225,9 @@ __global__ __launch_bounds__(256) void k_stage_walk(
             x = (float)dx;
             y = (float)dy;
             z = (float)dz;
+            xl = (float)(dx - (double)x);
+            yl = (float)(dy - (double)y);
+            zl = (float)(dz - (double)z);
             m = e.m;
             const double rs = (double)e.rs;
             r2 = (float)(rs * rs);
@@ -182,6 +251,32 @@ __global__ __launch_bounds__(256) void k_stage_walk(
             const long long slot = nearj ? base + on + __popcll(bn & below)
                                          : base + cap - 1 - (of + __popcll(bf & below));
             put_epj(staged_e, slot, x, y, z, m, r2);
+            if constexpr (CO) {
+                if (nearj) {
+                    const EpjCoDev co = a.epj_co[jx];
+                    float *blk = a.staged_ec + (slot >> 3) * PTH_CO_EXTRA;
+                    const int l = (int)(slot & 7);
+                    blk[l] = co.rin;
+                    blk[8 + l] = co.rout;
+                    blk[16 + l] = xl;
+                    blk[24 + l] = yl;
+                    blk[32 + l] = zl;
+                    // is this the EPJ of one of the walk's i?
+                    unsigned hb = co_hash(co.id);
+                    for (;;) {
+                        const int k = co_idx[hb];
+                        if (k < 0) break;
+                        if (co_key[hb] == co.id) {
+                            co_self[k] = (int)slot;
+                            co_rin[k] = co.rin;
+                            co_rout[k] = co.rout;
+                            co_m[k] = m;
+                            break;
+                        }
+                        hb = (hb + 1) & (PTH_CO_HASH - 1);
+                    }
+                }
+            }
         }
         n_near += tn;
         n_far += tf;
@@ -190,19 +285,33 @@ __global__ __launch_bounds__(256) void k_stage_walk(
     // pad the gap between the two regions (far, massless)
     for (long long s = base + n_near + tid; s < base + cap - n_far; s += 256)
         put_epj(staged_e, s, 1e18f, 1e18f, 1e18f, 0.f, 0.f);
-    if (tid == 0) near_blk[w] = (int)((base + ((n_near + 7) & ~7)) >> 3);
+    if (tid == 0) a.near_blk[w] = (int)((base + ((n_near + 7) & ~7)) >> 3);
+    if constexpr (CO) {
+        // (the loop above ended with a barrier: co_self is complete)
+        int miss = 0;
+        for (int k = tid; k < wk.n_i; k += 256) {
+            const bool in = k < n_co;
+            const int s = in ? co_self[k] : -1;
+            a.staged_ic[wk.i_off + k] =
+                make_float4(__int_as_float(s), in ? co_rin[k] : 0.f, in ? co_rout[k] : 0.f, in ? co_m[k] : 0.f);
+            miss |= (s < 0);
+        }
+        const int n_miss = __syncthreads_count(miss);
+        if (tid == 0) a.walk_err[w] = n_miss;
+    }
     // -- SPJ list: 12-float records relative to the centre, two per thread
     //    and pass so that two gathers are in flight
-    float4 *out_s = reinterpret_cast<float4 *>(staged_s) + ss_off[w] * 3;
-    const int *ids = id_spj + wk.s_off;
+    float4 *out_s = reinterpret_cast<float4 *>(a.staged_s) + a.ss_off[w] * 3;
+    const int *ids = a.id_spj + wk.s_off;
+    const SpjDev *__restrict__ spj = a.spj;
     for (int k0 = 0; k0 < wk.n_s; k0 += 512) {
         const int ka = k0 + tid, kb = ka + 256;
         const bool va = ka < wk.n_s, vb = kb < wk.n_s;
         SpjDev pa, pb;
         if (va) pa = spj[ids[ka]];
         if (vb) pb = spj[ids[kb]];
-        if (va) put_spj(out_s + 3 * ka, pa, c, eps2);
-        if (vb) put_spj(out_s + 3 * kb, pb, c, eps2);
+        if (va) put_spj(out_s + 3 * ka, pa, c, a.eps2);
+        if (vb) put_spj(out_s + 3 * kb, pb, c, a.eps2);
     }
 }
 
@@ -273,6 +382,143 @@ __device__ __forceinline__ void epj_blocks(cfp se, int b_begin, int b_end, const
     }
 }
 
+// Changeover weight of one neighbour pair (src/changeover.hpp:294-334 with
+// the pair rule of :366-375, i being the first changeover): the Newtonian
+// terms weighted by the changeover, f = m (1 - W_acc) / r^3 (acc factor),
+// pg = m (1 - W_pot) / r (pot_soft) and pt = -m W_pot / r (pot_tot -
+// pot_soft), i.e. what src/hard.hpp:1404-1476 leaves for a single j once the
+// linear-cutoff terms are removed.
+__device__ __forceinline__ void co_pair(float r2, float mj, float rin_i, float rout_i, float rin_j,
+                                        float rout_j, float &f, float &pg, float &pt)
+{
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;  // sqrt(r^2 + eps^2), as dr_eps of src/hard.hpp:1420
+    const bool use_i = rout_i > rout_j;
+    const float rin = use_i ? rin_i : rin_j;
+    const float rout = use_i ? rout_i : rout_j;
+    const float xr = (r - rin) * __builtin_amdgcn_rcpf(rout - rin);
+    const float coff = (rout - rin) * __builtin_amdgcn_rcpf(rout + rin);
+    const float x = __builtin_fminf(__builtin_fmaxf(xr, 0.f), 1.f);
+    const float x_1 = x - 1.f, x_2 = x_1 * x_1, x_4 = x_2 * x_2;
+    const float wacc = x_4 * __builtin_fmaf(
+                                 x, __builtin_fmaf(x, __builtin_fmaf(x, __builtin_fmaf(x, 35.f * coff, 20.f), 10.f), 4.f),
+                                 1.f);
+    const float x2 = x * x, x5 = x2 * x2 * x;
+    float wpot = __builtin_fmaf(-coff * x5,
+                                __builtin_fmaf(x, __builtin_fmaf(x, __builtin_fmaf(x, 5.f, -20.f), 28.f), -14.f), 1.f);
+    if (xr >= 1.f) wpot = (1.f + coff) * r * __builtin_amdgcn_rcpf(rout);
+    const float mri = mj * rinv;
+    f = mri * rinv * rinv * (1.f - wacc);
+    pg = mri * (1.f - wpot);
+    pt = -mri * wpot;
+}
+
+// Near EPJ blocks in changeover mode.  Every pair first gets the
+// linear-cutoff terms of src/force_gpu_cuda.cu:50-77 (the same instructions
+// as epj_blocks), and is tested for being a neighbour pair: r^2 + eps^2 <
+// ngb2 max(rs_i, rs_j)^2 + eps^2 with ngb2 = 0.99^2 (getRSearch() = 0.99
+// r_search of src/soft_ptcl.hpp:97-101,349-353 on both sides), self excluded
+// (by staged slot).  When some lane of the wave has a neighbour pair in a
+// block of 8 j, the block is revisited and, on the neighbour pairs only, the
+// linear-cutoff terms are replaced by co_pair's.
+// The self pair's -m_i/r_out is removed by the caller (src/hard.hpp:1669-1673
+// adds the same term back on the CPU).
+template <int IPL, int NNB_LE>
+__device__ __forceinline__ void epj_near_co(cfp se, cfp sec, int b_begin, int b_end, const float (&xi)[IPL],
+                                            const float (&yi)[IPL], const float (&zi)[IPL],
+                                            const float (&xil)[IPL], const float (&yil)[IPL],
+                                            const float (&zil)[IPL], const float (&rsi2)[IPL],
+                                            const int (&self)[IPL], const float (&rin_i)[IPL],
+                                            const float (&rout_i)[IPL], const Consts &K, float ngb2,
+                                            double (&A)[IPL][4], double (&PT)[IPL], int (&nnb)[IPL])
+{
+    for (int b0 = b_begin; b0 < b_end; b0 += PTH_FLUSH_BLOCKS) {
+        const int b1 = min(b0 + PTH_FLUSH_BLOCKS, b_end);
+        float ax[IPL], ay[IPL], az[IPL], p[IPL], pt[IPL];
+        int nb[IPL];
+#pragma unroll
+        for (int q = 0; q < IPL; q++) { ax[q] = ay[q] = az[q] = p[q] = pt[q] = 0.f; nb[q] = 0; }
+        for (int b = b0; b < b1; b++) {
+            const cfp B = se + (size_t)b * 40;
+            bool hit = false;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float xj = B[k], yj = B[8 + k], zj = B[16 + k], mj = B[24 + k], rj = B[32 + k];
+                const int sj = b * 8 + k;
+#pragma unroll
+                for (int q = 0; q < IPL; q++) {
+                    const float dx = xj - xi[q];
+                    const float dy = yj - yi[q];
+                    const float dz = zj - zi[q];
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
+                    const float rs2 = __int_as_float(max(__float_as_int(rsi2[q]), __float_as_int(rj)));
+                    if (NNB_LE) nb[q] += (r2 <= rs2) ? 1 : 0;
+                    else nb[q] += (r2 < rs2) ? 1 : 0;
+                    const float ri = __builtin_amdgcn_rsqf(__builtin_fmaxf(r2, K.rout2));
+                    const float mri = mj * ri;
+                    const float ri2 = ri * ri;
+                    const float mri3 = mri * ri2;
+                    ax[q] = __builtin_fmaf(mri3, dx, ax[q]);
+                    ay[q] = __builtin_fmaf(mri3, dy, ay[q]);
+                    az[q] = __builtin_fmaf(mri3, dz, az[q]);
+                    p[q] -= mri;
+                    hit = ((r2 < __builtin_fmaf(ngb2, rs2, K.eps2)) && sj != self[q]) ? true : hit;
+                }
+            }
+            if (__ballot(hit)) {
+                // some lane has a neighbour pair in this block: revisit it
+                const cfp E = sec + (size_t)b * PTH_CO_EXTRA;
+#pragma unroll 1
+                for (int k = 0; k < 8; k++) {
+                    const float xj = B[k], yj = B[8 + k], zj = B[16 + k], mj = B[24 + k], rj = B[32 + k];
+                    const float rinj = E[k], routj = E[8 + k];
+                    const float xjl = E[16 + k], yjl = E[24 + k], zjl = E[32 + k];
+                    const int sj = b * 8 + k;
+#pragma unroll
+                    for (int q = 0; q < IPL; q++) {
+                        const float dx = xj - xi[q];
+                        const float dy = yj - yi[q];
+                        const float dz = zj - zi[q];
+                        const float r2 =
+                            __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
+                        const float rs2 = __int_as_float(max(__float_as_int(rsi2[q]), __float_as_int(rj)));
+                        const bool nbr = (r2 < __builtin_fmaf(ngb2, rs2, K.eps2)) && sj != self[q];
+                        // the linear-cutoff terms added above
+                        const float ri = __builtin_amdgcn_rsqf(__builtin_fmaxf(r2, K.rout2));
+                        const float mri = mj * ri;
+                        const float mri3 = mri * (ri * ri);
+                        // the pair in double-float relative coordinates: the
+                        // high parts are close, so their difference is exact
+                        // and the separation keeps full FP32 relative
+                        // precision however small it is next to |x - c|
+                        const float dxp = dx + (xjl - xil[q]);
+                        const float dyp = dy + (yjl - yil[q]);
+                        const float dzp = dz + (zjl - zil[q]);
+                        const float r2p =
+                            __builtin_fmaf(dzp, dzp, __builtin_fmaf(dyp, dyp, __builtin_fmaf(dxp, dxp, K.eps2)));
+                        float f, pg, ptj;
+                        co_pair(r2p, mj, rin_i[q], rout_i[q], rinj, routj, f, pg, ptj);
+                        ax[q] += nbr ? __builtin_fmaf(f, dxp, -mri3 * dx) : 0.f;
+                        ay[q] += nbr ? __builtin_fmaf(f, dyp, -mri3 * dy) : 0.f;
+                        az[q] += nbr ? __builtin_fmaf(f, dzp, -mri3 * dz) : 0.f;
+                        p[q] -= nbr ? pg - mri : 0.f;
+                        pt[q] += nbr ? ptj : 0.f;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            A[q][0] += (double)ax[q];
+            A[q][1] += (double)ay[q];
+            A[q][2] += (double)az[q];
+            A[q][3] += (double)p[q];
+            PT[q] += (double)pt[q];
+            nnb[q] += nb[q];
+        }
+    }
+}
+
 template <int IPL, bool VC>
 __device__ __forceinline__ void spj_records(cfp ss, int s_begin, int s_end, const float (&xi)[IPL],
                                             const float (&yi)[IPL], const float (&zi)[IPL], const Consts &K,
@@ -323,61 +569,105 @@ __device__ __forceinline__ void spj_records(cfp ss, int s_begin, int s_end, cons
     }
 }
 
-template <int IPL, int NNB_LE, bool VC>
-__device__ __forceinline__ void run_tile(const Item &it, int near_end, int lane,
-                                         const float4 *__restrict__ staged_i, cfp se, cfp ss, const Consts &K,
-                                         double G, double4 *__restrict__ out_accp, int *__restrict__ out_nnb,
-                                         double4 *__restrict__ part_accp, int *__restrict__ part_nnb)
+// Arguments of the fused force launches.
+struct ForceArgs {
+    const Item *items;
+    int n_items;
+    const int *near_blk;
+    const float4 *staged_i;
+    const float *staged_e, *staged_s;
+    const float4 *staged_ic;  // changeover mode
+    const float4 *staged_il;  // changeover mode
+    const float *staged_ec;   // changeover mode
+    float eps2, rout2, ngb2;
+    double G;
+    double4 *out_accp;
+    int *out_nnb;
+    double *out_pott;  // changeover mode
+    double4 *part_accp;
+    int *part_nnb;
+    double *part_pott;  // changeover mode
+};
+
+template <int IPL, int NNB_LE, bool VC, bool CO>
+__device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int near_end, int lane,
+                                         const Consts &K)
 {
     const int n_i = item_n_i(it.n_i_ipl);
     float xi[IPL], yi[IPL], zi[IPL], rsi2[IPL];
-    double A[IPL][4];
+    double A[IPL][4], PT[IPL];
     int nnb[IPL];
+    int self[IPL];
+    float rin_i[IPL], rout_i[IPL], m_i[IPL], xil[IPL], yil[IPL], zil[IPL];
 #pragma unroll
     for (int q = 0; q < IPL; q++) {
+        const bool in = lane + 64 * q < n_i;
         float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
-        if (lane + 64 * q < n_i) v = staged_i[it.i_begin + lane + 64 * q];
+        if (in) v = a.staged_i[it.i_begin + lane + 64 * q];
         xi[q] = v.x;
         yi[q] = v.y;
         zi[q] = v.z;
         rsi2[q] = v.w;
         A[q][0] = A[q][1] = A[q][2] = A[q][3] = 0.0;
+        PT[q] = 0.0;
         nnb[q] = 0;
+        if constexpr (CO) {
+            float4 c = make_float4(__int_as_float(-2), 0.f, 0.f, 0.f);
+            if (in) c = a.staged_ic[it.i_begin + lane + 64 * q];
+            self[q] = __float_as_int(c.x);
+            rin_i[q] = c.y;
+            rout_i[q] = c.z;
+            m_i[q] = c.w;
+            float4 l = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (in) l = a.staged_il[it.i_begin + lane + 64 * q];
+            xil[q] = l.x;
+            yil[q] = l.y;
+            zil[q] = l.z;
+        }
     }
+    const cfp se = as_const(a.staged_e);
     const int e_mid = min(max(near_end, it.e_blk_begin), it.e_blk_end);
-    epj_blocks<IPL, NNB_LE, true, VC>(se, it.e_blk_begin, e_mid, xi, yi, zi, rsi2, K, A, nnb);
+    if constexpr (CO) {
+        epj_near_co<IPL, NNB_LE>(se, as_const(a.staged_ec), it.e_blk_begin, e_mid, xi, yi, zi, xil, yil, zil, rsi2,
+                                 self, rin_i, rout_i, K, a.ngb2, A, PT, nnb);
+        // remove the self pair's linear-cutoff potential -m_i / max(eps, r_out)
+        // (added above with these same FP32 operations) in the item whose
+        // j-range holds the i's own entry
+#pragma unroll
+        for (int q = 0; q < IPL; q++) {
+            const int sb = self[q] >> 3;
+            if (self[q] >= 0 && sb >= it.e_blk_begin && sb < e_mid)
+                A[q][3] += (double)(m_i[q] * __builtin_amdgcn_rsqf(__builtin_fmaxf(K.eps2, K.rout2)));
+        }
+    } else
+        epj_blocks<IPL, NNB_LE, true, VC>(se, it.e_blk_begin, e_mid, xi, yi, zi, rsi2, K, A, nnb);
     epj_blocks<IPL, NNB_LE, false, VC>(se, e_mid, it.e_blk_end, xi, yi, zi, rsi2, K, A, nnb);
-    spj_records<IPL, VC>(ss, it.s_begin, it.s_end, xi, yi, zi, K, A);
+    spj_records<IPL, VC>(as_const(a.staged_s), it.s_begin, it.s_end, xi, yi, zi, K, A);
+    const double G = a.G;
 #pragma unroll
     for (int q = 0; q < IPL; q++) {
         const int i = lane + 64 * q;
         if (i >= n_i) continue;
         if (it.part < 0) {
-            out_accp[it.i_begin + i] = make_double4(G * A[q][0], G * A[q][1], G * A[q][2], G * A[q][3]);
-            out_nnb[it.i_begin + i] = nnb[q];
+            a.out_accp[it.i_begin + i] = make_double4(G * A[q][0], G * A[q][1], G * A[q][2], G * A[q][3]);
+            a.out_nnb[it.i_begin + i] = nnb[q];
+            if constexpr (CO) a.out_pott[it.i_begin + i] = G * (A[q][3] + PT[q]);
         } else {
             const size_t slot = (size_t)(it.part + q) * 64 + lane;
-            part_accp[slot] = make_double4(A[q][0], A[q][1], A[q][2], A[q][3]);
-            part_nnb[slot] = nnb[q];
+            a.part_accp[slot] = make_double4(A[q][0], A[q][1], A[q][2], A[q][3]);
+            a.part_nnb[slot] = nnb[q];
+            if constexpr (CO) a.part_pott[slot] = A[q][3] + PT[q];
         }
     }
 }
 
 // One launch per i-per-lane class: items [0, n_items) of this class.
-template <int NNB_LE, int IPL, bool VC>
-__global__ __launch_bounds__(256) void k_force(const Item *__restrict__ items, int n_items,
-                                               const int *__restrict__ near_blk,
-                                               const float4 *__restrict__ staged_i,
-                                               const float *__restrict__ staged_e,
-                                               const float *__restrict__ staged_s, float eps2,
-                                               float rout2, double G, double4 *__restrict__ out_accp,
-                                               int *__restrict__ out_nnb,
-                                               double4 *__restrict__ part_accp,
-                                               int *__restrict__ part_nnb)
+template <int NNB_LE, int IPL, bool VC, bool CO>
+__global__ __launch_bounds__(256) void k_force(const ForceArgs a)
 {
     const int idx = uniform((int)blockIdx.x * 4 + ((int)threadIdx.x >> 6));
-    if (idx >= n_items) return;
-    const cip ci = (cip)(items) + (size_t)idx * 8;
+    if (idx >= a.n_items) return;
+    const cip ci = (cip)(a.items) + (size_t)idx * 8;
     Item it;
     it.walk = ci[0];
     it.i_begin = ci[1];
@@ -388,10 +678,9 @@ __global__ __launch_bounds__(256) void k_force(const Item *__restrict__ items, i
     it.s_end = ci[6];
     it.part = ci[7];
     if (it.walk < 0) return;
-    const int near_end = ((cip)(near_blk))[it.walk];
-    const Consts K{jv<VC>(eps2), jv<VC>(rout2), jv<VC>(0.5f), jv<VC>(2.5f)};
-    run_tile<IPL, NNB_LE, VC>(it, near_end, threadIdx.x & 63, staged_i, as_const(staged_e), as_const(staged_s),
-                              K, G, out_accp, out_nnb, part_accp, part_nnb);
+    const int near_end = ((cip)(a.near_blk))[it.walk];
+    const Consts K{jv<VC>(a.eps2), jv<VC>(a.rout2), jv<VC>(0.5f), jv<VC>(2.5f)};
+    run_tile<IPL, NNB_LE, VC, CO>(a, it, near_end, threadIdx.x & 63, K);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,17 +787,19 @@ __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__
 }
 
 // One 256-thread workgroup per split tile.
+template <bool CO>
 __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems,
                                                 const double4 *__restrict__ part_accp,
-                                                const int *__restrict__ part_nnb, double G,
-                                                double4 *__restrict__ out_accp,
-                                                int *__restrict__ out_nnb)
+                                                const int *__restrict__ part_nnb,
+                                                const double *__restrict__ part_pott, double G,
+                                                double4 *__restrict__ out_accp, int *__restrict__ out_nnb,
+                                                double *__restrict__ out_pott)
 {
     const RItem r = ritems[blockIdx.x];
     const int t = threadIdx.x;
     if (t >= r.n_i) return;
     const int sub = t >> 6, lane = t & 63;
-    double ax = 0, ay = 0, az = 0, p = 0;
+    double ax = 0, ay = 0, az = 0, p = 0, pt = 0;
     int nb = 0;
     for (int k = 0; k < r.n_part; k++) {
         const size_t slot = (size_t)(r.part_first + k * r.ipl + sub) * 64 + lane;
@@ -518,9 +809,11 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
         az += a.z;
         p += a.w;
         nb += part_nnb[slot];
+        if constexpr (CO) pt += part_pott[slot];
     }
     out_accp[r.out + t] = make_double4(G * ax, G * ay, G * az, G * p);
     out_nnb[r.out + t] = nb;
+    if constexpr (CO) out_pott[r.out + t] = G * pt;
 }
 
 // ---------------------------------------------------------------------------
@@ -529,10 +822,39 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 hipError_t launch_stage(const BatchDev &b, hipStream_t st)
 {
     if (b.n_walk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stage_walk, dim3(b.n_walk), dim3(256), 0, st, b.epi, b.walks, b.es_off, b.id_epj,
-                       b.epj, b.ss_off, b.id_spj, b.spj, std::sqrt((double)b.rout2), (double)b.eps2, b.centre,
-                       b.near_blk, b.staged_i, b.staged_e, b.staged_s);
+    StageArgs a;
+    a.epi = b.epi;
+    a.walks = b.walks;
+    a.es_off = b.es_off;
+    a.ss_off = b.ss_off;
+    a.id_epj = b.id_epj;
+    a.id_spj = b.id_spj;
+    a.epj = b.epj;
+    a.spj = b.spj;
+    a.r_out = std::sqrt((double)b.rout2);
+    a.eps2 = (double)b.eps2;
+    a.centre = b.centre;
+    a.near_blk = b.near_blk;
+    a.staged_i = b.staged_i;
+    a.staged_e = b.staged_e;
+    a.staged_s = b.staged_s;
+    a.epi_id = b.epi_id;
+    a.epj_co = b.epj_co;
+    a.staged_ic = b.staged_ic;
+    a.staged_il = b.staged_il;
+    a.staged_ec = b.staged_ec;
+    a.walk_err = b.walk_err;
+    if (b.co) hipLaunchKernelGGL(k_stage_walk<true>, dim3(b.n_walk), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_stage_walk<false>, dim3(b.n_walk), dim3(256), 0, st, a);
     return hipGetLastError();
+}
+
+template <int NNB, bool VC, bool CO>
+static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t st)
+{
+    if (ipl == 1) hipLaunchKernelGGL((k_force<NNB, 1, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
+    else if (ipl == 2) hipLaunchKernelGGL((k_force<NNB, 2, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_force<NNB, 4, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
 }
 
 hipError_t launch_force(const BatchDev &b, hipStream_t st)
@@ -550,30 +872,41 @@ hipError_t launch_force(const BatchDev &b, hipStream_t st)
         return hipGetLastError();
     }
     static const int ipls[3] = {1, 2, 4};
+    ForceArgs a;
+    a.near_blk = b.near_blk;
+    a.staged_i = b.staged_i;
+    a.staged_e = b.staged_e;
+    a.staged_s = b.staged_s;
+    a.staged_ic = b.staged_ic;
+    a.staged_il = b.staged_il;
+    a.staged_ec = b.staged_ec;
+    a.eps2 = b.eps2;
+    a.rout2 = b.rout2;
+    a.ngb2 = b.ngb2;
+    a.G = b.G;
+    a.out_accp = b.out_accp;
+    a.out_nnb = b.out_nnb;
+    a.out_pott = b.out_pott;
+    a.part_accp = b.part_accp;
+    a.part_nnb = b.part_nnb;
+    a.part_pott = b.part_pott;
     for (int c = 0; c < 3; c++) {
         const int n = b.seg_count[c];
         if (n <= 0) continue;
-        const Item *it = b.items + b.seg_begin[c];
+        a.items = b.items + b.seg_begin[c];
+        a.n_items = n;
         const unsigned nblk = (unsigned)((n + 3) / 4);
-#define PTH_LAUNCH(NNB, IPL, VC)                                                                           \
-    hipLaunchKernelGGL((k_force<NNB, IPL, VC>), dim3(nblk), dim3(256), 0, st, it, n, b.near_blk, b.staged_i, \
-                       b.staged_e, b.staged_s, b.eps2, b.rout2, b.G, b.out_accp, b.out_nnb, b.part_accp,     \
-                       b.part_nnb)
-#define PTH_LAUNCH_IPL(NNB, VC)                  \
-    do {                                         \
-        if (ipls[c] == 1) PTH_LAUNCH(NNB, 1, VC); \
-        else if (ipls[c] == 2) PTH_LAUNCH(NNB, 2, VC); \
-        else PTH_LAUNCH(NNB, 4, VC);             \
-    } while (0)
-        if (b.nnb_le) {
-            if (b.j_vgpr) PTH_LAUNCH_IPL(1, true);
-            else PTH_LAUNCH_IPL(1, false);
+        // the changeover mode always uses SGPR operands
+        if (b.co) {
+            if (b.nnb_le) launch_class<1, false, true>(ipls[c], nblk, a, st);
+            else launch_class<0, false, true>(ipls[c], nblk, a, st);
+        } else if (b.j_vgpr) {
+            if (b.nnb_le) launch_class<1, true, false>(ipls[c], nblk, a, st);
+            else launch_class<0, true, false>(ipls[c], nblk, a, st);
         } else {
-            if (b.j_vgpr) PTH_LAUNCH_IPL(0, true);
-            else PTH_LAUNCH_IPL(0, false);
+            if (b.nnb_le) launch_class<1, false, false>(ipls[c], nblk, a, st);
+            else launch_class<0, false, false>(ipls[c], nblk, a, st);
         }
-#undef PTH_LAUNCH_IPL
-#undef PTH_LAUNCH
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -583,8 +916,12 @@ hipError_t launch_force(const BatchDev &b, hipStream_t st)
 hipError_t launch_reduce(const BatchDev &b, hipStream_t st)
 {
     if (b.plain || b.n_ritems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)b.n_ritems), dim3(256), 0, st, b.ritems, b.part_accp,
-                       b.part_nnb, b.G, b.out_accp, b.out_nnb);
+    if (b.co)
+        hipLaunchKernelGGL(k_reduce<true>, dim3((unsigned)b.n_ritems), dim3(256), 0, st, b.ritems, b.part_accp,
+                           b.part_nnb, b.part_pott, b.G, b.out_accp, b.out_nnb, b.out_pott);
+    else
+        hipLaunchKernelGGL(k_reduce<false>, dim3((unsigned)b.n_ritems), dim3(256), 0, st, b.ritems, b.part_accp,
+                           b.part_nnb, b.part_pott, b.G, b.out_accp, b.out_nnb, b.out_pott);
     return hipGetLastError();
 }
 

@@ -43,17 +43,21 @@ SPJ_SOFT = np.dtype([("mass", "<f8"), ("pos", "<f8", 3),
                      ("quad", [("xx", "<f8"), ("yy", "<f8"), ("zz", "<f8"), ("xy", "<f8"), ("xz", "<f8"),
                                ("yz", "<f8")])])
 FORCE_SOFT = np.dtype([("acc", "<f8", 3), ("pot", "<f8"), ("n_ngb", "<i8")])
+# ForceSoft with the total potential the changeover mode also returns (pot
+# is then the changeover-weighted soft potential)
+FORCE_SOFT_CO = np.dtype([("acc", "<f8", 3), ("pot", "<f8"), ("n_ngb", "<i8"), ("pot_tot", "<f8")])
 assert EPI_SOFT.itemsize == 48 and EPJ_SOFT.itemsize == 120 and FORCE_SOFT.itemsize == 40
 
 
 class Params(C.Structure):
     _fields_ = [("eps2", C.c_double), ("rcut2", C.c_double), ("G", C.c_double), ("nnb_mode", C.c_int),
                 ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int), ("max_ipl", C.c_int),
-                ("j_vgpr", C.c_int)]
+                ("j_vgpr", C.c_int), ("changeover", C.c_int), ("ngb_factor", C.c_double)]
 
 
 class EpjView(C.Structure):
-    _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_mass", "off_rsearch")]
+    _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_mass", "off_rsearch", "off_id", "off_rin",
+                                          "off_rout")]
 
 
 class SpjView(C.Structure):
@@ -62,18 +66,18 @@ class SpjView(C.Structure):
 
 
 class EpiView(C.Structure):
-    _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_rsearch")]
+    _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_rsearch", "off_id")]
 
 
 class ForceView(C.Structure):
     _fields_ = [("stride", C.c_size_t), ("off_acc", C.c_size_t), ("off_pot", C.c_size_t),
-                ("off_nngb", C.c_size_t), ("nngb_bytes", C.c_int)]
+                ("off_nngb", C.c_size_t), ("nngb_bytes", C.c_int), ("off_pot_tot", C.c_longlong)]
 
 
 EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_error", "pth_destroy", "pth_last_error",
             "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve", "pth_resident_create",
-            "pth_resident_run", "pth_resident_sync", "pth_resident_results", "pth_resident_timing",
-            "pth_resident_stats", "pth_resident_free", "pth_plan_debug")
+            "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
+            "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug")
 
 
 def lib():
@@ -103,7 +107,8 @@ def lib():
                                       C.c_int, dp, dp, dp]
     L.pth_resident_run.argtypes = [vp]
     L.pth_resident_sync.argtypes = [vp]
-    L.pth_resident_results.argtypes = [vp, dp, dp, C.POINTER(C.c_longlong)]
+    L.pth_resident_changeover.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp, dp]
+    L.pth_resident_results.argtypes = [vp, dp, dp, C.POINTER(C.c_longlong), dp]
     L.pth_resident_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pth_resident_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 4
     L.pth_resident_free.argtypes = [vp]
@@ -132,7 +137,7 @@ class PthError(RuntimeError):
 
 
 def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=1000,
-                   max_ipl=None, j_vgpr=None):
+                   max_ipl=None, j_vgpr=None, changeover=0, ngb_factor=None):
     p = Params()
     lib().pth_params_default(C.byref(p))
     p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk = (
@@ -141,6 +146,9 @@ def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED,
         p.max_ipl = max_ipl
     if j_vgpr is not None:
         p.j_vgpr = j_vgpr
+    p.changeover = changeover
+    if ngb_factor is not None:
+        p.ngb_factor = ngb_factor
     return p
 
 
@@ -181,17 +189,24 @@ class Context:
 
 def _views():
     f = EPJ_SOFT.fields
-    ev = EpjView(EPJ_SOFT.itemsize, f["pos"][1], f["mass"][1], f["r_search"][1])
+    ev = EpjView(EPJ_SOFT.itemsize, f["pos"][1], f["mass"][1], f["r_search"][1], f["id"][1], f["r_in"][1],
+                 f["r_out"][1])
     sf = SPJ_SOFT.fields
     qf = SPJ_SOFT.fields["quad"][0].fields
     qo = sf["quad"][1]
     sv = SpjView(SPJ_SOFT.itemsize, sf["pos"][1], sf["mass"][1], qo + qf["xx"][1], qo + qf["yy"][1],
                  qo + qf["zz"][1], qo + qf["xy"][1], qo + qf["xz"][1], qo + qf["yz"][1])
     fi = EPI_SOFT.fields
-    iv = EpiView(EPI_SOFT.itemsize, fi["pos"][1], fi["r_search"][1])
-    ff = FORCE_SOFT.fields
-    fv = ForceView(FORCE_SOFT.itemsize, ff["acc"][1], ff["pot"][1], ff["n_ngb"][1], 8)
-    return ev, sv, iv, fv
+    iv = EpiView(EPI_SOFT.itemsize, fi["pos"][1], fi["r_search"][1], fi["id"][1])
+    return ev, sv, iv, force_view(FORCE_SOFT)
+
+
+def force_view(dt):
+    """View of a force record dtype with acc, pot, n_ngb (int64) and,
+    optionally, pot_tot (written in changeover mode)."""
+    ff = dt.fields
+    pt = ff["pot_tot"][1] if "pot_tot" in ff else -1
+    return ForceView(dt.itemsize, ff["acc"][1], ff["pot"][1], ff["n_ngb"][1], 8, pt)
 
 
 class CalcForceWithLinearCutoffMultiWalk:
@@ -203,16 +218,18 @@ class CalcForceWithLinearCutoffMultiWalk:
 
     _ctx = {}
 
-    def __init__(self, my_rank=0, eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0):
+    def __init__(self, my_rank=0, eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0,
+                 changeover=0):
         self.my_rank, self.eps2, self.rcut2, self.G = my_rank, eps2, rcut2, G
-        self.nnb_mode, self.exec_mode, self.chunk_j = nnb_mode, exec_mode, chunk_j
+        self.nnb_mode, self.exec_mode, self.chunk_j, self.changeover = nnb_mode, exec_mode, chunk_j, changeover
 
     def context(self) -> Context:
-        key = (self.my_rank, self.eps2, self.rcut2, self.G, self.nnb_mode, self.exec_mode, self.chunk_j)
+        key = (self.my_rank, self.eps2, self.rcut2, self.G, self.nnb_mode, self.exec_mode, self.chunk_j,
+               self.changeover)
         c = self._ctx.get(key)
         if c is None:
             c = Context(self.my_rank, eps2=self.eps2, rcut2=self.rcut2, G=self.G, nnb_mode=self.nnb_mode,
-                        exec_mode=self.exec_mode, chunk_j=self.chunk_j)
+                        exec_mode=self.exec_mode, chunk_j=self.chunk_j, changeover=self.changeover)
             self._ctx[key] = c
         return c
 
@@ -240,10 +257,13 @@ class CalcForceWithLinearCutoffMultiWalk:
 
     def retrieve(self, tag, n_walk, ni, force):
         c = self.context()
-        _, _, _, fv = _views()
+        dt = force[0].dtype if n_walk > 0 else FORCE_SOFT
+        if dt not in (FORCE_SOFT, FORCE_SOFT_CO):
+            raise TypeError("force records must be FORCE_SOFT or FORCE_SOFT_CO arrays")
+        fv = force_view(dt)
         for f in force[:n_walk]:
-            if f.dtype != FORCE_SOFT or not f.flags.c_contiguous:
-                raise TypeError("force records must be contiguous FORCE_SOFT arrays")
+            if f.dtype != dt or not f.flags.c_contiguous:
+                raise TypeError("force records must be contiguous arrays of one dtype")
         pf = (C.c_void_p * max(n_walk, 1))(*[f.ctypes.data for f in force[:n_walk]])
         n = _i32(ni)
         c.check(lib().pth_retrieve(c.handle, tag, n_walk, _ptr(n, C.c_int), pf, C.byref(fv)), "pth_retrieve")
@@ -289,6 +309,7 @@ class Resident:
         self.n_walk = k[0].size
         self.ni = int(k[0].sum())
         self._h = C.c_void_p()
+        self.co = bool(ctx.params.changeover)
         ctx.check(L.pth_resident_create(ctx.handle, C.byref(self._h), self.n_walk, _ptr(k[0], C.c_int),
                                         _ptr(k[1], C.c_int), _ptr(k[2], C.c_int), _ptr(k[3], C.c_double),
                                         _ptr(k[4], C.c_double), _ptr(k[5], C.c_int), _ptr(k[6], C.c_int),
@@ -297,12 +318,28 @@ class Resident:
                                         _ptr(k[11], C.c_double), _ptr(k[12], C.c_double)), "pth_resident_create")
         self._keep = None
 
+    def changeover(self, epi_id, epj_id, epj_rin, epj_rout):
+        """Changeover mode: ids of the EPI (concatenated per walk), ids and
+        changeover radii of the EPJ."""
+        ei = np.ascontiguousarray(epi_id, dtype=np.int64)
+        ej = np.ascontiguousarray(epj_id, dtype=np.int64)
+        ri, ro = _f64(epj_rin), _f64(epj_rout)
+        self.ctx.check(lib().pth_resident_changeover(self._h, _ptr(ei, C.c_longlong), _ptr(ej, C.c_longlong),
+                                                     _ptr(ri, C.c_double), _ptr(ro, C.c_double)),
+                       "pth_resident_changeover")
+
     @classmethod
-    def from_lists(cls, ctx, lists):
+    def from_lists(cls, ctx, lists, rin=None, rout=None):
+        """A resident evaluation of host.build_lists output; in changeover
+        mode, rin / rout are the per-particle changeover radii (the EPJ ids are
+        the particle indices)."""
         idx = np.concatenate([np.arange(b, b + n) for b, n in zip(lists.epi_begin, lists.n_epi)])
-        return cls(ctx, lists.n_epi, lists.n_epj, lists.n_spj, lists.epj_pos[idx], lists.epj_rs[idx], lists.id_epj,
-                   lists.id_spj, lists.epj_pos, lists.epj_mass, lists.epj_rs, lists.spj_pos, lists.spj_mass,
-                   lists.spj_quad)
+        r = cls(ctx, lists.n_epi, lists.n_epj, lists.n_spj, lists.epj_pos[idx], lists.epj_rs[idx], lists.id_epj,
+                lists.id_spj, lists.epj_pos, lists.epj_mass, lists.epj_rs, lists.spj_pos, lists.spj_mass,
+                lists.spj_quad)
+        if r.co:
+            r.changeover(idx, np.arange(len(lists.epj_mass)), rin, rout)
+        return r
 
     def run(self):
         self.ctx.check(lib().pth_resident_run(self._h), "pth_resident_run")
@@ -311,12 +348,15 @@ class Resident:
         self.ctx.check(lib().pth_resident_sync(self._h), "pth_resident_sync")
 
     def results(self):
+        """acc [ni, 3], pot [ni], nnb [ni] (+ pot_tot [ni] in changeover mode)."""
         acc = np.zeros((self.ni, 3))
         pot = np.zeros(self.ni)
         nnb = np.zeros(self.ni, np.int64)
+        pt = np.zeros(self.ni)
         self.ctx.check(lib().pth_resident_results(self._h, _ptr(acc, C.c_double), _ptr(pot, C.c_double),
-                                                  _ptr(nnb, C.c_longlong)), "pth_resident_results")
-        return acc, pot, nnb
+                                                  _ptr(nnb, C.c_longlong), _ptr(pt, C.c_double)),
+                       "pth_resident_results")
+        return (acc, pot, nnb, pt) if self.co else (acc, pot, nnb)
 
     def timing(self):
         a, b, c = C.c_float(), C.c_float(), C.c_float()

@@ -36,8 +36,8 @@ def test_python_binding_covers_header():
 def test_params_default():
     p = sf.Params()
     sf.lib().pth_params_default(C.byref(p))
-    assert (p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk, p.max_ipl, p.j_vgpr) == \
-        (0.0, 0.0, 1.0, 0, 0, 0, 1000, 2, 0)
+    assert (p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk, p.max_ipl, p.j_vgpr,
+            p.changeover, p.ngb_factor) == (0.0, 0.0, 1.0, 0, 0, 0, 1000, 2, 0, 0, 0.99)
 
 
 def test_create_without_device_fails_cleanly():
@@ -53,7 +53,8 @@ def test_create_without_device_fails_cleanly():
 
 def test_create_rejects_bad_params():
     h = C.c_void_p()
-    for kw in (dict(nnb_mode=3), dict(exec_mode=7), dict(rcut2=-1.0)):
+    for kw in (dict(nnb_mode=3), dict(exec_mode=7), dict(rcut2=-1.0), dict(changeover=2),
+               dict(changeover=1, exec_mode=sf.EXEC_PLAIN), dict(ngb_factor=-1.0)):
         p = sf.default_params(**kw)
         assert sf.lib().pth_create(C.byref(h), 0, C.byref(p)) == sf.PTH_ERR_ARG
     assert sf.lib().pth_create(None, 0, None) == sf.PTH_ERR_ARG

@@ -307,3 +307,91 @@ def test_changeover_correction_gives_weighted_newton():
         np.testing.assert_allclose(acc, -(1 - wacc(r, r_in, r_out)) * G * mj * d / r ** 3, rtol=1e-10, atol=1e-300)
         assert ps[0] - self_term == pytest.approx(-G * mj * (1 - wpot(r, r_in, r_out)) / r, rel=1e-10, abs=1e-15)
         assert pt[0] - self_term == pytest.approx(-G * mj / r, rel=1e-12)
+
+
+# ---- changeover mode over walks (src/hard.hpp:1404-1476,1655-1691) --------------
+
+def _co_two_body(r, rin=(0.001, 0.001), rout=(0.01, 0.01), m=(2e-4, 3e-4), rs=(0.015, 0.015), G=1.3,
+                 r_out_glb=0.01):
+    """Particles A (id 10) and B (id 20) at distance r along a random axis;
+    walk 0 holds A as i, walk 1 holds B, both lists hold A and B."""
+    rng = np.random.default_rng(int(r * 1e7) % 1000)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    xa = np.array([0.1, -0.2, 0.3])
+    pos = np.array([xa, xa + r * u])
+    acc, ps, pt, nnb, amb = O.soft_force_walks_co(
+        [1, 1], [2, 2], [0, 0], pos, np.array(rs), np.array([10, 20]), [0, 1, 0, 1], [], pos, np.array(m),
+        np.array(rs), np.array([10, 20]), np.array(rin), np.array(rout), [], [], [], 0.0, r_out_glb ** 2, G)
+    return u, acc, ps, pt, nnb
+
+
+@pytest.mark.parametrize("r", [0.0004, 0.004, 0.009, 0.012])
+def test_co_two_body_closed_forms(r):
+    """Within the neighbour radius: acc = G m (1 - W_acc) dx / r^3, pot_soft =
+    -G m (1 - W_pot) / r, pot_tot = -G m / r, no self term; W from the larger
+    r_out (A's here, B's in the mirrored pair)."""
+    wpot, wacc, _ = _w()
+    G = 1.3
+    rin, rout, m = (0.001, 0.0005), (0.01, 0.005), (2e-4, 3e-4)
+    u, acc, ps, pt, nnb = _co_two_body(r, rin=rin, rout=rout, m=m, G=G)
+    for i, j, sgn in ((0, 1, 1.0), (1, 0, -1.0)):
+        wa, wp = wacc(r, rin[0], rout[0]), wpot(r, rin[0], rout[0])  # A has the larger r_out
+        np.testing.assert_allclose(acc[i], sgn * G * m[j] * (1 - wa) * u / r ** 2, rtol=1e-12, atol=1e-300)
+        assert ps[i] == pytest.approx(-G * m[j] * (1 - wp) / r, rel=1e-12, abs=1e-300)
+        assert pt[i] == pytest.approx(-G * m[j] / r, rel=1e-12)
+        assert nnb[i] == 2
+    if r < rin[0]:
+        assert np.all(acc == 0.0) and np.all(ps == 0.0)
+
+
+def test_co_beyond_neighbour_radius_is_linear_cutoff_without_self():
+    G, r = 1.3, 0.02  # > 0.99 * 0.015
+    u, acc, ps, pt, nnb = _co_two_body(r, G=G)
+    m = (2e-4, 3e-4)
+    np.testing.assert_allclose(acc[0], G * m[1] * u / r ** 2, rtol=1e-12)
+    assert ps[0] == pytest.approx(-G * m[1] / r, rel=1e-12) and pt[0] == ps[0]
+    assert nnb[0] == 1  # self only (r > r_search)
+
+
+def test_co_soft_potential_is_potential_of_acc():
+    """-grad_i pot_soft = acc for a small cluster whose pairs are all within
+    the neighbour radius, with unequal mass-scaled changeover radii (a wrong
+    sign, a W_acc/W_pot mix-up or a wrong pair rule breaks this)."""
+    rng = np.random.default_rng(12)
+    n = 6
+    base = rng.normal(size=(n, 3)) * 0.004
+    m = rng.uniform(1e-5, 1e-3, n)
+    fac = np.maximum(np.cbrt(m / m.mean()), 1.0)
+    rin, rout = 0.002 * fac, 0.012 * fac
+    rs = 2.5 * rout
+    ids = np.arange(100, 100 + n)
+    G = 0.7
+
+    def evaluate(pos):
+        lst = np.tile(np.arange(n), n)
+        return O.soft_force_walks_co(np.ones(n, int), np.full(n, n), np.zeros(n, int), pos, rs, ids, lst, [], pos, m,
+                                     rs, ids, rin, rout, [], [], [], 0.0, 0.01 ** 2, G)
+
+    acc, ps, pt, _, _ = evaluate(base)
+    h = 1e-7
+    for i in range(n):
+        for d in range(3):
+            vals = []
+            for s in (-2, -1, 1, 2):
+                p = base.copy()
+                p[i, d] += s * h
+                vals.append(evaluate(p)[1][i])
+            grad = (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
+            assert -grad == pytest.approx(acc[i, d], rel=2e-6, abs=1e-6 * np.abs(acc[i]).max())
+    # and pot_tot is the plain Newtonian potential over the other particles
+    for i in range(n):
+        d = np.linalg.norm(base - base[i], axis=1)
+        assert pt[i] == pytest.approx(-G * np.sum(np.delete(m / np.where(d > 0, d, 1), i)), rel=1e-12)
+
+
+def test_co_missing_own_entry_is_an_error():
+    pos = np.array([[0.0, 0.0, 0.0], [0.005, 0.0, 0.0]])
+    with pytest.raises(RuntimeError):
+        O.soft_force_walks_co([1], [1], [0], pos[:1], [0.015], [7], [1], [], pos, [1e-4, 1e-4], [0.015, 0.015],
+                              [6, 8], [0.001, 0.001], [0.01, 0.01], [], [], [], 0.0, 1e-4)
