@@ -173,6 +173,60 @@ int pth_resident_stats(const pth_resident *res, long long *n_epep, long long *n_
                        long long *n_items, long long *staged_bytes);
 void pth_resident_free(pth_resident *res);
 
+/* ---- serving: many independent evaluations ("clusters") -------------------
+ * No counterpart in the reference (one simulation per process there).  A
+ * server holds any number of clusters, each one complete soft-force
+ * evaluation (the inputs of pth_resident_create); a request asks for one
+ * evaluation of one cluster.
+ *
+ * PTH_SERVE_FUSED: clusters are admitted into one HBM pool of hbm_budget
+ *   bytes when their footprint fits (idle clusters are evicted, least recently
+ *   used first, to make room; results are saved to the host on eviction), and
+ *   stay resident; a worker thread starts each admitted request as one
+ *   hipGraph launch (stage + force + reduce, captured at admission) on one of
+ *   n_streams streams, so requests that arrive while others run overlap on the
+ *   device.  A cluster runs one request at a time; requests start in arrival
+ *   order as streams free up.  Each cluster's results do not depend on which
+ *   other requests run beside it (bit-identical).
+ * PTH_SERVE_PLAIN: the plain per-walk form as PeTar would drive it per
+ *   request: all EPJ/SPJ sent, then batches of at most 200 walks dispatched
+ *   and retrieved with synchronous copies (PTH_EXEC_PLAIN), one request at a
+ *   time.  The comparison baseline.
+ * Latency is measured from submission to results ready in device memory
+ * (fused) or in host memory (plain). */
+typedef struct pth_server pth_server;
+#define PTH_SERVE_FUSED 0
+#define PTH_SERVE_PLAIN 1
+
+int pth_server_create(pth_ctx *ctx, pth_server **srv, int mode, size_t hbm_budget, int n_streams);
+/* Registers a cluster (arguments as pth_resident_create); nothing is sent to
+ * the device until its first request is admitted.  Fails with
+ * PTH_ERR_CAPACITY if its footprint exceeds the whole budget. */
+int pth_server_add_cluster(pth_server *srv, int *cluster, int n_walk, const int *n_epi, const int *n_epj,
+                           const int *n_spj, const double *epi_pos, const double *epi_rs, const int *id_epj,
+                           const int *id_spj, int n_epj_tot, const double *epj_pos, const double *epj_mass,
+                           const double *epj_rs, int n_spj_tot, const double *spj_pos, const double *spj_mass,
+                           const double *spj_quad);
+/* Device bytes a cluster takes while resident. */
+long long pth_server_footprint(const pth_server *srv, int cluster);
+/* Queues one evaluation of a cluster (returns at once). */
+int pth_server_submit(pth_server *srv, int cluster, long long *ticket);
+/* Waits for a request; latency_ms (may be NULL) from submission to done. */
+int pth_server_wait(pth_server *srv, long long ticket, double *latency_ms);
+/* Load generator: submits request k for cluster[k] at arrival_s[k] seconds
+ * after the call (non-decreasing), waits for all, and returns each latency
+ * and the wall time from the first arrival to the last completion. */
+int pth_server_replay(pth_server *srv, int n_req, const int *cluster, const double *arrival_s,
+                      double *latency_ms, double *wall_s);
+/* Results of the cluster's last completed request (waits for one in
+ * flight): acc [ni][3], pot [ni], nnb [ni]; any pointer may be NULL. */
+int pth_server_results(pth_server *srv, int cluster, double *acc, double *pot, long long *nnb);
+/* Counters: completed requests, admissions, evictions, interactions
+ * completed, peak pool bytes in use. */
+int pth_server_stats(const pth_server *srv, long long *n_done, long long *n_admit, long long *n_evict,
+                     long long *interactions, long long *pool_peak);
+void pth_server_free(pth_server *srv);
+
 /* ---- diagnostics (host only, no device needed) ----------------------------- */
 /* The work decomposition pth_dispatch / pth_resident_create use for these walk
  * sizes: returns the number of work items in launch order (or a negative

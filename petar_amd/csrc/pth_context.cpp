@@ -1,7 +1,7 @@
 // pth_context.cpp -- host side of libpetar_hip.so: contexts, the FDPS
 // multi-walk hooks (send / dispatch / retrieve) and device-resident
 // evaluations.  See include/petar_hip.h for the interface and the reference
-// functions each entry point replaces.
+// functions each entry point replaces, pth_internal.h for the shared state.
 #include <hip/hip_runtime.h>
 #include <omp.h>
 
@@ -11,281 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/petar_hip.h"
-#include "pth_device.h"
-#include "pth_plan.h"
-
-using namespace pth;
-
-namespace {
-
-constexpr int MAX_TAG = 8;
-
-// Test hook: PTH_FAIL_INJECT=alloc makes every device allocation fail, so
-// the error paths can be exercised from the host.
-bool inject_alloc_failure()
-{
-    static int v = -1;
-    if (v < 0) {
-        const char *e = std::getenv("PTH_FAIL_INJECT");
-        v = (e && std::strcmp(e, "alloc") == 0) ? 1 : 0;
-    }
-    return v == 1;
-}
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    // grows (never shrinks); the caller has synchronised the stream
-    hipError_t ensure(size_t bytes)
-    {
-        if (bytes <= cap) return hipSuccess;
-        if (inject_alloc_failure()) return hipErrorOutOfMemory;
-        if (p) {
-            hipError_t e = hipFree(p);
-            p = nullptr;
-            cap = 0;
-            if (e != hipSuccess) return e;
-        }
-        size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        cap = want;
-        return hipSuccess;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
-};
-
-struct PinBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes)
-    {
-        if (bytes <= cap) return hipSuccess;
-        if (inject_alloc_failure()) return hipErrorOutOfMemory;
-        if (p) {
-            (void)hipHostFree(p);
-            p = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
-        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        cap = want;
-        return hipSuccess;
-    }
-    void release()
-    {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
-};
-
-inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
-// Layout of one upload arena (pinned host mirror and device copy).
-struct Arena {
-    size_t epi, epi_id, walks, es, ss, ide, ids, items, ritems, total;
-    void layout(const Plan &P, long long n_ide, long long n_ids, bool co)
-    {
-        size_t o = 0;
-        epi = o; o = align256(o + sizeof(EpiDev) * (size_t)P.ni_tot);
-        epi_id = o; o = align256(o + (co ? sizeof(long long) * (size_t)P.ni_tot : 0));
-        walks = o; o = align256(o + sizeof(WalkDev) * P.walks.size());
-        es = o; o = align256(o + sizeof(long long) * P.es_off.size());
-        ss = o; o = align256(o + sizeof(long long) * P.ss_off.size());
-        ide = o; o = align256(o + sizeof(int) * (size_t)n_ide);
-        ids = o; o = align256(o + sizeof(int) * (size_t)n_ids);
-        items = o; o = align256(o + sizeof(Item) * P.items.size());
-        ritems = o; o = align256(o + sizeof(RItem) * P.ritems.size());
-        total = o;
-    }
-};
-
-// Device work buffers of one batch (staging, partials, outputs).
-struct Work {
-    DevBuf centre, near_blk, staged_i, staged_e, staged_s, part_accp, part_nnb;
-    DevBuf staged_ic, staged_il, staged_ec, part_pott, walk_err;  // changeover mode
-    hipError_t ensure(const Plan &P, bool co)
-    {
-        hipError_t e;
-        if (co) {
-            if ((e = staged_ic.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
-            if ((e = staged_il.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
-            if ((e = staged_ec.ensure(sizeof(float) * (PTH_CO_EXTRA / 8) * std::max<long long>(P.n_e_slots, 8))))
-                return e;
-            if ((e = part_pott.ensure(sizeof(double) * 64 * std::max(P.n_part, 1)))) return e;
-            if ((e = walk_err.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)))) return e;
-        }
-        if ((e = centre.ensure(sizeof(double4) * std::max<size_t>(P.walks.size(), 1)))) return e;
-        if ((e = near_blk.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)))) return e;
-        if ((e = staged_i.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
-        if ((e = staged_e.ensure(sizeof(float) * 5 * std::max<long long>(P.n_e_slots, 8)))) return e;
-        if ((e = staged_s.ensure(sizeof(float) * 12 * std::max<long long>(P.n_s_rec, 1)))) return e;
-        if ((e = part_accp.ensure(sizeof(double4) * 64 * std::max(P.n_part, 1)))) return e;
-        if ((e = part_nnb.ensure(sizeof(int) * 64 * std::max(P.n_part, 1)))) return e;
-        return hipSuccess;
-    }
-    void release()
-    {
-        centre.release(); near_blk.release(); staged_i.release(); staged_e.release(); staged_s.release();
-        part_accp.release(); part_nnb.release();
-        staged_ic.release(); staged_il.release(); staged_ec.release(); part_pott.release(); walk_err.release();
-    }
-};
-
-void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf &arena,
-                const Arena &A, Work &W, const EpjDev *epj, const SpjDev *spj, double4 *out_accp,
-                int *out_nnb, const EpjCoDev *epj_co, double *out_pott)
-{
-    b.n_walk = (int)P.walks.size();
-    b.ni_tot = P.ni_tot;
-    b.n_e_slots = P.n_e_slots;
-    b.n_s_rec = P.n_s_rec;
-    b.n_items = (int)P.items.size();
-    b.n_ritems = (int)P.ritems.size();
-    for (int c = 0; c < 3; c++) {
-        b.seg_begin[c] = P.seg_begin[c];
-        b.seg_count[c] = P.seg_count[c];
-    }
-    b.j_vgpr = par.j_vgpr ? 1 : 0;
-    b.plain = (par.exec_mode == PTH_EXEC_PLAIN);
-    b.nnb_le = (par.nnb_mode == 1);
-    b.eps2 = (float)par.eps2;
-    b.rout2 = (float)par.rcut2;
-    b.G = par.G;
-    b.co = par.changeover ? 1 : 0;
-    {
-        const double f = par.ngb_factor > 0.0 ? par.ngb_factor : 0.99;
-        b.ngb2 = (float)(f * f);
-    }
-    char *base = arena.as<char>();
-    b.epi = reinterpret_cast<const EpiDev *>(base + A.epi);
-    b.walks = reinterpret_cast<const WalkDev *>(base + A.walks);
-    b.es_off = reinterpret_cast<const long long *>(base + A.es);
-    b.ss_off = reinterpret_cast<const long long *>(base + A.ss);
-    b.id_epj = reinterpret_cast<const int *>(base + A.ide);
-    b.id_spj = reinterpret_cast<const int *>(base + A.ids);
-    b.items = reinterpret_cast<const Item *>(base + A.items);
-    b.ritems = reinterpret_cast<const RItem *>(base + A.ritems);
-    b.epj = epj;
-    b.spj = spj;
-    b.centre = W.centre.as<double4>();
-    b.near_blk = W.near_blk.as<int>();
-    b.staged_i = W.staged_i.as<float4>();
-    b.staged_e = W.staged_e.as<float>();
-    b.staged_s = W.staged_s.as<float>();
-    b.part_accp = W.part_accp.as<double4>();
-    b.part_nnb = W.part_nnb.as<int>();
-    b.out_accp = out_accp;
-    b.out_nnb = out_nnb;
-    if (b.co) {
-        b.epi_id = reinterpret_cast<const long long *>(base + A.epi_id);
-        b.epj_co = epj_co;
-        b.staged_ic = W.staged_ic.as<float4>();
-        b.staged_il = W.staged_il.as<float4>();
-        b.staged_ec = W.staged_ec.as<float>();
-        b.walk_err = W.walk_err.as<int>();
-        b.part_pott = W.part_pott.as<double>();
-        b.out_pott = out_pott;
-    }
-}
-
-// Checks of the changeover-mode preconditions on a plan.
-const char *co_plan_error(const Plan &P)
-{
-    for (const WalkDev &w : P.walks)
-        if (w.n_i > PTH_CO_MAX_NI) return "changeover mode: a walk has more than 1024 i-particles";
-    if (P.n_e_slots >= (1ll << 31)) return "changeover mode: staged EPJ slots exceed the int32 range";
-    return nullptr;
-}
-
-void copy_plan_to_arena(char *h, const Plan &P, const Arena &A)
-{
-    std::memcpy(h + A.walks, P.walks.data(), sizeof(WalkDev) * P.walks.size());
-    std::memcpy(h + A.es, P.es_off.data(), sizeof(long long) * P.es_off.size());
-    std::memcpy(h + A.ss, P.ss_off.data(), sizeof(long long) * P.ss_off.size());
-    if (!P.items.empty()) std::memcpy(h + A.items, P.items.data(), sizeof(Item) * P.items.size());
-    if (!P.ritems.empty()) std::memcpy(h + A.ritems, P.ritems.data(), sizeof(RItem) * P.ritems.size());
-}
-
-}  // namespace
-
-struct TagState {
-    DevBuf out_accp, out_nnb, out_pott;
-    PinBuf h_accp, h_nnb, h_pott, h_err;
-    bool co = false;
-    std::vector<int> n_i;
-    long long ni_tot = 0;
-    bool pending = false;
-    hipEvent_t done = nullptr;
-};
-
-struct pth_ctx {
-    int device = -1;
-    hipStream_t stream = nullptr;
-    pth_params par{};
-    std::string err;
-    // resident j arrays of the current force call
-    DevBuf d_epj, d_spj, d_epj_co;
-    PinBuf h_epj, h_spj, h_epj_co;
-    int n_epj = 0, n_spj = 0;
-    bool j_sent = false;
-    bool co_sent = false;  // the last send carried the changeover data
-    hipEvent_t ev_send = nullptr;
-    // batch upload
-    PinBuf h_arena;
-    DevBuf d_arena;
-    hipEvent_t ev_upload = nullptr;
-    Work work;
-    TagState tag[MAX_TAG];
-
-    int fail(int code, const std::string &msg)
-    {
-        err = msg;
-        return code;
-    }
-    int hip_fail(hipError_t e, const char *what)
-    {
-        err = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
-        return PTH_ERR_HIP;
-    }
-};
-
-struct pth_resident {
-    pth_ctx *ctx = nullptr;
-    Plan plan;
-    Arena A{};
-    DevBuf arena, epj, spj, out_accp, out_nnb;
-    DevBuf epj_co, out_pott;  // changeover mode
-    bool co = false, co_ready = false;
-    int n_epj_tot = 0;
-    Work work;
-    BatchDev b;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool ran = false;
-    long long staged_bytes = 0;
-};
-
-#define HIPCHK(ctx, call, what)                      \
-    do {                                             \
-        hipError_t e_ = (call);                      \
-        if (e_ != hipSuccess) return (ctx)->hip_fail(e_, what); \
-    } while (0)
+#include "pth_internal.h"
 
 extern "C" {
 

@@ -77,7 +77,9 @@ class ForceView(C.Structure):
 EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_error", "pth_destroy", "pth_last_error",
             "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve", "pth_resident_create",
             "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
-            "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug")
+            "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
+            "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
+            "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
 
 
 def lib():
@@ -112,6 +114,17 @@ def lib():
     L.pth_resident_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pth_resident_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 4
     L.pth_resident_free.argtypes = [vp]
+    L.pth_server_create.argtypes = [vp, C.POINTER(vp), C.c_int, C.c_size_t, C.c_int]
+    L.pth_server_add_cluster.argtypes = [vp, ip, C.c_int, ip, ip, ip, dp, dp, ip, ip, C.c_int, dp, dp, dp, C.c_int,
+                                         dp, dp, dp]
+    L.pth_server_footprint.argtypes = [vp, C.c_int]
+    L.pth_server_footprint.restype = C.c_longlong
+    L.pth_server_submit.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
+    L.pth_server_wait.argtypes = [vp, C.c_longlong, dp]
+    L.pth_server_replay.argtypes = [vp, C.c_int, ip, dp, dp, dp]
+    L.pth_server_results.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_longlong)]
+    L.pth_server_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 5
+    L.pth_server_free.argtypes = [vp]
     L.pth_plan_debug.restype = C.c_longlong
     L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
                                  C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip, C.c_longlong,
@@ -371,6 +384,105 @@ class Resident:
     def close(self):
         if self._h:
             lib().pth_resident_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+SERVE_FUSED, SERVE_PLAIN = 0, 1
+
+
+class Server:
+    """Many independent evaluations behind one queue (pth_server_*).
+
+    mode SERVE_FUSED keeps admitted clusters resident in an HBM pool of
+    hbm_budget bytes and runs each request as one graph launch on one of
+    n_streams streams; SERVE_PLAIN replays the plain per-walk form per
+    request.  Clusters are host.Lists objects (or the arguments of Resident)."""
+
+    def __init__(self, ctx: Context, mode=SERVE_FUSED, hbm_budget=0, n_streams=4):
+        self.ctx = ctx
+        self.mode = mode
+        self._h = C.c_void_p()
+        self._ni = []
+        self._nint = []
+        ctx.check(lib().pth_server_create(ctx.handle, C.byref(self._h), mode, int(hbm_budget), n_streams),
+                  "pth_server_create")
+
+    def add(self, n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos, epj_mass, epj_rs, spj_pos,
+            spj_mass, spj_quad):
+        k = [_i32(n_epi), _i32(n_epj), _i32(n_spj), _f64(epi_pos).reshape(-1, 3), _f64(epi_rs),
+             _i32(id_epj) if len(id_epj) else np.zeros(1, np.int32),
+             _i32(id_spj) if len(id_spj) else np.zeros(1, np.int32),
+             _f64(epj_pos).reshape(-1, 3) if len(epj_pos) else np.zeros((1, 3)),
+             _f64(epj_mass) if len(epj_mass) else np.zeros(1), _f64(epj_rs) if len(epj_rs) else np.zeros(1),
+             _f64(spj_pos).reshape(-1, 3) if len(spj_pos) else np.zeros((1, 3)),
+             _f64(spj_mass) if len(spj_mass) else np.zeros(1),
+             _f64(spj_quad).reshape(-1, 6) if len(spj_quad) else np.zeros((1, 6))]
+        cid = C.c_int()
+        self.ctx.check(lib().pth_server_add_cluster(
+            self._h, C.byref(cid), k[0].size, _ptr(k[0], C.c_int), _ptr(k[1], C.c_int), _ptr(k[2], C.c_int),
+            _ptr(k[3], C.c_double), _ptr(k[4], C.c_double), _ptr(k[5], C.c_int), _ptr(k[6], C.c_int), len(epj_mass),
+            _ptr(k[7], C.c_double), _ptr(k[8], C.c_double), _ptr(k[9], C.c_double), len(spj_mass),
+            _ptr(k[10], C.c_double), _ptr(k[11], C.c_double), _ptr(k[12], C.c_double)), "pth_server_add_cluster")
+        self._ni.append(int(k[0].sum()))
+        self._nint.append(int(np.dot(k[0].astype(np.int64), (k[1] + k[2]).astype(np.int64))))
+        return cid.value
+
+    def add_lists(self, lists):
+        idx = np.concatenate([np.arange(b, b + n) for b, n in zip(lists.epi_begin, lists.n_epi)])
+        return self.add(lists.n_epi, lists.n_epj, lists.n_spj, lists.epj_pos[idx], lists.epj_rs[idx],
+                        lists.id_epj, lists.id_spj, lists.epj_pos, lists.epj_mass, lists.epj_rs, lists.spj_pos,
+                        lists.spj_mass, lists.spj_quad)
+
+    def interactions(self, cluster):
+        return self._nint[cluster]
+
+    def footprint(self, cluster):
+        return int(lib().pth_server_footprint(self._h, cluster))
+
+    def submit(self, cluster):
+        t = C.c_longlong()
+        self.ctx.check(lib().pth_server_submit(self._h, cluster, C.byref(t)), "pth_server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        lat = np.zeros(1)
+        self.ctx.check(lib().pth_server_wait(self._h, ticket, _ptr(lat, C.c_double)), "pth_server_wait")
+        return float(lat[0])
+
+    def replay(self, clusters, arrival_s):
+        """Submits clusters[k] at arrival_s[k] seconds; returns the latencies
+        (ms) and the wall time (s) from the first arrival to the last
+        completion."""
+        n = len(clusters)
+        cl = _i32(clusters) if n else np.zeros(1, np.int32)
+        ar = _f64(arrival_s) if n else np.zeros(1)
+        lat = np.zeros(max(n, 1))
+        wall = np.zeros(1)
+        self.ctx.check(lib().pth_server_replay(self._h, n, _ptr(cl, C.c_int), _ptr(ar, C.c_double),
+                                               _ptr(lat, C.c_double), _ptr(wall, C.c_double)), "pth_server_replay")
+        return lat[:n], float(wall[0])
+
+    def results(self, cluster):
+        n = self._ni[cluster]
+        acc, pot, nnb = np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.int64)
+        self.ctx.check(lib().pth_server_results(self._h, cluster, _ptr(acc, C.c_double), _ptr(pot, C.c_double),
+                                                _ptr(nnb, C.c_longlong)), "pth_server_results")
+        return acc, pot, nnb
+
+    def stats(self):
+        v = [C.c_longlong() for _ in range(5)]
+        self.ctx.check(lib().pth_server_stats(self._h, *[C.byref(x) for x in v]), "pth_server_stats")
+        return dict(zip(("done", "admitted", "evicted", "interactions", "pool_peak"), (x.value for x in v)))
+
+    def close(self):
+        if self._h:
+            lib().pth_server_free(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
