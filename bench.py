@@ -63,6 +63,77 @@ def build_workload(name, nthreads=0):
     return L, r_out, desc, dict(ic_s=round(t1 - t0, 2), tree_walk_s=round(t2 - t1, 2))
 
 
+def build_serving(n_clusters=256, seed=4, f_bin=0.3, r_out=0.01, nthreads=0):
+    """The serving stream: n_clusters independent Plummer clusters, N
+    log-uniform in [1e3, 2e5], Kroupa masses, 30% primordial binaries, each
+    with its own tree (theta 0.3, quadrupole SPJ)."""
+    from petar_amd import host
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_clusters)).astype(int)
+    t0 = time.time()
+    out = []
+    for k, n in enumerate(sizes):
+        m, x, _ = host.cluster(int(n), "plummer", kroupa=True, f_bin=f_bin, seed=seed * 1000 + k)
+        out.append(host.build_lists(x, m, 1.5 * r_out, theta=0.3, nthreads=nthreads))
+    log(f"[bench] serving stream: {n_clusters} clusters, N sum {int(sizes.sum())}, built in {time.time() - t0:.1f}s")
+    return out, sizes
+
+
+def serving(r_out, n_streams=4, loads=(0.5, 0.8, 0.95), seed=5, accuracy_clusters=6):
+    """Fused server vs plain-form server on the 256-cluster stream."""
+    from petar_amd import soft_force as sf
+    from oracle import oracle_py as O
+    lists, sizes = build_serving(r_out=r_out)
+    ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0)
+    srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=0, n_streams=n_streams)
+    ids = [srv.add_lists(L) for L in lists]
+    ints = np.array([srv.interactions(k) for k in ids], dtype=np.int64)
+    rng = np.random.default_rng(seed)
+    # first request of each cluster: admission (upload + graph capture)
+    lat_first, _ = srv.replay(ids, np.zeros(len(ids)))
+    alone = {k: srv.results(k) for k in ids[:accuracy_clusters]}
+    # saturated: every cluster once, all arriving at t = 0
+    order = rng.permutation(ids)
+    _, wall_sat = srv.replay(order, np.zeros(len(order)))
+    cap_req_s = len(order) / wall_sat
+    sat_gint = ints.sum() / wall_sat / 1e9
+    sweep = []
+    for f in loads:
+        seq = rng.permutation(np.repeat(ids, 2))
+        arr = np.cumsum(rng.exponential(1.0 / (f * cap_req_s), seq.size))
+        arr -= arr[0]
+        lat, wall = srv.replay(seq, arr)
+        sweep.append(dict(load=f, rate_req_s=round(f * cap_req_s, 1), p50_ms=round(float(np.percentile(lat, 50)), 4),
+                          p99_ms=round(float(np.percentile(lat, 99)), 4),
+                          gint_s=round(float(ints[seq].sum() / wall / 1e9), 2)))
+        log(f"[bench] serving load {f}: {sweep[-1]}")
+    same = sum(all(np.array_equal(a, b) for a, b in zip(srv.results(k), alone[k])) for k in alone)
+    st = srv.stats()
+    srv.close()
+    # accuracy of a few clusters against the FP64 oracle (same lists)
+    errs = []
+    for k in list(alone)[:accuracy_clusters]:
+        ref = O.soft_force_walks(**O.lists_args(lists[k]), eps2=0.0, rout2=r_out ** 2)
+        errs.append(np.linalg.norm(alone[k][0] - ref[0], axis=1) / np.linalg.norm(ref[0], axis=1))
+    errs = np.concatenate(errs)
+    # plain per-walk form, same stream, saturated
+    srvp = sf.Server(ctx, sf.SERVE_PLAIN)
+    idp = [srvp.add_lists(L) for L in lists]
+    _, wall_p = srvp.replay(rng.permutation(idp), np.zeros(len(idp)))
+    srvp.close()
+    ctx.close()
+    plain_gint = ints.sum() / wall_p / 1e9
+    return dict(clusters=len(ids), particles=int(sizes.sum()), interactions_per_pass=int(ints.sum()),
+                n_streams=n_streams, saturated_gint_s=round(float(sat_gint), 2),
+                saturated_req_s=round(float(cap_req_s), 1), load_sweep=sweep,
+                first_request_ms_p50=round(float(np.percentile(lat_first, 50)), 3),
+                plain_form_saturated_gint_s=round(float(plain_gint), 2),
+                speedup_vs_plain=round(float(sat_gint / plain_gint), 2),
+                alone_vs_mix_bit_identical=f"{same}/{len(alone)}",
+                acc_rel_err_median=float(np.median(errs)), acc_rel_err_p99=float(np.percentile(errs, 99)),
+                pool_peak_bytes=st["pool_peak"], evictions=st["evicted"])
+
+
 def read_traffic(name):
     """HBM bytes per force-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
@@ -138,6 +209,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--streams", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the soft-force path is "
@@ -225,6 +298,13 @@ def main():
         rp.close()
         ctxp.close()
     ctx.close()
+    if not a.no_serving:
+        out["serving"] = serving(r_out, n_streams=a.streams)
+        sv = out["serving"]
+        p80 = [x for x in sv["load_sweep"] if x["load"] == 0.8]
+        if p80:
+            out["latency_ms"]["serving_p50_at_80pct_load"] = p80[0]["p50_ms"]
+            out["latency_ms"]["serving_p99_at_80pct_load"] = p80[0]["p99_ms"]
     if not a.no_cpu:
         from oracle import oracle_py as O
         out["cpu_baseline"] = cpu_baseline(O.lists_args(L), r_out, a.cpu_seconds)
