@@ -210,6 +210,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--streams", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -228,6 +229,12 @@ def main():
     ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=a.chunk_j,
                      max_ipl=a.max_ipl, j_vgpr=a.j_vgpr)
     res = sf.Resident.from_lists(ctx, L)
+    tuned = None
+    if a.exec == "fused" and not a.no_autotune:
+        t0 = time.time()
+        ipl, cj, tms = res.autotune(ipl=(1, 2, 4), chunk=(1024, 2048, 4096), reps=3)
+        tuned = dict(max_ipl=ipl, chunk_j=cj, ms=round(tms, 4), tuning_s=round(time.time() - t0, 2))
+        log(f"[bench] autotuned: {tuned}")
     st = res.stats()
     log(f"[bench] resident: N_e {ne:.3e} N_s {ns:.3e} items {st['n_items']} staged {st['staged_bytes'] / 2**20:.0f} MiB")
     for _ in range(a.warmup):
@@ -279,6 +286,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "stage_ms": round(float(np.mean(k_stage)), 4), "reduce_ms": round(float(np.mean(k_red)), 4)},
         "prep": prep,
+        "autotuned": tuned,
     }
     if not a.no_accuracy:
         acc, pot, nnb = res.results()

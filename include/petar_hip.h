@@ -25,6 +25,14 @@
  *   pth_last_error
  *       no counterpart (the reference asserts and ignores CUDA errors,
  *       src/cuda_pointer.h:7); text of the last failure of a context.
+ *   pth_resident_*, pth_server_*, pth_plan_debug
+ *       no counterpart: a force call kept resident in HBM, the serving loop
+ *       over many independent evaluations, and a host-only view of the work
+ *       decomposition.
+ *
+ * pth_params.changeover = 1 also folds PeTar's changeover correction
+ * (src/hard.hpp:1404-1476,1655-1691, src/changeover.hpp:294-375) into the
+ * kernel; see below.
  *
  * Records are read and written in place through "views" (stride and byte
  * offsets of the members FDPS's EPISoft / EPJSoft / SPJSoft / ForceSoft carry,
@@ -70,7 +78,7 @@ typedef struct pth_params {
     int chunk_j;   /* fused form: max j-entries per work item (0 = default) */
     int max_walk;  /* largest n_walk accepted by pth_dispatch (0 = 1000, the
                       reference's N_WALK_LIMIT, src/force_gpu_cuda.cu:14) */
-    int max_ipl;   /* fused form: most i-particles per lane, 1, 2 or 4 (0 = 4) */
+    int max_ipl;   /* fused form: most i-particles per lane, 1, 2 or 4 (other values: 4) */
     int j_vgpr;    /* fused form: 1 copies each j record into VGPRs before use,
                       0 uses it as SGPR operands */
     int changeover; /* 0: linear cutoff only (the reference GPU kernels);
@@ -99,9 +107,10 @@ void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */
 int pth_create(pth_ctx **ctx, int rank, const pth_params *params);
-/* Changes eps2, rcut2, G and nnb_mode of a context between force calls (the
- * reference functor receives them at every construction,
- * src/force_gpu_cuda.hpp:108-118); fails while a batch awaits retrieval. */
+/* Changes eps2, rcut2, G, nnb_mode, changeover and ngb_factor of a context
+ * between force calls (the reference functor receives its parameters at every
+ * construction, src/force_gpu_cuda.hpp:108-118); fails while a batch awaits
+ * retrieval.  Turning changeover on takes effect from the next pth_send_j. */
 int pth_set_params(pth_ctx *ctx, const pth_params *params);
 /* Text of the last pth_create failure of the calling thread. */
 const char *pth_create_error(void);
@@ -168,6 +177,14 @@ int pth_resident_results(pth_resident *res, double *acc, double *pot, long long 
 /* Per-kernel GPU times of the last evaluation (ms), from HIP events on the
  * context's stream: stage (centres + gather), force, reduce. */
 int pth_resident_timing(pth_resident *res, float *ms_stage, float *ms_force, float *ms_reduce);
+/* Re-plans the work decomposition (fused form) with another i-per-lane
+ * bound and chunk size; the uploaded inputs stay. */
+int pth_resident_replan(pth_resident *res, int max_ipl, int chunk_j);
+/* Autotuner: times (median of reps evaluations, HIP events) every pair of
+ * candidate max_ipl x chunk_j on this evaluation and keeps the fastest plan;
+ * returns the choice and its time. */
+int pth_resident_autotune(pth_resident *res, const int *ipl, int n_ipl, const int *chunk, int n_chunk,
+                          int reps, int *best_ipl, int *best_chunk, float *best_ms);
 /* Sizes: interactions N_e, N_s; work items; staged bytes. */
 int pth_resident_stats(const pth_resident *res, long long *n_epep, long long *n_epsp,
                        long long *n_items, long long *staged_bytes);

@@ -539,6 +539,8 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
                r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(),
                r->out_pott.as<double>());
     r->n_epj_tot = n_epj_tot;
+    r->max_ipl = c->par.max_ipl;
+    r->chunk_j = c->par.chunk_j;
     r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 48 * P.n_s_rec);
     *out = r;
     return PTH_OK;
@@ -652,6 +654,82 @@ int pth_resident_timing(pth_resident *r, float *ms_stage, float *ms_force, float
     return PTH_OK;
 }
 
+int pth_resident_replan(pth_resident *r, int max_ipl, int chunk_j)
+{
+    if (!r) return PTH_ERR_ARG;
+    pth_ctx *c = r->ctx;
+    if (c->par.exec_mode != PTH_EXEC_FUSED) return c->fail(PTH_ERR_STATE, "pth_resident_replan: fused form only");
+    const Plan &P0 = r->plan;
+    const int nw = (int)P0.walks.size();
+    std::vector<int> ne(nw), nj(nw), ns(nw);
+    for (int w = 0; w < nw; w++) {
+        ne[w] = P0.walks[w].n_i;
+        nj[w] = P0.walks[w].n_e;
+        ns[w] = P0.walks[w].n_s;
+    }
+    Plan P;
+    if (!make_plan(nw, ne.data(), nj.data(), ns.data(), PTH_EXEC_FUSED, chunk_j, max_ipl, P))
+        return c->fail(PTH_ERR_CAPACITY, "pth_resident_replan: sizes exceed the int32 index range");
+    // the staged layout depends on the walk sizes only
+    if (P.es_off != P0.es_off || P.ss_off != P0.ss_off || P.ni_tot != P0.ni_tot)
+        return c->fail(PTH_ERR_STATE, "pth_resident_replan: staged layout changed");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const size_t bi = sizeof(Item) * P.items.size(), br = sizeof(RItem) * P.ritems.size();
+    const size_t off_r = align256(bi);
+    HIPCHK(c, r->plan_dev.ensure(std::max<size_t>(off_r + br, 256)), "plan allocation");
+    HIPCHK(c, r->work.ensure(P, r->co), "resident work buffers");
+    if (bi) HIPCHK(c, hipMemcpy(r->plan_dev.p, P.items.data(), bi, hipMemcpyHostToDevice), "plan upload");
+    if (br) HIPCHK(c, hipMemcpy(r->plan_dev.as<char>() + off_r, P.ritems.data(), br, hipMemcpyHostToDevice),
+                   "plan upload");
+    fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
+               r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(), r->out_pott.as<double>());
+    r->b.items = r->plan_dev.as<const Item>();
+    r->b.ritems = reinterpret_cast<const RItem *>(r->plan_dev.as<char>() + off_r);
+    r->plan = std::move(P);
+    r->max_ipl = max_ipl;
+    r->chunk_j = chunk_j;
+    return PTH_OK;
+}
+
+int pth_resident_autotune(pth_resident *r, const int *ipl, int n_ipl, const int *chunk, int n_chunk, int reps,
+                          int *best_ipl, int *best_chunk, float *best_ms)
+{
+    if (!r) return PTH_ERR_ARG;
+    pth_ctx *c = r->ctx;
+    if (n_ipl <= 0 || n_chunk <= 0 || !ipl || !chunk || reps <= 0)
+        return c->fail(PTH_ERR_ARG, "pth_resident_autotune: empty candidate list");
+    if (r->co && !r->co_ready)
+        return c->fail(PTH_ERR_STATE, "pth_resident_autotune: changeover mode needs pth_resident_changeover first");
+    float bt = 1e30f;
+    int bi = ipl[0], bc = chunk[0];
+    for (int a = 0; a < n_ipl; a++)
+        for (int k = 0; k < n_chunk; k++) {
+            int rc = pth_resident_replan(r, ipl[a], chunk[k]);
+            if (rc) return rc;
+            if ((rc = pth_resident_run(r))) return rc;  // warm-up
+            std::vector<float> t(reps);
+            for (int q = 0; q < reps; q++) {
+                if ((rc = pth_resident_run(r))) return rc;
+                float s = 0, f = 0, d = 0;
+                if ((rc = pth_resident_timing(r, &s, &f, &d))) return rc;
+                t[q] = s + f + d;
+            }
+            std::nth_element(t.begin(), t.begin() + reps / 2, t.end());
+            if (t[reps / 2] < bt) {
+                bt = t[reps / 2];
+                bi = ipl[a];
+                bc = chunk[k];
+            }
+        }
+    const int rc = pth_resident_replan(r, bi, bc);
+    if (rc) return rc;
+    if (best_ipl) *best_ipl = bi;
+    if (best_chunk) *best_chunk = bc;
+    if (best_ms) *best_ms = bt;
+    return PTH_OK;
+}
+
 int pth_resident_stats(const pth_resident *r, long long *n_epep, long long *n_epsp, long long *n_items,
                        long long *staged_bytes)
 {
@@ -677,6 +755,7 @@ void pth_resident_free(pth_resident *r)
     r->out_nnb.release();
     r->epj_co.release();
     r->out_pott.release();
+    r->plan_dev.release();
     r->work.release();
     for (int k = 0; k < 4; k++)
         if (r->ev[k]) (void)hipEventDestroy(r->ev[k]);

@@ -274,6 +274,8 @@ struct pth_resident {
     DevBuf epj_co, out_pott;  // changeover mode
     bool co = false, co_ready = false;
     int n_epj_tot = 0;
+    DevBuf plan_dev;  // work items and reduction items after a re-plan
+    int max_ipl = 0, chunk_j = 0;
     Work work;
     BatchDev b;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -78,6 +78,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_er
             "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve", "pth_resident_create",
             "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
             "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
+            "pth_resident_replan", "pth_resident_autotune",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
 
@@ -125,6 +126,8 @@ def lib():
     L.pth_server_results.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_longlong)]
     L.pth_server_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 5
     L.pth_server_free.argtypes = [vp]
+    L.pth_resident_replan.argtypes = [vp, C.c_int, C.c_int]
+    L.pth_resident_autotune.argtypes = [vp, ip, C.c_int, ip, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_float)]
     L.pth_plan_debug.restype = C.c_longlong
     L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
                                  C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip, C.c_longlong,
@@ -370,6 +373,18 @@ class Resident:
                                                   _ptr(nnb, C.c_longlong), _ptr(pt, C.c_double)),
                        "pth_resident_results")
         return (acc, pot, nnb, pt) if self.co else (acc, pot, nnb)
+
+    def replan(self, max_ipl, chunk_j):
+        self.ctx.check(lib().pth_resident_replan(self._h, max_ipl, chunk_j), "pth_resident_replan")
+
+    def autotune(self, ipl=(1, 2, 4), chunk=(1024, 2048, 4096), reps=3):
+        """Times every (max_ipl, chunk_j) pair on this evaluation, keeps the
+        fastest; returns (max_ipl, chunk_j, ms)."""
+        a, b = _i32(ipl), _i32(chunk)
+        bi, bc, bt = C.c_int(), C.c_int(), C.c_float()
+        self.ctx.check(lib().pth_resident_autotune(self._h, _ptr(a, C.c_int), a.size, _ptr(b, C.c_int), b.size, reps,
+                                                   C.byref(bi), C.byref(bc), C.byref(bt)), "pth_resident_autotune")
+        return bi.value, bc.value, bt.value
 
     def timing(self):
         a, b, c = C.c_float(), C.c_float(), C.c_float()

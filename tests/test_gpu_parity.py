@@ -241,3 +241,28 @@ def test_error_paths():
         fn(0, 1001, [np.zeros(1, sf.EPI_SOFT)] * 1001, [1] * 1001, [np.zeros(1, np.int32)] * 1001, [1] * 1001,
            [np.zeros(0, np.int32)] * 1001, [0] * 1001, epj, 4, spj, 0, False)
     ctx.close()
+
+
+def test_replan_and_autotune(c1):
+    """A re-planned resident evaluation equals one created with the same
+    tiling parameters; the autotuner keeps one of the candidates and the
+    result stays within the parity bar."""
+    L, a, ref, amb = c1
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    r = sf.Resident.from_lists(ctx, L)
+    r.replan(1, 512)
+    r.run()
+    res = r.results()
+    ctx2 = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, max_ipl=1, chunk_j=512)
+    r2 = sf.Resident.from_lists(ctx2, L)
+    r2.run()
+    assert all(np.array_equal(x, y) for x, y in zip(res, r2.results()))
+    r2.close()
+    ctx2.close()
+    ipl, cj, ms = r.autotune(ipl=(1, 2, 4), chunk=(512, 2048), reps=2)
+    print(f"[autotune c1] max_ipl {ipl} chunk_j {cj}: {ms:.3f} ms", flush=True)
+    assert ipl in (1, 2, 4) and cj in (512, 2048) and ms > 0
+    r.run()
+    _check(r.results(), ref, amb, "autotuned")
+    r.close()
+    ctx.close()
