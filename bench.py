@@ -63,6 +63,23 @@ def build_workload(name, nthreads=0):
     return L, r_out, desc, dict(ic_s=round(t1 - t0, 2), tree_walk_s=round(t2 - t1, 2))
 
 
+def tree_error(res_acc, res_pot, L, r_out, n_sample, seed=1):
+    """Error of the tree force (theta truncation + kernel) against the FP64
+    all-pairs direct sum with the same linear cutoff, on sampled i
+    (informational: the tree's own approximation error)."""
+    from oracle import oracle_py as O
+    rng = np.random.default_rng(seed)
+    ni = res_acc.shape[0]
+    rows = np.sort(rng.choice(ni, min(n_sample, ni), replace=False))
+    idx = np.concatenate([np.arange(b, b + k) for b, k in zip(L.epi_begin, L.n_epi)])[rows]
+    t0 = time.time()
+    dacc, dpot, _ = O.direct_linear_cutoff(L.epj_pos, L.epj_mass, L.epj_rs, sample=idx, eps2=0.0, rout2=r_out ** 2)
+    ea = np.linalg.norm(res_acc[rows] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+    ep = np.abs(res_pot[rows] - dpot) / np.abs(dpot)
+    return dict(sample_i=int(rows.size), acc_rel_median=float(np.median(ea)), acc_rel_p99=float(np.percentile(ea, 99)),
+                pot_rel_median=float(np.median(ep)), direct_sum_s=round(time.time() - t0, 1))
+
+
 def build_serving(n_clusters=256, seed=4, f_bin=0.3, r_out=0.01, nthreads=0):
     """The serving stream: n_clusters independent Plummer clusters, N
     log-uniform in [1e3, 2e5], Kroupa masses, 30% primordial binaries, each
@@ -211,6 +228,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--direct-sample", type=int, default=4096,
+                    help="i sampled for the error against the FP64 all-pairs direct sum (0: skip)")
     ap.add_argument("--streams", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -291,6 +310,8 @@ def main():
     if not a.no_accuracy:
         acc, pot, nnb = res.results()
         out["accuracy_vs_fp64_same_lists"] = accuracy(acc, pot, nnb, L, r_out)
+        if a.direct_sample > 0:
+            out["error_vs_fp64_direct_sum"] = tree_error(acc, pot, L, r_out, a.direct_sample)
     res.close()
     if not a.no_plain and a.exec == "fused":
         ctxp = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, exec_mode=sf.EXEC_PLAIN)

@@ -9,7 +9,14 @@
 // program writes the inputs it used and the retrieved forces to a binary
 // file that tests/test_gpu_dropin.py checks against the FP64 oracle.
 //
+// Built with -DPETAR_HIP_CHANGEOVER (fdps_replay_co) it runs the adapter in
+// changeover mode: changeover radii and r_search scale with
+// max((m/<m>)^(1/3), 1) per particle (src/changeover.hpp:44-59) and the
+// retrieved ForceSoft carries pot_tot.
+//
 // usage: fdps_replay N seed theta n_walk_limit r_out out_file
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -46,7 +53,16 @@ int main(int argc, char **argv)
 
     std::vector<double> mass(n), pos(3 * n);
     if (pth_ic_cluster(n, 0, 0.0, 1, 0.3, 1e-5, 1e-2, 0, seed, mass.data(), pos.data(), nullptr)) die("ic");
-    std::vector<double> rs(n, r_search);
+#ifdef PETAR_HIP_CHANGEOVER
+    double mbar = 0.0;
+    for (double m : mass) mbar += m;
+    mbar /= (double)n;
+    auto fac = [&](double m) { return std::max(std::cbrt(m / mbar), 1.0); };
+#else
+    auto fac = [](double) { return 1.0; };
+#endif
+    std::vector<double> rs(n);
+    for (long i = 0; i < n; i++) rs[i] = r_search * fac(mass[i]);
     pth_tree *tree = nullptr;
     if (pth_tree_build(&tree, n, pos.data(), mass.data(), rs.data(), 20, 512, 0)) die("tree");
     if (pth_tree_walk(tree, theta)) die("walk");
@@ -64,20 +80,24 @@ int main(int argc, char **argv)
 
     // FDPS-side records (tree order)
     std::vector<EPJSoft> epj(n);
-    std::vector<double> epj_pos(3 * n), epj_mass(n), epj_rs(n);
+    std::vector<double> epj_pos(3 * n), epj_mass(n), epj_rs(n), epj_rin(n), epj_rout(n);
+    std::vector<long long> epj_id(n);
     for (long k = 0; k < n; k++) {
         const long i = order[k];
-        std::memset(&epj[k], 0, sizeof(EPJSoft));
+        epj[k] = EPJSoft{};
         epj[k].id = i + 1;
         epj[k].mass = mass[i];
         epj[k].pos = PS::F64vec(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-        epj[k].r_in = 0.1 * r_out;
-        epj[k].r_out = r_out;
-        epj[k].r_search = r_search;
+        epj[k].r_in = 0.1 * r_out * fac(mass[i]);
+        epj[k].r_out = r_out * fac(mass[i]);
+        epj[k].r_search = rs[i];
         epj[k].r_scale_next = 1.0;
         for (int d = 0; d < 3; d++) epj_pos[3 * k + d] = pos[3 * i + d];
         epj_mass[k] = mass[i];
-        epj_rs[k] = r_search;
+        epj_rs[k] = rs[i];
+        epj_rin[k] = epj[k].r_in;
+        epj_rout[k] = epj[k].r_out;
+        epj_id[k] = epj[k].id;
     }
     std::vector<PS::SPJQuadrupoleInAndOut> spj(n_node);
     for (long s = 0; s < n_node; s++) {
@@ -102,6 +122,9 @@ int main(int argc, char **argv)
             epi[g][i].rank_org = 0;
             epi[g][i].type = 1;
             force[g][i].pot = 99.0;  // retrieve must overwrite
+#ifdef PETAR_HIP_CHANGEOVER
+            force[g][i].pot_tot = 99.0;
+#endif
         }
     }
 
@@ -127,7 +150,7 @@ int main(int argc, char **argv)
         RetrieveForceCUDA(0, nw, n_epi.data() + g0, pf.data());
     }
 
-    std::vector<double> acc, pot;
+    std::vector<double> acc, pot, pot_tot;
     std::vector<long long> nngb;
     for (long g = 0; g < n_group; g++)
         for (auto &fr : force[g]) {
@@ -136,6 +159,11 @@ int main(int argc, char **argv)
             acc.push_back(fr.acc.z);
             pot.push_back(fr.pot);
             nngb.push_back(fr.n_ngb);
+#ifdef PETAR_HIP_CHANGEOVER
+            pot_tot.push_back(fr.pot_tot);
+#else
+            pot_tot.push_back(fr.pot);
+#endif
         }
     FILE *out = std::fopen(argv[6], "wb");
     if (!out) die("open output");
@@ -156,6 +184,10 @@ int main(int argc, char **argv)
     put(out, acc);
     put(out, pot);
     put(out, nngb);
+    put(out, epj_id);
+    put(out, epj_rin);
+    put(out, epj_rout);
+    put(out, pot_tot);
     std::fclose(out);
     std::printf("fdps_replay: N=%ld walks=%ld batches=%ld written %s\n", n, n_group,
                 (n_group + n_walk_limit - 1) / n_walk_limit, argv[6]);

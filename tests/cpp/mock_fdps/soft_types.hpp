@@ -7,6 +7,9 @@ struct ForceSoft {
     PS::F64vec acc;
     PS::F64 pot;
     PS::S64 n_ngb;
+#ifdef PETAR_HIP_CHANGEOVER
+    PS::F64 pot_tot;  // the member a changeover-mode PeTar build adds
+#endif
 };
 struct EPISoft {
     PS::S64 id;
@@ -31,5 +34,9 @@ struct EPJSoft {
     PS::S32 rank_org;
     PS::S32 adr_org;
 };
-static_assert(sizeof(EPISoft) == 48 && sizeof(EPJSoft) == 120 && sizeof(ForceSoft) == 40,
-              "layouts of src/soft_ptcl.hpp");
+#ifdef PETAR_HIP_CHANGEOVER
+static_assert(sizeof(ForceSoft) == 48, "ForceSoft + pot_tot");
+#else
+static_assert(sizeof(ForceSoft) == 40, "layout of src/soft_ptcl.hpp");
+#endif
+static_assert(sizeof(EPISoft) == 48 && sizeof(EPJSoft) == 120, "layouts of src/soft_ptcl.hpp");

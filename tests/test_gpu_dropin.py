@@ -27,7 +27,10 @@ def _read(path):
         ni = int(d["n_epi"].sum())
         res = (np.fromfile(f, np.float64, 3 * ni).reshape(-1, 3), np.fromfile(f, np.float64, ni),
                np.fromfile(f, np.int64, ni))
-    return d, res
+        d.update(epj_id=np.fromfile(f, np.int64, n), epj_rin=np.fromfile(f, np.float64, n),
+                 epj_rout=np.fromfile(f, np.float64, n))
+        pot_tot = np.fromfile(f, np.float64, ni)
+    return d, res + (pot_tot,)
 
 
 @pytest.mark.parametrize("n,walk_limit", [(20000, 200), (3000, 7)])
@@ -41,7 +44,7 @@ def test_cpp_dropin_fdps_replay(tmp_path, n, walk_limit):
                        text=True, timeout=240)
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0
-    d, (acc, pot, nnb) = _read(out)
+    d, (acc, pot, nnb, _) = _read(out)
     idx = np.concatenate([np.arange(b, b + k) for b, k in zip(d["epi_begin"], d["n_epi"])])
     a = dict(n_epi=d["n_epi"], n_epj=d["n_epj"], n_spj=d["n_spj"], epi_pos=d["epj_pos"][idx],
              epi_rs=d["epj_rs"][idx], id_epj=d["id_epj"], id_spj=d["id_spj"], epj_pos=d["epj_pos"],
@@ -69,3 +72,33 @@ def test_cpp_dropin_fdps_replay(tmp_path, n, walk_limit):
     amb = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=0.0)
     bad_exact, bad_band, _ = nnb_check(nnb, rnnb, amb)
     assert bad_exact == 0 and bad_band == 0
+
+
+def test_cpp_dropin_changeover_mode(tmp_path):
+    """The adapter compiled with PETAR_HIP_CHANGEOVER (ForceSoft with
+    pot_tot, mass-scaled changeover radii) against the FP64 changeover
+    oracle on the lists the program used."""
+    exe = EXE + "_co"
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} not built (build() compiles it)")
+    out = tmp_path / "replay_co.bin"
+    r_out = 0.01
+    p = subprocess.run([exe, "20000", "13", "0.3", "200", str(r_out), str(out)], capture_output=True, text=True,
+                       timeout=240)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0
+    d, (acc, pot, nnb, pot_tot) = _read(out)
+    idx = np.concatenate([np.arange(b, b + k) for b, k in zip(d["epi_begin"], d["n_epi"])])
+    a = dict(n_epi=d["n_epi"], n_epj=d["n_epj"], n_spj=d["n_spj"], epi_pos=d["epj_pos"][idx],
+             epi_rs=d["epj_rs"][idx], epi_id=d["epj_id"][idx], id_epj=d["id_epj"], id_spj=d["id_spj"],
+             epj_pos=d["epj_pos"], epj_mass=d["epj_mass"], epj_rs=d["epj_rs"], epj_id=d["epj_id"],
+             epj_rin=d["epj_rin"], epj_rout=d["epj_rout"], spj_pos=d["spj_pos"], spj_mass=d["spj_mass"],
+             spj_quad=d["spj_quad"])
+    racc, rps, rpt, rnnb, amb = O.soft_force_walks_co(**a, eps2=0.0, rout2=r_out ** 2, amb_rel=1e-4)
+    ea, es, et = rel_err(acc, racc), rel_err(pot[amb == 0], rps[amb == 0]), rel_err(pot_tot, rpt)
+    print(f"[dropin co] acc med {np.median(ea):.2e} p99 {np.percentile(ea, 99):.2e}; pot_soft med "
+          f"{np.median(es):.2e}; pot_tot med {np.median(et):.2e} p99 {np.percentile(et, 99):.2e}; "
+          f"i with neighbours {(rps != rpt).sum()}", flush=True)
+    assert np.all(pot != 99.0) and np.all(pot_tot != 99.0)
+    assert np.median(ea) <= 1e-6 and np.median(es) <= 1e-6 and np.median(et) <= 1e-6
+    assert np.percentile(et, 99) <= 1e-5 and np.percentile(es, 99) <= 1e-5

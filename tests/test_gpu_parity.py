@@ -266,3 +266,26 @@ def test_replan_and_autotune(c1):
     _check(r.results(), ref, amb, "autotuned")
     r.close()
     ctx.close()
+
+
+def test_theta0_is_direct_summation():
+    """theta = 0 turns every j into an EPJ, so the GPU result is the direct
+    sum with the linear cutoff: compare with the FP64 all-pairs oracle
+    (the small plain configuration, 16,384 particles, 2.7e8 pairs)."""
+    L = c1_lists(16384, theta=0.0, seed=1)
+    assert L.n_spj.sum() == 0 and np.all(L.n_epj == L.epj_mass.size)
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    r = sf.Resident.from_lists(ctx, L)
+    r.run()
+    acc, pot, nnb = r.results()
+    r.close()
+    ctx.close()
+    idx = np.concatenate([np.arange(b, b + k) for b, k in zip(L.epi_begin, L.n_epi)])
+    dacc, dpot, dnnb = O.direct_linear_cutoff(L.epj_pos, L.epj_mass, L.epj_rs, sample=idx, eps2=0.0,
+                                              rout2=R_OUT ** 2)
+    ea, ep = rel_err(acc, dacc), rel_err(pot, dpot)
+    print(f"[theta=0 direct] acc med {np.median(ea):.3e} p99 {np.percentile(ea, 99):.3e} max {ea.max():.3e}; "
+          f"pot med {np.median(ep):.3e}; nnb mismatches {(nnb != dnnb).sum()}", flush=True)
+    assert np.median(ea) <= MED and np.percentile(ea, 99) <= P99 and ea.max() <= MAX
+    assert np.median(ep) <= MED and np.percentile(ep, 99) <= P99
+    assert np.mean(nnb != dnnb) <= 1e-3
