@@ -85,6 +85,11 @@ typedef struct pth_params {
                        1: changeover mode (fused form only), below */
     double ngb_factor; /* changeover mode: neighbour radius factor on r_search
                           (0 = 0.99, SAFTY_FACTOR_FOR_SEARCH of src/ptcl.hpp:8) */
+    int segments;  /* resident evaluations: walks split into this many segments
+                      of equal cost (1..32), run concurrently on `streams`
+                      streams so staging overlaps the force kernels; results do
+                      not depend on it.  pth_dispatch always uses one. */
+    int streams;   /* resident evaluations: streams for the segments (1..8) */
 } pth_params;
 
 /* Changeover mode folds PeTar's changeover correction of the soft force
@@ -102,7 +107,8 @@ typedef struct pth_params {
  * off_pot_tot when that is >= 0. */
 
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
- * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99. */
+ * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99,
+ * segments 8, streams 4. */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */
@@ -175,7 +181,9 @@ int pth_resident_changeover(pth_resident *res, const long long *epi_id, const lo
  * [ni] in changeover mode (else left alone).  Any pointer may be NULL. */
 int pth_resident_results(pth_resident *res, double *acc, double *pot, long long *nnb, double *pot_tot);
 /* Per-kernel GPU times of the last evaluation (ms), from HIP events on the
- * context's stream: stage (centres + gather), force, reduce. */
+ * context's stream: stage (centres + gather), force, reduce.  With more than
+ * one segment the phases overlap: ms_force is then the whole evaluation and
+ * ms_stage = ms_reduce = 0. */
 int pth_resident_timing(pth_resident *res, float *ms_stage, float *ms_force, float *ms_reduce);
 /* Re-plans the work decomposition (fused form) with another i-per-lane
  * bound and chunk size; the uploaded inputs stay. */
@@ -249,13 +257,15 @@ void pth_server_free(pth_server *srv);
  * sizes: returns the number of work items in launch order (or a negative
  * code).  items (8 ints each: walk, i_begin, n_i | ipl << 16, e_blk_begin,
  * e_blk_end, s_begin, s_end, part; walk < 0 marks an empty slot), the staged
- * offsets es_off / ss_off (n_walk + 1 each) and the reduction items (8 ints
- * each: out, n_i, part_first, n_part, ipl, 0, 0, 0) are written when the
- * pointers are not NULL. */
+ * offsets es_off / ss_off (n_walk + 1 each), the reduction items (8 ints
+ * each: out, n_i, part_first, n_part, ipl, 0, 0, 0) and the segments (10
+ * ints for each of 32: walk_begin, walk_end, item begin of ipl 1/2/4, item
+ * count of ipl 1/2/4, ritem_begin, ritem_count; -1 past the last) are
+ * written when the pointers are not NULL. */
 long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
-                         int exec_mode, int chunk_j, int max_ipl, int *items, long long max_items,
-                         long long *es_off, long long *ss_off, int *ritems,
-                         long long max_ritems, long long *n_ritems);
+                         int exec_mode, int chunk_j, int max_ipl, int n_seg, int *items,
+                         long long max_items, long long *es_off, long long *ss_off, int *ritems,
+                         long long max_ritems, long long *n_ritems, int *segs);
 
 #ifdef __cplusplus
 }

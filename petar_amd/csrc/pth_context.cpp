@@ -29,6 +29,8 @@ void pth_params_default(pth_params *p)
     p->j_vgpr = 0;   // SGPR operands beat VGPR copies at every max_ipl
     p->changeover = 0;
     p->ngb_factor = 0.99;
+    p->segments = 8;  // measured on the precision workload: 8 segments on 4 streams
+    p->streams = 4;
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -73,6 +75,8 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
         return PTH_ERR_ARG;
     }
     if (par.max_walk <= 0) par.max_walk = 1000;
+    par.segments = std::min(std::max(par.segments, 1), PTH_MAX_SEGMENTS);
+    par.streams = std::min(std::max(par.streams, 1), 8);
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev <= 0) {
@@ -122,6 +126,8 @@ int pth_set_params(pth_ctx *c, const pth_params *p)
     c->par.nnb_mode = p->nnb_mode;
     c->par.changeover = p->changeover;
     c->par.ngb_factor = p->ngb_factor;
+    c->par.segments = std::min(std::max(p->segments, 1), PTH_MAX_SEGMENTS);
+    c->par.streams = std::min(std::max(p->streams, 1), 8);
     return PTH_OK;
 }
 
@@ -353,9 +359,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     fill_batch(b, P, c->par, c->d_arena, A, c->work, c->d_epj.as<EpjDev>(), c->d_spj.as<SpjDev>(),
                T.out_accp.as<double4>(), T.out_nnb.as<int>(), c->d_epj_co.as<EpjCoDev>(),
                T.out_pott.as<double>());
-    HIPCHK(c, launch_stage(b, c->stream), "stage kernels");
-    HIPCHK(c, launch_force(b, c->stream), "force kernel");
-    HIPCHK(c, launch_reduce(b, c->stream), "reduce kernel");
+    HIPCHK(c, launch_all(b, c->stream), "soft-force kernels");
     HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     if (co) {
@@ -461,7 +465,9 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
             return c->fail(PTH_ERR_ARG, "pth_resident_create: SPJ index out of range");
     pth_resident *r = new pth_resident;
     r->ctx = c;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan)) {
+    r->n_seg = (c->par.exec_mode == PTH_EXEC_FUSED) ? c->par.segments : 1;
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
+                   r->n_seg)) {
         delete r;
         return c->fail(PTH_ERR_CAPACITY, "pth_resident_create: sizes exceed the int32 index range");
     }
@@ -494,6 +500,16 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
     }
     for (int k = 0; k < 4; k++)
         if ((e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
+    if ((int)P.segs.size() > 1) {
+        const int ns = std::min(c->par.streams, (int)P.segs.size());
+        r->side.assign(ns, nullptr);
+        r->side_done.assign(ns, nullptr);
+        for (int k = 0; k < ns; k++) {
+            if ((e = hipStreamCreateWithFlags(&r->side[k], hipStreamNonBlocking))) return bail(e, "hipStreamCreate");
+            if ((e = hipEventCreateWithFlags(&r->side_done[k], hipEventDisableTiming)))
+                return bail(e, "hipEventCreate");
+        }
+    }
     // host mirrors (pageable is fine: uploaded once)
     std::vector<char> h(r->A.total);
     EpiDev *he = reinterpret_cast<EpiDev *>(h.data() + r->A.epi);
@@ -581,12 +597,32 @@ int pth_resident_run(pth_resident *r)
         return c->fail(PTH_ERR_STATE, "pth_resident_run: changeover mode needs pth_resident_changeover first");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     HIPCHK(c, hipEventRecord(r->ev[0], c->stream), "hipEventRecord");
-    HIPCHK(c, launch_stage(r->b, c->stream), "stage kernels");
-    HIPCHK(c, hipEventRecord(r->ev[1], c->stream), "hipEventRecord");
-    HIPCHK(c, launch_force(r->b, c->stream), "force kernel");
-    HIPCHK(c, hipEventRecord(r->ev[2], c->stream), "hipEventRecord");
-    HIPCHK(c, launch_reduce(r->b, c->stream), "reduce kernel");
-    HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
+    if (r->side.empty() || r->b.n_seg < 2) {
+        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_stage(r->b, k, c->stream), "stage kernels");
+        HIPCHK(c, hipEventRecord(r->ev[1], c->stream), "hipEventRecord");
+        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_force(r->b, k, c->stream), "force kernel");
+        HIPCHK(c, hipEventRecord(r->ev[2], c->stream), "hipEventRecord");
+        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_reduce(r->b, k, c->stream), "reduce kernel");
+        HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
+        r->pipelined = false;
+    } else {
+        // segment k on side stream k mod n: its staging overlaps the force
+        // kernels of the segments before it on the other streams
+        const int ns = (int)r->side.size();
+        for (int s = 0; s < ns; s++) HIPCHK(c, hipStreamWaitEvent(r->side[s], r->ev[0], 0), "hipStreamWaitEvent");
+        for (int k = 0; k < r->b.n_seg; k++) {
+            hipStream_t st = r->side[k % ns];
+            HIPCHK(c, launch_stage(r->b, k, st), "stage kernels");
+            HIPCHK(c, launch_force(r->b, k, st), "force kernel");
+            HIPCHK(c, launch_reduce(r->b, k, st), "reduce kernel");
+        }
+        for (int s = 0; s < ns; s++) {
+            HIPCHK(c, hipEventRecord(r->side_done[s], r->side[s]), "hipEventRecord");
+            HIPCHK(c, hipStreamWaitEvent(c->stream, r->side_done[s], 0), "hipStreamWaitEvent");
+        }
+        HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
+        r->pipelined = true;
+    }
     r->ran = true;
     return PTH_OK;
 }
@@ -648,6 +684,12 @@ int pth_resident_timing(pth_resident *r, float *ms_stage, float *ms_force, float
     if (!r->ran) return c->fail(PTH_ERR_STATE, "pth_resident_timing: no evaluation run");
     HIPCHK(c, hipEventSynchronize(r->ev[3]), "hipEventSynchronize");
     float t = 0;
+    if (r->pipelined) {
+        if (ms_stage) *ms_stage = 0.f;
+        if (ms_reduce) *ms_reduce = 0.f;
+        if (ms_force) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[0], r->ev[3]), "hipEventElapsedTime"); *ms_force = t; }
+        return PTH_OK;
+    }
     if (ms_stage) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[0], r->ev[1]), "hipEventElapsedTime"); *ms_stage = t; }
     if (ms_force) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[1], r->ev[2]), "hipEventElapsedTime"); *ms_force = t; }
     if (ms_reduce) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[2], r->ev[3]), "hipEventElapsedTime"); *ms_reduce = t; }
@@ -668,7 +710,7 @@ int pth_resident_replan(pth_resident *r, int max_ipl, int chunk_j)
         ns[w] = P0.walks[w].n_s;
     }
     Plan P;
-    if (!make_plan(nw, ne.data(), nj.data(), ns.data(), PTH_EXEC_FUSED, chunk_j, max_ipl, P))
+    if (!make_plan(nw, ne.data(), nj.data(), ns.data(), PTH_EXEC_FUSED, chunk_j, max_ipl, P, r->n_seg))
         return c->fail(PTH_ERR_CAPACITY, "pth_resident_replan: sizes exceed the int32 index range");
     // the staged layout depends on the walk sizes only
     if (P.es_off != P0.es_off || P.ss_off != P0.ss_off || P.ni_tot != P0.ni_tot)
@@ -759,16 +801,36 @@ void pth_resident_free(pth_resident *r)
     r->work.release();
     for (int k = 0; k < 4; k++)
         if (r->ev[k]) (void)hipEventDestroy(r->ev[k]);
+    for (hipEvent_t x : r->side_done)
+        if (x) (void)hipEventDestroy(x);
+    for (hipStream_t x : r->side)
+        if (x) (void)hipStreamDestroy(x);
     delete r;
 }
 
 long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-                         int chunk_j, int max_ipl, int *items, long long max_items, long long *es_off, long long *ss_off,
-                         int *ritems, long long max_ritems, long long *n_ritems)
+                         int chunk_j, int max_ipl, int n_seg, int *items, long long max_items, long long *es_off,
+                         long long *ss_off, int *ritems, long long max_ritems, long long *n_ritems, int *segs)
 {
     if (n_walk < 0 || (n_walk > 0 && (!n_epi || !n_epj || !n_spj))) return PTH_ERR_ARG;
     Plan P;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, exec_mode, chunk_j, max_ipl, P)) return PTH_ERR_CAPACITY;
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, exec_mode, chunk_j, max_ipl, P, n_seg)) return PTH_ERR_CAPACITY;
+    if (segs) {
+        for (int k = 0; k < PTH_MAX_SEGMENTS; k++) {
+            int *o = segs + 10 * k;
+            for (int q = 0; q < 10; q++) o[q] = -1;
+            if (k >= (int)P.segs.size()) continue;
+            const PlanSeg &g = P.segs[k];
+            o[0] = g.walk_begin;
+            o[1] = g.walk_end;
+            for (int c = 0; c < 3; c++) {
+                o[2 + c] = g.seg_begin[c];
+                o[5 + c] = g.seg_count[c];
+            }
+            o[8] = g.ritem_begin;
+            o[9] = g.ritem_count;
+        }
+    }
     if (es_off) std::copy(P.es_off.begin(), P.es_off.end(), es_off);
     if (ss_off) std::copy(P.ss_off.begin(), P.ss_off.end(), ss_off);
     if (n_ritems) *n_ritems = (long long)P.ritems.size();

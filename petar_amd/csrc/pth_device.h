@@ -71,13 +71,23 @@ struct RItem {
     int out, n_i, part_first, n_part, ipl, pad[3];
 };
 
+// One segment of a batch: its walks, its items per ipl class and its
+// reduction items (the launches of one segment run in order on one stream).
+struct SegDev {
+    int walk_begin = 0, n_walk = 0;
+    int seg_begin[3] = {0, 0, 0}, seg_count[3] = {0, 0, 0};
+    int ritem_begin = 0, n_ritems = 0;
+};
+#define PTH_MAX_SEGMENTS 32
+
 struct BatchDev {
+    int n_seg = 1;
+    SegDev seg[PTH_MAX_SEGMENTS];
     int n_walk = 0;
     long long ni_tot = 0;
     long long n_e_slots = 0;  // staged EPJ slots (per walk: 8*ceil(n_e/8) + 8)
     long long n_s_rec = 0;    // staged SPJ records
     int n_items = 0, n_ritems = 0;
-    int seg_begin[3] = {0, 0, 0}, seg_count[3] = {0, 0, 0};  // items of ipl 1, 2, 4
     int plain = 0, nnb_le = 0, j_vgpr = 0;
     int co = 0;        // changeover mode: (1 - K) weight for neighbours, no self term
     float eps2 = 0.f, rout2 = 0.f;
@@ -113,8 +123,11 @@ struct BatchDev {
     int *part_nnb = nullptr;
 };
 
-hipError_t launch_stage(const BatchDev &b, hipStream_t st);
-hipError_t launch_force(const BatchDev &b, hipStream_t st);
-hipError_t launch_reduce(const BatchDev &b, hipStream_t st);
+// Launches of segment k of a batch (k < b.n_seg).
+hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st);
+hipError_t launch_force(const BatchDev &b, int k, hipStream_t st);
+hipError_t launch_reduce(const BatchDev &b, int k, hipStream_t st);
+// All segments of a batch in order on one stream.
+hipError_t launch_all(const BatchDev &b, hipStream_t st);
 
 }  // namespace pth

@@ -146,6 +146,29 @@ struct Work {
     }
 };
 
+// Copies the plan's segments into the launch descriptor.
+inline void set_segments(BatchDev &b, const Plan &P)
+{
+    b.n_seg = (int)std::min<size_t>(std::max<size_t>(P.segs.size(), 1), PTH_MAX_SEGMENTS);
+    for (int k = 0; k < b.n_seg; k++) {
+        SegDev &d = b.seg[k];
+        d = SegDev();
+        if (k >= (int)P.segs.size()) {
+            d.n_walk = (int)P.walks.size();
+            continue;
+        }
+        const PlanSeg &g = P.segs[k];
+        d.walk_begin = g.walk_begin;
+        d.n_walk = g.walk_end - g.walk_begin;
+        for (int c = 0; c < 3; c++) {
+            d.seg_begin[c] = g.seg_begin[c];
+            d.seg_count[c] = g.seg_count[c];
+        }
+        d.ritem_begin = g.ritem_begin;
+        d.n_ritems = g.ritem_count;
+    }
+}
+
 inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf &arena,
                 const Arena &A, Work &W, const EpjDev *epj, const SpjDev *spj, double4 *out_accp,
                 int *out_nnb, const EpjCoDev *epj_co, double *out_pott)
@@ -156,10 +179,7 @@ inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const 
     b.n_s_rec = P.n_s_rec;
     b.n_items = (int)P.items.size();
     b.n_ritems = (int)P.ritems.size();
-    for (int c = 0; c < 3; c++) {
-        b.seg_begin[c] = P.seg_begin[c];
-        b.seg_count[c] = P.seg_count[c];
-    }
+    set_segments(b, P);
     b.j_vgpr = par.j_vgpr ? 1 : 0;
     b.plain = (par.exec_mode == PTH_EXEC_PLAIN);
     b.nnb_le = (par.nnb_mode == 1);
@@ -276,6 +296,10 @@ struct pth_resident {
     int n_epj_tot = 0;
     DevBuf plan_dev;  // work items and reduction items after a re-plan
     int max_ipl = 0, chunk_j = 0;
+    int n_seg = 1;                       // walk segments of the plan
+    std::vector<hipStream_t> side;       // streams the segments run on
+    std::vector<hipEvent_t> side_done;   // one per side stream
+    bool pipelined = false;              // the last run used the side streams
     Work work;
     BatchDev b;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -96,6 +96,7 @@ __device__ __forceinline__ unsigned co_hash(long long id)
 }
 
 struct StageArgs {
+    int walk_begin;  // block b stages walk walk_begin + b
     const EpiDev *epi;
     const WalkDev *walks;
     const long long *es_off, *ss_off;
@@ -128,7 +129,7 @@ struct StageArgs {
 template <bool CO>
 __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
 {
-    const int w = blockIdx.x;
+    const int w = a.walk_begin + (int)blockIdx.x;
     const WalkDev wk = a.walks[w];
     const EpiDev *__restrict__ epi = a.epi;
     const EpjDev *__restrict__ epj = a.epj;
@@ -519,45 +520,59 @@ __device__ __forceinline__ void epj_near_co(cfp se, cfp sec, int b_begin, int b_
     }
 }
 
+// One staged SPJ record held in registers (SGPRs unless VC).
+struct SpjRec {
+    float x, y, z, m, qxx, qyy, qzz, qxy, qyz, qzx, mtr;
+};
+
+template <bool VC> __device__ __forceinline__ SpjRec spj_load(cfp R)
+{
+    return SpjRec{jv<VC>(R[0]), jv<VC>(R[1]), jv<VC>(R[2]), jv<VC>(R[3]), jv<VC>(R[4]), jv<VC>(R[5]),
+                  jv<VC>(R[6]), jv<VC>(R[7]), jv<VC>(R[8]), jv<VC>(R[9]), jv<VC>(R[10])};
+}
+
+// SPJ records [s_begin, s_end): the record after the current one is loaded
+// before the current one is used (scalar loads in flight during the pair
+// arithmetic instead of a wait at the top of every record).
 template <int IPL, bool VC>
 __device__ __forceinline__ void spj_records(cfp ss, int s_begin, int s_end, const float (&xi)[IPL],
                                             const float (&yi)[IPL], const float (&zi)[IPL], const Consts &K,
                                             double (&A)[IPL][4])
 {
+    if (s_begin >= s_end) return;
+    SpjRec c = spj_load<VC>(ss + (size_t)s_begin * 12);
     for (int s0 = s_begin; s0 < s_end; s0 += PTH_FLUSH_SPJ) {
         const int s1 = min(s0 + PTH_FLUSH_SPJ, s_end);
         float ax[IPL], ay[IPL], az[IPL], p[IPL];
 #pragma unroll
         for (int q = 0; q < IPL; q++) ax[q] = ay[q] = az[q] = p[q] = 0.f;
         for (int s = s0; s < s1; s++) {
-            const cfp R = ss + (size_t)s * 12;
-            const float xs = jv<VC>(R[0]), ys = jv<VC>(R[1]), zs = jv<VC>(R[2]), m = jv<VC>(R[3]);
-            const float qxx = jv<VC>(R[4]), qyy = jv<VC>(R[5]), qzz = jv<VC>(R[6]);
-            const float qxy = jv<VC>(R[7]), qyz = jv<VC>(R[8]), qzx = jv<VC>(R[9]);
-            const float mtr = jv<VC>(R[10]);
+            // next record (the last one is re-read: stays in bounds)
+            const SpjRec n = spj_load<VC>(ss + (size_t)min(s + 1, s_end - 1) * 12);
 #pragma unroll
             for (int q = 0; q < IPL; q++) {
-                const float dx = xi[q] - xs;
-                const float dy = yi[q] - ys;
-                const float dz = zi[q] - zs;
+                const float dx = xi[q] - c.x;
+                const float dy = yi[q] - c.y;
+                const float dz = zi[q] - c.z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
                 const float ri = __builtin_amdgcn_rsqf(r2);
                 const float ri2 = ri * ri;
                 const float ri3 = ri * ri2;
                 const float ri4 = ri2 * ri2;
                 const float ri5 = ri2 * ri3;
-                const float qrx = __builtin_fmaf(qzx, dz, __builtin_fmaf(qxy, dy, qxx * dx));
-                const float qry = __builtin_fmaf(qyz, dz, __builtin_fmaf(qxy, dx, qyy * dy));
-                const float qrz = __builtin_fmaf(qyz, dy, __builtin_fmaf(qzx, dx, qzz * dz));
-                const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, mtr)));
+                const float qrx = __builtin_fmaf(c.qzx, dz, __builtin_fmaf(c.qxy, dy, c.qxx * dx));
+                const float qry = __builtin_fmaf(c.qyz, dz, __builtin_fmaf(c.qxy, dx, c.qyy * dy));
+                const float qrz = __builtin_fmaf(c.qyz, dy, __builtin_fmaf(c.qzx, dx, c.qzz * dz));
+                const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, c.mtr)));
                 const float rqr_ri4 = rqr * ri4;
-                const float meff = __builtin_fmaf(rqr_ri4, K.half, m);
-                const float meff3 = __builtin_fmaf(rqr_ri4, K.two_half, m) * ri3;
+                const float meff = __builtin_fmaf(rqr_ri4, K.half, c.m);
+                const float meff3 = __builtin_fmaf(rqr_ri4, K.two_half, c.m) * ri3;
                 p[q] = __builtin_fmaf(-meff, ri, p[q]);
                 ax[q] = __builtin_fmaf(-meff3, dx, __builtin_fmaf(ri5, qrx, ax[q]));
                 ay[q] = __builtin_fmaf(-meff3, dy, __builtin_fmaf(ri5, qry, ay[q]));
                 az[q] = __builtin_fmaf(-meff3, dz, __builtin_fmaf(ri5, qrz, az[q]));
             }
+            c = n;
         }
 #pragma unroll
         for (int q = 0; q < IPL; q++) {
@@ -819,10 +834,12 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 // ---------------------------------------------------------------------------
 // launchers (host)
 
-hipError_t launch_stage(const BatchDev &b, hipStream_t st)
+hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st)
 {
-    if (b.n_walk <= 0) return hipSuccess;
+    const SegDev &g = b.seg[k];
+    if (g.n_walk <= 0) return hipSuccess;
     StageArgs a;
+    a.walk_begin = g.walk_begin;
     a.epi = b.epi;
     a.walks = b.walks;
     a.es_off = b.es_off;
@@ -844,8 +861,8 @@ hipError_t launch_stage(const BatchDev &b, hipStream_t st)
     a.staged_il = b.staged_il;
     a.staged_ec = b.staged_ec;
     a.walk_err = b.walk_err;
-    if (b.co) hipLaunchKernelGGL(k_stage_walk<true>, dim3(b.n_walk), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_stage_walk<false>, dim3(b.n_walk), dim3(256), 0, st, a);
+    if (b.co) hipLaunchKernelGGL(k_stage_walk<true>, dim3(g.n_walk), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_stage_walk<false>, dim3(g.n_walk), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -857,8 +874,9 @@ static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t
     else hipLaunchKernelGGL((k_force<NNB, 4, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
 }
 
-hipError_t launch_force(const BatchDev &b, hipStream_t st)
+hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
 {
+    const SegDev &g = b.seg[k];
     if (b.plain) {
         if (b.n_walk <= 0) return hipSuccess;
         if (b.nnb_le)
@@ -891,9 +909,9 @@ hipError_t launch_force(const BatchDev &b, hipStream_t st)
     a.part_nnb = b.part_nnb;
     a.part_pott = b.part_pott;
     for (int c = 0; c < 3; c++) {
-        const int n = b.seg_count[c];
+        const int n = g.seg_count[c];
         if (n <= 0) continue;
-        a.items = b.items + b.seg_begin[c];
+        a.items = b.items + g.seg_begin[c];
         a.n_items = n;
         const unsigned nblk = (unsigned)((n + 3) / 4);
         // the changeover mode always uses SGPR operands
@@ -913,16 +931,27 @@ hipError_t launch_force(const BatchDev &b, hipStream_t st)
     return hipSuccess;
 }
 
-hipError_t launch_reduce(const BatchDev &b, hipStream_t st)
+hipError_t launch_reduce(const BatchDev &b, int k, hipStream_t st)
 {
-    if (b.plain || b.n_ritems <= 0) return hipSuccess;
+    const SegDev &g = b.seg[k];
+    if (b.plain || g.n_ritems <= 0) return hipSuccess;
+    const RItem *ri = b.ritems + g.ritem_begin;
     if (b.co)
-        hipLaunchKernelGGL(k_reduce<true>, dim3((unsigned)b.n_ritems), dim3(256), 0, st, b.ritems, b.part_accp,
+        hipLaunchKernelGGL(k_reduce<true>, dim3((unsigned)g.n_ritems), dim3(256), 0, st, ri, b.part_accp,
                            b.part_nnb, b.part_pott, b.G, b.out_accp, b.out_nnb, b.out_pott);
     else
-        hipLaunchKernelGGL(k_reduce<false>, dim3((unsigned)b.n_ritems), dim3(256), 0, st, b.ritems, b.part_accp,
+        hipLaunchKernelGGL(k_reduce<false>, dim3((unsigned)g.n_ritems), dim3(256), 0, st, ri, b.part_accp,
                            b.part_nnb, b.part_pott, b.G, b.out_accp, b.out_nnb, b.out_pott);
     return hipGetLastError();
+}
+
+hipError_t launch_all(const BatchDev &b, hipStream_t st)
+{
+    for (int k = 0; k < b.n_seg; k++) {
+        hipError_t e;
+        if ((e = launch_stage(b, k, st)) || (e = launch_force(b, k, st)) || (e = launch_reduce(b, k, st))) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace pth

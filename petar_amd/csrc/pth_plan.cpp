@@ -2,8 +2,8 @@
 //
 // The fused form cuts every walk into tiles of 64, 128 or 256 i-particles
 // (1, 2 or 4 per lane), choosing for each walk the tiling of least issue
-// cost: more i per lane share the per-wave copy of each j record, fewer
-// leave fewer lanes idle on small walks.  When the walk's list is long it
+// cost: more i per lane share each j record's per-wave scalar load and loop
+// overhead, fewer leave fewer lanes idle on small walks.  When the walk's list is long it
 // is cut into chunks of at most chunk_j list entries (an SPJ counts twice),
 // EPJ blocks first, then SPJ records.  Tiling and chunk boundaries depend
 // only on the walk's own sizes, so a walk gives bit-identical results
@@ -12,7 +12,8 @@
 // hence staged lists, so each list is fetched into one XCD's L2), most
 // expensive first inside each run, and written in the order the hardware
 // hands workgroups to XCDs (round robin), padding short runs with empty
-// items.
+// items.  With several segments (contiguous walk ranges of equal cost, run
+// concurrently on different streams) the dealing is done per segment.
 #include "pth_plan.h"
 
 #include <algorithm>
@@ -69,7 +70,7 @@ std::vector<Tile> tile_walk(int n, long long n_e, long long n_s, int max_ipl)
 }  // namespace
 
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, int max_ipl, Plan &P)
+               int chunk_j, int max_ipl, Plan &P, int n_seg)
 {
     P = Plan();
     P.walks.resize(n_walk);
@@ -98,7 +99,12 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     P.n_e_slots = P.es_off[n_walk];
     P.n_s_rec = P.ss_off[n_walk];
     if (P.n_e_slots / 8 > INT_MAX || P.n_s_rec > INT_MAX) return false;
-    if (exec_mode == PTH_EXEC_PLAIN) return true;
+    if (exec_mode == PTH_EXEC_PLAIN) {
+        PlanSeg g;
+        g.walk_end = n_walk;
+        P.segs.push_back(g);
+        return true;
+    }
     if (max_ipl != 1 && max_ipl != 2) max_ipl = 4;
 
     const long long cj = (chunk_j > 0) ? chunk_j : DEFAULT_CHUNK_J;
@@ -107,7 +113,11 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         double cost;
     };
     std::vector<Tmp> seq;
+    std::vector<size_t> walk_item_begin(n_walk + 1, 0);
+    std::vector<int> walk_ritem_begin(n_walk + 1, 0);
     for (int w = 0; w < n_walk; w++) {
+        walk_item_begin[w] = seq.size();
+        walk_ritem_begin[w] = (int)P.ritems.size();
         const WalkDev &W = P.walks[w];
         if (W.n_i <= 0) continue;
         const long long nb = (P.es_off[w + 1] - P.es_off[w]) / 8;
@@ -157,7 +167,26 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
             }
         }
     }
+    walk_item_begin[n_walk] = seq.size();
+    walk_ritem_begin[n_walk] = (int)P.ritems.size();
     P.n_real_items = (int)seq.size();
+    // segments: contiguous walk ranges of about equal item cost
+    if (n_seg < 1) n_seg = 1;
+    if (n_seg > MAX_SEGMENTS) n_seg = MAX_SEGMENTS;
+    if (n_seg > n_walk) n_seg = std::max(n_walk, 1);
+    std::vector<double> wcost(n_walk + 1, 0.0);
+    for (int w = 0; w < n_walk; w++) {
+        double c = 0;
+        for (size_t k = walk_item_begin[w]; k < walk_item_begin[w + 1]; k++) c += seq[k].cost;
+        wcost[w + 1] = wcost[w] + c;
+    }
+    std::vector<int> cut(n_seg + 1, 0);
+    cut[n_seg] = n_walk;
+    for (int g = 1; g < n_seg; g++) {
+        const double target = wcost[n_walk] * g / n_seg;
+        cut[g] = (int)(std::lower_bound(wcost.begin(), wcost.end(), target) - wcost.begin());
+        cut[g] = std::min(std::max(cut[g], cut[g - 1]), n_walk);
+    }
     // one launch per ipl class (each kernel has its own register budget);
     // inside a class, 8 contiguous runs of equal cost, one per XCD
     Item dummy;
@@ -165,43 +194,57 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     dummy.i_begin = dummy.n_i_ipl = dummy.e_blk_begin = dummy.e_blk_end = dummy.s_begin = dummy.s_end = 0;
     dummy.part = -1;
     static const int class_ipl[3] = {1, 2, 4};
-    for (int cls = 0; cls < 3; cls++) {
-        std::vector<Tmp> sub;
-        for (auto &s : seq)
-            if (item_ipl(s.it.n_i_ipl) == class_ipl[cls]) sub.push_back(s);
-        P.seg_begin[cls] = (int)P.items.size();
-        if (sub.empty()) continue;
-        double total = 0;
-        for (auto &s : sub) total += s.cost;
-        std::vector<std::vector<Item>> run(XCDS);
-        double acc = 0;
-        int x = 0;
-        std::vector<Tmp> cur;
-        auto flush = [&](int xx) {
-            std::stable_sort(cur.begin(), cur.end(), [](const Tmp &a, const Tmp &b) { return a.cost > b.cost; });
-            for (auto &c : cur) run[xx].push_back(c.it);
-            cur.clear();
-        };
-        for (size_t k = 0; k < sub.size(); k++) {
-            // an item goes to run x while the running cost is below (x+1)/8 of the total
-            while (x < XCDS - 1 && acc >= total * (x + 1) / XCDS) {
-                flush(x);
-                x++;
+    // the items of walks [wb, we), per ipl class, in launch order
+    auto arrange = [&](int wb, int we) {
+        PlanSeg sg;
+        sg.walk_begin = wb;
+        sg.walk_end = we;
+        sg.ritem_begin = walk_ritem_begin[wb];
+        sg.ritem_count = walk_ritem_begin[we] - sg.ritem_begin;
+        for (int cls = 0; cls < 3; cls++) {
+            std::vector<Tmp> sub;
+            for (size_t k = walk_item_begin[wb]; k < walk_item_begin[we]; k++)
+                if (item_ipl(seq[k].it.n_i_ipl) == class_ipl[cls]) sub.push_back(seq[k]);
+            sg.seg_begin[cls] = (int)P.items.size();
+            if (sub.empty()) continue;
+            double total = 0;
+            for (auto &t : sub) total += t.cost;
+            std::vector<std::vector<Item>> run(XCDS);
+            double acc = 0;
+            int x = 0;
+            std::vector<Tmp> cur;
+            auto flush = [&](int xx) {
+                std::stable_sort(cur.begin(), cur.end(), [](const Tmp &l, const Tmp &r) { return l.cost > r.cost; });
+                for (auto &c : cur) run[xx].push_back(c.it);
+                cur.clear();
+            };
+            for (size_t k = 0; k < sub.size(); k++) {
+                // an item goes to run x while the running cost is below (x+1)/8 of the total
+                while (x < XCDS - 1 && acc >= total * (x + 1) / XCDS) {
+                    flush(x);
+                    x++;
+                }
+                cur.push_back(sub[k]);
+                acc += sub[k].cost;
             }
-            cur.push_back(sub[k]);
-            acc += sub[k].cost;
+            flush(x);
+            size_t max_blocks = 0;
+            for (auto &r : run) max_blocks = std::max(max_blocks, (r.size() + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
+            const size_t base = P.items.size();
+            P.items.resize(base + max_blocks * XCDS * ITEMS_PER_BLOCK, dummy);
+            for (int xx = 0; xx < XCDS; xx++)
+                for (size_t k = 0; k < run[xx].size(); k++) {
+                    const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
+                    P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = run[xx][k];
+                }
+            sg.seg_count[cls] = (int)(P.items.size() - base);
         }
-        flush(x);
-        size_t max_blocks = 0;
-        for (auto &r : run) max_blocks = std::max(max_blocks, (r.size() + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
-        const size_t base = P.items.size();
-        P.items.resize(base + max_blocks * XCDS * ITEMS_PER_BLOCK, dummy);
-        for (int xx = 0; xx < XCDS; xx++)
-            for (size_t k = 0; k < run[xx].size(); k++) {
-                const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
-                P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = run[xx][k];
-            }
-        P.seg_count[cls] = (int)(P.items.size() - base);
+        return sg;
+    };
+    for (int g = 0; g < n_seg; g++) P.segs.push_back(arrange(cut[g], cut[g + 1]));
+    for (int cls = 0; cls < 3; cls++) {  // whole-plan view (one segment)
+        P.seg_begin[cls] = P.segs[0].seg_begin[cls];
+        P.seg_count[cls] = P.segs[0].seg_count[cls];
     }
     return true;
 }

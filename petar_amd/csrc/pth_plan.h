@@ -15,10 +15,21 @@ constexpr int DEFAULT_CHUNK_J = 2048;
 
 // Issue cost of one pair per lane, in units of one full-rate (4-cycle)
 // wave64 VALU instruction, for a tile holding ipl i-particles per lane:
-// EPJ (mix of near and far blocks) and SPJ, including the per-wave copy of
-// the j record into VGPRs shared by the ipl i-particles.
+// EPJ (mix of near and far blocks) and SPJ, plus the per-record overhead
+// shared by the ipl i-particles of a lane.
 inline double epj_unit_cost(int ipl) { return 9.5 + 2.25 / ipl; }
 inline double spj_unit_cost(int ipl) { return 18.5 + 5.5 / ipl; }
+
+// A contiguous range of walks planned on its own (items of each class
+// arranged over the XCDs within the segment), so segments can run
+// concurrently on different streams.
+struct PlanSeg {
+    int walk_begin = 0, walk_end = 0;
+    int seg_begin[3] = {0, 0, 0};  // items of ipl 1, 2, 4
+    int seg_count[3] = {0, 0, 0};
+    int ritem_begin = 0, ritem_count = 0;
+};
+constexpr int MAX_SEGMENTS = PTH_MAX_SEGMENTS;
 
 struct Plan {
     std::vector<WalkDev> walks;
@@ -31,6 +42,7 @@ struct Plan {
     int n_real_items = 0;
     long long ni_tot = 0, n_e_slots = 0, n_s_rec = 0;
     long long n_epep = 0, n_epsp = 0;       // sum n_i n_e, sum n_i n_s
+    std::vector<PlanSeg> segs;              // at least one
 };
 
 // Staged EPJ slots of a walk: near and far regions with at least 8 slots of
@@ -39,9 +51,11 @@ inline long long epj_region_slots(long long n_e) { return n_e > 0 ? (n_e + 7) / 
 
 // Builds the plan; exec_mode PTH_EXEC_FUSED or PTH_EXEC_PLAIN, chunk_j the
 // largest list length (EPJ entries + 2 x SPJ entries) of one item (<= 0:
-// default), max_ipl the largest number of i per lane (1, 2 or 4).
-// Returns false if a size overflows the int32 ranges of the device records.
+// default), max_ipl the largest number of i per lane (1, 2 or 4), n_seg the
+// number of walk segments of about equal cost (fused form; the items of a
+// walk do not depend on it).  Returns false if a size overflows the int32
+// ranges of the device records.
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, int max_ipl, Plan &P);
+               int chunk_j, int max_ipl, Plan &P, int n_seg = 1);
 
 }  // namespace pth

@@ -198,10 +198,7 @@ void bind(SCluster &c, char *base, const pth_params &par)
     b.n_s_rec = P.n_s_rec;
     b.n_items = (int)P.items.size();
     b.n_ritems = (int)P.ritems.size();
-    for (int k = 0; k < 3; k++) {
-        b.seg_begin[k] = P.seg_begin[k];
-        b.seg_count[k] = P.seg_count[k];
-    }
+    set_segments(b, P);
     b.j_vgpr = par.j_vgpr ? 1 : 0;
     b.nnb_le = (par.nnb_mode == 1);
     b.eps2 = (float)par.eps2;
@@ -281,9 +278,7 @@ bool admit(pth_server *s, int k, hipStream_t st, hipError_t &e)
     bind(c, base, s->ctx->par);
     if ((e = hipMemcpyAsync(base, c.image.data(), c.L.upload, hipMemcpyHostToDevice, st))) return false;
     if ((e = hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal))) return false;
-    hipError_t el = launch_stage(c.b, s->cap);
-    if (el == hipSuccess) el = launch_force(c.b, s->cap);
-    if (el == hipSuccess) el = launch_reduce(c.b, s->cap);
+    const hipError_t el = launch_all(c.b, s->cap);
     e = hipStreamEndCapture(s->cap, &c.graph);
     if (el != hipSuccess) e = el;
     if (e) return false;

@@ -52,7 +52,8 @@ assert EPI_SOFT.itemsize == 48 and EPJ_SOFT.itemsize == 120 and FORCE_SOFT.items
 class Params(C.Structure):
     _fields_ = [("eps2", C.c_double), ("rcut2", C.c_double), ("G", C.c_double), ("nnb_mode", C.c_int),
                 ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int), ("max_ipl", C.c_int),
-                ("j_vgpr", C.c_int), ("changeover", C.c_int), ("ngb_factor", C.c_double)]
+                ("j_vgpr", C.c_int), ("changeover", C.c_int), ("ngb_factor", C.c_double), ("segments", C.c_int),
+                ("streams", C.c_int)]
 
 
 class EpjView(C.Structure):
@@ -129,9 +130,9 @@ def lib():
     L.pth_resident_replan.argtypes = [vp, C.c_int, C.c_int]
     L.pth_resident_autotune.argtypes = [vp, ip, C.c_int, ip, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_float)]
     L.pth_plan_debug.restype = C.c_longlong
-    L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
+    L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
                                  C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip, C.c_longlong,
-                                 C.POINTER(C.c_longlong)]
+                                 C.POINTER(C.c_longlong), ip]
     _lib = L
     return L
 
@@ -153,7 +154,7 @@ class PthError(RuntimeError):
 
 
 def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=1000,
-                   max_ipl=None, j_vgpr=None, changeover=0, ngb_factor=None):
+                   max_ipl=None, j_vgpr=None, changeover=0, ngb_factor=None, segments=None, streams=None):
     p = Params()
     lib().pth_params_default(C.byref(p))
     p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk = (
@@ -165,6 +166,10 @@ def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED,
     p.changeover = changeover
     if ngb_factor is not None:
         p.ngb_factor = ngb_factor
+    if segments is not None:
+        p.segments = segments
+    if streams is not None:
+        p.streams = streams
     return p
 
 
@@ -507,24 +512,27 @@ class Server:
             pass
 
 
-def plan_debug(n_epi, n_epj, n_spj, exec_mode=EXEC_FUSED, chunk_j=0, max_ipl=4):
+def plan_debug(n_epi, n_epj, n_spj, exec_mode=EXEC_FUSED, chunk_j=0, max_ipl=4, n_seg=1, segments=False):
     """The work decomposition (host only): items [k, 8] (column 2 packs
-    n_i | ipl << 16), es_off, ss_off, ritems [k, 8]."""
+    n_i | ipl << 16), es_off, ss_off, ritems [k, 8] (and the segments
+    [g, 10] when segments=True)."""
     L = lib()
     ne, nj, ns = _i32(n_epi), _i32(n_epj), _i32(n_spj)
     nw = ne.size
     nr = C.c_longlong()
     n = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j, max_ipl,
-                         None, 0, None, None, None, 0, C.byref(nr))
+                         n_seg, None, 0, None, None, None, 0, C.byref(nr), None)
     if n < 0:
         raise PthError(f"pth_plan_debug failed ({n})")
     items = np.zeros((max(n, 1), 8), np.int32)
     rit = np.zeros((max(nr.value, 1), 8), np.int32)
     es = np.zeros(nw + 1, np.int64)
     ss = np.zeros(nw + 1, np.int64)
+    sg = np.zeros((32, 10), np.int32)
     n2 = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j, max_ipl,
-                          _ptr(items, C.c_int), items.shape[0], _ptr(es, C.c_longlong), _ptr(ss, C.c_longlong),
-                          _ptr(rit, C.c_int), rit.shape[0], C.byref(nr))
+                          n_seg, _ptr(items, C.c_int), items.shape[0], _ptr(es, C.c_longlong), _ptr(ss, C.c_longlong),
+                          _ptr(rit, C.c_int), rit.shape[0], C.byref(nr), _ptr(sg, C.c_int))
     if n2 < 0:
         raise PthError(f"pth_plan_debug failed ({n2})")
-    return items[:n], es, ss, rit[:nr.value]
+    out = (items[:n], es, ss, rit[:nr.value])
+    return out + (sg[sg[:, 0] >= 0],) if segments else out

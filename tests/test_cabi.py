@@ -37,7 +37,7 @@ def test_params_default():
     p = sf.Params()
     sf.lib().pth_params_default(C.byref(p))
     assert (p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk, p.max_ipl, p.j_vgpr,
-            p.changeover, p.ngb_factor) == (0.0, 0.0, 1.0, 0, 0, 0, 1000, 2, 0, 0, 0.99)
+            p.changeover, p.ngb_factor, p.segments, p.streams) == (0.0, 0.0, 1.0, 0, 0, 0, 1000, 2, 0, 0, 0.99, 8, 4)
 
 
 def test_create_without_device_fails_cleanly():
@@ -169,8 +169,8 @@ def test_plan_plain_has_no_items():
 
 def test_plan_rejects_bad_input():
     L = sf.lib()
-    assert L.pth_plan_debug(-1, None, None, None, 0, 0, 4, None, 0, None, None, None, 0, None) == sf.PTH_ERR_ARG
-    assert L.pth_plan_debug(2, None, None, None, 0, 0, 4, None, 0, None, None, None, 0, None) == sf.PTH_ERR_ARG
+    assert L.pth_plan_debug(-1, None, None, None, 0, 0, 4, 1, None, 0, None, None, None, 0, None, None) == sf.PTH_ERR_ARG
+    assert L.pth_plan_debug(2, None, None, None, 0, 0, 4, 1, None, 0, None, None, None, 0, None, None) == sf.PTH_ERR_ARG
 
 
 def test_fdps_record_layouts():
@@ -179,3 +179,38 @@ def test_fdps_record_layouts():
     assert sf.EPJ_SOFT.itemsize == 120 and sf.EPJ_SOFT.fields["pos"][1] == 16
     assert sf.EPJ_SOFT.fields["r_search"][1] == 80
     assert sf.FORCE_SOFT.itemsize == 40 and sf.FORCE_SOFT.fields["n_ngb"][1] == 32
+
+
+def test_plan_segments_keep_every_walks_items():
+    """Splitting a batch into segments changes only the launch arrangement:
+    every walk keeps exactly its items (so results are bit-identical), each
+    segment is a contiguous walk range holding only its own items and its own
+    reduction items, and the segments cover all walks."""
+    rng = np.random.default_rng(11)
+    ne = rng.integers(1, 400, 200)
+    nj = (10 ** rng.uniform(2, 4.5, 200)).astype(int)
+    ns = (10 ** rng.uniform(1, 4, 200)).astype(int)
+
+    def per_walk(items):
+        d = {}
+        for it in items:
+            if it[0] >= 0:
+                d.setdefault(int(it[0]), []).append(tuple(int(x) for x in it[1:]))
+        return {k: sorted(v) for k, v in d.items()}
+
+    base, es0, ss0, rit0 = sf.plan_debug(ne, nj, ns, sf.EXEC_FUSED, 1024, 2)
+    for n_seg in (2, 5, 8, 32):
+        items, es, ss, rit, sg = sf.plan_debug(ne, nj, ns, sf.EXEC_FUSED, 1024, 2, n_seg=n_seg, segments=True)
+        assert per_walk(items) == per_walk(base)
+        assert np.array_equal(es, es0) and np.array_equal(ss, ss0)
+        assert sorted(map(tuple, rit)) == sorted(map(tuple, rit0))
+        assert len(sg) == n_seg and sg[0, 0] == 0 and sg[-1, 1] == len(ne)
+        assert all(sg[k, 1] == sg[k + 1, 0] for k in range(len(sg) - 1))
+        ioff = np.concatenate([[0], np.cumsum(ne)])
+        for g in sg:
+            wb, we = g[0], g[1]
+            for c in range(3):
+                for it in items[g[2 + c]:g[2 + c] + g[5 + c]]:
+                    assert it[0] < 0 or wb <= it[0] < we
+            for r in rit[g[8]:g[8] + g[9]]:
+                assert ioff[wb] <= r[0] < ioff[we]

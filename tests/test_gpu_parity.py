@@ -289,3 +289,21 @@ def test_theta0_is_direct_summation():
     assert np.median(ea) <= MED and np.percentile(ea, 99) <= P99 and ea.max() <= MAX
     assert np.median(ep) <= MED and np.percentile(ep, 99) <= P99
     assert np.mean(nnb != dnnb) <= 1e-3
+
+
+@pytest.mark.parametrize("segments,streams", [(1, 1), (4, 2), (8, 4), (32, 8)])
+def test_segments_are_bit_identical(c1, segments, streams):
+    """Walk segments run concurrently on several streams; every walk keeps
+    its items, so the results equal the one-stream evaluation bit for bit."""
+    L, a, ref, amb = c1
+    out = []
+    for seg, st in ((1, 1), (segments, streams)):
+        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=seg, streams=st)
+        r = sf.Resident.from_lists(ctx, L)
+        r.run()
+        r.run()
+        out.append(r.results())
+        r.close()
+        ctx.close()
+    assert all(np.array_equal(x, y) for x, y in zip(out[0], out[1]))
+    _check(out[1], ref, amb, f"segments={segments} streams={streams}")
