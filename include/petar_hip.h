@@ -252,6 +252,35 @@ int pth_server_stats(const pth_server *srv, long long *n_done, long long *n_admi
                      long long *interactions, long long *pool_peak);
 void pth_server_free(pth_server *srv);
 
+/* ---- tree and interaction lists on the device -------------------------------
+ * No counterpart in the reference GPU code (FDPS builds the tree and the lists
+ * on the CPU, src/petar.hpp:619-623,2741,894-899).  Builds the octree
+ * (Morton order, cells split above n_leaf_limit particles), its moments
+ * (mass, centre of mass, raw second moment, particle box, box widened by
+ * r_search), the i-groups (largest cells with at most n_group_limit
+ * particles) and every group's EPJ / SPJ lists (theta opening with the
+ * symmetric r_search test) with the semantics of pth_tree_build +
+ * pth_tree_walk of include/petar_host.h: the same nodes, moments, groups
+ * and, per group, the same list entries (in another order).  Positions,
+ * masses and r_search (may be NULL: 0) are host arrays [n][3], [n], [n]. */
+typedef struct pth_gtree pth_gtree;
+int pth_gtree_build(pth_ctx *ctx, pth_gtree **tree, long long n, const double *pos, const double *mass,
+                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta);
+int pth_gtree_sizes(const pth_gtree *tree, long long *n_node, long long *n_group, long long *n_epj_list,
+                    long long *n_spj_list);
+/* Copies the tree to the host (any pointer may be NULL): order [n] (sorted
+ * position -> input index), nodes [n_node] (begin, end, level, mass, centre
+ * of mass [3], raw second moment [6]: xx yy zz xy xz yz), groups [n_group]
+ * (first particle, size, list lengths) and the concatenated lists (EPJ
+ * entries are sorted positions, SPJ entries node indices). */
+int pth_gtree_download(const pth_gtree *tree, long long *order, long long *node_begin, long long *node_end,
+                       int *node_level, double *node_mass, double *node_com, double *node_quad,
+                       long long *epi_begin, int *n_epi, int *n_epj, int *n_spj, int *id_epj, int *id_spj);
+/* Host wall time of the last build (ms): tree (sort, nodes, moments, groups)
+ * and walk (lists). */
+int pth_gtree_timing(const pth_gtree *tree, float *ms_build, float *ms_walk);
+void pth_gtree_free(pth_gtree *tree);
+
 /* ---- diagnostics (host only, no device needed) ----------------------------- */
 /* The work decomposition pth_dispatch / pth_resident_create use for these walk
  * sizes: returns the number of work items in launch order (or a negative

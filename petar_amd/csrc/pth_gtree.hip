@@ -1,0 +1,727 @@
+// pth_gtree.hip -- the octree, its moments, the i-groups and the interaction
+// lists built on the device (pth_gtree_* of include/petar_hip.h), with the
+// semantics of the host tree (petar_amd/csrc/host/tree.cpp), which stands in
+// for FDPS's TreeForForceLong<...>::QuadrupoleWithSymmetrySearch
+// (src/petar.hpp:619-623, 2741) and its multi-walk lists
+// (src/petar.hpp:894-899):
+//   1. bounding cube (FP64 min/max), 63-bit Morton keys, stable radix sort of
+//      (key, index);
+//   2. nodes level by level: a node with more than n_leaf_limit particles is
+//      split into its non-empty octants (binary search of the key ranges);
+//      the nodes of a level follow their parents' order, octants in order;
+//   3. moments bottom-up (mass, centre of mass, raw second moment about it,
+//      particle box and box widened by r_search), each node by one thread in
+//      the host's summation order;
+//   4. i-groups: largest cells with at most n_group_limit particles (or
+//      leaves), in Morton order;
+//   5. walk: one wave per group traverses the tree with a stack in LDS,
+//      popping up to 64 nodes at a time; a node is opened when it contains
+//      the group, when the group box is closer to its centre of mass than
+//      L/theta, or when the boxes overlap with either side widened by
+//      r_search; opened leaves give their particles (EPJ), closed nodes give
+//      one SPJ.  A counting pass, a scan, then a writing pass.
+// Every decision is the same FP64 expression as on the host (no FMA
+// contraction in this file), so the lists are the host's lists as sets (the
+// order inside a list differs: the walk pops nodes in batches).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <chrono>
+
+#include "pth_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int GT_MAX_LEVEL = 21;
+constexpr int WALK_STACK = 4096;  // LDS node stack of one walking wave
+
+struct GNode {
+    long long begin, end;
+    int level, n_child;
+    long long first_child;
+};
+
+__device__ __forceinline__ unsigned long long spread3(unsigned long long v)
+{
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// per-block min/max of the coordinates: part[block * 6 + {0..2: min, 3..5: max}]
+__global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ pos, long long n, double *__restrict__ part)
+{
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        for (int d = 0; d < 3; d++) {
+            const double x = pos[3 * i + d];
+            lo[d] = fmin(lo[d], x);
+            hi[d] = fmax(hi[d], x);
+        }
+    __shared__ double red[4][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
+        }
+    if (lane == 0)
+        for (int d = 0; d < 3; d++) {
+            red[wave][d] = lo[d];
+            red[wave][3 + d] = hi[d];
+        }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        double v = red[0][d];
+        for (int w = 1; w < 4; w++) v = (d < 3) ? fmin(v, red[w][d]) : fmax(v, red[w][d]);
+        part[blockIdx.x * 6 + d] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_keys(const double *__restrict__ pos, long long n, double r0, double r1,
+                                              double r2, double scale, unsigned long long *__restrict__ key,
+                                              long long *__restrict__ idx)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double r[3] = {r0, r1, r2};
+    unsigned long long c[3];
+    for (int d = 0; d < 3; d++) {
+        const double f = (pos[3 * i + d] - r[d]) * scale;
+        long long v = (long long)floor(f);
+        v = min(max(v, 0LL), (long long)(1u << GT_MAX_LEVEL) - 1);
+        c[d] = (unsigned long long)v;
+    }
+    key[i] = spread3(c[0]) << 2 | spread3(c[1]) << 1 | spread3(c[2]);
+    idx[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_gather(long long n, const long long *__restrict__ order,
+                                                const double *__restrict__ pos, const double *__restrict__ mass,
+                                                const double *__restrict__ rs, double *__restrict__ spos,
+                                                double *__restrict__ smass, double *__restrict__ srs)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const long long i = order[k];
+    for (int d = 0; d < 3; d++) spos[3 * k + d] = pos[3 * i + d];
+    smass[k] = mass[i];
+    srs[k] = rs ? rs[i] : 0.0;
+}
+
+// octant boundaries and non-empty child count of nodes [lb, lb + nl)
+__global__ __launch_bounds__(256) void k_split_count(const GNode *__restrict__ nodes, long long lb, long long nl,
+                                                     const unsigned long long *__restrict__ key, int n_leaf_limit,
+                                                     long long *__restrict__ bnd, int *__restrict__ cnt)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nl) return;
+    const GNode nd = nodes[lb + k];
+    long long *b = bnd + 9 * k;
+    const long long m = nd.end - nd.begin;
+    if (m <= n_leaf_limit || nd.level >= GT_MAX_LEVEL) {
+        cnt[k] = 0;
+        return;
+    }
+    const int shift = 3 * (GT_MAX_LEVEL - 1 - nd.level);
+    b[0] = nd.begin;
+    b[8] = nd.end;
+    for (int d = 1; d < 8; d++) {
+        // first position in [begin, end) whose octant digit is >= d
+        long long lo = nd.begin, hi = nd.end;
+        while (lo < hi) {
+            const long long mid = (lo + hi) >> 1;
+            if ((int)((key[mid] >> shift) & 7u) < d) lo = mid + 1;
+            else hi = mid;
+        }
+        b[d] = lo;
+    }
+    int c = 0;
+    for (int d = 0; d < 8; d++) c += (b[d + 1] > b[d]);
+    cnt[k] = c;
+}
+
+__global__ __launch_bounds__(256) void k_split_write(GNode *__restrict__ nodes, long long lb, long long nl,
+                                                     long long le, const long long *__restrict__ bnd,
+                                                     const int *__restrict__ cnt, const int *__restrict__ off)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nl || cnt[k] == 0) return;
+    GNode &nd = nodes[lb + k];
+    nd.first_child = le + off[k];
+    nd.n_child = cnt[k];
+    const long long *b = bnd + 9 * k;
+    long long c = nd.first_child;
+    for (int d = 0; d < 8; d++)
+        if (b[d + 1] > b[d]) nodes[c++] = GNode{b[d], b[d + 1], nd.level + 1, 0, -1};
+}
+
+// moments of nodes [a, b) (children are done): the host's sums in its order
+__global__ __launch_bounds__(256) void k_moments(const GNode *__restrict__ nodes, long long a, long long b,
+                                                 const double *__restrict__ pos, const double *__restrict__ mass,
+                                                 const double *__restrict__ rs, double *__restrict__ nmass,
+                                                 double *__restrict__ ncom, double *__restrict__ nquad,
+                                                 double *__restrict__ box_in, double *__restrict__ box_out)
+{
+    const long long k = a + (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= b) return;
+    const GNode nd = nodes[k];
+    double M = 0, c[3] = {0, 0, 0}, q[6] = {0, 0, 0, 0, 0, 0};
+    double bi[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    double bo[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+    if (nd.n_child == 0) {
+        for (long long i = nd.begin; i < nd.end; i++) {
+            M += mass[i];
+            for (int d = 0; d < 3; d++) {
+                const double x = pos[3 * i + d];
+                c[d] += mass[i] * x;
+                bi[d] = fmin(bi[d], x);
+                bi[3 + d] = fmax(bi[3 + d], x);
+                bo[d] = fmin(bo[d], x - rs[i]);
+                bo[3 + d] = fmax(bo[3 + d], x + rs[i]);
+            }
+        }
+        if (M > 0) {
+            for (int d = 0; d < 3; d++) c[d] /= M;
+        } else {
+            for (int d = 0; d < 3; d++) c[d] = 0.5 * (bi[d] + bi[3 + d]);
+        }
+        for (long long i = nd.begin; i < nd.end; i++) {
+            const double x = pos[3 * i] - c[0], y = pos[3 * i + 1] - c[1], z = pos[3 * i + 2] - c[2], m = mass[i];
+            q[0] += m * x * x;
+            q[1] += m * y * y;
+            q[2] += m * z * z;
+            q[3] += m * x * y;
+            q[4] += m * x * z;
+            q[5] += m * y * z;
+        }
+    } else {
+        for (int ch = 0; ch < nd.n_child; ch++) {
+            const long long cc = nd.first_child + ch;
+            M += nmass[cc];
+            for (int d = 0; d < 3; d++) {
+                c[d] += nmass[cc] * ncom[3 * cc + d];
+                bi[d] = fmin(bi[d], box_in[6 * cc + d]);
+                bi[3 + d] = fmax(bi[3 + d], box_in[6 * cc + 3 + d]);
+                bo[d] = fmin(bo[d], box_out[6 * cc + d]);
+                bo[3 + d] = fmax(bo[3 + d], box_out[6 * cc + 3 + d]);
+            }
+        }
+        if (M > 0) {
+            for (int d = 0; d < 3; d++) c[d] /= M;
+        } else {
+            for (int d = 0; d < 3; d++) c[d] = 0.5 * (bi[d] + bi[3 + d]);
+        }
+        for (int ch = 0; ch < nd.n_child; ch++) {
+            const long long cc = nd.first_child + ch;
+            const double m = nmass[cc];
+            const double x = ncom[3 * cc] - c[0], y = ncom[3 * cc + 1] - c[1], z = ncom[3 * cc + 2] - c[2];
+            const double *qc = nquad + 6 * cc;
+            q[0] += qc[0] + m * x * x;
+            q[1] += qc[1] + m * y * y;
+            q[2] += qc[2] + m * z * z;
+            q[3] += qc[3] + m * x * y;
+            q[4] += qc[4] + m * x * z;
+            q[5] += qc[5] + m * y * z;
+        }
+    }
+    nmass[k] = M;
+    for (int d = 0; d < 3; d++) ncom[3 * k + d] = c[d];
+    for (int d = 0; d < 6; d++) {
+        nquad[6 * k + d] = q[d];
+        box_in[6 * k + d] = bi[d];
+        box_out[6 * k + d] = bo[d];
+    }
+}
+
+// group flags: the root when it qualifies (at most n_group_limit particles,
+// or a leaf); k_group_children adds the qualifying children of every
+// non-qualifying node, i.e. the largest qualifying cells
+__global__ __launch_bounds__(256) void k_group_flags(const GNode *__restrict__ nodes, long long nn, int n_group_limit,
+                                                     char *__restrict__ flag)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nn) return;
+    char f = 0;
+    if (k == 0) {
+        const GNode nd = nodes[0];
+        f = (char)((nd.end - nd.begin <= n_group_limit) || nd.n_child == 0);
+    }
+    flag[k] = f;
+}
+
+// marks the children of every non-qualifying internal node that qualify
+__global__ __launch_bounds__(256) void k_group_children(const GNode *__restrict__ nodes, long long nn,
+                                                        int n_group_limit, char *__restrict__ flag)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nn) return;
+    const GNode nd = nodes[k];
+    const bool q = (nd.end - nd.begin <= n_group_limit) || nd.n_child == 0;
+    if (q) return;
+    for (int ch = 0; ch < nd.n_child; ch++) {
+        const GNode c = nodes[nd.first_child + ch];
+        if ((c.end - c.begin <= n_group_limit) || c.n_child == 0) flag[nd.first_child + ch] = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_group_keys(const GNode *__restrict__ nodes, const int *__restrict__ gnode,
+                                                    long long ng, long long *__restrict__ gbeg)
+{
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= ng) return;
+    gbeg[g] = nodes[gnode[g]].begin;
+}
+
+struct WalkArgs {
+    const GNode *nodes;
+    const double *ncom, *box_in, *box_out;
+    const int *groups;
+    long long n_group;
+    double root_len, inv_theta2;
+    int theta_pos;
+    long long *cnt_e, *cnt_s;                // counting pass
+    const long long *off_e, *off_s;          // writing pass
+    int *id_epj, *id_spj;
+};
+
+__device__ __forceinline__ bool overlap6(const double *a, const double *b)
+{
+    for (int d = 0; d < 3; d++)
+        if (a[d] > b[3 + d] || b[d] > a[3 + d]) return false;
+    return true;
+}
+
+// inclusive wave scan of v
+__device__ __forceinline__ long long wave_scan(long long v, int lane)
+{
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// One 64-thread workgroup (one wave) per group at a time.
+template <bool WRITE>
+__global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
+{
+    __shared__ int stk[WALK_STACK];
+    const int lane = threadIdx.x;
+    for (long long g = blockIdx.x; g < a.n_group; g += gridDim.x) {
+        const int gn = a.groups[g];
+        const GNode G = a.nodes[gn];
+        double bin[6], bout[6];
+        for (int d = 0; d < 6; d++) {
+            bin[d] = a.box_in[6 * (long long)gn + d];
+            bout[d] = a.box_out[6 * (long long)gn + d];
+        }
+        long long ne = 0, ns = 0;
+        int *oe = nullptr, *os = nullptr;
+        if (WRITE) {
+            oe = a.id_epj + a.off_e[g];
+            os = a.id_spj + a.off_s[g];
+        }
+        if (lane == 0) stk[0] = 0;
+        int size = 1;
+        __syncthreads();
+        while (size > 0) {
+            // pop k nodes; the stack bound keeps WALK_STACK - 160 free for a
+            // sequential descent (each pop adds at most 7 entries per level)
+            int k = (WALK_STACK - 160 - size) / 8;
+            k = max(1, min(64, min(k, size)));
+            const bool act = lane < k;
+            int c = 0;
+            if (act) c = stk[size - 1 - lane];
+            __syncthreads();
+            long long me = 0, mc = 0, ms = 0;
+            GNode nd{0, 0, 0, 0, 0};
+            if (act) {
+                nd = a.nodes[c];
+                bool open;
+                if (nd.begin <= G.begin && G.end <= nd.end) open = true;
+                else if (!a.theta_pos) open = true;
+                else {
+                    const double L = a.root_len / (double)(1LL << nd.level);
+                    double s = 0;
+                    for (int d = 0; d < 3; d++) {
+                        const double p = a.ncom[3 * (long long)c + d];
+                        double e = 0;
+                        if (p < bin[d]) e = bin[d] - p;
+                        else if (p > bin[3 + d]) e = p - bin[3 + d];
+                        s += e * e;
+                    }
+                    open = s < L * L * a.inv_theta2 || overlap6(bout, a.box_in + 6 * (long long)c) ||
+                           overlap6(bin, a.box_out + 6 * (long long)c);
+                }
+                if (open) {
+                    if (nd.n_child == 0) me = nd.end - nd.begin;
+                    else mc = nd.n_child;
+                } else {
+                    ms = 1;
+                }
+            }
+            const long long pe = wave_scan(me, lane), pc = wave_scan(mc, lane), ps = wave_scan(ms, lane);
+            const long long te = __shfl(pe, 63, 64), tc = __shfl(pc, 63, 64), ts = __shfl(ps, 63, 64);
+            if (WRITE) {
+                for (long long t = 0; t < me; t++) oe[ne + pe - me + t] = (int)(nd.begin + t);
+                if (ms) os[ns + ps - 1] = c;
+            }
+            ne += te;
+            ns += ts;
+            size -= k;
+            for (long long t = 0; t < mc; t++) stk[size + (pc - mc) + t] = (int)(nd.first_child + (mc - 1 - t));
+            size += (int)tc;
+            __syncthreads();
+        }
+        if (!WRITE && lane == 0) {
+            a.cnt_e[g] = ne;
+            a.cnt_s[g] = ns;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct pth_gtree {
+    pth_ctx *ctx = nullptr;
+    long long n = 0, n_node = 0, n_group = 0, n_le = 0, n_ls = 0;
+    DevBuf order, key, skey, sidx, spos, smass, srs, nodes, nmass, ncom, nquad, box_in, box_out, groups, off_e, off_s,
+        id_epj, id_spj, tmp;
+    // build-time scratch, released when the build ends (or by pth_gtree_free)
+    DevBuf dpos, dmass, drs, part, bnd, cnt, off, flag, gsel, nsel, gkey, gkey2, ce, cs;
+    void release_scratch()
+    {
+        DevBuf *s[] = {&dpos, &dmass, &drs, &part, &bnd, &cnt, &off, &flag, &gsel, &nsel, &gkey, &gkey2, &ce, &cs};
+        for (DevBuf *b : s) b->release();
+    }
+    std::vector<long long> level_start;
+    float ms_build = 0.f, ms_walk = 0.f;
+};
+
+namespace {
+
+#define GT_CHK(call, what)                               \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return t->ctx->hip_fail(e_, what); \
+    } while (0)
+
+inline unsigned blocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+// temp storage for a hipcub call, grown on demand
+hipError_t cub_temp(pth_gtree *t, size_t bytes) { return t->tmp.ensure(std::max<size_t>(bytes, 256)); }
+
+}  // namespace
+
+extern "C" {
+
+int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *pos, const double *mass,
+                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta)
+{
+    if (!ctx) return PTH_ERR_ARG;
+    if (!out) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: null output");
+    *out = nullptr;
+    if (n <= 0 || n > 0x7fffffffLL) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: n outside [1, 2^31)");
+    if (!pos || !mass) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: null particle arrays");
+    if (n_leaf_limit < 1 || n_group_limit < 1) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: limits must be >= 1");
+    HIPCHK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    pth_gtree *t = new pth_gtree;
+    t->ctx = ctx;
+    t->n = n;
+    auto bail = [&](int rc) {
+        pth_gtree_free(t);
+        return rc;
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = ctx->stream;
+    hipError_t e;
+    // particles in (unsorted), then sorted copies
+    DevBuf &dpos = t->dpos, &dmass = t->dmass, &drs = t->drs;
+    auto up = [&](DevBuf &b, const double *h, size_t cnt) -> hipError_t {
+        hipError_t r = b.ensure(sizeof(double) * cnt);
+        if (r == hipSuccess) r = hipMemcpyAsync(b.p, h, sizeof(double) * cnt, hipMemcpyHostToDevice, st);
+        return r;
+    };
+    if ((e = up(dpos, pos, 3 * (size_t)n)) || (e = up(dmass, mass, (size_t)n)) ||
+        (r_search && (e = up(drs, r_search, (size_t)n))))
+        return bail(ctx->hip_fail(e, "pth_gtree_build: upload"));
+    // 1. bounding cube (per-block partials reduced on the host)
+    const int nb = (int)std::min<long long>(1024, (n + 255) / 256);
+    DevBuf &part = t->part;
+    std::vector<double> hp(6 * (size_t)nb);
+    if ((e = part.ensure(sizeof(double) * 6 * nb))) return bail(ctx->hip_fail(e, "bbox"));
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, dpos.as<double>(), n, part.as<double>());
+    if ((e = hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return bail(ctx->hip_fail(e, "bbox"));
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int b = 0; b < nb; b++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], hp[6 * b + d]);
+            hi[d] = std::max(hi[d], hp[6 * b + 3 + d]);
+        }
+    double len = 0;
+    for (int d = 0; d < 3; d++) len = std::max(len, hi[d] - lo[d]);
+    len = (len > 0) ? len * (1.0 + 1e-9) : 1.0;
+    double root[3];
+    for (int d = 0; d < 3; d++) root[d] = 0.5 * (lo[d] + hi[d]) - 0.5 * len;
+    const double scale = (double)(1u << GT_MAX_LEVEL) / len;
+    // 2. keys, stable sort of (key, index), sorted particles
+    if ((e = t->key.ensure(8 * (size_t)n)) || (e = t->skey.ensure(8 * (size_t)n)) ||
+        (e = t->sidx.ensure(8 * (size_t)n)) || (e = t->order.ensure(8 * (size_t)n)))
+        return bail(ctx->hip_fail(e, "key allocation"));
+    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(256), 0, st, dpos.as<double>(), n, root[0], root[1], root[2],
+                       scale, t->key.as<unsigned long long>(), t->sidx.as<long long>());
+    size_t tb = 0;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, t->key.as<unsigned long long>(),
+                                                t->skey.as<unsigned long long>(), t->sidx.as<long long>(),
+                                                t->order.as<long long>(), n, 0, 3 * GT_MAX_LEVEL, st)) ||
+        (e = cub_temp(t, tb)) ||
+        (e = hipcub::DeviceRadixSort::SortPairs(t->tmp.p, tb, t->key.as<unsigned long long>(),
+                                                t->skey.as<unsigned long long>(), t->sidx.as<long long>(),
+                                                t->order.as<long long>(), n, 0, 3 * GT_MAX_LEVEL, st)))
+        return bail(ctx->hip_fail(e, "key sort"));
+    if ((e = t->spos.ensure(24 * (size_t)n)) || (e = t->smass.ensure(8 * (size_t)n)) ||
+        (e = t->srs.ensure(8 * (size_t)n)))
+        return bail(ctx->hip_fail(e, "particle allocation"));
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(256), 0, st, n, t->order.as<long long>(), dpos.as<double>(),
+                       dmass.as<double>(), r_search ? drs.as<double>() : nullptr, t->spos.as<double>(),
+                       t->smass.as<double>(), t->srs.as<double>());
+    // 3. nodes level by level
+    long long cap = std::max<long long>(2 * n / std::max(n_leaf_limit, 1) + 64, 1024);
+    if ((e = t->nodes.ensure(sizeof(GNode) * cap))) return bail(ctx->hip_fail(e, "node allocation"));
+    {
+        const GNode root_node{0, n, 0, 0, -1};
+        if ((e = hipMemcpyAsync(t->nodes.p, &root_node, sizeof(GNode), hipMemcpyHostToDevice, st)) ||
+            (e = hipStreamSynchronize(st)))
+            return bail(ctx->hip_fail(e, "node upload"));
+    }
+    t->level_start.assign(1, 0);
+    long long lb = 0, le = 1;
+    DevBuf &bnd = t->bnd, &cnt = t->cnt, &off = t->off;
+    for (;;) {
+        t->level_start.push_back(le);
+        const long long nl = le - lb;
+        if ((e = bnd.ensure(sizeof(long long) * 9 * nl)) || (e = cnt.ensure(sizeof(int) * (nl + 1))) ||
+            (e = off.ensure(sizeof(int) * (nl + 1))))
+            return bail(ctx->hip_fail(e, "level buffers"));
+        hipLaunchKernelGGL(k_split_count, dim3(blocks(nl)), dim3(256), 0, st, t->nodes.as<GNode>(), lb, nl,
+                           t->skey.as<unsigned long long>(), n_leaf_limit, bnd.as<long long>(), cnt.as<int>());
+        if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<int>(), off.as<int>(), (int)nl, st)) ||
+            (e = cub_temp(t, tb)) ||
+            (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, cnt.as<int>(), off.as<int>(), (int)nl, st)))
+            return bail(ctx->hip_fail(e, "level scan"));
+        int last[2] = {0, 0};
+        if ((e = hipMemcpyAsync(&last[0], cnt.as<int>() + nl - 1, sizeof(int), hipMemcpyDeviceToHost, st)) ||
+            (e = hipMemcpyAsync(&last[1], off.as<int>() + nl - 1, sizeof(int), hipMemcpyDeviceToHost, st)) ||
+            (e = hipStreamSynchronize(st)))
+            return bail(ctx->hip_fail(e, "level size"));
+        const long long total = (long long)last[0] + last[1];
+        if (total == 0) break;
+        if (le + total > cap) {
+            // grow the node array, keeping its contents
+            const long long ncap = std::max(2 * cap, le + total);
+            DevBuf nb2;
+            if ((e = nb2.ensure(sizeof(GNode) * ncap)) ||
+                (e = hipMemcpyAsync(nb2.p, t->nodes.p, sizeof(GNode) * le, hipMemcpyDeviceToDevice, st)) ||
+                (e = hipStreamSynchronize(st)))
+                return bail(ctx->hip_fail(e, "node growth"));
+            t->nodes.release();
+            t->nodes = nb2;
+            cap = ncap;
+        }
+        hipLaunchKernelGGL(k_split_write, dim3(blocks(nl)), dim3(256), 0, st, t->nodes.as<GNode>(), lb, nl, le,
+                           bnd.as<long long>(), cnt.as<int>(), off.as<int>());
+        lb = le;
+        le += total;
+    }
+    t->n_node = le;
+    const long long nn = le;
+    // 4. moments bottom-up
+    if ((e = t->nmass.ensure(8 * nn)) || (e = t->ncom.ensure(24 * nn)) || (e = t->nquad.ensure(48 * nn)) ||
+        (e = t->box_in.ensure(48 * nn)) || (e = t->box_out.ensure(48 * nn)))
+        return bail(ctx->hip_fail(e, "moment allocation"));
+    for (int l = (int)t->level_start.size() - 2; l >= 0; l--) {
+        const long long a = t->level_start[l], b = t->level_start[l + 1];
+        if (b <= a) continue;
+        hipLaunchKernelGGL(k_moments, dim3(blocks(b - a)), dim3(256), 0, st, t->nodes.as<GNode>(), a, b,
+                           t->spos.as<double>(), t->smass.as<double>(), t->srs.as<double>(), t->nmass.as<double>(),
+                           t->ncom.as<double>(), t->nquad.as<double>(), t->box_in.as<double>(),
+                           t->box_out.as<double>());
+    }
+    // 5. groups in Morton order
+    DevBuf &flag = t->flag, &gsel = t->gsel, &nsel = t->nsel, &gkey = t->gkey, &gkey2 = t->gkey2;
+    if ((e = flag.ensure(nn)) || (e = gsel.ensure(sizeof(int) * nn)) || (e = nsel.ensure(sizeof(int))))
+        return bail(ctx->hip_fail(e, "group allocation"));
+    hipLaunchKernelGGL(k_group_flags, dim3(blocks(nn)), dim3(256), 0, st, t->nodes.as<GNode>(), nn, n_group_limit,
+                       flag.as<char>());
+    hipLaunchKernelGGL(k_group_children, dim3(blocks(nn)), dim3(256), 0, st, t->nodes.as<GNode>(), nn, n_group_limit,
+                       flag.as<char>());
+    {
+        hipcub::CountingInputIterator<int> it(0);
+        if ((e = hipcub::DeviceSelect::Flagged(nullptr, tb, it, flag.as<char>(), gsel.as<int>(), nsel.as<int>(),
+                                               (int)nn, st)) ||
+            (e = cub_temp(t, tb)) ||
+            (e = hipcub::DeviceSelect::Flagged(t->tmp.p, tb, it, flag.as<char>(), gsel.as<int>(), nsel.as<int>(),
+                                               (int)nn, st)))
+            return bail(ctx->hip_fail(e, "group selection"));
+    }
+    int ngi = 0;
+    if ((e = hipMemcpyAsync(&ngi, nsel.p, sizeof(int), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
+        return bail(ctx->hip_fail(e, "group count"));
+    const long long ng = ngi;
+    t->n_group = ng;
+    if ((e = gkey.ensure(8 * ng)) || (e = gkey2.ensure(8 * ng)) || (e = t->groups.ensure(sizeof(int) * ng)))
+        return bail(ctx->hip_fail(e, "group allocation"));
+    hipLaunchKernelGGL(k_group_keys, dim3(blocks(ng)), dim3(256), 0, st, t->nodes.as<GNode>(), gsel.as<int>(), ng,
+                       gkey.as<long long>());
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, gkey.as<long long>(), gkey2.as<long long>(),
+                                                gsel.as<int>(), t->groups.as<int>(), (int)ng, 0, 32, st)) ||
+        (e = cub_temp(t, tb)) ||
+        (e = hipcub::DeviceRadixSort::SortPairs(t->tmp.p, tb, gkey.as<long long>(), gkey2.as<long long>(),
+                                                gsel.as<int>(), t->groups.as<int>(), (int)ng, 0, 32, st)))
+        return bail(ctx->hip_fail(e, "group sort"));
+    if ((e = hipStreamSynchronize(st))) return bail(ctx->hip_fail(e, "tree build"));
+    const auto t1 = std::chrono::steady_clock::now();
+    t->ms_build = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    // 6. walk: count, scan, write
+    WalkArgs wa{};
+    wa.nodes = t->nodes.as<GNode>();
+    wa.ncom = t->ncom.as<double>();
+    wa.box_in = t->box_in.as<double>();
+    wa.box_out = t->box_out.as<double>();
+    wa.groups = t->groups.as<int>();
+    wa.n_group = ng;
+    wa.root_len = len;
+    wa.inv_theta2 = (theta > 0) ? 1.0 / (theta * theta) : 0.0;
+    wa.theta_pos = theta > 0;
+    DevBuf &ce = t->ce, &cs = t->cs;
+    if ((e = ce.ensure(8 * (ng + 1))) || (e = cs.ensure(8 * (ng + 1))) || (e = t->off_e.ensure(8 * (ng + 1))) ||
+        (e = t->off_s.ensure(8 * (ng + 1))))
+        return bail(ctx->hip_fail(e, "walk allocation"));
+    wa.cnt_e = ce.as<long long>();
+    wa.cnt_s = cs.as<long long>();
+    const unsigned wg = (unsigned)std::min<long long>(ng, 65536);
+    if (ng > 0) hipLaunchKernelGGL(k_walk<false>, dim3(wg), dim3(64), 0, st, wa);
+    long long zero = 0;
+    if ((e = hipMemcpyAsync(ce.as<long long>() + ng, &zero, 8, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(cs.as<long long>() + ng, &zero, 8, hipMemcpyHostToDevice, st)))
+        return bail(ctx->hip_fail(e, "walk counts"));
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ce.as<long long>(), t->off_e.as<long long>(), (int)ng + 1,
+                                              st)) ||
+        (e = cub_temp(t, tb)) ||
+        (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, ce.as<long long>(), t->off_e.as<long long>(), (int)ng + 1,
+                                              st)) ||
+        (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, cs.as<long long>(), t->off_s.as<long long>(), (int)ng + 1,
+                                              st)))
+        return bail(ctx->hip_fail(e, "walk scan"));
+    long long tot[2] = {0, 0};
+    if ((e = hipMemcpyAsync(&tot[0], t->off_e.as<long long>() + ng, 8, hipMemcpyDeviceToHost, st)) ||
+        (e = hipMemcpyAsync(&tot[1], t->off_s.as<long long>() + ng, 8, hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
+        return bail(ctx->hip_fail(e, "walk totals"));
+    if (tot[0] > 0x7fffffffLL || tot[1] > 0x7fffffffLL)
+        return bail(ctx->fail(PTH_ERR_CAPACITY, "pth_gtree_build: lists exceed the int32 range"));
+    t->n_le = tot[0];
+    t->n_ls = tot[1];
+    if ((e = t->id_epj.ensure(4 * std::max<long long>(tot[0], 1))) ||
+        (e = t->id_spj.ensure(4 * std::max<long long>(tot[1], 1))))
+        return bail(ctx->hip_fail(e, "list allocation"));
+    wa.off_e = t->off_e.as<long long>();
+    wa.off_s = t->off_s.as<long long>();
+    wa.id_epj = t->id_epj.as<int>();
+    wa.id_spj = t->id_spj.as<int>();
+    if (ng > 0) hipLaunchKernelGGL(k_walk<true>, dim3(wg), dim3(64), 0, st, wa);
+    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(ctx->hip_fail(e, "tree walk"));
+    t->ms_walk = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    t->release_scratch();
+    *out = t;
+    return PTH_OK;
+}
+
+int pth_gtree_sizes(const pth_gtree *t, long long *n_node, long long *n_group, long long *n_epj_list,
+                    long long *n_spj_list)
+{
+    if (!t) return PTH_ERR_ARG;
+    if (n_node) *n_node = t->n_node;
+    if (n_group) *n_group = t->n_group;
+    if (n_epj_list) *n_epj_list = t->n_le;
+    if (n_spj_list) *n_spj_list = t->n_ls;
+    return PTH_OK;
+}
+
+int pth_gtree_download(const pth_gtree *t, long long *order, long long *node_begin, long long *node_end,
+                       int *node_level, double *node_mass, double *node_com, double *node_quad, long long *epi_begin,
+                       int *n_epi, int *n_epj, int *n_spj, int *id_epj, int *id_spj)
+{
+    if (!t) return PTH_ERR_ARG;
+    pth_ctx *c = t->ctx;
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    auto d2h = [&](void *h, const DevBuf &b, size_t bytes) -> hipError_t {
+        return (h && bytes) ? hipMemcpy(h, b.p, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+    };
+    const long long nn = t->n_node, ng = t->n_group;
+    HIPCHK(c, d2h(order, t->order, 8 * t->n), "download");
+    HIPCHK(c, d2h(node_mass, t->nmass, 8 * nn), "download");
+    HIPCHK(c, d2h(node_com, t->ncom, 24 * nn), "download");
+    HIPCHK(c, d2h(node_quad, t->nquad, 48 * nn), "download");
+    HIPCHK(c, d2h(id_epj, t->id_epj, 4 * t->n_le), "download");
+    HIPCHK(c, d2h(id_spj, t->id_spj, 4 * t->n_ls), "download");
+    std::vector<GNode> nodes(nn);
+    std::vector<int> groups(ng);
+    std::vector<long long> oe(ng + 1), os(ng + 1);
+    HIPCHK(c, d2h(nodes.data(), t->nodes, sizeof(GNode) * nn), "download");
+    HIPCHK(c, d2h(groups.data(), t->groups, 4 * ng), "download");
+    HIPCHK(c, d2h(oe.data(), t->off_e, 8 * (ng + 1)), "download");
+    HIPCHK(c, d2h(os.data(), t->off_s, 8 * (ng + 1)), "download");
+    for (long long k = 0; k < nn; k++) {
+        if (node_begin) node_begin[k] = nodes[k].begin;
+        if (node_end) node_end[k] = nodes[k].end;
+        if (node_level) node_level[k] = nodes[k].level;
+    }
+    for (long long g = 0; g < ng; g++) {
+        const GNode &G = nodes[groups[g]];
+        if (epi_begin) epi_begin[g] = G.begin;
+        if (n_epi) n_epi[g] = (int)(G.end - G.begin);
+        if (n_epj) n_epj[g] = (int)(oe[g + 1] - oe[g]);
+        if (n_spj) n_spj[g] = (int)(os[g + 1] - os[g]);
+    }
+    return PTH_OK;
+}
+
+int pth_gtree_timing(const pth_gtree *t, float *ms_build, float *ms_walk)
+{
+    if (!t) return PTH_ERR_ARG;
+    if (ms_build) *ms_build = t->ms_build;
+    if (ms_walk) *ms_walk = t->ms_walk;
+    return PTH_OK;
+}
+
+void pth_gtree_free(pth_gtree *t)
+{
+    if (!t) return;
+    if (t->ctx && t->ctx->device >= 0) {
+        (void)hipSetDevice(t->ctx->device);
+        if (t->ctx->stream) (void)hipStreamSynchronize(t->ctx->stream);
+    }
+    DevBuf *all[] = {&t->order, &t->key, &t->skey, &t->sidx, &t->spos, &t->smass, &t->srs, &t->nodes, &t->nmass,
+                     &t->ncom, &t->nquad, &t->box_in, &t->box_out, &t->groups, &t->off_e, &t->off_s, &t->id_epj,
+                     &t->id_spj, &t->tmp};
+    for (DevBuf *b : all) b->release();
+    t->release_scratch();
+    delete t;
+}
+
+}  // extern "C"

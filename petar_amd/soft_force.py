@@ -79,7 +79,8 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_er
             "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve", "pth_resident_create",
             "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
             "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
-            "pth_resident_replan", "pth_resident_autotune",
+            "pth_resident_replan", "pth_resident_autotune", "pth_gtree_build", "pth_gtree_sizes",
+            "pth_gtree_download", "pth_gtree_timing", "pth_gtree_free",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
 
@@ -129,6 +130,12 @@ def lib():
     L.pth_server_free.argtypes = [vp]
     L.pth_resident_replan.argtypes = [vp, C.c_int, C.c_int]
     L.pth_resident_autotune.argtypes = [vp, ip, C.c_int, ip, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_float)]
+    llp = C.POINTER(C.c_longlong)
+    L.pth_gtree_build.argtypes = [vp, C.POINTER(vp), C.c_longlong, dp, dp, dp, C.c_int, C.c_int, C.c_double]
+    L.pth_gtree_sizes.argtypes = [vp, llp, llp, llp, llp]
+    L.pth_gtree_download.argtypes = [vp, llp, llp, llp, ip, dp, dp, dp, llp, ip, ip, ip, ip, ip]
+    L.pth_gtree_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.pth_gtree_free.argtypes = [vp]
     L.pth_plan_debug.restype = C.c_longlong
     L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
                                  C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), ip, C.c_longlong,
@@ -404,6 +411,61 @@ class Resident:
     def close(self):
         if self._h:
             lib().pth_resident_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuTree:
+    """Octree, moments, i-groups and interaction lists built on the device
+    (pth_gtree_*), with the semantics of petar_amd.host.build_lists."""
+
+    def __init__(self, ctx: Context, pos, mass, r_search=None, theta=0.3, n_leaf_limit=20, n_group_limit=512):
+        self.ctx = ctx
+        pos = _f64(pos).reshape(-1, 3)
+        self.n = pos.shape[0]
+        mass = _f64(mass)
+        rs = None if r_search is None else _f64(np.broadcast_to(r_search, (self.n,)))
+        self._h = C.c_void_p()
+        ctx.check(lib().pth_gtree_build(ctx.handle, C.byref(self._h), self.n, _ptr(pos, C.c_double),
+                                        _ptr(mass, C.c_double), None if rs is None else _ptr(rs, C.c_double),
+                                        n_leaf_limit, n_group_limit, theta), "pth_gtree_build")
+
+    def sizes(self):
+        v = [C.c_longlong() for _ in range(4)]
+        self.ctx.check(lib().pth_gtree_sizes(self._h, *[C.byref(x) for x in v]), "pth_gtree_sizes")
+        return dict(zip(("n_node", "n_group", "n_epj_list", "n_spj_list"), (x.value for x in v)))
+
+    def download(self):
+        z = self.sizes()
+        nn, ng = z["n_node"], z["n_group"]
+        d = dict(order=np.zeros(self.n, np.int64), node_begin=np.zeros(nn, np.int64), node_end=np.zeros(nn, np.int64),
+                 node_level=np.zeros(nn, np.int32), node_mass=np.zeros(nn), node_com=np.zeros((nn, 3)),
+                 node_quad=np.zeros((nn, 6)), epi_begin=np.zeros(ng, np.int64), n_epi=np.zeros(ng, np.int32),
+                 n_epj=np.zeros(ng, np.int32), n_spj=np.zeros(ng, np.int32),
+                 id_epj=np.zeros(max(z["n_epj_list"], 1), np.int32), id_spj=np.zeros(max(z["n_spj_list"], 1), np.int32))
+        ll, i32, f = C.c_longlong, C.c_int, C.c_double
+        self.ctx.check(lib().pth_gtree_download(
+            self._h, _ptr(d["order"], ll), _ptr(d["node_begin"], ll), _ptr(d["node_end"], ll),
+            _ptr(d["node_level"], i32), _ptr(d["node_mass"], f), _ptr(d["node_com"], f), _ptr(d["node_quad"], f),
+            _ptr(d["epi_begin"], ll), _ptr(d["n_epi"], i32), _ptr(d["n_epj"], i32), _ptr(d["n_spj"], i32),
+            _ptr(d["id_epj"], i32), _ptr(d["id_spj"], i32)), "pth_gtree_download")
+        d["id_epj"] = d["id_epj"][:z["n_epj_list"]]
+        d["id_spj"] = d["id_spj"][:z["n_spj_list"]]
+        return d
+
+    def timing(self):
+        a, b = C.c_float(), C.c_float()
+        self.ctx.check(lib().pth_gtree_timing(self._h, C.byref(a), C.byref(b)), "pth_gtree_timing")
+        return a.value, b.value
+
+    def close(self):
+        if self._h:
+            lib().pth_gtree_free(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
