@@ -261,11 +261,15 @@ void pth_server_free(pth_server *srv);
  * particles) and every group's EPJ / SPJ lists (theta opening with the
  * symmetric r_search test) with the semantics of pth_tree_build +
  * pth_tree_walk of include/petar_host.h: the same nodes, moments, groups
- * and, per group, the same list entries (in another order).  Positions,
- * masses and r_search (may be NULL: 0) are host arrays [n][3], [n], [n]. */
+ * and, per group, the same list entries.  With canonical != 0 every list is
+ * put in the host walk's order (EPJ entries ascending, SPJ entries by the
+ * cell's first particle: a depth-first walk's order), so the lists equal the
+ * host lists entry for entry; with 0 the order inside a list is the walk's
+ * batch order.  Positions, masses and r_search (may be NULL: 0) are host
+ * arrays [n][3], [n], [n]. */
 typedef struct pth_gtree pth_gtree;
 int pth_gtree_build(pth_ctx *ctx, pth_gtree **tree, long long n, const double *pos, const double *mass,
-                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta);
+                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta, int canonical);
 int pth_gtree_sizes(const pth_gtree *tree, long long *n_node, long long *n_group, long long *n_epj_list,
                     long long *n_spj_list);
 /* Copies the tree to the host (any pointer may be NULL): order [n] (sorted
@@ -276,10 +280,20 @@ int pth_gtree_sizes(const pth_gtree *tree, long long *n_node, long long *n_group
 int pth_gtree_download(const pth_gtree *tree, long long *order, long long *node_begin, long long *node_end,
                        int *node_level, double *node_mass, double *node_com, double *node_quad,
                        long long *epi_begin, int *n_epi, int *n_epj, int *n_spj, int *id_epj, int *id_spj);
-/* Host wall time of the last build (ms): tree (sort, nodes, moments, groups)
- * and walk (lists). */
-int pth_gtree_timing(const pth_gtree *tree, float *ms_build, float *ms_walk);
+/* Host wall time of the last build (ms): tree (upload, sort, nodes, moments,
+ * groups), walk (count, scan, write) and the canonical ordering (0 when not
+ * asked for). */
+int pth_gtree_timing(const pth_gtree *tree, float *ms_build, float *ms_walk, float *ms_order);
 void pth_gtree_free(pth_gtree *tree);
+/* A resident evaluation (as pth_resident_create) fed from the tree without a
+ * host round trip: walk w = group w, EPI = the particles in tree order, EPJ
+ * records = the particles in tree order (FP64 position, mass, r_search),
+ * SPJ records = the nodes (centre of mass, mass, raw second moment).  The
+ * tree may be freed afterwards.  In changeover mode call
+ * pth_resident_changeover with arrays in tree order (pth_gtree_download's
+ * order maps them).  With a canonical tree the results equal those of
+ * pth_resident_create on the host tree's lists bit for bit. */
+int pth_resident_from_gtree(pth_ctx *ctx, pth_resident **res, const pth_gtree *tree);
 
 /* ---- diagnostics (host only, no device needed) ----------------------------- */
 /* The work decomposition pth_dispatch / pth_resident_create use for these walk

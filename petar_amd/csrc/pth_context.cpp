@@ -463,53 +463,17 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
     for (long long k = 0; k < n_ids; k++)
         if (id_spj[k] < 0 || id_spj[k] >= n_spj_tot)
             return c->fail(PTH_ERR_ARG, "pth_resident_create: SPJ index out of range");
-    pth_resident *r = new pth_resident;
-    r->ctx = c;
-    r->n_seg = (c->par.exec_mode == PTH_EXEC_FUSED) ? c->par.segments : 1;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
-                   r->n_seg)) {
-        delete r;
-        return c->fail(PTH_ERR_CAPACITY, "pth_resident_create: sizes exceed the int32 index range");
-    }
+    pth_resident *r = nullptr;
+    int rc = resident_prepare(c, r, n_walk, n_epi, n_epj, n_spj, n_ide, n_ids, n_epj_tot, n_spj_tot,
+                              "pth_resident_create");
+    if (rc != PTH_OK) return rc;
     const Plan &P = r->plan;
-    r->co = c->par.changeover != 0;
-    if (r->co) {
-        const char *m = co_plan_error(P);
-        if (m) {
-            delete r;
-            return c->fail(PTH_ERR_CAPACITY, std::string("pth_resident_create: ") + m);
-        }
-    }
-    r->A.layout(P, n_ide, n_ids, r->co);
     auto bail = [&](hipError_t e, const char *what) {
-        int rc = c->hip_fail(e, what);
+        int rc2 = c->hip_fail(e, what);
         pth_resident_free(r);
-        return rc;
+        return rc2;
     };
     hipError_t e;
-    if ((e = hipSetDevice(c->device))) return bail(e, "hipSetDevice");
-    if ((e = r->arena.ensure(r->A.total))) return bail(e, "resident arena");
-    if ((e = r->epj.ensure(sizeof(EpjDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
-    if ((e = r->spj.ensure(sizeof(SpjDev) * std::max(n_spj_tot, 1)))) return bail(e, "resident SPJ");
-    if ((e = r->out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
-    if ((e = r->out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
-    if ((e = r->work.ensure(P, r->co))) return bail(e, "resident work buffers");
-    if (r->co) {
-        if ((e = r->epj_co.ensure(sizeof(EpjCoDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
-        if ((e = r->out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
-    }
-    for (int k = 0; k < 4; k++)
-        if ((e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
-    if ((int)P.segs.size() > 1) {
-        const int ns = std::min(c->par.streams, (int)P.segs.size());
-        r->side.assign(ns, nullptr);
-        r->side_done.assign(ns, nullptr);
-        for (int k = 0; k < ns; k++) {
-            if ((e = hipStreamCreateWithFlags(&r->side[k], hipStreamNonBlocking))) return bail(e, "hipStreamCreate");
-            if ((e = hipEventCreateWithFlags(&r->side_done[k], hipEventDisableTiming)))
-                return bail(e, "hipEventCreate");
-        }
-    }
     // host mirrors (pageable is fine: uploaded once)
     std::vector<char> h(r->A.total);
     EpiDev *he = reinterpret_cast<EpiDev *>(h.data() + r->A.epi);
@@ -551,13 +515,7 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
         return bail(e, "resident upload");
     if (n_spj_tot && (e = hipMemcpy(r->spj.p, sj.data(), sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice)))
         return bail(e, "resident upload");
-    fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
-               r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(),
-               r->out_pott.as<double>());
-    r->n_epj_tot = n_epj_tot;
-    r->max_ipl = c->par.max_ipl;
-    r->chunk_j = c->par.chunk_j;
-    r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 48 * P.n_s_rec);
+    resident_finish(r);
     *out = r;
     return PTH_OK;
 }
@@ -846,3 +804,76 @@ long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const i
 }
 
 }  // extern "C"
+
+namespace pth {
+
+int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_epi, const int *n_epj,
+                     const int *n_spj, long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot,
+                     const char *who)
+{
+    out = nullptr;
+    pth_resident *r = new pth_resident;
+    r->ctx = c;
+    r->n_seg = (c->par.exec_mode == PTH_EXEC_FUSED) ? c->par.segments : 1;
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
+                   r->n_seg)) {
+        delete r;
+        return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
+    }
+    const Plan &P = r->plan;
+    r->co = c->par.changeover != 0;
+    if (r->co) {
+        const char *m = co_plan_error(P);
+        if (m) {
+            delete r;
+            return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": " + m);
+        }
+    }
+    r->A.layout(P, n_ide, n_ids, r->co);
+    auto bail = [&](hipError_t e, const char *what) {
+        int rc = c->hip_fail(e, what);
+        pth_resident_free(r);
+        return rc;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(c->device))) return bail(e, "hipSetDevice");
+    if ((e = r->arena.ensure(r->A.total))) return bail(e, "resident arena");
+    if ((e = r->epj.ensure(sizeof(EpjDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
+    if ((e = r->spj.ensure(sizeof(SpjDev) * std::max(n_spj_tot, 1)))) return bail(e, "resident SPJ");
+    if ((e = r->out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
+    if ((e = r->out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
+    if ((e = r->work.ensure(P, r->co))) return bail(e, "resident work buffers");
+    if (r->co) {
+        if ((e = r->epj_co.ensure(sizeof(EpjCoDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
+        if ((e = r->out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
+    }
+    for (int k = 0; k < 4; k++)
+        if ((e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
+    if ((int)P.segs.size() > 1) {
+        const int ns = std::min(c->par.streams, (int)P.segs.size());
+        r->side.assign(ns, nullptr);
+        r->side_done.assign(ns, nullptr);
+        for (int k = 0; k < ns; k++) {
+            if ((e = hipStreamCreateWithFlags(&r->side[k], hipStreamNonBlocking))) return bail(e, "hipStreamCreate");
+            if ((e = hipEventCreateWithFlags(&r->side_done[k], hipEventDisableTiming)))
+                return bail(e, "hipEventCreate");
+        }
+    }
+    r->n_epj_tot = n_epj_tot;
+    out = r;
+    return PTH_OK;
+}
+
+void resident_finish(pth_resident *r)
+{
+    const Plan &P = r->plan;
+    pth_ctx *c = r->ctx;
+    fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
+               r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(),
+               r->out_pott.as<double>());
+    r->max_ipl = c->par.max_ipl;
+    r->chunk_j = c->par.chunk_j;
+    r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 48 * P.n_s_rec);
+}
+
+}  // namespace pth

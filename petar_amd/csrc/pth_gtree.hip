@@ -20,9 +20,11 @@
 //      L/theta, or when the boxes overlap with either side widened by
 //      r_search; opened leaves give their particles (EPJ), closed nodes give
 //      one SPJ.  A counting pass, a scan, then a writing pass.
+//   6. optionally (canonical order) a segmented sort of every list into the
+//      host walk's depth-first order: EPJ ascending, SPJ by first particle.
 // Every decision is the same FP64 expression as on the host (no FMA
-// contraction in this file), so the lists are the host's lists as sets (the
-// order inside a list differs: the walk pops nodes in batches).
+// contraction in this file), so the lists are the host's lists as sets, and
+// entry for entry after the canonical ordering.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -280,6 +282,75 @@ __global__ __launch_bounds__(256) void k_group_keys(const GNode *__restrict__ no
     gbeg[g] = nodes[gnode[g]].begin;
 }
 
+// per group: first particle, size, list lengths (host copy for the planner)
+__global__ __launch_bounds__(256) void k_group_info(const GNode *__restrict__ nodes, const int *__restrict__ groups,
+                                                    long long ng, const long long *__restrict__ off_e,
+                                                    const long long *__restrict__ off_s, int4 *__restrict__ info)
+{
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (g >= ng) return;
+    const GNode G = nodes[groups[g]];
+    info[g] = make_int4((int)G.begin, (int)(G.end - G.begin), (int)(off_e[g + 1] - off_e[g]),
+                        (int)(off_s[g + 1] - off_s[g]));
+}
+
+// sort key of an SPJ entry: the node's first particle
+__global__ __launch_bounds__(256) void k_spj_keys(const GNode *__restrict__ nodes, const int *__restrict__ id_spj,
+                                                  long long n, int *__restrict__ key)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    key[k] = (int)nodes[id_spj[k]].begin;
+}
+
+// resident records from the tree (the conversions of pth_resident_create)
+__global__ __launch_bounds__(256) void k_res_particles(long long n, const double *__restrict__ spos,
+                                                       const double *__restrict__ smass,
+                                                       const double *__restrict__ srs, pth::EpiDev *__restrict__ epi,
+                                                       pth::EpjDev *__restrict__ epj)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const double x = spos[3 * k], y = spos[3 * k + 1], z = spos[3 * k + 2];
+    const float rs = (float)srs[k];
+    pth::EpiDev e;
+    e.x = x;
+    e.y = y;
+    e.z = z;
+    e.rs = rs;
+    e.pad = 0;
+    epi[k] = e;
+    pth::EpjDev j;
+    j.x = x;
+    j.y = y;
+    j.z = z;
+    j.m = (float)smass[k];
+    j.rs = rs;
+    epj[k] = j;
+}
+
+__global__ __launch_bounds__(256) void k_res_nodes(long long nn, const double *__restrict__ nmass,
+                                                   const double *__restrict__ ncom, const double *__restrict__ nquad,
+                                                   pth::SpjDev *__restrict__ spj)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nn) return;
+    const double *q = nquad + 6 * k;
+    pth::SpjDev r;
+    r.x = ncom[3 * k];
+    r.y = ncom[3 * k + 1];
+    r.z = ncom[3 * k + 2];
+    r.m = (float)nmass[k];
+    r.qxx = (float)q[0];
+    r.qyy = (float)q[1];
+    r.qzz = (float)q[2];
+    r.qxy = (float)q[3];
+    r.qxz = (float)q[4];
+    r.qyz = (float)q[5];
+    r.pad[0] = r.pad[1] = r.pad[2] = 0.f;
+    spj[k] = r;
+}
+
 struct WalkArgs {
     const GNode *nodes;
     const double *ncom, *box_in, *box_out;
@@ -400,14 +471,17 @@ struct pth_gtree {
     DevBuf order, key, skey, sidx, spos, smass, srs, nodes, nmass, ncom, nquad, box_in, box_out, groups, off_e, off_s,
         id_epj, id_spj, tmp;
     // build-time scratch, released when the build ends (or by pth_gtree_free)
-    DevBuf dpos, dmass, drs, part, bnd, cnt, off, flag, gsel, nsel, gkey, gkey2, ce, cs;
+    DevBuf dpos, dmass, drs, part, bnd, cnt, off, flag, gsel, nsel, gkey, gkey2, ce, cs, alt, lkey, lkey2, info;
     void release_scratch()
     {
-        DevBuf *s[] = {&dpos, &dmass, &drs, &part, &bnd, &cnt, &off, &flag, &gsel, &nsel, &gkey, &gkey2, &ce, &cs};
+        DevBuf *s[] = {&dpos, &dmass, &drs, &part, &bnd, &cnt, &off, &flag, &gsel, &nsel, &gkey, &gkey2, &ce, &cs,
+                       &alt, &lkey, &lkey2, &info};
         for (DevBuf *b : s) b->release();
     }
     std::vector<long long> level_start;
-    float ms_build = 0.f, ms_walk = 0.f;
+    std::vector<int4> groups_h;  // per group: first particle, size, n_epj, n_spj
+    bool canonical = false;
+    float ms_build = 0.f, ms_walk = 0.f, ms_order = 0.f;
 };
 
 namespace {
@@ -428,7 +502,7 @@ hipError_t cub_temp(pth_gtree *t, size_t bytes) { return t->tmp.ensure(std::max<
 extern "C" {
 
 int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *pos, const double *mass,
-                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta)
+                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta, int canonical)
 {
     if (!ctx) return PTH_ERR_ARG;
     if (!out) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: null output");
@@ -645,7 +719,55 @@ int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *po
     wa.id_spj = t->id_spj.as<int>();
     if (ng > 0) hipLaunchKernelGGL(k_walk<true>, dim3(wg), dim3(64), 0, st, wa);
     if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(ctx->hip_fail(e, "tree walk"));
-    t->ms_walk = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    const auto t2 = std::chrono::steady_clock::now();
+    t->ms_walk = std::chrono::duration<float, std::milli>(t2 - t1).count();
+    // 7. canonical order: segmented sorts of the lists (one segment per group)
+    t->canonical = canonical != 0;
+    if (t->canonical && ng > 0) {
+        int bits = 1;
+        while (bits < 31 && (1LL << bits) < n) bits++;  // keys are particle positions < n
+        const long long *ob = t->off_e.as<long long>(), *os = t->off_s.as<long long>();
+        if (tot[0] > 0) {
+            if ((e = t->alt.ensure(4 * tot[0])) ||
+                (e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, t->id_epj.as<int>(), t->alt.as<int>(),
+                                                                (int)tot[0], (int)ng, ob, ob + 1, 0, bits, st)) ||
+                (e = cub_temp(t, tb)) ||
+                (e = hipcub::DeviceSegmentedRadixSort::SortKeys(t->tmp.p, tb, t->id_epj.as<int>(), t->alt.as<int>(),
+                                                                (int)tot[0], (int)ng, ob, ob + 1, 0, bits, st)) ||
+                (e = hipStreamSynchronize(st)))
+                return bail(ctx->hip_fail(e, "EPJ list ordering"));
+            std::swap(t->id_epj, t->alt);
+        }
+        if (tot[1] > 0) {
+            if ((e = t->alt.ensure(4 * tot[1])) || (e = t->lkey.ensure(4 * tot[1])) ||
+                (e = t->lkey2.ensure(4 * tot[1])))
+                return bail(ctx->hip_fail(e, "SPJ list ordering"));
+            hipLaunchKernelGGL(k_spj_keys, dim3(blocks(tot[1])), dim3(256), 0, st, t->nodes.as<GNode>(),
+                               t->id_spj.as<int>(), tot[1], t->lkey.as<int>());
+            if ((e = hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb, t->lkey.as<int>(), t->lkey2.as<int>(),
+                                                                 t->id_spj.as<int>(), t->alt.as<int>(), (int)tot[1],
+                                                                 (int)ng, os, os + 1, 0, bits, st)) ||
+                (e = cub_temp(t, tb)) ||
+                (e = hipcub::DeviceSegmentedRadixSort::SortPairs(t->tmp.p, tb, t->lkey.as<int>(), t->lkey2.as<int>(),
+                                                                 t->id_spj.as<int>(), t->alt.as<int>(), (int)tot[1],
+                                                                 (int)ng, os, os + 1, 0, bits, st)) ||
+                (e = hipStreamSynchronize(st)))
+                return bail(ctx->hip_fail(e, "SPJ list ordering"));
+            std::swap(t->id_spj, t->alt);
+        }
+    }
+    const auto t3 = std::chrono::steady_clock::now();
+    t->ms_order = t->canonical ? std::chrono::duration<float, std::milli>(t3 - t2).count() : 0.f;
+    // per-group sizes for the planner
+    t->groups_h.assign(ng, make_int4(0, 0, 0, 0));
+    if (ng > 0) {
+        if ((e = t->info.ensure(sizeof(int4) * ng))) return bail(ctx->hip_fail(e, "group sizes"));
+        hipLaunchKernelGGL(k_group_info, dim3(blocks(ng)), dim3(256), 0, st, t->nodes.as<GNode>(), t->groups.as<int>(),
+                           ng, t->off_e.as<long long>(), t->off_s.as<long long>(), t->info.as<int4>());
+        if ((e = hipMemcpyAsync(t->groups_h.data(), t->info.p, sizeof(int4) * ng, hipMemcpyDeviceToHost, st)) ||
+            (e = hipStreamSynchronize(st)))
+            return bail(ctx->hip_fail(e, "group sizes"));
+    }
     t->release_scratch();
     *out = t;
     return PTH_OK;
@@ -680,32 +802,91 @@ int pth_gtree_download(const pth_gtree *t, long long *order, long long *node_beg
     HIPCHK(c, d2h(id_epj, t->id_epj, 4 * t->n_le), "download");
     HIPCHK(c, d2h(id_spj, t->id_spj, 4 * t->n_ls), "download");
     std::vector<GNode> nodes(nn);
-    std::vector<int> groups(ng);
-    std::vector<long long> oe(ng + 1), os(ng + 1);
     HIPCHK(c, d2h(nodes.data(), t->nodes, sizeof(GNode) * nn), "download");
-    HIPCHK(c, d2h(groups.data(), t->groups, 4 * ng), "download");
-    HIPCHK(c, d2h(oe.data(), t->off_e, 8 * (ng + 1)), "download");
-    HIPCHK(c, d2h(os.data(), t->off_s, 8 * (ng + 1)), "download");
     for (long long k = 0; k < nn; k++) {
         if (node_begin) node_begin[k] = nodes[k].begin;
         if (node_end) node_end[k] = nodes[k].end;
         if (node_level) node_level[k] = nodes[k].level;
     }
     for (long long g = 0; g < ng; g++) {
-        const GNode &G = nodes[groups[g]];
-        if (epi_begin) epi_begin[g] = G.begin;
-        if (n_epi) n_epi[g] = (int)(G.end - G.begin);
-        if (n_epj) n_epj[g] = (int)(oe[g + 1] - oe[g]);
-        if (n_spj) n_spj[g] = (int)(os[g + 1] - os[g]);
+        const int4 G = t->groups_h[g];
+        if (epi_begin) epi_begin[g] = G.x;
+        if (n_epi) n_epi[g] = G.y;
+        if (n_epj) n_epj[g] = G.z;
+        if (n_spj) n_spj[g] = G.w;
     }
     return PTH_OK;
 }
 
-int pth_gtree_timing(const pth_gtree *t, float *ms_build, float *ms_walk)
+int pth_gtree_timing(const pth_gtree *t, float *ms_build, float *ms_walk, float *ms_order)
 {
     if (!t) return PTH_ERR_ARG;
     if (ms_build) *ms_build = t->ms_build;
     if (ms_walk) *ms_walk = t->ms_walk;
+    if (ms_order) *ms_order = t->ms_order;
+    return PTH_OK;
+}
+
+int pth_resident_from_gtree(pth_ctx *c, pth_resident **out, const pth_gtree *t)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (!out) return c->fail(PTH_ERR_ARG, "pth_resident_from_gtree: null output");
+    *out = nullptr;
+    if (!t) return c->fail(PTH_ERR_ARG, "pth_resident_from_gtree: null tree");
+    if (t->ctx->device != c->device)
+        return c->fail(PTH_ERR_ARG, "pth_resident_from_gtree: the tree lives on another device");
+    if (t->n_node > 0x7fffffffLL)
+        return c->fail(PTH_ERR_CAPACITY, "pth_resident_from_gtree: node count exceeds the int32 range");
+    const long long ng = t->n_group;
+    std::vector<int> ni(ng), ne(ng), ns(ng);
+    long long next = 0;
+    for (long long g = 0; g < ng; g++) {
+        if (t->groups_h[g].x != next)
+            return c->fail(PTH_ERR_STATE, "pth_resident_from_gtree: the groups do not tile the particles");
+        next += t->groups_h[g].y;
+        ni[g] = t->groups_h[g].y;
+        ne[g] = t->groups_h[g].z;
+        ns[g] = t->groups_h[g].w;
+    }
+    pth_resident *r = nullptr;
+    int rc = resident_prepare(c, r, (int)ng, ni.data(), ne.data(), ns.data(), t->n_le, t->n_ls, (int)t->n,
+                              (int)t->n_node, "pth_resident_from_gtree");
+    if (rc != PTH_OK) return rc;
+    const Plan &P = r->plan;
+    const Arena &A = r->A;
+    auto bail = [&](hipError_t e, const char *what) {
+        int rc2 = c->hip_fail(e, what);
+        pth_resident_free(r);
+        return rc2;
+    };
+    if (P.ni_tot != t->n) {
+        pth_resident_free(r);
+        return c->fail(PTH_ERR_STATE, "pth_resident_from_gtree: the groups do not cover the particles");
+    }
+    hipStream_t st = c->stream;
+    hipError_t e;
+    char *d = r->arena.as<char>();
+    // plan tables from the host, lists and records on the device
+    auto h2d = [&](size_t off, const void *h, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpyAsync(d + off, h, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+    };
+    if ((e = h2d(A.walks, P.walks.data(), sizeof(WalkDev) * P.walks.size())) ||
+        (e = h2d(A.es, P.es_off.data(), sizeof(long long) * P.es_off.size())) ||
+        (e = h2d(A.ss, P.ss_off.data(), sizeof(long long) * P.ss_off.size())) ||
+        (e = h2d(A.items, P.items.data(), sizeof(Item) * P.items.size())) ||
+        (e = h2d(A.ritems, P.ritems.data(), sizeof(RItem) * P.ritems.size())))
+        return bail(e, "resident upload");
+    if ((t->n_le && (e = hipMemcpyAsync(d + A.ide, t->id_epj.p, 4 * (size_t)t->n_le, hipMemcpyDeviceToDevice, st))) ||
+        (t->n_ls && (e = hipMemcpyAsync(d + A.ids, t->id_spj.p, 4 * (size_t)t->n_ls, hipMemcpyDeviceToDevice, st))))
+        return bail(e, "resident lists");
+    hipLaunchKernelGGL(k_res_particles, dim3(blocks(t->n)), dim3(256), 0, st, t->n, t->spos.as<double>(),
+                       t->smass.as<double>(), t->srs.as<double>(), reinterpret_cast<EpiDev *>(d + A.epi),
+                       r->epj.as<EpjDev>());
+    hipLaunchKernelGGL(k_res_nodes, dim3(blocks(t->n_node)), dim3(256), 0, st, t->n_node, t->nmass.as<double>(),
+                       t->ncom.as<double>(), t->nquad.as<double>(), r->spj.as<SpjDev>());
+    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(e, "resident records");
+    resident_finish(r);
+    *out = r;
     return PTH_OK;
 }
 

@@ -307,6 +307,17 @@ struct pth_resident {
     long long staged_bytes = 0;
 };
 
+namespace pth {
+// A resident evaluation with its plan made and its device memory allocated
+// (contents not yet uploaded); on failure the context holds the error and
+// out is null.
+int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_epi, const int *n_epj,
+                     const int *n_spj, long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot,
+                     const char *who);
+// Launch descriptors of a prepared resident whose contents are uploaded.
+void resident_finish(pth_resident *r);
+}  // namespace pth
+
 #define HIPCHK(ctx, call, what)                      \
     do {                                             \
         hipError_t e_ = (call);                      \

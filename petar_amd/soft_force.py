@@ -80,7 +80,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_er
             "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
             "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
             "pth_resident_replan", "pth_resident_autotune", "pth_gtree_build", "pth_gtree_sizes",
-            "pth_gtree_download", "pth_gtree_timing", "pth_gtree_free",
+            "pth_gtree_download", "pth_gtree_timing", "pth_gtree_free", "pth_resident_from_gtree",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
 
@@ -131,10 +131,11 @@ def lib():
     L.pth_resident_replan.argtypes = [vp, C.c_int, C.c_int]
     L.pth_resident_autotune.argtypes = [vp, ip, C.c_int, ip, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_float)]
     llp = C.POINTER(C.c_longlong)
-    L.pth_gtree_build.argtypes = [vp, C.POINTER(vp), C.c_longlong, dp, dp, dp, C.c_int, C.c_int, C.c_double]
+    L.pth_gtree_build.argtypes = [vp, C.POINTER(vp), C.c_longlong, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_int]
     L.pth_gtree_sizes.argtypes = [vp, llp, llp, llp, llp]
     L.pth_gtree_download.argtypes = [vp, llp, llp, llp, ip, dp, dp, dp, llp, ip, ip, ip, ip, ip]
-    L.pth_gtree_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.pth_gtree_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.pth_resident_from_gtree.argtypes = [vp, C.POINTER(vp), vp]
     L.pth_gtree_free.argtypes = [vp]
     L.pth_plan_debug.restype = C.c_longlong
     L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
@@ -321,9 +322,16 @@ class Resident:
     """A whole soft-force evaluation kept in HBM (pth_resident_*)."""
 
     def __init__(self, ctx: Context, n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
-                 spj_pos, spj_mass, spj_quad):
+                 spj_pos, spj_mass, spj_quad, _tree=None):
         self.ctx = ctx
         L = lib()
+        self.co = bool(ctx.params.changeover)
+        self._h = C.c_void_p()
+        if _tree is not None:
+            self.n_walk = int(_tree.sizes()["n_group"])
+            self.ni = _tree.n
+            ctx.check(L.pth_resident_from_gtree(ctx.handle, C.byref(self._h), _tree._h), "pth_resident_from_gtree")
+            return
         self._keep = [_i32(n_epi), _i32(n_epj), _i32(n_spj), _f64(epi_pos).reshape(-1, 3), _f64(epi_rs),
                       _i32(id_epj) if len(id_epj) else np.zeros(1, np.int32),
                       _i32(id_spj) if len(id_spj) else np.zeros(1, np.int32),
@@ -336,8 +344,6 @@ class Resident:
         k = self._keep
         self.n_walk = k[0].size
         self.ni = int(k[0].sum())
-        self._h = C.c_void_p()
-        self.co = bool(ctx.params.changeover)
         ctx.check(L.pth_resident_create(ctx.handle, C.byref(self._h), self.n_walk, _ptr(k[0], C.c_int),
                                         _ptr(k[1], C.c_int), _ptr(k[2], C.c_int), _ptr(k[3], C.c_double),
                                         _ptr(k[4], C.c_double), _ptr(k[5], C.c_int), _ptr(k[6], C.c_int),
@@ -368,6 +374,12 @@ class Resident:
         if r.co:
             r.changeover(idx, np.arange(len(lists.epj_mass)), rin, rout)
         return r
+
+    @classmethod
+    def from_gtree(cls, ctx, tree):
+        """A resident evaluation fed from a GpuTree on the device (walk w =
+        group w, particles in tree order; pth_resident_from_gtree)."""
+        return cls(ctx, *([None] * 13), _tree=tree)
 
     def run(self):
         self.ctx.check(lib().pth_resident_run(self._h), "pth_resident_run")
@@ -424,7 +436,8 @@ class GpuTree:
     """Octree, moments, i-groups and interaction lists built on the device
     (pth_gtree_*), with the semantics of petar_amd.host.build_lists."""
 
-    def __init__(self, ctx: Context, pos, mass, r_search=None, theta=0.3, n_leaf_limit=20, n_group_limit=512):
+    def __init__(self, ctx: Context, pos, mass, r_search=None, theta=0.3, n_leaf_limit=20, n_group_limit=512,
+                 canonical=True):
         self.ctx = ctx
         pos = _f64(pos).reshape(-1, 3)
         self.n = pos.shape[0]
@@ -433,7 +446,7 @@ class GpuTree:
         self._h = C.c_void_p()
         ctx.check(lib().pth_gtree_build(ctx.handle, C.byref(self._h), self.n, _ptr(pos, C.c_double),
                                         _ptr(mass, C.c_double), None if rs is None else _ptr(rs, C.c_double),
-                                        n_leaf_limit, n_group_limit, theta), "pth_gtree_build")
+                                        n_leaf_limit, n_group_limit, theta, int(canonical)), "pth_gtree_build")
 
     def sizes(self):
         v = [C.c_longlong() for _ in range(4)]
@@ -459,9 +472,10 @@ class GpuTree:
         return d
 
     def timing(self):
-        a, b = C.c_float(), C.c_float()
-        self.ctx.check(lib().pth_gtree_timing(self._h, C.byref(a), C.byref(b)), "pth_gtree_timing")
-        return a.value, b.value
+        """Host wall ms of the last build: (tree, walk, canonical ordering)."""
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        self.ctx.check(lib().pth_gtree_timing(self._h, C.byref(a), C.byref(b), C.byref(c)), "pth_gtree_timing")
+        return a.value, b.value, c.value
 
     def close(self):
         if self._h:

@@ -1,9 +1,11 @@
 """The tree built on the device (pth_gtree_*) against the host tree
 (petar_amd.host.build_lists, the FDPS-semantics stand-in the lists of every
 other test come from): same particle order, same nodes, bit-identical
-moments, same i-groups and, per group, the same EPJ and SPJ entries (the
-order inside a list may differ); and a force evaluation fed from the device
-lists equals the FP64 oracle on them."""
+moments, same i-groups and, per group, the same EPJ and SPJ entries (entry
+for entry in canonical order, as sets in batch order); and a force
+evaluation fed from the device tree (pth_resident_from_gtree) equals the
+one fed from the host lists bit for bit and the FP64 oracle within the
+kernel tolerance."""
 import time
 
 import numpy as np
@@ -16,19 +18,23 @@ from tests._common import R_OUT, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _compare(n, seed, f_bin, theta, leaf=20, group=512, r_search=1.5 * R_OUT, model="plummer"):
-    m, x, _ = host.cluster(n, model, kroupa=True, f_bin=f_bin, seed=seed)
+def _compare(n, seed, f_bin, theta, leaf=20, group=512, r_search=1.5 * R_OUT, model="plummer", canonical=True,
+             keep=False, mx=None):
+    m, x = mx if mx is not None else host.cluster(n, model, kroupa=True, f_bin=f_bin, seed=seed)[:2]
     t0 = time.time()
     L = host.build_lists(x, m, r_search, theta=theta, n_leaf_limit=leaf, n_group_limit=group)
     t_host = time.time() - t0
-    ctx = sf.Context(0)
-    g = sf.GpuTree(ctx, x, m, np.full(n, r_search), theta=theta, n_leaf_limit=leaf, n_group_limit=group)
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    g = sf.GpuTree(ctx, x, m, np.full(n, r_search), theta=theta, n_leaf_limit=leaf, n_group_limit=group,
+                   canonical=canonical)
     d = g.download()
-    tb, tw = g.timing()
-    g.close()
-    ctx.close()
-    print(f"[gtree N={n} theta={theta}] host tree+walk {t_host * 1e3:.1f} ms, device build {tb:.2f} ms walk {tw:.2f} ms; "
-          f"{len(d['n_epi'])} groups, {d['id_epj'].size} EPJ + {d['id_spj'].size} SPJ entries", flush=True)
+    tb, tw, to = g.timing()
+    if not keep:
+        g.close()
+        ctx.close()
+    print(f"[gtree N={n} theta={theta} canonical={canonical}] host tree+walk {t_host * 1e3:.1f} ms, device build "
+          f"{tb:.2f} ms walk {tw:.2f} ms order {to:.2f} ms; {len(d['n_epi'])} groups, {d['id_epj'].size} EPJ + "
+          f"{d['id_spj'].size} SPJ entries", flush=True)
     assert np.array_equal(d["order"], L.order)
     assert np.array_equal(d["node_begin"], L.node_begin) and np.array_equal(d["node_end"], L.node_end)
     assert np.array_equal(d["node_level"], L.node_level)
@@ -37,12 +43,15 @@ def _compare(n, seed, f_bin, theta, leaf=20, group=512, r_search=1.5 * R_OUT, mo
     assert np.array_equal(d["node_com"], L.spj_pos) and np.array_equal(d["node_quad"], L.spj_quad)
     assert np.array_equal(d["epi_begin"], L.epi_begin) and np.array_equal(d["n_epi"], L.n_epi)
     assert np.array_equal(d["n_epj"], L.n_epj) and np.array_equal(d["n_spj"], L.n_spj)
-    eo, so = L.walk_offsets()
-    ge = np.concatenate([[0], np.cumsum(d["n_epj"])])
-    gs = np.concatenate([[0], np.cumsum(d["n_spj"])])
-    for w in range(L.n_walk):
-        assert np.array_equal(np.sort(d["id_epj"][ge[w]:ge[w + 1]]), np.sort(L.id_epj[eo[w]:eo[w + 1]]))
-        assert np.array_equal(np.sort(d["id_spj"][gs[w]:gs[w + 1]]), np.sort(L.id_spj[so[w]:so[w + 1]]))
+    if canonical:
+        assert np.array_equal(d["id_epj"], L.id_epj) and np.array_equal(d["id_spj"], L.id_spj)
+    else:
+        eo, so = L.walk_offsets()
+        for w in range(L.n_walk):
+            assert np.array_equal(np.sort(d["id_epj"][eo[w]:eo[w + 1]]), np.sort(L.id_epj[eo[w]:eo[w + 1]]))
+            assert np.array_equal(np.sort(d["id_spj"][so[w]:so[w + 1]]), np.sort(L.id_spj[so[w]:so[w + 1]]))
+    if keep:
+        return L, d, ctx, g, m
     return L, d
 
 
@@ -55,25 +64,71 @@ def test_device_tree_equals_host_tree(n, seed, f_bin, theta):
 def test_device_tree_limits():
     _compare(20000, 5, 0.3, 0.4, leaf=4, group=64)
     _compare(20000, 5, 0.3, 0.4, leaf=64, group=2048)
+    _compare(20000, 5, 0.3, 0.4, leaf=1, group=1)
+
+
+def test_device_tree_batch_order():
+    _compare(30000, 4, 0.3, 0.3, canonical=False)
+
+
+def test_device_tree_tiny():
+    _compare(1, 0, 0.0, 0.3, mx=(np.ones(1), np.zeros((1, 3))))
+    _compare(2, 0, 0.0, 0.3, mx=(np.ones(2), np.zeros((2, 3))))  # coincident: a zero-size cube
+    for n in (2, 3, 21):
+        _compare(n, 11, 0.0, 0.3)
 
 
 def test_device_lists_feed_the_force_path():
-    """The device lists (their own order) through the resident evaluation
-    against the FP64 oracle on the same lists."""
-    L, d = _compare(16384, 2, 0.3, 0.3)
-    ge = np.concatenate([[0], np.cumsum(d["n_epj"])])
-    idx = np.concatenate([np.arange(b, b + k) for b, k in zip(d["epi_begin"], d["n_epi"])])
-    args = dict(n_epi=d["n_epi"], n_epj=d["n_epj"], n_spj=d["n_spj"], epi_pos=L.epj_pos[idx], epi_rs=L.epj_rs[idx],
-                id_epj=d["id_epj"], id_spj=d["id_spj"], epj_pos=L.epj_pos, epj_mass=L.epj_mass, epj_rs=L.epj_rs,
-                spj_pos=d["node_com"], spj_mass=d["node_mass"], spj_quad=d["node_quad"])
-    racc, rpot, rnnb = O.soft_force_walks(**args, eps2=0.0, rout2=R_OUT ** 2)
-    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
-    r = sf.Resident(ctx, **args)
-    r.run()
-    acc, pot, nnb = r.results()
-    r.close()
+    """pth_resident_from_gtree against pth_resident_create on the host lists
+    (bit for bit) and against the FP64 oracle on the same lists."""
+    L, d, ctx, g, _ = _compare(16384, 2, 0.3, 0.3, keep=True)
+    rd = sf.Resident.from_gtree(ctx, g)
+    g.close()
+    rd.run()
+    acc, pot, nnb = rd.results()
+    rd.close()
+    rh = sf.Resident.from_lists(ctx, L)
+    rh.run()
+    acc_h, pot_h, nnb_h = rh.results()
+    rh.close()
     ctx.close()
+    assert np.array_equal(acc, acc_h) and np.array_equal(pot, pot_h) and np.array_equal(nnb, nnb_h)
+    idx = np.concatenate([np.arange(b, b + k) for b, k in zip(L.epi_begin, L.n_epi)])
+    racc, rpot, rnnb = O.soft_force_walks(L.n_epi, L.n_epj, L.n_spj, L.epj_pos[idx], L.epj_rs[idx], L.id_epj,
+                                          L.id_spj, L.epj_pos, L.epj_mass, L.epj_rs, L.spj_pos, L.spj_mass,
+                                          L.spj_quad, eps2=0.0, rout2=R_OUT ** 2)
     ea, ep = rel_err(acc, racc), rel_err(pot, rpot)
     assert np.median(ea) <= 1e-6 and np.median(ep) <= 1e-6 and np.percentile(ea, 99) <= 1e-4
     assert np.mean(nnb != rnnb) <= 1e-3
-    assert ge[-1] == d["id_epj"].size
+
+
+def test_device_tree_changeover_mode():
+    """Changeover mode fed from the device tree (radii and ids in tree order)
+    equals the host-list evaluation bit for bit."""
+    n = 12000
+    m, x, _ = host.cluster(n, "plummer", kroupa=True, f_bin=0.5, seed=21)
+    rout = R_OUT * np.maximum(np.cbrt(m / m.mean()), 1.0)
+    rin = 0.1 * rout
+    rs = 1.5 * rout
+    L = host.build_lists(x, m, rs, theta=0.3)
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, changeover=1)
+    g = sf.GpuTree(ctx, x, m, rs, theta=0.3)
+    rd = sf.Resident.from_gtree(ctx, g)
+    g.close()
+    rd.changeover(np.arange(n), np.arange(n), rin[L.order], rout[L.order])
+    rd.run()
+    out_d = rd.results()
+    rd.close()
+    rh = sf.Resident.from_lists(ctx, L, rin[L.order], rout[L.order])
+    rh.run()
+    out_h = rh.results()
+    rh.close()
+    ctx.close()
+    for a, b in zip(out_d, out_h):
+        assert np.array_equal(a, b)
+
+
+def test_device_tree_large():
+    """N = 262144 with 50% binaries: the lists equal the host's entry for
+    entry."""
+    _compare(262144, 8, 0.5, 0.3)
