@@ -60,7 +60,60 @@ def build_workload(name, nthreads=0):
     L = host.build_lists(x, m, 1.5 * r_out, theta=0.3, n_leaf_limit=20, n_group_limit=512, nthreads=nthreads)
     t2 = time.time()
     log(f"[bench] workload {name}: {desc}; ic {t1 - t0:.1f}s, tree+walk {t2 - t1:.1f}s, {L.n_walk} walks")
-    return L, r_out, desc, dict(ic_s=round(t1 - t0, 2), tree_walk_s=round(t2 - t1, 2))
+    return L, r_out, desc, dict(ic_s=round(t1 - t0, 2), tree_walk_s=round(t2 - t1, 2)), (m, x)
+
+
+def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
+    """The whole soft-force step with the tree on the device, in its
+    steady-state form (tree and resident evaluation rebuilt in place each
+    step): octree + moments + groups, group walk, canonical list order,
+    resident records and plan, force evaluation.  Timed with the particles
+    already in HBM (torch tensors) and, separately, uploaded from host
+    arrays; the forces are checked bit for bit against the evaluation fed
+    from the host lists."""
+    import torch
+    from petar_amd import soft_force as sf
+    m, x = particles
+    rs = np.full(m.size, 1.5 * r_out)
+    dev = torch.device("cuda:0")
+    tx, tm, tr = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (x, m, rs))
+    torch.cuda.synchronize()
+    g = sf.GpuTree(ctx, x, m, rs, theta=0.3, n_leaf_limit=20, n_group_limit=512, canonical=True)
+    r = sf.Resident.from_gtree(ctx, g)
+    r.run()
+    r.sync()
+    z = g.sizes()
+    assert z["n_group"] == L.n_walk and z["n_epj_list"] == L.id_epj.size and z["n_spj_list"] == L.id_spj.size
+    acc, pot, nnb = r.results()
+    same = bool(np.array_equal(acc, ref[0]) and np.array_equal(pot, ref[1]) and np.array_equal(nnb, ref[2]))
+    res = {}
+    for label, inputs in (("device_inputs", (tx, tm, tr)), ("host_inputs", (x, m, rs))):
+        phases, walls = [], []
+        for k in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.update(*inputs)
+            t1 = time.perf_counter()
+            r.refill(g)
+            t2 = time.perf_counter()
+            r.run()
+            r.sync()
+            t3 = time.perf_counter()
+            if k > 0:
+                tb, tw, to = g.timing()
+                phases.append((tb, tw, to, (t2 - t1) * 1e3, (t3 - t2) * 1e3))
+                walls.append((t3 - t0) * 1e3)
+        ph = np.median(np.array(phases), axis=0)
+        res[label] = dict(step_ms=round(float(np.median(walls)), 3), tree_build_ms=round(float(ph[0]), 3),
+                          walk_ms=round(float(ph[1]), 3), canonical_order_ms=round(float(ph[2]), 3),
+                          refill_ms=round(float(ph[3]), 3), force_ms=round(float(ph[4]), 3))
+    acc2, pot2, nnb2 = r.results()
+    same = same and bool(np.array_equal(acc2, ref[0]) and np.array_equal(pot2, ref[1]) and np.array_equal(nnb2, ref[2]))
+    r.close()
+    g.close()
+    out = dict(res, forces_bit_identical_to_host_lists=same, reps=reps)
+    log(f"[bench] device tree step: {out}")
+    return out
 
 
 def tree_error(res_acc, res_pot, L, r_out, n_sample, seed=1):
@@ -233,6 +286,7 @@ def main():
     ap.add_argument("--direct-sample", type=int, default=4096,
                     help="i sampled for the error against the FP64 all-pairs direct sum (0: skip)")
     ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--no-device-tree", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the soft-force path is "
@@ -240,7 +294,7 @@ def main():
         sys.exit(2)
 
     from petar_amd import soft_force as sf
-    L, r_out, desc, prep = build_workload(a.workload)
+    L, r_out, desc, prep, particles = build_workload(a.workload)
     ne, ns = L.interactions()
     n_int = ne + ns
     flops = FLOP_EPJ * ne + FLOP_SPJ * ns
@@ -329,12 +383,20 @@ def main():
         "prep": prep,
         "autotuned": tuned,
     }
+    acc, pot, nnb = res.results()
     if not a.no_accuracy:
-        acc, pot, nnb = res.results()
         out["accuracy_vs_fp64_same_lists"] = accuracy(acc, pot, nnb, L, r_out)
         if a.direct_sample > 0:
             out["error_vs_fp64_direct_sum"] = tree_error(acc, pot, L, r_out, a.direct_sample)
     res.close()
+    if not a.no_device_tree and a.exec == "fused":
+        # same tiles and segments as the timed evaluation, so the forces can
+        # be compared bit for bit
+        ctxd = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=cj_used,
+                          max_ipl=ipl_used, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams)
+        out["device_tree_step"] = device_tree_step(ctxd, particles, r_out, L, (acc, pot, nnb))
+        ctxd.close()
+        out["device_tree_step"]["host_tree_walk_s"] = prep["tree_walk_s"]
     if not a.no_plain and a.exec == "fused":
         ctxp = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, exec_mode=sf.EXEC_PLAIN)
         rp = sf.Resident.from_lists(ctxp, L)

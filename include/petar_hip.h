@@ -270,6 +270,14 @@ void pth_server_free(pth_server *srv);
 typedef struct pth_gtree pth_gtree;
 int pth_gtree_build(pth_ctx *ctx, pth_gtree **tree, long long n, const double *pos, const double *mass,
                     const double *r_search, int n_leaf_limit, int n_group_limit, double theta, int canonical);
+/* Rebuilds the tree for new particles (same limits, theta and ordering),
+ * reusing its device memory (grown when n or the lists grow).  With
+ * on_device != 0 the arrays are device (HBM) arrays of this process holding
+ * n entries, so a step whose particles are already resident needs no host
+ * copy; every range read is checked to lie inside one device allocation
+ * (host pointers are rejected) before any launch. */
+int pth_gtree_update(pth_gtree *tree, long long n, const double *pos, const double *mass, const double *r_search,
+                     int on_device);
 int pth_gtree_sizes(const pth_gtree *tree, long long *n_node, long long *n_group, long long *n_epj_list,
                     long long *n_spj_list);
 /* Copies the tree to the host (any pointer may be NULL): order [n] (sorted
@@ -294,6 +302,10 @@ void pth_gtree_free(pth_gtree *tree);
  * order maps them).  With a canonical tree the results equal those of
  * pth_resident_create on the host tree's lists bit for bit. */
 int pth_resident_from_gtree(pth_ctx *ctx, pth_resident **res, const pth_gtree *tree);
+/* Refills an existing resident evaluation from a (re)built tree, reusing its
+ * device memory: the per-step form of pth_resident_from_gtree.  In
+ * changeover mode pth_resident_changeover must be called again. */
+int pth_resident_refill(pth_resident *res, const pth_gtree *tree);
 
 /* ---- diagnostics (host only, no device needed) ----------------------------- */
 /* The work decomposition pth_dispatch / pth_resident_create use for these walk

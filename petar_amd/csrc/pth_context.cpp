@@ -807,34 +807,25 @@ long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const i
 
 namespace pth {
 
-int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_epi, const int *n_epj,
-                     const int *n_spj, long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot,
-                     const char *who)
+int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
+                   long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot, const char *who)
 {
-    out = nullptr;
-    pth_resident *r = new pth_resident;
-    r->ctx = c;
+    pth_ctx *c = r->ctx;
+    r->ran = false;
+    r->co_ready = false;
+    r->pipelined = false;
     r->n_seg = (c->par.exec_mode == PTH_EXEC_FUSED) ? c->par.segments : 1;
     if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
-                   r->n_seg)) {
-        delete r;
+                   r->n_seg))
         return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
-    }
     const Plan &P = r->plan;
     r->co = c->par.changeover != 0;
     if (r->co) {
         const char *m = co_plan_error(P);
-        if (m) {
-            delete r;
-            return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": " + m);
-        }
+        if (m) return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": " + m);
     }
     r->A.layout(P, n_ide, n_ids, r->co);
-    auto bail = [&](hipError_t e, const char *what) {
-        int rc = c->hip_fail(e, what);
-        pth_resident_free(r);
-        return rc;
-    };
+    auto bail = [&](hipError_t e, const char *what) { return c->hip_fail(e, what); };
     hipError_t e;
     if ((e = hipSetDevice(c->device))) return bail(e, "hipSetDevice");
     if ((e = r->arena.ensure(r->A.total))) return bail(e, "resident arena");
@@ -848,18 +839,34 @@ int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_ep
         if ((e = r->out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
     }
     for (int k = 0; k < 4; k++)
-        if ((e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
+        if (!r->ev[k] && (e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
     if ((int)P.segs.size() > 1) {
         const int ns = std::min(c->par.streams, (int)P.segs.size());
-        r->side.assign(ns, nullptr);
-        r->side_done.assign(ns, nullptr);
-        for (int k = 0; k < ns; k++) {
-            if ((e = hipStreamCreateWithFlags(&r->side[k], hipStreamNonBlocking))) return bail(e, "hipStreamCreate");
-            if ((e = hipEventCreateWithFlags(&r->side_done[k], hipEventDisableTiming)))
+        // side streams are only ever added (a later plan may use fewer)
+        while ((int)r->side.size() < ns) {
+            r->side.push_back(nullptr);
+            r->side_done.push_back(nullptr);
+            if ((e = hipStreamCreateWithFlags(&r->side.back(), hipStreamNonBlocking))) return bail(e, "hipStreamCreate");
+            if ((e = hipEventCreateWithFlags(&r->side_done.back(), hipEventDisableTiming)))
                 return bail(e, "hipEventCreate");
         }
     }
     r->n_epj_tot = n_epj_tot;
+    return PTH_OK;
+}
+
+int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_epi, const int *n_epj,
+                     const int *n_spj, long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot,
+                     const char *who)
+{
+    out = nullptr;
+    pth_resident *r = new pth_resident;
+    r->ctx = c;
+    const int rc = resident_setup(r, n_walk, n_epi, n_epj, n_spj, n_ide, n_ids, n_epj_tot, n_spj_tot, who);
+    if (rc != PTH_OK) {
+        pth_resident_free(r);
+        return rc;
+    }
     out = r;
     return PTH_OK;
 }

@@ -480,7 +480,10 @@ struct pth_gtree {
     }
     std::vector<long long> level_start;
     std::vector<int4> groups_h;  // per group: first particle, size, n_epj, n_spj
+    int n_leaf_limit = 20, n_group_limit = 512;
+    double theta = 0.3;
     bool canonical = false;
+    bool built = false;
     float ms_build = 0.f, ms_walk = 0.f, ms_order = 0.f;
 };
 
@@ -499,44 +502,66 @@ hipError_t cub_temp(pth_gtree *t, size_t bytes) { return t->tmp.ensure(std::max<
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *pos, const double *mass,
-                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta, int canonical)
+// true when [p, p + bytes) lies inside one device allocation of this process
+bool device_range(const void *p, size_t bytes)
 {
-    if (!ctx) return PTH_ERR_ARG;
-    if (!out) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: null output");
-    *out = nullptr;
-    if (n <= 0 || n > 0x7fffffffLL) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: n outside [1, 2^31)");
-    if (!pos || !mass) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: null particle arrays");
-    if (n_leaf_limit < 1 || n_group_limit < 1) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: limits must be >= 1");
-    HIPCHK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    pth_gtree *t = new pth_gtree;
-    t->ctx = ctx;
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (at.type != hipMemoryTypeDevice) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return (const char *)p + bytes <= (const char *)base + size;
+}
+
+// Builds (or rebuilds, reusing t's device memory) the tree of n particles.
+// pos / mass / rs are host arrays, or device arrays when on_device.
+int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, const double *r_search,
+              bool on_device)
+{
+    pth_ctx *ctx = t->ctx;
+    const int n_leaf_limit = t->n_leaf_limit, n_group_limit = t->n_group_limit;
+    const double theta = t->theta;
+    t->built = false;
     t->n = n;
-    auto bail = [&](int rc) {
-        pth_gtree_free(t);
-        return rc;
-    };
+    t->n_node = t->n_group = t->n_le = t->n_ls = 0;
+    auto bail = [&](int rc) { return rc; };
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = ctx->stream;
     hipError_t e;
     // particles in (unsorted), then sorted copies
-    DevBuf &dpos = t->dpos, &dmass = t->dmass, &drs = t->drs;
-    auto up = [&](DevBuf &b, const double *h, size_t cnt) -> hipError_t {
-        hipError_t r = b.ensure(sizeof(double) * cnt);
-        if (r == hipSuccess) r = hipMemcpyAsync(b.p, h, sizeof(double) * cnt, hipMemcpyHostToDevice, st);
-        return r;
-    };
-    if ((e = up(dpos, pos, 3 * (size_t)n)) || (e = up(dmass, mass, (size_t)n)) ||
-        (r_search && (e = up(drs, r_search, (size_t)n))))
-        return bail(ctx->hip_fail(e, "pth_gtree_build: upload"));
+    const double *dpos = pos, *dmass = mass, *drs = r_search;
+    if (on_device) {
+        if (!device_range(pos, 24 * (size_t)n) || !device_range(mass, 8 * (size_t)n) ||
+            (r_search && !device_range(r_search, 8 * (size_t)n)))
+            return ctx->fail(PTH_ERR_ARG, "pth_gtree: the particle arrays are not device allocations of n entries");
+    } else {
+        auto up = [&](DevBuf &b, const double *h, size_t cnt) -> hipError_t {
+            hipError_t r = b.ensure(sizeof(double) * cnt);
+            if (r == hipSuccess) r = hipMemcpyAsync(b.p, h, sizeof(double) * cnt, hipMemcpyHostToDevice, st);
+            return r;
+        };
+        if ((e = up(t->dpos, pos, 3 * (size_t)n)) || (e = up(t->dmass, mass, (size_t)n)) ||
+            (r_search && (e = up(t->drs, r_search, (size_t)n))))
+            return bail(ctx->hip_fail(e, "pth_gtree: upload"));
+        dpos = t->dpos.as<double>();
+        dmass = t->dmass.as<double>();
+        drs = r_search ? t->drs.as<double>() : nullptr;
+    }
     // 1. bounding cube (per-block partials reduced on the host)
     const int nb = (int)std::min<long long>(1024, (n + 255) / 256);
     DevBuf &part = t->part;
     std::vector<double> hp(6 * (size_t)nb);
     if ((e = part.ensure(sizeof(double) * 6 * nb))) return bail(ctx->hip_fail(e, "bbox"));
-    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, dpos.as<double>(), n, part.as<double>());
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, dpos, n, part.as<double>());
     if ((e = hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
         return bail(ctx->hip_fail(e, "bbox"));
@@ -556,7 +581,7 @@ int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *po
     if ((e = t->key.ensure(8 * (size_t)n)) || (e = t->skey.ensure(8 * (size_t)n)) ||
         (e = t->sidx.ensure(8 * (size_t)n)) || (e = t->order.ensure(8 * (size_t)n)))
         return bail(ctx->hip_fail(e, "key allocation"));
-    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(256), 0, st, dpos.as<double>(), n, root[0], root[1], root[2],
+    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(256), 0, st, dpos, n, root[0], root[1], root[2],
                        scale, t->key.as<unsigned long long>(), t->sidx.as<long long>());
     size_t tb = 0;
     if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, t->key.as<unsigned long long>(),
@@ -570,12 +595,13 @@ int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *po
     if ((e = t->spos.ensure(24 * (size_t)n)) || (e = t->smass.ensure(8 * (size_t)n)) ||
         (e = t->srs.ensure(8 * (size_t)n)))
         return bail(ctx->hip_fail(e, "particle allocation"));
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(256), 0, st, n, t->order.as<long long>(), dpos.as<double>(),
-                       dmass.as<double>(), r_search ? drs.as<double>() : nullptr, t->spos.as<double>(),
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(256), 0, st, n, t->order.as<long long>(), dpos,
+                       dmass, drs, t->spos.as<double>(),
                        t->smass.as<double>(), t->srs.as<double>());
     // 3. nodes level by level
-    long long cap = std::max<long long>(2 * n / std::max(n_leaf_limit, 1) + 64, 1024);
-    if ((e = t->nodes.ensure(sizeof(GNode) * cap))) return bail(ctx->hip_fail(e, "node allocation"));
+    if ((e = t->nodes.ensure(sizeof(GNode) * std::max<long long>(2 * n / std::max(n_leaf_limit, 1) + 64, 1024))))
+        return bail(ctx->hip_fail(e, "node allocation"));
+    long long cap = (long long)(t->nodes.cap / sizeof(GNode));
     {
         const GNode root_node{0, n, 0, 0, -1};
         if ((e = hipMemcpyAsync(t->nodes.p, &root_node, sizeof(GNode), hipMemcpyHostToDevice, st)) ||
@@ -722,7 +748,6 @@ int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *po
     const auto t2 = std::chrono::steady_clock::now();
     t->ms_walk = std::chrono::duration<float, std::milli>(t2 - t1).count();
     // 7. canonical order: segmented sorts of the lists (one segment per group)
-    t->canonical = canonical != 0;
     if (t->canonical && ng > 0) {
         int bits = 1;
         while (bits < 31 && (1LL << bits) < n) bits++;  // keys are particle positions < n
@@ -758,6 +783,7 @@ int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *po
     }
     const auto t3 = std::chrono::steady_clock::now();
     t->ms_order = t->canonical ? std::chrono::duration<float, std::milli>(t3 - t2).count() : 0.f;
+    (void)t3;
     // per-group sizes for the planner
     t->groups_h.assign(ng, make_int4(0, 0, 0, 0));
     if (ng > 0) {
@@ -768,15 +794,109 @@ int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *po
             (e = hipStreamSynchronize(st)))
             return bail(ctx->hip_fail(e, "group sizes"));
     }
-    t->release_scratch();
+    t->built = true;
+    return PTH_OK;
+}
+
+// (Re)fills resident r from tree t: plan, buffers, records and lists.
+int resident_fill_gtree(pth_resident *r, const pth_gtree *t, const char *who)
+{
+    pth_ctx *c = r->ctx;
+    const std::string w(who);
+    if (!t) return c->fail(PTH_ERR_ARG, w + ": null tree");
+    if (!t->built) return c->fail(PTH_ERR_STATE, w + ": the last build of the tree failed");
+    if (t->ctx->device != c->device) return c->fail(PTH_ERR_ARG, w + ": the tree lives on another device");
+    if (t->n_node > 0x7fffffffLL) return c->fail(PTH_ERR_CAPACITY, w + ": node count exceeds the int32 range");
+    const long long ng = t->n_group;
+    std::vector<int> ni(ng), ne(ng), ns(ng);
+    long long next = 0;
+    for (long long g = 0; g < ng; g++) {
+        if (t->groups_h[g].x != next)
+            return c->fail(PTH_ERR_STATE, w + ": the groups do not tile the particles");
+        next += t->groups_h[g].y;
+        ni[g] = t->groups_h[g].y;
+        ne[g] = t->groups_h[g].z;
+        ns[g] = t->groups_h[g].w;
+    }
+    const int rc = resident_setup(r, (int)ng, ni.data(), ne.data(), ns.data(), t->n_le, t->n_ls, (int)t->n,
+                                  (int)t->n_node, who);
+    if (rc != PTH_OK) return rc;
+    const Plan &P = r->plan;
+    const Arena &A = r->A;
+    auto bail = [&](hipError_t e, const char *what) { return c->hip_fail(e, what); };
+    if (P.ni_tot != t->n) return c->fail(PTH_ERR_STATE, w + ": the groups do not cover the particles");
+    hipStream_t st = c->stream;
+    hipError_t e;
+    char *d = r->arena.as<char>();
+    // plan tables from the host, lists and records on the device
+    auto h2d = [&](size_t off, const void *h, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpyAsync(d + off, h, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+    };
+    if ((e = h2d(A.walks, P.walks.data(), sizeof(WalkDev) * P.walks.size())) ||
+        (e = h2d(A.es, P.es_off.data(), sizeof(long long) * P.es_off.size())) ||
+        (e = h2d(A.ss, P.ss_off.data(), sizeof(long long) * P.ss_off.size())) ||
+        (e = h2d(A.items, P.items.data(), sizeof(Item) * P.items.size())) ||
+        (e = h2d(A.ritems, P.ritems.data(), sizeof(RItem) * P.ritems.size())))
+        return bail(e, "resident upload");
+    if ((t->n_le && (e = hipMemcpyAsync(d + A.ide, t->id_epj.p, 4 * (size_t)t->n_le, hipMemcpyDeviceToDevice, st))) ||
+        (t->n_ls && (e = hipMemcpyAsync(d + A.ids, t->id_spj.p, 4 * (size_t)t->n_ls, hipMemcpyDeviceToDevice, st))))
+        return bail(e, "resident lists");
+    hipLaunchKernelGGL(k_res_particles, dim3(blocks(t->n)), dim3(256), 0, st, t->n, t->spos.as<double>(),
+                       t->smass.as<double>(), t->srs.as<double>(), reinterpret_cast<EpiDev *>(d + A.epi),
+                       r->epj.as<EpjDev>());
+    hipLaunchKernelGGL(k_res_nodes, dim3(blocks(t->n_node)), dim3(256), 0, st, t->n_node, t->nmass.as<double>(),
+                       t->ncom.as<double>(), t->nquad.as<double>(), r->spj.as<SpjDev>());
+    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(e, "resident records");
+    resident_finish(r);
+    return PTH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pth_gtree_build(pth_ctx *ctx, pth_gtree **out, long long n, const double *pos, const double *mass,
+                    const double *r_search, int n_leaf_limit, int n_group_limit, double theta, int canonical)
+{
+    if (!ctx) return PTH_ERR_ARG;
+    if (!out) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: null output");
+    *out = nullptr;
+    if (n <= 0 || n > 0x7fffffffLL) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: n outside [1, 2^31)");
+    if (!pos || !mass) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: null particle arrays");
+    if (n_leaf_limit < 1 || n_group_limit < 1) return ctx->fail(PTH_ERR_ARG, "pth_gtree_build: limits must be >= 1");
+    HIPCHK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    pth_gtree *t = new pth_gtree;
+    t->ctx = ctx;
+    t->n_leaf_limit = n_leaf_limit;
+    t->n_group_limit = n_group_limit;
+    t->theta = theta;
+    t->canonical = canonical != 0;
+    const int rc = gtree_run(t, n, pos, mass, r_search, false);
+    if (rc != PTH_OK) {
+        pth_gtree_free(t);
+        return rc;
+    }
     *out = t;
     return PTH_OK;
+}
+
+int pth_gtree_update(pth_gtree *t, long long n, const double *pos, const double *mass, const double *r_search,
+                     int on_device)
+{
+    if (!t) return PTH_ERR_ARG;
+    pth_ctx *ctx = t->ctx;
+    if (n <= 0 || n > 0x7fffffffLL) return ctx->fail(PTH_ERR_ARG, "pth_gtree_update: n outside [1, 2^31)");
+    if (!pos || !mass) return ctx->fail(PTH_ERR_ARG, "pth_gtree_update: null particle arrays");
+    HIPCHK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return gtree_run(t, n, pos, mass, r_search, on_device != 0);
 }
 
 int pth_gtree_sizes(const pth_gtree *t, long long *n_node, long long *n_group, long long *n_epj_list,
                     long long *n_spj_list)
 {
     if (!t) return PTH_ERR_ARG;
+    if (!t->built) return t->ctx->fail(PTH_ERR_STATE, "pth_gtree_sizes: the last build failed");
     if (n_node) *n_node = t->n_node;
     if (n_group) *n_group = t->n_group;
     if (n_epj_list) *n_epj_list = t->n_le;
@@ -790,6 +910,7 @@ int pth_gtree_download(const pth_gtree *t, long long *order, long long *node_beg
 {
     if (!t) return PTH_ERR_ARG;
     pth_ctx *c = t->ctx;
+    if (!t->built) return c->fail(PTH_ERR_STATE, "pth_gtree_download: the last build failed");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     auto d2h = [&](void *h, const DevBuf &b, size_t bytes) -> hipError_t {
         return (h && bytes) ? hipMemcpy(h, b.p, bytes, hipMemcpyDeviceToHost) : hipSuccess;
@@ -832,62 +953,23 @@ int pth_resident_from_gtree(pth_ctx *c, pth_resident **out, const pth_gtree *t)
     if (!c) return PTH_ERR_ARG;
     if (!out) return c->fail(PTH_ERR_ARG, "pth_resident_from_gtree: null output");
     *out = nullptr;
-    if (!t) return c->fail(PTH_ERR_ARG, "pth_resident_from_gtree: null tree");
-    if (t->ctx->device != c->device)
-        return c->fail(PTH_ERR_ARG, "pth_resident_from_gtree: the tree lives on another device");
-    if (t->n_node > 0x7fffffffLL)
-        return c->fail(PTH_ERR_CAPACITY, "pth_resident_from_gtree: node count exceeds the int32 range");
-    const long long ng = t->n_group;
-    std::vector<int> ni(ng), ne(ng), ns(ng);
-    long long next = 0;
-    for (long long g = 0; g < ng; g++) {
-        if (t->groups_h[g].x != next)
-            return c->fail(PTH_ERR_STATE, "pth_resident_from_gtree: the groups do not tile the particles");
-        next += t->groups_h[g].y;
-        ni[g] = t->groups_h[g].y;
-        ne[g] = t->groups_h[g].z;
-        ns[g] = t->groups_h[g].w;
-    }
-    pth_resident *r = nullptr;
-    int rc = resident_prepare(c, r, (int)ng, ni.data(), ne.data(), ns.data(), t->n_le, t->n_ls, (int)t->n,
-                              (int)t->n_node, "pth_resident_from_gtree");
-    if (rc != PTH_OK) return rc;
-    const Plan &P = r->plan;
-    const Arena &A = r->A;
-    auto bail = [&](hipError_t e, const char *what) {
-        int rc2 = c->hip_fail(e, what);
+    pth_resident *r = new pth_resident;
+    r->ctx = c;
+    const int rc = resident_fill_gtree(r, t, "pth_resident_from_gtree");
+    if (rc != PTH_OK) {
         pth_resident_free(r);
-        return rc2;
-    };
-    if (P.ni_tot != t->n) {
-        pth_resident_free(r);
-        return c->fail(PTH_ERR_STATE, "pth_resident_from_gtree: the groups do not cover the particles");
+        return rc;
     }
-    hipStream_t st = c->stream;
-    hipError_t e;
-    char *d = r->arena.as<char>();
-    // plan tables from the host, lists and records on the device
-    auto h2d = [&](size_t off, const void *h, size_t bytes) -> hipError_t {
-        return bytes ? hipMemcpyAsync(d + off, h, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
-    };
-    if ((e = h2d(A.walks, P.walks.data(), sizeof(WalkDev) * P.walks.size())) ||
-        (e = h2d(A.es, P.es_off.data(), sizeof(long long) * P.es_off.size())) ||
-        (e = h2d(A.ss, P.ss_off.data(), sizeof(long long) * P.ss_off.size())) ||
-        (e = h2d(A.items, P.items.data(), sizeof(Item) * P.items.size())) ||
-        (e = h2d(A.ritems, P.ritems.data(), sizeof(RItem) * P.ritems.size())))
-        return bail(e, "resident upload");
-    if ((t->n_le && (e = hipMemcpyAsync(d + A.ide, t->id_epj.p, 4 * (size_t)t->n_le, hipMemcpyDeviceToDevice, st))) ||
-        (t->n_ls && (e = hipMemcpyAsync(d + A.ids, t->id_spj.p, 4 * (size_t)t->n_ls, hipMemcpyDeviceToDevice, st))))
-        return bail(e, "resident lists");
-    hipLaunchKernelGGL(k_res_particles, dim3(blocks(t->n)), dim3(256), 0, st, t->n, t->spos.as<double>(),
-                       t->smass.as<double>(), t->srs.as<double>(), reinterpret_cast<EpiDev *>(d + A.epi),
-                       r->epj.as<EpjDev>());
-    hipLaunchKernelGGL(k_res_nodes, dim3(blocks(t->n_node)), dim3(256), 0, st, t->n_node, t->nmass.as<double>(),
-                       t->ncom.as<double>(), t->nquad.as<double>(), r->spj.as<SpjDev>());
-    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(e, "resident records");
-    resident_finish(r);
     *out = r;
     return PTH_OK;
+}
+
+int pth_resident_refill(pth_resident *r, const pth_gtree *t)
+{
+    if (!r) return PTH_ERR_ARG;
+    HIPCHK(r->ctx, hipSetDevice(r->ctx->device), "hipSetDevice");
+    HIPCHK(r->ctx, hipStreamSynchronize(r->ctx->stream), "hipStreamSynchronize");
+    return resident_fill_gtree(r, t, "pth_resident_refill");
 }
 
 void pth_gtree_free(pth_gtree *t)

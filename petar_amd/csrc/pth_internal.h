@@ -314,6 +314,10 @@ namespace pth {
 int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_epi, const int *n_epj,
                      const int *n_spj, long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot,
                      const char *who);
+// Re-plans an existing resident for new sizes, growing its device memory as
+// needed (contents not yet uploaded; the changeover data is invalidated).
+int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
+                   long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot, const char *who);
 // Launch descriptors of a prepared resident whose contents are uploaded.
 void resident_finish(pth_resident *r);
 }  // namespace pth

@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdint>
+#include <cstring>
 
 #include "../../include/petar_hip.h"
 
@@ -29,12 +31,31 @@ struct Tile {
     int begin, n, ipl;
 };
 
+struct Tmp {
+    Item it;
+    double cost;
+};
+
+// Working memory of make_plan, kept per thread between calls: a plan of a
+// large evaluation touches tens of MB, and fresh pages cost more than the
+// planning itself.
+struct Scratch {
+    std::vector<Tmp> seq;
+    std::vector<size_t> walk_item_begin;
+    std::vector<int> walk_ritem_begin;
+    std::vector<double> wcost;
+    std::vector<uint16_t> bucket;
+    std::vector<uint32_t> sub, cnt, sorted;
+    std::vector<long long> eb, sb;
+    std::vector<Tile> tiles;
+};
+
 // Least-cost cover of n i-particles by tiles of 256 (ipl 4), 128 (ipl 2)
 // and 64 (ipl 1) lanes x i, for a walk with n_e EPJ and n_s SPJ.
-std::vector<Tile> tile_walk(int n, long long n_e, long long n_s, int max_ipl)
+void tile_walk(int n, long long n_e, long long n_s, int max_ipl, std::vector<Tile> &out)
 {
-    std::vector<Tile> out;
-    if (n <= 0) return out;
+    out.clear();
+    if (n <= 0) return;
     auto cost = [&](int ipl) { return 64.0 * ipl * (epj_unit_cost(ipl) * n_e + spj_unit_cost(ipl) * n_s); };
     const int amax = (max_ipl >= 4) ? (n + 255) / 256 : 0;
     const int bmax = (max_ipl >= 2) ? 3 : 0;
@@ -64,7 +85,6 @@ std::vector<Tile> tile_walk(int n, long long n_e, long long n_s, int max_ipl)
     emit(ba, 4);
     emit(bb, 2);
     emit(bc, 1);
-    return out;
 }
 
 }  // namespace
@@ -72,7 +92,7 @@ std::vector<Tile> tile_walk(int n, long long n_e, long long n_s, int max_ipl)
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
                int chunk_j, int max_ipl, Plan &P, int n_seg)
 {
-    P = Plan();
+    P.reset();
     P.walks.resize(n_walk);
     P.es_off.assign(n_walk + 1, 0);
     P.ss_off.assign(n_walk + 1, 0);
@@ -108,13 +128,13 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     if (max_ipl != 1 && max_ipl != 2) max_ipl = 4;
 
     const long long cj = (chunk_j > 0) ? chunk_j : DEFAULT_CHUNK_J;
-    struct Tmp {
-        Item it;
-        double cost;
-    };
-    std::vector<Tmp> seq;
-    std::vector<size_t> walk_item_begin(n_walk + 1, 0);
-    std::vector<int> walk_ritem_begin(n_walk + 1, 0);
+    static thread_local Scratch S;
+    std::vector<Tmp> &seq = S.seq;
+    seq.clear();
+    std::vector<size_t> &walk_item_begin = S.walk_item_begin;
+    std::vector<int> &walk_ritem_begin = S.walk_ritem_begin;
+    walk_item_begin.assign(n_walk + 1, 0);
+    walk_ritem_begin.assign(n_walk + 1, 0);
     for (int w = 0; w < n_walk; w++) {
         walk_item_begin[w] = seq.size();
         walk_ritem_begin[w] = (int)P.ritems.size();
@@ -126,7 +146,9 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         if (nch < 1) nch = 1;
         // chunk k covers list units [k*units/nch, (k+1)*units/nch); EPJ
         // boundaries are rounded to whole blocks of 8
-        std::vector<long long> eb(nch + 1), sb(nch + 1);
+        std::vector<long long> &eb = S.eb, &sb = S.sb;
+        eb.resize(nch + 1);
+        sb.resize(nch + 1);
         for (long long k = 0; k <= nch; k++) {
             const long long u = (k == nch) ? units : (k * units) / nch;
             eb[k] = std::min((std::min(u, nb * 8) + 7) / 8, nb);
@@ -134,7 +156,8 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         }
         eb[nch] = nb;
         sb[nch] = W.n_s;
-        for (const Tile &t : tile_walk(W.n_i, W.n_e, W.n_s, max_ipl)) {
+        tile_walk(W.n_i, W.n_e, W.n_s, max_ipl, S.tiles);
+        for (const Tile &t : S.tiles) {
             const int part_first = P.n_part;
             for (long long k = 0; k < nch; k++) {
                 Item it;
@@ -174,7 +197,8 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     if (n_seg < 1) n_seg = 1;
     if (n_seg > MAX_SEGMENTS) n_seg = MAX_SEGMENTS;
     if (n_seg > n_walk) n_seg = std::max(n_walk, 1);
-    std::vector<double> wcost(n_walk + 1, 0.0);
+    std::vector<double> &wcost = S.wcost;
+    wcost.assign(n_walk + 1, 0.0);
     for (int w = 0; w < n_walk; w++) {
         double c = 0;
         for (size_t k = walk_item_begin[w]; k < walk_item_begin[w + 1]; k++) c += seq[k].cost;
@@ -195,6 +219,21 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     dummy.part = -1;
     static const int class_ipl[3] = {1, 2, 4};
     // the items of walks [wb, we), per ipl class, in launch order
+    // cost buckets for the most-expensive-first order inside a run: a
+    // counting sort on the cost's binary exponent and top 4 mantissa bits
+    // (16 steps per octave), stable inside a bucket
+    constexpr int NBUCKET = 1024;
+    std::vector<uint16_t> &bucket = S.bucket;
+    bucket.resize(seq.size());
+    for (size_t k = 0; k < seq.size(); k++) {
+        const double c = std::max(seq[k].cost, 1.0);  // >= 1: biased exponent >= 1023
+        uint64_t bits;
+        std::memcpy(&bits, &c, 8);
+        const int q = (int)((bits >> 48) - (1023ull << 4));  // exponent * 16 + top mantissa bits
+        bucket[k] = (uint16_t)std::min(NBUCKET - 1, q);
+    }
+    std::vector<uint32_t> &sub = S.sub, &cnt = S.cnt, &sorted = S.sorted;
+    cnt.resize(NBUCKET + 1);
     auto arrange = [&](int wb, int we) {
         PlanSeg sg;
         sg.walk_begin = wb;
@@ -202,41 +241,48 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         sg.ritem_begin = walk_ritem_begin[wb];
         sg.ritem_count = walk_ritem_begin[we] - sg.ritem_begin;
         for (int cls = 0; cls < 3; cls++) {
-            std::vector<Tmp> sub;
+            sub.clear();
+            double total = 0;
             for (size_t k = walk_item_begin[wb]; k < walk_item_begin[we]; k++)
-                if (item_ipl(seq[k].it.n_i_ipl) == class_ipl[cls]) sub.push_back(seq[k]);
+                if (item_ipl(seq[k].it.n_i_ipl) == class_ipl[cls]) {
+                    sub.push_back((uint32_t)k);
+                    total += seq[k].cost;
+                }
             sg.seg_begin[cls] = (int)P.items.size();
             if (sub.empty()) continue;
-            double total = 0;
-            for (auto &t : sub) total += t.cost;
-            std::vector<std::vector<Item>> run(XCDS);
-            double acc = 0;
-            int x = 0;
-            std::vector<Tmp> cur;
-            auto flush = [&](int xx) {
-                std::stable_sort(cur.begin(), cur.end(), [](const Tmp &l, const Tmp &r) { return l.cost > r.cost; });
-                for (auto &c : cur) run[xx].push_back(c.it);
-                cur.clear();
-            };
-            for (size_t k = 0; k < sub.size(); k++) {
-                // an item goes to run x while the running cost is below (x+1)/8 of the total
-                while (x < XCDS - 1 && acc >= total * (x + 1) / XCDS) {
-                    flush(x);
-                    x++;
+            // runs: an item goes to run x while the running cost is below
+            // (x+1)/8 of the total
+            size_t run_begin[XCDS + 1];
+            {
+                double acc = 0;
+                int x = 0;
+                run_begin[0] = 0;
+                for (size_t k = 0; k < sub.size(); k++) {
+                    while (x < XCDS - 1 && acc >= total * (x + 1) / XCDS) run_begin[++x] = k;
+                    acc += seq[sub[k]].cost;
                 }
-                cur.push_back(sub[k]);
-                acc += sub[k].cost;
+                while (x < XCDS - 1) run_begin[++x] = sub.size();
+                run_begin[XCDS] = sub.size();
             }
-            flush(x);
             size_t max_blocks = 0;
-            for (auto &r : run) max_blocks = std::max(max_blocks, (r.size() + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
+            for (int xx = 0; xx < XCDS; xx++)
+                max_blocks = std::max(max_blocks, (run_begin[xx + 1] - run_begin[xx] + ITEMS_PER_BLOCK - 1) /
+                                                      ITEMS_PER_BLOCK);
             const size_t base = P.items.size();
             P.items.resize(base + max_blocks * XCDS * ITEMS_PER_BLOCK, dummy);
-            for (int xx = 0; xx < XCDS; xx++)
-                for (size_t k = 0; k < run[xx].size(); k++) {
+            for (int xx = 0; xx < XCDS; xx++) {
+                const size_t a = run_begin[xx], b = run_begin[xx + 1];
+                // descending bucket order, stable
+                std::fill(cnt.begin(), cnt.end(), 0u);
+                for (size_t k = a; k < b; k++) cnt[NBUCKET - 1 - bucket[sub[k]] + 1]++;
+                for (int q = 0; q < NBUCKET; q++) cnt[q + 1] += cnt[q];
+                sorted.resize(b - a);
+                for (size_t k = a; k < b; k++) sorted[cnt[NBUCKET - 1 - bucket[sub[k]]]++] = sub[k];
+                for (size_t k = 0; k < sorted.size(); k++) {
                     const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
-                    P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = run[xx][k];
+                    P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = seq[sorted[k]].it;
                 }
+            }
             sg.seg_count[cls] = (int)(P.items.size() - base);
         }
         return sg;

@@ -43,6 +43,19 @@ struct Plan {
     long long ni_tot = 0, n_e_slots = 0, n_s_rec = 0;
     long long n_epep = 0, n_epsp = 0;       // sum n_i n_e, sum n_i n_s
     std::vector<PlanSeg> segs;              // at least one
+    // empties the plan, keeping the vectors' memory for the next one
+    void reset()
+    {
+        walks.clear();
+        es_off.clear();
+        ss_off.clear();
+        items.clear();
+        ritems.clear();
+        segs.clear();
+        for (int k = 0; k < 3; k++) seg_begin[k] = seg_count[k] = 0;
+        n_part = n_real_items = 0;
+        ni_tot = n_e_slots = n_s_rec = n_epep = n_epsp = 0;
+    }
 };
 
 // Staged EPJ slots of a walk: near and far regions with at least 8 slots of

@@ -81,6 +81,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_er
             "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
             "pth_resident_replan", "pth_resident_autotune", "pth_gtree_build", "pth_gtree_sizes",
             "pth_gtree_download", "pth_gtree_timing", "pth_gtree_free", "pth_resident_from_gtree",
+            "pth_gtree_update", "pth_resident_refill",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
 
@@ -136,6 +137,8 @@ def lib():
     L.pth_gtree_download.argtypes = [vp, llp, llp, llp, ip, dp, dp, dp, llp, ip, ip, ip, ip, ip]
     L.pth_gtree_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pth_resident_from_gtree.argtypes = [vp, C.POINTER(vp), vp]
+    L.pth_gtree_update.argtypes = [vp, C.c_longlong, vp, vp, vp, C.c_int]
+    L.pth_resident_refill.argtypes = [vp, vp]
     L.pth_gtree_free.argtypes = [vp]
     L.pth_plan_debug.restype = C.c_longlong
     L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
@@ -375,6 +378,13 @@ class Resident:
             r.changeover(idx, np.arange(len(lists.epj_mass)), rin, rout)
         return r
 
+    def refill(self, tree):
+        """Refills this evaluation from a (re)built GpuTree, reusing device
+        memory (pth_resident_refill)."""
+        self.ctx.check(lib().pth_resident_refill(self._h, tree._h), "pth_resident_refill")
+        self.n_walk = int(tree.sizes()["n_group"])
+        self.ni = tree.n
+
     @classmethod
     def from_gtree(cls, ctx, tree):
         """A resident evaluation fed from a GpuTree on the device (walk w =
@@ -447,6 +457,27 @@ class GpuTree:
         ctx.check(lib().pth_gtree_build(ctx.handle, C.byref(self._h), self.n, _ptr(pos, C.c_double),
                                         _ptr(mass, C.c_double), None if rs is None else _ptr(rs, C.c_double),
                                         n_leaf_limit, n_group_limit, theta, int(canonical)), "pth_gtree_build")
+
+    def update(self, pos, mass, r_search=None):
+        """Rebuilds for new particles, reusing device memory.  numpy arrays
+        are host input; torch CUDA tensors (float64, contiguous) are read in
+        place on the device."""
+        if hasattr(pos, "data_ptr") and getattr(pos, "is_cuda", False):
+            n = int(pos.shape[0])
+            for a in (pos, mass) + (() if r_search is None else (r_search,)):
+                if a.dtype != __import__("torch").float64 or not a.is_contiguous() or not a.is_cuda:
+                    raise ValueError("device inputs must be contiguous float64 CUDA tensors")
+            ptrs = [pos.data_ptr(), mass.data_ptr(), None if r_search is None else r_search.data_ptr()]
+            keep, on_dev = None, 1
+        else:
+            p = _f64(pos).reshape(-1, 3)
+            n = p.shape[0]
+            keep = [p, _f64(mass)] + ([] if r_search is None else [_f64(np.broadcast_to(r_search, (n,)))])
+            ptrs = [keep[0].ctypes.data, keep[1].ctypes.data, None if r_search is None else keep[2].ctypes.data]
+            on_dev = 0
+        self.n = n
+        self.ctx.check(lib().pth_gtree_update(self._h, n, ptrs[0], ptrs[1], ptrs[2], on_dev), "pth_gtree_update")
+        del keep
 
     def sizes(self):
         v = [C.c_longlong() for _ in range(4)]

@@ -6,6 +6,7 @@ for entry in canonical order, as sets in batch order); and a force
 evaluation fed from the device tree (pth_resident_from_gtree) equals the
 one fed from the host lists bit for bit and the FP64 oracle within the
 kernel tolerance."""
+import ctypes as C
 import time
 
 import numpy as np
@@ -132,3 +133,84 @@ def test_device_tree_large():
     """N = 262144 with 50% binaries: the lists equal the host's entry for
     entry."""
     _compare(262144, 8, 0.5, 0.3)
+
+
+def _hip():
+    """The HIP runtime the library is linked against (already loaded), for
+    raw device buffers in tests that have no torch in the process."""
+    sf.lib()
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    return h
+
+
+def _to_device(hip, a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    p = C.c_void_p()
+    assert hip.hipMalloc(C.byref(p), a.nbytes) == 0
+    assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+    return p.value
+
+
+def test_device_tree_update_and_refill():
+    """A tree rebuilt in place (larger, then smaller N; host and device
+    inputs) equals a fresh build, and a refilled resident evaluation equals
+    a fresh one bit for bit."""
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    hip = _hip()
+    sets = []
+    for n, seed in ((8000, 31), (20000, 32), (5000, 33)):
+        m, x, _ = host.cluster(n, "plummer", kroupa=True, f_bin=0.3, seed=seed)
+        sets.append((m, x, np.full(n, 1.5 * R_OUT)))
+    g = sf.GpuTree(ctx, sets[0][1], sets[0][0], sets[0][2])
+    r = sf.Resident.from_gtree(ctx, g)
+    for k, (m, x, rs) in enumerate(sets[1:] + sets[:1]):
+        if k % 2 == 0:
+            g.update(x, m, rs)
+        else:
+            bufs = [_to_device(hip, a) for a in (x, m, rs)]
+            assert sf.lib().pth_gtree_update(g._h, len(m), bufs[0], bufs[1], bufs[2], 1) == 0
+            g.n = len(m)
+            for p in bufs:
+                assert hip.hipFree(C.c_void_p(p)) == 0
+        d = g.download()
+        r.refill(g)
+        r.run()
+        out = r.results()
+        f = sf.GpuTree(ctx, x, m, rs)
+        d2 = f.download()
+        rf = sf.Resident.from_gtree(ctx, f)
+        rf.run()
+        out2 = rf.results()
+        rf.close()
+        f.close()
+        for key in d:
+            assert np.array_equal(d[key], d2[key]), key
+        for a, b in zip(out, out2):
+            assert np.array_equal(a, b)
+        L = host.build_lists(x, m, rs, theta=0.3)
+        assert np.array_equal(d["id_epj"], L.id_epj) and np.array_equal(d["id_spj"], L.id_spj)
+    r.close()
+    g.close()
+    ctx.close()
+
+
+def test_device_tree_rejects_host_pointers_as_device_input():
+    """on_device inputs are checked on the host before any launch: arrays
+    that are not device allocations (here host arrays) are an error."""
+    ctx = sf.Context(0)
+    m, x, _ = host.cluster(1000, "plummer", seed=1)
+    g = sf.GpuTree(ctx, x, m)
+    lib = sf.lib()
+    xh = np.ascontiguousarray(x)
+    mh = np.ascontiguousarray(m)
+    assert lib.pth_gtree_update(g._h, 1000, xh.ctypes.data, mh.ctypes.data, None, 1) != 0
+    assert "device allocations" in sf.lib().pth_last_error(ctx.handle).decode()
+    with pytest.raises(RuntimeError):
+        g.sizes()  # the failed update leaves no tree
+    g.update(x, m)
+    assert g.sizes()["n_group"] >= 1
+    g.close()
+    ctx.close()
