@@ -107,6 +107,15 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
         res[label] = dict(step_ms=round(float(np.median(walls)), 3), tree_build_ms=round(float(ph[0]), 3),
                           walk_ms=round(float(ph[1]), 3), canonical_order_ms=round(float(ph[2]), 3),
                           refill_ms=round(float(ph[3]), 3), force_ms=round(float(ph[4]), 3))
+    g.update(tx, tm, tr)
+    ngb_ms = []
+    for _ in range(reps):
+        cnt, _, _, ms = g.neighbors()
+        ngb_ms.append(ms)
+    res["neighbor_search"] = dict(ms=round(float(np.median(ngb_ms)), 3), entries=int(cnt.sum()),
+                                  particles_with_neighbours=int((cnt > 1).sum()),
+                                  note="FP64 lists of |dx|^2 < max(rs_i, rs_j)^2 on the group EPJ lists, "
+                                       "count + scan + write on the device")
     acc2, pot2, nnb2 = r.results()
     same = same and bool(np.array_equal(acc2, ref[0]) and np.array_equal(pot2, ref[1]) and np.array_equal(nnb2, ref[2]))
     r.close()

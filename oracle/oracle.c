@@ -31,6 +31,12 @@
  *                            Which pairs FDPS's neighbour search returns at
  *                            the search radius is parity unpinned (FDPS is
  *                            not available).
+ *   orc_ngb_walks            all-pairs FP64 brute force (a k-d tree query
+ *                            widened to the largest r_search, filtered with
+ *                            the same FP64 expression) on clusters with
+ *                            binaries: equal lists prove both the
+ *                            restatement and that the symmetric-search
+ *                            lists hold every neighbour.
  */
 #include "oracle.h"
 #include <math.h>
@@ -505,3 +511,50 @@ int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, cons
     free(eo);
     return missing ? -1 : 0;
 }
+
+
+/* Neighbour lists over walks, src/soft_force.hpp:11-34
+ * (SearchNeighborEpEpNoSimd: rij = xi - xj, r2 = rij*rij in FP64,
+ * r_search = max(rs_i, rs_j), neighbour when r2 < r_search^2; self
+ * included) applied to each walk's EPJ list, as FDPS's
+ * getNeighborListOneParticle returns the list PeTar's cluster search reads
+ * (src/cluster_list.hpp:461-493).  count[i] is assigned for the walk-ordered
+ * i; with ngb != NULL the entries (EPJ indices, list order) are also written
+ * at ngb[offset[i]..].  Returns the total count. */
+long long orc_ngb_walks(int n_walk, const int *n_epi, const int *n_epj, const double *epi_pos,
+                        const double *epi_rs, const int *id_epj, const double *epj_pos,
+                        const double *epj_rs, long long *count, const long long *offset, int *ngb)
+{
+    long long *io = (long long *)malloc(sizeof(long long) * (size_t)(n_walk + 1));
+    long long *jo = (long long *)malloc(sizeof(long long) * (size_t)(n_walk + 1));
+    io[0] = jo[0] = 0;
+    for (int w = 0; w < n_walk; w++) {
+        io[w + 1] = io[w] + n_epi[w];
+        jo[w + 1] = jo[w] + n_epj[w];
+    }
+    long long total = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+    for (int w = 0; w < n_walk; w++) {
+        for (long long i = io[w]; i < io[w + 1]; i++) {
+            const double *xi = epi_pos + 3 * i;
+            long long c = 0;
+            for (long long k = jo[w]; k < jo[w + 1]; k++) {
+                const int j = id_epj[k];
+                const double *xj = epj_pos + 3 * (size_t)j;
+                const double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double rs = (epi_rs[i] < epj_rs[j]) ? epj_rs[j] : epi_rs[i];
+                if (r2 < rs * rs) {
+                    if (ngb) ngb[offset[i] + c] = j;
+                    c++;
+                }
+            }
+            count[i] = c;
+            total += c;
+        }
+    }
+    free(io);
+    free(jo);
+    return total;
+}
+

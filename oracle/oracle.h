@@ -107,6 +107,12 @@ int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, cons
                             double *acc, double *pot_soft, double *pot_tot, long long *nnb,
                             long long *n_amb);
 
+/* Neighbour lists (FP64, r^2 < max(rs_i, rs_j)^2, self included) over each
+ * walk's EPJ list: counts, and with ngb != NULL the entries at offset[i]. */
+long long orc_ngb_walks(int n_walk, const int *n_epi, const int *n_epj, const double *epi_pos,
+                        const double *epi_rs, const int *id_epj, const double *epj_pos,
+                        const double *epj_rs, long long *count, const long long *offset, int *ngb);
+
 #ifdef __cplusplus
 }
 #endif

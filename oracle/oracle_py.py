@@ -45,6 +45,8 @@ def orc():
         _llp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
         L.orc_soft_force_walks_co.argtypes = wl + [_dp, _dp, _llp, _ip, _ip, _dp, _dp, _dp, _llp, _dp, _dp,
                                                    _dp, _dp, _dp] + [C.c_double] * 5 + [_dp, _dp, _dp, _lp, _lp]
+        L.orc_ngb_walks.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _ip, _dp, _dp, _llp, C.c_void_p, C.c_void_p]
+        L.orc_ngb_walks.restype = C.c_longlong
         L.orc_changeover_setr.argtypes = [C.c_double] * 3 + [C.POINTER(C.c_double)] * 2
         for f in ("orc_changeover_wpot", "orc_changeover_wacc"):
             getattr(L, f).argtypes = [C.c_double] * 3
@@ -256,3 +258,24 @@ def moments(pos, mass):
     q = np.zeros(6)
     orc().orc_moments(pos.shape[0], pos.ravel(), _f64(mass), m, c, q)
     return m[0], c, q
+
+
+def ngb_walks(n_epi, n_epj, epi_pos, epi_rs, id_epj, epj_pos, epj_rs):
+    """FP64 neighbour lists over walks (src/soft_force.hpp:11-34 on each
+    walk's EPJ list): returns (count [ni], offset [ni + 1], ngb) with ngb the
+    EPJ indices of every i's neighbours in list order, self included."""
+    n_epi, n_epj = _i32(n_epi), _i32(n_epj)
+    ni = int(n_epi.sum())
+    epi_pos, epi_rs = _f64(epi_pos).reshape(-1, 3), _f64(epi_rs)
+    epj_pos, epj_rs = _f64(epj_pos).reshape(-1, 3), _f64(epj_rs)
+    ide = _i32(id_epj) if len(id_epj) else np.zeros(1, np.int32)
+    cnt = np.zeros(max(ni, 1), np.int64)
+    L = orc()
+    args = (n_epi.size, n_epi, n_epj, epi_pos, epi_rs, ide, epj_pos, epj_rs, cnt)
+    total = L.orc_ngb_walks(*args, None, None)
+    off = np.zeros(ni + 1, np.int64)
+    np.cumsum(cnt[:ni], out=off[1:])
+    ngb = np.zeros(max(total, 1), np.int32)
+    L.orc_ngb_walks(*args, off.ctypes.data, ngb.ctypes.data)
+    return cnt[:ni], off, ngb[:total]
+

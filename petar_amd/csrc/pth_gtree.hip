@@ -274,6 +274,11 @@ __global__ __launch_bounds__(256) void k_group_children(const GNode *__restrict_
     }
 }
 
+__global__ __launch_bounds__(64) void k_fill_int(int *__restrict__ p, long long n, int v)
+{
+    for (long long k = threadIdx.x; k < n; k += 64) p[k] = v;
+}
+
 __global__ __launch_bounds__(256) void k_group_keys(const GNode *__restrict__ nodes, const int *__restrict__ gnode,
                                                     long long ng, long long *__restrict__ gbeg)
 {
@@ -460,6 +465,153 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
     }
 }
 
+// Neighbour search (src/soft_force.hpp:11-34, SearchNeighborEpEpNoSimd, on
+// each group's EPJ list, which holds every symmetric-search neighbour): for
+// every i of a group the list entries j with |x_i - x_j|^2 <
+// max(rs_i, rs_j)^2 in FP64, self included.  One wave per group at a time,
+// up to NGB_T tiles of 64 i per pass (lane l holds i = l + 64 t), j staged
+// 64 at a time in LDS (read as broadcasts).  A (tile, chunk) pair is
+// skipped when the boxes of the tile's and the chunk's positions are at
+// least R = max(rs) apart: the gap g (per axis, >= 0) bounds every pair's
+// |dx| from below, and since each rounded operation is monotone,
+// gx*gx + gy*gy + gz*gz >= R*R implies dx*dx + dy*dy + dz*dz >= rs*rs in
+// the same FP64 arithmetic, so the skip never drops a neighbour.
+// Counting pass: cnt[input index of i]; writing pass: input indices of the
+// neighbours at ngb[off[input index of i]..], in list order.
+constexpr int NGB_T = 8;
+
+struct NgbArgs {
+    const double *spos, *srs;
+    const long long *order;
+    const int4 *info;  // per group: first particle, size, n_epj, n_spj
+    const long long *off_e;
+    const int *id_epj;
+    long long n_group;
+    int *cnt;
+    const long long *off;
+    int *ngb;
+};
+
+struct NgbJ {
+    double x, y, z, rs;
+};
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+struct Box {
+    double lo[3], hi[3], rs;
+};
+
+// box of the lane values (inactive lanes pass act = false)
+__device__ __forceinline__ Box wave_box(bool act, double x, double y, double z, double rs)
+{
+    Box b;
+    b.lo[0] = wave_min(act ? x : 1e300);
+    b.lo[1] = wave_min(act ? y : 1e300);
+    b.lo[2] = wave_min(act ? z : 1e300);
+    b.hi[0] = wave_max(act ? x : -1e300);
+    b.hi[1] = wave_max(act ? y : -1e300);
+    b.hi[2] = wave_max(act ? z : -1e300);
+    b.rs = wave_max(act ? rs : 0.0);
+    return b;
+}
+
+// true when no pair between the two boxes can be neighbours
+__device__ __forceinline__ bool boxes_apart(const Box &a, const Box &b)
+{
+    const double gx = fmax(0.0, fmax(b.lo[0] - a.hi[0], a.lo[0] - b.hi[0]));
+    const double gy = fmax(0.0, fmax(b.lo[1] - a.hi[1], a.lo[1] - b.hi[1]));
+    const double gz = fmax(0.0, fmax(b.lo[2] - a.hi[2], a.lo[2] - b.hi[2]));
+    const double r = fmax(a.rs, b.rs);
+    return gx * gx + gy * gy + gz * gz >= r * r;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(64) void k_ngb(const NgbArgs a)
+{
+    __shared__ NgbJ sj[64];
+    __shared__ int sid[64];
+    __shared__ Box tb[NGB_T];  // boxes of the tiles (wave-uniform)
+    const int lane = threadIdx.x;
+    for (long long g = blockIdx.x; g < a.n_group; g += gridDim.x) {
+        const int4 G = a.info[g];
+        const long long eb = a.off_e[g], ee = a.off_e[g + 1];
+        for (int s0 = 0; s0 < G.y; s0 += 64 * NGB_T) {
+            const int nt = min(NGB_T, (G.y - s0 + 63) / 64);
+            double xi[NGB_T], yi[NGB_T], zi[NGB_T], ri[NGB_T];
+            long long w[NGB_T];
+            int c[NGB_T];
+            bool act[NGB_T];
+#pragma unroll
+            for (int t = 0; t < NGB_T; t++) {
+                c[t] = 0;
+                act[t] = false;
+                xi[t] = yi[t] = zi[t] = ri[t] = 0.0;
+                w[t] = 0;
+                if (t < nt) {
+                    act[t] = s0 + 64 * t + lane < G.y;
+                    if (act[t]) {
+                        const long long i = (long long)G.x + s0 + 64 * t + lane;
+                        xi[t] = a.spos[3 * i];
+                        yi[t] = a.spos[3 * i + 1];
+                        zi[t] = a.spos[3 * i + 2];
+                        ri[t] = a.srs[i];
+                        if (WRITE) w[t] = a.off[a.order[i]];
+                    }
+                    const Box b = wave_box(act[t], xi[t], yi[t], zi[t], ri[t]);
+                    if (lane == 0) tb[t] = b;
+                }
+            }
+            for (long long k0 = eb; k0 < ee; k0 += 64) {
+                const long long k = k0 + lane;
+                const bool jact = k < ee;
+                NgbJ J{0, 0, 0, 0};
+                if (jact) {
+                    const int j = a.id_epj[k];
+                    J = NgbJ{a.spos[3 * (long long)j], a.spos[3 * (long long)j + 1], a.spos[3 * (long long)j + 2],
+                             a.srs[j]};
+                    sj[lane] = J;
+                    if (WRITE) sid[lane] = (int)a.order[j];
+                }
+                const Box cb = wave_box(jact, J.x, J.y, J.z, J.rs);
+                __syncthreads();
+                const int m = (int)min((long long)64, ee - k0);
+#pragma unroll
+                for (int t = 0; t < NGB_T; t++) {
+                    if (t >= nt || boxes_apart(tb[t], cb)) continue;  // wave-uniform (LDS broadcast)
+                    if (act[t])
+                        for (int q = 0; q < m; q++) {
+                            const NgbJ Q = sj[q];
+                            const double dx = xi[t] - Q.x, dy = yi[t] - Q.y, dz = zi[t] - Q.z;
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            const double rs = (ri[t] < Q.rs) ? Q.rs : ri[t];
+                            if (r2 < rs * rs) {
+                                if (WRITE) a.ngb[w[t] + c[t]] = sid[q];
+                                c[t]++;
+                            }
+                        }
+                }
+                __syncthreads();
+            }
+            if (!WRITE) {
+#pragma unroll
+                for (int t = 0; t < NGB_T; t++)
+                    if (t < nt && act[t]) a.cnt[a.order[(long long)G.x + s0 + 64 * t + lane]] = c[t];
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -472,6 +624,10 @@ struct pth_gtree {
         id_epj, id_spj, tmp;
     // build-time scratch, released when the build ends (or by pth_gtree_free)
     DevBuf dpos, dmass, drs, part, bnd, cnt, off, flag, gsel, nsel, gkey, gkey2, ce, cs, alt, lkey, lkey2, info;
+    // neighbour lists (pth_gtree_neighbors): count per input index, offsets
+    DevBuf ngb_cnt, ngb_off, ngb;
+    long long n_ngb = -1;  // total entries, -1 before the first search
+    float ms_ngb = 0.f;
     void release_scratch()
     {
         DevBuf *s[] = {&dpos, &dmass, &drs, &part, &bnd, &cnt, &off, &flag, &gsel, &nsel, &gkey, &gkey2, &ce, &cs,
@@ -531,6 +687,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     const int n_leaf_limit = t->n_leaf_limit, n_group_limit = t->n_group_limit;
     const double theta = t->theta;
     t->built = false;
+    t->n_ngb = -1;
     t->n = n;
     t->n_node = t->n_group = t->n_le = t->n_ls = 0;
     auto bail = [&](int rc) { return rc; };
@@ -972,6 +1129,70 @@ int pth_resident_refill(pth_resident *r, const pth_gtree *t)
     return resident_fill_gtree(r, t, "pth_resident_refill");
 }
 
+int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
+{
+    if (!t) return PTH_ERR_ARG;
+    pth_ctx *c = t->ctx;
+    if (!t->built) return c->fail(PTH_ERR_STATE, "pth_gtree_neighbors: the last build failed");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    hipStream_t st = c->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    const long long n = t->n, ng = t->n_group;
+    HIPCHK(c, t->ngb_cnt.ensure(sizeof(int) * (n + 1)), "neighbour buffers");
+    HIPCHK(c, t->ngb_off.ensure(sizeof(long long) * (n + 1)), "neighbour buffers");
+    NgbArgs na{};
+    na.spos = t->spos.as<double>();
+    na.srs = t->srs.as<double>();
+    na.order = t->order.as<long long>();
+    na.info = t->info.as<int4>();
+    na.off_e = t->off_e.as<long long>();
+    na.id_epj = t->id_epj.as<int>();
+    na.n_group = ng;
+    na.cnt = t->ngb_cnt.as<int>();
+    const unsigned wg = (unsigned)std::max<long long>(1, std::min<long long>(ng, 65536));
+    // the count of entry n stays 0 so the scan's last entry is the total
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, t->ngb_cnt.as<int>() + n, 1, 0);
+    if (ng > 0) hipLaunchKernelGGL(k_ngb<false>, dim3(wg), dim3(64), 0, st, na);
+    size_t tb = 0;
+    hipError_t e;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t->ngb_cnt.as<int>(), t->ngb_off.as<long long>(),
+                                              (int)(n + 1), st)) ||
+        (e = cub_temp(t, tb)) ||
+        (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, t->ngb_cnt.as<int>(), t->ngb_off.as<long long>(),
+                                              (int)(n + 1), st)))
+        return c->hip_fail(e, "neighbour scan");
+    long long total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, t->ngb_off.as<long long>() + n, 8, hipMemcpyDeviceToHost, st), "neighbour total");
+    HIPCHK(c, hipStreamSynchronize(st), "neighbour count");
+    if (total > 0x7fffffffLL) return c->fail(PTH_ERR_CAPACITY, "pth_gtree_neighbors: lists exceed the int32 range");
+    HIPCHK(c, t->ngb.ensure(sizeof(int) * std::max<long long>(total, 1)), "neighbour buffers");
+    na.off = t->ngb_off.as<long long>();
+    na.ngb = t->ngb.as<int>();
+    if (ng > 0 && total > 0) hipLaunchKernelGGL(k_ngb<true>, dim3(wg), dim3(64), 0, st, na);
+    HIPCHK(c, hipGetLastError(), "neighbour search");
+    HIPCHK(c, hipStreamSynchronize(st), "neighbour search");
+    t->n_ngb = total;
+    t->ms_ngb = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (n_total) *n_total = total;
+    return PTH_OK;
+}
+
+int pth_gtree_neighbors_download(const pth_gtree *t, int *count, long long *offset, int *ngb, float *ms)
+{
+    if (!t) return PTH_ERR_ARG;
+    pth_ctx *c = t->ctx;
+    if (!t->built || t->n_ngb < 0)
+        return c->fail(PTH_ERR_STATE, "pth_gtree_neighbors_download: no neighbour search since the last build");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    if (count) HIPCHK(c, hipMemcpy(count, t->ngb_cnt.p, sizeof(int) * t->n, hipMemcpyDeviceToHost), "download");
+    if (offset)
+        HIPCHK(c, hipMemcpy(offset, t->ngb_off.p, sizeof(long long) * (t->n + 1), hipMemcpyDeviceToHost), "download");
+    if (ngb && t->n_ngb > 0)
+        HIPCHK(c, hipMemcpy(ngb, t->ngb.p, sizeof(int) * t->n_ngb, hipMemcpyDeviceToHost), "download");
+    if (ms) *ms = t->ms_ngb;
+    return PTH_OK;
+}
+
 void pth_gtree_free(pth_gtree *t)
 {
     if (!t) return;
@@ -981,7 +1202,7 @@ void pth_gtree_free(pth_gtree *t)
     }
     DevBuf *all[] = {&t->order, &t->key, &t->skey, &t->sidx, &t->spos, &t->smass, &t->srs, &t->nodes, &t->nmass,
                      &t->ncom, &t->nquad, &t->box_in, &t->box_out, &t->groups, &t->off_e, &t->off_s, &t->id_epj,
-                     &t->id_spj, &t->tmp};
+                     &t->id_spj, &t->tmp, &t->ngb_cnt, &t->ngb_off, &t->ngb};
     for (DevBuf *b : all) b->release();
     t->release_scratch();
     delete t;

@@ -81,7 +81,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_er
             "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
             "pth_resident_replan", "pth_resident_autotune", "pth_gtree_build", "pth_gtree_sizes",
             "pth_gtree_download", "pth_gtree_timing", "pth_gtree_free", "pth_resident_from_gtree",
-            "pth_gtree_update", "pth_resident_refill",
+            "pth_gtree_update", "pth_resident_refill", "pth_gtree_neighbors", "pth_gtree_neighbors_download",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
 
@@ -139,6 +139,8 @@ def lib():
     L.pth_resident_from_gtree.argtypes = [vp, C.POINTER(vp), vp]
     L.pth_gtree_update.argtypes = [vp, C.c_longlong, vp, vp, vp, C.c_int]
     L.pth_resident_refill.argtypes = [vp, vp]
+    L.pth_gtree_neighbors.argtypes = [vp, llp]
+    L.pth_gtree_neighbors_download.argtypes = [vp, ip, llp, ip, C.POINTER(C.c_float)]
     L.pth_gtree_free.argtypes = [vp]
     L.pth_plan_debug.restype = C.c_longlong
     L.pth_plan_debug.argtypes = [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_longlong,
@@ -501,6 +503,21 @@ class GpuTree:
         d["id_epj"] = d["id_epj"][:z["n_epj_list"]]
         d["id_spj"] = d["id_spj"][:z["n_spj_list"]]
         return d
+
+    def neighbors(self):
+        """Neighbour lists on the device (pth_gtree_neighbors): returns
+        (count [n], offset [n + 1], ngb, ms) indexed by input particle index,
+        self included."""
+        tot = C.c_longlong()
+        self.ctx.check(lib().pth_gtree_neighbors(self._h, C.byref(tot)), "pth_gtree_neighbors")
+        cnt = np.zeros(self.n, np.int32)
+        off = np.zeros(self.n + 1, np.int64)
+        ngb = np.zeros(max(tot.value, 1), np.int32)
+        ms = C.c_float()
+        self.ctx.check(lib().pth_gtree_neighbors_download(self._h, _ptr(cnt, C.c_int), _ptr(off, C.c_longlong),
+                                                          _ptr(ngb, C.c_int), C.byref(ms)),
+                       "pth_gtree_neighbors_download")
+        return cnt, off, ngb[:tot.value], ms.value
 
     def timing(self):
         """Host wall ms of the last build: (tree, walk, canonical ordering)."""

@@ -214,3 +214,58 @@ def test_device_tree_rejects_host_pointers_as_device_input():
     assert g.sizes()["n_group"] >= 1
     g.close()
     ctx.close()
+
+
+def _expected_ngb(L, ocnt, ooff, ongb):
+    """Oracle lists (tree order, EPJ positions) re-indexed by input index."""
+    n = L.order.size
+    rowp = np.repeat(np.arange(n), ocnt)
+    o = np.argsort(L.order[rowp], kind="stable")
+    cnt = np.zeros(n, np.int64)
+    cnt[L.order] = ocnt
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(cnt, out=off[1:])
+    return cnt, off, L.order[ongb[o]]
+
+
+@pytest.mark.parametrize("n,f_bin,theta", [(30000, 0.5, 0.3), (4000, 0.3, 0.0), (20000, 0.0, 0.8)])
+def test_device_neighbors_equal_oracle(n, f_bin, theta):
+    """pth_gtree_neighbors against the FP64 restatement on the host tree's
+    lists: counts, offsets and entries equal (order included)."""
+    m, x, _ = host.cluster(n, "plummer", kroupa=True, f_bin=f_bin, seed=41)
+    rs = 1.5 * R_OUT * np.maximum(np.cbrt(m / m.mean()), 1.0)
+    L = host.build_lists(x, m, rs, theta=theta)
+    ocnt, ooff, ongb = O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj, L.epj_pos, L.epj_rs)
+    ctx = sf.Context(0)
+    g = sf.GpuTree(ctx, x, m, rs, theta=theta)
+    cnt, off, ngb, ms = g.neighbors()
+    g.close()
+    ctx.close()
+    ecnt, eoff, engb = _expected_ngb(L, ocnt, ooff, ongb)
+    print(f"[ngb N={n} theta={theta}] {ngb.size} entries, {(cnt > 1).sum()} particles with neighbours, {ms:.2f} ms",
+          flush=True)
+    assert np.array_equal(cnt, ecnt) and np.array_equal(off, eoff) and np.array_equal(ngb, engb)
+
+
+def test_device_neighbors_edge_cases():
+    """No r_search (no neighbours, not even self), one particle, and a
+    repeated search after an update."""
+    ctx = sf.Context(0)
+    m, x, _ = host.cluster(3000, "plummer", seed=3)
+    g = sf.GpuTree(ctx, x, m)
+    cnt, off, ngb, _ = g.neighbors()
+    assert cnt.sum() == 0 and off[-1] == 0 and ngb.size == 0
+    g.close()
+    g = sf.GpuTree(ctx, np.zeros((1, 3)), np.ones(1), np.full(1, 0.1))
+    cnt, off, ngb, _ = g.neighbors()
+    assert list(cnt) == [1] and list(off) == [0, 1] and list(ngb) == [0]
+    m2, x2, _ = host.cluster(5000, "plummer", kroupa=True, f_bin=0.5, seed=4)
+    rs2 = np.full(5000, 1.5 * R_OUT)
+    g.update(x2, m2, rs2)
+    cnt, off, ngb, _ = g.neighbors()
+    L = host.build_lists(x2, m2, rs2, theta=0.3)
+    ecnt, eoff, engb = _expected_ngb(L, *O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj, L.epj_pos,
+                                                     L.epj_rs))
+    assert np.array_equal(cnt, ecnt) and np.array_equal(ngb, engb)
+    g.close()
+    ctx.close()

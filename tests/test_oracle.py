@@ -395,3 +395,61 @@ def test_co_missing_own_entry_is_an_error():
     with pytest.raises(RuntimeError):
         O.soft_force_walks_co([1], [1], [0], pos[:1], [0.015], [7], [1], [], pos, [1e-4, 1e-4], [0.015, 0.015],
                               [6, 8], [0.001, 0.001], [0.01, 0.01], [], [], [], 0.0, 1e-4)
+
+
+def _brute_ngb_pairs(pos, rs):
+    """All (i, j) with |x_i - x_j|^2 < max(rs_i, rs_j)^2 in FP64 (self
+    included), by a k-d tree query widened to the largest radius and the
+    exact FP64 filter; sorted by (i, j)."""
+    from scipy.spatial import cKDTree
+    pairs = cKDTree(pos).query_pairs(float(rs.max()) * (1 + 1e-9), output_type="ndarray")
+    i, j = pairs[:, 0], pairs[:, 1]
+    d = pos[i] - pos[j]
+    r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+    r = np.maximum(rs[i], rs[j])
+    keep = r2 < r * r
+    i, j = i[keep], j[keep]
+    n = len(pos)
+    selfp = np.arange(n)[rs > 0]
+    a = np.concatenate([i, j, selfp])
+    b = np.concatenate([j, i, selfp])
+    o = np.lexsort((b, a))
+    return a[o], b[o]
+
+
+@pytest.mark.parametrize("f_bin,theta", [(0.5, 0.3), (0.0, 0.7)])
+def test_ngb_oracle_equals_brute_force(f_bin, theta):
+    """The neighbour-list restatement on the symmetric-search tree lists
+    equals an all-pairs FP64 search: the restatement is right and the
+    lists hold every neighbour (mass-scaled r_search, binaries)."""
+    from petar_amd import host
+    m, x, _ = host.cluster(12000, "plummer", kroupa=True, f_bin=f_bin, seed=17)
+    rs = 1.5 * R_OUT * np.maximum(np.cbrt(m / m.mean()), 1.0)
+    L = host.build_lists(x, m, rs, theta=theta)
+    idx = np.concatenate([np.arange(b, b + k) for b, k in zip(L.epi_begin, L.n_epi)])
+    assert np.array_equal(idx, np.arange(len(m)))
+    cnt, off, ngb = O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos[idx], L.epj_rs[idx], L.id_epj, L.epj_pos, L.epj_rs)
+    a = np.repeat(np.arange(len(m)), cnt)
+    o = np.lexsort((ngb, a))
+    ba, bb = _brute_ngb_pairs(L.epj_pos, L.epj_rs)
+    assert np.array_equal(a[o], ba) and np.array_equal(ngb[o], bb)
+    assert cnt.min() >= 1 and (cnt > 1).sum() > 100  # self always, binaries give real neighbours
+
+
+def test_ngb_oracle_catches_a_dropped_neighbour():
+    """A planted mistake (the last neighbour of one particle dropped) is
+    seen by the brute-force comparison."""
+    from petar_amd import host
+    m, x, _ = host.cluster(3000, "plummer", kroupa=True, f_bin=0.5, seed=18)
+    rs = np.full(len(m), 1.5 * R_OUT)
+    L = host.build_lists(x, m, rs, theta=0.3)
+    cnt, off, ngb = O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj, L.epj_pos, L.epj_rs)
+    k = int(np.argmax(cnt))
+    keep = np.ones(ngb.size, bool)
+    keep[off[k + 1] - 1] = False
+    cnt2 = cnt.copy()
+    cnt2[k] -= 1
+    a = np.repeat(np.arange(len(m)), cnt2)
+    o = np.lexsort((ngb[keep], a))
+    ba, bb = _brute_ngb_pairs(L.epj_pos, L.epj_rs)
+    assert not (a.size == ba.size and np.array_equal(a[o], ba) and np.array_equal(ngb[keep][o], bb))
