@@ -90,6 +90,38 @@ int pth_tree_lists(const pth_tree *tree, int *id_epj, int *id_spj);
 
 void pth_tree_free(pth_tree *tree);
 
+/* ---- PeTar snapshot files ------------------------------------------------ */
+/* The files PeTar writes and reads with writeParticleAscii / Binary
+ * (src/petar.hpp:1905-1908, 2757-2763): a FileHeader (src/io.hpp:210-282:
+ * file id, N, time [, pos_offset[3], vel_offset[3]]) and one FPSoft record
+ * per particle (ParticleBase, Ptcl, FPSoft fields, src/particle_base.hpp,
+ * src/ptcl.hpp, src/soft_ptcl.hpp).  Columns, by the names of PeTar's own
+ * reader (tools/analysis/data.py:390-399): mass, pos[3], vel[3],
+ * binary_state, [radius, dm, time_record, time_interrupt], r_search, id,
+ * mass_bk, status, r_in, r_out, acc_soft[3], pot, pot_soft, [pot_ext], n_nb;
+ * binary_state, id, mass_bk, status and n_nb are int64, the others FP64. */
+#define PTH_SNAP_BINARY 0   /* data_format r/w BINARY (petar -i 0) */
+#define PTH_SNAP_ASCII 1    /* data_format r/w ASCII (petar -i 1, the default) */
+#define PTH_SNAP_EXTERNAL 1 /* layout flag: external-potential build (header offsets, pot_ext) */
+#define PTH_SNAP_SE 2       /* layout flag: STELLAR_EVOLUTION base fields (no BSE star data) */
+typedef struct pth_snapshot pth_snapshot;
+/* Reads a snapshot of the given format and layout flags; a file whose size
+ * or records do not match the layout is an error naming the mismatch. */
+int pth_snapshot_read(const char *path, int format, int flags, pth_snapshot **snap);
+/* An empty (zeroed) snapshot of n particles, for writing. */
+int pth_snapshot_new(long long n, int flags, pth_snapshot **snap);
+int pth_snapshot_header(const pth_snapshot *snap, long long *file_id, long long *n, double *time,
+                        double *pos_offset, double *vel_offset);
+int pth_snapshot_set_header(pth_snapshot *snap, long long file_id, double time, const double *pos_offset,
+                            const double *vel_offset);
+/* Width (1 or 3) of a column, 0 when the layout has none of that name. */
+int pth_snapshot_column_width(const pth_snapshot *snap, const char *name);
+/* Copies a column out ([n] or [n][3], int64 or FP64 as listed above) / in. */
+int pth_snapshot_column(const pth_snapshot *snap, const char *name, void *out);
+int pth_snapshot_set_column(pth_snapshot *snap, const char *name, const void *in);
+int pth_snapshot_write(const pth_snapshot *snap, const char *path, int format);
+void pth_snapshot_free(pth_snapshot *snap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -269,3 +269,62 @@ def test_device_neighbors_edge_cases():
     assert np.array_equal(cnt, ecnt) and np.array_equal(ngb, engb)
     g.close()
     ctx.close()
+
+
+def test_snapshot_feeds_the_device_step(tmp_path):
+    """A PeTar snapshot (ASCII, as PeTar writes by default) read back and fed
+    to the device tree and the changeover-mode evaluation with its own
+    r_search, r_in, r_out and ids: bit-identical to the same step fed from
+    the arrays the snapshot was written from; then the same clusters through
+    the serving loop equal their resident evaluations."""
+    from petar_amd import snapshot as S
+    arrays, paths = [], []
+    for k, n in enumerate((6000, 9000, 3000)):
+        m, x, v = host.cluster(n, "plummer", kroupa=True, f_bin=0.3, seed=60 + k, with_vel=True)
+        rout = R_OUT * np.maximum(np.cbrt(m / m.mean()), 1.0)
+        s = S.Snapshot.new(n, file_id=k, time=0.125 * k)
+        s["mass"], s["pos"], s["vel"] = m, x, v
+        s["r_search"], s["r_in"], s["r_out"], s["id"] = 1.5 * rout, 0.1 * rout, rout, np.arange(1, n + 1) * 3
+        p = tmp_path / f"data.{k}"
+        s.write(p, "ascii")
+        s.close()
+        arrays.append((m, x, 1.5 * rout, 0.1 * rout, rout, np.arange(1, n + 1) * 3))
+        paths.append(p)
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, changeover=1)
+    for p, (m, x, rs, rin, rout, ids) in zip(paths, arrays):
+        inp = S.soft_inputs(S.Snapshot.read(p, "ascii"))
+        g = sf.GpuTree(ctx, inp["pos"], inp["mass"], inp["r_search"])
+        order = g.download()["order"]
+        rd = sf.Resident.from_gtree(ctx, g)
+        g.close()
+        rd.changeover(inp["id"][order], inp["id"][order], inp["r_in"][order], inp["r_out"][order])
+        rd.run()
+        out_d = rd.results()
+        rd.close()
+        L = host.build_lists(x, m, rs, theta=0.3)
+        rh = sf.Resident.from_lists(ctx, L, rin[L.order], rout[L.order])
+        rh.changeover(ids[L.order], ids[L.order], rin[L.order], rout[L.order])
+        rh.run()
+        out_h = rh.results()
+        rh.close()
+        for a, b in zip(out_d, out_h):
+            assert np.array_equal(a, b)
+    ctx.close()
+    # serving: the snapshot clusters as requests
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=0, n_streams=2)
+    lists = []
+    for p in paths:
+        inp = S.soft_inputs(S.Snapshot.read(p, "ascii"))
+        lists.append(host.build_lists(inp["pos"], inp["mass"], inp["r_search"], theta=0.3))
+    ids = [srv.add_lists(L) for L in lists]
+    srv.replay(np.array(ids * 2), np.zeros(2 * len(ids)))
+    for k, L in zip(ids, lists):
+        r = sf.Resident.from_lists(ctx, L)
+        r.run()
+        ref = r.results()
+        r.close()
+        got = srv.results(k)
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+    srv.close()
+    ctx.close()
