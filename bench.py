@@ -291,7 +291,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--segments", type=int, default=8)
-    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--streams", type=int, default=3)
     ap.add_argument("--direct-sample", type=int, default=4096,
                     help="i sampled for the error against the FP64 all-pairs direct sum (0: skip)")
     ap.add_argument("--serve-streams", type=int, default=4)

@@ -86,10 +86,13 @@ typedef struct pth_params {
     double ngb_factor; /* changeover mode: neighbour radius factor on r_search
                           (0 = 0.99, SAFTY_FACTOR_FOR_SEARCH of src/ptcl.hpp:8) */
     int segments;  /* resident evaluations: walks split into this many segments
-                      of equal cost (1..32), run concurrently on `streams`
-                      streams so staging overlaps the force kernels; results do
-                      not depend on it.  pth_dispatch always uses one. */
-    int streams;   /* resident evaluations: streams for the segments (1..8) */
+                      of equal cost (1..32) and pipelined: one stream stages
+                      the segments in turn, the others run each segment's
+                      force and reduce kernels once it is staged, so staging
+                      overlaps the force kernels; results do not depend on
+                      it.  pth_dispatch always uses one. */
+    int streams;   /* resident evaluations: streams of the pipeline (1..8):
+                      one staging, streams - 1 force streams */
 } pth_params;
 
 /* Changeover mode folds PeTar's changeover correction of the soft force
@@ -108,7 +111,7 @@ typedef struct pth_params {
 
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
  * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99,
- * segments 8, streams 4. */
+ * segments 8, streams 3. */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */

@@ -29,8 +29,8 @@ void pth_params_default(pth_params *p)
     p->j_vgpr = 0;   // SGPR operands beat VGPR copies at every max_ipl
     p->changeover = 0;
     p->ngb_factor = 0.99;
-    p->segments = 8;  // measured on the precision workload: 8 segments on 4 streams
-    p->streams = 4;
+    p->segments = 8;  // measured on the precision workload: 8 segments, 1 staging + 2 force streams
+    p->streams = 3;
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -555,7 +555,7 @@ int pth_resident_run(pth_resident *r)
         return c->fail(PTH_ERR_STATE, "pth_resident_run: changeover mode needs pth_resident_changeover first");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     HIPCHK(c, hipEventRecord(r->ev[0], c->stream), "hipEventRecord");
-    if (r->side.empty() || r->b.n_seg < 2) {
+    if (r->side.empty() || r->b.n_seg < 2 || (int)r->staged.size() < r->b.n_seg) {
         for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_stage(r->b, k, c->stream), "stage kernels");
         HIPCHK(c, hipEventRecord(r->ev[1], c->stream), "hipEventRecord");
         for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_force(r->b, k, c->stream), "force kernel");
@@ -564,13 +564,19 @@ int pth_resident_run(pth_resident *r)
         HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
         r->pipelined = false;
     } else {
-        // segment k on side stream k mod n: its staging overlaps the force
-        // kernels of the segments before it on the other streams
+        // a pipeline: the first side stream stages the segments one after
+        // another; segment k's force and reduce run on side stream
+        // 1 + k mod (n - 1) once its staging is done, so the staging of the
+        // segments after k overlaps k's force kernels
         const int ns = (int)r->side.size();
         for (int s = 0; s < ns; s++) HIPCHK(c, hipStreamWaitEvent(r->side[s], r->ev[0], 0), "hipStreamWaitEvent");
         for (int k = 0; k < r->b.n_seg; k++) {
-            hipStream_t st = r->side[k % ns];
-            HIPCHK(c, launch_stage(r->b, k, st), "stage kernels");
+            HIPCHK(c, launch_stage(r->b, k, r->side[0]), "stage kernels");
+            HIPCHK(c, hipEventRecord(r->staged[k], r->side[0]), "hipEventRecord");
+        }
+        for (int k = 0; k < r->b.n_seg; k++) {
+            hipStream_t st = r->side[ns > 1 ? 1 + k % (ns - 1) : 0];
+            HIPCHK(c, hipStreamWaitEvent(st, r->staged[k], 0), "hipStreamWaitEvent");
             HIPCHK(c, launch_force(r->b, k, st), "force kernel");
             HIPCHK(c, launch_reduce(r->b, k, st), "reduce kernel");
         }
@@ -761,6 +767,8 @@ void pth_resident_free(pth_resident *r)
         if (r->ev[k]) (void)hipEventDestroy(r->ev[k]);
     for (hipEvent_t x : r->side_done)
         if (x) (void)hipEventDestroy(x);
+    for (hipEvent_t x : r->staged)
+        if (x) (void)hipEventDestroy(x);
     for (hipStream_t x : r->side)
         if (x) (void)hipStreamDestroy(x);
     delete r;
@@ -849,6 +857,10 @@ int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_e
             if ((e = hipStreamCreateWithFlags(&r->side.back(), hipStreamNonBlocking))) return bail(e, "hipStreamCreate");
             if ((e = hipEventCreateWithFlags(&r->side_done.back(), hipEventDisableTiming)))
                 return bail(e, "hipEventCreate");
+        }
+        while (r->staged.size() < P.segs.size()) {
+            r->staged.push_back(nullptr);
+            if ((e = hipEventCreateWithFlags(&r->staged.back(), hipEventDisableTiming))) return bail(e, "hipEventCreate");
         }
     }
     r->n_epj_tot = n_epj_tot;

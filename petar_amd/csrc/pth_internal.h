@@ -299,6 +299,7 @@ struct pth_resident {
     int n_seg = 1;                       // walk segments of the plan
     std::vector<hipStream_t> side;       // streams the segments run on
     std::vector<hipEvent_t> side_done;   // one per side stream
+    std::vector<hipEvent_t> staged;      // one per segment: its staging is done
     bool pipelined = false;              // the last run used the side streams
     Work work;
     BatchDev b;
