@@ -114,8 +114,8 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
         ngb_ms.append(ms)
     res["neighbor_search"] = dict(ms=round(float(np.median(ngb_ms)), 3), entries=int(cnt.sum()),
                                   particles_with_neighbours=int((cnt > 1).sum()),
-                                  note="FP64 lists of |dx|^2 < max(rs_i, rs_j)^2 on the group EPJ lists, "
-                                       "count + scan + write on the device")
+                                  note="FP64 lists of |dx|^2 < max(rs_i, rs_j)^2: per-tile r_search walks, "
+                                       "exact count + scan + write on the device")
     acc2, pot2, nnb2 = r.results()
     same = same and bool(np.array_equal(acc2, ref[0]) and np.array_equal(pot2, ref[1]) and np.array_equal(nnb2, ref[2]))
     r.close()

@@ -301,14 +301,16 @@ void pth_gtree_free(pth_gtree *tree);
  * src/petar.hpp:767-788, src/soft_force.hpp:11-34,239-283, and the
  * getNeighborListOneParticle lists its cluster search reads,
  * src/cluster_list.hpp:461-493): for every particle i the particles j with
- * |x_i - x_j|^2 < max(r_search_i, r_search_j)^2 in FP64, self included,
- * taken from i's group EPJ list (which holds every such j).  Runs on the
- * device and keeps the lists there; n_total (may be NULL) receives the
- * number of entries. */
+ * |x_i - x_j|^2 < max(r_search_i, r_search_j)^2 in FP64, self included.
+ * Candidates come from one symmetric r_search walk per tile (the largest
+ * cells with at most 64 particles, boxes widened beyond any rounding), so
+ * the rows are every such j, in ascending tree order.  Runs on the device
+ * and keeps the lists there; n_total (may be NULL) receives the number of
+ * entries. */
 int pth_gtree_neighbors(pth_gtree *tree, long long *n_total);
 /* Copies the last search's lists, indexed by input particle index: count
  * [n], offset [n + 1] (row i is ngb[offset[i] .. offset[i + 1])), ngb
- * [n_total] (input indices, in the group list's order); ms (may be NULL)
+ * [n_total] (input indices, in ascending tree order); ms (may be NULL)
  * receives the search's host wall time.  Any pointer may be NULL. */
 int pth_gtree_neighbors_download(const pth_gtree *tree, int *count, long long *offset, int *ngb, float *ms);
 /* A resident evaluation (as pth_resident_create) fed from the tree without a

@@ -56,34 +56,42 @@ __device__ __forceinline__ unsigned long long spread3(unsigned long long v)
     return v;
 }
 
-// per-block min/max of the coordinates: part[block * 6 + {0..2: min, 3..5: max}]
-__global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ pos, long long n, double *__restrict__ part)
+// per-block min/max of the coordinates and max of r_search (rs may be null):
+// part[block * 7 + {0..2: min, 3..5: max, 6: max rs}]
+__global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ pos, const double *__restrict__ rs,
+                                              long long n, double *__restrict__ part)
 {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rmax = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
         for (int d = 0; d < 3; d++) {
             const double x = pos[3 * i + d];
             lo[d] = fmin(lo[d], x);
             hi[d] = fmax(hi[d], x);
         }
-    __shared__ double red[4][6];
+        if (rs) rmax = fmax(rmax, rs[i]);
+    }
+    __shared__ double red[4][7];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1) {
         for (int d = 0; d < 3; d++) {
             lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
             hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
         }
-    if (lane == 0)
+        rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+    }
+    if (lane == 0) {
         for (int d = 0; d < 3; d++) {
             red[wave][d] = lo[d];
             red[wave][3 + d] = hi[d];
         }
+        red[wave][6] = rmax;
+    }
     __syncthreads();
-    if (threadIdx.x < 6) {
+    if (threadIdx.x < 7) {
         const int d = threadIdx.x;
         double v = red[0][d];
         for (int w = 1; w < 4; w++) v = (d < 3) ? fmin(v, red[w][d]) : fmax(v, red[w][d]);
-        part[blockIdx.x * 6 + d] = v;
+        part[blockIdx.x * 7 + d] = v;
     }
 }
 
@@ -361,6 +369,10 @@ struct WalkArgs {
     const double *ncom, *box_in, *box_out;
     const int *groups;
     long long n_group;
+    // tile walks (tbox != null): group g is node groups[g] with the boxes
+    // tbox[12 g ..] (particle box, box widened by r_search) instead of the
+    // node's own
+    const double *tbox;
     double root_len, inv_theta2;
     int theta_pos;
     long long *cnt_e, *cnt_s;                // counting pass
@@ -394,16 +406,17 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
     for (long long g = blockIdx.x; g < a.n_group; g += gridDim.x) {
         const int gn = a.groups[g];
         const GNode G = a.nodes[gn];
+        const long long gb = G.begin, ge = G.end;
         double bin[6], bout[6];
         for (int d = 0; d < 6; d++) {
-            bin[d] = a.box_in[6 * (long long)gn + d];
-            bout[d] = a.box_out[6 * (long long)gn + d];
+            bin[d] = a.tbox ? a.tbox[12 * g + d] : a.box_in[6 * (long long)gn + d];
+            bout[d] = a.tbox ? a.tbox[12 * g + 6 + d] : a.box_out[6 * (long long)gn + d];
         }
         long long ne = 0, ns = 0;
         int *oe = nullptr, *os = nullptr;
         if (WRITE) {
             oe = a.id_epj + a.off_e[g];
-            os = a.id_spj + a.off_s[g];
+            if (a.id_spj) os = a.id_spj + a.off_s[g];
         }
         if (lane == 0) stk[0] = 0;
         int size = 1;
@@ -422,7 +435,7 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
             if (act) {
                 nd = a.nodes[c];
                 bool open;
-                if (nd.begin <= G.begin && G.end <= nd.end) open = true;
+                if (nd.begin <= gb && ge <= nd.end) open = true;
                 else if (!a.theta_pos) open = true;
                 else {
                     const double L = a.root_len / (double)(1LL << nd.level);
@@ -448,7 +461,7 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
             const long long te = __shfl(pe, 63, 64), tc = __shfl(pc, 63, 64), ts = __shfl(ps, 63, 64);
             if (WRITE) {
                 for (long long t = 0; t < me; t++) oe[ne + pe - me + t] = (int)(nd.begin + t);
-                if (ms) os[ns + ps - 1] = c;
+                if (ms && os) os[ns + ps - 1] = c;
             }
             ne += te;
             ns += ts;
@@ -459,17 +472,54 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
         }
         if (!WRITE && lane == 0) {
             a.cnt_e[g] = ne;
-            a.cnt_s[g] = ns;
+            if (a.cnt_s) a.cnt_s[g] = ns;
         }
         __syncthreads();
     }
 }
 
-// Neighbour search (src/soft_force.hpp:11-34, SearchNeighborEpEpNoSimd, on
-// each group's EPJ list, which holds every symmetric-search neighbour): for
-// every i of a group the list entries j with |x_i - x_j|^2 <
-// max(rs_i, rs_j)^2 in FP64, self included.  One wave per group at a time,
-// up to NGB_T tiles of 64 i per pass (lane l holds i = l + 64 t), j staged
+// Boxes of the neighbour-search tiles (the largest cells with at most 64
+// particles): the cell's particle box and its box widened by r_search,
+// pushed outwards by (|v| + rs_max) 2^-40, far above the rounding of any
+// coordinate, difference or box bound involved, so a tile walk with these
+// boxes opens every leaf holding a neighbour of a tile particle under the
+// exact FP64 test (the walk's lists are a superset; k_ngb filters exactly).
+__global__ __launch_bounds__(256) void k_tile_boxes(const int *__restrict__ tiles, long long nt,
+                                                    const double *__restrict__ box_in,
+                                                    const double *__restrict__ box_out, double rs_max,
+                                                    double *__restrict__ tbox)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    const long long c = tiles[t];
+    const double m = 0x1p-40;
+    for (int d = 0; d < 3; d++) {
+        const double lo = box_in[6 * c + d], hi = box_in[6 * c + 3 + d];
+        const double lo2 = box_out[6 * c + d], hi2 = box_out[6 * c + 3 + d];
+        tbox[12 * t + d] = lo - (fabs(lo) + rs_max) * m;
+        tbox[12 * t + 3 + d] = hi + (fabs(hi) + rs_max) * m;
+        tbox[12 * t + 6 + d] = lo2 - (fabs(lo2) + rs_max) * m;
+        tbox[12 * t + 9 + d] = hi2 + (fabs(hi2) + rs_max) * m;
+    }
+}
+
+// per tile: first particle, size, candidate count (k_ngb's group record)
+__global__ __launch_bounds__(256) void k_tile_info(const GNode *__restrict__ nodes, const int *__restrict__ tiles,
+                                                   long long nt, const long long *__restrict__ off,
+                                                   int4 *__restrict__ info)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    const GNode c = nodes[tiles[t]];
+    info[t] = make_int4((int)c.begin, (int)(c.end - c.begin), (int)(off[t + 1] - off[t]), 0);
+}
+
+// Neighbour search (src/soft_force.hpp:11-34, SearchNeighborEpEpNoSimd) on
+// candidate lists that hold every symmetric-search neighbour (the tile walks
+// of pth_gtree_neighbors, or a group's EPJ list): for every i of a tile the
+// list entries j with |x_i - x_j|^2 < max(rs_i, rs_j)^2 in FP64, self
+// included.  One wave per tile at a time, up to NGB_T sub-tiles of 64 i per
+// pass (lane l holds i = l + 64 t), j staged
 // 64 at a time in LDS (read as broadcasts).  A (tile, chunk) pair is
 // skipped when the boxes of the tile's and the chunk's positions are at
 // least R = max(rs) apart: the gap g (per axis, >= 0) bounds every pair's
@@ -624,9 +674,11 @@ struct pth_gtree {
         id_epj, id_spj, tmp;
     // build-time scratch, released when the build ends (or by pth_gtree_free)
     DevBuf dpos, dmass, drs, part, bnd, cnt, off, flag, gsel, nsel, gkey, gkey2, ce, cs, alt, lkey, lkey2, info;
-    // neighbour lists (pth_gtree_neighbors): count per input index, offsets
-    DevBuf ngb_cnt, ngb_off, ngb;
+    // neighbour lists (pth_gtree_neighbors): count per input index, offsets,
+    // entries; tile boxes, candidate counts / offsets / lists, tile records
+    DevBuf ngb_cnt, ngb_off, ngb, tiles, tbox, tcnt, toff, tl, tl_alt, tinfo;
     long long n_ngb = -1;  // total entries, -1 before the first search
+    long long n_cand = 0;  // candidate pairs of the last search
     float ms_ngb = 0.f;
     void release_scratch()
     {
@@ -638,6 +690,7 @@ struct pth_gtree {
     std::vector<int4> groups_h;  // per group: first particle, size, n_epj, n_spj
     int n_leaf_limit = 20, n_group_limit = 512;
     double theta = 0.3;
+    double rs_max = 0.0, root_len = 1.0;  // largest r_search, root cell side
     bool canonical = false;
     bool built = false;
     float ms_build = 0.f, ms_walk = 0.f, ms_order = 0.f;
@@ -678,6 +731,48 @@ bool device_range(const void *p, size_t bytes)
     return (const char *)p + bytes <= (const char *)base + size;
 }
 
+// The largest cells with at most `limit` particles (or leaves), as node
+// indices in Morton order (by first particle): the i-groups for limit =
+// n_group_limit, the neighbour-search tiles for limit = 64.
+int select_cells(pth_gtree *t, int limit, DevBuf &out, long long &n_out)
+{
+    pth_ctx *ctx = t->ctx;
+    hipStream_t st = ctx->stream;
+    const long long nn = t->n_node;
+    size_t tb = 0;
+    hipError_t e;
+    DevBuf &flag = t->flag, &gsel = t->gsel, &nsel = t->nsel, &gkey = t->gkey, &gkey2 = t->gkey2;
+    if ((e = flag.ensure(nn)) || (e = gsel.ensure(sizeof(int) * nn)) || (e = nsel.ensure(sizeof(int))))
+        return ctx->hip_fail(e, "cell selection allocation");
+    hipLaunchKernelGGL(k_group_flags, dim3(blocks(nn)), dim3(256), 0, st, t->nodes.as<GNode>(), nn, limit,
+                       flag.as<char>());
+    hipLaunchKernelGGL(k_group_children, dim3(blocks(nn)), dim3(256), 0, st, t->nodes.as<GNode>(), nn, limit,
+                       flag.as<char>());
+    hipcub::CountingInputIterator<int> it(0);
+    if ((e = hipcub::DeviceSelect::Flagged(nullptr, tb, it, flag.as<char>(), gsel.as<int>(), nsel.as<int>(), (int)nn,
+                                           st)) ||
+        (e = cub_temp(t, tb)) ||
+        (e = hipcub::DeviceSelect::Flagged(t->tmp.p, tb, it, flag.as<char>(), gsel.as<int>(), nsel.as<int>(),
+                                           (int)nn, st)))
+        return ctx->hip_fail(e, "cell selection");
+    int ngi = 0;
+    if ((e = hipMemcpyAsync(&ngi, nsel.p, sizeof(int), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
+        return ctx->hip_fail(e, "cell count");
+    const long long ng = ngi;
+    if ((e = gkey.ensure(8 * ng)) || (e = gkey2.ensure(8 * ng)) || (e = out.ensure(sizeof(int) * std::max(ng, 1LL))))
+        return ctx->hip_fail(e, "cell selection allocation");
+    hipLaunchKernelGGL(k_group_keys, dim3(blocks(ng)), dim3(256), 0, st, t->nodes.as<GNode>(), gsel.as<int>(), ng,
+                       gkey.as<long long>());
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, gkey.as<long long>(), gkey2.as<long long>(),
+                                                gsel.as<int>(), out.as<int>(), (int)ng, 0, 32, st)) ||
+        (e = cub_temp(t, tb)) ||
+        (e = hipcub::DeviceRadixSort::SortPairs(t->tmp.p, tb, gkey.as<long long>(), gkey2.as<long long>(),
+                                                gsel.as<int>(), out.as<int>(), (int)ng, 0, 32, st)))
+        return ctx->hip_fail(e, "cell sort");
+    n_out = ng;
+    return PTH_OK;
+}
+
 // Builds (or rebuilds, reusing t's device memory) the tree of n particles.
 // pos / mass / rs are host arrays, or device arrays when on_device.
 int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, const double *r_search,
@@ -716,24 +811,28 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     // 1. bounding cube (per-block partials reduced on the host)
     const int nb = (int)std::min<long long>(1024, (n + 255) / 256);
     DevBuf &part = t->part;
-    std::vector<double> hp(6 * (size_t)nb);
-    if ((e = part.ensure(sizeof(double) * 6 * nb))) return bail(ctx->hip_fail(e, "bbox"));
-    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, dpos, n, part.as<double>());
-    if ((e = hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, st)) ||
+    std::vector<double> hp(7 * (size_t)nb);
+    if ((e = part.ensure(sizeof(double) * 7 * nb))) return bail(ctx->hip_fail(e, "bbox"));
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, dpos, drs, n, part.as<double>());
+    if ((e = hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 7 * nb, hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
         return bail(ctx->hip_fail(e, "bbox"));
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int b = 0; b < nb; b++)
+    t->rs_max = 0.0;
+    for (int b = 0; b < nb; b++) {
         for (int d = 0; d < 3; d++) {
-            lo[d] = std::min(lo[d], hp[6 * b + d]);
-            hi[d] = std::max(hi[d], hp[6 * b + 3 + d]);
+            lo[d] = std::min(lo[d], hp[7 * b + d]);
+            hi[d] = std::max(hi[d], hp[7 * b + 3 + d]);
         }
+        t->rs_max = std::max(t->rs_max, hp[7 * b + 6]);
+    }
     double len = 0;
     for (int d = 0; d < 3; d++) len = std::max(len, hi[d] - lo[d]);
     len = (len > 0) ? len * (1.0 + 1e-9) : 1.0;
     double root[3];
     for (int d = 0; d < 3; d++) root[d] = 0.5 * (lo[d] + hi[d]) - 0.5 * len;
     const double scale = (double)(1u << GT_MAX_LEVEL) / len;
+    t->root_len = len;
     // 2. keys, stable sort of (key, index), sorted particles
     if ((e = t->key.ensure(8 * (size_t)n)) || (e = t->skey.ensure(8 * (size_t)n)) ||
         (e = t->sidx.ensure(8 * (size_t)n)) || (e = t->order.ensure(8 * (size_t)n)))
@@ -819,37 +918,13 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
                            t->box_out.as<double>());
     }
     // 5. groups in Morton order
-    DevBuf &flag = t->flag, &gsel = t->gsel, &nsel = t->nsel, &gkey = t->gkey, &gkey2 = t->gkey2;
-    if ((e = flag.ensure(nn)) || (e = gsel.ensure(sizeof(int) * nn)) || (e = nsel.ensure(sizeof(int))))
-        return bail(ctx->hip_fail(e, "group allocation"));
-    hipLaunchKernelGGL(k_group_flags, dim3(blocks(nn)), dim3(256), 0, st, t->nodes.as<GNode>(), nn, n_group_limit,
-                       flag.as<char>());
-    hipLaunchKernelGGL(k_group_children, dim3(blocks(nn)), dim3(256), 0, st, t->nodes.as<GNode>(), nn, n_group_limit,
-                       flag.as<char>());
     {
-        hipcub::CountingInputIterator<int> it(0);
-        if ((e = hipcub::DeviceSelect::Flagged(nullptr, tb, it, flag.as<char>(), gsel.as<int>(), nsel.as<int>(),
-                                               (int)nn, st)) ||
-            (e = cub_temp(t, tb)) ||
-            (e = hipcub::DeviceSelect::Flagged(t->tmp.p, tb, it, flag.as<char>(), gsel.as<int>(), nsel.as<int>(),
-                                               (int)nn, st)))
-            return bail(ctx->hip_fail(e, "group selection"));
+        long long ng_sel = 0;
+        const int rc = select_cells(t, n_group_limit, t->groups, ng_sel);
+        if (rc != PTH_OK) return rc;
+        t->n_group = ng_sel;
     }
-    int ngi = 0;
-    if ((e = hipMemcpyAsync(&ngi, nsel.p, sizeof(int), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
-        return bail(ctx->hip_fail(e, "group count"));
-    const long long ng = ngi;
-    t->n_group = ng;
-    if ((e = gkey.ensure(8 * ng)) || (e = gkey2.ensure(8 * ng)) || (e = t->groups.ensure(sizeof(int) * ng)))
-        return bail(ctx->hip_fail(e, "group allocation"));
-    hipLaunchKernelGGL(k_group_keys, dim3(blocks(ng)), dim3(256), 0, st, t->nodes.as<GNode>(), gsel.as<int>(), ng,
-                       gkey.as<long long>());
-    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, gkey.as<long long>(), gkey2.as<long long>(),
-                                                gsel.as<int>(), t->groups.as<int>(), (int)ng, 0, 32, st)) ||
-        (e = cub_temp(t, tb)) ||
-        (e = hipcub::DeviceRadixSort::SortPairs(t->tmp.p, tb, gkey.as<long long>(), gkey2.as<long long>(),
-                                                gsel.as<int>(), t->groups.as<int>(), (int)ng, 0, 32, st)))
-        return bail(ctx->hip_fail(e, "group sort"));
+    const long long ng = t->n_group;
     if ((e = hipStreamSynchronize(st))) return bail(ctx->hip_fail(e, "tree build"));
     const auto t1 = std::chrono::steady_clock::now();
     t->ms_build = std::chrono::duration<float, std::milli>(t1 - t0).count();
@@ -1137,24 +1212,84 @@ int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     hipStream_t st = c->stream;
     const auto t0 = std::chrono::steady_clock::now();
-    const long long n = t->n, ng = t->n_group;
+    const long long n = t->n;
+    size_t tb = 0;
+    hipError_t e;
+    // 1. candidate lists: one overlap-only walk per tile (the largest cells
+    //    with at most 64 particles; theta opening off: inv_theta2 = 0), the
+    //    group walk's symmetric r_search tests on widened tile boxes
+    long long nt = 0;
+    {
+        const int rc = select_cells(t, 64, t->tiles, nt);
+        if (rc != PTH_OK) return rc;
+    }
+    HIPCHK(c, t->tbox.ensure(sizeof(double) * 12 * nt), "neighbour buffers");
+    HIPCHK(c, t->tcnt.ensure(sizeof(long long) * (nt + 1)), "neighbour buffers");
+    HIPCHK(c, t->toff.ensure(sizeof(long long) * (nt + 1)), "neighbour buffers");
+    HIPCHK(c, t->tinfo.ensure(sizeof(int4) * nt), "neighbour buffers");
+    hipLaunchKernelGGL(k_tile_boxes, dim3(blocks(nt)), dim3(256), 0, st, t->tiles.as<int>(), nt,
+                       t->box_in.as<double>(), t->box_out.as<double>(), t->rs_max, t->tbox.as<double>());
+    WalkArgs wa{};
+    wa.nodes = t->nodes.as<GNode>();
+    wa.ncom = t->ncom.as<double>();
+    wa.box_in = t->box_in.as<double>();
+    wa.box_out = t->box_out.as<double>();
+    wa.groups = t->tiles.as<int>();
+    wa.n_group = nt;
+    wa.tbox = t->tbox.as<double>();
+    wa.root_len = t->root_len;
+    wa.inv_theta2 = 0.0;
+    wa.theta_pos = 1;
+    wa.cnt_e = t->tcnt.as<long long>();
+    const unsigned wgt = (unsigned)std::min<long long>(nt, 65536);
+    hipLaunchKernelGGL(k_walk<false>, dim3(wgt), dim3(64), 0, st, wa);
+    const long long zero = 0;
+    HIPCHK(c, hipMemcpyAsync(t->tcnt.as<long long>() + nt, &zero, 8, hipMemcpyHostToDevice, st), "neighbour counts");
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t->tcnt.as<long long>(), t->toff.as<long long>(),
+                                              (int)(nt + 1), st)) ||
+        (e = cub_temp(t, tb)) ||
+        (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, t->tcnt.as<long long>(), t->toff.as<long long>(),
+                                              (int)(nt + 1), st)))
+        return c->hip_fail(e, "candidate scan");
+    long long ncand = 0;
+    HIPCHK(c, hipMemcpyAsync(&ncand, t->toff.as<long long>() + nt, 8, hipMemcpyDeviceToHost, st), "candidates");
+    HIPCHK(c, hipStreamSynchronize(st), "candidates");
+    if (ncand > 0x7fffffffLL) return c->fail(PTH_ERR_CAPACITY, "pth_gtree_neighbors: candidates exceed the int32 range");
+    HIPCHK(c, t->tl.ensure(sizeof(int) * std::max<long long>(ncand, 1)), "neighbour buffers");
+    HIPCHK(c, t->tl_alt.ensure(sizeof(int) * std::max<long long>(ncand, 1)), "neighbour buffers");
+    wa.off_e = t->toff.as<long long>();
+    wa.id_epj = t->tl.as<int>();
+    hipLaunchKernelGGL(k_walk<true>, dim3(wgt), dim3(64), 0, st, wa);
+    // candidates ascending in tree order, so rows come out in that order
+    if (ncand > 0) {
+        int bits = 1;
+        while (bits < 31 && (1LL << bits) < n) bits++;
+        const long long *ob = t->toff.as<long long>();
+        if ((e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, t->tl.as<int>(), t->tl_alt.as<int>(),
+                                                            (int)ncand, (int)nt, ob, ob + 1, 0, bits, st)) ||
+            (e = cub_temp(t, tb)) ||
+            (e = hipcub::DeviceSegmentedRadixSort::SortKeys(t->tmp.p, tb, t->tl.as<int>(), t->tl_alt.as<int>(),
+                                                            (int)ncand, (int)nt, ob, ob + 1, 0, bits, st)))
+            return c->hip_fail(e, "candidate ordering");
+        std::swap(t->tl, t->tl_alt);
+    }
+    hipLaunchKernelGGL(k_tile_info, dim3(blocks(nt)), dim3(256), 0, st, t->nodes.as<GNode>(), t->tiles.as<int>(), nt,
+                       t->toff.as<long long>(), t->tinfo.as<int4>());
+    // 2. exact FP64 filter of the candidates: count, scan, write
     HIPCHK(c, t->ngb_cnt.ensure(sizeof(int) * (n + 1)), "neighbour buffers");
     HIPCHK(c, t->ngb_off.ensure(sizeof(long long) * (n + 1)), "neighbour buffers");
     NgbArgs na{};
     na.spos = t->spos.as<double>();
     na.srs = t->srs.as<double>();
     na.order = t->order.as<long long>();
-    na.info = t->info.as<int4>();
-    na.off_e = t->off_e.as<long long>();
-    na.id_epj = t->id_epj.as<int>();
-    na.n_group = ng;
+    na.info = t->tinfo.as<int4>();
+    na.off_e = t->toff.as<long long>();
+    na.id_epj = t->tl.as<int>();
+    na.n_group = nt;
     na.cnt = t->ngb_cnt.as<int>();
-    const unsigned wg = (unsigned)std::max<long long>(1, std::min<long long>(ng, 65536));
     // the count of entry n stays 0 so the scan's last entry is the total
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, t->ngb_cnt.as<int>() + n, 1, 0);
-    if (ng > 0) hipLaunchKernelGGL(k_ngb<false>, dim3(wg), dim3(64), 0, st, na);
-    size_t tb = 0;
-    hipError_t e;
+    hipLaunchKernelGGL(k_ngb<false>, dim3(wgt), dim3(64), 0, st, na);
     if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t->ngb_cnt.as<int>(), t->ngb_off.as<long long>(),
                                               (int)(n + 1), st)) ||
         (e = cub_temp(t, tb)) ||
@@ -1168,10 +1303,11 @@ int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
     HIPCHK(c, t->ngb.ensure(sizeof(int) * std::max<long long>(total, 1)), "neighbour buffers");
     na.off = t->ngb_off.as<long long>();
     na.ngb = t->ngb.as<int>();
-    if (ng > 0 && total > 0) hipLaunchKernelGGL(k_ngb<true>, dim3(wg), dim3(64), 0, st, na);
+    if (total > 0) hipLaunchKernelGGL(k_ngb<true>, dim3(wgt), dim3(64), 0, st, na);
     HIPCHK(c, hipGetLastError(), "neighbour search");
     HIPCHK(c, hipStreamSynchronize(st), "neighbour search");
     t->n_ngb = total;
+    t->n_cand = ncand;
     t->ms_ngb = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (n_total) *n_total = total;
     return PTH_OK;
@@ -1202,7 +1338,8 @@ void pth_gtree_free(pth_gtree *t)
     }
     DevBuf *all[] = {&t->order, &t->key, &t->skey, &t->sidx, &t->spos, &t->smass, &t->srs, &t->nodes, &t->nmass,
                      &t->ncom, &t->nquad, &t->box_in, &t->box_out, &t->groups, &t->off_e, &t->off_s, &t->id_epj,
-                     &t->id_spj, &t->tmp, &t->ngb_cnt, &t->ngb_off, &t->ngb};
+                     &t->id_spj, &t->tmp, &t->ngb_cnt, &t->ngb_off, &t->ngb, &t->tiles, &t->tbox, &t->tcnt,
+                     &t->toff, &t->tl, &t->tl_alt, &t->tinfo};
     for (DevBuf *b : all) b->release();
     t->release_scratch();
     delete t;
