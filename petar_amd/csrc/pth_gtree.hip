@@ -904,6 +904,8 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         le += total;
     }
     t->n_node = le;
+    if (le > 0x7fffffffLL)  // the walks keep node indices as int32
+        return bail(ctx->fail(PTH_ERR_CAPACITY, "pth_gtree: node count exceeds the int32 range"));
     const long long nn = le;
     // 4. moments bottom-up
     if ((e = t->nmass.ensure(8 * nn)) || (e = t->ncom.ensure(24 * nn)) || (e = t->nquad.ensure(48 * nn)) ||

@@ -17,7 +17,9 @@
 #include "pth_plan.h"
 
 #include <algorithm>
+#include <array>
 #include <climits>
+#include <thread>
 #include <cstdint>
 #include <cstring>
 
@@ -39,16 +41,42 @@ struct Tmp {
 // Working memory of make_plan, kept per thread between calls: a plan of a
 // large evaluation touches tens of MB, and fresh pages cost more than the
 // planning itself.
+// The items of one segment and ipl class and their 8 runs (one per XCD).
+struct ClassRuns {
+    std::vector<uint32_t> sub;  // items of the class, walk order
+    size_t run_begin[XCDS + 1];
+    size_t max_blocks = 0;
+};
+
 struct Scratch {
     std::vector<Tmp> seq;
     std::vector<size_t> walk_item_begin;
     std::vector<int> walk_ritem_begin;
+    std::vector<long long> walk_part_begin;
     std::vector<double> wcost;
     std::vector<uint16_t> bucket;
-    std::vector<uint32_t> sub, cnt, sorted;
-    std::vector<long long> eb, sb;
-    std::vector<Tile> tiles;
+    std::vector<std::array<ClassRuns, 3>> cr;
 };
+
+// Runs f(begin, end) over [0, n) on up to 16 host threads, at least `grain`
+// indices each (fewer, shorter loops stay on the calling thread).
+template <class F> void parallel_for(int n, int grain, F f)
+{
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int t = std::min({hw, 16, std::max(1, n / std::max(grain, 1))});
+    if (t <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(t - 1);
+    for (int k = 1; k < t; k++) th.emplace_back(f, (int)((long long)n * k / t), (int)((long long)n * (k + 1) / t));
+    f(0, (int)((long long)n / t));
+    for (std::thread &x : th) x.join();
+}
+
+constexpr int PLAN_WALKS_PER_THREAD = 512;
+constexpr int PLAN_ITEMS_PER_THREAD = 32768;
 
 // Least-cost cover of n i-particles by tiles of 256 (ipl 4), 128 (ipl 2)
 // and 64 (ipl 1) lanes x i, for a walk with n_e EPJ and n_s SPJ.
@@ -129,24 +157,15 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
 
     const long long cj = (chunk_j > 0) ? chunk_j : DEFAULT_CHUNK_J;
     static thread_local Scratch S;
-    std::vector<Tmp> &seq = S.seq;
-    seq.clear();
-    std::vector<size_t> &walk_item_begin = S.walk_item_begin;
-    std::vector<int> &walk_ritem_begin = S.walk_ritem_begin;
-    walk_item_begin.assign(n_walk + 1, 0);
-    walk_ritem_begin.assign(n_walk + 1, 0);
-    for (int w = 0; w < n_walk; w++) {
-        walk_item_begin[w] = seq.size();
-        walk_ritem_begin[w] = (int)P.ritems.size();
+    // chunking of walk w: nch chunks, EPJ block / SPJ record bounds
+    auto chunks = [&](int w, std::vector<long long> &eb, std::vector<long long> &sb) -> long long {
         const WalkDev &W = P.walks[w];
-        if (W.n_i <= 0) continue;
         const long long nb = (P.es_off[w + 1] - P.es_off[w]) / 8;
         const long long units = nb * 8 + 2LL * W.n_s;
         long long nch = (units + cj - 1) / cj;
         if (nch < 1) nch = 1;
         // chunk k covers list units [k*units/nch, (k+1)*units/nch); EPJ
         // boundaries are rounded to whole blocks of 8
-        std::vector<long long> &eb = S.eb, &sb = S.sb;
         eb.resize(nch + 1);
         sb.resize(nch + 1);
         for (long long k = 0; k <= nch; k++) {
@@ -156,44 +175,87 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         }
         eb[nch] = nb;
         sb[nch] = W.n_s;
-        tile_walk(W.n_i, W.n_e, W.n_s, max_ipl, S.tiles);
-        for (const Tile &t : S.tiles) {
-            const int part_first = P.n_part;
-            for (long long k = 0; k < nch; k++) {
-                Item it;
-                it.walk = w;
-                it.i_begin = W.i_off + t.begin;
-                it.n_i_ipl = item_pack(t.n, t.ipl);
-                it.e_blk_begin = (int)(P.es_off[w] / 8 + eb[k]);
-                it.e_blk_end = (int)(P.es_off[w] / 8 + eb[k + 1]);
-                it.s_begin = (int)(P.ss_off[w] + sb[k]);
-                it.s_end = (int)(P.ss_off[w] + sb[k + 1]);
-                it.part = -1;
-                if (nch > 1) {
-                    it.part = P.n_part;
-                    P.n_part += t.ipl;
+        return nch;
+    };
+    // 1. per walk: items (tiles x chunks), reduction items and partial slots
+    std::vector<size_t> &walk_item_begin = S.walk_item_begin;
+    std::vector<int> &walk_ritem_begin = S.walk_ritem_begin;
+    std::vector<long long> &walk_part_begin = S.walk_part_begin;
+    walk_item_begin.assign(n_walk + 1, 0);
+    walk_ritem_begin.assign(n_walk + 1, 0);
+    walk_part_begin.assign(n_walk + 1, 0);
+    parallel_for(n_walk, PLAN_WALKS_PER_THREAD, [&](int wb, int we) {
+        std::vector<long long> eb, sb;
+        std::vector<Tile> tiles;
+        for (int w = wb; w < we; w++) {
+            const WalkDev &W = P.walks[w];
+            if (W.n_i <= 0) continue;
+            const long long nch = chunks(w, eb, sb);
+            tile_walk(W.n_i, W.n_e, W.n_s, max_ipl, tiles);
+            long long ipl_sum = 0;
+            for (const Tile &t : tiles) ipl_sum += t.ipl;
+            walk_item_begin[w + 1] = tiles.size() * (size_t)nch;
+            walk_ritem_begin[w + 1] = nch > 1 ? (int)tiles.size() : 0;
+            walk_part_begin[w + 1] = nch > 1 ? nch * ipl_sum : 0;
+        }
+    });
+    for (int w = 0; w < n_walk; w++) {
+        walk_item_begin[w + 1] += walk_item_begin[w];
+        walk_ritem_begin[w + 1] += walk_ritem_begin[w];
+        walk_part_begin[w + 1] += walk_part_begin[w];
+    }
+    if (walk_part_begin[n_walk] > INT_MAX) return false;
+    std::vector<Tmp> &seq = S.seq;
+    seq.resize(walk_item_begin[n_walk]);
+    P.ritems.resize(walk_ritem_begin[n_walk]);
+    P.n_part = (int)walk_part_begin[n_walk];
+    parallel_for(n_walk, PLAN_WALKS_PER_THREAD, [&](int wb, int we) {
+        std::vector<long long> eb, sb;
+        std::vector<Tile> tiles;
+        for (int w = wb; w < we; w++) {
+            const WalkDev &W = P.walks[w];
+            if (W.n_i <= 0) continue;
+            const long long nch = chunks(w, eb, sb);
+            tile_walk(W.n_i, W.n_e, W.n_s, max_ipl, tiles);
+            size_t o = walk_item_begin[w];
+            int r_o = walk_ritem_begin[w];
+            int part = (int)walk_part_begin[w];
+            for (const Tile &t : tiles) {
+                const int part_first = part;
+                for (long long k = 0; k < nch; k++) {
+                    Item it;
+                    it.walk = w;
+                    it.i_begin = W.i_off + t.begin;
+                    it.n_i_ipl = item_pack(t.n, t.ipl);
+                    it.e_blk_begin = (int)(P.es_off[w] / 8 + eb[k]);
+                    it.e_blk_end = (int)(P.es_off[w] / 8 + eb[k + 1]);
+                    it.s_begin = (int)(P.ss_off[w] + sb[k]);
+                    it.s_end = (int)(P.ss_off[w] + sb[k + 1]);
+                    it.part = -1;
+                    if (nch > 1) {
+                        it.part = part;
+                        part += t.ipl;
+                    }
+                    const double c = 64.0 * t.ipl *
+                                     (epj_unit_cost(t.ipl) * 8.0 * (it.e_blk_end - it.e_blk_begin) +
+                                      spj_unit_cost(t.ipl) * (it.s_end - it.s_begin) + 50.0);
+                    seq[o++] = Tmp{it, c};
                 }
-                const double c = 64.0 * t.ipl *
-                                 (epj_unit_cost(t.ipl) * 8.0 * (it.e_blk_end - it.e_blk_begin) +
-                                  spj_unit_cost(t.ipl) * (it.s_end - it.s_begin) + 50.0);
-                seq.push_back({it, c});
-            }
-            if (nch > 1) {
-                RItem r;
-                r.out = W.i_off + t.begin;
-                r.n_i = t.n;
-                r.part_first = part_first;
-                r.n_part = (int)nch;
-                r.ipl = t.ipl;
-                r.pad[0] = r.pad[1] = r.pad[2] = 0;
-                P.ritems.push_back(r);
+                if (nch > 1) {
+                    RItem r;
+                    r.out = W.i_off + t.begin;
+                    r.n_i = t.n;
+                    r.part_first = part_first;
+                    r.n_part = (int)nch;
+                    r.ipl = t.ipl;
+                    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+                    P.ritems[r_o++] = r;
+                }
             }
         }
-    }
-    walk_item_begin[n_walk] = seq.size();
-    walk_ritem_begin[n_walk] = (int)P.ritems.size();
+    });
     P.n_real_items = (int)seq.size();
-    // segments: contiguous walk ranges of about equal item cost
+    // 2. segments: contiguous walk ranges of about equal item cost
     if (n_seg < 1) n_seg = 1;
     if (n_seg > MAX_SEGMENTS) n_seg = MAX_SEGMENTS;
     if (n_seg > n_walk) n_seg = std::max(n_walk, 1);
@@ -211,83 +273,99 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         cut[g] = (int)(std::lower_bound(wcost.begin(), wcost.end(), target) - wcost.begin());
         cut[g] = std::min(std::max(cut[g], cut[g - 1]), n_walk);
     }
-    // one launch per ipl class (each kernel has its own register budget);
-    // inside a class, 8 contiguous runs of equal cost, one per XCD
+    // 3. cost buckets for the most-expensive-first order inside a run: a
+    //    counting sort on the cost's binary exponent and top 4 mantissa bits
+    //    (16 steps per octave), stable inside a bucket
+    constexpr int NBUCKET = 1024;
+    std::vector<uint16_t> &bucket = S.bucket;
+    bucket.resize(seq.size());
+    parallel_for((int)std::min<size_t>(seq.size(), INT_MAX), PLAN_ITEMS_PER_THREAD, [&](int kb, int ke) {
+        for (int k = kb; k < ke; k++) {
+            const double c = std::max(seq[k].cost, 1.0);  // >= 1: biased exponent >= 1023
+            uint64_t bits;
+            std::memcpy(&bits, &c, 8);
+            const int q = (int)((bits >> 48) - (1023ull << 4));  // exponent * 16 + top mantissa bits
+            bucket[k] = (uint16_t)std::min(NBUCKET - 1, q);
+        }
+    });
+    // 4. per segment and ipl class (one launch each; each kernel has its
+    //    own register budget): 8 contiguous runs of equal cost, one per XCD.
+    //    First the runs and sizes of every segment, then the items, both in
+    //    parallel over segments.
+    static const int class_ipl[3] = {1, 2, 4};
+    std::vector<std::array<ClassRuns, 3>> &cr = S.cr;
+    cr.resize(n_seg);
+    parallel_for(n_seg, 1, [&](int gb, int ge) {
+        for (int g = gb; g < ge; g++)
+            for (int cls = 0; cls < 3; cls++) {
+                ClassRuns &R = cr[g][cls];
+                R.sub.clear();
+                double total = 0;
+                for (size_t k = walk_item_begin[cut[g]]; k < walk_item_begin[cut[g + 1]]; k++)
+                    if (item_ipl(seq[k].it.n_i_ipl) == class_ipl[cls]) {
+                        R.sub.push_back((uint32_t)k);
+                        total += seq[k].cost;
+                    }
+                // an item goes to run x while the running cost is below
+                // (x+1)/8 of the total
+                double acc = 0;
+                int x = 0;
+                R.run_begin[0] = 0;
+                for (size_t k = 0; k < R.sub.size(); k++) {
+                    while (x < XCDS - 1 && acc >= total * (x + 1) / XCDS) R.run_begin[++x] = k;
+                    acc += seq[R.sub[k]].cost;
+                }
+                while (x < XCDS - 1) R.run_begin[++x] = R.sub.size();
+                R.run_begin[XCDS] = R.sub.size();
+                R.max_blocks = 0;
+                for (int xx = 0; xx < XCDS; xx++)
+                    R.max_blocks = std::max(R.max_blocks, (R.run_begin[xx + 1] - R.run_begin[xx] + ITEMS_PER_BLOCK - 1) /
+                                                              ITEMS_PER_BLOCK);
+            }
+    });
+    size_t cursor = 0;
+    P.segs.resize(n_seg);
+    for (int g = 0; g < n_seg; g++) {
+        PlanSeg &sg = P.segs[g];
+        sg.walk_begin = cut[g];
+        sg.walk_end = cut[g + 1];
+        sg.ritem_begin = walk_ritem_begin[cut[g]];
+        sg.ritem_count = walk_ritem_begin[cut[g + 1]] - sg.ritem_begin;
+        for (int cls = 0; cls < 3; cls++) {
+            sg.seg_begin[cls] = (int)cursor;
+            const size_t sz = cr[g][cls].sub.empty() ? 0 : cr[g][cls].max_blocks * XCDS * ITEMS_PER_BLOCK;
+            sg.seg_count[cls] = (int)sz;
+            cursor += sz;
+        }
+    }
+    if (cursor > (size_t)INT_MAX) return false;
     Item dummy;
     dummy.walk = -1;
     dummy.i_begin = dummy.n_i_ipl = dummy.e_blk_begin = dummy.e_blk_end = dummy.s_begin = dummy.s_end = 0;
     dummy.part = -1;
-    static const int class_ipl[3] = {1, 2, 4};
-    // the items of walks [wb, we), per ipl class, in launch order
-    // cost buckets for the most-expensive-first order inside a run: a
-    // counting sort on the cost's binary exponent and top 4 mantissa bits
-    // (16 steps per octave), stable inside a bucket
-    constexpr int NBUCKET = 1024;
-    std::vector<uint16_t> &bucket = S.bucket;
-    bucket.resize(seq.size());
-    for (size_t k = 0; k < seq.size(); k++) {
-        const double c = std::max(seq[k].cost, 1.0);  // >= 1: biased exponent >= 1023
-        uint64_t bits;
-        std::memcpy(&bits, &c, 8);
-        const int q = (int)((bits >> 48) - (1023ull << 4));  // exponent * 16 + top mantissa bits
-        bucket[k] = (uint16_t)std::min(NBUCKET - 1, q);
-    }
-    std::vector<uint32_t> &sub = S.sub, &cnt = S.cnt, &sorted = S.sorted;
-    cnt.resize(NBUCKET + 1);
-    auto arrange = [&](int wb, int we) {
-        PlanSeg sg;
-        sg.walk_begin = wb;
-        sg.walk_end = we;
-        sg.ritem_begin = walk_ritem_begin[wb];
-        sg.ritem_count = walk_ritem_begin[we] - sg.ritem_begin;
-        for (int cls = 0; cls < 3; cls++) {
-            sub.clear();
-            double total = 0;
-            for (size_t k = walk_item_begin[wb]; k < walk_item_begin[we]; k++)
-                if (item_ipl(seq[k].it.n_i_ipl) == class_ipl[cls]) {
-                    sub.push_back((uint32_t)k);
-                    total += seq[k].cost;
-                }
-            sg.seg_begin[cls] = (int)P.items.size();
-            if (sub.empty()) continue;
-            // runs: an item goes to run x while the running cost is below
-            // (x+1)/8 of the total
-            size_t run_begin[XCDS + 1];
-            {
-                double acc = 0;
-                int x = 0;
-                run_begin[0] = 0;
-                for (size_t k = 0; k < sub.size(); k++) {
-                    while (x < XCDS - 1 && acc >= total * (x + 1) / XCDS) run_begin[++x] = k;
-                    acc += seq[sub[k]].cost;
-                }
-                while (x < XCDS - 1) run_begin[++x] = sub.size();
-                run_begin[XCDS] = sub.size();
-            }
-            size_t max_blocks = 0;
-            for (int xx = 0; xx < XCDS; xx++)
-                max_blocks = std::max(max_blocks, (run_begin[xx + 1] - run_begin[xx] + ITEMS_PER_BLOCK - 1) /
-                                                      ITEMS_PER_BLOCK);
-            const size_t base = P.items.size();
-            P.items.resize(base + max_blocks * XCDS * ITEMS_PER_BLOCK, dummy);
-            for (int xx = 0; xx < XCDS; xx++) {
-                const size_t a = run_begin[xx], b = run_begin[xx + 1];
-                // descending bucket order, stable
-                std::fill(cnt.begin(), cnt.end(), 0u);
-                for (size_t k = a; k < b; k++) cnt[NBUCKET - 1 - bucket[sub[k]] + 1]++;
-                for (int q = 0; q < NBUCKET; q++) cnt[q + 1] += cnt[q];
-                sorted.resize(b - a);
-                for (size_t k = a; k < b; k++) sorted[cnt[NBUCKET - 1 - bucket[sub[k]]]++] = sub[k];
-                for (size_t k = 0; k < sorted.size(); k++) {
-                    const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
-                    P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = seq[sorted[k]].it;
+    P.items.assign(cursor, dummy);
+    parallel_for(n_seg, 1, [&](int gb, int ge) {
+        std::vector<uint32_t> cnt(NBUCKET + 1), sorted;
+        for (int g = gb; g < ge; g++)
+            for (int cls = 0; cls < 3; cls++) {
+                const ClassRuns &R = cr[g][cls];
+                if (R.sub.empty()) continue;
+                const size_t base = P.segs[g].seg_begin[cls];
+                for (int xx = 0; xx < XCDS; xx++) {
+                    const size_t a = R.run_begin[xx], b = R.run_begin[xx + 1];
+                    // descending bucket order, stable
+                    std::fill(cnt.begin(), cnt.end(), 0u);
+                    for (size_t k = a; k < b; k++) cnt[NBUCKET - 1 - bucket[R.sub[k]] + 1]++;
+                    for (int q = 0; q < NBUCKET; q++) cnt[q + 1] += cnt[q];
+                    sorted.resize(b - a);
+                    for (size_t k = a; k < b; k++) sorted[cnt[NBUCKET - 1 - bucket[R.sub[k]]]++] = R.sub[k];
+                    for (size_t k = 0; k < sorted.size(); k++) {
+                        const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
+                        P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = seq[sorted[k]].it;
+                    }
                 }
             }
-            sg.seg_count[cls] = (int)(P.items.size() - base);
-        }
-        return sg;
-    };
-    for (int g = 0; g < n_seg; g++) P.segs.push_back(arrange(cut[g], cut[g + 1]));
+    });
     for (int cls = 0; cls < 3; cls++) {  // whole-plan view (one segment)
         P.seg_begin[cls] = P.segs[0].seg_begin[cls];
         P.seg_count[cls] = P.segs[0].seg_count[cls];
