@@ -68,6 +68,17 @@ def test_null_context_calls():
     assert L.pth_resident_run(None) == sf.PTH_ERR_ARG
     assert L.pth_device(None) == -1
     assert L.pth_last_error(None) == b"null context"
+    # the device tree's entry points check their handles before any HIP call
+    assert L.pth_gtree_build(None, None, 10, None, None, None, 20, 512, 0.3, 1) == sf.PTH_ERR_ARG
+    assert L.pth_gtree_update(None, 10, None, None, None, 0) == sf.PTH_ERR_ARG
+    assert L.pth_gtree_sizes(None, None, None, None, None) == sf.PTH_ERR_ARG
+    assert L.pth_gtree_download(None, *([None] * 13)) == sf.PTH_ERR_ARG
+    assert L.pth_gtree_timing(None, None, None, None) == sf.PTH_ERR_ARG
+    assert L.pth_gtree_neighbors(None, None) == sf.PTH_ERR_ARG
+    assert L.pth_gtree_neighbors_download(None, None, None, None, None) == sf.PTH_ERR_ARG
+    assert L.pth_resident_from_gtree(None, None, None) == sf.PTH_ERR_ARG
+    assert L.pth_resident_refill(None, None) == sf.PTH_ERR_ARG
+    L.pth_gtree_free(None)
 
 
 def _coverage(ne, nj, ns, mode, cj, max_ipl=4):

@@ -328,3 +328,23 @@ def test_snapshot_feeds_the_device_step(tmp_path):
         assert all(np.array_equal(a, b) for a, b in zip(got, ref))
     srv.close()
     ctx.close()
+
+
+def test_device_tree_rejects_bad_arguments():
+    """Argument errors come back as error codes with a message, before any
+    launch, and leave the context usable."""
+    ctx = sf.Context(0)
+    m, x, _ = host.cluster(500, "plummer", seed=2)
+    with pytest.raises(sf.PthError, match="n outside"):
+        sf.GpuTree(ctx, np.zeros((0, 3)), np.zeros(0))
+    with pytest.raises(sf.PthError, match="limits must be"):
+        sf.GpuTree(ctx, x, m, n_leaf_limit=0)
+    with pytest.raises(sf.PthError, match="limits must be"):
+        sf.GpuTree(ctx, x, m, n_group_limit=0)
+    g = sf.GpuTree(ctx, x, m)
+    lib = sf.lib()
+    assert lib.pth_gtree_update(g._h, 500, None, None, None, 0) == sf.PTH_ERR_ARG
+    assert lib.pth_gtree_update(g._h, -1, x.ctypes.data, m.ctypes.data, None, 0) == sf.PTH_ERR_ARG
+    assert g.sizes()["n_group"] >= 1  # an argument error does not discard the tree
+    g.close()
+    ctx.close()
