@@ -282,7 +282,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--exec", default="fused", choices=["fused", "plain"])
     ap.add_argument("--chunk-j", type=int, default=0)
-    ap.add_argument("--max-ipl", type=int, default=2, choices=[1, 2, 4])
+    ap.add_argument("--max-ipl", type=int, default=2, choices=[1, 2, 3, 4])
     ap.add_argument("--j-vgpr", type=int, default=0, choices=[0, 1])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -316,7 +316,7 @@ def main():
     tuned = None
     if a.exec == "fused" and not a.no_autotune:
         t0 = time.time()
-        ipl, cj, tms = res.autotune(ipl=(1, 2, 4), chunk=(1024, 2048, 4096), reps=3)
+        ipl, cj, tms = res.autotune(ipl=(1, 2, 3, 4), chunk=(1024, 2048, 4096), reps=3)
         tuned = dict(max_ipl=ipl, chunk_j=cj, ms=round(tms, 4), tuning_s=round(time.time() - t0, 2))
         log(f"[bench] autotuned: {tuned}")
     st = res.stats()

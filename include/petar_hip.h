@@ -78,7 +78,8 @@ typedef struct pth_params {
     int chunk_j;   /* fused form: max j-entries per work item (0 = default) */
     int max_walk;  /* largest n_walk accepted by pth_dispatch (0 = 1000, the
                       reference's N_WALK_LIMIT, src/force_gpu_cuda.cu:14) */
-    int max_ipl;   /* fused form: most i-particles per lane, 1, 2 or 4 (other values: 4) */
+    int max_ipl;   /* fused form: most i-particles per lane, 1 to 4 (tiles of 64, 128,
+                      192, 256 i; other values: 4) */
     int j_vgpr;    /* fused form: 1 copies each j record into VGPRs before use,
                       0 uses it as SGPR operands */
     int changeover; /* 0: linear cutoff only (the reference GPU kernels);
@@ -333,9 +334,9 @@ int pth_resident_refill(pth_resident *res, const pth_gtree *tree);
  * code).  items (8 ints each: walk, i_begin, n_i | ipl << 16, e_blk_begin,
  * e_blk_end, s_begin, s_end, part; walk < 0 marks an empty slot), the staged
  * offsets es_off / ss_off (n_walk + 1 each), the reduction items (8 ints
- * each: out, n_i, part_first, n_part, ipl, 0, 0, 0) and the segments (10
- * ints for each of 32: walk_begin, walk_end, item begin of ipl 1/2/4, item
- * count of ipl 1/2/4, ritem_begin, ritem_count; -1 past the last) are
+ * each: out, n_i, part_first, n_part, ipl, 0, 0, 0) and the segments (12
+ * ints for each of 32: walk_begin, walk_end, item begin of ipl 1/2/3/4, item
+ * count of ipl 1/2/3/4, ritem_begin, ritem_count; -1 past the last) are
  * written when the pointers are not NULL. */
 long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
                          int exec_mode, int chunk_j, int max_ipl, int n_seg, int *items,

@@ -25,7 +25,7 @@ void pth_params_default(pth_params *p)
     p->exec_mode = PTH_EXEC_FUSED;
     p->chunk_j = 0;
     p->max_walk = 1000;
-    p->max_ipl = 2;  // measured best on the precision workload (1, 2, 4: 11.2, 10.6, 11.8 ms)
+    p->max_ipl = 2;  // measured best on the precision workload (steps of 2, 3, 4: 11.05, 11.58, 12.12 ms)
     p->j_vgpr = 0;   // SGPR operands beat VGPR copies at every max_ipl
     p->changeover = 0;
     p->ngb_factor = 0.99;
@@ -783,18 +783,18 @@ long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const i
     if (!make_plan(n_walk, n_epi, n_epj, n_spj, exec_mode, chunk_j, max_ipl, P, n_seg)) return PTH_ERR_CAPACITY;
     if (segs) {
         for (int k = 0; k < PTH_MAX_SEGMENTS; k++) {
-            int *o = segs + 10 * k;
-            for (int q = 0; q < 10; q++) o[q] = -1;
+            int *o = segs + 12 * k;
+            for (int q = 0; q < 12; q++) o[q] = -1;
             if (k >= (int)P.segs.size()) continue;
             const PlanSeg &g = P.segs[k];
             o[0] = g.walk_begin;
             o[1] = g.walk_end;
-            for (int c = 0; c < 3; c++) {
+            for (int c = 0; c < PLAN_CLASSES; c++) {
                 o[2 + c] = g.seg_begin[c];
-                o[5 + c] = g.seg_count[c];
+                o[6 + c] = g.seg_count[c];
             }
-            o[8] = g.ritem_begin;
-            o[9] = g.ritem_count;
+            o[10] = g.ritem_begin;
+            o[11] = g.ritem_count;
         }
     }
     if (es_off) std::copy(P.es_off.begin(), P.es_off.end(), es_off);

@@ -10,6 +10,10 @@
 
 namespace pth {
 
+// Work items come in classes of 1, 2, 3 or 4 i-particles per lane, one
+// launch per class (each kernel instance has its own register budget).
+constexpr int PLAN_CLASSES = 4;
+
 // Resident EPJ record (32 B): FP64 position, mass, r_search.
 struct __align__(32) EpjDev {
     double x, y, z;
@@ -75,7 +79,7 @@ struct RItem {
 // reduction items (the launches of one segment run in order on one stream).
 struct SegDev {
     int walk_begin = 0, n_walk = 0;
-    int seg_begin[3] = {0, 0, 0}, seg_count[3] = {0, 0, 0};
+    int seg_begin[4] = {0, 0, 0, 0}, seg_count[4] = {0, 0, 0, 0};  // items of ipl 1, 2, 3, 4
     int ritem_begin = 0, n_ritems = 0;
 };
 #define PTH_MAX_SEGMENTS 32

@@ -160,7 +160,7 @@ inline void set_segments(BatchDev &b, const Plan &P)
         const PlanSeg &g = P.segs[k];
         d.walk_begin = g.walk_begin;
         d.n_walk = g.walk_end - g.walk_begin;
-        for (int c = 0; c < 3; c++) {
+        for (int c = 0; c < PLAN_CLASSES; c++) {
             d.seg_begin[c] = g.seg_begin[c];
             d.seg_count[c] = g.seg_count[c];
         }

@@ -871,6 +871,7 @@ static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t
 {
     if (ipl == 1) hipLaunchKernelGGL((k_force<NNB, 1, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
     else if (ipl == 2) hipLaunchKernelGGL((k_force<NNB, 2, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
+    else if (ipl == 3) hipLaunchKernelGGL((k_force<NNB, 3, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((k_force<NNB, 4, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
 }
 
@@ -889,7 +890,7 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
                                b.out_accp, b.out_nnb);
         return hipGetLastError();
     }
-    static const int ipls[3] = {1, 2, 4};
+    static const int ipls[PLAN_CLASSES] = {1, 2, 3, 4};
     ForceArgs a;
     a.near_blk = b.near_blk;
     a.staged_i = b.staged_i;
@@ -908,7 +909,7 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
     a.part_accp = b.part_accp;
     a.part_nnb = b.part_nnb;
     a.part_pott = b.part_pott;
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < PLAN_CLASSES; c++) {
         const int n = g.seg_count[c];
         if (n <= 0) continue;
         a.items = b.items + g.seg_begin[c];

@@ -55,7 +55,7 @@ struct Scratch {
     std::vector<long long> walk_part_begin;
     std::vector<double> wcost;
     std::vector<uint16_t> bucket;
-    std::vector<std::array<ClassRuns, 3>> cr;
+    std::vector<std::array<ClassRuns, PLAN_CLASSES>> cr;
 };
 
 // Runs f(begin, end) over [0, n) on up to 16 host threads, at least `grain`
@@ -78,30 +78,36 @@ template <class F> void parallel_for(int n, int grain, F f)
 constexpr int PLAN_WALKS_PER_THREAD = 512;
 constexpr int PLAN_ITEMS_PER_THREAD = 32768;
 
-// Least-cost cover of n i-particles by tiles of 256 (ipl 4), 128 (ipl 2)
-// and 64 (ipl 1) lanes x i, for a walk with n_e EPJ and n_s SPJ.
+// Least-cost cover of n i-particles by tiles of 256 (ipl 4), 192 (ipl 3),
+// 128 (ipl 2) and 64 (ipl 1) lanes x i, for a walk with n_e EPJ and n_s
+// SPJ; ties go to the cover with fewer 1-i tiles (the least efficient
+// class: most idle lanes, least work per scalar load).
 void tile_walk(int n, long long n_e, long long n_s, int max_ipl, std::vector<Tile> &out)
 {
     out.clear();
     if (n <= 0) return;
     auto cost = [&](int ipl) { return 64.0 * ipl * (epj_unit_cost(ipl) * n_e + spj_unit_cost(ipl) * n_s); };
+    // the largest allowed class any number of times, each smaller one (but
+    // the 1-i remainder) at most 3 times
     const int amax = (max_ipl >= 4) ? (n + 255) / 256 : 0;
-    const int bmax = (max_ipl >= 2) ? 3 : 0;
+    const int dmax = (max_ipl >= 4) ? 3 : (max_ipl == 3) ? (n + 191) / 192 : 0;
+    const int bmax = (max_ipl >= 3) ? 3 : (max_ipl == 2) ? (n + 127) / 128 : 0;
     double best = -1;
-    int ba = 0, bb = 0, bc = 0;
+    int ba = 0, bd = 0, bb = 0, bc = 0;
     for (int a = 0; a <= amax; a++)
-        for (int b = 0; b <= bmax; b++) {
-            const long long covered = 256LL * a + 128LL * b;
-            const int c = covered >= n ? 0 : (int)((n - covered + 63) / 64);
-            if (max_ipl < 2 && b > 0) continue;
-            const double t = a * cost(4) + b * cost(2) + c * cost(1);
-            if (best < 0 || t < best - 1e-9) {
-                best = t;
-                ba = a;
-                bb = b;
-                bc = c;
+        for (int d = 0; d <= dmax; d++)
+            for (int b = 0; b <= bmax; b++) {
+                const long long covered = 256LL * a + 192LL * d + 128LL * b;
+                const int c = covered >= n ? 0 : (int)((n - covered + 63) / 64);
+                const double t = a * cost(4) + d * cost(3) + b * cost(2) + c * cost(1);
+                if (best < 0 || t < best - 1e-9 || (t < best + 1e-9 && c < bc)) {
+                    best = t;
+                    ba = a;
+                    bd = d;
+                    bb = b;
+                    bc = c;
+                }
             }
-        }
     int pos = 0;
     auto emit = [&](int count, int ipl) {
         for (int k = 0; k < count && pos < n; k++) {
@@ -111,6 +117,7 @@ void tile_walk(int n, long long n_e, long long n_s, int max_ipl, std::vector<Til
         }
     };
     emit(ba, 4);
+    emit(bd, 3);
     emit(bb, 2);
     emit(bc, 1);
 }
@@ -153,7 +160,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         P.segs.push_back(g);
         return true;
     }
-    if (max_ipl != 1 && max_ipl != 2) max_ipl = 4;
+    if (max_ipl < 1 || max_ipl > 4) max_ipl = 4;
 
     const long long cj = (chunk_j > 0) ? chunk_j : DEFAULT_CHUNK_J;
     static thread_local Scratch S;
@@ -292,12 +299,12 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     //    own register budget): 8 contiguous runs of equal cost, one per XCD.
     //    First the runs and sizes of every segment, then the items, both in
     //    parallel over segments.
-    static const int class_ipl[3] = {1, 2, 4};
-    std::vector<std::array<ClassRuns, 3>> &cr = S.cr;
+    static const int class_ipl[PLAN_CLASSES] = {1, 2, 3, 4};
+    std::vector<std::array<ClassRuns, PLAN_CLASSES>> &cr = S.cr;
     cr.resize(n_seg);
     parallel_for(n_seg, 1, [&](int gb, int ge) {
         for (int g = gb; g < ge; g++)
-            for (int cls = 0; cls < 3; cls++) {
+            for (int cls = 0; cls < PLAN_CLASSES; cls++) {
                 ClassRuns &R = cr[g][cls];
                 R.sub.clear();
                 double total = 0;
@@ -331,7 +338,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         sg.walk_end = cut[g + 1];
         sg.ritem_begin = walk_ritem_begin[cut[g]];
         sg.ritem_count = walk_ritem_begin[cut[g + 1]] - sg.ritem_begin;
-        for (int cls = 0; cls < 3; cls++) {
+        for (int cls = 0; cls < PLAN_CLASSES; cls++) {
             sg.seg_begin[cls] = (int)cursor;
             const size_t sz = cr[g][cls].sub.empty() ? 0 : cr[g][cls].max_blocks * XCDS * ITEMS_PER_BLOCK;
             sg.seg_count[cls] = (int)sz;
@@ -347,7 +354,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     parallel_for(n_seg, 1, [&](int gb, int ge) {
         std::vector<uint32_t> cnt(NBUCKET + 1), sorted;
         for (int g = gb; g < ge; g++)
-            for (int cls = 0; cls < 3; cls++) {
+            for (int cls = 0; cls < PLAN_CLASSES; cls++) {
                 const ClassRuns &R = cr[g][cls];
                 if (R.sub.empty()) continue;
                 const size_t base = P.segs[g].seg_begin[cls];
@@ -366,7 +373,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
                 }
             }
     });
-    for (int cls = 0; cls < 3; cls++) {  // whole-plan view (one segment)
+    for (int cls = 0; cls < PLAN_CLASSES; cls++) {  // whole-plan view (one segment)
         P.seg_begin[cls] = P.segs[0].seg_begin[cls];
         P.seg_count[cls] = P.segs[0].seg_count[cls];
     }

@@ -25,8 +25,8 @@ inline double spj_unit_cost(int ipl) { return 18.5 + 5.5 / ipl; }
 // concurrently on different streams.
 struct PlanSeg {
     int walk_begin = 0, walk_end = 0;
-    int seg_begin[3] = {0, 0, 0};  // items of ipl 1, 2, 4
-    int seg_count[3] = {0, 0, 0};
+    int seg_begin[4] = {0, 0, 0, 0};  // items of ipl 1, 2, 3, 4
+    int seg_count[4] = {0, 0, 0, 0};
     int ritem_begin = 0, ritem_count = 0;
 };
 constexpr int MAX_SEGMENTS = PTH_MAX_SEGMENTS;
@@ -35,8 +35,8 @@ struct Plan {
     std::vector<WalkDev> walks;
     std::vector<long long> es_off, ss_off;  // n_walk + 1
     std::vector<Item> items;                // launch order, empty slots have walk < 0
-    int seg_begin[3] = {0, 0, 0};           // items of ipl 1, 2, 4: one launch each
-    int seg_count[3] = {0, 0, 0};
+    int seg_begin[4] = {0, 0, 0, 0};        // items of ipl 1, 2, 3, 4: one launch each
+    int seg_count[4] = {0, 0, 0, 0};
     std::vector<RItem> ritems;
     int n_part = 0;                         // partial slots of 64 entries
     int n_real_items = 0;
@@ -52,7 +52,7 @@ struct Plan {
         items.clear();
         ritems.clear();
         segs.clear();
-        for (int k = 0; k < 3; k++) seg_begin[k] = seg_count[k] = 0;
+        for (int k = 0; k < PLAN_CLASSES; k++) seg_begin[k] = seg_count[k] = 0;
         n_part = n_real_items = 0;
         ni_tot = n_e_slots = n_s_rec = n_epep = n_epsp = 0;
     }
@@ -64,7 +64,7 @@ inline long long epj_region_slots(long long n_e) { return n_e > 0 ? (n_e + 7) / 
 
 // Builds the plan; exec_mode PTH_EXEC_FUSED or PTH_EXEC_PLAIN, chunk_j the
 // largest list length (EPJ entries + 2 x SPJ entries) of one item (<= 0:
-// default), max_ipl the largest number of i per lane (1, 2 or 4), n_seg the
+// default), max_ipl the largest number of i per lane (1 to 4), n_seg the
 // number of walk segments of about equal cost (fused form; the items of a
 // walk do not depend on it).  Returns false if a size overflows the int32
 // ranges of the device records.

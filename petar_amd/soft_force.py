@@ -413,7 +413,7 @@ class Resident:
     def replan(self, max_ipl, chunk_j):
         self.ctx.check(lib().pth_resident_replan(self._h, max_ipl, chunk_j), "pth_resident_replan")
 
-    def autotune(self, ipl=(1, 2, 4), chunk=(1024, 2048, 4096), reps=3):
+    def autotune(self, ipl=(1, 2, 3, 4), chunk=(1024, 2048, 4096), reps=3):
         """Times every (max_ipl, chunk_j) pair on this evaluation, keeps the
         fastest; returns (max_ipl, chunk_j, ms)."""
         a, b = _i32(ipl), _i32(chunk)
@@ -639,7 +639,7 @@ class Server:
 def plan_debug(n_epi, n_epj, n_spj, exec_mode=EXEC_FUSED, chunk_j=0, max_ipl=4, n_seg=1, segments=False):
     """The work decomposition (host only): items [k, 8] (column 2 packs
     n_i | ipl << 16), es_off, ss_off, ritems [k, 8] (and the segments
-    [g, 10] when segments=True)."""
+    [g, 12] when segments=True)."""
     L = lib()
     ne, nj, ns = _i32(n_epi), _i32(n_epj), _i32(n_spj)
     nw = ne.size
@@ -652,7 +652,7 @@ def plan_debug(n_epi, n_epj, n_spj, exec_mode=EXEC_FUSED, chunk_j=0, max_ipl=4, 
     rit = np.zeros((max(nr.value, 1), 8), np.int32)
     es = np.zeros(nw + 1, np.int64)
     ss = np.zeros(nw + 1, np.int64)
-    sg = np.zeros((32, 10), np.int32)
+    sg = np.zeros((32, 12), np.int32)
     n2 = L.pth_plan_debug(nw, _ptr(ne, C.c_int), _ptr(nj, C.c_int), _ptr(ns, C.c_int), exec_mode, chunk_j, max_ipl,
                           n_seg, _ptr(items, C.c_int), items.shape[0], _ptr(es, C.c_longlong), _ptr(ss, C.c_longlong),
                           _ptr(rit, C.c_int), rit.shape[0], C.byref(nr), _ptr(sg, C.c_int))

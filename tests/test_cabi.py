@@ -90,7 +90,7 @@ def _coverage(ne, nj, ns, mode, cj, max_ipl=4):
         if w < 0:
             continue
         ni, ipl = nipl & 0xFFFF, nipl >> 16
-        assert ipl in (1, 2, 4) and ipl <= max_ipl
+        assert ipl in (1, 2, 3, 4) and ipl <= max_ipl
         assert 0 < ni <= 64 * ipl and ioff[w] <= ib and ib + ni <= ioff[w + 1]
         assert es[w] // 8 <= eb0 <= eb1 <= es[w + 1] // 8 and ss[w] <= s0 <= s1 <= ss[w + 1]
         assert tiles.setdefault((w, ib), (ni, ipl)) == (ni, ipl)
@@ -220,8 +220,8 @@ def test_plan_segments_keep_every_walks_items():
         ioff = np.concatenate([[0], np.cumsum(ne)])
         for g in sg:
             wb, we = g[0], g[1]
-            for c in range(3):
-                for it in items[g[2 + c]:g[2 + c] + g[5 + c]]:
+            for c in range(4):
+                for it in items[g[2 + c]:g[2 + c] + g[6 + c]]:
                     assert it[0] < 0 or wb <= it[0] < we
-            for r in rit[g[8]:g[8] + g[9]]:
+            for r in rit[g[10]:g[10] + g[11]]:
                 assert ioff[wb] <= r[0] < ioff[we]

@@ -96,7 +96,7 @@ def test_co_resident_dense(dense):
     _check(_resident(L, rin, rout, r_out=0.03), ref, amb, "co dense")
 
 
-@pytest.mark.parametrize("max_ipl", [1, 2, 4])
+@pytest.mark.parametrize("max_ipl", [1, 2, 3, 4])
 def test_co_split_chunks(binaries, max_ipl):
     """Lists split over many items: the self term is removed exactly once
     and pot_tot goes through the reduction."""

@@ -58,7 +58,7 @@ def test_resident_c1(c1, exec_mode):
     ctx.close()
 
 
-@pytest.mark.parametrize("max_ipl", [1, 2, 4])
+@pytest.mark.parametrize("max_ipl", [1, 2, 3, 4])
 def test_fused_split_chunks(c1, max_ipl):
     """Small chunk size forces every walk's list over many items + reduction,
     for each number of i-particles per lane."""
@@ -72,7 +72,7 @@ def test_fused_split_chunks(c1, max_ipl):
     ctx.close()
 
 
-@pytest.mark.parametrize("max_ipl", [1, 2, 4])
+@pytest.mark.parametrize("max_ipl", [1, 2, 3, 4])
 def test_ipl_variants_agree(c1, max_ipl):
     """Unsplit walks: every tiling gives results within FP32 rounding of each other."""
     L, a, ref, amb = c1
@@ -259,9 +259,9 @@ def test_replan_and_autotune(c1):
     assert all(np.array_equal(x, y) for x, y in zip(res, r2.results()))
     r2.close()
     ctx2.close()
-    ipl, cj, ms = r.autotune(ipl=(1, 2, 4), chunk=(512, 2048), reps=2)
+    ipl, cj, ms = r.autotune(ipl=(1, 2, 3, 4), chunk=(512, 2048), reps=2)
     print(f"[autotune c1] max_ipl {ipl} chunk_j {cj}: {ms:.3f} ms", flush=True)
-    assert ipl in (1, 2, 4) and cj in (512, 2048) and ms > 0
+    assert ipl in (1, 2, 3, 4) and cj in (512, 2048) and ms > 0
     r.run()
     _check(r.results(), ref, amb, "autotuned")
     r.close()
