@@ -274,6 +274,49 @@ def accuracy(res_acc, res_pot, res_nnb, L, r_out, frac=0.05, seed=0):
                 pot_rel_p99=float(np.percentile(ep, 99)), nnb_mismatch=int((res_nnb[rows] != rnnb).sum()))
 
 
+def changeover_mode(L, r_out, tiles, steps=10, frac=0.02, seed=0):
+    """The same lists in changeover mode (the (1 - K) weight and the
+    pot_soft / pot_tot rules fused into the EPJ loop): ms per evaluation,
+    Gint/s, and the error against the FP64 changeover oracle on a sample of
+    walks.  Radii r_in = 0.1 r_out, r_out as the linear cutoff (the lists'
+    r_search is 1.5 r_out); ids are the tree positions."""
+    from oracle import oracle_py as O
+    from petar_amd import soft_force as sf
+    rin, rout = np.full(L.epj_mass.size, 0.1 * r_out), np.full(L.epj_mass.size, r_out)
+    ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, changeover=1, max_ipl=tiles[0], chunk_j=tiles[1])
+    res = sf.Resident.from_lists(ctx, L, rin, rout)
+    res.run()
+    res.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res.run()
+    res.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    acc, ps, _, pt = res.results()
+    res.close()
+    ctx.close()
+    rng = np.random.default_rng(seed)
+    w_sel = np.sort(rng.choice(L.n_walk, max(1, int(frac * L.n_walk)), replace=False))
+    eoff, soff = L.walk_offsets()
+    ioff = np.concatenate([[0], np.cumsum(L.n_epi)])
+    idx = np.concatenate([np.arange(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w]) for w in w_sel])
+    rows = np.concatenate([np.arange(ioff[w], ioff[w + 1]) for w in w_sel])
+    racc, rps, rpt, _, _ = O.soft_force_walks_co(
+        L.n_epi[w_sel], L.n_epj[w_sel], L.n_spj[w_sel], L.epj_pos[idx], L.epj_rs[idx], idx,
+        np.concatenate([L.id_epj[eoff[w]:eoff[w + 1]] for w in w_sel]),
+        np.concatenate([L.id_spj[soff[w]:soff[w + 1]] for w in w_sel]), L.epj_pos, L.epj_mass, L.epj_rs,
+        np.arange(L.epj_mass.size), rin, rout, L.spj_pos, L.spj_mass, L.spj_quad, eps2=0.0, rout2=r_out ** 2)
+    ea = np.linalg.norm(acc[rows] - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    et = np.abs(pt[rows] - rpt) / np.abs(rpt)
+    ne, ns = L.interactions()
+    out = dict(ms_per_step=round(ms, 4), gint_s=round((ne + ns) / (ms * 1e-3) / 1e9, 2), sample_i=int(rows.size),
+               acc_rel_median=float(np.median(ea)), acc_rel_p99=float(np.percentile(ea, 99)),
+               pot_tot_rel_median=float(np.median(et)), pot_tot_rel_p99=float(np.percentile(et, 99)),
+               radii="r_in = 0.1 r_out, r_out = the cutoff radius, r_search = 1.5 r_out")
+    log(f"[bench] changeover mode: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +339,7 @@ def main():
                     help="i sampled for the error against the FP64 all-pairs direct sum (0: skip)")
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--no-device-tree", action="store_true")
+    ap.add_argument("--no-changeover", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the soft-force path is "
@@ -398,6 +442,8 @@ def main():
         if a.direct_sample > 0:
             out["error_vs_fp64_direct_sum"] = tree_error(acc, pot, L, r_out, a.direct_sample)
     res.close()
+    if not a.no_changeover and a.exec == "fused":
+        out["changeover_mode"] = changeover_mode(L, r_out, (ipl_used, cj_used))
     if not a.no_device_tree and a.exec == "fused":
         # same tiles and segments as the timed evaluation, so the forces can
         # be compared bit for bit
