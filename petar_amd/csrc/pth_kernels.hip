@@ -525,10 +525,57 @@ struct SpjRec {
     float x, y, z, m, qxx, qyy, qzz, qxy, qyz, qzx, mtr;
 };
 
+// One staged SPJ record from a wave-uniform address: scalar loads, used as
+// SGPR operands, or with VC copied into VGPRs first.
 template <bool VC> __device__ __forceinline__ SpjRec spj_load(cfp R)
 {
     return SpjRec{jv<VC>(R[0]), jv<VC>(R[1]), jv<VC>(R[2]), jv<VC>(R[3]), jv<VC>(R[4]), jv<VC>(R[5]),
                   jv<VC>(R[6]), jv<VC>(R[7]), jv<VC>(R[8]), jv<VC>(R[9]), jv<VC>(R[10])};
+}
+
+// One SPJ record against the lane's i (src/force_gpu_cuda.cu:276-323 in the
+// traceless form of src/phantomquad_for_p3t_x86.hpp:144-163)
+template <int IPL>
+__device__ __forceinline__ void spj_pair(const SpjRec &c, const float (&xi)[IPL], const float (&yi)[IPL],
+                                         const float (&zi)[IPL], const Consts &K, float (&ax)[IPL],
+                                         float (&ay)[IPL], float (&az)[IPL], float (&p)[IPL])
+{
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        const float dx = xi[q] - c.x;
+        const float dy = yi[q] - c.y;
+        const float dz = zi[q] - c.z;
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
+        const float ri = __builtin_amdgcn_rsqf(r2);
+        const float ri2 = ri * ri;
+        const float ri3 = ri * ri2;
+        const float ri4 = ri2 * ri2;
+        const float ri5 = ri2 * ri3;
+        const float qrx = __builtin_fmaf(c.qzx, dz, __builtin_fmaf(c.qxy, dy, c.qxx * dx));
+        const float qry = __builtin_fmaf(c.qyz, dz, __builtin_fmaf(c.qxy, dx, c.qyy * dy));
+        const float qrz = __builtin_fmaf(c.qyz, dy, __builtin_fmaf(c.qzx, dx, c.qzz * dz));
+        const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, c.mtr)));
+        const float rqr_ri4 = rqr * ri4;
+        const float meff = __builtin_fmaf(rqr_ri4, K.half, c.m);
+        const float meff3 = __builtin_fmaf(rqr_ri4, K.two_half, c.m) * ri3;
+        p[q] = __builtin_fmaf(-meff, ri, p[q]);
+        ax[q] = __builtin_fmaf(-meff3, dx, __builtin_fmaf(ri5, qrx, ax[q]));
+        ay[q] = __builtin_fmaf(-meff3, dy, __builtin_fmaf(ri5, qry, ay[q]));
+        az[q] = __builtin_fmaf(-meff3, dz, __builtin_fmaf(ri5, qrz, az[q]));
+    }
+}
+
+template <int IPL>
+__device__ __forceinline__ void spj_flush(const float (&ax)[IPL], const float (&ay)[IPL], const float (&az)[IPL],
+                                          const float (&p)[IPL], double (&A)[IPL][4])
+{
+#pragma unroll
+    for (int q = 0; q < IPL; q++) {
+        A[q][0] += (double)ax[q];
+        A[q][1] += (double)ay[q];
+        A[q][2] += (double)az[q];
+        A[q][3] += (double)p[q];
+    }
 }
 
 // SPJ records [s_begin, s_end): the record after the current one is loaded
@@ -549,38 +596,10 @@ __device__ __forceinline__ void spj_records(cfp ss, int s_begin, int s_end, cons
         for (int s = s0; s < s1; s++) {
             // next record (the last one is re-read: stays in bounds)
             const SpjRec n = spj_load<VC>(ss + (size_t)min(s + 1, s_end - 1) * 12);
-#pragma unroll
-            for (int q = 0; q < IPL; q++) {
-                const float dx = xi[q] - c.x;
-                const float dy = yi[q] - c.y;
-                const float dz = zi[q] - c.z;
-                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
-                const float ri = __builtin_amdgcn_rsqf(r2);
-                const float ri2 = ri * ri;
-                const float ri3 = ri * ri2;
-                const float ri4 = ri2 * ri2;
-                const float ri5 = ri2 * ri3;
-                const float qrx = __builtin_fmaf(c.qzx, dz, __builtin_fmaf(c.qxy, dy, c.qxx * dx));
-                const float qry = __builtin_fmaf(c.qyz, dz, __builtin_fmaf(c.qxy, dx, c.qyy * dy));
-                const float qrz = __builtin_fmaf(c.qyz, dy, __builtin_fmaf(c.qzx, dx, c.qzz * dz));
-                const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, c.mtr)));
-                const float rqr_ri4 = rqr * ri4;
-                const float meff = __builtin_fmaf(rqr_ri4, K.half, c.m);
-                const float meff3 = __builtin_fmaf(rqr_ri4, K.two_half, c.m) * ri3;
-                p[q] = __builtin_fmaf(-meff, ri, p[q]);
-                ax[q] = __builtin_fmaf(-meff3, dx, __builtin_fmaf(ri5, qrx, ax[q]));
-                ay[q] = __builtin_fmaf(-meff3, dy, __builtin_fmaf(ri5, qry, ay[q]));
-                az[q] = __builtin_fmaf(-meff3, dz, __builtin_fmaf(ri5, qrz, az[q]));
-            }
+            spj_pair<IPL>(c, xi, yi, zi, K, ax, ay, az, p);
             c = n;
         }
-#pragma unroll
-        for (int q = 0; q < IPL; q++) {
-            A[q][0] += (double)ax[q];
-            A[q][1] += (double)ay[q];
-            A[q][2] += (double)az[q];
-            A[q][3] += (double)p[q];
-        }
+        spj_flush<IPL>(ax, ay, az, p, A);
     }
 }
 

@@ -84,6 +84,25 @@ def test_ipl_variants_agree(c1, max_ipl):
     ctx.close()
 
 
+@pytest.mark.parametrize("j_vgpr", [1])
+@pytest.mark.parametrize("max_ipl", [1, 2, 4])
+def test_j_record_modes_are_bit_identical(c1, j_vgpr, max_ipl):
+    """The j-record modes (SGPR operands, VGPR copies) compute the same FP32
+    operations in the same order: bit-identical results, also with split
+    walks."""
+    L = c1[0]
+    out = []
+    for jm in (0, j_vgpr):
+        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, chunk_j=512, max_ipl=max_ipl, j_vgpr=jm)
+        r = sf.Resident.from_lists(ctx, L)
+        r.run()
+        out.append(r.results())
+        r.close()
+        ctx.close()
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
 def _fdps_replay(L, functor, n_walk_limit=200):
     """Replays FDPS's multi-walk call order (send once, then dispatch/retrieve
     per batch of <= n_walk_limit walks) with FDPS-layout records."""
