@@ -87,7 +87,8 @@ typedef struct pth_params {
     double ngb_factor; /* changeover mode: neighbour radius factor on r_search
                           (0 = 0.99, SAFTY_FACTOR_FOR_SEARCH of src/ptcl.hpp:8) */
     int segments;  /* resident evaluations: walks split into this many segments
-                      of equal cost (1..32) and pipelined: one stream stages
+                      of equal cost (1..32; the first of half the cost) and
+                      pipelined: one stream stages
                       the segments in turn, the others run each segment's
                       force and reduce kernels once it is staged, so staging
                       overlaps the force kernels; results do not depend on

@@ -12,8 +12,9 @@
 // hence staged lists, so each list is fetched into one XCD's L2), most
 // expensive first inside each run, and written in the order the hardware
 // hands workgroups to XCDs (round robin), padding short runs with empty
-// items.  With several segments (contiguous walk ranges of equal cost, run
-// concurrently on different streams) the dealing is done per segment.
+// items.  With several segments (contiguous walk ranges of equal cost, the
+// first of half that, pipelined over several streams) the dealing is done
+// per segment.
 #include "pth_plan.h"
 
 #include <algorithm>
@@ -262,7 +263,8 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         }
     });
     P.n_real_items = (int)seq.size();
-    // 2. segments: contiguous walk ranges of about equal item cost
+    // 2. segments: contiguous walk ranges of about equal item cost (the
+    //    first one half of it)
     if (n_seg < 1) n_seg = 1;
     if (n_seg > MAX_SEGMENTS) n_seg = MAX_SEGMENTS;
     if (n_seg > n_walk) n_seg = std::max(n_walk, 1);
@@ -273,10 +275,12 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         for (size_t k = walk_item_begin[w]; k < walk_item_begin[w + 1]; k++) c += seq[k].cost;
         wcost[w + 1] = wcost[w] + c;
     }
+    // the first segment gets half the cost of the others: its staging is the
+    // one part of a pipelined evaluation that nothing overlaps
     std::vector<int> cut(n_seg + 1, 0);
     cut[n_seg] = n_walk;
     for (int g = 1; g < n_seg; g++) {
-        const double target = wcost[n_walk] * g / n_seg;
+        const double target = wcost[n_walk] * (g - 0.5) / (n_seg - 0.5);
         cut[g] = (int)(std::lower_bound(wcost.begin(), wcost.end(), target) - wcost.begin());
         cut[g] = std::min(std::max(cut[g], cut[g - 1]), n_walk);
     }
