@@ -105,7 +105,7 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
                 walls.append((t3 - t0) * 1e3)
         ph = np.median(np.array(phases), axis=0)
         res[label] = dict(step_ms=round(float(np.median(walls)), 3), tree_build_ms=round(float(ph[0]), 3),
-                          walk_ms=round(float(ph[1]), 3), canonical_order_ms=round(float(ph[2]), 3),
+                          walk_count_ms=round(float(ph[1]), 3), write_order_plan_ms=round(float(ph[2]), 3),
                           refill_ms=round(float(ph[3]), 3), force_ms=round(float(ph[4]), 3))
     g.update(tx, tm, tr)
     ngb_ms = []

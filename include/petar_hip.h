@@ -294,8 +294,10 @@ int pth_gtree_download(const pth_gtree *tree, long long *order, long long *node_
                        int *node_level, double *node_mass, double *node_com, double *node_quad,
                        long long *epi_begin, int *n_epi, int *n_epj, int *n_spj, int *id_epj, int *id_spj);
 /* Host wall time of the last build (ms): tree (upload, sort, nodes, moments,
- * groups), walk (count, scan, write) and the canonical ordering (0 when not
- * asked for). */
+ * groups), walk (counting pass, scan, list sizes) and the rest (writing
+ * pass, canonical ordering when asked for, and meanwhile on the host the
+ * plan of a resident evaluation under the context's parameters, which
+ * pth_resident_from_gtree / pth_resident_refill then take over). */
 int pth_gtree_timing(const pth_gtree *tree, float *ms_build, float *ms_walk, float *ms_order);
 void pth_gtree_free(pth_gtree *tree);
 /* Neighbour search on the tree (replaces the FDPS short-range tree search

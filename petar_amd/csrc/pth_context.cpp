@@ -816,15 +816,19 @@ long long pth_plan_debug(int n_walk, const int *n_epi, const int *n_epj, const i
 namespace pth {
 
 int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
-                   long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot, const char *who)
+                   long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot, const char *who, Plan *pre,
+                   bool pre_ok)
 {
     pth_ctx *c = r->ctx;
     r->ran = false;
     r->co_ready = false;
     r->pipelined = false;
     r->n_seg = (c->par.exec_mode == PTH_EXEC_FUSED) ? c->par.segments : 1;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
-                   r->n_seg))
+    if (pre) {
+        if (!pre_ok) return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
+        std::swap(r->plan, *pre);
+    } else if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
+                          r->n_seg))
         return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
     const Plan &P = r->plan;
     r->co = c->par.changeover != 0;

@@ -29,6 +29,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <chrono>
+#include <cmath>
 
 #include "pth_internal.h"
 
@@ -58,21 +59,26 @@ __device__ __forceinline__ unsigned long long spread3(unsigned long long v)
     return v;
 }
 
-// per-block min/max of the coordinates and max of r_search (rs may be null):
-// part[block * 7 + {0..2: min, 3..5: max, 6: max rs}]
+// per-block min/max of the coordinates, max of r_search (rs may be null) and
+// a flag for non-finite values (which fmin/fmax would skip):
+// part[block * 8 + {0..2: min, 3..5: max, 6: max rs, 7: 1 if any non-finite}]
 __global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ pos, const double *__restrict__ rs,
                                               long long n, double *__restrict__ part)
 {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rmax = 0.0;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rmax = 0.0, bad = 0.0;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
         for (int d = 0; d < 3; d++) {
             const double x = pos[3 * i + d];
             lo[d] = fmin(lo[d], x);
             hi[d] = fmax(hi[d], x);
+            if (!isfinite(x)) bad = 1.0;
         }
-        if (rs) rmax = fmax(rmax, rs[i]);
+        if (rs) {
+            rmax = fmax(rmax, rs[i]);
+            if (!isfinite(rs[i])) bad = 1.0;
+        }
     }
-    __shared__ double red[4][7];
+    __shared__ double red[4][8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int off = 32; off > 0; off >>= 1) {
         for (int d = 0; d < 3; d++) {
@@ -80,6 +86,7 @@ __global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ pos, co
             hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
         }
         rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+        bad = fmax(bad, __shfl_xor(bad, off, 64));
     }
     if (lane == 0) {
         for (int d = 0; d < 3; d++) {
@@ -87,13 +94,14 @@ __global__ __launch_bounds__(256) void k_bbox(const double *__restrict__ pos, co
             red[wave][3 + d] = hi[d];
         }
         red[wave][6] = rmax;
+        red[wave][7] = bad;
     }
     __syncthreads();
-    if (threadIdx.x < 7) {
+    if (threadIdx.x < 8) {
         const int d = threadIdx.x;
         double v = red[0][d];
         for (int w = 1; w < 4; w++) v = (d < 3) ? fmin(v, red[w][d]) : fmax(v, red[w][d]);
-        part[blockIdx.x * 7 + d] = v;
+        part[blockIdx.x * 8 + d] = v;
     }
 }
 
@@ -692,6 +700,13 @@ struct pth_gtree {
     }
     std::vector<long long> level_start;
     std::vector<int4> groups_h;  // per group: first particle, size, n_epj, n_spj
+    // the plan of a resident evaluation of the last build's lists, made on
+    // the host while the device writes and orders them, and the parameters
+    // it was made for
+    mutable Plan pre_plan;  // taken over (swapped out) by a refill
+    PlanKey pre_key;
+    bool pre_ok = false;
+    mutable bool pre_valid = false;
     int n_leaf_limit = 20, n_group_limit = 512;
     double theta = 0.3;
     double rs_max = 0.0, root_len = 1.0;  // largest r_search, root cell side
@@ -786,10 +801,10 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     const int n_leaf_limit = t->n_leaf_limit, n_group_limit = t->n_group_limit;
     const double theta = t->theta;
     t->built = false;
+    t->pre_valid = false;
     t->n_ngb = -1;
     t->n = n;
     t->n_node = t->n_group = t->n_le = t->n_ls = 0;
-    auto bail = [&](int rc) { return rc; };
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = ctx->stream;
     hipError_t e;
@@ -807,7 +822,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         };
         if ((e = up(t->dpos, pos, 3 * (size_t)n)) || (e = up(t->dmass, mass, (size_t)n)) ||
             (r_search && (e = up(t->drs, r_search, (size_t)n))))
-            return bail(ctx->hip_fail(e, "pth_gtree: upload"));
+            return ctx->hip_fail(e, "pth_gtree: upload");
         dpos = t->dpos.as<double>();
         dmass = t->dmass.as<double>();
         drs = r_search ? t->drs.as<double>() : nullptr;
@@ -815,21 +830,24 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     // 1. bounding cube (per-block partials reduced on the host)
     const int nb = (int)std::min<long long>(1024, (n + 255) / 256);
     DevBuf &part = t->part;
-    std::vector<double> hp(7 * (size_t)nb);
-    if ((e = part.ensure(sizeof(double) * 7 * nb))) return bail(ctx->hip_fail(e, "bbox"));
+    std::vector<double> hp(8 * (size_t)nb);
+    if ((e = part.ensure(sizeof(double) * 8 * nb))) return ctx->hip_fail(e, "bbox");
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, dpos, drs, n, part.as<double>());
-    if ((e = hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 7 * nb, hipMemcpyDeviceToHost, st)) ||
+    if ((e = hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
-        return bail(ctx->hip_fail(e, "bbox"));
+        return ctx->hip_fail(e, "bbox");
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     t->rs_max = 0.0;
+    bool bad = false;
     for (int b = 0; b < nb; b++) {
         for (int d = 0; d < 3; d++) {
-            lo[d] = std::min(lo[d], hp[7 * b + d]);
-            hi[d] = std::max(hi[d], hp[7 * b + 3 + d]);
+            lo[d] = std::min(lo[d], hp[8 * b + d]);
+            hi[d] = std::max(hi[d], hp[8 * b + 3 + d]);
         }
-        t->rs_max = std::max(t->rs_max, hp[7 * b + 6]);
+        t->rs_max = std::max(t->rs_max, hp[8 * b + 6]);
+        bad = bad || hp[8 * b + 7] != 0.0;
     }
+    if (bad) return ctx->fail(PTH_ERR_ARG, "pth_gtree: non-finite particle positions or r_search");
     double len = 0;
     for (int d = 0; d < 3; d++) len = std::max(len, hi[d] - lo[d]);
     len = (len > 0) ? len * (1.0 + 1e-9) : 1.0;
@@ -840,7 +858,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     // 2. keys, stable sort of (key, index), sorted particles
     if ((e = t->key.ensure(8 * (size_t)n)) || (e = t->skey.ensure(8 * (size_t)n)) ||
         (e = t->sidx.ensure(8 * (size_t)n)) || (e = t->order.ensure(8 * (size_t)n)))
-        return bail(ctx->hip_fail(e, "key allocation"));
+        return ctx->hip_fail(e, "key allocation");
     hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(256), 0, st, dpos, n, root[0], root[1], root[2],
                        scale, t->key.as<unsigned long long>(), t->sidx.as<long long>());
     size_t tb = 0;
@@ -851,22 +869,22 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         (e = hipcub::DeviceRadixSort::SortPairs(t->tmp.p, tb, t->key.as<unsigned long long>(),
                                                 t->skey.as<unsigned long long>(), t->sidx.as<long long>(),
                                                 t->order.as<long long>(), n, 0, 3 * GT_MAX_LEVEL, st)))
-        return bail(ctx->hip_fail(e, "key sort"));
+        return ctx->hip_fail(e, "key sort");
     if ((e = t->spos.ensure(24 * (size_t)n)) || (e = t->smass.ensure(8 * (size_t)n)) ||
         (e = t->srs.ensure(8 * (size_t)n)))
-        return bail(ctx->hip_fail(e, "particle allocation"));
+        return ctx->hip_fail(e, "particle allocation");
     hipLaunchKernelGGL(k_gather, dim3(blocks(n)), dim3(256), 0, st, n, t->order.as<long long>(), dpos,
                        dmass, drs, t->spos.as<double>(),
                        t->smass.as<double>(), t->srs.as<double>());
     // 3. nodes level by level
     if ((e = t->nodes.ensure(sizeof(GNode) * std::max<long long>(2 * n / std::max(n_leaf_limit, 1) + 64, 1024))))
-        return bail(ctx->hip_fail(e, "node allocation"));
+        return ctx->hip_fail(e, "node allocation");
     long long cap = (long long)(t->nodes.cap / sizeof(GNode));
     {
         const GNode root_node{0, n, 0, 0, -1};
         if ((e = hipMemcpyAsync(t->nodes.p, &root_node, sizeof(GNode), hipMemcpyHostToDevice, st)) ||
             (e = hipStreamSynchronize(st)))
-            return bail(ctx->hip_fail(e, "node upload"));
+            return ctx->hip_fail(e, "node upload");
     }
     t->level_start.assign(1, 0);
     long long lb = 0, le = 1;
@@ -876,18 +894,18 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         const long long nl = le - lb;
         if ((e = bnd.ensure(sizeof(long long) * 9 * nl)) || (e = cnt.ensure(sizeof(int) * (nl + 1))) ||
             (e = off.ensure(sizeof(int) * (nl + 1))))
-            return bail(ctx->hip_fail(e, "level buffers"));
+            return ctx->hip_fail(e, "level buffers");
         hipLaunchKernelGGL(k_split_count, dim3(blocks(nl)), dim3(256), 0, st, t->nodes.as<GNode>(), lb, nl,
                            t->skey.as<unsigned long long>(), n_leaf_limit, bnd.as<long long>(), cnt.as<int>());
         if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<int>(), off.as<int>(), (int)nl, st)) ||
             (e = cub_temp(t, tb)) ||
             (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, cnt.as<int>(), off.as<int>(), (int)nl, st)))
-            return bail(ctx->hip_fail(e, "level scan"));
+            return ctx->hip_fail(e, "level scan");
         int last[2] = {0, 0};
         if ((e = hipMemcpyAsync(&last[0], cnt.as<int>() + nl - 1, sizeof(int), hipMemcpyDeviceToHost, st)) ||
             (e = hipMemcpyAsync(&last[1], off.as<int>() + nl - 1, sizeof(int), hipMemcpyDeviceToHost, st)) ||
             (e = hipStreamSynchronize(st)))
-            return bail(ctx->hip_fail(e, "level size"));
+            return ctx->hip_fail(e, "level size");
         const long long total = (long long)last[0] + last[1];
         if (total == 0) break;
         if (le + total > cap) {
@@ -897,7 +915,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
             if ((e = nb2.ensure(sizeof(GNode) * ncap)) ||
                 (e = hipMemcpyAsync(nb2.p, t->nodes.p, sizeof(GNode) * le, hipMemcpyDeviceToDevice, st)) ||
                 (e = hipStreamSynchronize(st)))
-                return bail(ctx->hip_fail(e, "node growth"));
+                return ctx->hip_fail(e, "node growth");
             t->nodes.release();
             t->nodes = nb2;
             cap = ncap;
@@ -909,12 +927,12 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     }
     t->n_node = le;
     if (le > 0x7fffffffLL)  // the walks keep node indices as int32
-        return bail(ctx->fail(PTH_ERR_CAPACITY, "pth_gtree: node count exceeds the int32 range"));
+        return ctx->fail(PTH_ERR_CAPACITY, "pth_gtree: node count exceeds the int32 range");
     const long long nn = le;
     // 4. moments bottom-up
     if ((e = t->nmass.ensure(8 * nn)) || (e = t->ncom.ensure(24 * nn)) || (e = t->nquad.ensure(48 * nn)) ||
         (e = t->box_in.ensure(48 * nn)) || (e = t->box_out.ensure(48 * nn)))
-        return bail(ctx->hip_fail(e, "moment allocation"));
+        return ctx->hip_fail(e, "moment allocation");
     for (int l = (int)t->level_start.size() - 2; l >= 0; l--) {
         const long long a = t->level_start[l], b = t->level_start[l + 1];
         if (b <= a) continue;
@@ -931,7 +949,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         t->n_group = ng_sel;
     }
     const long long ng = t->n_group;
-    if ((e = hipStreamSynchronize(st))) return bail(ctx->hip_fail(e, "tree build"));
+    if ((e = hipStreamSynchronize(st))) return ctx->hip_fail(e, "tree build");
     const auto t1 = std::chrono::steady_clock::now();
     t->ms_build = std::chrono::duration<float, std::milli>(t1 - t0).count();
     // 6. walk: count, scan, write
@@ -948,7 +966,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     DevBuf &ce = t->ce, &cs = t->cs;
     if ((e = ce.ensure(8 * (ng + 1))) || (e = cs.ensure(8 * (ng + 1))) || (e = t->off_e.ensure(8 * (ng + 1))) ||
         (e = t->off_s.ensure(8 * (ng + 1))))
-        return bail(ctx->hip_fail(e, "walk allocation"));
+        return ctx->hip_fail(e, "walk allocation");
     wa.cnt_e = ce.as<long long>();
     wa.cnt_s = cs.as<long long>();
     const unsigned wg = (unsigned)std::min<long long>(ng, 65536);
@@ -956,7 +974,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     long long zero = 0;
     if ((e = hipMemcpyAsync(ce.as<long long>() + ng, &zero, 8, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(cs.as<long long>() + ng, &zero, 8, hipMemcpyHostToDevice, st)))
-        return bail(ctx->hip_fail(e, "walk counts"));
+        return ctx->hip_fail(e, "walk counts");
     if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ce.as<long long>(), t->off_e.as<long long>(), (int)ng + 1,
                                               st)) ||
         (e = cub_temp(t, tb)) ||
@@ -964,74 +982,98 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
                                               st)) ||
         (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, cs.as<long long>(), t->off_s.as<long long>(), (int)ng + 1,
                                               st)))
-        return bail(ctx->hip_fail(e, "walk scan"));
+        return ctx->hip_fail(e, "walk scan");
+    // per-group sizes (first particle, size, list lengths) and the totals
+    t->groups_h.assign(ng, make_int4(0, 0, 0, 0));
+    if ((e = t->info.ensure(sizeof(int4) * std::max<long long>(ng, 1)))) return ctx->hip_fail(e, "group sizes");
+    if (ng > 0)
+        hipLaunchKernelGGL(k_group_info, dim3(blocks(ng)), dim3(256), 0, st, t->nodes.as<GNode>(), t->groups.as<int>(),
+                           ng, t->off_e.as<long long>(), t->off_s.as<long long>(), t->info.as<int4>());
     long long tot[2] = {0, 0};
-    if ((e = hipMemcpyAsync(&tot[0], t->off_e.as<long long>() + ng, 8, hipMemcpyDeviceToHost, st)) ||
+    if ((ng > 0 &&
+         (e = hipMemcpyAsync(t->groups_h.data(), t->info.p, sizeof(int4) * ng, hipMemcpyDeviceToHost, st))) ||
+        (e = hipMemcpyAsync(&tot[0], t->off_e.as<long long>() + ng, 8, hipMemcpyDeviceToHost, st)) ||
         (e = hipMemcpyAsync(&tot[1], t->off_s.as<long long>() + ng, 8, hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
-        return bail(ctx->hip_fail(e, "walk totals"));
+        return ctx->hip_fail(e, "walk totals");
     if (tot[0] > 0x7fffffffLL || tot[1] > 0x7fffffffLL)
-        return bail(ctx->fail(PTH_ERR_CAPACITY, "pth_gtree_build: lists exceed the int32 range"));
+        return ctx->fail(PTH_ERR_CAPACITY, "pth_gtree_build: lists exceed the int32 range");
     t->n_le = tot[0];
     t->n_ls = tot[1];
-    if ((e = t->id_epj.ensure(4 * std::max<long long>(tot[0], 1))) ||
-        (e = t->id_spj.ensure(4 * std::max<long long>(tot[1], 1))))
-        return bail(ctx->hip_fail(e, "list allocation"));
+    // every buffer of the write pass and the ordering sorts, before anything
+    // is enqueued (a growing buffer's free would wait for the device)
+    const bool order = t->canonical && ng > 0;
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) < n) bits++;  // keys are particle positions < n
+    const long long *ob = t->off_e.as<long long>(), *osp = t->off_s.as<long long>();
+    size_t tb_e = 0, tb_s = 0;
+    // the ordering swaps id_epj / id_spj with alt, so each of the three may
+    // hold either list next: all three are sized for the longer one
+    const size_t lbytes = 4 * (size_t)std::max<long long>(std::max(tot[0], tot[1]), 1);
+    if ((e = t->id_epj.ensure(lbytes)) || (e = t->id_spj.ensure(lbytes)))
+        return ctx->hip_fail(e, "list allocation");
+    if (order) {
+        if ((e = t->alt.ensure(lbytes)) || (e = t->lkey.ensure(4 * std::max<long long>(tot[1], 1))) ||
+            (e = t->lkey2.ensure(4 * std::max<long long>(tot[1], 1))) ||
+            (e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb_e, t->id_epj.as<int>(), t->alt.as<int>(),
+                                                            (int)tot[0], (int)ng, ob, ob + 1, 0, bits, st)) ||
+            (e = hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb_s, t->lkey.as<int>(), t->lkey2.as<int>(),
+                                                             t->id_spj.as<int>(), t->alt.as<int>(), (int)tot[1],
+                                                             (int)ng, osp, osp + 1, 0, bits, st)) ||
+            (e = cub_temp(t, std::max(tb_e, tb_s))))
+            return ctx->hip_fail(e, "list ordering allocation");
+    }
+    // write pass, then the ordering, all enqueued without waits
     wa.off_e = t->off_e.as<long long>();
     wa.off_s = t->off_s.as<long long>();
     wa.id_epj = t->id_epj.as<int>();
     wa.id_spj = t->id_spj.as<int>();
-    if (ng > 0) hipLaunchKernelGGL(k_walk<true>, dim3(wg), dim3(64), 0, st, wa);
-    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(ctx->hip_fail(e, "tree walk"));
     const auto t2 = std::chrono::steady_clock::now();
     t->ms_walk = std::chrono::duration<float, std::milli>(t2 - t1).count();
+    if (t->id_epj.cap < 4 * (size_t)tot[0] || t->id_spj.cap < 4 * (size_t)tot[1])
+        return ctx->fail(PTH_ERR_STATE, "pth_gtree: list buffers smaller than the lists");
+    if (ng > 0) hipLaunchKernelGGL(k_walk<true>, dim3(wg), dim3(64), 0, st, wa);
     // 7. canonical order: segmented sorts of the lists (one segment per group)
-    if (t->canonical && ng > 0) {
-        int bits = 1;
-        while (bits < 31 && (1LL << bits) < n) bits++;  // keys are particle positions < n
-        const long long *ob = t->off_e.as<long long>(), *os = t->off_s.as<long long>();
+    if (order) {
+        if (t->alt.cap < lbytes || t->id_epj.cap < lbytes || t->id_spj.cap < lbytes ||
+            t->lkey.cap < 4 * (size_t)tot[1] || t->lkey2.cap < 4 * (size_t)tot[1] ||
+            t->tmp.cap < std::max(tb_e, tb_s))
+            return ctx->fail(PTH_ERR_STATE, "pth_gtree: ordering buffers smaller than the lists");
         if (tot[0] > 0) {
-            if ((e = t->alt.ensure(4 * tot[0])) ||
-                (e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, t->id_epj.as<int>(), t->alt.as<int>(),
-                                                                (int)tot[0], (int)ng, ob, ob + 1, 0, bits, st)) ||
-                (e = cub_temp(t, tb)) ||
-                (e = hipcub::DeviceSegmentedRadixSort::SortKeys(t->tmp.p, tb, t->id_epj.as<int>(), t->alt.as<int>(),
-                                                                (int)tot[0], (int)ng, ob, ob + 1, 0, bits, st)) ||
-                (e = hipStreamSynchronize(st)))
-                return bail(ctx->hip_fail(e, "EPJ list ordering"));
+            if ((e = hipcub::DeviceSegmentedRadixSort::SortKeys(t->tmp.p, tb_e, t->id_epj.as<int>(), t->alt.as<int>(),
+                                                                (int)tot[0], (int)ng, ob, ob + 1, 0, bits, st)))
+                return ctx->hip_fail(e, "EPJ list ordering");
             std::swap(t->id_epj, t->alt);
         }
         if (tot[1] > 0) {
-            if ((e = t->alt.ensure(4 * tot[1])) || (e = t->lkey.ensure(4 * tot[1])) ||
-                (e = t->lkey2.ensure(4 * tot[1])))
-                return bail(ctx->hip_fail(e, "SPJ list ordering"));
             hipLaunchKernelGGL(k_spj_keys, dim3(blocks(tot[1])), dim3(256), 0, st, t->nodes.as<GNode>(),
                                t->id_spj.as<int>(), tot[1], t->lkey.as<int>());
-            if ((e = hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb, t->lkey.as<int>(), t->lkey2.as<int>(),
+            if ((e = hipcub::DeviceSegmentedRadixSort::SortPairs(t->tmp.p, tb_s, t->lkey.as<int>(), t->lkey2.as<int>(),
                                                                  t->id_spj.as<int>(), t->alt.as<int>(), (int)tot[1],
-                                                                 (int)ng, os, os + 1, 0, bits, st)) ||
-                (e = cub_temp(t, tb)) ||
-                (e = hipcub::DeviceSegmentedRadixSort::SortPairs(t->tmp.p, tb, t->lkey.as<int>(), t->lkey2.as<int>(),
-                                                                 t->id_spj.as<int>(), t->alt.as<int>(), (int)tot[1],
-                                                                 (int)ng, os, os + 1, 0, bits, st)) ||
-                (e = hipStreamSynchronize(st)))
-                return bail(ctx->hip_fail(e, "SPJ list ordering"));
+                                                                 (int)ng, osp, osp + 1, 0, bits, st)))
+                return ctx->hip_fail(e, "SPJ list ordering");
             std::swap(t->id_spj, t->alt);
         }
     }
-    const auto t3 = std::chrono::steady_clock::now();
-    t->ms_order = t->canonical ? std::chrono::duration<float, std::milli>(t3 - t2).count() : 0.f;
-    (void)t3;
-    // per-group sizes for the planner
-    t->groups_h.assign(ng, make_int4(0, 0, 0, 0));
-    if (ng > 0) {
-        if ((e = t->info.ensure(sizeof(int4) * ng))) return bail(ctx->hip_fail(e, "group sizes"));
-        hipLaunchKernelGGL(k_group_info, dim3(blocks(ng)), dim3(256), 0, st, t->nodes.as<GNode>(), t->groups.as<int>(),
-                           ng, t->off_e.as<long long>(), t->off_s.as<long long>(), t->info.as<int4>());
-        if ((e = hipMemcpyAsync(t->groups_h.data(), t->info.p, sizeof(int4) * ng, hipMemcpyDeviceToHost, st)) ||
-            (e = hipStreamSynchronize(st)))
-            return bail(ctx->hip_fail(e, "group sizes"));
+    // meanwhile, the plan of a resident evaluation of these lists under the
+    // context's parameters (pth_resident_from_gtree / _refill take it over)
+    {
+        std::vector<int> ni(ng), nje(ng), njs(ng);
+        for (long long g = 0; g < ng; g++) {
+            ni[g] = t->groups_h[g].y;
+            nje[g] = t->groups_h[g].z;
+            njs[g] = t->groups_h[g].w;
+        }
+        const pth_params &par = ctx->par;
+        t->pre_key = PlanKey{par.exec_mode, par.chunk_j, par.max_ipl,
+                             par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1};
+        t->pre_ok = make_plan((int)ng, ni.data(), nje.data(), njs.data(), t->pre_key.exec_mode, t->pre_key.chunk_j,
+                              t->pre_key.max_ipl, t->pre_plan, t->pre_key.n_seg);
+        t->pre_valid = true;
     }
+    if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return ctx->hip_fail(e, "tree walk");
+    const auto t3 = std::chrono::steady_clock::now();
+    t->ms_order = std::chrono::duration<float, std::milli>(t3 - t2).count();
     t->built = true;
     return PTH_OK;
 }
@@ -1056,8 +1098,14 @@ int resident_fill_gtree(pth_resident *r, const pth_gtree *t, const char *who)
         ne[g] = t->groups_h[g].z;
         ns[g] = t->groups_h[g].w;
     }
+    // the plan made during the build, when it was made for this context's
+    // parameters (then taken over: a second refill plans again)
+    const pth_params &par = c->par;
+    const PlanKey key{par.exec_mode, par.chunk_j, par.max_ipl, par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1};
+    const bool take = t->pre_valid && t->pre_key == key;
     const int rc = resident_setup(r, (int)ng, ni.data(), ne.data(), ns.data(), t->n_le, t->n_ls, (int)t->n,
-                                  (int)t->n_node, who);
+                                  (int)t->n_node, who, take ? &t->pre_plan : nullptr, t->pre_ok);
+    if (take) t->pre_valid = false;
     if (rc != PTH_OK) return rc;
     const Plan &P = r->plan;
     const Arena &A = r->A;

@@ -317,8 +317,11 @@ int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_ep
                      const char *who);
 // Re-plans an existing resident for new sizes, growing its device memory as
 // needed (contents not yet uploaded; the changeover data is invalidated).
+// With pre, that plan (made for these sizes and the context's parameters;
+// pre_ok its make_plan result) is taken over instead of planning again.
 int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
-                   long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot, const char *who);
+                   long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot, const char *who,
+                   Plan *pre = nullptr, bool pre_ok = true);
 // Launch descriptors of a prepared resident whose contents are uploaded.
 void resident_finish(pth_resident *r);
 }  // namespace pth

@@ -58,6 +58,15 @@ struct Plan {
     }
 };
 
+// The parameters a plan was made with (exec_mode, chunk_j, max_ipl, n_seg).
+struct PlanKey {
+    int exec_mode = 0, chunk_j = 0, max_ipl = 0, n_seg = 1;
+    bool operator==(const PlanKey &o) const
+    {
+        return exec_mode == o.exec_mode && chunk_j == o.chunk_j && max_ipl == o.max_ipl && n_seg == o.n_seg;
+    }
+};
+
 // Staged EPJ slots of a walk: near and far regions with at least 8 slots of
 // padding between them.
 inline long long epj_region_slots(long long n_e) { return n_e > 0 ? (n_e + 7) / 8 * 8 + 8 : 0; }

@@ -520,7 +520,9 @@ class GpuTree:
         return cnt, off, ngb[:tot.value], ms.value
 
     def timing(self):
-        """Host wall ms of the last build: (tree, walk, canonical ordering)."""
+        """Host wall ms of the last build: (tree, walk counting pass and list
+        sizes, writing pass + canonical ordering with the host plan made
+        meanwhile)."""
         a, b, c = C.c_float(), C.c_float(), C.c_float()
         self.ctx.check(lib().pth_gtree_timing(self._h, C.byref(a), C.byref(b), C.byref(c)), "pth_gtree_timing")
         return a.value, b.value, c.value

@@ -34,8 +34,8 @@ def _compare(n, seed, f_bin, theta, leaf=20, group=512, r_search=1.5 * R_OUT, mo
         g.close()
         ctx.close()
     print(f"[gtree N={n} theta={theta} canonical={canonical}] host tree+walk {t_host * 1e3:.1f} ms, device build "
-          f"{tb:.2f} ms walk {tw:.2f} ms order {to:.2f} ms; {len(d['n_epi'])} groups, {d['id_epj'].size} EPJ + "
-          f"{d['id_spj'].size} SPJ entries", flush=True)
+          f"{tb:.2f} ms walk count {tw:.2f} ms write+order {to:.2f} ms; {len(d['n_epi'])} groups, "
+          f"{d['id_epj'].size} EPJ + {d['id_spj'].size} SPJ entries", flush=True)
     assert np.array_equal(d["order"], L.order)
     assert np.array_equal(d["node_begin"], L.node_begin) and np.array_equal(d["node_end"], L.node_end)
     assert np.array_equal(d["node_level"], L.node_level)
@@ -348,3 +348,46 @@ def test_device_tree_rejects_bad_arguments():
     assert g.sizes()["n_group"] >= 1  # an argument error does not discard the tree
     g.close()
     ctx.close()
+
+
+def test_device_tree_rejects_non_finite_input():
+    ctx = sf.Context(0)
+    m, x, _ = host.cluster(1000, "plummer", seed=4)
+    xb = x.copy()
+    xb[17, 1] = np.nan
+    with pytest.raises(sf.PthError, match="non-finite"):
+        sf.GpuTree(ctx, xb, m)
+    rs = np.full(1000, 0.01)
+    rs[3] = np.inf
+    with pytest.raises(sf.PthError, match="non-finite"):
+        sf.GpuTree(ctx, x, m, rs)
+    g = sf.GpuTree(ctx, x, m, np.full(1000, 0.01))
+    assert g.sizes()["n_group"] >= 1
+    g.close()
+    ctx.close()
+
+
+def test_resident_from_a_tree_of_another_context():
+    """The plan made during the build is taken over only under the same
+    parameters: a resident evaluation in a context with other tiles and
+    segments plans again and still equals the host-list evaluation."""
+    m, x, _ = host.cluster(12000, "plummer", kroupa=True, f_bin=0.3, seed=71)
+    rs = np.full(m.size, 1.5 * R_OUT)
+    L = host.build_lists(x, m, rs, theta=0.3)
+    kw = dict(eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    ca = sf.Context(0, **kw)
+    cb = sf.Context(0, segments=1, max_ipl=4, chunk_j=512, **kw)
+    g = sf.GpuTree(ca, x, m, rs)
+    outs = []
+    for ctx, src in ((cb, "tree"), (cb, "lists"), (ca, "tree"), (ca, "tree")):
+        r = sf.Resident.from_gtree(ctx, g) if src == "tree" else sf.Resident.from_lists(ctx, L)
+        r.run()
+        outs.append(r.results())
+        r.close()
+    g.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    for a, b in zip(outs[2], outs[3]):  # taken-over plan, then a plan made again
+        assert np.array_equal(a, b)
+    ca.close()
+    cb.close()
