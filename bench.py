@@ -251,6 +251,22 @@ def cpu_baseline(args_, r_out, target_s):
                        f"phantomquad {isa} -DRSQRT_NR_EPJ_X2, {nt} OpenMP threads")
 
 
+def cpu_baseline_neighbors(L):
+    """The neighbour lists on the host cores: the FP64 restatement of
+    SearchNeighborEpEpNoSimd (src/soft_force.hpp:11-34) over the same group
+    EPJ lists, OpenMP over groups, counting and writing passes (FDPS's own
+    short-range tree is not in the reference repository, so this is the
+    CPU path timed beside the device neighbour search)."""
+    from oracle import oracle_py as O
+    nt = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    t = time.time()
+    cnt, _, _ = O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj, L.epj_pos, L.epj_rs)
+    dt = time.time() - t
+    return dict(value=round(dt * 1e3, 1), unit="ms", cores=nt, kind="port",
+                sample=f"the full workload: {int(cnt.sum())} entries from {L.id_epj.size} EPJ list entries, "
+                       f"one run, {nt} OpenMP threads")
+
+
 def accuracy(res_acc, res_pot, res_nnb, L, r_out, frac=0.05, seed=0):
     """Error vs the FP64 oracle on the same lists, on a sample of walks."""
     from oracle import oracle_py as O
@@ -476,6 +492,8 @@ def main():
     if not a.no_cpu:
         from oracle import oracle_py as O
         out["cpu_baseline"] = cpu_baseline(O.lists_args(L), r_out, a.cpu_seconds)
+        if "device_tree_step" in out:
+            out["device_tree_step"]["neighbor_search"]["cpu_baseline"] = cpu_baseline_neighbors(L)
     print(json.dumps(out), flush=True)
 
 
