@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
     float *__restrict__ staged_e = a.staged_e;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     __shared__ double red[4][7];
-    __shared__ int cnt[2][4];
+    __shared__ int cnt[4][4];
     __shared__ long long co_key[CO ? PTH_CO_HASH : 1];
     __shared__ int co_idx[CO ? PTH_CO_HASH : 1];
     __shared__ int co_self[CO ? PTH_CO_MAX_NI : 1];
@@ -201,86 +201,106 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
             a.staged_il[wk.i_off + k] =
                 make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.f);
     }
-    // -- EPJ list: stable split into near (front) and far (back, reversed)
+    // -- EPJ list: stable split into near (front) and far (back, reversed);
+    //    two entries per thread and pass (entries t0 + tid and t0 + 256 + tid)
+    //    so that two gathers are in flight per thread between the barriers
     const long long base = a.es_off[w];
     const long long cap = a.es_off[w + 1] - base;
     const double hmax = fmax(h[0], fmax(h[1], h[2]));
     int n_near = 0, n_far = 0;
-    for (int t0 = 0; t0 < wk.n_e; t0 += 256) {
-        const int t = t0 + tid;
-        const bool valid = t < wk.n_e;
-        bool nearj = false;
-        float x = 0.f, y = 0.f, z = 0.f, m = 0.f, r2 = 0.f;
-        float xl = 0.f, yl = 0.f, zl = 0.f;
-        int jx = 0;
-        if (valid) {
-            jx = id_epj[(long long)wk.e_off + t];
-            const EpjDev e = epj[jx];
-            const double dx = e.x - c[0], dy = e.y - c[1], dz = e.z - c[2];
-            // near: inside the walk's box widened by max(rs_i, rs_j, r_out),
-            // with a margin that covers the FP32 rounding of the pair arithmetic
-            const double R = fmax(fmax(rsm, (double)e.rs), r_out);
-            const double dm = fmax(fabs(dx), fmax(fabs(dy), fabs(dz)));
-            const double Rm = R * 1.00001 + 1e-5 * (dm + hmax);
-            nearj = fabs(dx) <= h[0] + Rm && fabs(dy) <= h[1] + Rm && fabs(dz) <= h[2] + Rm;
-            x = (float)dx;
-            y = (float)dy;
-            z = (float)dz;
-            xl = (float)(dx - (double)x);
-            yl = (float)(dy - (double)y);
-            zl = (float)(dz - (double)z);
-            m = e.m;
-            const double rs = (double)e.rs;
-            r2 = (float)(rs * rs);
-        }
-        const unsigned long long bn = __ballot(valid && nearj);
-        const unsigned long long bf = __ballot(valid && !nearj);
-        const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        if (lane == 0) {
-            cnt[0][wave] = __popcll(bn);
-            cnt[1][wave] = __popcll(bf);
-        }
-        __syncthreads();
-        int on = n_near, of = n_far;
-        for (int q = 0; q < wave; q++) {
-            on += cnt[0][q];
-            of += cnt[1][q];
-        }
-        const int tn = cnt[0][0] + cnt[0][1] + cnt[0][2] + cnt[0][3];
-        const int tf = cnt[1][0] + cnt[1][1] + cnt[1][2] + cnt[1][3];
-        if (valid) {
-            const long long slot = nearj ? base + on + __popcll(bn & below)
-                                         : base + cap - 1 - (of + __popcll(bf & below));
-            put_epj(staged_e, slot, x, y, z, m, r2);
-            if constexpr (CO) {
-                if (nearj) {
-                    const EpjCoDev co = a.epj_co[jx];
-                    float *blk = a.staged_ec + (slot >> 3) * PTH_CO_EXTRA;
-                    const int l = (int)(slot & 7);
-                    blk[l] = co.rin;
-                    blk[8 + l] = co.rout;
-                    blk[16 + l] = xl;
-                    blk[24 + l] = yl;
-                    blk[32 + l] = zl;
-                    // is this the EPJ of one of the walk's i?
-                    unsigned hb = co_hash(co.id);
-                    for (;;) {
-                        const int k = co_idx[hb];
-                        if (k < 0) break;
-                        if (co_key[hb] == co.id) {
-                            co_self[k] = (int)slot;
-                            co_rin[k] = co.rin;
-                            co_rout[k] = co.rout;
-                            co_m[k] = m;
-                            break;
-                        }
-                        hb = (hb + 1) & (PTH_CO_HASH - 1);
+    struct Ent {
+        bool valid, nearj;
+        float x, y, z, m, r2, xl, yl, zl;
+        int jx;
+    };
+    auto load = [&](int t) {
+        Ent q{t < wk.n_e, false, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+        if (q.valid) q.jx = id_epj[(long long)wk.e_off + t];
+        return q;
+    };
+    auto classify = [&](Ent &q) {
+        if (!q.valid) return;
+        const EpjDev e = epj[q.jx];
+        const double dx = e.x - c[0], dy = e.y - c[1], dz = e.z - c[2];
+        // near: inside the walk's box widened by max(rs_i, rs_j, r_out),
+        // with a margin that covers the FP32 rounding of the pair arithmetic
+        const double R = fmax(fmax(rsm, (double)e.rs), r_out);
+        const double dm = fmax(fabs(dx), fmax(fabs(dy), fabs(dz)));
+        const double Rm = R * 1.00001 + 1e-5 * (dm + hmax);
+        q.nearj = fabs(dx) <= h[0] + Rm && fabs(dy) <= h[1] + Rm && fabs(dz) <= h[2] + Rm;
+        q.x = (float)dx;
+        q.y = (float)dy;
+        q.z = (float)dz;
+        q.xl = (float)(dx - (double)q.x);
+        q.yl = (float)(dy - (double)q.y);
+        q.zl = (float)(dz - (double)q.z);
+        q.m = e.m;
+        const double rs = (double)e.rs;
+        q.r2 = (float)(rs * rs);
+    };
+    auto place = [&](const Ent &q, long long slot) {
+        put_epj(staged_e, slot, q.x, q.y, q.z, q.m, q.r2);
+        if constexpr (CO) {
+            if (q.nearj) {
+                const EpjCoDev co = a.epj_co[q.jx];
+                float *blk = a.staged_ec + (slot >> 3) * PTH_CO_EXTRA;
+                const int l = (int)(slot & 7);
+                blk[l] = co.rin;
+                blk[8 + l] = co.rout;
+                blk[16 + l] = q.xl;
+                blk[24 + l] = q.yl;
+                blk[32 + l] = q.zl;
+                // is this the EPJ of one of the walk's i?
+                unsigned hb = co_hash(co.id);
+                for (;;) {
+                    const int k = co_idx[hb];
+                    if (k < 0) break;
+                    if (co_key[hb] == co.id) {
+                        co_self[k] = (int)slot;
+                        co_rin[k] = co.rin;
+                        co_rout[k] = co.rout;
+                        co_m[k] = q.m;
+                        break;
                     }
+                    hb = (hb + 1) & (PTH_CO_HASH - 1);
                 }
             }
         }
-        n_near += tn;
-        n_far += tf;
+    };
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int t0 = 0; t0 < wk.n_e; t0 += 512) {
+        Ent qa = load(t0 + tid), qb = load(t0 + 256 + tid);
+        classify(qa);
+        classify(qb);
+        const unsigned long long an = __ballot(qa.valid && qa.nearj), af = __ballot(qa.valid && !qa.nearj);
+        const unsigned long long bn = __ballot(qb.valid && qb.nearj), bf = __ballot(qb.valid && !qb.nearj);
+        if (lane == 0) {
+            cnt[0][wave] = __popcll(an);
+            cnt[1][wave] = __popcll(af);
+            cnt[2][wave] = __popcll(bn);
+            cnt[3][wave] = __popcll(bf);
+        }
+        __syncthreads();
+        int oan = n_near, oaf = n_far, obn = 0, obf = 0;
+        for (int q = 0; q < wave; q++) {
+            oan += cnt[0][q];
+            oaf += cnt[1][q];
+            obn += cnt[2][q];
+            obf += cnt[3][q];
+        }
+        const int tan = cnt[0][0] + cnt[0][1] + cnt[0][2] + cnt[0][3];
+        const int taf = cnt[1][0] + cnt[1][1] + cnt[1][2] + cnt[1][3];
+        const int tbn = cnt[2][0] + cnt[2][1] + cnt[2][2] + cnt[2][3];
+        const int tbf = cnt[3][0] + cnt[3][1] + cnt[3][2] + cnt[3][3];
+        // the second half's entries follow all of the first half's
+        obn += n_near + tan;
+        obf += n_far + taf;
+        if (qa.valid)
+            place(qa, qa.nearj ? base + oan + __popcll(an & below) : base + cap - 1 - (oaf + __popcll(af & below)));
+        if (qb.valid)
+            place(qb, qb.nearj ? base + obn + __popcll(bn & below) : base + cap - 1 - (obf + __popcll(bf & below)));
+        n_near += tan + tbn;
+        n_far += taf + tbf;
         __syncthreads();
     }
     // pad the gap between the two regions (far, massless)
@@ -300,19 +320,26 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         const int n_miss = __syncthreads_count(miss);
         if (tid == 0) a.walk_err[w] = n_miss;
     }
-    // -- SPJ list: 12-float records relative to the centre, two per thread
-    //    and pass so that two gathers are in flight
+    // -- SPJ list: 12-float records relative to the centre, four per thread
+    //    and pass so that four gathers are in flight
     float4 *out_s = reinterpret_cast<float4 *>(a.staged_s) + a.ss_off[w] * 3;
     const int *ids = a.id_spj + wk.s_off;
     const SpjDev *__restrict__ spj = a.spj;
-    for (int k0 = 0; k0 < wk.n_s; k0 += 512) {
-        const int ka = k0 + tid, kb = ka + 256;
-        const bool va = ka < wk.n_s, vb = kb < wk.n_s;
-        SpjDev pa, pb;
-        if (va) pa = spj[ids[ka]];
-        if (vb) pb = spj[ids[kb]];
-        if (va) put_spj(out_s + 3 * ka, pa, c, a.eps2);
-        if (vb) put_spj(out_s + 3 * kb, pb, c, a.eps2);
+    constexpr int U = 4;
+    for (int k0 = 0; k0 < wk.n_s; k0 += 256 * U) {
+        int id[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + 256 * u + tid;
+            id[u] = k < wk.n_s ? ids[k] : -1;
+        }
+        SpjDev p[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (id[u] >= 0) p[u] = spj[id[u]];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (id[u] >= 0) put_spj(out_s + 3 * (k0 + 256 * u + tid), p[u], c, a.eps2);
     }
 }
 

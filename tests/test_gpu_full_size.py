@@ -1,10 +1,16 @@
-"""The precision workload at its full size (Plummer N = 1,048,576, 50%
-primordial binaries, Kroupa masses, theta 0.3, r_out 0.01, r_search 0.015),
+"""The precision workload and the two largest workloads at their full sizes,
 checked through properties that do not need an all-pairs reference: the
 device tree equals the host tree entry for entry; the pipelined, the
 unsegmented and the device-fed evaluations agree bit for bit; sampled walks
 agree with the FP64 oracle on the same lists; the neighbour lists equal the
-FP64 restatement over all walks."""
+FP64 restatement over all walks.
+
+Workloads (theta 0.3, n_leaf 20, n_group 512, r_search = 1.5 r_out):
+precision, Plummer N = 1,048,576 with 50% primordial binaries, Kroupa
+masses, r_out 0.01; batching, King W0 = 7 N = 4,194,304, Kroupa,
+mass-segregated, r_out 0.01 (132M EPJ + 230M SPJ list entries); large,
+Plummer N = 8,388,608 with 100% binaries, r_out 0.003 (251M EPJ + 585M SPJ
+list entries)."""
 import numpy as np
 import pytest
 
@@ -14,16 +20,23 @@ from tests._common import rel_err
 
 pytestmark = pytest.mark.gpu
 
-R_OUT_C2 = 0.01
-RS_C2 = 0.015
+WORKLOADS = {
+    "precision": (dict(n=1 << 20, model="plummer", f_bin=0.5, seed=2), 0.01),
+    "batching": (dict(n=1 << 22, model="king", f_bin=0.0, seed=3, segregate=True), 0.01),
+    "large": (dict(n=1 << 23, model="plummer", f_bin=1.0, seed=5), 0.003),
+}
 
 
-@pytest.fixture(scope="module")
-def c2():
-    m, x, _ = host.cluster(1 << 20, "plummer", kroupa=True, f_bin=0.5, seed=2)
-    L = host.build_lists(x, m, RS_C2, theta=0.3, n_leaf_limit=20, n_group_limit=512)
-    print(f"[full size] {L.n_walk} walks, {L.id_epj.size} EPJ + {L.id_spj.size} SPJ entries", flush=True)
-    return m, x, L
+@pytest.fixture(scope="module", params=list(WORKLOADS))
+def workload(request):
+    kw, r_out = WORKLOADS[request.param]
+    kw = dict(kw)
+    m, x, _ = host.cluster(kw.pop("n"), kw.pop("model"), kroupa=True, **kw)
+    rs = 1.5 * r_out
+    L = host.build_lists(x, m, rs, theta=0.3, n_leaf_limit=20, n_group_limit=512)
+    print(f"[full size {request.param}] {L.n_walk} walks, {L.id_epj.size} EPJ + {L.id_spj.size} SPJ entries",
+          flush=True)
+    return m, x, L, r_out, rs
 
 
 def _run(ctx, res):
@@ -33,10 +46,10 @@ def _run(ctx, res):
     return out
 
 
-def test_full_size_device_tree_equals_host_tree(c2):
-    m, x, L = c2
+def test_full_size_device_tree_equals_host_tree(workload):
+    m, x, L, r_out, rs = workload
     ctx = sf.Context(0)
-    g = sf.GpuTree(ctx, x, m, np.full(m.size, RS_C2), theta=0.3)
+    g = sf.GpuTree(ctx, x, m, np.full(m.size, rs), theta=0.3)
     d = g.download()
     g.close()
     ctx.close()
@@ -49,14 +62,14 @@ def test_full_size_device_tree_equals_host_tree(c2):
     assert np.array_equal(d["id_epj"], L.id_epj) and np.array_equal(d["id_spj"], L.id_spj)
 
 
-def test_full_size_evaluations_agree(c2):
-    m, x, L = c2
-    kw = dict(eps2=0.0, rcut2=R_OUT_C2 ** 2, G=1.0)
+def test_full_size_evaluations_agree(workload):
+    m, x, L, r_out, rs = workload
+    kw = dict(eps2=0.0, rcut2=r_out ** 2, G=1.0)
     ctx = sf.Context(0, **kw)  # pipelined over 8 segments on 3 streams
     ref = _run(ctx, sf.Resident.from_lists(ctx, L))
     ctx1 = sf.Context(0, segments=1, **kw)
     one = _run(ctx1, sf.Resident.from_lists(ctx1, L))
-    g = sf.GpuTree(ctx, x, m, np.full(m.size, RS_C2), theta=0.3)
+    g = sf.GpuTree(ctx, x, m, np.full(m.size, rs), theta=0.3)
     dev = _run(ctx, sf.Resident.from_gtree(ctx, g))
     g.close()
     for a, b, c in zip(ref, one, dev):
@@ -75,7 +88,7 @@ def test_full_size_evaluations_agree(c2):
     racc, rpot, rnnb = O.soft_force_walks(L.n_epi[walks], L.n_epj[walks], L.n_spj[walks], L.epj_pos[rows],
                                           L.epj_rs[rows], np.concatenate(ide), np.concatenate(ids), L.epj_pos,
                                           L.epj_mass, L.epj_rs, L.spj_pos, L.spj_mass, L.spj_quad, eps2=0.0,
-                                          rout2=R_OUT_C2 ** 2)
+                                          rout2=r_out ** 2)
     # the evaluation's i are the walks' EPI in walk order, which here is the tree order
     ea, ep = rel_err(ref[0][rows], racc), rel_err(ref[1][rows], rpot)
     print(f"[full size] {rows.size} sampled i: acc median {np.median(ea):.2e} p99 {np.percentile(ea, 99):.2e}",
@@ -86,11 +99,11 @@ def test_full_size_evaluations_agree(c2):
     ctx1.close()
 
 
-def test_full_size_neighbour_lists(c2):
-    m, x, L = c2
+def test_full_size_neighbour_lists(workload):
+    m, x, L, r_out, rs = workload
     ocnt, ooff, ongb = O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj, L.epj_pos, L.epj_rs)
     ctx = sf.Context(0)
-    g = sf.GpuTree(ctx, x, m, np.full(m.size, RS_C2), theta=0.3)
+    g = sf.GpuTree(ctx, x, m, np.full(m.size, rs), theta=0.3)
     cnt, off, ngb, ms = g.neighbors()
     g.close()
     ctx.close()
