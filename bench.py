@@ -290,7 +290,7 @@ def accuracy(res_acc, res_pot, res_nnb, L, r_out, frac=0.05, seed=0):
                 pot_rel_p99=float(np.percentile(ep, 99)), nnb_mismatch=int((res_nnb[rows] != rnnb).sum()))
 
 
-def changeover_mode(L, r_out, tiles, steps=10, frac=0.02, seed=0):
+def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0):
     """The same lists in changeover mode (the (1 - K) weight and the
     pot_soft / pot_tot rules fused into the EPJ loop): ms per evaluation,
     Gint/s, and the error against the FP64 changeover oracle on a sample of
@@ -301,7 +301,8 @@ def changeover_mode(L, r_out, tiles, steps=10, frac=0.02, seed=0):
     rin, rout = np.full(L.epj_mass.size, 0.1 * r_out), np.full(L.epj_mass.size, r_out)
     ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, changeover=1, max_ipl=tiles[0], chunk_j=tiles[1])
     res = sf.Resident.from_lists(ctx, L, rin, rout)
-    res.run()
+    for _ in range(max(warmup, 1)):
+        res.run()
     res.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -459,7 +460,8 @@ def main():
             out["error_vs_fp64_direct_sum"] = tree_error(acc, pot, L, r_out, a.direct_sample)
     res.close()
     if not a.no_changeover and a.exec == "fused":
-        out["changeover_mode"] = changeover_mode(L, r_out, (ipl_used, cj_used))
+        # same warm-up and step count as the timed evaluation above
+        out["changeover_mode"] = changeover_mode(L, r_out, (ipl_used, cj_used), steps=a.steps, warmup=a.warmup)
     if not a.no_device_tree and a.exec == "fused":
         # same tiles and segments as the timed evaluation, so the forces can
         # be compared bit for bit

@@ -446,9 +446,10 @@ __device__ __forceinline__ void co_pair(float r2, float mj, float rin_i, float r
 // as epj_blocks), and is tested for being a neighbour pair: r^2 + eps^2 <
 // ngb2 max(rs_i, rs_j)^2 + eps^2 with ngb2 = 0.99^2 (getRSearch() = 0.99
 // r_search of src/soft_ptcl.hpp:97-101,349-353 on both sides), self excluded
-// (by staged slot).  When some lane of the wave has a neighbour pair in a
-// block of 8 j, the block is revisited and, on the neighbour pairs only, the
-// linear-cutoff terms are replaced by co_pair's.
+// (by staged slot).  The j of a block of 8 for which some lane of the wave
+// has a neighbour pair are revisited (in order) and, on the neighbour pairs
+// only, the linear-cutoff terms are replaced by co_pair's; the other j would
+// only add zeros there.
 // The self pair's -m_i/r_out is removed by the caller (src/hard.hpp:1669-1673
 // adds the same term back on the CPU).
 template <int IPL, int NNB_LE>
@@ -468,11 +469,12 @@ __device__ __forceinline__ void epj_near_co(cfp se, cfp sec, int b_begin, int b_
         for (int q = 0; q < IPL; q++) { ax[q] = ay[q] = az[q] = p[q] = pt[q] = 0.f; nb[q] = 0; }
         for (int b = b0; b < b1; b++) {
             const cfp B = se + (size_t)b * 40;
-            bool hit = false;
+            unsigned hits = 0;  // bit k: some lane has a neighbour pair with j k
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const float xj = B[k], yj = B[8 + k], zj = B[16 + k], mj = B[24 + k], rj = B[32 + k];
                 const int sj = b * 8 + k;
+                bool hit = false;
 #pragma unroll
                 for (int q = 0; q < IPL; q++) {
                     const float dx = xj - xi[q];
@@ -492,12 +494,13 @@ __device__ __forceinline__ void epj_near_co(cfp se, cfp sec, int b_begin, int b_
                     p[q] -= mri;
                     hit = ((r2 < __builtin_fmaf(ngb2, rs2, K.eps2)) && sj != self[q]) ? true : hit;
                 }
+                if (__ballot(hit)) hits |= 1u << k;
             }
-            if (__ballot(hit)) {
-                // some lane has a neighbour pair in this block: revisit it
+            if (hits) {
+                // revisit the j with neighbour pairs
                 const cfp E = sec + (size_t)b * PTH_CO_EXTRA;
-#pragma unroll 1
-                for (int k = 0; k < 8; k++) {
+                for (unsigned mk = (unsigned)uniform((int)hits); mk; mk &= mk - 1) {
+                    const int k = uniform(__builtin_ctz(mk));
                     const float xj = B[k], yj = B[8 + k], zj = B[16 + k], mj = B[24 + k], rj = B[32 + k];
                     const float rinj = E[k], routj = E[8 + k];
                     const float xjl = E[16 + k], yjl = E[24 + k], zjl = E[32 + k];
