@@ -122,8 +122,9 @@ struct StageArgs {
 // r_in[8], r_out[8], xlo[8], ylo[8], zlo[8]; x - c = x_hi + x_lo as a
 // double-float), every i the low-order parts of its coordinates, and each i
 // finds its own EPJ entry
-// (same id) among the walk's near entries through an LDS hash table of the
-// walk's i ids; the i's staged slot of that entry and its radii go to
+// (same id) among the walk's near entries through an LDS hash table that
+// maps the hash of an id to the i's index (the ids themselves are compared
+// in place, in global memory); the i's staged slot of that entry and its radii go to
 // staged_ic (the force kernel drops the self pair by slot and takes the i's
 // changeover radii from there).  walk_err counts i with no such entry.
 template <bool CO>
@@ -140,7 +141,6 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     __shared__ double red[4][7];
     __shared__ int cnt[4][4];
-    __shared__ long long co_key[CO ? PTH_CO_HASH : 1];
     __shared__ int co_idx[CO ? PTH_CO_HASH : 1];
     __shared__ int co_self[CO ? PTH_CO_MAX_NI : 1];
     __shared__ float co_rin[CO ? PTH_CO_MAX_NI : 1], co_rout[CO ? PTH_CO_MAX_NI : 1], co_m[CO ? PTH_CO_MAX_NI : 1];
@@ -153,7 +153,6 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
             unsigned h = co_hash(id);
             // at most PTH_CO_MAX_NI keys in PTH_CO_HASH buckets: a free one exists
             while (atomicCAS(&co_idx[h], -1, k) != -1) h = (h + 1) & (PTH_CO_HASH - 1);
-            co_key[h] = id;
             co_self[k] = -1;
             co_rin[k] = 0.f;
             co_rout[k] = 0.f;
@@ -255,7 +254,8 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
                 for (;;) {
                     const int k = co_idx[hb];
                     if (k < 0) break;
-                    if (co_key[hb] == co.id) {
+                    // (the walk's ids are compared in place, not copied to LDS)
+                    if (a.epi_id[wk.i_off + k] == co.id) {
                         co_self[k] = (int)slot;
                         co_rin[k] = co.rin;
                         co_rout[k] = co.rout;
