@@ -3,7 +3,9 @@ checked through properties that do not need an all-pairs reference: the
 device tree equals the host tree entry for entry; the pipelined, the
 unsegmented and the device-fed evaluations agree bit for bit; sampled walks
 agree with the FP64 oracle on the same lists; the neighbour lists equal the
-FP64 restatement over all walks.
+FP64 restatement over all walks; in changeover mode the pipelined and
+unsegmented evaluations agree bit for bit and sampled walks agree with the
+FP64 changeover oracle.
 
 Workloads (theta 0.3, n_leaf 20, n_group 512, r_search = 1.5 r_out):
 precision, Plummer N = 1,048,576 with 50% primordial binaries, Kroupa
@@ -115,3 +117,40 @@ def test_full_size_neighbour_lists(workload):
     o = np.argsort(L.order[rowp], kind="stable")
     assert np.array_equal(ngb, L.order[ongb[o]])
     print(f"[full size] neighbour lists: {ngb.size} entries, {ms:.2f} ms", flush=True)
+
+
+def test_full_size_changeover_mode(workload):
+    """Changeover mode (r_in = 0.1 r_out) on the full lists: the pipelined
+    and the unsegmented evaluations agree bit for bit, and sampled walks
+    agree with the FP64 changeover oracle (acc and pot_tot everywhere,
+    pot_soft away from pairs within FP32 rounding of the neighbour radius,
+    where it jumps)."""
+    m, x, L, r_out, rs = workload
+    n = m.size
+    rin, rout = np.full(n, 0.1 * r_out), np.full(n, r_out)
+    out = []
+    for seg in (8, 1):
+        ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, changeover=1, segments=seg)
+        out.append(_run(ctx, sf.Resident.from_lists(ctx, L, rin, rout)))
+        ctx.close()
+    assert all(np.array_equal(a, b) for a, b in zip(*out))
+    acc, ps, nnb, pt = out[0]
+    rng = np.random.default_rng(9)
+    walks = np.sort(rng.choice(L.n_walk, 48, replace=False))
+    eo, so = L.walk_offsets()
+    ioff = np.concatenate([[0], np.cumsum(L.n_epi)])
+    idx = np.concatenate([np.arange(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w]) for w in walks])
+    rows = np.concatenate([np.arange(ioff[w], ioff[w + 1]) for w in walks])
+    racc, rps, rpt, _, amb = O.soft_force_walks_co(
+        L.n_epi[walks], L.n_epj[walks], L.n_spj[walks], L.epj_pos[idx], L.epj_rs[idx], idx,
+        np.concatenate([L.id_epj[eo[w]:eo[w + 1]] for w in walks]),
+        np.concatenate([L.id_spj[so[w]:so[w + 1]] for w in walks]), L.epj_pos, L.epj_mass, L.epj_rs,
+        np.arange(n), rin, rout, L.spj_pos, L.spj_mass, L.spj_quad, eps2=0.0, rout2=r_out ** 2, amb_rel=1e-4)
+    ea, et = rel_err(acc[rows], racc), rel_err(pt[rows], rpt)
+    keep = amb == 0
+    es = rel_err(ps[rows][keep], rps[keep])
+    print(f"[full size] changeover, {rows.size} sampled i: acc median {np.median(ea):.2e} p99 "
+          f"{np.percentile(ea, 99):.2e}; pot_tot median {np.median(et):.2e}; pot_soft median {np.median(es):.2e} "
+          f"({int((~keep).sum())} i at the neighbour radius left out)", flush=True)
+    for e in (ea, et, es):
+        assert np.median(e) <= 1e-6 and np.percentile(e, 99) <= 1e-4
