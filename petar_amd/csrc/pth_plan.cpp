@@ -18,9 +18,9 @@
 #include "pth_plan.h"
 
 #include <algorithm>
+#include <omp.h>
 #include <array>
 #include <climits>
-#include <thread>
 #include <cstdint>
 #include <cstring>
 
@@ -63,17 +63,15 @@ struct Scratch {
 // indices each (fewer, shorter loops stay on the calling thread).
 template <class F> void parallel_for(int n, int grain, F f)
 {
-    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    const int t = std::min({hw, 16, std::max(1, n / std::max(grain, 1))});
+    // OpenMP's thread pool: a plan makes several of these calls, and
+    // starting threads for each one cost about a millisecond apiece
+    const int t = std::min({omp_get_max_threads(), 16, std::max(1, n / std::max(grain, 1))});
     if (t <= 1) {
         f(0, n);
         return;
     }
-    std::vector<std::thread> th;
-    th.reserve(t - 1);
-    for (int k = 1; k < t; k++) th.emplace_back(f, (int)((long long)n * k / t), (int)((long long)n * (k + 1) / t));
-    f(0, (int)((long long)n / t));
-    for (std::thread &x : th) x.join();
+#pragma omp parallel for num_threads(t) schedule(static, 1)
+    for (int k = 0; k < t; k++) f((int)((long long)n * k / t), (int)((long long)n * (k + 1) / t));
 }
 
 constexpr int PLAN_WALKS_PER_THREAD = 512;
@@ -128,7 +126,12 @@ void tile_walk(int n, long long n_e, long long n_s, int max_ipl, std::vector<Til
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
                int chunk_j, int max_ipl, Plan &P, int n_seg)
 {
+    // the item array keeps its size from the previous plan (every slot is
+    // written below, so growing it is the only initialisation it needs)
+    std::vector<Item> kept;
+    kept.swap(P.items);
     P.reset();
+    P.items.swap(kept);
     P.walks.resize(n_walk);
     P.es_off.assign(n_walk + 1, 0);
     P.ss_off.assign(n_walk + 1, 0);
@@ -215,6 +218,9 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     if (walk_part_begin[n_walk] > INT_MAX) return false;
     std::vector<Tmp> &seq = S.seq;
     seq.resize(walk_item_begin[n_walk]);
+    // wcost[w + 1]: cost of walk w's items, summed into prefix form below
+    std::vector<double> &wcost = S.wcost;
+    wcost.assign(n_walk + 1, 0.0);
     P.ritems.resize(walk_ritem_begin[n_walk]);
     P.n_part = (int)walk_part_begin[n_walk];
     parallel_for(n_walk, PLAN_WALKS_PER_THREAD, [&](int wb, int we) {
@@ -228,6 +234,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
             size_t o = walk_item_begin[w];
             int r_o = walk_ritem_begin[w];
             int part = (int)walk_part_begin[w];
+            double wc = 0.0;
             for (const Tile &t : tiles) {
                 const int part_first = part;
                 for (long long k = 0; k < nch; k++) {
@@ -248,6 +255,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
                                      (epj_unit_cost(t.ipl) * 8.0 * (it.e_blk_end - it.e_blk_begin) +
                                       spj_unit_cost(t.ipl) * (it.s_end - it.s_begin) + 50.0);
                     seq[o++] = Tmp{it, c};
+                    wc += c;
                 }
                 if (nch > 1) {
                     RItem r;
@@ -260,6 +268,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
                     P.ritems[r_o++] = r;
                 }
             }
+            wcost[w + 1] = wc;
         }
     });
     P.n_real_items = (int)seq.size();
@@ -268,13 +277,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     if (n_seg < 1) n_seg = 1;
     if (n_seg > MAX_SEGMENTS) n_seg = MAX_SEGMENTS;
     if (n_seg > n_walk) n_seg = std::max(n_walk, 1);
-    std::vector<double> &wcost = S.wcost;
-    wcost.assign(n_walk + 1, 0.0);
-    for (int w = 0; w < n_walk; w++) {
-        double c = 0;
-        for (size_t k = walk_item_begin[w]; k < walk_item_begin[w + 1]; k++) c += seq[k].cost;
-        wcost[w + 1] = wcost[w] + c;
-    }
+    for (int w = 0; w < n_walk; w++) wcost[w + 1] += wcost[w];
     // the first segment gets half the cost of the others: its staging is the
     // one part of a pipelined evaluation that nothing overlaps
     std::vector<int> cut(n_seg + 1, 0);
@@ -354,7 +357,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     dummy.walk = -1;
     dummy.i_begin = dummy.n_i_ipl = dummy.e_blk_begin = dummy.e_blk_end = dummy.s_begin = dummy.s_end = 0;
     dummy.part = -1;
-    P.items.assign(cursor, dummy);
+    P.items.resize(cursor);
     parallel_for(n_seg, 1, [&](int gb, int ge) {
         std::vector<uint32_t> cnt(NBUCKET + 1), sorted;
         for (int g = gb; g < ge; g++)
@@ -370,9 +373,13 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
                     for (int q = 0; q < NBUCKET; q++) cnt[q + 1] += cnt[q];
                     sorted.resize(b - a);
                     for (size_t k = a; k < b; k++) sorted[cnt[NBUCKET - 1 - bucket[R.sub[k]]]++] = R.sub[k];
-                    for (size_t k = 0; k < sorted.size(); k++) {
+                    // the run's slots: its items, then empty items up to the
+                    // longest run's length
+                    const size_t n_slot = (size_t)R.max_blocks * ITEMS_PER_BLOCK;
+                    for (size_t k = 0; k < n_slot; k++) {
                         const size_t r = k / ITEMS_PER_BLOCK, q = k % ITEMS_PER_BLOCK;
-                        P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] = seq[sorted[k]].it;
+                        P.items[base + (r * XCDS + xx) * ITEMS_PER_BLOCK + q] =
+                            k < sorted.size() ? seq[sorted[k]].it : dummy;
                     }
                 }
             }
