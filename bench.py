@@ -87,6 +87,7 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
     acc, pot, nnb = r.results()
     same = bool(np.array_equal(acc, ref[0]) and np.array_equal(pot, ref[1]) and np.array_equal(nnb, ref[2]))
     res = {}
+    passes = []
     for label, inputs in (("device_inputs", (tx, tm, tr)), ("host_inputs", (x, m, rs))):
         phases, walls = [], []
         for k in range(reps + 1):
@@ -103,9 +104,10 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
                 tb, tw, to = g.timing()
                 phases.append((tb, tw, to, (t2 - t1) * 1e3, (t3 - t2) * 1e3))
                 walls.append((t3 - t0) * 1e3)
+                passes.append(g.walk_passes())
         ph = np.median(np.array(phases), axis=0)
         res[label] = dict(step_ms=round(float(np.median(walls)), 3), tree_build_ms=round(float(ph[0]), 3),
-                          walk_count_ms=round(float(ph[1]), 3), write_order_plan_ms=round(float(ph[2]), 3),
+                          walk_first_pass_ms=round(float(ph[1]), 3), write_order_plan_ms=round(float(ph[2]), 3),
                           refill_ms=round(float(ph[3]), 3), force_ms=round(float(ph[4]), 3))
     g.update(tx, tm, tr)
     ngb_ms = []
@@ -120,7 +122,10 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
     same = same and bool(np.array_equal(acc2, ref[0]) and np.array_equal(pot2, ref[1]) and np.array_equal(nnb2, ref[2]))
     r.close()
     g.close()
-    out = dict(res, forces_bit_identical_to_host_lists=same, reps=reps)
+    # rebuilds of the same particles write their lists in one walk pass
+    # (regions sized from the previous build's list lengths)
+    out = dict(res, forces_bit_identical_to_host_lists=same, reps=reps,
+               walk_passes_per_rebuild=sorted(set(passes)))
     log(f"[bench] device tree step: {out}")
     return out
 

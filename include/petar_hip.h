@@ -125,6 +125,9 @@ int pth_create(pth_ctx **ctx, int rank, const pth_params *params);
 int pth_set_params(pth_ctx *ctx, const pth_params *params);
 /* Text of the last pth_create failure of the calling thread. */
 const char *pth_create_error(void);
+/* Frees the context.  Its resident evaluations, trees and servers use its
+ * device and stream: free them first (pth_resident_free, pth_gtree_free,
+ * pth_server_free). */
 void pth_destroy(pth_ctx *ctx);
 const char *pth_last_error(const pth_ctx *ctx);
 /* Device index the context runs on, or -1. */
@@ -294,11 +297,19 @@ int pth_gtree_download(const pth_gtree *tree, long long *order, long long *node_
                        int *node_level, double *node_mass, double *node_com, double *node_quad,
                        long long *epi_begin, int *n_epi, int *n_epj, int *n_spj, int *id_epj, int *id_spj);
 /* Host wall time of the last build (ms): tree (upload, sort, nodes, moments,
- * groups), walk (counting pass, scan, list sizes) and the rest (writing
- * pass, canonical ordering when asked for, and meanwhile on the host the
- * plan of a resident evaluation under the context's parameters, which
- * pth_resident_from_gtree / pth_resident_refill then take over). */
+ * groups), walk (first walk pass, scan, list sizes) and the rest (writing
+ * pass or the move of the lists written by the first pass, canonical
+ * ordering when asked for, and meanwhile on the host the plan of a resident
+ * evaluation under the context's parameters, which pth_resident_from_gtree /
+ * pth_resident_refill then take over). */
 int pth_gtree_timing(const pth_gtree *tree, float *ms_build, float *ms_walk, float *ms_order);
+/* Walk passes of the last build (0 without groups): 1 when the lists were
+ * written in one pass into regions sized from the previous build's list
+ * lengths (a rebuild with the same number of groups, every list fitting),
+ * 2 when a counting or padded pass was followed by an exact writing pass.
+ * No counterpart in the reference (FDPS walks on the host); for callers that
+ * monitor rebuild costs. */
+int pth_gtree_stats(const pth_gtree *tree, int *walk_passes);
 void pth_gtree_free(pth_gtree *tree);
 /* Neighbour search on the tree (replaces the FDPS short-range tree search
  * PeTar runs with SearchNeighborEpEpNoSimd / SearchNeighborEpEpSimd,

@@ -29,6 +29,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <chrono>
+#include <climits>
 #include <cmath>
 
 #include "pth_internal.h"
@@ -385,9 +386,12 @@ struct WalkArgs {
     const double *tbox;
     double root_len, inv_theta2;
     int theta_pos;
-    long long *cnt_e, *cnt_s;                // counting pass
+    long long *cnt_e, *cnt_s;                // counting pass (and a padded writing pass)
     const long long *off_e, *off_s;          // writing pass
     int *id_epj, *id_spj;
+    // padded writing pass: group g writes into [off[g], off[g + 1]) at most,
+    // skipping what does not fit, and stores its exact counts
+    int padded;
 };
 
 __device__ __forceinline__ bool overlap6(const double *a, const double *b)
@@ -424,9 +428,14 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
         }
         long long ne = 0, ns = 0;
         int *oe = nullptr, *os = nullptr;
+        long long cap_e = LLONG_MAX, cap_s = LLONG_MAX;
         if (WRITE) {
             oe = a.id_epj + a.off_e[g];
             if (a.id_spj) os = a.id_spj + a.off_s[g];
+            if (a.padded) {
+                cap_e = a.off_e[g + 1] - a.off_e[g];
+                cap_s = a.off_s[g + 1] - a.off_s[g];
+            }
         }
         if (lane == 0) stk[0] = 0;
         int size = 1;
@@ -472,8 +481,11 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
             const long long pe = wave_scan(me, lane), pc = wave_scan(mc, lane), ps = wave_scan(ms, lane);
             const long long te = __shfl(pe, 63, 64), tc = __shfl(pc, 63, 64), ts = __shfl(ps, 63, 64);
             if (WRITE) {
-                for (long long t = 0; t < me; t++) oe[ne + pe - me + t] = (int)(nd.begin + t);
-                if (ms && os) os[ns + ps - 1] = c;
+                for (long long t = 0; t < me; t++) {
+                    const long long k = ne + pe - me + t;
+                    if (k < cap_e) oe[k] = (int)(nd.begin + t);
+                }
+                if (ms && os && ns + ps - 1 < cap_s) os[ns + ps - 1] = c;
             }
             ne += te;
             ns += ts;
@@ -482,11 +494,25 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
             size += (int)tc;
             __syncthreads();
         }
-        if (!WRITE && lane == 0) {
+        if ((!WRITE || a.padded) && lane == 0) {
             a.cnt_e[g] = ne;
             if (a.cnt_s) a.cnt_s[g] = ns;
         }
         __syncthreads();
+    }
+}
+
+// Group g's list, written by a padded walk at poff[g], moved to off[g]
+// (off[g + 1] - off[g] entries, checked on the host to fit the region).
+__global__ __launch_bounds__(256) void k_compact(long long ng, const long long *__restrict__ poff,
+                                                 const long long *__restrict__ off, const int *__restrict__ src,
+                                                 int *__restrict__ dst)
+{
+    for (long long g = blockIdx.x; g < ng; g += gridDim.x) {
+        const long long n = off[g + 1] - off[g];
+        const int *s = src + poff[g];
+        int *d = dst + off[g];
+        for (long long k = threadIdx.x; k < n; k += 256) d[k] = s[k];
     }
 }
 
@@ -689,6 +715,11 @@ struct pth_gtree {
     // neighbour lists (pth_gtree_neighbors): count per input index, offsets,
     // entries; tile boxes, candidate counts / offsets / lists, tile records
     DevBuf ngb_cnt, ngb_off, ngb, tiles, tbox, tcnt, toff, tl, tl_alt, tinfo;
+    // one-pass walks: list regions sized from the previous build's lengths
+    // (padded offsets, EPJ then SPJ), the lists written there
+    DevBuf pad_off, pad_e, pad_s;
+    std::vector<long long> pad_h;
+    int walk_passes = 0;  // of the last build: 1 (one padded pass) or 2 (count, then write)
     long long n_ngb = -1;  // total entries, -1 before the first search
     long long n_cand = 0;  // candidate pairs of the last search
     float ms_ngb = 0.f;
@@ -800,6 +831,7 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     pth_ctx *ctx = t->ctx;
     const int n_leaf_limit = t->n_leaf_limit, n_group_limit = t->n_group_limit;
     const double theta = t->theta;
+    const bool was_built = t->built;  // groups_h then holds the last build's list lengths
     t->built = false;
     t->pre_valid = false;
     t->n_ngb = -1;
@@ -952,7 +984,12 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     if ((e = hipStreamSynchronize(st))) return ctx->hip_fail(e, "tree build");
     const auto t1 = std::chrono::steady_clock::now();
     t->ms_build = std::chrono::duration<float, std::milli>(t1 - t0).count();
-    // 6. walk: count, scan, write
+    // 6. walk. With the previous build's list lengths per group at hand (same
+    //    number of groups: a rebuild of slowly moving particles), one writing
+    //    pass into regions of those lengths + 1/4 + 64 that also counts;
+    //    otherwise a counting pass first.  Either way the exact counts give
+    //    the list offsets; a region that turned out too small (checked on the
+    //    host) means a second, exact writing pass.
     WalkArgs wa{};
     wa.nodes = t->nodes.as<GNode>();
     wa.ncom = t->ncom.as<double>();
@@ -970,7 +1007,33 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     wa.cnt_e = ce.as<long long>();
     wa.cnt_s = cs.as<long long>();
     const unsigned wg = (unsigned)std::min<long long>(ng, 65536);
-    if (ng > 0) hipLaunchKernelGGL(k_walk<false>, dim3(wg), dim3(64), 0, st, wa);
+    const bool one_pass = ng > 0 && was_built && (long long)t->groups_h.size() == ng;
+    if (one_pass) {
+        std::vector<long long> &ph = t->pad_h;  // padded offsets: EPJ [0, ng], SPJ [ng + 1, 2 ng + 1]
+        ph.assign(2 * (ng + 1), 0);
+        long long pe = 0, ps = 0;
+        for (long long g = 0; g < ng; g++) {
+            const long long le = std::max(t->groups_h[g].z, 0), ls = std::max(t->groups_h[g].w, 0);
+            ph[g] = pe;
+            ph[ng + 1 + g] = ps;
+            pe += le + le / 4 + 64;
+            ps += ls + ls / 4 + 64;
+        }
+        ph[ng] = pe;
+        ph[2 * ng + 1] = ps;
+        if ((e = t->pad_off.ensure(8 * ph.size())) || (e = t->pad_e.ensure(4 * (size_t)pe)) ||
+            (e = t->pad_s.ensure(4 * (size_t)ps)) ||
+            (e = hipMemcpyAsync(t->pad_off.p, ph.data(), 8 * ph.size(), hipMemcpyHostToDevice, st)))
+            return ctx->hip_fail(e, "padded list allocation");
+        wa.off_e = t->pad_off.as<long long>();
+        wa.off_s = t->pad_off.as<long long>() + ng + 1;
+        wa.id_epj = t->pad_e.as<int>();
+        wa.id_spj = t->pad_s.as<int>();
+        wa.padded = 1;
+        hipLaunchKernelGGL(k_walk<true>, dim3(wg), dim3(64), 0, st, wa);
+    } else if (ng > 0) {
+        hipLaunchKernelGGL(k_walk<false>, dim3(wg), dim3(64), 0, st, wa);
+    }
     long long zero = 0;
     if ((e = hipMemcpyAsync(ce.as<long long>() + ng, &zero, 8, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(cs.as<long long>() + ng, &zero, 8, hipMemcpyHostToDevice, st)))
@@ -1000,6 +1063,12 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         return ctx->fail(PTH_ERR_CAPACITY, "pth_gtree_build: lists exceed the int32 range");
     t->n_le = tot[0];
     t->n_ls = tot[1];
+    // did every group's lists fit the padded regions?
+    bool rewalk = !one_pass;
+    for (long long g = 0; g < ng && !rewalk; g++)
+        rewalk = t->groups_h[g].z > t->pad_h[g + 1] - t->pad_h[g] ||
+                 t->groups_h[g].w > t->pad_h[ng + 2 + g] - t->pad_h[ng + 1 + g];
+    t->walk_passes = ng > 0 ? (one_pass && !rewalk ? 1 : 2) : 0;
     // every buffer of the write pass and the ordering sorts, before anything
     // is enqueued (a growing buffer's free would wait for the device)
     const bool order = t->canonical && ng > 0;
@@ -1023,16 +1092,26 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
             (e = cub_temp(t, std::max(tb_e, tb_s))))
             return ctx->hip_fail(e, "list ordering allocation");
     }
-    // write pass, then the ordering, all enqueued without waits
+    // write pass (or the move of the padded lists), then the ordering, all
+    // enqueued without waits
     wa.off_e = t->off_e.as<long long>();
     wa.off_s = t->off_s.as<long long>();
     wa.id_epj = t->id_epj.as<int>();
     wa.id_spj = t->id_spj.as<int>();
+    wa.padded = 0;
     const auto t2 = std::chrono::steady_clock::now();
     t->ms_walk = std::chrono::duration<float, std::milli>(t2 - t1).count();
     if (t->id_epj.cap < 4 * (size_t)tot[0] || t->id_spj.cap < 4 * (size_t)tot[1])
         return ctx->fail(PTH_ERR_STATE, "pth_gtree: list buffers smaller than the lists");
-    if (ng > 0) hipLaunchKernelGGL(k_walk<true>, dim3(wg), dim3(64), 0, st, wa);
+    if (ng > 0 && rewalk) {
+        hipLaunchKernelGGL(k_walk<true>, dim3(wg), dim3(64), 0, st, wa);
+    } else if (ng > 0) {
+        // every group fits its region (checked above): n <= region size
+        hipLaunchKernelGGL(k_compact, dim3(wg), dim3(256), 0, st, ng, t->pad_off.as<long long>(),
+                           t->off_e.as<long long>(), t->pad_e.as<int>(), t->id_epj.as<int>());
+        hipLaunchKernelGGL(k_compact, dim3(wg), dim3(256), 0, st, ng, t->pad_off.as<long long>() + ng + 1,
+                           t->off_s.as<long long>(), t->pad_s.as<int>(), t->id_spj.as<int>());
+    }
     // 7. canonical order: segmented sorts of the lists (one segment per group)
     if (order) {
         if (t->alt.cap < lbytes || t->id_epj.cap < lbytes || t->id_spj.cap < lbytes ||
@@ -1234,6 +1313,13 @@ int pth_gtree_timing(const pth_gtree *t, float *ms_build, float *ms_walk, float 
     return PTH_OK;
 }
 
+int pth_gtree_stats(const pth_gtree *t, int *walk_passes)
+{
+    if (!t) return PTH_ERR_ARG;
+    if (walk_passes) *walk_passes = t->walk_passes;
+    return PTH_OK;
+}
+
 int pth_resident_from_gtree(pth_ctx *c, pth_resident **out, const pth_gtree *t)
 {
     if (!c) return PTH_ERR_ARG;
@@ -1393,7 +1479,7 @@ void pth_gtree_free(pth_gtree *t)
     DevBuf *all[] = {&t->order, &t->key, &t->skey, &t->sidx, &t->spos, &t->smass, &t->srs, &t->nodes, &t->nmass,
                      &t->ncom, &t->nquad, &t->box_in, &t->box_out, &t->groups, &t->off_e, &t->off_s, &t->id_epj,
                      &t->id_spj, &t->tmp, &t->ngb_cnt, &t->ngb_off, &t->ngb, &t->tiles, &t->tbox, &t->tcnt,
-                     &t->toff, &t->tl, &t->tl_alt, &t->tinfo};
+                     &t->toff, &t->tl, &t->tl_alt, &t->tinfo, &t->pad_off, &t->pad_e, &t->pad_s};
     for (DevBuf *b : all) b->release();
     t->release_scratch();
     delete t;

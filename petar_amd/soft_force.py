@@ -22,7 +22,9 @@ no GPU is visible; there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -80,7 +82,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_er
             "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
             "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
             "pth_resident_replan", "pth_resident_autotune", "pth_gtree_build", "pth_gtree_sizes",
-            "pth_gtree_download", "pth_gtree_timing", "pth_gtree_free", "pth_resident_from_gtree",
+            "pth_gtree_download", "pth_gtree_timing", "pth_gtree_stats", "pth_gtree_free", "pth_resident_from_gtree",
             "pth_gtree_update", "pth_resident_refill", "pth_gtree_neighbors", "pth_gtree_neighbors_download",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
@@ -136,6 +138,7 @@ def lib():
     L.pth_gtree_sizes.argtypes = [vp, llp, llp, llp, llp]
     L.pth_gtree_download.argtypes = [vp, llp, llp, llp, ip, dp, dp, dp, llp, ip, ip, ip, ip, ip]
     L.pth_gtree_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.pth_gtree_stats.argtypes = [vp, C.POINTER(C.c_int)]
     L.pth_resident_from_gtree.argtypes = [vp, C.POINTER(vp), vp]
     L.pth_gtree_update.argtypes = [vp, C.c_longlong, vp, vp, vp, C.c_int]
     L.pth_resident_refill.argtypes = [vp, vp]
@@ -186,6 +189,20 @@ def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED,
     return p
 
 
+# live contexts, closed (with their trees, evaluations and servers) when the
+# interpreter exits, while the HIP runtime is still whole
+_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_contexts():
+    for ctx in list(_CONTEXTS):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One device context (pth_create); device = rank % visible devices."""
 
@@ -193,9 +210,16 @@ class Context:
         L = lib()
         self._h = C.c_void_p()
         self.params = default_params(**params)
+        # resident evaluations, trees and servers made on this context: they
+        # use its device and stream, so they are closed before it is
+        self._children = weakref.WeakSet()
         rc = L.pth_create(C.byref(self._h), rank, C.byref(self.params))
         if rc != PTH_OK:
             raise PthError(f"pth_create failed ({rc}): {L.pth_create_error().decode()}")
+        _CONTEXTS.add(self)
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     @property
     def handle(self):
@@ -211,6 +235,8 @@ class Context:
 
     def close(self):
         if self._h:
+            for child in list(self._children):
+                child.close()
             lib().pth_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -332,6 +358,7 @@ class Resident:
         L = lib()
         self.co = bool(ctx.params.changeover)
         self._h = C.c_void_p()
+        ctx._adopt(self)
         if _tree is not None:
             self.n_walk = int(_tree.sizes()["n_group"])
             self.ni = _tree.n
@@ -451,11 +478,12 @@ class GpuTree:
     def __init__(self, ctx: Context, pos, mass, r_search=None, theta=0.3, n_leaf_limit=20, n_group_limit=512,
                  canonical=True):
         self.ctx = ctx
+        self._h = C.c_void_p()
+        ctx._adopt(self)
         pos = _f64(pos).reshape(-1, 3)
         self.n = pos.shape[0]
         mass = _f64(mass)
         rs = None if r_search is None else _f64(np.broadcast_to(r_search, (self.n,)))
-        self._h = C.c_void_p()
         ctx.check(lib().pth_gtree_build(ctx.handle, C.byref(self._h), self.n, _ptr(pos, C.c_double),
                                         _ptr(mass, C.c_double), None if rs is None else _ptr(rs, C.c_double),
                                         n_leaf_limit, n_group_limit, theta, int(canonical)), "pth_gtree_build")
@@ -520,12 +548,19 @@ class GpuTree:
         return cnt, off, ngb[:tot.value], ms.value
 
     def timing(self):
-        """Host wall ms of the last build: (tree, walk counting pass and list
-        sizes, writing pass + canonical ordering with the host plan made
-        meanwhile)."""
+        """Host wall ms of the last build: (tree, first walk pass and list
+        sizes, writing pass or list move + canonical ordering with the host
+        plan made meanwhile)."""
         a, b, c = C.c_float(), C.c_float(), C.c_float()
         self.ctx.check(lib().pth_gtree_timing(self._h, C.byref(a), C.byref(b), C.byref(c)), "pth_gtree_timing")
         return a.value, b.value, c.value
+
+    def walk_passes(self):
+        """Walk passes of the last build: 1 when the lists were written in one
+        pass into regions sized from the previous build, 2 otherwise."""
+        n = C.c_int()
+        self.ctx.check(lib().pth_gtree_stats(self._h, C.byref(n)), "pth_gtree_stats")
+        return n.value
 
     def close(self):
         if self._h:
@@ -554,6 +589,7 @@ class Server:
         self.ctx = ctx
         self.mode = mode
         self._h = C.c_void_p()
+        ctx._adopt(self)
         self._ni = []
         self._nint = []
         ctx.check(lib().pth_server_create(ctx.handle, C.byref(self._h), mode, int(hbm_budget), n_streams),

@@ -197,6 +197,48 @@ def test_device_tree_update_and_refill():
     ctx.close()
 
 
+@pytest.mark.parametrize("canonical", [True, False])
+def test_device_tree_one_pass_rebuild(canonical):
+    """A rebuild with as many groups as the last build writes its lists in
+    one pass into regions sized from the last build's list lengths; lists
+    that outgrow their regions (here: r_search 0.5, which keeps the tree
+    and groups but puts most of the cluster on every EPJ list) are written
+    again by an exact second pass.  Either way the lists and the forces equal a fresh
+    build's."""
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    m, x, _ = host.cluster(20000, "plummer", kroupa=True, f_bin=0.3, seed=41)
+    rs = np.full(m.size, 1.5 * R_OUT)
+    g = sf.GpuTree(ctx, x, m, rs, canonical=canonical)
+    assert g.walk_passes() == 2
+    r = sf.Resident.from_gtree(ctx, g)
+    m2, x2, _ = host.cluster(12000, "plummer", kroupa=True, f_bin=0.3, seed=42)
+    rs4 = np.full(m.size, 50.0 * R_OUT)
+    steps = [(x, m, rs, 1), (x, m, rs, 1), (x, m, rs4, 2), (x, m, rs4, 1), (0.5 * x, m, rs, None),
+             (x2, m2, np.full(m2.size, 1.5 * R_OUT), None), (x, m, rs, None)]
+    for xs, ms, rss, passes in steps:
+        g.update(xs, ms, rss)
+        if passes is not None:
+            assert g.walk_passes() == passes
+        d = g.download()
+        r.refill(g)
+        r.run()
+        out = r.results()
+        f = sf.GpuTree(ctx, xs, ms, rss, canonical=canonical)
+        d2 = f.download()
+        rf = sf.Resident.from_gtree(ctx, f)
+        rf.run()
+        out2 = rf.results()
+        rf.close()
+        f.close()
+        for key in d:
+            assert np.array_equal(d[key], d2[key]), key
+        for a, b in zip(out, out2):
+            assert np.array_equal(a, b)
+    r.close()
+    g.close()
+    ctx.close()
+
+
 def test_device_tree_rejects_host_pointers_as_device_input():
     """on_device inputs are checked on the host before any launch: arrays
     that are not device allocations (here host arrays) are an error."""
@@ -391,3 +433,20 @@ def test_resident_from_a_tree_of_another_context():
         assert np.array_equal(a, b)
     ca.close()
     cb.close()
+
+
+def test_context_close_frees_its_objects_first():
+    """Closing a context first closes the trees, resident evaluations and
+    servers made on it (they use its device and stream), in whatever order
+    the caller or the garbage collector gets to them."""
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    m, x, _ = host.cluster(3000, "plummer", seed=5)
+    g = sf.GpuTree(ctx, x, m, np.full(m.size, 1.5 * R_OUT))
+    r = sf.Resident.from_gtree(ctx, g)
+    r.run()
+    srv = sf.Server(ctx)
+    ctx.close()
+    assert not g._h and not r._h and not srv._h
+    g.close()
+    r.close()
+    srv.close()
