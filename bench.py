@@ -317,6 +317,19 @@ def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0):
     acc, ps, _, pt = res.results()
     res.close()
     ctx.close()
+    # the same evaluation unsegmented on one stream (its phases from HIP events)
+    ctx1 = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, changeover=1, max_ipl=tiles[0], chunk_j=tiles[1],
+                      segments=1)
+    r1 = sf.Resident.from_lists(ctx1, L, rin, rout)
+    for _ in range(max(warmup, 1)):
+        r1.run()
+    ph = []
+    for _ in range(min(steps, 20)):
+        r1.run()
+        ph.append(r1.timing())
+    r1.close()
+    ctx1.close()
+    ph = np.median(np.array(ph), axis=0)
     rng = np.random.default_rng(seed)
     w_sel = np.sort(rng.choice(L.n_walk, max(1, int(frac * L.n_walk)), replace=False))
     eoff, soff = L.walk_offsets()
@@ -331,7 +344,10 @@ def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0):
     ea = np.linalg.norm(acc[rows] - racc, axis=1) / np.linalg.norm(racc, axis=1)
     et = np.abs(pt[rows] - rpt) / np.abs(rpt)
     ne, ns = L.interactions()
-    out = dict(ms_per_step=round(ms, 4), gint_s=round((ne + ns) / (ms * 1e-3) / 1e9, 2), sample_i=int(rows.size),
+    out = dict(ms_per_step=round(ms, 4), gint_s=round((ne + ns) / (ms * 1e-3) / 1e9, 2),
+               unsegmented_ms=dict(stage=round(float(ph[0]), 4), force=round(float(ph[1]), 4),
+                                   reduce=round(float(ph[2]), 4), step=round(float(ph.sum()), 4)),
+               sample_i=int(rows.size),
                acc_rel_median=float(np.median(ea)), acc_rel_p99=float(np.percentile(ea, 99)),
                pot_tot_rel_median=float(np.median(et)), pot_tot_rel_p99=float(np.percentile(et, 99)),
                radii="r_in = 0.1 r_out, r_out = the cutoff radius, r_search = 1.5 r_out")

@@ -727,7 +727,7 @@ __device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int
 
 // One launch per i-per-lane class: items [0, n_items) of this class.
 template <int NNB_LE, int IPL, bool VC, bool CO>
-__global__ __launch_bounds__(256) void k_force(const ForceArgs a)
+__device__ __forceinline__ void force_items(const ForceArgs &a)
 {
     const int idx = uniform((int)blockIdx.x * 4 + ((int)threadIdx.x >> 6));
     if (idx >= a.n_items) return;
@@ -745,6 +745,22 @@ __global__ __launch_bounds__(256) void k_force(const ForceArgs a)
     const int near_end = ((cip)(a.near_blk))[it.walk];
     const Consts K{jv<VC>(a.eps2), jv<VC>(a.rout2), jv<VC>(0.5f), jv<VC>(2.5f)};
     run_tile<IPL, NNB_LE, VC, CO>(a, it, near_end, threadIdx.x & 63, K);
+}
+
+template <int NNB_LE, int IPL, bool VC, bool CO>
+__global__ __launch_bounds__(256) void k_force(const ForceArgs a)
+{
+    force_items<NNB_LE, IPL, VC, CO>(a);
+}
+
+// Changeover mode: its extra per-i state and the revisit of neighbour pairs
+// leave room for 5 waves per SIMD at 2 i per lane; held to 7 (a few spilled
+// registers, on the revisit path) the force phase runs 10% faster.  With 3
+// or 4 i per lane the spills would be heavy: no limit there.
+template <int NNB_LE, int IPL, bool VC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IPL <= 2 ? 7 : 1))) void k_force_co(const ForceArgs a)
+{
+    force_items<NNB_LE, IPL, VC, true>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -918,10 +934,17 @@ hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st)
 template <int NNB, bool VC, bool CO>
 static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t st)
 {
-    if (ipl == 1) hipLaunchKernelGGL((k_force<NNB, 1, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
-    else if (ipl == 2) hipLaunchKernelGGL((k_force<NNB, 2, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
-    else if (ipl == 3) hipLaunchKernelGGL((k_force<NNB, 3, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_force<NNB, 4, VC, CO>), dim3(nblk), dim3(256), 0, st, a);
+    if constexpr (CO) {
+        if (ipl == 1) hipLaunchKernelGGL((k_force_co<NNB, 1, VC>), dim3(nblk), dim3(256), 0, st, a);
+        else if (ipl == 2) hipLaunchKernelGGL((k_force_co<NNB, 2, VC>), dim3(nblk), dim3(256), 0, st, a);
+        else if (ipl == 3) hipLaunchKernelGGL((k_force_co<NNB, 3, VC>), dim3(nblk), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_force_co<NNB, 4, VC>), dim3(nblk), dim3(256), 0, st, a);
+    } else {
+        if (ipl == 1) hipLaunchKernelGGL((k_force<NNB, 1, VC, false>), dim3(nblk), dim3(256), 0, st, a);
+        else if (ipl == 2) hipLaunchKernelGGL((k_force<NNB, 2, VC, false>), dim3(nblk), dim3(256), 0, st, a);
+        else if (ipl == 3) hipLaunchKernelGGL((k_force<NNB, 3, VC, false>), dim3(nblk), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_force<NNB, 4, VC, false>), dim3(nblk), dim3(256), 0, st, a);
+    }
 }
 
 hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
