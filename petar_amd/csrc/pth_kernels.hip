@@ -320,12 +320,15 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         const int n_miss = __syncthreads_count(miss);
         if (tid == 0) a.walk_err[w] = n_miss;
     }
-    // -- SPJ list: 12-float records relative to the centre, four per thread
-    //    and pass so that four gathers are in flight
+    // -- SPJ list: 12-float records relative to the centre, U per thread
+    //    and pass so that U gathers are in flight
     float4 *out_s = reinterpret_cast<float4 *>(a.staged_s) + a.ss_off[w] * 3;
     const int *ids = a.id_spj + wk.s_off;
     const SpjDev *__restrict__ spj = a.spj;
-    constexpr int U = 4;
+    // (two in changeover mode: fewer registers, so the staging of the next
+    // segment fits beside the changeover force kernels, which are held to 7
+    // waves per SIMD; 4.5% off the pipelined changeover step)
+    constexpr int U = CO ? 2 : 4;
     for (int k0 = 0; k0 < wk.n_s; k0 += 256 * U) {
         int id[U];
 #pragma unroll
