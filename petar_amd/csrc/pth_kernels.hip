@@ -984,7 +984,10 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
     a.part_accp = b.part_accp;
     a.part_nnb = b.part_nnb;
     a.part_pott = b.part_pott;
-    for (int c = 0; c < PLAN_CLASSES; c++) {
+    // the longer classes first: the 1-i launch is about one round of waves,
+    // and its run-down tail then overlaps the other force stream's work
+    // instead of holding back a longer launch behind it on this stream
+    for (int c = PLAN_CLASSES - 1; c >= 0; c--) {
         const int n = g.seg_count[c];
         if (n <= 0) continue;
         a.items = b.items + g.seg_begin[c];
