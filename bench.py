@@ -116,6 +116,7 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
         ngb_ms.append(ms)
     res["neighbor_search"] = dict(ms=round(float(np.median(ngb_ms)), 3), entries=int(cnt.sum()),
                                   particles_with_neighbours=int((cnt > 1).sum()),
+                                  tile_walk_passes_last=g.tile_walk_passes(),
                                   note="FP64 lists of |dx|^2 < max(rs_i, rs_j)^2: per-tile r_search walks, "
                                        "exact count + scan + write on the device")
     acc2, pot2, nnb2 = r.results()

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
             if (a.id_spj) os = a.id_spj + a.off_s[g];
             if (a.padded) {
                 cap_e = a.off_e[g + 1] - a.off_e[g];
-                cap_s = a.off_s[g + 1] - a.off_s[g];
+                if (a.id_spj) cap_s = a.off_s[g + 1] - a.off_s[g];
             }
         }
         if (lane == 0) stk[0] = 0;
@@ -720,6 +720,11 @@ struct pth_gtree {
     DevBuf pad_off, pad_e, pad_s;
     std::vector<long long> pad_h;
     int walk_passes = 0;  // of the last build: 1 (one padded pass) or 2 (count, then write)
+    // one-pass tile walks of the neighbour search: the last search's
+    // candidate counts per tile (nt + 1 entries), padded offsets, lists
+    std::vector<long long> tcnt_h, tpad_h;
+    DevBuf tpad_off, tl_pad;
+    int tile_passes = 0;  // of the last search's tile walk
     long long n_ngb = -1;  // total entries, -1 before the first search
     long long n_cand = 0;  // candidate pairs of the last search
     float ms_ngb = 0.f;
@@ -1313,10 +1318,11 @@ int pth_gtree_timing(const pth_gtree *t, float *ms_build, float *ms_walk, float 
     return PTH_OK;
 }
 
-int pth_gtree_stats(const pth_gtree *t, int *walk_passes)
+int pth_gtree_stats(const pth_gtree *t, int *walk_passes, int *tile_walk_passes)
 {
     if (!t) return PTH_ERR_ARG;
     if (walk_passes) *walk_passes = t->walk_passes;
+    if (tile_walk_passes) *tile_walk_passes = t->tile_passes;
     return PTH_OK;
 }
 
@@ -1382,7 +1388,33 @@ int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
     wa.theta_pos = 1;
     wa.cnt_e = t->tcnt.as<long long>();
     const unsigned wgt = (unsigned)std::min<long long>(nt, 65536);
-    hipLaunchKernelGGL(k_walk<false>, dim3(wgt), dim3(64), 0, st, wa);
+    // as many tiles as the last search (the same tree, or one rebuilt from
+    // particles that moved a little): one writing pass into regions of the
+    // last search's candidate counts + 1/4 + 64 that also counts; otherwise
+    // a counting pass first (a region that turns out too small means an
+    // exact writing pass, as for the group walk)
+    const bool one_pass = nt > 0 && (long long)t->tcnt_h.size() == nt + 1;
+    if (one_pass) {
+        std::vector<long long> &ph = t->tpad_h;
+        ph.assign(nt + 1, 0);
+        long long pe = 0;
+        for (long long g = 0; g < nt; g++) {
+            const long long k = std::max(t->tcnt_h[g], 0LL);
+            ph[g] = pe;
+            pe += k + k / 4 + 64;
+        }
+        ph[nt] = pe;
+        HIPCHK(c, t->tpad_off.ensure(8 * (size_t)(nt + 1)), "neighbour buffers");
+        HIPCHK(c, t->tl_pad.ensure(4 * (size_t)pe), "neighbour buffers");
+        HIPCHK(c, hipMemcpyAsync(t->tpad_off.p, ph.data(), 8 * (size_t)(nt + 1), hipMemcpyHostToDevice, st),
+               "neighbour regions");
+        wa.off_e = t->tpad_off.as<long long>();
+        wa.id_epj = t->tl_pad.as<int>();
+        wa.padded = 1;
+        hipLaunchKernelGGL(k_walk<true>, dim3(wgt), dim3(64), 0, st, wa);
+    } else if (nt > 0) {
+        hipLaunchKernelGGL(k_walk<false>, dim3(wgt), dim3(64), 0, st, wa);
+    }
     const long long zero = 0;
     HIPCHK(c, hipMemcpyAsync(t->tcnt.as<long long>() + nt, &zero, 8, hipMemcpyHostToDevice, st), "neighbour counts");
     if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t->tcnt.as<long long>(), t->toff.as<long long>(),
@@ -1392,14 +1424,31 @@ int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
                                               (int)(nt + 1), st)))
         return c->hip_fail(e, "candidate scan");
     long long ncand = 0;
+    t->tcnt_h.resize(nt + 1);
     HIPCHK(c, hipMemcpyAsync(&ncand, t->toff.as<long long>() + nt, 8, hipMemcpyDeviceToHost, st), "candidates");
+    HIPCHK(c, hipMemcpyAsync(t->tcnt_h.data(), t->tcnt.p, 8 * (size_t)(nt + 1), hipMemcpyDeviceToHost, st),
+           "candidates");
     HIPCHK(c, hipStreamSynchronize(st), "candidates");
-    if (ncand > 0x7fffffffLL) return c->fail(PTH_ERR_CAPACITY, "pth_gtree_neighbors: candidates exceed the int32 range");
+    if (ncand > 0x7fffffffLL) {
+        t->tcnt_h.clear();
+        return c->fail(PTH_ERR_CAPACITY, "pth_gtree_neighbors: candidates exceed the int32 range");
+    }
+    // did every tile's candidates fit its region?
+    bool rewalk = !one_pass;
+    for (long long g = 0; g < nt && !rewalk; g++) rewalk = t->tcnt_h[g] > t->tpad_h[g + 1] - t->tpad_h[g];
+    t->tile_passes = nt > 0 ? (rewalk ? 2 : 1) : 0;
     HIPCHK(c, t->tl.ensure(sizeof(int) * std::max<long long>(ncand, 1)), "neighbour buffers");
     HIPCHK(c, t->tl_alt.ensure(sizeof(int) * std::max<long long>(ncand, 1)), "neighbour buffers");
     wa.off_e = t->toff.as<long long>();
     wa.id_epj = t->tl.as<int>();
-    hipLaunchKernelGGL(k_walk<true>, dim3(wgt), dim3(64), 0, st, wa);
+    wa.padded = 0;
+    if (nt > 0 && rewalk) {
+        hipLaunchKernelGGL(k_walk<true>, dim3(wgt), dim3(64), 0, st, wa);
+    } else if (nt > 0) {
+        // every tile fits its region (checked above)
+        hipLaunchKernelGGL(k_compact, dim3(wgt), dim3(256), 0, st, nt, t->tpad_off.as<long long>(),
+                           t->toff.as<long long>(), t->tl_pad.as<int>(), t->tl.as<int>());
+    }
     // candidates ascending in tree order, so rows come out in that order
     if (ncand > 0) {
         int bits = 1;
@@ -1479,7 +1528,8 @@ void pth_gtree_free(pth_gtree *t)
     DevBuf *all[] = {&t->order, &t->key, &t->skey, &t->sidx, &t->spos, &t->smass, &t->srs, &t->nodes, &t->nmass,
                      &t->ncom, &t->nquad, &t->box_in, &t->box_out, &t->groups, &t->off_e, &t->off_s, &t->id_epj,
                      &t->id_spj, &t->tmp, &t->ngb_cnt, &t->ngb_off, &t->ngb, &t->tiles, &t->tbox, &t->tcnt,
-                     &t->toff, &t->tl, &t->tl_alt, &t->tinfo, &t->pad_off, &t->pad_e, &t->pad_s};
+                     &t->toff, &t->tl, &t->tl_alt, &t->tinfo, &t->pad_off, &t->pad_e, &t->pad_s,
+                     &t->tpad_off, &t->tl_pad};
     for (DevBuf *b : all) b->release();
     t->release_scratch();
     delete t;

@@ -289,6 +289,28 @@ def test_device_neighbors_equal_oracle(n, f_bin, theta):
     assert np.array_equal(cnt, ecnt) and np.array_equal(off, eoff) and np.array_equal(ngb, engb)
 
 
+def test_device_neighbors_one_pass_repeat():
+    """A search with as many tiles as the last one writes its candidate lists
+    in one pass into regions of the last search's candidate counts; a larger
+    r_search (same tree and tiles, more candidates) outgrows them and takes
+    the exact second pass.  The lists equal the FP64 restatement each time."""
+    ctx = sf.Context(0)
+    m, x, _ = host.cluster(20000, "plummer", kroupa=True, f_bin=0.5, seed=43)
+    rs = np.full(m.size, 1.5 * R_OUT)
+    g = sf.GpuTree(ctx, x, m, rs)
+    for rss, rebuild, passes in ((rs, False, 2), (rs, False, 1), (4.0 * rs, True, 2), (4.0 * rs, False, 1)):
+        if rebuild:
+            g.update(x, m, rss)
+        cnt, off, ngb, _ = g.neighbors()
+        assert g.tile_walk_passes() == passes
+        L = host.build_lists(x, m, rss, theta=0.3)
+        ecnt, eoff, engb = _expected_ngb(L, *O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj,
+                                                         L.epj_pos, L.epj_rs))
+        assert np.array_equal(cnt, ecnt) and np.array_equal(off, eoff) and np.array_equal(ngb, engb)
+    g.close()
+    ctx.close()
+
+
 def test_device_neighbors_edge_cases():
     """No r_search (no neighbours, not even self), one particle, and a
     repeated search after an update."""
