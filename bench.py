@@ -116,9 +116,10 @@ def device_tree_step(ctx, particles, r_out, L, ref, reps=5):
         ngb_ms.append(ms)
     res["neighbor_search"] = dict(ms=round(float(np.median(ngb_ms)), 3), entries=int(cnt.sum()),
                                   particles_with_neighbours=int((cnt > 1).sum()),
-                                  tile_walk_passes_last=g.tile_walk_passes(),
+                                  tile_walk_passes_last=g.tile_walk_passes(), filter_passes_last=g.filter_passes(),
                                   note="FP64 lists of |dx|^2 < max(rs_i, rs_j)^2: per-tile r_search walks, "
-                                       "exact count + scan + write on the device")
+                                       "exact FP64 filter on the device; a repeated search writes both in one "
+                                       "pass into regions sized from the last search")
     acc2, pot2, nnb2 = r.results()
     same = same and bool(np.array_equal(acc2, ref[0]) and np.array_equal(pot2, ref[1]) and np.array_equal(nnb2, ref[2]))
     r.close()

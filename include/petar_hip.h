@@ -309,9 +309,11 @@ int pth_gtree_timing(const pth_gtree *tree, float *ms_build, float *ms_walk, flo
  * 2 when a counting or padded pass was followed by an exact writing pass;
  * tile_walk_passes: the same for the candidate walk of the last neighbour
  * search (regions from the previous search's candidate counts, when it had
- * as many tiles).  No counterpart in the reference (FDPS walks on the host);
- * for callers that monitor rebuild costs.  Either pointer may be null. */
-int pth_gtree_stats(const pth_gtree *tree, int *walk_passes, int *tile_walk_passes);
+ * as many tiles); filter_passes: the same for its exact filter (regions
+ * from the previous completed search's per-particle counts, same N).  No
+ * counterpart in the reference (FDPS walks on the host); for callers that
+ * monitor rebuild costs.  Any pointer may be null. */
+int pth_gtree_stats(const pth_gtree *tree, int *walk_passes, int *tile_walk_passes, int *filter_passes);
 void pth_gtree_free(pth_gtree *tree);
 /* Neighbour search on the tree (replaces the FDPS short-range tree search
  * PeTar runs with SearchNeighborEpEpNoSimd / SearchNeighborEpEpSimd,

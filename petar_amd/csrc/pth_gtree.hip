@@ -516,6 +516,33 @@ __global__ __launch_bounds__(256) void k_compact(long long ng, const long long *
     }
 }
 
+// Regions of a one-pass neighbour filter: the last search's count of
+// particle i + 1/4 + 4 (cap[n] = 0, so the scan's last entry is the total).
+__global__ __launch_bounds__(256) void k_ngb_caps(long long n, const int *__restrict__ cnt, int *__restrict__ cap)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const int c = max(cnt[i], 0);
+        cap[i] = c + c / 4 + 4;
+    } else if (i == n) {
+        cap[n] = 0;
+    }
+}
+
+// Particle i's neighbours, written by a padded filter pass at poff[i], moved
+// to off[i] (checked on the host to fit the region).
+__global__ __launch_bounds__(256) void k_ngb_compact(long long n, const long long *__restrict__ poff,
+                                                     const long long *__restrict__ off, const int *__restrict__ src,
+                                                     int *__restrict__ dst)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const long long m = off[i + 1] - off[i];
+    const int *s = src + poff[i];
+    int *d = dst + off[i];
+    for (long long k = 0; k < m; k++) d[k] = s[k];
+}
+
 // Boxes of the neighbour-search tiles (the largest cells with at most 64
 // particles): the cell's particle box and its box widened by r_search,
 // pushed outwards by (|v| + rs_max) 2^-40, far above the rounding of any
@@ -578,6 +605,10 @@ struct NgbArgs {
     int *cnt;
     const long long *off;
     int *ngb;
+    // padded writing pass: particle i writes into [off[i], off[i + 1]) at
+    // most, stores its exact count, and sets *flag when it did not fit
+    int padded;
+    int *flag;
 };
 
 struct NgbJ {
@@ -638,11 +669,12 @@ __global__ __launch_bounds__(64) void k_ngb(const NgbArgs a)
             const int nt = min(NGB_T, (G.y - s0 + 63) / 64);
             double xi[NGB_T], yi[NGB_T], zi[NGB_T], ri[NGB_T];
             long long w[NGB_T];
-            int c[NGB_T];
+            int c[NGB_T], cap[NGB_T];
             bool act[NGB_T];
 #pragma unroll
             for (int t = 0; t < NGB_T; t++) {
                 c[t] = 0;
+                cap[t] = INT_MAX;
                 act[t] = false;
                 xi[t] = yi[t] = zi[t] = ri[t] = 0.0;
                 w[t] = 0;
@@ -654,7 +686,11 @@ __global__ __launch_bounds__(64) void k_ngb(const NgbArgs a)
                         yi[t] = a.spos[3 * i + 1];
                         zi[t] = a.spos[3 * i + 2];
                         ri[t] = a.srs[i];
-                        if (WRITE) w[t] = a.off[a.order[i]];
+                        if (WRITE) {
+                            const long long oi = a.order[i];
+                            w[t] = a.off[oi];
+                            if (a.padded) cap[t] = (int)(a.off[oi + 1] - w[t]);
+                        }
                     }
                     const Box b = wave_box(act[t], xi[t], yi[t], zi[t], ri[t]);
                     if (lane == 0) tb[t] = b;
@@ -684,17 +720,22 @@ __global__ __launch_bounds__(64) void k_ngb(const NgbArgs a)
                             const double r2 = dx * dx + dy * dy + dz * dz;
                             const double rs = (ri[t] < Q.rs) ? Q.rs : ri[t];
                             if (r2 < rs * rs) {
-                                if (WRITE) a.ngb[w[t] + c[t]] = sid[q];
+                                if (WRITE && c[t] < cap[t]) a.ngb[w[t] + c[t]] = sid[q];
                                 c[t]++;
                             }
                         }
                 }
                 __syncthreads();
             }
-            if (!WRITE) {
+            if (!WRITE || a.padded) {
+                bool over = false;
 #pragma unroll
                 for (int t = 0; t < NGB_T; t++)
-                    if (t < nt && act[t]) a.cnt[a.order[(long long)G.x + s0 + 64 * t + lane]] = c[t];
+                    if (t < nt && act[t]) {
+                        a.cnt[a.order[(long long)G.x + s0 + 64 * t + lane]] = c[t];
+                        over = over || c[t] > cap[t];
+                    }
+                if (WRITE && over) atomicOr(a.flag, 1);
             }
         }
     }
@@ -725,6 +766,12 @@ struct pth_gtree {
     std::vector<long long> tcnt_h, tpad_h;
     DevBuf tpad_off, tl_pad;
     int tile_passes = 0;  // of the last search's tile walk
+    // one-pass neighbour filters: regions from the last completed search's
+    // per-particle counts (ngb_cnt), valid while ngb_prev_ok
+    DevBuf ngb_cap, ngb_poff, ngb_pad, ngb_flag;
+    bool ngb_prev_ok = false;
+    long long ngb_prev_n = -1, ngb_prev_total = 0;
+    int ngb_passes = 0;  // of the last search's filter
     long long n_ngb = -1;  // total entries, -1 before the first search
     long long n_cand = 0;  // candidate pairs of the last search
     float ms_ngb = 0.f;
@@ -1318,11 +1365,12 @@ int pth_gtree_timing(const pth_gtree *t, float *ms_build, float *ms_walk, float 
     return PTH_OK;
 }
 
-int pth_gtree_stats(const pth_gtree *t, int *walk_passes, int *tile_walk_passes)
+int pth_gtree_stats(const pth_gtree *t, int *walk_passes, int *tile_walk_passes, int *filter_passes)
 {
     if (!t) return PTH_ERR_ARG;
     if (walk_passes) *walk_passes = t->walk_passes;
     if (tile_walk_passes) *tile_walk_passes = t->tile_passes;
+    if (filter_passes) *filter_passes = t->ngb_passes;
     return PTH_OK;
 }
 
@@ -1464,7 +1512,17 @@ int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
     }
     hipLaunchKernelGGL(k_tile_info, dim3(blocks(nt)), dim3(256), 0, st, t->nodes.as<GNode>(), t->tiles.as<int>(), nt,
                        t->toff.as<long long>(), t->tinfo.as<int4>());
-    // 2. exact FP64 filter of the candidates: count, scan, write
+    // 2. exact FP64 filter of the candidates.  After a completed search of
+    //    as many particles: one writing pass into regions of that search's
+    //    counts + 1/4 + 4 that also counts (their total is at most 1.25 times
+    //    its total + 4 n, since ngb_cnt holds exactly its counts), then the
+    //    move into place, or an exact writing pass if some particle did not
+    //    fit.  Otherwise: count, scan, write.
+    const long long prev_total = t->ngb_prev_total;
+    // (the region scan accumulates in int: its total must stay below 2^31)
+    const bool one_pass_ngb = t->ngb_prev_ok && t->ngb_prev_n == n && n > 0 &&
+                              prev_total + prev_total / 4 + 4 * n + 64 < 0x7fffffffLL;
+    t->ngb_prev_ok = false;  // ngb_cnt is rewritten below
     HIPCHK(c, t->ngb_cnt.ensure(sizeof(int) * (n + 1)), "neighbour buffers");
     HIPCHK(c, t->ngb_off.ensure(sizeof(long long) * (n + 1)), "neighbour buffers");
     NgbArgs na{};
@@ -1476,9 +1534,30 @@ int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
     na.id_epj = t->tl.as<int>();
     na.n_group = nt;
     na.cnt = t->ngb_cnt.as<int>();
+    if (one_pass_ngb) {
+        const long long bound = prev_total + prev_total / 4 + 4 * n + 64;
+        HIPCHK(c, t->ngb_cap.ensure(sizeof(int) * (n + 1)), "neighbour buffers");
+        HIPCHK(c, t->ngb_poff.ensure(sizeof(long long) * (n + 1)), "neighbour buffers");
+        HIPCHK(c, t->ngb_pad.ensure(sizeof(int) * (size_t)bound), "neighbour buffers");
+        HIPCHK(c, t->ngb_flag.ensure(sizeof(int)), "neighbour buffers");
+        hipLaunchKernelGGL(k_ngb_caps, dim3(blocks(n + 1)), dim3(256), 0, st, n, t->ngb_cnt.as<int>(),
+                           t->ngb_cap.as<int>());
+        if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t->ngb_cap.as<int>(), t->ngb_poff.as<long long>(),
+                                                  (int)(n + 1), st)) ||
+            (e = cub_temp(t, tb)) ||
+            (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, t->ngb_cap.as<int>(), t->ngb_poff.as<long long>(),
+                                                  (int)(n + 1), st)))
+            return c->hip_fail(e, "neighbour regions");
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, t->ngb_flag.as<int>(), 1, 0);
+        na.off = t->ngb_poff.as<long long>();
+        na.ngb = t->ngb_pad.as<int>();
+        na.padded = 1;
+        na.flag = t->ngb_flag.as<int>();
+    }
     // the count of entry n stays 0 so the scan's last entry is the total
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, t->ngb_cnt.as<int>() + n, 1, 0);
-    hipLaunchKernelGGL(k_ngb<false>, dim3(wgt), dim3(64), 0, st, na);
+    if (one_pass_ngb) hipLaunchKernelGGL(k_ngb<true>, dim3(wgt), dim3(64), 0, st, na);
+    else hipLaunchKernelGGL(k_ngb<false>, dim3(wgt), dim3(64), 0, st, na);
     if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t->ngb_cnt.as<int>(), t->ngb_off.as<long long>(),
                                               (int)(n + 1), st)) ||
         (e = cub_temp(t, tb)) ||
@@ -1486,15 +1565,30 @@ int pth_gtree_neighbors(pth_gtree *t, long long *n_total)
                                               (int)(n + 1), st)))
         return c->hip_fail(e, "neighbour scan");
     long long total = 0;
+    int over = 0;
     HIPCHK(c, hipMemcpyAsync(&total, t->ngb_off.as<long long>() + n, 8, hipMemcpyDeviceToHost, st), "neighbour total");
+    if (one_pass_ngb)
+        HIPCHK(c, hipMemcpyAsync(&over, t->ngb_flag.p, sizeof(int), hipMemcpyDeviceToHost, st), "neighbour total");
     HIPCHK(c, hipStreamSynchronize(st), "neighbour count");
     if (total > 0x7fffffffLL) return c->fail(PTH_ERR_CAPACITY, "pth_gtree_neighbors: lists exceed the int32 range");
     HIPCHK(c, t->ngb.ensure(sizeof(int) * std::max<long long>(total, 1)), "neighbour buffers");
-    na.off = t->ngb_off.as<long long>();
-    na.ngb = t->ngb.as<int>();
-    if (total > 0) hipLaunchKernelGGL(k_ngb<true>, dim3(wgt), dim3(64), 0, st, na);
+    const bool rewrite = !one_pass_ngb || over != 0;
+    if (total > 0 && rewrite) {
+        na.off = t->ngb_off.as<long long>();
+        na.ngb = t->ngb.as<int>();
+        na.padded = 0;
+        hipLaunchKernelGGL(k_ngb<true>, dim3(wgt), dim3(64), 0, st, na);
+    } else if (total > 0) {
+        // every particle fits its region (no flag raised)
+        hipLaunchKernelGGL(k_ngb_compact, dim3(blocks(n)), dim3(256), 0, st, n, t->ngb_poff.as<long long>(),
+                           t->ngb_off.as<long long>(), t->ngb_pad.as<int>(), t->ngb.as<int>());
+    }
     HIPCHK(c, hipGetLastError(), "neighbour search");
     HIPCHK(c, hipStreamSynchronize(st), "neighbour search");
+    t->ngb_passes = rewrite ? 2 : 1;
+    t->ngb_prev_ok = true;
+    t->ngb_prev_n = n;
+    t->ngb_prev_total = total;
     t->n_ngb = total;
     t->n_cand = ncand;
     t->ms_ngb = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1529,7 +1623,7 @@ void pth_gtree_free(pth_gtree *t)
                      &t->ncom, &t->nquad, &t->box_in, &t->box_out, &t->groups, &t->off_e, &t->off_s, &t->id_epj,
                      &t->id_spj, &t->tmp, &t->ngb_cnt, &t->ngb_off, &t->ngb, &t->tiles, &t->tbox, &t->tcnt,
                      &t->toff, &t->tl, &t->tl_alt, &t->tinfo, &t->pad_off, &t->pad_e, &t->pad_s,
-                     &t->tpad_off, &t->tl_pad};
+                     &t->tpad_off, &t->tl_pad, &t->ngb_cap, &t->ngb_poff, &t->ngb_pad, &t->ngb_flag};
     for (DevBuf *b : all) b->release();
     t->release_scratch();
     delete t;

@@ -138,7 +138,7 @@ def lib():
     L.pth_gtree_sizes.argtypes = [vp, llp, llp, llp, llp]
     L.pth_gtree_download.argtypes = [vp, llp, llp, llp, ip, dp, dp, dp, llp, ip, ip, ip, ip, ip]
     L.pth_gtree_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
-    L.pth_gtree_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pth_gtree_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pth_resident_from_gtree.argtypes = [vp, C.POINTER(vp), vp]
     L.pth_gtree_update.argtypes = [vp, C.c_longlong, vp, vp, vp, C.c_int]
     L.pth_resident_refill.argtypes = [vp, vp]
@@ -555,19 +555,24 @@ class GpuTree:
         self.ctx.check(lib().pth_gtree_timing(self._h, C.byref(a), C.byref(b), C.byref(c)), "pth_gtree_timing")
         return a.value, b.value, c.value
 
+    def _stats(self):
+        v = [C.c_int(), C.c_int(), C.c_int()]
+        self.ctx.check(lib().pth_gtree_stats(self._h, *(C.byref(x) for x in v)), "pth_gtree_stats")
+        return [x.value for x in v]
+
     def walk_passes(self):
         """Walk passes of the last build: 1 when the lists were written in one
         pass into regions sized from the previous build, 2 otherwise."""
-        n, m = C.c_int(), C.c_int()
-        self.ctx.check(lib().pth_gtree_stats(self._h, C.byref(n), C.byref(m)), "pth_gtree_stats")
-        return n.value
+        return self._stats()[0]
 
     def tile_walk_passes(self):
         """Passes of the last neighbour search's candidate walk (1 or 2, as
         walk_passes)."""
-        n, m = C.c_int(), C.c_int()
-        self.ctx.check(lib().pth_gtree_stats(self._h, C.byref(n), C.byref(m)), "pth_gtree_stats")
-        return m.value
+        return self._stats()[1]
+
+    def filter_passes(self):
+        """Passes of the last neighbour search's exact filter (1 or 2)."""
+        return self._stats()[2]
 
     def close(self):
         if self._h:

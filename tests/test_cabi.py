@@ -74,7 +74,7 @@ def test_null_context_calls():
     assert L.pth_gtree_sizes(None, None, None, None, None) == sf.PTH_ERR_ARG
     assert L.pth_gtree_download(None, *([None] * 13)) == sf.PTH_ERR_ARG
     assert L.pth_gtree_timing(None, None, None, None) == sf.PTH_ERR_ARG
-    assert L.pth_gtree_stats(None, None, None) == sf.PTH_ERR_ARG
+    assert L.pth_gtree_stats(None, None, None, None) == sf.PTH_ERR_ARG
     assert L.pth_gtree_neighbors(None, None) == sf.PTH_ERR_ARG
     assert L.pth_gtree_neighbors_download(None, None, None, None, None) == sf.PTH_ERR_ARG
     assert L.pth_resident_from_gtree(None, None, None) == sf.PTH_ERR_ARG

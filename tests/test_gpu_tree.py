@@ -291,18 +291,23 @@ def test_device_neighbors_equal_oracle(n, f_bin, theta):
 
 def test_device_neighbors_one_pass_repeat():
     """A search with as many tiles as the last one writes its candidate lists
-    in one pass into regions of the last search's candidate counts; a larger
-    r_search (same tree and tiles, more candidates) outgrows them and takes
-    the exact second pass.  The lists equal the FP64 restatement each time."""
+    in one pass into regions of the last search's candidate counts, and its
+    neighbour lists into regions of the last search's per-particle counts; a
+    larger r_search (same tree and tiles, more candidates and neighbours)
+    outgrows them and takes the exact second passes.  The lists equal the
+    FP64 restatement each time."""
     ctx = sf.Context(0)
     m, x, _ = host.cluster(20000, "plummer", kroupa=True, f_bin=0.5, seed=43)
     rs = np.full(m.size, 1.5 * R_OUT)
     g = sf.GpuTree(ctx, x, m, rs)
-    for rss, rebuild, passes in ((rs, False, 2), (rs, False, 1), (4.0 * rs, True, 2), (4.0 * rs, False, 1)):
+    # (tile walk passes, filter passes): a first search counts first; a
+    # repeated one writes once; a 4x r_search outgrows both kinds of region
+    for rss, rebuild, passes in ((rs, False, (2, 2)), (rs, False, (1, 1)), (4.0 * rs, True, (2, 2)),
+                                 (4.0 * rs, False, (1, 1))):
         if rebuild:
             g.update(x, m, rss)
         cnt, off, ngb, _ = g.neighbors()
-        assert g.tile_walk_passes() == passes
+        assert (g.tile_walk_passes(), g.filter_passes()) == passes
         L = host.build_lists(x, m, rss, theta=0.3)
         ecnt, eoff, engb = _expected_ngb(L, *O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj,
                                                          L.epj_pos, L.epj_rs))
