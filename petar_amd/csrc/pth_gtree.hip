@@ -143,6 +143,7 @@ __global__ __launch_bounds__(256) void k_split_count(const GNode *__restrict__ n
                                                      long long *__restrict__ bnd, int *__restrict__ cnt)
 {
     const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k == 0) cnt[nl] = 0;  // so that the scan's entry nl is the level's child count
     if (k >= nl) return;
     const GNode nd = nodes[lb + k];
     long long *b = bnd + 9 * k;
@@ -761,6 +762,7 @@ struct pth_gtree {
     DevBuf pad_off, pad_e, pad_s;
     std::vector<long long> pad_h;
     int walk_passes = 0;  // of the last build: 1 (one padded pass) or 2 (count, then write)
+    void *pinned = nullptr;  // 64 B of pinned host memory for the per-level node counts
     // one-pass tile walks of the neighbour search: the last search's
     // candidate counts per tile (nt + 1 entries), padded offsets, lists
     std::vector<long long> tcnt_h, tpad_h;
@@ -970,6 +972,10 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
             (e = hipStreamSynchronize(st)))
             return ctx->hip_fail(e, "node upload");
     }
+    if (!t->pinned && (e = hipHostMalloc(&t->pinned, 64, hipHostMallocDefault))) {
+        t->pinned = nullptr;
+        return ctx->hip_fail(e, "pinned buffer");
+    }
     t->level_start.assign(1, 0);
     long long lb = 0, le = 1;
     DevBuf &bnd = t->bnd, &cnt = t->cnt, &off = t->off;
@@ -981,16 +987,15 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
             return ctx->hip_fail(e, "level buffers");
         hipLaunchKernelGGL(k_split_count, dim3(blocks(nl)), dim3(256), 0, st, t->nodes.as<GNode>(), lb, nl,
                            t->skey.as<unsigned long long>(), n_leaf_limit, bnd.as<long long>(), cnt.as<int>());
-        if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<int>(), off.as<int>(), (int)nl, st)) ||
+        if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<int>(), off.as<int>(), (int)nl + 1, st)) ||
             (e = cub_temp(t, tb)) ||
-            (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, cnt.as<int>(), off.as<int>(), (int)nl, st)))
+            (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, cnt.as<int>(), off.as<int>(), (int)nl + 1, st)))
             return ctx->hip_fail(e, "level scan");
-        int last[2] = {0, 0};
-        if ((e = hipMemcpyAsync(&last[0], cnt.as<int>() + nl - 1, sizeof(int), hipMemcpyDeviceToHost, st)) ||
-            (e = hipMemcpyAsync(&last[1], off.as<int>() + nl - 1, sizeof(int), hipMemcpyDeviceToHost, st)) ||
+        // one small copy into pinned memory per level
+        if ((e = hipMemcpyAsync(t->pinned, off.as<int>() + nl, sizeof(int), hipMemcpyDeviceToHost, st)) ||
             (e = hipStreamSynchronize(st)))
             return ctx->hip_fail(e, "level size");
-        const long long total = (long long)last[0] + last[1];
+        const long long total = *static_cast<volatile int *>(t->pinned);
         if (total == 0) break;
         if (le + total > cap) {
             // grow the node array, keeping its contents
@@ -1626,6 +1631,7 @@ void pth_gtree_free(pth_gtree *t)
                      &t->tpad_off, &t->tl_pad, &t->ngb_cap, &t->ngb_poff, &t->ngb_pad, &t->ngb_flag};
     for (DevBuf *b : all) b->release();
     t->release_scratch();
+    if (t->pinned) (void)hipHostFree(t->pinned);
     delete t;
 }
 
