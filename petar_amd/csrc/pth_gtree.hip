@@ -762,7 +762,7 @@ struct pth_gtree {
     DevBuf pad_off, pad_e, pad_s;
     std::vector<long long> pad_h;
     int walk_passes = 0;  // of the last build: 1 (one padded pass) or 2 (count, then write)
-    void *pinned = nullptr;  // 64 B of pinned host memory for the per-level node counts
+    void *pinned = nullptr;  // pinned host block for small read-backs and uploads (PIN_* layout)
     // one-pass tile walks of the neighbour search: the last search's
     // candidate counts per tile (nt + 1 entries), padded offsets, lists
     std::vector<long long> tcnt_h, tpad_h;
@@ -809,6 +809,21 @@ namespace {
     } while (0)
 
 inline unsigned blocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+// Pinned host block of a tree: small counts read back at byte 0, the root
+// node's upload at byte 64, the bounding-box partials (8 doubles for each of
+// at most 1024 blocks) from byte 128.
+constexpr size_t PIN_COUNT = 0, PIN_ROOT = 64, PIN_BBOX = 128, PIN_BYTES = PIN_BBOX + 8 * 8 * 1024;
+
+hipError_t ensure_pinned(pth_gtree *t)
+{
+    if (t->pinned) return hipSuccess;
+    const hipError_t e = hipHostMalloc(&t->pinned, PIN_BYTES, hipHostMallocDefault);
+    if (e != hipSuccess) t->pinned = nullptr;
+    return e;
+}
+
+template <class T> T *pinned_at(pth_gtree *t, size_t off) { return reinterpret_cast<T *>(static_cast<char *>(t->pinned) + off); }
 
 // temp storage for a hipcub call, grown on demand
 hipError_t cub_temp(pth_gtree *t, size_t bytes) { return t->tmp.ensure(std::max<size_t>(bytes, 256)); }
@@ -859,10 +874,11 @@ int select_cells(pth_gtree *t, int limit, DevBuf &out, long long &n_out)
         (e = hipcub::DeviceSelect::Flagged(t->tmp.p, tb, it, flag.as<char>(), gsel.as<int>(), nsel.as<int>(),
                                            (int)nn, st)))
         return ctx->hip_fail(e, "cell selection");
-    int ngi = 0;
-    if ((e = hipMemcpyAsync(&ngi, nsel.p, sizeof(int), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
+    if ((e = ensure_pinned(t)) ||
+        (e = hipMemcpyAsync(pinned_at<int>(t, PIN_COUNT), nsel.p, sizeof(int), hipMemcpyDeviceToHost, st)) ||
+        (e = hipStreamSynchronize(st)))
         return ctx->hip_fail(e, "cell count");
-    const long long ng = ngi;
+    const long long ng = *static_cast<volatile int *>(pinned_at<int>(t, PIN_COUNT));
     if ((e = gkey.ensure(8 * ng)) || (e = gkey2.ensure(8 * ng)) || (e = out.ensure(sizeof(int) * std::max(ng, 1LL))))
         return ctx->hip_fail(e, "cell selection allocation");
     hipLaunchKernelGGL(k_group_keys, dim3(blocks(ng)), dim3(256), 0, st, t->nodes.as<GNode>(), gsel.as<int>(), ng,
@@ -916,10 +932,12 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     // 1. bounding cube (per-block partials reduced on the host)
     const int nb = (int)std::min<long long>(1024, (n + 255) / 256);
     DevBuf &part = t->part;
-    std::vector<double> hp(8 * (size_t)nb);
+    if ((e = ensure_pinned(t))) return ctx->hip_fail(e, "pinned buffer");
+    const double *hp = pinned_at<double>(t, PIN_BBOX);
     if ((e = part.ensure(sizeof(double) * 8 * nb))) return ctx->hip_fail(e, "bbox");
     hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, st, dpos, drs, n, part.as<double>());
-    if ((e = hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, st)) ||
+    if ((e = hipMemcpyAsync(pinned_at<double>(t, PIN_BBOX), part.p, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost,
+                            st)) ||
         (e = hipStreamSynchronize(st)))
         return ctx->hip_fail(e, "bbox");
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -966,16 +984,11 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
     if ((e = t->nodes.ensure(sizeof(GNode) * std::max<long long>(2 * n / std::max(n_leaf_limit, 1) + 64, 1024))))
         return ctx->hip_fail(e, "node allocation");
     long long cap = (long long)(t->nodes.cap / sizeof(GNode));
-    {
-        const GNode root_node{0, n, 0, 0, -1};
-        if ((e = hipMemcpyAsync(t->nodes.p, &root_node, sizeof(GNode), hipMemcpyHostToDevice, st)) ||
-            (e = hipStreamSynchronize(st)))
-            return ctx->hip_fail(e, "node upload");
-    }
-    if (!t->pinned && (e = hipHostMalloc(&t->pinned, 64, hipHostMallocDefault))) {
-        t->pinned = nullptr;
-        return ctx->hip_fail(e, "pinned buffer");
-    }
+    // (from pinned memory: the copy is ordered before the level loop's
+    // launches and needs no wait)
+    *pinned_at<GNode>(t, PIN_ROOT) = GNode{0, n, 0, 0, -1};
+    if ((e = hipMemcpyAsync(t->nodes.p, pinned_at<GNode>(t, PIN_ROOT), sizeof(GNode), hipMemcpyHostToDevice, st)))
+        return ctx->hip_fail(e, "node upload");
     t->level_start.assign(1, 0);
     long long lb = 0, le = 1;
     DevBuf &bnd = t->bnd, &cnt = t->cnt, &off = t->off;
@@ -992,10 +1005,11 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
             (e = hipcub::DeviceScan::ExclusiveSum(t->tmp.p, tb, cnt.as<int>(), off.as<int>(), (int)nl + 1, st)))
             return ctx->hip_fail(e, "level scan");
         // one small copy into pinned memory per level
-        if ((e = hipMemcpyAsync(t->pinned, off.as<int>() + nl, sizeof(int), hipMemcpyDeviceToHost, st)) ||
+        if ((e = hipMemcpyAsync(pinned_at<int>(t, PIN_COUNT), off.as<int>() + nl, sizeof(int), hipMemcpyDeviceToHost,
+                                st)) ||
             (e = hipStreamSynchronize(st)))
             return ctx->hip_fail(e, "level size");
-        const long long total = *static_cast<volatile int *>(t->pinned);
+        const long long total = *static_cast<volatile int *>(pinned_at<int>(t, PIN_COUNT));
         if (total == 0) break;
         if (le + total > cap) {
             // grow the node array, keeping its contents
