@@ -146,6 +146,25 @@ def _fdps_replay(L, functor, n_walk_limit=200):
     return f["acc"], f["pot"], f["n_ngb"]
 
 
+@pytest.mark.parametrize("eps", [0.1, 0.5, 2.0])
+def test_softened_c1(c1, eps):
+    """Plummer softening eps = 0.1, 0.5 and 2 r_out: eps^2 enters every EPJ
+    pair, the SPJ monopole and the softened quadrupole term
+    (src/soft_force.hpp:44-60,132-140); at eps = 2 r_out every EPJ pair is
+    outside the cutoff."""
+    L, a, _, _ = c1
+    eps2 = (eps * R_OUT) ** 2
+    ref = O.soft_force_walks(**a, eps2=eps2, rout2=R_OUT ** 2, G=1.0, nnb_mode=0)
+    amb = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=eps2)
+    for mode in (sf.EXEC_FUSED, sf.EXEC_PLAIN):
+        ctx = sf.Context(0, eps2=eps2, rcut2=R_OUT ** 2, G=1.0, nnb_mode=0, exec_mode=mode)
+        r = sf.Resident.from_lists(ctx, L)
+        r.run()
+        _check(r.results(), ref, amb, f"softened eps={eps} r_out exec={mode}")
+        r.close()
+        ctx.close()
+
+
 @pytest.mark.parametrize("exec_mode", [sf.EXEC_FUSED, sf.EXEC_PLAIN])
 def test_fdps_multiwalk_replay(c1, exec_mode):
     L, a, ref, amb = c1

@@ -85,15 +85,20 @@ def test_golden_wrong_quadrupole_sign_is_caught(gold):
 # ---- live comparison with the reference kernel on tree lists -------------------
 
 @pytest.mark.skipif(not O.ref_available(), reason="reference kernel not built (needs /root/reference)")
-def test_oracle_vs_reference_kernel_tree_lists():
+@pytest.mark.parametrize("eps", [0.0, 0.1, 0.5, 2.0])
+def test_oracle_vs_reference_kernel_tree_lists(eps):
+    """eps is the Plummer softening length in units of r_out: eps2 enters
+    every EPJ and SPJ pair as r^2 + eps^2 (src/soft_force.hpp:44-60,132-140)."""
+    eps2 = (eps * R_OUT) ** 2
     L = c1_lists(8192, seed=4)
     a = O.lists_args(L)
-    acc, pot, nnb = O.soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2, nnb_mode=1)
-    racc, rpot, rnnb = O.ref_soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2)
+    acc, pot, nnb = O.soft_force_walks(**a, eps2=eps2, rout2=R_OUT ** 2, nnb_mode=1)
+    racc, rpot, rnnb = O.ref_soft_force_walks(**a, eps2=eps2, rout2=R_OUT ** 2)
     avx512 = O.cpu_has_avx512()
-    ea = rel_err(racc, acc)
+    ea, ep = rel_err(racc, acc), rel_err(rpot, pot)
     assert np.median(ea) < (5e-5 if avx512 else 2e-4) and ea.max() < (1e-3 if avx512 else 5e-3)
-    amb = O.nnb_ambiguous(**a, centres=None, eps2=0.0)
+    assert np.median(ep) < (5e-5 if avx512 else 2e-4) and ep.max() < (1e-3 if avx512 else 5e-3)
+    amb = O.nnb_ambiguous(**a, centres=None, eps2=eps2)
     d = np.abs(rnnb - nnb)
     assert np.all(d[amb == 0] == 0) and np.all(d <= amb)
 
@@ -284,11 +289,14 @@ def test_changeover_setr_mass_scaling():
     assert a.value == pytest.approx(0.1) and b.value == pytest.approx(1.0)
 
 
-def test_changeover_correction_gives_weighted_newton():
+@pytest.mark.parametrize("eps", [0.0, 0.3, 2.0])
+def test_changeover_correction_gives_weighted_newton(eps):
     """Linear-cutoff pair + the correction of src/hard.hpp:1404-1476 =
-    (1 - W_acc) Newton for acc and -Gm (1 - W_pot)/r for the soft potential."""
+    (1 - W_acc) Newton for acc and -Gm (1 - W_pot)/r for the soft potential,
+    with r the softened distance sqrt(|d|^2 + eps^2) (eps in units of r_in)."""
     rng = np.random.default_rng(8)
-    G, r_out, r_in, eps2 = 1.0, 0.01, 0.001, 0.0
+    G, r_out, r_in = 1.0, 0.01, 0.001
+    eps2 = (eps * r_in) ** 2
     wpot, wacc, _ = _w()
     for _ in range(20):
         xi = rng.normal(size=3) * 0.01
@@ -302,7 +310,7 @@ def test_changeover_correction_gives_weighted_newton():
             np.ascontiguousarray(xi), mi, r_in, r_out, 1, 1, np.ascontiguousarray(xj), np.array([mj]),
             np.array([r_in]), np.array([r_out]), np.array([2], np.int64), eps2, r_out, G, acc, ps, pt)
         d = xi - xj
-        r = np.linalg.norm(d)
+        r = np.sqrt(d @ d + eps2)
         self_term = G * mi / r_out
         np.testing.assert_allclose(acc, -(1 - wacc(r, r_in, r_out)) * G * mj * d / r ** 3, rtol=1e-10, atol=1e-300)
         assert ps[0] - self_term == pytest.approx(-G * mj * (1 - wpot(r, r_in, r_out)) / r, rel=1e-10, abs=1e-15)
