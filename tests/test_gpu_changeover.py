@@ -23,10 +23,10 @@ MED, P99, MAX = 1e-6, 1e-5, 1e-3
 AMB_REL = 1e-4
 
 
-def _workload(n, seed, f_bin, r_out=R_OUT, eps2=0.0):
+def _workload(n, seed, f_bin, r_out=R_OUT, eps2=0.0, G=1.0):
     L, rin, rout = co_lists(n, seed=seed, f_bin=f_bin, r_out=r_out, r_in=0.1 * r_out)
     a = co_args(L, rin, rout)
-    ref = O.soft_force_walks_co(**a, eps2=eps2, rout2=r_out ** 2, G=1.0, amb_rel=AMB_REL)
+    ref = O.soft_force_walks_co(**a, eps2=eps2, rout2=r_out ** 2, G=G, amb_rel=AMB_REL)
     amb_nnb = O.nnb_ambiguous(**{k: a[k] for k in ("n_epi", "n_epj", "epi_pos", "epi_rs", "id_epj", "epj_pos",
                                                   "epj_rs")}, centres=walk_centres(a), eps2=eps2)
     return L, rin, rout, ref, amb_nnb
@@ -66,8 +66,8 @@ def _check(res, ref, amb_nnb, label):
     assert bad_exact == 0 and bad_band == 0
 
 
-def _resident(L, rin, rout, r_out=R_OUT, eps2=0.0, **kw):
-    ctx = sf.Context(0, eps2=eps2, rcut2=r_out ** 2, G=1.0, changeover=1, **kw)
+def _resident(L, rin, rout, r_out=R_OUT, eps2=0.0, G=1.0, **kw):
+    ctx = sf.Context(0, eps2=eps2, rcut2=r_out ** 2, G=G, changeover=1, **kw)
     r = sf.Resident.from_lists(ctx, L, rin, rout)
     r.run()
     res = r.results()
@@ -96,14 +96,15 @@ def test_co_resident_dense(dense):
     _check(_resident(L, rin, rout, r_out=0.03), ref, amb, "co dense")
 
 
-@pytest.mark.parametrize("eps", [0.05, 0.3])
-def test_co_softened(eps):
+@pytest.mark.parametrize("eps,G", [(0.05, 1.0), (0.3, 1.7)])
+def test_co_softened(eps, G):
     """Softening eps = 0.05 and 0.3 r_out in changeover mode: the neighbour
     correction takes K and the 1/r terms at sqrt(r^2 + eps^2)
-    (src/hard.hpp:1408-1433), the linear-cutoff part as without changeover."""
+    (src/hard.hpp:1408-1433), the linear-cutoff part as without changeover;
+    G = 1.7 scales the pair terms and the self term G m_i / r_out."""
     eps2 = (eps * 0.03) ** 2
-    L, rin, rout, ref, amb = _workload(16384, 6, 0.3, r_out=0.03, eps2=eps2)
-    _check(_resident(L, rin, rout, r_out=0.03, eps2=eps2), ref, amb, f"co softened eps={eps} r_out")
+    L, rin, rout, ref, amb = _workload(16384, 6, 0.3, r_out=0.03, eps2=eps2, G=G)
+    _check(_resident(L, rin, rout, r_out=0.03, eps2=eps2, G=G), ref, amb, f"co softened eps={eps} r_out G={G}")
 
 
 @pytest.mark.parametrize("max_ipl", [1, 2, 3, 4])
