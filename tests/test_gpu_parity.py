@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_py as O
-from petar_amd import soft_force as sf
+from petar_amd import host, soft_force as sf
 from tests._common import R_OUT, c1_lists, nnb_check, rel_err, walk_centres
 
 pytestmark = pytest.mark.gpu
@@ -161,6 +161,26 @@ def test_softened_c1(c1, eps):
         r = sf.Resident.from_lists(ctx, L)
         r.run()
         _check(r.results(), ref, amb, f"softened eps={eps} r_out exec={mode}")
+        r.close()
+        ctx.close()
+
+
+def test_translated_cluster():
+    """The cluster moved to (8000, -300, 20), a star cluster on a galactic
+    orbit in pc with r_out = 0.01 pc: absolute FP32 coordinates (spacing
+    4.9e-4 at 8000) could not resolve the cutoff, walk-relative ones (the
+    walk centre subtracted in FP64) keep the errors of the untranslated
+    cluster."""
+    m, x, _ = host.cluster(16384, "plummer", kroupa=True, f_bin=0.0, seed=12)
+    L = host.build_lists(x + np.array([8000.0, -300.0, 20.0]), m, 1.5 * R_OUT, theta=0.3)
+    a = O.lists_args(L)
+    ref = O.soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2, G=1.0, nnb_mode=0)
+    amb = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=0.0)
+    for mode in (sf.EXEC_FUSED, sf.EXEC_PLAIN):
+        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, nnb_mode=0, exec_mode=mode)
+        r = sf.Resident.from_lists(ctx, L)
+        r.run()
+        _check(r.results(), ref, amb, f"translated to x = 8000 exec={mode}")
         r.close()
         ctx.close()
 

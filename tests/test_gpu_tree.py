@@ -103,6 +103,25 @@ def test_device_lists_feed_the_force_path():
     assert np.mean(nnb != rnnb) <= 1e-3
 
 
+def test_device_tree_translated_cluster():
+    """The cluster centred at (8000, -300, 20): the device tree still equals
+    the host tree (keys from the bounding box, FP64 moments in the same
+    order) and its evaluation the host-list one bit for bit."""
+    m, x, _ = host.cluster(16384, "plummer", kroupa=True, f_bin=0.3, seed=12)
+    L, d, ctx, g, _ = _compare(16384, 12, 0.3, 0.3, keep=True, mx=(m, x + np.array([8000.0, -300.0, 20.0])))
+    rd = sf.Resident.from_gtree(ctx, g)
+    g.close()
+    rd.run()
+    res_d = rd.results()
+    rd.close()
+    rh = sf.Resident.from_lists(ctx, L)
+    rh.run()
+    res_h = rh.results()
+    rh.close()
+    ctx.close()
+    assert all(np.array_equal(u, v) for u, v in zip(res_d, res_h))
+
+
 def test_device_tree_changeover_mode():
     """Changeover mode fed from the device tree (radii and ids in tree order)
     equals the host-list evaluation bit for bit."""
