@@ -289,11 +289,15 @@ def _expected_ngb(L, ocnt, ooff, ongb):
     return cnt, off, L.order[ongb[o]]
 
 
-@pytest.mark.parametrize("n,f_bin,theta", [(30000, 0.5, 0.3), (4000, 0.3, 0.0), (20000, 0.0, 0.8)])
-def test_device_neighbors_equal_oracle(n, f_bin, theta):
+@pytest.mark.parametrize("n,f_bin,theta,shift", [(30000, 0.5, 0.3, 0.0), (4000, 0.3, 0.0, 0.0),
+                                                 (20000, 0.0, 0.8, 0.0), (30000, 0.5, 0.3, 8000.0)])
+def test_device_neighbors_equal_oracle(n, f_bin, theta, shift):
     """pth_gtree_neighbors against the FP64 restatement on the host tree's
-    lists: counts, offsets and entries equal (order included)."""
+    lists: counts, offsets and entries equal (order included); shift moves
+    the cluster to (shift, -300, 20) when nonzero."""
     m, x, _ = host.cluster(n, "plummer", kroupa=True, f_bin=f_bin, seed=41)
+    if shift:
+        x = x + np.array([shift, -300.0, 20.0])
     rs = 1.5 * R_OUT * np.maximum(np.cbrt(m / m.mean()), 1.0)
     L = host.build_lists(x, m, rs, theta=theta)
     ocnt, ooff, ongb = O.ngb_walks(L.n_epi, L.n_epj, L.epj_pos, L.epj_rs, L.id_epj, L.epj_pos, L.epj_rs)
@@ -303,7 +307,7 @@ def test_device_neighbors_equal_oracle(n, f_bin, theta):
     g.close()
     ctx.close()
     ecnt, eoff, engb = _expected_ngb(L, ocnt, ooff, ongb)
-    print(f"[ngb N={n} theta={theta}] {ngb.size} entries, {(cnt > 1).sum()} particles with neighbours, {ms:.2f} ms",
+    print(f"[ngb N={n} theta={theta} shift={shift}] {ngb.size} entries, {(cnt > 1).sum()} particles with neighbours, {ms:.2f} ms",
           flush=True)
     assert np.array_equal(cnt, ecnt) and np.array_equal(off, eoff) and np.array_equal(ngb, engb)
 
