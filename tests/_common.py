@@ -46,12 +46,13 @@ def nnb_check(nnb, nnb_ref, n_amb):
     return int((d[exact] != 0).sum()), int((d > n_amb).sum()), int((~exact).sum())
 
 
-def co_lists(n=16384, seed=1, f_bin=0.0, r_out=R_OUT, r_in=0.1 * R_OUT, theta=0.3):
+def co_lists(n=16384, seed=1, f_bin=0.0, r_out=R_OUT, r_in=0.1 * R_OUT, theta=0.3, shift=(0.0, 0.0, 0.0)):
     """A changeover-mode workload: changeover radii scaled by
     max((m/<m>)^(1/3), 1) as ChangeOver::setR does (src/changeover.hpp:44-59),
-    r_search = 1.5 r_out per particle.  Returns (lists, r_in, r_out) with the
-    radii in the lists' EPJ (tree) order."""
+    r_search = 1.5 r_out per particle, the cluster moved by shift.  Returns
+    (lists, r_in, r_out) with the radii in the lists' EPJ (tree) order."""
     m, x, _ = host.cluster(n, "plummer", kroupa=True, f_bin=f_bin, seed=seed)
+    x = x + np.asarray(shift, np.float64)
     mbar = m.mean()
     L = host.build_lists(x, m, 1.5 * r_out * np.maximum(np.cbrt(m / mbar), 1.0), theta=theta)
     fac = np.maximum(np.cbrt(L.epj_mass / mbar), 1.0)

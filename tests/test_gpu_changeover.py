@@ -23,8 +23,8 @@ MED, P99, MAX = 1e-6, 1e-5, 1e-3
 AMB_REL = 1e-4
 
 
-def _workload(n, seed, f_bin, r_out=R_OUT, eps2=0.0, G=1.0):
-    L, rin, rout = co_lists(n, seed=seed, f_bin=f_bin, r_out=r_out, r_in=0.1 * r_out)
+def _workload(n, seed, f_bin, r_out=R_OUT, eps2=0.0, G=1.0, shift=(0.0, 0.0, 0.0)):
+    L, rin, rout = co_lists(n, seed=seed, f_bin=f_bin, r_out=r_out, r_in=0.1 * r_out, shift=shift)
     a = co_args(L, rin, rout)
     ref = O.soft_force_walks_co(**a, eps2=eps2, rout2=r_out ** 2, G=G, amb_rel=AMB_REL)
     amb_nnb = O.nnb_ambiguous(**{k: a[k] for k in ("n_epi", "n_epj", "epi_pos", "epi_rs", "id_epj", "epj_pos",
@@ -105,6 +105,14 @@ def test_co_softened(eps, G):
     eps2 = (eps * 0.03) ** 2
     L, rin, rout, ref, amb = _workload(16384, 6, 0.3, r_out=0.03, eps2=eps2, G=G)
     _check(_resident(L, rin, rout, r_out=0.03, eps2=eps2, G=G), ref, amb, f"co softened eps={eps} r_out G={G}")
+
+
+def test_co_translated_binaries():
+    """Changeover mode with the binaries cluster moved to (8000, -300, 20):
+    neighbour pairs use separations taken in FP64 from the walk centre, so
+    the errors stay those of the cluster at the origin."""
+    L, rin, rout, ref, amb = _workload(16384, 4, 0.3, shift=(8000.0, -300.0, 20.0))
+    _check(_resident(L, rin, rout), ref, amb, "co binaries at x = 8000")
 
 
 @pytest.mark.parametrize("max_ipl", [1, 2, 3, 4])
