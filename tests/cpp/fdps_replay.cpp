@@ -14,8 +14,15 @@
 // max((m/<m>)^(1/3), 1) per particle (src/changeover.hpp:44-59) and the
 // retrieved ForceSoft carries pot_tot.
 //
-// usage: fdps_replay N seed theta n_walk_limit r_out out_file
+// With a timed-pass count the FDPS call sequence (send, then every batch
+// dispatched and retrieved) is repeated that many times after one untimed
+// pass, and each pass's wall time and interaction rate are printed: the
+// drop-in rate with every list crossing PCIe, from host records in and to
+// host records out.
+//
+// usage: fdps_replay N seed theta n_walk_limit r_out out_file [f_bin [timed_passes]]
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -40,10 +47,13 @@ template <class T> static void put(FILE *f, const std::vector<T> &v)
 
 int main(int argc, char **argv)
 {
-    if (argc != 7) {
-        std::fprintf(stderr, "usage: %s N seed theta n_walk_limit r_out out_file\n", argv[0]);
+    if (argc < 7 || argc > 9) {
+        std::fprintf(stderr, "usage: %s N seed theta n_walk_limit r_out out_file [f_bin [timed_passes]]\n",
+                     argv[0]);
         return 1;
     }
+    const double f_bin = argc > 7 ? std::atof(argv[7]) : 0.3;
+    const int timed_passes = argc > 8 ? std::atoi(argv[8]) : 0;
     const long n = std::atol(argv[1]);
     const unsigned seed = (unsigned)std::atoi(argv[2]);
     const double theta = std::atof(argv[3]);
@@ -52,7 +62,7 @@ int main(int argc, char **argv)
     const double r_search = 1.5 * r_out;
 
     std::vector<double> mass(n), pos(3 * n);
-    if (pth_ic_cluster(n, 0, 0.0, 1, 0.3, 1e-5, 1e-2, 0, seed, mass.data(), pos.data(), nullptr)) die("ic");
+    if (pth_ic_cluster(n, 0, 0.0, 1, f_bin, 1e-5, 1e-2, 0, seed, mass.data(), pos.data(), nullptr)) die("ic");
 #ifdef PETAR_HIP_CHANGEOVER
     double mbar = 0.0;
     for (double m : mass) mbar += m;
@@ -130,24 +140,37 @@ int main(int argc, char **argv)
 
     // FDPS multi-walk call sequence
     const PS::F64 eps2 = 0.0, rout2 = r_out * r_out, G = 1.0;
-    CalcForceWithLinearCutoffCUDAMultiWalk f(0, eps2, rout2, G);
-    f(0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, epj.data(), (PS::S32)n, spj.data(),
-      (PS::S32)n_node, true);
-    for (long g0 = 0; g0 < n_group; g0 += n_walk_limit) {
-        const int nw = (int)std::min<long>(n_walk_limit, n_group - g0);
-        std::vector<const EPISoft *> pe(nw);
-        std::vector<const PS::S32 *> pj(nw), ps(nw);
-        std::vector<ForceSoft *> pf(nw);
-        for (int w = 0; w < nw; w++) {
-            pe[w] = epi[g0 + w].data();
-            pj[w] = id_epj.data() + eoff[g0 + w];
-            ps[w] = id_spj.data() + soff[g0 + w];
-            pf[w] = force[g0 + w].data();
+    auto sequence = [&]() {
+        CalcForceWithLinearCutoffCUDAMultiWalk f(0, eps2, rout2, G);
+        f(0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, epj.data(), (PS::S32)n, spj.data(),
+          (PS::S32)n_node, true);
+        for (long g0 = 0; g0 < n_group; g0 += n_walk_limit) {
+            const int nw = (int)std::min<long>(n_walk_limit, n_group - g0);
+            std::vector<const EPISoft *> pe(nw);
+            std::vector<const PS::S32 *> pj(nw), ps(nw);
+            std::vector<ForceSoft *> pf(nw);
+            for (int w = 0; w < nw; w++) {
+                pe[w] = epi[g0 + w].data();
+                pj[w] = id_epj.data() + eoff[g0 + w];
+                ps[w] = id_spj.data() + soff[g0 + w];
+                pf[w] = force[g0 + w].data();
+            }
+            CalcForceWithLinearCutoffCUDAMultiWalk fw(0, eps2, rout2, G);
+            fw(0, nw, pe.data(), n_epi.data() + g0, pj.data(), n_epj.data() + g0, ps.data(), n_spj.data() + g0,
+               epj.data(), (PS::S32)n, spj.data(), (PS::S32)n_node, false);
+            RetrieveForceCUDA(0, nw, n_epi.data() + g0, pf.data());
         }
-        CalcForceWithLinearCutoffCUDAMultiWalk fw(0, eps2, rout2, G);
-        fw(0, nw, pe.data(), n_epi.data() + g0, pj.data(), n_epj.data() + g0, ps.data(), n_spj.data() + g0,
-           epj.data(), (PS::S32)n, spj.data(), (PS::S32)n_node, false);
-        RetrieveForceCUDA(0, nw, n_epi.data() + g0, pf.data());
+    };
+    sequence();
+    double inter = 0.0;
+    for (long g = 0; g < n_group; g++) inter += (double)n_epi[g] * ((double)n_epj[g] + (double)n_spj[g]);
+    for (int k = 0; k < timed_passes; k++) {
+        const auto t0 = std::chrono::steady_clock::now();
+        sequence();
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("fdps_replay: pass %d: %.3f ms, %.2f Gint/s (%.4g interactions, %ld walks in batches of %d)\n",
+                    k, ms, inter / (ms * 1e6), inter, n_group, n_walk_limit);
+        std::fflush(stdout);
     }
 
     std::vector<double> acc, pot, pot_tot;
