@@ -75,7 +75,7 @@ typedef struct pth_params {
     double G;      /* ForceSoft::grav_const */
     int nnb_mode;  /* 0: r2+eps2 < rs^2 (GPU reference), 1: <= (x86 reference) */
     int exec_mode; /* PTH_EXEC_FUSED or PTH_EXEC_PLAIN */
-    int chunk_j;   /* fused form: max j-entries per work item (0 = default) */
+    int chunk_j;   /* fused form: max j-entries per work item (0 = default, 1024) */
     int max_walk;  /* largest n_walk accepted by pth_dispatch (0 = 1000, the
                       reference's N_WALK_LIMIT, src/force_gpu_cuda.cu:14) */
     int max_ipl;   /* fused form: most i-particles per lane, 1 to 4 (tiles of 64, 128,

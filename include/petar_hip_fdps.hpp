@@ -61,6 +61,9 @@ inline pth_ctx *context(PS::S32 rank, PS::F64 eps2, PS::F64 rcut2, PS::F64 G)
 #ifdef PETAR_HIP_CHANGEOVER
     p.changeover = 1;
 #endif
+#ifdef PETAR_HIP_CHUNK_J
+    p.chunk_j = PETAR_HIP_CHUNK_J;  // list entries per work item (0: the library default)
+#endif
     if (!c) {
         if (pth_create(&c, rank, &p) != PTH_OK) fail("pth_create", nullptr);
     } else if (pth_set_params(c, &p) != PTH_OK) {

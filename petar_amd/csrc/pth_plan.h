@@ -11,7 +11,11 @@ namespace pth {
 
 constexpr int XCDS = 8;             // MI355X: 8 XCDs, workgroup b runs on XCD b % 8
 constexpr int ITEMS_PER_BLOCK = 4;  // one item per wave, 256-thread workgroups
-constexpr int DEFAULT_CHUNK_J = 2048;
+// Measured on the precision workload (N = 1M, 7,110 walks): the resident
+// autotuner picks 1024, and FDPS's 200-walk dispatch batches run 40 ms per
+// step at 512 or 1024 against 49 ms at 2048 (one batch at 2048 holds fewer
+// waves than the chip has slots, so each launch is mostly its tail).
+constexpr int DEFAULT_CHUNK_J = 1024;
 
 // Issue cost of one pair per lane, in units of one full-rate (4-cycle)
 // wave64 VALU instruction, for a tile holding ipl i-particles per lane:
