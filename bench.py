@@ -1,14 +1,15 @@
 """Benchmark of the soft-force path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
 
 A step is one complete soft-force evaluation (all walks of the force call:
 staging of the lists + force kernel + reduction of split walks) over inputs
-already resident in HBM.  The default workload is the precision
-configuration: Plummer N = 1,048,576 with 50% primordial binaries (Kroupa
+already resident in HBM.  The default workload is the largest single-GPU
+configuration: Plummer N = 8,388,608 with 100% primordial binaries (Kroupa
 masses, log-uniform a in [1e-5, 1e-2], thermal e), theta = 0.3,
-n_leaf_limit 20, n_group_limit 512, quadrupole SPJ, r_out = 0.01,
-r_in = 0.001, r_search = 0.015, G = 1, eps = 0.
+n_leaf_limit 20, n_group_limit 512, quadrupole SPJ, r_out = 0.003,
+r_in = 0.0003, r_search = 0.0045, G = 1, eps = 0.  The serving object
+always uses the 256-cluster stream (r_out = 0.01).
 
 Metric: pair interactions per second, I = N_e + N_s = sum over walks of
 n_i (n_epj + n_spj).  Flop convention (reference formulas, FMA = 2, rsqrt =
@@ -236,26 +237,69 @@ def read_traffic(name):
     return None
 
 
+def host_cpu():
+    """Model, logical CPUs and SMT of the host (from /proc/cpuinfo)."""
+    model, siblings, cores = "unknown", None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "siblings" and siblings is None:
+                    siblings = int(v)
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+    except OSError:
+        pass
+    smt = None if not (siblings and cores) else siblings // cores
+    return dict(model=model, logical_cpus=os.cpu_count(), smt_threads_per_core=smt,
+                affinity_cpus=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None)
+
+
 def cpu_baseline(args_, r_out, target_s):
     """Reference x86 kernel (oracle/_ref, built from the reference source) on
-    the host cores, same lists; repeated until target_s of CPU work."""
+    the host cores, same lists; repeated until target_s of CPU work.  Returns
+    the baseline object and the kernel's outputs of the last run."""
     from oracle import oracle_py as O
     if not O.ref_available():
-        return None
+        return None, None
     nt = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     isa = "avx512" if O.cpu_has_avx512() else "avx2"
     n_int = int(np.dot(args_["n_epi"].astype(np.int64), (args_["n_epj"] + args_["n_spj"]).astype(np.int64)))
     O.ref_soft_force_walks(**args_, eps2=0.0, rout2=r_out ** 2, nthreads=nt, isa=isa)  # warm-up
     ts = []
     t_start = time.time()
+    outs = None
     while time.time() - t_start < target_s or len(ts) < 3:
         t = time.time()
-        O.ref_soft_force_walks(**args_, eps2=0.0, rout2=r_out ** 2, nthreads=nt, isa=isa)
+        outs = O.ref_soft_force_walks(**args_, eps2=0.0, rout2=r_out ** 2, nthreads=nt, isa=isa)
         ts.append(time.time() - t)
     med = float(np.median(ts))
+    cpu = host_cpu()
     return dict(value=round(n_int / med / 1e9, 3), unit="Gint/s", cores=nt, kind="reference",
                 sample=f"full workload lists ({n_int:.3e} interactions), median of {len(ts)} runs, "
-                       f"phantomquad {isa} -DRSQRT_NR_EPJ_X2, {nt} OpenMP threads")
+                       f"phantomquad {isa} -DRSQRT_NR_EPJ_X2, {nt} OpenMP threads",
+                cpu_model=cpu["model"], logical_cpus=cpu["logical_cpus"], affinity_cpus=cpu["affinity_cpus"],
+                smt_threads_per_core=cpu["smt_threads_per_core"], isa=isa), outs
+
+
+def vs_reference_kernel(res, ref_out, L):
+    """The GPU evaluation against the reference x86 kernel's outputs on the
+    same lists (every i): relative acc/pot errors and the neighbour counts,
+    mismatches classified against the FP32 bands of both (the GPU's
+    walk-relative coordinates and the x86 kernel's absolute ones; the x86
+    kernel counts r^2 <= rs^2, the GPU r^2 < rs^2)."""
+    acc, pot, nnb = res
+    racc, rpot, rnnb = ref_out
+    ea = np.linalg.norm(acc - racc, axis=1) / np.linalg.norm(racc, axis=1)
+    ep = np.abs(pot - rpot) / np.abs(rpot)
+    out = dict(i=int(ea.size), acc_rel_median=float(np.median(ea)), acc_rel_p99=float(np.percentile(ea, 99)),
+               acc_rel_max=float(ea.max()), pot_rel_median=float(np.median(ep)),
+               pot_rel_p99=float(np.percentile(ep, 99)), pot_rel_max=float(ep.max()))
+    out.update(nnb_band(nnb, rnnb, L, absolute=True))
+    return out
 
 
 def cpu_baseline_neighbors(L):
@@ -274,27 +318,85 @@ def cpu_baseline_neighbors(L):
                        f"one run, {nt} OpenMP threads")
 
 
+def walk_centres(n_epi, epi_pos):
+    """Bounding-box centre of each walk's i (the centre the kernels subtract)."""
+    off = np.concatenate([[0], np.cumsum(n_epi)])
+    c = np.zeros((len(n_epi), 3))
+    for w in range(len(n_epi)):
+        if n_epi[w]:
+            p = epi_pos[off[w]:off[w + 1]]
+            c[w] = 0.5 * (p.min(0) + p.max(0))
+    return c
+
+
+def walk_subset(L, walks):
+    """Oracle arguments of a subset of the walks of L, and the rows of their
+    i in the evaluation's output (walk order)."""
+    eoff, soff = L.walk_offsets()
+    ioff = np.concatenate([[0], np.cumsum(L.n_epi)])
+    idx = np.concatenate([np.arange(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w]) for w in walks])
+    rows = np.concatenate([np.arange(ioff[w], ioff[w + 1]) for w in walks])
+    a = dict(n_epi=L.n_epi[walks], n_epj=L.n_epj[walks], n_spj=L.n_spj[walks], epi_pos=L.epj_pos[idx],
+             epi_rs=L.epj_rs[idx],
+             id_epj=np.concatenate([L.id_epj[eoff[w]:eoff[w + 1]] for w in walks]),
+             id_spj=np.concatenate([L.id_spj[soff[w]:soff[w + 1]] for w in walks]),
+             epj_pos=L.epj_pos, epj_mass=L.epj_mass, epj_rs=L.epj_rs, spj_pos=L.spj_pos, spj_mass=L.spj_mass,
+             spj_quad=L.spj_quad)
+    return a, rows
+
+
+def nnb_band(nnb, nnb_ref, L, rows_all=None, absolute=False):
+    """Neighbour counts against a reference, with the pairs that lie within
+    FP32 rounding of the search radius counted: the walks holding a
+    mismatching i are searched for such pairs (the GPU's walk-relative FP32
+    band; with absolute, also the band of FP32 absolute coordinates, the
+    reference x86 kernel's), and every mismatch is classified as inside
+    (|d| <= pairs in the band) or outside it (a wrong count)."""
+    from oracle import oracle_py as O
+    d = np.abs(np.asarray(nnb) - np.asarray(nnb_ref))
+    bad = np.nonzero(d)[0]
+    out = dict(nnb_mismatch=int(bad.size), nnb_mismatch_in_band=0, nnb_mismatch_outside_band=0)
+    if bad.size == 0:
+        return out
+    ioff = np.concatenate([[0], np.cumsum(L.n_epi)])
+    rows_all = np.arange(ioff[-1]) if rows_all is None else rows_all
+    walks = np.unique(np.searchsorted(ioff, rows_all[bad], side="right") - 1)
+    a, rows = walk_subset(L, walks)
+    ka = {k: a[k] for k in ("n_epi", "n_epj", "epi_pos", "epi_rs", "id_epj", "epj_pos", "epj_rs")}
+    amb = O.nnb_ambiguous(**ka, centres=walk_centres(a["n_epi"], a["epi_pos"]), eps2=0.0)
+    if absolute:
+        amb = amb + O.nnb_ambiguous(**ka, centres=None, eps2=0.0)
+    pos = {r: k for k, r in enumerate(rows)}
+    for b in bad:
+        k = pos[int(rows_all[b])]
+        if d[b] <= amb[k]:
+            out["nnb_mismatch_in_band"] += 1
+        else:
+            out["nnb_mismatch_outside_band"] += 1
+    return out
+
+
 def accuracy(res_acc, res_pot, res_nnb, L, r_out, frac=0.05, seed=0):
-    """Error vs the FP64 oracle on the same lists, on a sample of walks."""
+    """Error vs the FP64 oracle on the same lists, on a sample of walks;
+    neighbour counts compared exactly, mismatches classified against the
+    pairs within FP32 rounding of the search radius (counted over the
+    sample)."""
     from oracle import oracle_py as O
     rng = np.random.default_rng(seed)
     w_sel = np.sort(rng.choice(L.n_walk, max(1, int(frac * L.n_walk)), replace=False))
-    eoff, soff = L.walk_offsets()
-    ioff = np.concatenate([[0], np.cumsum(L.n_epi)])
-    idx = np.concatenate([np.arange(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w]) for w in w_sel])
-    rows = np.concatenate([np.arange(ioff[w], ioff[w + 1]) for w in w_sel])
-    a = dict(n_epi=L.n_epi[w_sel], n_epj=L.n_epj[w_sel], n_spj=L.n_spj[w_sel], epi_pos=L.epj_pos[idx],
-             epi_rs=L.epj_rs[idx],
-             id_epj=np.concatenate([L.id_epj[eoff[w]:eoff[w + 1]] for w in w_sel]),
-             id_spj=np.concatenate([L.id_spj[soff[w]:soff[w + 1]] for w in w_sel]),
-             epj_pos=L.epj_pos, epj_mass=L.epj_mass, epj_rs=L.epj_rs, spj_pos=L.spj_pos, spj_mass=L.spj_mass,
-             spj_quad=L.spj_quad)
+    a, rows = walk_subset(L, w_sel)
     racc, rpot, rnnb = O.soft_force_walks(**a, eps2=0.0, rout2=r_out ** 2)
     ea = np.linalg.norm(res_acc[rows] - racc, axis=1) / np.linalg.norm(racc, axis=1)
     ep = np.abs(res_pot[rows] - rpot) / np.abs(rpot)
-    return dict(sample_i=int(rows.size), acc_rel_median=float(np.median(ea)), acc_rel_p99=float(np.percentile(ea, 99)),
-                acc_rel_max=float(ea.max()), pot_rel_median=float(np.median(ep)),
-                pot_rel_p99=float(np.percentile(ep, 99)), nnb_mismatch=int((res_nnb[rows] != rnnb).sum()))
+    ka = {k: a[k] for k in ("n_epi", "n_epj", "epi_pos", "epi_rs", "id_epj", "epj_pos", "epj_rs")}
+    amb = O.nnb_ambiguous(**ka, centres=walk_centres(a["n_epi"], a["epi_pos"]), eps2=0.0)
+    d = np.abs(res_nnb[rows] - rnnb)
+    return dict(sample_i=int(rows.size), sample_walks=int(w_sel.size), acc_rel_median=float(np.median(ea)),
+                acc_rel_p99=float(np.percentile(ea, 99)), acc_rel_max=float(ea.max()),
+                pot_rel_median=float(np.median(ep)), pot_rel_p99=float(np.percentile(ep, 99)),
+                pot_rel_max=float(ep.max()), nnb_mismatch=int((d != 0).sum()),
+                nnb_mismatch_outside_band=int((d > amb).sum()), nnb_mismatch_in_band=int(((d != 0) & (d <= amb)).sum()),
+                nnb_pairs_in_fp32_band=int(amb.sum()), nnb_i_with_band_pairs=int((amb > 0).sum()))
 
 
 def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0):
@@ -362,7 +464,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--exec", default="fused", choices=["fused", "plain"])
     ap.add_argument("--chunk-j", type=int, default=0)
     ap.add_argument("--max-ipl", type=int, default=2, choices=[1, 2, 3, 4])
@@ -375,11 +477,13 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--segments", type=int, default=8)
     ap.add_argument("--streams", type=int, default=3)
-    ap.add_argument("--direct-sample", type=int, default=4096,
+    ap.add_argument("--direct-sample", type=int, default=65536,
                     help="i sampled for the error against the FP64 all-pairs direct sum (0: skip)")
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--no-device-tree", action="store_true")
     ap.add_argument("--no-changeover", action="store_true")
+    ap.add_argument("--df-near", type=int, default=1, choices=[0, 1],
+                    help="near EPJ pairs in double-float walk-relative coordinates (linear-cutoff mode)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the soft-force path is "
@@ -395,7 +499,7 @@ def main():
     import torch  # device-wide synchronisation around the timed region
     exec_mode = sf.EXEC_FUSED if a.exec == "fused" else sf.EXEC_PLAIN
     ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=a.chunk_j,
-                     max_ipl=a.max_ipl, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams)
+                     max_ipl=a.max_ipl, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams, df_near=a.df_near)
     res = sf.Resident.from_lists(ctx, L)
     tuned = None
     if a.exec == "fused" and not a.no_autotune:
@@ -430,7 +534,7 @@ def main():
     ipl_used = tuned["max_ipl"] if tuned else a.max_ipl
     cj_used = tuned["chunk_j"] if tuned else a.chunk_j
     ctx1 = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, exec_mode=exec_mode, chunk_j=cj_used, max_ipl=ipl_used,
-                      j_vgpr=a.j_vgpr, segments=1)
+                      j_vgpr=a.j_vgpr, segments=1, df_near=a.df_near)
     r1 = sf.Resident.from_lists(ctx1, L)
     r1.run()
     k_force, k_stage, k_red = [], [], []
@@ -463,7 +567,7 @@ def main():
         "config": {"workload": a.workload, "description": desc, "exec": a.exec,
                    "interactions_per_step": n_int, "n_epep": ne, "n_epsp": ns, "walks": L.n_walk,
                    "flops_per_step": flops, "flop_convention": "26/EPJ pair + 65/SPJ pair"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": P32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": P32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / P32_PEAK_TFLOPS, 4), "traffic": traffic,
                      "kernel": "k_force (FP32 vector ALU, no MFMA); all class launches of one evaluation, "
                                "unsegmented, HIP events on their stream",
@@ -473,10 +577,15 @@ def main():
                        "stage_ms": round(float(np.mean(k_stage)), 4), "reduce_ms": round(float(np.mean(k_red)), 4),
                        "unsegmented_step_ms": round(unseg_ms, 4)},
         "segments": {"segments": a.segments, "streams": a.streams},
+        "df_near": a.df_near,
         "prep": prep,
         "autotuned": tuned,
     }
     acc, pot, nnb = res.results()
+    out["config"]["flop_convention_note"] = (
+        "EPJ pair 26 (src/force_gpu_cuda.cu:57-74): dx 3 sub, r2 3 FMA (6), search radius max + square + "
+        "compare (3), cutoff max (1), rsqrt (1), m rinv (1), G rinv rinv pij (3), acc 3 FMA (6), pot G pij + sub "
+        "(2); SPJ pair 65 (quadrupole, :283-312), FMA = 2")
     if not a.no_accuracy:
         out["accuracy_vs_fp64_same_lists"] = accuracy(acc, pot, nnb, L, r_out)
         if a.direct_sample > 0:
@@ -489,7 +598,8 @@ def main():
         # same tiles and segments as the timed evaluation, so the forces can
         # be compared bit for bit
         ctxd = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=cj_used,
-                          max_ipl=ipl_used, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams)
+                          max_ipl=ipl_used, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams,
+                          df_near=a.df_near)
         out["device_tree_step"] = device_tree_step(ctxd, particles, r_out, L, (acc, pot, nnb))
         ctxd.close()
         out["device_tree_step"]["host_tree_walk_s"] = prep["tree_walk_s"]
@@ -508,7 +618,9 @@ def main():
         ctxp.close()
     ctx.close()
     if not a.no_serving:
-        out["serving"] = serving(r_out, n_streams=a.serve_streams)
+        # the 256-cluster stream has its own definition (r_out = 0.01 as in
+        # the precision configuration), whatever the headline workload
+        out["serving"] = serving(0.01, n_streams=a.serve_streams)
         sv = out["serving"]
         p80 = [x for x in sv["load_sweep"] if x["load"] == 0.8]
         if p80:
@@ -516,7 +628,10 @@ def main():
             out["latency_ms"]["serving_p99_at_80pct_load"] = p80[0]["p99_ms"]
     if not a.no_cpu:
         from oracle import oracle_py as O
-        out["cpu_baseline"] = cpu_baseline(O.lists_args(L), r_out, a.cpu_seconds)
+        out["cpu_baseline"], ref_out = cpu_baseline(O.lists_args(L), r_out, a.cpu_seconds)
+        if ref_out is not None:
+            out["error_vs_reference_cpu_kernel"] = vs_reference_kernel((acc, pot, nnb), ref_out, L)
+            log(f"[bench] vs reference x86 kernel: {out['error_vs_reference_cpu_kernel']}")
         if "device_tree_step" in out:
             out["device_tree_step"]["neighbor_search"]["cpu_baseline"] = cpu_baseline_neighbors(L)
     print(json.dumps(out), flush=True)

@@ -95,6 +95,22 @@ typedef struct pth_params {
                       it.  pth_dispatch always uses one. */
     int streams;   /* resident evaluations: streams of the pipeline (1..8):
                       one staging, streams - 1 force streams */
+    int df_near;   /* fused form, linear cutoff: 1 computes the separation of
+                      every EPJ pair of a walk's near region (pairs that may
+                      lie within max(r_search, r_out) of each other) from
+                      double-float walk-relative coordinates (the FP32
+                      coordinates plus their staged low-order parts), so a
+                      tight pair keeps FP32 relative precision however far it
+                      lies from the walk centre; 0 uses the FP32 coordinates
+                      alone.  Changeover mode always does this for its
+                      neighbour pairs. */
+    int spj_monopole; /* 0: quadrupole superparticles (SPJSoft =
+                      PS::SPJQuadrupoleInAndOut, PeTar built with USE_QUAD);
+                      1: monopole superparticles (PS::SPJMonopoleInAndOut, a
+                      build configured with --disable-quad): mass and centre
+                      only, the pair of src/soft_force.hpp:125-158 (no cutoff,
+                      no neighbour count); the quadrupole members of the SPJ
+                      view and arrays are not read */
 } pth_params;
 
 /* Changeover mode folds PeTar's changeover correction of the soft force
@@ -113,16 +129,45 @@ typedef struct pth_params {
 
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
  * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99,
- * segments 8, streams 3. */
+ * segments 8, streams 3, df_near 1, spj_monopole 0. */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */
 int pth_create(pth_ctx **ctx, int rank, const pth_params *params);
-/* Changes eps2, rcut2, G, nnb_mode, changeover and ngb_factor of a context
- * between force calls (the reference functor receives its parameters at every
- * construction, src/force_gpu_cuda.hpp:108-118); fails while a batch awaits
- * retrieval.  Turning changeover on takes effect from the next pth_send_j. */
+/* Replaces every parameter of a context (the reference functor receives its
+ * parameters at every construction, src/force_gpu_cuda.hpp:108-118).  Values
+ * are read when a batch is dispatched, so batches already in flight keep the
+ * values they were dispatched with; turning changeover on takes effect from
+ * the next pth_send_j.  To change a few fields, read the others with
+ * pth_get_params first. */
 int pth_set_params(pth_ctx *ctx, const pth_params *params);
+/* The context's current parameters (after the clamping pth_create and
+ * pth_set_params apply). */
+int pth_get_params(const pth_ctx *ctx, pth_params *params);
+/* ---- profile of the FDPS hooks -------------------------------------------
+ * What a reference GPU build compiled with GPU_PROFILE collects
+ * (GPUProfile / GPUCounter, src/force_gpu_cuda.hpp:6-93, filled at
+ * src/force_gpu_cuda.cu:578-697,836-863 and written to the profile output by
+ * src/petar.hpp:2081-2155): times of the host packing, the copies and the
+ * kernels of pth_send_j / pth_dispatch / pth_retrieve, and the batch
+ * counts. */
+typedef struct pth_profile {
+    double copy_s;  /* host wall time packing records (send and dispatch) and
+                       assigning results (retrieve) */
+    double send_s;  /* host-to-device copies of send and dispatch (HIP events) */
+    double calc_s;  /* the kernels of each batch (HIP events) */
+    double recv_s;  /* the device-to-host copy of each batch's results */
+    long long n_walk, n_epi, n_epj, n_spj, n_call;  /* dispatched walks, i,
+                       EPJ and SPJ list entries, dispatch calls */
+} pth_profile;
+/* Turns the profile on (on != 0: timing events are recorded from the next
+ * call) or off. */
+int pth_profile_enable(pth_ctx *ctx, int on);
+/* Returns what was accumulated since the last take (device times of
+ * batches count once they have been retrieved) and starts again from
+ * zero. */
+int pth_profile_take(pth_ctx *ctx, pth_profile *out);
+
 /* Text of the last pth_create failure of the calling thread. */
 const char *pth_create_error(void);
 /* Frees the context.  Its resident evaluations, trees and servers use its

@@ -101,6 +101,23 @@ static inline void spj_pair(const double *xi, const double *xs, double m, const 
     *p -= m * rinv - 0.5 * tr * rinv3 + qrr_r5;
 }
 
+/* One monopole SPJ pair in FP64, src/soft_force.hpp:125-158
+   (CalcForceEpSpMonoNoSimd; rij = xi - xs, no cutoff, no neighbour count). */
+static inline void spj_mono_pair(const double *xi, const double *xs, double m, double eps2,
+                                 double *ax, double *ay, double *az, double *p)
+{
+    double rx = xi[0] - xs[0], ry = xi[1] - xs[1], rz = xi[2] - xs[2];
+    double r3_inv = rx * rx + ry * ry + rz * rz + eps2;
+    double r_inv = 1.0 / sqrt(r3_inv);
+    r3_inv = r_inv * r_inv;
+    r_inv *= m;
+    r3_inv *= r_inv;
+    *ax -= r3_inv * rx;
+    *ay -= r3_inv * ry;
+    *az -= r3_inv * rz;
+    *p -= r_inv;
+}
+
 static void prefix(int n, const int *cnt, long long *off)
 {
     off[0] = 0;
@@ -158,8 +175,11 @@ int orc_spj_walks(int n_walk, const int *n_epi, const int *n_spj,
             double ax = 0, ay = 0, az = 0, p = 0;
             for (long long k = so[w]; k < so[w + 1]; k++) {
                 long long s = id_spj[k];
-                spj_pair(epi_pos + 3 * i, spj_pos + 3 * s, spj_mass[s], spj_quad + 6 * s, eps2,
-                         &ax, &ay, &az, &p);
+                if (spj_quad)
+                    spj_pair(epi_pos + 3 * i, spj_pos + 3 * s, spj_mass[s], spj_quad + 6 * s, eps2,
+                             &ax, &ay, &az, &p);
+                else
+                    spj_mono_pair(epi_pos + 3 * i, spj_pos + 3 * s, spj_mass[s], eps2, &ax, &ay, &az, &p);
             }
             acc[3 * i + 0] = G * ax;
             acc[3 * i + 1] = G * ay;
@@ -245,25 +265,102 @@ int orc_nnb_ambiguous_walks(int n_walk, const int *n_epi, const int *n_epj,
     return 0;
 }
 
+/* One row of the all-pairs sum: every j against particle i.  The pair is
+   epj_pair's arithmetic term for term (same order, no contraction); only the
+   summation over j is split into vector-lane partial sums (an OpenMP simd
+   reduction, which permits exactly that reordering and no other).  Cloned
+   for AVX-512, AVX2 and plain x86-64, picked at load time. */
+static inline __attribute__((always_inline)) void direct_row_body(long long n, const double *x, const double *y, const double *z,
+                                   const double *m, const double *rsj, double xi, double yi, double zi,
+                                   double rsi, double eps2, double rout2, const int mode, double out[4],
+                                   long long *nb_out)
+{
+    double ax = 0, ay = 0, az = 0, p = 0;
+    long long nb = 0;
+#pragma omp simd reduction(+ : ax, ay, az, p, nb)
+    for (long long j = 0; j < n; j++) {
+        const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+        const double r2raw = dx * dx + dy * dy + dz * dz;
+        const double r2 = eps2 + r2raw;
+        const double rs = rsi > rsj[j] ? rsi : rsj[j];
+        const double rs2 = rs * rs;
+        nb += (mode == 1) ? (r2 <= rs2) : (mode == 2) ? (r2raw < rs2) : (r2 < rs2);
+        const double r2c = (r2 > rout2) ? r2 : rout2;
+        const double rinv = 1.0 / sqrt(r2c);
+        const double pij = m[j] * rinv;
+        const double mri3 = rinv * rinv * pij;
+        ax += mri3 * dx;
+        ay += mri3 * dy;
+        az += mri3 * dz;
+        p -= pij;
+    }
+    out[0] = ax;
+    out[1] = ay;
+    out[2] = az;
+    out[3] = p;
+    *nb_out = nb;
+}
+
+__attribute__((target_clones("avx512f", "avx2", "default")))
+static void direct_row(long long n, const double *x, const double *y, const double *z, const double *m,
+                       const double *rsj, double xi, double yi, double zi, double rsi, double eps2,
+                       double rout2, int mode, double out[4], long long *nb_out)
+{
+    if (mode == 1) direct_row_body(n, x, y, z, m, rsj, xi, yi, zi, rsi, eps2, rout2, 1, out, nb_out);
+    else if (mode == 2) direct_row_body(n, x, y, z, m, rsj, xi, yi, zi, rsi, eps2, rout2, 2, out, nb_out);
+    else direct_row_body(n, x, y, z, m, rsj, xi, yi, zi, rsi, eps2, rout2, 0, out, nb_out);
+}
+
 int orc_direct_linear_cutoff(int n, const double *pos, const double *mass, const double *rs,
                              int n_sample, const int *idx_sample,
                              double eps2, double rout2, double G, int nnb_mode,
                              double *acc, double *pot, long long *nnb)
 {
-#pragma omp parallel for schedule(dynamic, 16)
-    for (int s = 0; s < n_sample; s++) {
-        long long i = idx_sample ? idx_sample[s] : s;
-        double ax = 0, ay = 0, az = 0, p = 0;
-        long long nb = 0;
-        for (long long j = 0; j < n; j++)
-            epj_pair(pos + 3 * i, rs[i], pos + 3 * j, mass[j], rs[j], eps2, rout2, nnb_mode,
-                     &ax, &ay, &az, &p, &nb);
-        acc[3 * s + 0] = G * ax;
-        acc[3 * s + 1] = G * ay;
-        acc[3 * s + 2] = G * az;
-        pot[s] = G * p;
-        nnb[s] = nb;
+    /* coordinates as three planes, so each partial sum reads unit-stride */
+    double *xs = malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    if (!xs) return -1;
+    double *ys = xs + n, *zs = ys + n;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; j++) {
+        xs[j] = pos[3 * (size_t)j];
+        ys[j] = pos[3 * (size_t)j + 1];
+        zs[j] = pos[3 * (size_t)j + 2];
     }
+    /* tiles of DS_TI sampled rows against chunks of DS_JB j (which stay in
+       the core's L2 while the tile's rows pass over them): each row's sum
+       runs over the chunks in order */
+    enum { DS_TI = 16, DS_JB = 4096 };
+    const int n_tile = (n_sample + DS_TI - 1) / DS_TI;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < n_tile; t++) {
+        const int s0 = t * DS_TI, s1 = (s0 + DS_TI < n_sample) ? s0 + DS_TI : n_sample;
+        double A[DS_TI][4];
+        long long NB[DS_TI];
+        for (int s = s0; s < s1; s++) {
+            A[s - s0][0] = A[s - s0][1] = A[s - s0][2] = A[s - s0][3] = 0.0;
+            NB[s - s0] = 0;
+        }
+        for (long long jb = 0; jb < n; jb += DS_JB) {
+            const long long nj = (n - jb < DS_JB) ? n - jb : DS_JB;
+            for (int s = s0; s < s1; s++) {
+                const long long i = idx_sample ? idx_sample[s] : s;
+                double r[4];
+                long long nb = 0;
+                direct_row(nj, xs + jb, ys + jb, zs + jb, mass + jb, rs + jb, xs[i], ys[i], zs[i], rs[i], eps2,
+                           rout2, nnb_mode, r, &nb);
+                for (int d = 0; d < 4; d++) A[s - s0][d] += r[d];
+                NB[s - s0] += nb;
+            }
+        }
+        for (int s = s0; s < s1; s++) {
+            acc[3 * s + 0] = G * A[s - s0][0];
+            acc[3 * s + 1] = G * A[s - s0][1];
+            acc[3 * s + 2] = G * A[s - s0][2];
+            pot[s] = G * A[s - s0][3];
+            nnb[s] = NB[s - s0];
+        }
+    }
+    free(xs);
     return 0;
 }
 

@@ -29,7 +29,9 @@
 extern "C" {
 #endif
 
-/* EPJ (linear cutoff) + quadrupole SPJ force over a set of walks. */
+/* EPJ (linear cutoff) + SPJ force over a set of walks: quadrupole SPJ
+ * (src/force_gpu_cuda.cu:276-323, src/soft_force.hpp:160-200), or with
+ * spj_quad == NULL the monopole SPJ (src/soft_force.hpp:125-158). */
 int orc_soft_force_walks(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
                          const double *epi_pos, const double *epi_rs,
                          const int *id_epj, const int *id_spj,
@@ -45,7 +47,8 @@ int orc_epj_walks(int n_walk, const int *n_epi, const int *n_epj,
                   double eps2, double rout2, double G, int nnb_mode,
                   double *acc, double *pot, long long *nnb);
 
-/* SPJ part only (acc/pot of the quadrupole superparticle list). */
+/* SPJ part only (acc/pot of the superparticle list; spj_quad == NULL:
+ * monopole). */
 int orc_spj_walks(int n_walk, const int *n_epi, const int *n_spj,
                   const double *epi_pos, const int *id_spj,
                   const double *spj_pos, const double *spj_mass, const double *spj_quad,

@@ -29,11 +29,12 @@ def orc():
             raise RuntimeError(f"{path} missing: run `make -C oracle` or __graft_entry__.build()")
         L = C.CDLL(path)
         wl = [C.c_int, _ip, _ip, _ip]
-        L.orc_soft_force_walks.argtypes = wl + [_dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+        # spj_quad is a raw pointer: NULL selects the monopole SPJ pair
+        L.orc_soft_force_walks.argtypes = wl + [_dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _dp, C.c_void_p,
                                                 C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, _lp]
         L.orc_epj_walks.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _ip, _dp, _dp, _dp, C.c_double,
                                     C.c_double, C.c_double, C.c_int, _dp, _dp, _lp]
-        L.orc_spj_walks.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _dp, _dp, _dp, C.c_double,
+        L.orc_spj_walks.argtypes = [C.c_int, _ip, _ip, _dp, _ip, _dp, _dp, C.c_void_p, C.c_double,
                                     C.c_double, _dp, _dp]
         L.orc_nnb_ambiguous_walks.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _ip, _dp, _dp, C.c_void_p,
                                               C.c_double, C.c_double, _lp]
@@ -44,7 +45,8 @@ def orc():
         L.orc_moments.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
         _llp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
         L.orc_soft_force_walks_co.argtypes = wl + [_dp, _dp, _llp, _ip, _ip, _dp, _dp, _dp, _llp, _dp, _dp,
-                                                   _dp, _dp, _dp] + [C.c_double] * 5 + [_dp, _dp, _dp, _lp, _lp]
+                                                   _dp, _dp, C.c_void_p] + [C.c_double] * 5 + [_dp, _dp, _dp, _lp,
+                                                                                               _lp]
         L.orc_ngb_walks.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _ip, _dp, _dp, _llp, C.c_void_p, C.c_void_p]
         L.orc_ngb_walks.restype = C.c_longlong
         L.orc_changeover_setr.argtypes = [C.c_double] * 3 + [C.POINTER(C.c_double)] * 2
@@ -109,8 +111,9 @@ def _f64(a):
 
 
 def soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
-                     spj_pos, spj_mass, spj_quad, eps2, rout2, G=1.0, nnb_mode=0, part="all"):
-    """FP64 oracle over walks; part in {'all', 'epj', 'spj'}."""
+                     spj_pos, spj_mass, spj_quad, eps2, rout2, G=1.0, nnb_mode=0, part="all", mono=False):
+    """FP64 oracle over walks; part in {'all', 'epj', 'spj'}; mono selects the
+    monopole SPJ pair (src/soft_force.hpp:125-158; spj_quad is ignored)."""
     n_epi, n_epj, n_spj = _i32(n_epi), _i32(n_epj), _i32(n_spj)
     ni = int(n_epi.sum())
     acc = np.zeros((ni, 3))
@@ -129,16 +132,17 @@ def soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_p
     nw = n_epi.size
     if ni == 0:
         return acc, pot, nnb
+    qp = None if mono else spj_quad.ctypes.data
     if part == "all":
         rc = L.orc_soft_force_walks(nw, n_epi, n_epj, n_spj, epi_pos.ravel(), epi_rs, id_epj, id_spj,
                                     epj_pos.ravel(), epj_mass, epj_rs, spj_pos.ravel(), spj_mass,
-                                    spj_quad.ravel(), eps2, rout2, G, nnb_mode, acc.ravel(), pot, nnb)
+                                    qp, eps2, rout2, G, nnb_mode, acc.ravel(), pot, nnb)
     elif part == "epj":
         rc = L.orc_epj_walks(nw, n_epi, n_epj, epi_pos.ravel(), epi_rs, id_epj, epj_pos.ravel(), epj_mass,
                              epj_rs, eps2, rout2, G, nnb_mode, acc.ravel(), pot, nnb)
     else:
         rc = L.orc_spj_walks(nw, n_epi, n_spj, epi_pos.ravel(), id_spj, spj_pos.ravel(), spj_mass,
-                             spj_quad.ravel(), eps2, G, acc.ravel(), pot)
+                             qp, eps2, G, acc.ravel(), pot)
     if rc != 0:
         raise RuntimeError("oracle failed")
     return acc, pot, nnb
@@ -146,8 +150,9 @@ def soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_p
 
 def ref_soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
                          spj_pos, spj_mass, spj_quad, eps2, rout2, G=1.0, part="all", nthreads=0,
-                         isa=None, epi_type=None):
-    """The reference phantomquad kernel over walks (soft_force.hpp wrapper semantics)."""
+                         isa=None, epi_type=None, mono=False):
+    """The reference phantomquad kernel over walks (soft_force.hpp wrapper
+    semantics); mono: monopole superparticles (CalcForceEpSpMonoSimd)."""
     n_epi, n_epj, n_spj = _i32(n_epi), _i32(n_epj), _i32(n_spj)
     ni = int(n_epi.sum())
     acc = np.zeros((ni, 3))
@@ -169,7 +174,8 @@ def ref_soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, e
                                 et.ctypes.data if et is not None else None, id_epj, id_spj,
                                 epj_pos.ravel(), epj_mass, epj_rs, spj_pos.ravel(), spj_mass,
                                 spj_quad.ravel(), eps2, rout2, G, int(part in ("all", "epj")),
-                                int(part in ("all", "spj")), nthreads, acc.ravel(), pot, nnb)
+                                (2 if mono else 1) if part in ("all", "spj") else 0, nthreads, acc.ravel(), pot,
+                                nnb)
     if rc != 0:
         raise RuntimeError(f"reference kernel failed ({rc})")
     return acc, pot, nnb
@@ -177,7 +183,7 @@ def ref_soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, e
 
 def soft_force_walks_co(n_epi, n_epj, n_spj, epi_pos, epi_rs, epi_id, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
                         epj_id, epj_rin, epj_rout, spj_pos, spj_mass, spj_quad, eps2, rout2, G=1.0, ngb_factor=0.99,
-                        amb_rel=1e-5):
+                        amb_rel=1e-5, mono=False):
     """FP64 oracle of the changeover mode: acc, pot_soft, pot_tot, nnb and the
     per-i count of neighbour pairs at the search radius within amb_rel."""
     n_epi, n_epj, n_spj = _i32(n_epi), _i32(n_epj), _i32(n_spj)
@@ -191,11 +197,12 @@ def soft_force_walks_co(n_epi, n_epj, n_spj, epi_pos, epi_rs, epi_id, id_epj, id
         # flat contiguous copy; a one-element dummy stands in for an empty array
         a = np.ascontiguousarray(a, dtype=dt).ravel()
         return a if a.size else np.zeros(1, dt)
+    q = None if mono else arr(spj_quad)  # kept alive through the call
     rc = orc().orc_soft_force_walks_co(
         n_epi.size, n_epi, n_epj, n_spj, arr(epi_pos), arr(epi_rs), arr(epi_id, np.int64),
         arr(id_epj, np.int32), arr(id_spj, np.int32), arr(epj_pos), arr(epj_mass), arr(epj_rs),
-        arr(epj_id, np.int64), arr(epj_rin), arr(epj_rout), arr(spj_pos), arr(spj_mass), arr(spj_quad),
-        eps2, rout2, G, ngb_factor, amb_rel, acc.ravel(), ps, pt, nnb, amb)
+        arr(epj_id, np.int64), arr(epj_rin), arr(epj_rout), arr(spj_pos), arr(spj_mass),
+        None if q is None else q.ctypes.data, eps2, rout2, G, ngb_factor, amb_rel, acc.ravel(), ps, pt, nnb, amb)
     if rc != 0:
         raise RuntimeError("orc_soft_force_walks_co: an i-particle has no own EPJ entry in its walk")
     return acc, ps, pt, nnb, amb

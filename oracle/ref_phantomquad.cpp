@@ -12,6 +12,8 @@
 //   EP-EP  src/soft_force.hpp:350-425  (j with mass>0 only, i with type==1
 //          only, pg.set_r_crit2(r_out^2), x G, +=, n_ngb += (S32)(nngb*1.00001))
 //   EP-SP  src/soft_force.hpp:491-554  (i with type==1 only, x G, +=)
+//   EP-SP monopole (do_spj == 2) src/soft_force.hpp:427-480: the plain
+//          gravity kernel (run_epj) on the superparticles' mass and centre
 // The kernel header only needs the PS::S32 name for its diagnostics; the
 // driver defines it here before the include (no FDPS header is stood in).
 #include <cstdio>
@@ -162,7 +164,39 @@ int ref_soft_force_walks(int n_walk, const int *n_epi, const int *n_epj, const i
                     }
                 }
             }
-            if (do_spj) {
+            if (do_spj == 2) {
+                // ---- EP-SP monopole, src/soft_force.hpp:427-480
+                const int n_jp = n_spj[w];
+                pg->set_eps2(eps2);
+                ilist.clear();
+                int n_ip_local = 0;
+                for (int i = 0; i < n_ip; i++) {
+                    if (epi_type && epi_type[io[w] + i] != 1) continue;
+                    ilist.push_back(i);
+                    pg->set_xi_one(n_ip_local, xi[3 * i], xi[3 * i + 1], xi[3 * i + 2], 0.0);
+                    n_ip_local++;
+                }
+                const int loop_max = (n_jp - 1) / PhantomGrapeQuad::NJMAX + 1;
+                for (int loop = 0; loop < loop_max; loop++) {
+                    const int ih = PhantomGrapeQuad::NJMAX * loop;
+                    const int n_jp_tmp = (n_jp - ih < PhantomGrapeQuad::NJMAX)
+                                             ? n_jp - ih : PhantomGrapeQuad::NJMAX;
+                    for (int t = 0; t < n_jp_tmp; t++) {
+                        const long long s = id_spj[so[w] + ih + t];
+                        pg->set_epj_one(t, spj_pos[3 * s], spj_pos[3 * s + 1], spj_pos[3 * s + 2], spj_mass[s], 0.0);
+                    }
+                    pg->run_epj(n_ip, n_jp_tmp);
+                    for (int k = 0; k < n_ip_local; k++) {
+                        const long long i = io[w] + ilist[k];
+                        double p = 0, a[3] = {0, 0, 0};
+                        pg->accum_accp_one(k, a[0], a[1], a[2], p);
+                        acc[3 * i + 0] += G * a[0];
+                        acc[3 * i + 1] += G * a[1];
+                        acc[3 * i + 2] += G * a[2];
+                        pot[i] += G * p;
+                    }
+                }
+            } else if (do_spj) {
                 // ---- EP-SP quadrupole, src/soft_force.hpp:491-554
                 const int n_jp = n_spj[w];
                 pg->set_eps2(eps2);

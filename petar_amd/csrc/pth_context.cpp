@@ -6,6 +6,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -31,11 +32,18 @@ void pth_params_default(pth_params *p)
     p->ngb_factor = 0.99;
     p->segments = 8;  // measured on the precision workload: 8 segments, 1 staging + 2 force streams
     p->streams = 3;
+    p->df_near = 1;
+    p->spj_monopole = 0;
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int pth_device(const pth_ctx *ctx) { return ctx ? ctx->device : -1; }
+
+static double wall_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 static thread_local std::string g_create_err;
 const char *pth_create_error(void) { return g_create_err.c_str(); }
@@ -75,6 +83,8 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
         return PTH_ERR_ARG;
     }
     if (par.max_walk <= 0) par.max_walk = 1000;
+    par.df_near = par.df_near ? 1 : 0;
+    par.spj_monopole = par.spj_monopole ? 1 : 0;
     par.segments = std::min(std::max(par.segments, 1), PTH_MAX_SEGMENTS);
     par.streams = std::min(std::max(par.streams, 1), 8);
     int ndev = 0;
@@ -111,23 +121,39 @@ int pth_set_params(pth_ctx *c, const pth_params *p)
     if (!c) return PTH_ERR_ARG;
     if (!p) return c->fail(PTH_ERR_ARG, "pth_set_params: null params");
     if (p->nnb_mode != 0 && p->nnb_mode != 1) return c->fail(PTH_ERR_ARG, "pth_set_params: nnb_mode must be 0 or 1");
+    if (p->exec_mode != PTH_EXEC_FUSED && p->exec_mode != PTH_EXEC_PLAIN)
+        return c->fail(PTH_ERR_ARG, "pth_set_params: unknown exec_mode");
     if (!(p->rcut2 >= 0.0) || !(p->eps2 >= 0.0))
         return c->fail(PTH_ERR_ARG, "pth_set_params: eps2 and rcut2 must be >= 0");
     if (p->changeover != 0 && p->changeover != 1)
         return c->fail(PTH_ERR_ARG, "pth_set_params: changeover must be 0 or 1");
-    if (p->changeover && c->par.exec_mode != PTH_EXEC_FUSED)
+    if (p->changeover && p->exec_mode != PTH_EXEC_FUSED)
         return c->fail(PTH_ERR_ARG, "pth_set_params: changeover mode needs the fused form");
     if (!(p->ngb_factor >= 0.0)) return c->fail(PTH_ERR_ARG, "pth_set_params: ngb_factor must be >= 0");
-    for (int t = 0; t < MAX_TAG; t++)
-        if (c->tag[t].pending) return c->fail(PTH_ERR_STATE, "pth_set_params: a batch awaits retrieval");
+    // every value is read when a batch is dispatched (its plan and launch
+    // arguments are made then), so batches already in flight keep theirs
     c->par.eps2 = p->eps2;
     c->par.rcut2 = p->rcut2;
     c->par.G = p->G;
     c->par.nnb_mode = p->nnb_mode;
+    c->par.exec_mode = p->exec_mode;
+    c->par.chunk_j = p->chunk_j;
+    c->par.max_walk = p->max_walk > 0 ? p->max_walk : 1000;
+    c->par.max_ipl = p->max_ipl;
+    c->par.j_vgpr = p->j_vgpr;
     c->par.changeover = p->changeover;
     c->par.ngb_factor = p->ngb_factor;
     c->par.segments = std::min(std::max(p->segments, 1), PTH_MAX_SEGMENTS);
     c->par.streams = std::min(std::max(p->streams, 1), 8);
+    c->par.df_near = p->df_near ? 1 : 0;
+    c->par.spj_monopole = p->spj_monopole ? 1 : 0;
+    return PTH_OK;
+}
+
+int pth_get_params(const pth_ctx *c, pth_params *p)
+{
+    if (!c || !p) return PTH_ERR_ARG;
+    *p = c->par;
     return PTH_OK;
 }
 
@@ -154,7 +180,11 @@ void pth_destroy(pth_ctx *c)
         c->tag[t].h_pott.release();
         c->tag[t].h_err.release();
         if (c->tag[t].done) (void)hipEventDestroy(c->tag[t].done);
+        for (hipEvent_t &x : c->tag[t].prof)
+            if (x) (void)hipEventDestroy(x);
     }
+    for (hipEvent_t &x : c->prof_send)
+        if (x) (void)hipEventDestroy(x);
     if (c->ev_send) (void)hipEventDestroy(c->ev_send);
     if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -173,12 +203,15 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
     // the pinned staging and the resident arrays may still be in use by the
     // previous force call's copies and kernels
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const bool mono = c->par.spj_monopole != 0;
     const size_t be = sizeof(EpjDev) * (size_t)std::max(n_epj_tot, 1);
-    const size_t bs = sizeof(SpjDev) * (size_t)std::max(n_spj_tot, 1);
+    // monopole superparticles are kept as EPJ-like records (centre, mass)
+    const size_t bs = (mono ? sizeof(EpjDev) : sizeof(SpjDev)) * (size_t)std::max(n_spj_tot, 1);
     HIPCHK(c, c->h_epj.ensure(be), "pinned EPJ allocation");
     HIPCHK(c, c->h_spj.ensure(bs), "pinned SPJ allocation");
     HIPCHK(c, c->d_epj.ensure(be), "device EPJ allocation");
     HIPCHK(c, c->d_spj.ensure(bs), "device SPJ allocation");
+    const double t_pack = wall_s();
     EpjDev *he = c->h_epj.as<EpjDev>();
     SpjDev *hs = c->h_spj.as<SpjDev>();
     const char *eb = static_cast<const char *>(epj);
@@ -195,8 +228,23 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
         d.rs = (float)*reinterpret_cast<const double *>(r + ev->off_rsearch);
         he[i] = d;
     }
+    if (mono) {
+        EpjDev *hm = c->h_spj.as<EpjDev>();
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < n_spj_tot; i++) {
+        for (int i = 0; i < n_spj_tot; i++) {
+            const char *r = sb + (size_t)i * sv->stride;
+            const double *p = reinterpret_cast<const double *>(r + sv->off_pos);
+            EpjDev d;
+            d.x = p[0];
+            d.y = p[1];
+            d.z = p[2];
+            d.m = (float)*reinterpret_cast<const double *>(r + sv->off_mass);
+            d.rs = 0.f;
+            hm[i] = d;
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < (mono ? 0 : n_spj_tot); i++) {
         const char *r = sb + (size_t)i * sv->stride;
         const double *p = reinterpret_cast<const double *>(r + sv->off_pos);
         SpjDev d;
@@ -232,12 +280,19 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
                                      c->stream), "EPJ upload");
     }
     c->co_sent = c->par.changeover != 0;
+    c->mono_sent = mono;
+    c->prof.copy_s += wall_s() - t_pack;
+    if (c->prof_on) HIPCHK(c, hipEventRecord(c->prof_send[0], c->stream), "hipEventRecord");
     if (n_epj_tot > 0)
         HIPCHK(c, hipMemcpyAsync(c->d_epj.p, he, sizeof(EpjDev) * n_epj_tot, hipMemcpyHostToDevice, c->stream),
                "EPJ upload");
     if (n_spj_tot > 0)
-        HIPCHK(c, hipMemcpyAsync(c->d_spj.p, hs, sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice, c->stream),
-               "SPJ upload");
+        HIPCHK(c, hipMemcpyAsync(c->d_spj.p, c->h_spj.p, (mono ? sizeof(EpjDev) : sizeof(SpjDev)) * n_spj_tot,
+                                 hipMemcpyHostToDevice, c->stream), "SPJ upload");
+    if (c->prof_on) {
+        HIPCHK(c, hipEventRecord(c->prof_send[1], c->stream), "hipEventRecord");
+        c->prof_send_pending = true;
+    }
     HIPCHK(c, hipEventRecord(c->ev_send, c->stream), "hipEventRecord");
     if (c->par.exec_mode == PTH_EXEC_PLAIN) HIPCHK(c, hipStreamSynchronize(c->stream), "send sync");
     c->n_epj = n_epj_tot;
@@ -271,8 +326,11 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     const bool co = c->par.changeover != 0;
     if (co && !c->co_sent)
         return c->fail(PTH_ERR_STATE, "pth_dispatch: changeover mode, but the j arrays were sent without it");
+    if ((c->par.spj_monopole != 0) != c->mono_sent)
+        return c->fail(PTH_ERR_STATE, "pth_dispatch: spj_monopole differs from the last pth_send_j");
     Plan P;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, P))
+    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, P, 1,
+                   c->mono_sent))
         return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: batch exceeds the int32 index range");
     if (co) {
         const char *m = co_plan_error(P);
@@ -302,13 +360,19 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
             need |= sizeof(double) * 64 * (size_t)P.n_part > c->work.part_pott.cap;
             need |= sizeof(int) * P.walks.size() > c->work.walk_err.cap;
         }
+        const bool df = df_enabled(c->par) != 0;
+        if (df) {
+            need |= sizeof(float4) * (size_t)P.ni_tot > c->work.staged_il.cap;
+            need |= sizeof(float) * (PTH_DF_EXTRA / 8) * (size_t)P.n_e_slots > c->work.staged_el.cap;
+        }
         if (need) HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        HIPCHK(c, c->work.ensure(P, co), "work buffer allocation");
+        HIPCHK(c, c->work.ensure(P, co, df), "work buffer allocation");
     }
     HIPCHK(c, T.out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "output allocation");
     HIPCHK(c, T.out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "output allocation");
     if (co) HIPCHK(c, T.out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "output allocation");
 
+    const double t_pack = wall_s();
     char *h = c->h_arena.as<char>();
     EpiDev *he = reinterpret_cast<EpiDev *>(h + A.epi);
     long long *hid = reinterpret_cast<long long *>(h + A.epi_id);
@@ -352,14 +416,24 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     if (bad >= 0)
         return c->fail(PTH_ERR_ARG, "pth_dispatch: list index out of range in walk " + std::to_string(bad));
     copy_plan_to_arena(h, P, A);
+    c->prof.copy_s += wall_s() - t_pack;
+    c->prof.n_walk += n_walk;
+    c->prof.n_epi += P.ni_tot;
+    c->prof.n_epj += n_ide;
+    c->prof.n_spj += n_ids;
+    c->prof.n_call += 1;
+    T.timed = c->prof_on;
+    if (T.timed) HIPCHK(c, hipEventRecord(T.prof[0], c->stream), "hipEventRecord");
     HIPCHK(c, hipMemcpyAsync(c->d_arena.p, h, A.total, hipMemcpyHostToDevice, c->stream), "batch upload");
     HIPCHK(c, hipEventRecord(c->ev_upload, c->stream), "hipEventRecord");
+    if (T.timed) HIPCHK(c, hipEventRecord(T.prof[1], c->stream), "hipEventRecord");
     if (c->par.exec_mode == PTH_EXEC_PLAIN) HIPCHK(c, hipStreamSynchronize(c->stream), "upload sync");
     BatchDev b;
     fill_batch(b, P, c->par, c->d_arena, A, c->work, c->d_epj.as<EpjDev>(), c->d_spj.as<SpjDev>(),
                T.out_accp.as<double4>(), T.out_nnb.as<int>(), c->d_epj_co.as<EpjCoDev>(),
                T.out_pott.as<double>());
     HIPCHK(c, launch_all(b, c->stream), "soft-force kernels");
+    if (T.timed) HIPCHK(c, hipEventRecord(T.prof[2], c->stream), "hipEventRecord");
     HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     if (co) {
@@ -379,6 +453,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         HIPCHK(c, hipMemcpyAsync(T.h_err.p, c->work.walk_err.p, sizeof(int) * n_walk, hipMemcpyDeviceToHost,
                                  c->stream), "result download");
     T.co = co;
+    if (T.timed) HIPCHK(c, hipEventRecord(T.prof[3], c->stream), "hipEventRecord");
     HIPCHK(c, hipEventRecord(T.done, c->stream), "hipEventRecord");
     T.n_i.assign(n_epi, n_epi + n_walk);
     T.ni_tot = P.ni_tot;
@@ -407,6 +482,23 @@ int pth_retrieve(pth_ctx *c, int tag, int n_walk, const int *ni, void *const *fo
     T.pending = false;
     HIPCHK(c, hipEventSynchronize(T.done), "hipEventSynchronize");
     HIPCHK(c, hipGetLastError(), "kernel execution");
+    if (T.timed) {
+        float ms[3] = {0.f, 0.f, 0.f};
+        for (int k = 0; k < 3; k++)
+            HIPCHK(c, hipEventElapsedTime(&ms[k], T.prof[k], T.prof[k + 1]), "hipEventElapsedTime");
+        c->prof.send_s += 1e-3 * ms[0];
+        c->prof.calc_s += 1e-3 * ms[1];
+        c->prof.recv_s += 1e-3 * ms[2];
+        T.timed = false;
+    }
+    if (c->prof_send_pending) {
+        // the send's copies ran on this stream before the batch: complete
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->prof_send[0], c->prof_send[1]), "hipEventElapsedTime");
+        c->prof.send_s += 1e-3 * ms;
+        c->prof_send_pending = false;
+    }
+    const double t_scatter = wall_s();
     const double4 *a = T.h_accp.as<double4>();
     const int *nb = T.h_nnb.as<int>();
     const double *pt = T.h_pott.as<double>();
@@ -434,6 +526,30 @@ int pth_retrieve(pth_ctx *c, int tag, int n_walk, const int *ni, void *const *fo
             else *reinterpret_cast<int *>(r + fv->off_nngb) = nb[off[w] + i];
         }
     }
+    c->prof.copy_s += wall_s() - t_scatter;
+    return PTH_OK;
+}
+
+int pth_profile_enable(pth_ctx *c, int on)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (on) {
+        HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+        for (int t = 0; t < MAX_TAG; t++)
+            for (hipEvent_t &x : c->tag[t].prof)
+                if (!x) HIPCHK(c, hipEventCreate(&x), "hipEventCreate");
+        for (hipEvent_t &x : c->prof_send)
+            if (!x) HIPCHK(c, hipEventCreate(&x), "hipEventCreate");
+    }
+    c->prof_on = on != 0;
+    return PTH_OK;
+}
+
+int pth_profile_take(pth_ctx *c, pth_profile *out)
+{
+    if (!c || !out) return PTH_ERR_ARG;
+    *out = c->prof;
+    c->prof = pth_profile{};
     return PTH_OK;
 }
 
@@ -450,6 +566,8 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
     *out = nullptr;
     if (n_walk < 0 || n_epj_tot < 0 || n_spj_tot < 0)
         return c->fail(PTH_ERR_ARG, "pth_resident_create: negative count");
+    if (n_spj_tot > 0 && (!spj_pos || !spj_mass || (!spj_quad && !c->par.spj_monopole)))
+        return c->fail(PTH_ERR_ARG, "pth_resident_create: null SPJ array");
     long long n_ide = 0, n_ids = 0;
     for (int w = 0; w < n_walk; w++) {
         if (n_epi[w] < 0 || n_epj[w] < 0 || n_spj[w] < 0)
@@ -495,8 +613,16 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
         ej[i].m = (float)epj_mass[i];
         ej[i].rs = (float)epj_rs[i];
     }
-    std::vector<SpjDev> sj(std::max(n_spj_tot, 0));
-    for (int i = 0; i < n_spj_tot; i++) {
+    std::vector<SpjDev> sj(P.mono ? 0 : std::max(n_spj_tot, 0));
+    std::vector<EpjDev> sm(P.mono ? std::max(n_spj_tot, 0) : 0);
+    for (int i = 0; i < (P.mono ? n_spj_tot : 0); i++) {
+        sm[i].x = spj_pos[3 * (size_t)i];
+        sm[i].y = spj_pos[3 * (size_t)i + 1];
+        sm[i].z = spj_pos[3 * (size_t)i + 2];
+        sm[i].m = (float)spj_mass[i];
+        sm[i].rs = 0.f;
+    }
+    for (int i = 0; i < (P.mono ? 0 : n_spj_tot); i++) {
         const double *q = spj_quad + 6 * (size_t)i;
         sj[i].x = spj_pos[3 * (size_t)i];
         sj[i].y = spj_pos[3 * (size_t)i + 1];
@@ -513,7 +639,8 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
     if ((e = hipMemcpy(r->arena.p, h.data(), r->A.total, hipMemcpyHostToDevice))) return bail(e, "resident upload");
     if (n_epj_tot && (e = hipMemcpy(r->epj.p, ej.data(), sizeof(EpjDev) * n_epj_tot, hipMemcpyHostToDevice)))
         return bail(e, "resident upload");
-    if (n_spj_tot && (e = hipMemcpy(r->spj.p, sj.data(), sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice)))
+    if (n_spj_tot && (e = P.mono ? hipMemcpy(r->spj.p, sm.data(), sizeof(EpjDev) * n_spj_tot, hipMemcpyHostToDevice)
+                                 : hipMemcpy(r->spj.p, sj.data(), sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice)))
         return bail(e, "resident upload");
     resident_finish(r);
     *out = r;
@@ -674,7 +801,7 @@ int pth_resident_replan(pth_resident *r, int max_ipl, int chunk_j)
         ns[w] = P0.walks[w].n_s;
     }
     Plan P;
-    if (!make_plan(nw, ne.data(), nj.data(), ns.data(), PTH_EXEC_FUSED, chunk_j, max_ipl, P, r->n_seg))
+    if (!make_plan(nw, ne.data(), nj.data(), ns.data(), PTH_EXEC_FUSED, chunk_j, max_ipl, P, r->n_seg, P0.mono))
         return c->fail(PTH_ERR_CAPACITY, "pth_resident_replan: sizes exceed the int32 index range");
     // the staged layout depends on the walk sizes only
     if (P.es_off != P0.es_off || P.ss_off != P0.ss_off || P.ni_tot != P0.ni_tot)
@@ -684,7 +811,7 @@ int pth_resident_replan(pth_resident *r, int max_ipl, int chunk_j)
     const size_t bi = sizeof(Item) * P.items.size(), br = sizeof(RItem) * P.ritems.size();
     const size_t off_r = align256(bi);
     HIPCHK(c, r->plan_dev.ensure(std::max<size_t>(off_r + br, 256)), "plan allocation");
-    HIPCHK(c, r->work.ensure(P, r->co), "resident work buffers");
+    HIPCHK(c, r->work.ensure(P, r->co, df_enabled(c->par) != 0), "resident work buffers");
     if (bi) HIPCHK(c, hipMemcpy(r->plan_dev.p, P.items.data(), bi, hipMemcpyHostToDevice), "plan upload");
     if (br) HIPCHK(c, hipMemcpy(r->plan_dev.as<char>() + off_r, P.ritems.data(), br, hipMemcpyHostToDevice),
                    "plan upload");
@@ -828,7 +955,7 @@ int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_e
         if (!pre_ok) return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
         std::swap(r->plan, *pre);
     } else if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
-                          r->n_seg))
+                          r->n_seg, c->par.spj_monopole != 0))
         return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
     const Plan &P = r->plan;
     r->co = c->par.changeover != 0;
@@ -842,10 +969,11 @@ int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_e
     if ((e = hipSetDevice(c->device))) return bail(e, "hipSetDevice");
     if ((e = r->arena.ensure(r->A.total))) return bail(e, "resident arena");
     if ((e = r->epj.ensure(sizeof(EpjDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
-    if ((e = r->spj.ensure(sizeof(SpjDev) * std::max(n_spj_tot, 1)))) return bail(e, "resident SPJ");
+    if ((e = r->spj.ensure((P.mono ? sizeof(EpjDev) : sizeof(SpjDev)) * std::max(n_spj_tot, 1))))
+        return bail(e, "resident SPJ");
     if ((e = r->out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
     if ((e = r->out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");
-    if ((e = r->work.ensure(P, r->co))) return bail(e, "resident work buffers");
+    if ((e = r->work.ensure(P, r->co, df_enabled(c->par) != 0))) return bail(e, "resident work buffers");
     if (r->co) {
         if ((e = r->epj_co.ensure(sizeof(EpjCoDev) * std::max(n_epj_tot, 1)))) return bail(e, "resident EPJ");
         if ((e = r->out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)))) return bail(e, "resident output");

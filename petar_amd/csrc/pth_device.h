@@ -47,6 +47,9 @@ struct __align__(32) EpiDev {
 #define PTH_CO_HASH 2048
 // floats per staged EPJ block of 8 in the changeover extras
 #define PTH_CO_EXTRA 40
+// floats per staged near EPJ block of 8 in linear-cutoff mode with
+// double-float near pairs: the low-order parts x[8], y[8], z[8]
+#define PTH_DF_EXTRA 24
 
 // One walk (i-group) of a batch: i range, list ranges (in the concatenated
 // id lists) and sizes.
@@ -94,6 +97,8 @@ struct BatchDev {
     int n_items = 0, n_ritems = 0;
     int plain = 0, nnb_le = 0, j_vgpr = 0;
     int co = 0;        // changeover mode: (1 - K) weight for neighbours, no self term
+    int df = 0;        // linear-cutoff mode: near EPJ pairs in double-float coordinates
+    int mono = 0;      // monopole superparticles: spj holds EpjDev records, staged as far EPJ
     float eps2 = 0.f, rout2 = 0.f;
     float ngb2 = 0.f;  // changeover mode: squared neighbour-radius factor
     double G = 1.0;
@@ -101,7 +106,8 @@ struct BatchDev {
     const long long *epi_id = nullptr;
     const EpjCoDev *epj_co = nullptr;
     float4 *staged_ic = nullptr;  // per i: {self slot (int bits), r_in, r_out, m}
-    float4 *staged_il = nullptr;  // per i: low parts of x-c, y-c, z-c (double-float), 0
+    float4 *staged_il = nullptr;  // per i: low parts of x-c, y-c, z-c (double-float), 0 (co or df)
+    float *staged_el = nullptr;   // df: per staged near EPJ block of 8: {x, y, z low parts[8]}
     float *staged_ec = nullptr;   // per staged EPJ block of 8: {r_in[8], r_out[8], x, y, z low parts[8]}
     int *walk_err = nullptr;      // per walk: number of i without their own EPJ
     double *out_pott = nullptr;

@@ -376,6 +376,22 @@ __global__ __launch_bounds__(256) void k_res_nodes(long long nn, const double *_
     spj[k] = r;
 }
 
+// monopole superparticles: the node's mass at its centre of mass
+__global__ __launch_bounds__(256) void k_res_nodes_mono(long long nn, const double *__restrict__ nmass,
+                                                        const double *__restrict__ ncom,
+                                                        pth::EpjDev *__restrict__ spj)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nn) return;
+    pth::EpjDev r;
+    r.x = ncom[3 * k];
+    r.y = ncom[3 * k + 1];
+    r.z = ncom[3 * k + 2];
+    r.m = (float)nmass[k];
+    r.rs = 0.f;
+    spj[k] = r;
+}
+
 struct WalkArgs {
     const GNode *nodes;
     const double *ncom, *box_in, *box_out;
@@ -1216,9 +1232,9 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         }
         const pth_params &par = ctx->par;
         t->pre_key = PlanKey{par.exec_mode, par.chunk_j, par.max_ipl,
-                             par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1};
+                             par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1, par.spj_monopole};
         t->pre_ok = make_plan((int)ng, ni.data(), nje.data(), njs.data(), t->pre_key.exec_mode, t->pre_key.chunk_j,
-                              t->pre_key.max_ipl, t->pre_plan, t->pre_key.n_seg);
+                              t->pre_key.max_ipl, t->pre_plan, t->pre_key.n_seg, t->pre_key.mono != 0);
         t->pre_valid = true;
     }
     if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return ctx->hip_fail(e, "tree walk");
@@ -1251,7 +1267,8 @@ int resident_fill_gtree(pth_resident *r, const pth_gtree *t, const char *who)
     // the plan made during the build, when it was made for this context's
     // parameters (then taken over: a second refill plans again)
     const pth_params &par = c->par;
-    const PlanKey key{par.exec_mode, par.chunk_j, par.max_ipl, par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1};
+    const PlanKey key{par.exec_mode, par.chunk_j, par.max_ipl, par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1,
+                      par.spj_monopole};
     const bool take = t->pre_valid && t->pre_key == key;
     const int rc = resident_setup(r, (int)ng, ni.data(), ne.data(), ns.data(), t->n_le, t->n_ls, (int)t->n,
                                   (int)t->n_node, who, take ? &t->pre_plan : nullptr, t->pre_ok);
@@ -1280,8 +1297,12 @@ int resident_fill_gtree(pth_resident *r, const pth_gtree *t, const char *who)
     hipLaunchKernelGGL(k_res_particles, dim3(blocks(t->n)), dim3(256), 0, st, t->n, t->spos.as<double>(),
                        t->smass.as<double>(), t->srs.as<double>(), reinterpret_cast<EpiDev *>(d + A.epi),
                        r->epj.as<EpjDev>());
-    hipLaunchKernelGGL(k_res_nodes, dim3(blocks(t->n_node)), dim3(256), 0, st, t->n_node, t->nmass.as<double>(),
-                       t->ncom.as<double>(), t->nquad.as<double>(), r->spj.as<SpjDev>());
+    if (P.mono)
+        hipLaunchKernelGGL(k_res_nodes_mono, dim3(blocks(t->n_node)), dim3(256), 0, st, t->n_node,
+                           t->nmass.as<double>(), t->ncom.as<double>(), r->spj.as<EpjDev>());
+    else
+        hipLaunchKernelGGL(k_res_nodes, dim3(blocks(t->n_node)), dim3(256), 0, st, t->n_node, t->nmass.as<double>(),
+                           t->ncom.as<double>(), t->nquad.as<double>(), r->spj.as<SpjDev>());
     if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(e, "resident records");
     resident_finish(r);
     return PTH_OK;

@@ -118,9 +118,15 @@ struct Arena {
 struct Work {
     DevBuf centre, near_blk, staged_i, staged_e, staged_s, part_accp, part_nnb;
     DevBuf staged_ic, staged_il, staged_ec, part_pott, walk_err;  // changeover mode
-    hipError_t ensure(const Plan &P, bool co)
+    DevBuf staged_el;                                             // linear mode, double-float near pairs
+    hipError_t ensure(const Plan &P, bool co, bool df)
     {
         hipError_t e;
+        if (df) {
+            if ((e = staged_il.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
+            if ((e = staged_el.ensure(sizeof(float) * (PTH_DF_EXTRA / 8) * std::max<long long>(P.n_e_slots, 8))))
+                return e;
+        }
         if (co) {
             if ((e = staged_ic.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
             if ((e = staged_il.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
@@ -143,8 +149,16 @@ struct Work {
         centre.release(); near_blk.release(); staged_i.release(); staged_e.release(); staged_s.release();
         part_accp.release(); part_nnb.release();
         staged_ic.release(); staged_il.release(); staged_ec.release(); part_pott.release(); walk_err.release();
+        staged_el.release();
     }
 };
+
+// Linear-cutoff mode with double-float near pairs (fused form; the
+// changeover mode has its own double-float neighbour pairs).
+inline int df_enabled(const pth_params &par)
+{
+    return (par.df_near && !par.changeover && par.exec_mode == PTH_EXEC_FUSED) ? 1 : 0;
+}
 
 // Copies the plan's segments into the launch descriptor.
 inline void set_segments(BatchDev &b, const Plan &P)
@@ -187,6 +201,8 @@ inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const 
     b.rout2 = (float)par.rcut2;
     b.G = par.G;
     b.co = par.changeover ? 1 : 0;
+    b.df = df_enabled(par);
+    b.mono = P.mono ? 1 : 0;
     {
         const double f = par.ngb_factor > 0.0 ? par.ngb_factor : 0.99;
         b.ngb2 = (float)(f * f);
@@ -211,6 +227,10 @@ inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const 
     b.part_nnb = W.part_nnb.as<int>();
     b.out_accp = out_accp;
     b.out_nnb = out_nnb;
+    if (b.df) {
+        b.staged_il = W.staged_il.as<float4>();
+        b.staged_el = W.staged_el.as<float>();
+    }
     if (b.co) {
         b.epi_id = reinterpret_cast<const long long *>(base + A.epi_id);
         b.epj_co = epj_co;
@@ -253,6 +273,10 @@ struct TagState {
     long long ni_tot = 0;
     bool pending = false;
     hipEvent_t done = nullptr;
+    // profile (timing events): before the upload, after it, after the
+    // kernels, after the result copy; timed says the batch recorded them
+    hipEvent_t prof[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timed = false;
 };
 
 struct pth_ctx {
@@ -266,6 +290,7 @@ struct pth_ctx {
     int n_epj = 0, n_spj = 0;
     bool j_sent = false;
     bool co_sent = false;  // the last send carried the changeover data
+    bool mono_sent = false;  // the last send held monopole superparticles
     hipEvent_t ev_send = nullptr;
     // batch upload
     PinBuf h_arena;
@@ -273,6 +298,11 @@ struct pth_ctx {
     hipEvent_t ev_upload = nullptr;
     Work work;
     TagState tag[MAX_TAG];
+    // profile of the hooks (pth_profile_*)
+    bool prof_on = false;
+    pth_profile prof{};
+    hipEvent_t prof_send[2] = {nullptr, nullptr};  // around the last send's copies
+    bool prof_send_pending = false;
 
     int fail(int code, const std::string &msg)
     {

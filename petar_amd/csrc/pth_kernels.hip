@@ -115,6 +115,12 @@ struct StageArgs {
     float4 *staged_il;
     float *staged_ec;
     int *walk_err;
+    // linear-cutoff mode with double-float near pairs
+    int df;
+    float *staged_el;
+    // monopole superparticles (centre and mass), staged as far EPJ entries
+    int mono;
+    const EpjDev *spj_mono;
 };
 
 // CO (changeover mode): every near EPJ also gets its changeover radii and the
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         const double dx = e.x - c[0], dy = e.y - c[1], dz = e.z - c[2];
         const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
         staged_i[wk.i_off + k] = make_float4(hx, hy, hz, (float)(rs * rs));
-        if constexpr (CO)
+        if (CO || a.df)
             a.staged_il[wk.i_off + k] =
                 make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.f);
     }
@@ -239,6 +245,15 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
     };
     auto place = [&](const Ent &q, long long slot) {
         put_epj(staged_e, slot, q.x, q.y, q.z, q.m, q.r2);
+        if constexpr (!CO) {
+            if (a.df && q.nearj) {
+                float *blk = a.staged_el + (slot >> 3) * PTH_DF_EXTRA;
+                const int l = (int)(slot & 7);
+                blk[l] = q.xl;
+                blk[8 + l] = q.yl;
+                blk[16 + l] = q.zl;
+            }
+        }
         if constexpr (CO) {
             if (q.nearj) {
                 const EpjCoDev co = a.epj_co[q.jx];
@@ -303,9 +318,28 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         n_far += taf + tbf;
         __syncthreads();
     }
-    // pad the gap between the two regions (far, massless)
-    for (long long s = base + n_near + tid; s < base + cap - n_far; s += 256)
+    // monopole superparticles: far entries after the EPJ list's far ones (a
+    // monopole pair is a far pair: no cutoff, no neighbour count), in list
+    // order from the back
+    if (a.mono) {
+        const int *__restrict__ mid = a.id_spj + wk.s_off;
+        for (int k = tid; k < wk.n_s; k += 256) {
+            const EpjDev e = a.spj_mono[mid[k]];
+            put_epj(staged_e, base + cap - 1 - (n_far + k), (float)(e.x - c[0]), (float)(e.y - c[1]),
+                    (float)(e.z - c[2]), e.m, 0.f);
+        }
+        n_far += wk.n_s;
+    }
+    // pad the gap between the two regions (far, massless); the padding of the
+    // last near block is read by the double-float near loop: zero low parts
+    for (long long s = base + n_near + tid; s < base + cap - n_far; s += 256) {
         put_epj(staged_e, s, 1e18f, 1e18f, 1e18f, 0.f, 0.f);
+        if (!CO && a.df) {
+            float *blk = a.staged_el + (s >> 3) * PTH_DF_EXTRA;
+            const int l = (int)(s & 7);
+            blk[l] = blk[8 + l] = blk[16 + l] = 0.f;
+        }
+    }
     if (tid == 0) a.near_blk[w] = (int)((base + ((n_near + 7) & ~7)) >> 3);
     if constexpr (CO) {
         // (the loop above ended with a barrier: co_self is complete)
@@ -329,12 +363,13 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
     // segment fits beside the changeover force kernels, which are held to 7
     // waves per SIMD; 4.5% off the pipelined changeover step)
     constexpr int U = CO ? 2 : 4;
-    for (int k0 = 0; k0 < wk.n_s; k0 += 256 * U) {
+    const int n_srec = a.mono ? 0 : wk.n_s;
+    for (int k0 = 0; k0 < n_srec; k0 += 256 * U) {
         int id[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int k = k0 + 256 * u + tid;
-            id[u] = k < wk.n_s ? ids[k] : -1;
+            id[u] = k < n_srec ? ids[k] : -1;
         }
         SpjDev p[U];
 #pragma unroll
@@ -353,11 +388,18 @@ struct Consts {
     float eps2, rout2, half, two_half;
 };
 
-template <int IPL, int NNB_LE, bool NEAR, bool VC>
+// EPJ blocks [b_begin, b_end) of a walk's staged list against the lane's i.
+// NEAR blocks also count neighbours and apply the cutoff; with DF (near
+// blocks in linear-cutoff mode) each separation is taken from double-float
+// walk-relative coordinates, (x_j - x_i) + (xlo_j - xlo_i): the FP32 high
+// parts of two close particles differ exactly, so the separation keeps FP32
+// relative precision however far the pair lies from the walk centre.
+template <int IPL, int NNB_LE, bool NEAR, bool VC, bool DF = false>
 __device__ __forceinline__ void epj_blocks(cfp se, int b_begin, int b_end, const float (&xi)[IPL],
                                            const float (&yi)[IPL], const float (&zi)[IPL],
                                            const float (&rsi2)[IPL], const Consts &K, double (&A)[IPL][4],
-                                           int (&nnb)[IPL])
+                                           int (&nnb)[IPL], cfp sel = nullptr, const float *xil = nullptr,
+                                           const float *yil = nullptr, const float *zil = nullptr)
 {
     for (int b0 = b_begin; b0 < b_end; b0 += PTH_FLUSH_BLOCKS) {
         const int b1 = min(b0 + PTH_FLUSH_BLOCKS, b_end);
@@ -373,11 +415,23 @@ __device__ __forceinline__ void epj_blocks(cfp se, int b_begin, int b_end, const
                 const float mj = jv<VC>(B[24 + k]);
                 float rj = 0.f;
                 if (NEAR) rj = jv<VC>(B[32 + k]);
+                float xjl = 0.f, yjl = 0.f, zjl = 0.f;
+                if (DF) {
+                    const cfp E = sel + (size_t)b * PTH_DF_EXTRA;
+                    xjl = E[k];
+                    yjl = E[8 + k];
+                    zjl = E[16 + k];
+                }
 #pragma unroll
                 for (int q = 0; q < IPL; q++) {
-                    const float dx = xj - xi[q];
-                    const float dy = yj - yi[q];
-                    const float dz = zj - zi[q];
+                    float dx = xj - xi[q];
+                    float dy = yj - yi[q];
+                    float dz = zj - zi[q];
+                    if (DF) {
+                        dx += xjl - xil[q];
+                        dy += yjl - yil[q];
+                        dz += zjl - zil[q];
+                    }
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, K.eps2)));
                     float ri;
                     if (NEAR) {
@@ -646,6 +700,7 @@ struct ForceArgs {
     const float4 *staged_ic;  // changeover mode
     const float4 *staged_il;  // changeover mode
     const float *staged_ec;   // changeover mode
+    const float *staged_el;   // linear mode, double-float near pairs
     float eps2, rout2, ngb2;
     double G;
     double4 *out_accp;
@@ -656,7 +711,7 @@ struct ForceArgs {
     double *part_pott;  // changeover mode
 };
 
-template <int IPL, int NNB_LE, bool VC, bool CO>
+template <int IPL, int NNB_LE, bool VC, bool CO, bool DF>
 __device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int near_end, int lane,
                                          const Consts &K)
 {
@@ -678,6 +733,13 @@ __device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int
         A[q][0] = A[q][1] = A[q][2] = A[q][3] = 0.0;
         PT[q] = 0.0;
         nnb[q] = 0;
+        if constexpr (DF) {
+            float4 l = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (in) l = a.staged_il[it.i_begin + lane + 64 * q];
+            xil[q] = l.x;
+            yil[q] = l.y;
+            zil[q] = l.z;
+        }
         if constexpr (CO) {
             float4 c = make_float4(__int_as_float(-2), 0.f, 0.f, 0.f);
             if (in) c = a.staged_ic[it.i_begin + lane + 64 * q];
@@ -706,6 +768,9 @@ __device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int
             if (self[q] >= 0 && sb >= it.e_blk_begin && sb < e_mid)
                 A[q][3] += (double)(m_i[q] * __builtin_amdgcn_rsqf(__builtin_fmaxf(K.eps2, K.rout2)));
         }
+    } else if constexpr (DF) {
+        epj_blocks<IPL, NNB_LE, true, VC, true>(se, it.e_blk_begin, e_mid, xi, yi, zi, rsi2, K, A, nnb,
+                                                as_const(a.staged_el), xil, yil, zil);
     } else
         epj_blocks<IPL, NNB_LE, true, VC>(se, it.e_blk_begin, e_mid, xi, yi, zi, rsi2, K, A, nnb);
     epj_blocks<IPL, NNB_LE, false, VC>(se, e_mid, it.e_blk_end, xi, yi, zi, rsi2, K, A, nnb);
@@ -729,7 +794,7 @@ __device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int
 }
 
 // One launch per i-per-lane class: items [0, n_items) of this class.
-template <int NNB_LE, int IPL, bool VC, bool CO>
+template <int NNB_LE, int IPL, bool VC, bool CO, bool DF>
 __device__ __forceinline__ void force_items(const ForceArgs &a)
 {
     const int idx = uniform((int)blockIdx.x * 4 + ((int)threadIdx.x >> 6));
@@ -747,13 +812,13 @@ __device__ __forceinline__ void force_items(const ForceArgs &a)
     if (it.walk < 0) return;
     const int near_end = ((cip)(a.near_blk))[it.walk];
     const Consts K{jv<VC>(a.eps2), jv<VC>(a.rout2), jv<VC>(0.5f), jv<VC>(2.5f)};
-    run_tile<IPL, NNB_LE, VC, CO>(a, it, near_end, threadIdx.x & 63, K);
+    run_tile<IPL, NNB_LE, VC, CO, DF>(a, it, near_end, threadIdx.x & 63, K);
 }
 
-template <int NNB_LE, int IPL, bool VC, bool CO>
+template <int NNB_LE, int IPL, bool VC, bool CO, bool DF>
 __global__ __launch_bounds__(256) void k_force(const ForceArgs a)
 {
-    force_items<NNB_LE, IPL, VC, CO>(a);
+    force_items<NNB_LE, IPL, VC, CO, DF>(a);
 }
 
 // Changeover mode: its extra per-i state and the revisit of neighbour pairs
@@ -763,13 +828,16 @@ __global__ __launch_bounds__(256) void k_force(const ForceArgs a)
 template <int NNB_LE, int IPL, bool VC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IPL <= 2 ? 7 : 1))) void k_force_co(const ForceArgs a)
 {
-    force_items<NNB_LE, IPL, VC, true>(a);
+    force_items<NNB_LE, IPL, VC, true, false>(a);
 }
 
 // ---------------------------------------------------------------------------
 // force, plain per-walk form: SGPR operands, one i per lane, whole list
 
-template <int NNB_LE>
+// NEAR blocks count neighbours and apply the cutoff; far blocks (beyond
+// max(r_search, r_out) of every i of the walk, and monopole superparticles)
+// have neither.
+template <int NNB_LE, bool NEAR>
 __device__ __forceinline__ void plain_epj(cfp se, int b_begin, int b_end, float xi, float yi, float zi,
                                           float rsi2, float eps2, float rout2, double (&A)[4], int &nnb)
 {
@@ -785,10 +853,15 @@ __device__ __forceinline__ void plain_epj(cfp se, int b_begin, int b_end, float 
                 const float dy = B[8 + k] - yi;
                 const float dz = B[16 + k] - zi;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-                const float rs2 = __int_as_float(max(__float_as_int(rsi2), __float_as_int(B[32 + k])));
-                if (NNB_LE) nb += (r2 <= rs2) ? 1 : 0;
-                else nb += (r2 < rs2) ? 1 : 0;
-                const float ri = __builtin_amdgcn_rsqf(__builtin_fmaxf(r2, rout2));
+                float ri;
+                if (NEAR) {
+                    const float rs2 = __int_as_float(max(__float_as_int(rsi2), __float_as_int(B[32 + k])));
+                    if (NNB_LE) nb += (r2 <= rs2) ? 1 : 0;
+                    else nb += (r2 < rs2) ? 1 : 0;
+                    ri = __builtin_amdgcn_rsqf(__builtin_fmaxf(r2, rout2));
+                } else {
+                    ri = __builtin_amdgcn_rsqf(r2);
+                }
                 const float mri = B[24 + k] * ri;
                 const float mri3 = mri * (ri * ri);
                 ax = __builtin_fmaf(mri3, dx, ax);
@@ -839,6 +912,7 @@ __device__ __forceinline__ void plain_spj(cfp ss, int s_begin, int s_end, float 
 
 template <int NNB_LE>
 __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__ walks,
+                                                     const int *__restrict__ near_blk,
                                                      const float4 *__restrict__ staged_i,
                                                      const float *__restrict__ staged_e,
                                                      const float *__restrict__ staged_s,
@@ -853,6 +927,7 @@ __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__
     const int wave = uniform((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int n_tiles = (wk.n_i + 63) >> 6;
     const int eb = uniform((int)(es_off[w] >> 3)), ee = uniform((int)(es_off[w + 1] >> 3));
+    const int em = min(max(uniform(near_blk[w]), eb), ee);
     const int sb = uniform((int)ss_off[w]), se_ = uniform((int)ss_off[w] + wk.n_s);
     for (int t = wave; t < n_tiles; t += 4) {
         const int i = 64 * t + lane;
@@ -860,7 +935,8 @@ __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__
         if (i < wk.n_i) v = staged_i[wk.i_off + i];
         double A[4] = {0.0, 0.0, 0.0, 0.0};
         int nnb = 0;
-        plain_epj<NNB_LE>(as_const(staged_e), eb, ee, v.x, v.y, v.z, v.w, eps2, rout2, A, nnb);
+        plain_epj<NNB_LE, true>(as_const(staged_e), eb, em, v.x, v.y, v.z, v.w, eps2, rout2, A, nnb);
+        plain_epj<NNB_LE, false>(as_const(staged_e), em, ee, v.x, v.y, v.z, v.w, eps2, rout2, A, nnb);
         plain_spj(as_const(staged_s), sb, se_, v.x, v.y, v.z, eps2, A);
         if (i < wk.n_i) {
             out_accp[wk.i_off + i] = make_double4(G * A[0], G * A[1], G * A[2], G * A[3]);
@@ -929,12 +1005,16 @@ hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st)
     a.staged_il = b.staged_il;
     a.staged_ec = b.staged_ec;
     a.walk_err = b.walk_err;
+    a.df = b.df;
+    a.staged_el = b.staged_el;
+    a.mono = b.mono;
+    a.spj_mono = reinterpret_cast<const EpjDev *>(b.spj);
     if (b.co) hipLaunchKernelGGL(k_stage_walk<true>, dim3(g.n_walk), dim3(256), 0, st, a);
     else hipLaunchKernelGGL(k_stage_walk<false>, dim3(g.n_walk), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
-template <int NNB, bool VC, bool CO>
+template <int NNB, bool VC, bool CO, bool DF>
 static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t st)
 {
     if constexpr (CO) {
@@ -943,10 +1023,10 @@ static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t
         else if (ipl == 3) hipLaunchKernelGGL((k_force_co<NNB, 3, VC>), dim3(nblk), dim3(256), 0, st, a);
         else hipLaunchKernelGGL((k_force_co<NNB, 4, VC>), dim3(nblk), dim3(256), 0, st, a);
     } else {
-        if (ipl == 1) hipLaunchKernelGGL((k_force<NNB, 1, VC, false>), dim3(nblk), dim3(256), 0, st, a);
-        else if (ipl == 2) hipLaunchKernelGGL((k_force<NNB, 2, VC, false>), dim3(nblk), dim3(256), 0, st, a);
-        else if (ipl == 3) hipLaunchKernelGGL((k_force<NNB, 3, VC, false>), dim3(nblk), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_force<NNB, 4, VC, false>), dim3(nblk), dim3(256), 0, st, a);
+        if (ipl == 1) hipLaunchKernelGGL((k_force<NNB, 1, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
+        else if (ipl == 2) hipLaunchKernelGGL((k_force<NNB, 2, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
+        else if (ipl == 3) hipLaunchKernelGGL((k_force<NNB, 3, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_force<NNB, 4, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
     }
 }
 
@@ -956,11 +1036,11 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
     if (b.plain) {
         if (b.n_walk <= 0) return hipSuccess;
         if (b.nnb_le)
-            hipLaunchKernelGGL(k_force_plain<1>, dim3(b.n_walk), dim3(256), 0, st, b.walks, b.staged_i,
+            hipLaunchKernelGGL(k_force_plain<1>, dim3(b.n_walk), dim3(256), 0, st, b.walks, b.near_blk, b.staged_i,
                                b.staged_e, b.staged_s, b.es_off, b.ss_off, b.eps2, b.rout2, b.G,
                                b.out_accp, b.out_nnb);
         else
-            hipLaunchKernelGGL(k_force_plain<0>, dim3(b.n_walk), dim3(256), 0, st, b.walks, b.staged_i,
+            hipLaunchKernelGGL(k_force_plain<0>, dim3(b.n_walk), dim3(256), 0, st, b.walks, b.near_blk, b.staged_i,
                                b.staged_e, b.staged_s, b.es_off, b.ss_off, b.eps2, b.rout2, b.G,
                                b.out_accp, b.out_nnb);
         return hipGetLastError();
@@ -974,6 +1054,7 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
     a.staged_ic = b.staged_ic;
     a.staged_il = b.staged_il;
     a.staged_ec = b.staged_ec;
+    a.staged_el = b.staged_el;
     a.eps2 = b.eps2;
     a.rout2 = b.rout2;
     a.ngb2 = b.ngb2;
@@ -993,16 +1074,19 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
         a.items = b.items + g.seg_begin[c];
         a.n_items = n;
         const unsigned nblk = (unsigned)((n + 3) / 4);
-        // the changeover mode always uses SGPR operands
+        // the changeover and double-float near modes always use SGPR operands
         if (b.co) {
-            if (b.nnb_le) launch_class<1, false, true>(ipls[c], nblk, a, st);
-            else launch_class<0, false, true>(ipls[c], nblk, a, st);
+            if (b.nnb_le) launch_class<1, false, true, false>(ipls[c], nblk, a, st);
+            else launch_class<0, false, true, false>(ipls[c], nblk, a, st);
+        } else if (b.df) {
+            if (b.nnb_le) launch_class<1, false, false, true>(ipls[c], nblk, a, st);
+            else launch_class<0, false, false, true>(ipls[c], nblk, a, st);
         } else if (b.j_vgpr) {
-            if (b.nnb_le) launch_class<1, true, false>(ipls[c], nblk, a, st);
-            else launch_class<0, true, false>(ipls[c], nblk, a, st);
+            if (b.nnb_le) launch_class<1, true, false, false>(ipls[c], nblk, a, st);
+            else launch_class<0, true, false, false>(ipls[c], nblk, a, st);
         } else {
-            if (b.nnb_le) launch_class<1, false, false>(ipls[c], nblk, a, st);
-            else launch_class<0, false, false>(ipls[c], nblk, a, st);
+            if (b.nnb_le) launch_class<1, false, false, false>(ipls[c], nblk, a, st);
+            else launch_class<0, false, false, false>(ipls[c], nblk, a, st);
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -124,7 +124,7 @@ void tile_walk(int n, long long n_e, long long n_s, int max_ipl, std::vector<Til
 }  // namespace
 
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, int max_ipl, Plan &P, int n_seg)
+               int chunk_j, int max_ipl, Plan &P, int n_seg, bool mono)
 {
     // the item array keeps its size from the previous plan (every slot is
     // written below, so growing it is the only initialisation it needs)
@@ -132,6 +132,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     kept.swap(P.items);
     P.reset();
     P.items.swap(kept);
+    P.mono = mono;
     P.walks.resize(n_walk);
     P.es_off.assign(n_walk + 1, 0);
     P.ss_off.assign(n_walk + 1, 0);
@@ -148,8 +149,8 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         io += n_epi[w];
         eo += n_epj[w];
         so += n_spj[w];
-        P.es_off[w + 1] = P.es_off[w] + epj_region_slots(n_epj[w]);
-        P.ss_off[w + 1] = P.ss_off[w] + n_spj[w];
+        P.es_off[w + 1] = P.es_off[w] + epj_region_slots((long long)n_epj[w] + (mono ? n_spj[w] : 0));
+        P.ss_off[w + 1] = P.ss_off[w] + (mono ? 0 : n_spj[w]);
         P.n_epep += (long long)n_epi[w] * n_epj[w];
         P.n_epsp += (long long)n_epi[w] * n_spj[w];
         if (io > INT_MAX || eo > INT_MAX || so > INT_MAX) return false;
@@ -170,9 +171,9 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     static thread_local Scratch S;
     // chunking of walk w: nch chunks, EPJ block / SPJ record bounds
     auto chunks = [&](int w, std::vector<long long> &eb, std::vector<long long> &sb) -> long long {
-        const WalkDev &W = P.walks[w];
         const long long nb = (P.es_off[w + 1] - P.es_off[w]) / 8;
-        const long long units = nb * 8 + 2LL * W.n_s;
+        const long long n_s = P.ss_off[w + 1] - P.ss_off[w];  // staged SPJ records
+        const long long units = nb * 8 + 2LL * n_s;
         long long nch = (units + cj - 1) / cj;
         if (nch < 1) nch = 1;
         // chunk k covers list units [k*units/nch, (k+1)*units/nch); EPJ
@@ -182,10 +183,10 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
         for (long long k = 0; k <= nch; k++) {
             const long long u = (k == nch) ? units : (k * units) / nch;
             eb[k] = std::min((std::min(u, nb * 8) + 7) / 8, nb);
-            sb[k] = std::min((std::max(0LL, u - nb * 8) + 1) / 2, (long long)W.n_s);
+            sb[k] = std::min((std::max(0LL, u - nb * 8) + 1) / 2, n_s);
         }
         eb[nch] = nb;
-        sb[nch] = W.n_s;
+        sb[nch] = n_s;
         return nch;
     };
     // 1. per walk: items (tiles x chunks), reduction items and partial slots
@@ -202,7 +203,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
             const WalkDev &W = P.walks[w];
             if (W.n_i <= 0) continue;
             const long long nch = chunks(w, eb, sb);
-            tile_walk(W.n_i, W.n_e, W.n_s, max_ipl, tiles);
+            tile_walk(W.n_i, mono ? (long long)W.n_e + W.n_s : W.n_e, mono ? 0 : W.n_s, max_ipl, tiles);
             long long ipl_sum = 0;
             for (const Tile &t : tiles) ipl_sum += t.ipl;
             walk_item_begin[w + 1] = tiles.size() * (size_t)nch;
@@ -230,7 +231,7 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
             const WalkDev &W = P.walks[w];
             if (W.n_i <= 0) continue;
             const long long nch = chunks(w, eb, sb);
-            tile_walk(W.n_i, W.n_e, W.n_s, max_ipl, tiles);
+            tile_walk(W.n_i, mono ? (long long)W.n_e + W.n_s : W.n_e, mono ? 0 : W.n_s, max_ipl, tiles);
             size_t o = walk_item_begin[w];
             int r_o = walk_ritem_begin[w];
             int part = (int)walk_part_begin[w];

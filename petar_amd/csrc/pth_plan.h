@@ -46,6 +46,7 @@ struct Plan {
     int n_real_items = 0;
     long long ni_tot = 0, n_e_slots = 0, n_s_rec = 0;
     long long n_epep = 0, n_epsp = 0;       // sum n_i n_e, sum n_i n_s
+    bool mono = false;                      // monopole SPJ staged as far EPJ entries
     std::vector<PlanSeg> segs;              // at least one
     // empties the plan, keeping the vectors' memory for the next one
     void reset()
@@ -59,15 +60,18 @@ struct Plan {
         for (int k = 0; k < PLAN_CLASSES; k++) seg_begin[k] = seg_count[k] = 0;
         n_part = n_real_items = 0;
         ni_tot = n_e_slots = n_s_rec = n_epep = n_epsp = 0;
+        mono = false;
     }
 };
 
-// The parameters a plan was made with (exec_mode, chunk_j, max_ipl, n_seg).
+// The parameters a plan was made with (exec_mode, chunk_j, max_ipl, n_seg,
+// monopole superparticles).
 struct PlanKey {
-    int exec_mode = 0, chunk_j = 0, max_ipl = 0, n_seg = 1;
+    int exec_mode = 0, chunk_j = 0, max_ipl = 0, n_seg = 1, mono = 0;
     bool operator==(const PlanKey &o) const
     {
-        return exec_mode == o.exec_mode && chunk_j == o.chunk_j && max_ipl == o.max_ipl && n_seg == o.n_seg;
+        return exec_mode == o.exec_mode && chunk_j == o.chunk_j && max_ipl == o.max_ipl && n_seg == o.n_seg &&
+               mono == o.mono;
     }
 };
 
@@ -79,9 +83,12 @@ inline long long epj_region_slots(long long n_e) { return n_e > 0 ? (n_e + 7) / 
 // largest list length (EPJ entries + 2 x SPJ entries) of one item (<= 0:
 // default), max_ipl the largest number of i per lane (1 to 4), n_seg the
 // number of walk segments of about equal cost (fused form; the items of a
-// walk do not depend on it).  Returns false if a size overflows the int32
-// ranges of the device records.
+// walk do not depend on it).  With mono (monopole superparticles) a walk's
+// SPJ entries are staged as far EPJ entries after its EPJ list (a monopole
+// pair is a far EPJ pair: no cutoff, no neighbour count), so its staged EPJ
+// region holds n_epj + n_spj entries and it has no staged SPJ records.
+// Returns false if a size overflows the int32 ranges of the device records.
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, int max_ipl, Plan &P, int n_seg = 1);
+               int chunk_j, int max_ipl, Plan &P, int n_seg = 1, bool mono = false);
 
 }  // namespace pth

@@ -82,7 +82,7 @@ struct Layout {
     Arena A{};
     size_t epj = 0, spj = 0, upload = 0;
     size_t centre = 0, near_blk = 0, staged_i = 0, staged_e = 0, staged_s = 0, part_accp = 0, part_nnb = 0,
-           out_accp = 0, out_nnb = 0, total = 0;
+           staged_il = 0, staged_el = 0, out_accp = 0, out_nnb = 0, total = 0;
 };
 
 // Host records of the plain mode (FDPS-like array-of-structs).
@@ -163,7 +163,7 @@ struct pth_server {
 
 namespace {
 
-void layout(SCluster &c, long long n_ide, long long n_ids)
+void layout(SCluster &c, long long n_ide, long long n_ids, bool df)
 {
     const Plan &P = c.plan;
     Layout &L = c.L;
@@ -181,6 +181,10 @@ void layout(SCluster &c, long long n_ide, long long n_ids)
     L.staged_s = o; o = align256(o + sizeof(float) * 12 * (size_t)std::max<long long>(P.n_s_rec, 1));
     L.part_accp = o; o = align256(o + sizeof(double4) * 64 * np);
     L.part_nnb = o; o = align256(o + sizeof(int) * 64 * np);
+    if (df) {
+        L.staged_il = o; o = align256(o + sizeof(float4) * ni);
+        L.staged_el = o; o = align256(o + sizeof(float) * (PTH_DF_EXTRA / 8) * (size_t)std::max<long long>(P.n_e_slots, 8));
+    }
     L.out_accp = o; o = align256(o + sizeof(double4) * ni);
     L.out_nnb = o; o = align256(o + sizeof(int) * ni);
     L.total = o;
@@ -223,6 +227,11 @@ void bind(SCluster &c, char *base, const pth_params &par)
     b.part_nnb = reinterpret_cast<int *>(base + L.part_nnb);
     b.out_accp = reinterpret_cast<double4 *>(base + L.out_accp);
     b.out_nnb = reinterpret_cast<int *>(base + L.out_nnb);
+    b.df = df_enabled(par);
+    if (b.df) {
+        b.staged_il = reinterpret_cast<float4 *>(base + L.staged_il);
+        b.staged_el = reinterpret_cast<float *>(base + L.staged_el);
+    }
 }
 
 hipError_t save_results(pth_server *s, SCluster &c)
@@ -439,6 +448,8 @@ int pth_server_create(pth_ctx *ctx, pth_server **out, int mode, size_t hbm_budge
     if (mode != PTH_SERVE_FUSED && mode != PTH_SERVE_PLAIN)
         return ctx->fail(PTH_ERR_ARG, "pth_server_create: unknown mode");
     if (ctx->par.changeover) return ctx->fail(PTH_ERR_ARG, "pth_server_create: the changeover mode is not served");
+    if (ctx->par.spj_monopole)
+        return ctx->fail(PTH_ERR_ARG, "pth_server_create: monopole superparticles are not served");
     if (n_streams < 1 || n_streams > 16) return ctx->fail(PTH_ERR_ARG, "pth_server_create: n_streams outside [1, 16]");
     HIPCHK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     pth_server *s = new pth_server;
@@ -523,7 +534,7 @@ int pth_server_add_cluster(pth_server *s, int *cluster, int n_walk, const int *n
         pth_params par = ctx->par;
         if (!make_plan(n_walk, n_epi, n_epj, n_spj, PTH_EXEC_FUSED, par.chunk_j, par.max_ipl, c->plan))
             return ctx->fail(PTH_ERR_CAPACITY, "pth_server_add_cluster: sizes exceed the int32 index range");
-        layout(*c, n_ide, n_ids);
+        layout(*c, n_ide, n_ids, df_enabled(par) != 0);
         if (c->L.total > s->budget)
             return ctx->fail(PTH_ERR_CAPACITY, "pth_server_add_cluster: the cluster needs " +
                                                    std::to_string(c->L.total) + " bytes, more than the budget");

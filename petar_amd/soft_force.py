@@ -44,6 +44,8 @@ EPJ_SOFT = np.dtype([("id", "<i8"), ("mass", "<f8"), ("pos", "<f8", 3), ("vel", 
 SPJ_SOFT = np.dtype([("mass", "<f8"), ("pos", "<f8", 3),
                      ("quad", [("xx", "<f8"), ("yy", "<f8"), ("zz", "<f8"), ("xy", "<f8"), ("xz", "<f8"),
                                ("yz", "<f8")])])
+# PS::SPJMonopoleInAndOut, the superparticle of a build without USE_QUAD
+SPJ_SOFT_MONO = np.dtype([("mass", "<f8"), ("pos", "<f8", 3)])
 FORCE_SOFT = np.dtype([("acc", "<f8", 3), ("pot", "<f8"), ("n_ngb", "<i8")])
 # ForceSoft with the total potential the changeover mode also returns (pot
 # is then the changeover-weighted soft potential)
@@ -55,7 +57,14 @@ class Params(C.Structure):
     _fields_ = [("eps2", C.c_double), ("rcut2", C.c_double), ("G", C.c_double), ("nnb_mode", C.c_int),
                 ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int), ("max_ipl", C.c_int),
                 ("j_vgpr", C.c_int), ("changeover", C.c_int), ("ngb_factor", C.c_double), ("segments", C.c_int),
-                ("streams", C.c_int)]
+                ("streams", C.c_int), ("df_near", C.c_int), ("spj_monopole", C.c_int)]
+
+
+class Profile(C.Structure):
+    """pth_profile: times (s) and counts of the FDPS hooks."""
+    _fields_ = [("copy_s", C.c_double), ("send_s", C.c_double), ("calc_s", C.c_double), ("recv_s", C.c_double),
+                ("n_walk", C.c_longlong), ("n_epi", C.c_longlong), ("n_epj", C.c_longlong), ("n_spj", C.c_longlong),
+                ("n_call", C.c_longlong)]
 
 
 class EpjView(C.Structure):
@@ -77,15 +86,16 @@ class ForceView(C.Structure):
                 ("off_nngb", C.c_size_t), ("nngb_bytes", C.c_int), ("off_pot_tot", C.c_longlong)]
 
 
-EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_create_error", "pth_destroy", "pth_last_error",
-            "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve", "pth_resident_create",
-            "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
+EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_get_params", "pth_create_error",
+            "pth_destroy", "pth_last_error", "pth_device", "pth_send_j", "pth_dispatch", "pth_retrieve",
+            "pth_resident_create", "pth_resident_changeover", "pth_resident_run", "pth_resident_sync", "pth_resident_results",
             "pth_resident_timing", "pth_resident_stats", "pth_resident_free", "pth_plan_debug",
             "pth_resident_replan", "pth_resident_autotune", "pth_gtree_build", "pth_gtree_sizes",
             "pth_gtree_download", "pth_gtree_timing", "pth_gtree_stats", "pth_gtree_free", "pth_resident_from_gtree",
             "pth_gtree_update", "pth_resident_refill", "pth_gtree_neighbors", "pth_gtree_neighbors_download",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
-            "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free")
+            "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free",
+            "pth_profile_enable", "pth_profile_take")
 
 
 def lib():
@@ -102,6 +112,9 @@ def lib():
     L.pth_params_default.argtypes = [C.POINTER(Params)]
     L.pth_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(Params)]
     L.pth_set_params.argtypes = [vp, C.POINTER(Params)]
+    L.pth_get_params.argtypes = [vp, C.POINTER(Params)]
+    L.pth_profile_enable.argtypes = [vp, C.c_int]
+    L.pth_profile_take.argtypes = [vp, C.POINTER(Profile)]
     L.pth_create_error.restype = C.c_char_p
     L.pth_destroy.argtypes = [vp]
     L.pth_last_error.argtypes = [vp]
@@ -170,7 +183,8 @@ class PthError(RuntimeError):
 
 
 def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=1000,
-                   max_ipl=None, j_vgpr=None, changeover=0, ngb_factor=None, segments=None, streams=None):
+                   max_ipl=None, j_vgpr=None, changeover=0, ngb_factor=None, segments=None, streams=None,
+                   df_near=None, spj_monopole=0):
     p = Params()
     lib().pth_params_default(C.byref(p))
     p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk = (
@@ -186,6 +200,9 @@ def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED,
         p.segments = segments
     if streams is not None:
         p.streams = streams
+    if df_near is not None:
+        p.df_near = df_near
+    p.spj_monopole = spj_monopole
     return p
 
 
@@ -233,6 +250,27 @@ class Context:
         if rc != PTH_OK:
             raise PthError(f"{what} failed ({rc}): {lib().pth_last_error(self._h).decode()}")
 
+    def get_params(self) -> Params:
+        p = Params()
+        self.check(lib().pth_get_params(self._h, C.byref(p)), "pth_get_params")
+        return p
+
+    def profile_enable(self, on=True):
+        """Times the hooks' copies and kernels with HIP events (pth_profile_enable)."""
+        self.check(lib().pth_profile_enable(self._h, int(bool(on))), "pth_profile_enable")
+
+    def profile_take(self) -> dict:
+        """What the hooks accumulated since the last take (pth_profile_take)."""
+        p = Profile()
+        self.check(lib().pth_profile_take(self._h, C.byref(p)), "pth_profile_take")
+        return {k: getattr(p, k) for k, _ in Profile._fields_}
+
+    def set_params(self, p: Params):
+        """Replaces every parameter (pth_set_params); batches in flight keep
+        the values they were dispatched with."""
+        self.check(lib().pth_set_params(self._h, C.byref(p)), "pth_set_params")
+        self.params = p
+
     def close(self):
         if self._h:
             for child in list(self._children):
@@ -247,7 +285,7 @@ class Context:
             pass
 
 
-def _views():
+def _views(mono=False):
     f = EPJ_SOFT.fields
     ev = EpjView(EPJ_SOFT.itemsize, f["pos"][1], f["mass"][1], f["r_search"][1], f["id"][1], f["r_in"][1],
                  f["r_out"][1])
@@ -256,6 +294,9 @@ def _views():
     qo = sf["quad"][1]
     sv = SpjView(SPJ_SOFT.itemsize, sf["pos"][1], sf["mass"][1], qo + qf["xx"][1], qo + qf["yy"][1],
                  qo + qf["zz"][1], qo + qf["xy"][1], qo + qf["xz"][1], qo + qf["yz"][1])
+    if mono:
+        mf = SPJ_SOFT_MONO.fields
+        sv = SpjView(SPJ_SOFT_MONO.itemsize, mf["pos"][1], mf["mass"][1], 0, 0, 0, 0, 0, 0)
     fi = EPI_SOFT.fields
     iv = EpiView(EPI_SOFT.itemsize, fi["pos"][1], fi["r_search"][1], fi["id"][1])
     return ev, sv, iv, force_view(FORCE_SOFT)
@@ -273,34 +314,56 @@ class CalcForceWithLinearCutoffMultiWalk:
     """The dispatch functor of src/force_gpu_cuda.hpp:101-133 over the C ABI.
 
     One process-wide context per rank, created at the first call (as the
-    reference's first-call initialisation, src/force_gpu_cuda.cu:547-571).
+    reference's first-call initialisation, src/force_gpu_cuda.cu:547-571)
+    and updated with pth_set_params when a functor with other parameters
+    calls it (as the C++ adapter does), so j arrays sent through one functor
+    serve the dispatches of another.  Contexts are closed at interpreter
+    exit.
     """
 
     _ctx = {}
 
     def __init__(self, my_rank=0, eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0,
-                 changeover=0):
+                 changeover=0, spj_monopole=0):
         self.my_rank, self.eps2, self.rcut2, self.G = my_rank, eps2, rcut2, G
         self.nnb_mode, self.exec_mode, self.chunk_j, self.changeover = nnb_mode, exec_mode, chunk_j, changeover
+        self.spj_monopole = spj_monopole
+
+    @classmethod
+    def close_contexts(cls):
+        """Closes the per-rank contexts (with the j arrays they hold); the next
+        call creates a fresh one."""
+        for c in cls._ctx.values():
+            c.close()
+        cls._ctx.clear()
+
+    def _wanted(self):
+        return dict(eps2=float(self.eps2), rcut2=float(self.rcut2), G=float(self.G), nnb_mode=int(self.nnb_mode),
+                    exec_mode=int(self.exec_mode), chunk_j=int(self.chunk_j), changeover=int(self.changeover),
+                    spj_monopole=int(self.spj_monopole))
 
     def context(self) -> Context:
-        key = (self.my_rank, self.eps2, self.rcut2, self.G, self.nnb_mode, self.exec_mode, self.chunk_j,
-               self.changeover)
-        c = self._ctx.get(key)
-        if c is None:
-            c = Context(self.my_rank, eps2=self.eps2, rcut2=self.rcut2, G=self.G, nnb_mode=self.nnb_mode,
-                        exec_mode=self.exec_mode, chunk_j=self.chunk_j, changeover=self.changeover)
-            self._ctx[key] = c
+        want = self._wanted()
+        c = self._ctx.get(self.my_rank)
+        if c is None or not c.handle:
+            c = Context(self.my_rank, **want)
+            self._ctx[self.my_rank] = c
+            return c
+        cur = c.get_params()
+        if any(getattr(cur, k) != v for k, v in want.items()):
+            for k, v in want.items():
+                setattr(cur, k, v)
+            c.set_params(cur)
         return c
 
     def __call__(self, tag, n_walk, epi, n_epi, id_epj, n_epj, id_spj, n_spj, epj, n_epj_tot, spj, n_spj_tot,
                  send_flag):
         c = self.context()
-        ev, sv, iv, _ = _views()
+        ev, sv, iv, _ = _views(bool(self.spj_monopole))
         L = lib()
         if send_flag:
             epj = np.ascontiguousarray(epj, dtype=EPJ_SOFT)
-            spj = np.ascontiguousarray(spj, dtype=SPJ_SOFT)
+            spj = np.ascontiguousarray(spj, dtype=SPJ_SOFT_MONO if self.spj_monopole else SPJ_SOFT)
             c.check(L.pth_send_j(c.handle, epj.ctypes.data, int(n_epj_tot), C.byref(ev), spj.ctypes.data,
                                  int(n_spj_tot), C.byref(sv)), "pth_send_j")
             return 0
@@ -350,7 +413,9 @@ def make_functor(*args, **kw):
 
 
 class Resident:
-    """A whole soft-force evaluation kept in HBM (pth_resident_*)."""
+    """A whole soft-force evaluation kept in HBM (pth_resident_*).  With a
+    context made with spj_monopole=1 the superparticles are monopoles and
+    spj_quad is not read."""
 
     def __init__(self, ctx: Context, n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
                  spj_pos, spj_mass, spj_quad, _tree=None):

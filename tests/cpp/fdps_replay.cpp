@@ -20,6 +20,12 @@
 // drop-in rate with every list crossing PCIe, from host records in and to
 // host records out.
 //
+// Built without USE_QUAD (fdps_replay_mono) the superparticles are
+// PS::SPJMonopoleInAndOut records, as in a PeTar build configured with
+// --disable-quad.  Built with GPU_PROFILE (fdps_replay_prof) it prints
+// gpu_counter and gpu_profile after the passes, as PeTar's profile output
+// does (src/petar.hpp:2123-2130).
+//
 // usage: fdps_replay N seed theta n_walk_limit r_out out_file [f_bin [timed_passes]]
 #include <algorithm>
 #include <chrono>
@@ -28,6 +34,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+
+#include <iostream>
 
 #include "particle_simulator.hpp"
 #include "soft_types.hpp"
@@ -109,12 +117,14 @@ int main(int argc, char **argv)
         epj_rout[k] = epj[k].r_out;
         epj_id[k] = epj[k].id;
     }
-    std::vector<PS::SPJQuadrupoleInAndOut> spj(n_node);
+    std::vector<SPJSoft> spj(n_node);
     for (long s = 0; s < n_node; s++) {
         spj[s].mass = smass[s];
         spj[s].pos = PS::F64vec(scom[3 * s], scom[3 * s + 1], scom[3 * s + 2]);
+#if PETAR_HIP_SPJ_QUAD
         const double *q = &squad[6 * s];
         spj[s].quad = PS::F64mat{q[0], q[1], q[2], q[3], q[4], q[5]};
+#endif
     }
     std::vector<std::vector<EPISoft>> epi(n_group);
     std::vector<std::vector<ForceSoft>> force(n_group);
@@ -172,6 +182,24 @@ int main(int argc, char **argv)
                     k, ms, inter / (ms * 1e6), inter, n_group, n_walk_limit);
         std::fflush(stdout);
     }
+
+#ifdef GPU_PROFILE
+    // what PeTar prints: names, then values per pass (src/petar.hpp:2123-2130)
+    std::printf("fdps_replay: gpu_counter n_walk %lld n_epi %lld n_epj %lld n_spj %lld n_call %lld\n",
+                gpu_counter.n_walk.n, gpu_counter.n_epi.n, gpu_counter.n_epj.n, gpu_counter.n_spj.n,
+                gpu_counter.n_call.n);
+    std::printf("fdps_replay: gpu_profile copy %.6e send %.6e receive %.6e calc_force %.6e\n", gpu_profile.copy.time,
+                gpu_profile.send.time, gpu_profile.recv.time, gpu_profile.calc.time);
+    gpu_profile.dumpName(std::cout);
+    gpu_counter.dumpName(std::cout);
+    std::cout << std::endl;
+    gpu_profile.dump(std::cout, 1 + timed_passes);
+    gpu_counter.dump(std::cout, 1 + timed_passes);
+    std::cout << std::endl;
+    gpu_profile.clear();
+    gpu_counter.clear();
+    if (gpu_profile.calc.time != 0.0 || gpu_counter.n_call.n != 0) die("clear");
+#endif
 
     std::vector<double> acc, pot, pot_tot;
     std::vector<long long> nngb;

@@ -33,14 +33,22 @@ def _read(path):
     return d, res + (pot_tot,)
 
 
-@pytest.mark.parametrize("n,walk_limit", [(20000, 200), (3000, 7)])
-def test_cpp_dropin_fdps_replay(tmp_path, n, walk_limit):
-    if not os.path.exists(EXE):
-        pytest.fail(f"{EXE} not built (build() compiles it)")
+@pytest.mark.parametrize("exe,n,walk_limit", [("fdps_replay", 20000, 200), ("fdps_replay", 3000, 7),
+                                              ("fdps_replay_mono", 20000, 200), ("fdps_replay_chunk", 20000, 200)])
+def test_cpp_dropin_fdps_replay(tmp_path, exe, n, walk_limit):
+    """The adapter as a default PeTar GPU build compiles it (USE_QUAD), as a
+    build configured with --disable-quad compiles it (monopole
+    superparticles: the oracle's monopole pair, src/soft_force.hpp:125-158),
+    and with -DPETAR_HIP_CHUNK_J=256 (work items of at most 256 list
+    entries: long lists split over several items and summed by k_reduce)."""
+    path = os.path.join(ROOT, "tests", "cpp", exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (build() compiles it)")
+    mono = exe.endswith("_mono")
     out = tmp_path / "replay.bin"
     r_out = 0.01
-    print(f"[dropin] N={n} walk_limit={walk_limit}", flush=True)
-    p = subprocess.run([EXE, str(n), "11", "0.3", str(walk_limit), str(r_out), str(out)], capture_output=True,
+    print(f"[dropin] {exe} N={n} walk_limit={walk_limit}", flush=True)
+    p = subprocess.run([path, str(n), "11", "0.3", str(walk_limit), str(r_out), str(out)], capture_output=True,
                        text=True, timeout=240)
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0
@@ -50,7 +58,7 @@ def test_cpp_dropin_fdps_replay(tmp_path, n, walk_limit):
              epi_rs=d["epj_rs"][idx], id_epj=d["id_epj"], id_spj=d["id_spj"], epj_pos=d["epj_pos"],
              epj_mass=d["epj_mass"], epj_rs=d["epj_rs"], spj_pos=d["spj_pos"], spj_mass=d["spj_mass"],
              spj_quad=d["spj_quad"])
-    racc, rpot, rnnb = O.soft_force_walks(**a, eps2=0.0, rout2=r_out ** 2)
+    racc, rpot, rnnb = O.soft_force_walks(**a, eps2=0.0, rout2=r_out ** 2, mono=mono)
     ea, ep = rel_err(acc, racc), rel_err(pot, rpot)
     print(f"[dropin] acc med {np.median(ea):.2e} p99 {np.percentile(ea, 99):.2e} max {ea.max():.2e} "
           f"pot med {np.median(ep):.2e}", flush=True)
@@ -62,7 +70,7 @@ def test_cpp_dropin_fdps_replay(tmp_path, n, walk_limit):
     # up to ~1e-2 of small net accelerations.  The tail must be no worse than
     # the reference kernel's (absolute FP32 coordinates) on the same lists.
     if O.ref_available():
-        pacc, _, _ = O.ref_soft_force_walks(**a, eps2=0.0, rout2=r_out ** 2)
+        pacc, _, _ = O.ref_soft_force_walks(**a, eps2=0.0, rout2=r_out ** 2, mono=mono)
         er = rel_err(pacc, racc)
         print(f"[dropin] reference kernel: acc med {np.median(er):.2e} p99 {np.percentile(er, 99):.2e} "
               f"max {er.max():.2e}", flush=True)
@@ -102,3 +110,37 @@ def test_cpp_dropin_changeover_mode(tmp_path):
     assert np.all(pot != 99.0) and np.all(pot_tot != 99.0)
     assert np.median(ea) <= 1e-6 and np.median(es) <= 1e-6 and np.median(et) <= 1e-6
     assert np.percentile(et, 99) <= 1e-5 and np.percentile(es, 99) <= 1e-5
+
+
+def test_cpp_dropin_gpu_profile(tmp_path):
+    """The adapter built with GPU_PROFILE (as every reference GPU build is,
+    and under C++11): gpu_counter holds the replayed sums (walks, i, EPJ and
+    SPJ list entries, dispatch calls) over all passes, gpu_profile's four
+    times are positive, the dump lines carry the nine columns PeTar's
+    profile reader expects, and clear() zeroes both."""
+    path = os.path.join(ROOT, "tests", "cpp", "fdps_replay_prof")
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (build() compiles it)")
+    out = tmp_path / "replay_prof.bin"
+    n, walk_limit, passes = 20000, 200, 3
+    p = subprocess.run([path, str(n), "11", "0.3", str(walk_limit), "0.01", str(out), "0.3", str(passes - 1)],
+                       capture_output=True, text=True, timeout=240)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0
+    d, _ = _read(out)
+    lines = p.stdout.splitlines()
+    cnt = [l for l in lines if "gpu_counter" in l][0].split()
+    got = {cnt[k]: int(cnt[k + 1]) for k in range(2, len(cnt) - 1, 2)}
+    n_group = d["n_epi"].size
+    want = dict(n_walk=passes * n_group, n_epi=passes * int(d["n_epi"].sum()),
+                n_epj=passes * int(d["n_epj"].sum()), n_spj=passes * int(d["n_spj"].sum()),
+                n_call=passes * ((n_group + walk_limit - 1) // walk_limit))
+    assert got == want
+    prof = [l for l in lines if "gpu_profile" in l][0].split()
+    times = {prof[k]: float(prof[k + 1]) for k in range(2, len(prof) - 1, 2)}
+    assert set(times) == {"copy", "send", "receive", "calc_force"} and all(v > 0 for v in times.values())
+    names = lines.index([l for l in lines if l.strip().startswith("copy")][0])
+    vals = lines[names + 1].split()
+    assert len(vals) == 9
+    np.testing.assert_allclose([float(v) for v in vals[4:]], [want[k] / passes for k in
+                                                              ("n_walk", "n_epi", "n_epj", "n_spj", "n_call")])

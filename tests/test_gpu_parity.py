@@ -103,9 +103,8 @@ def test_j_record_modes_are_bit_identical(c1, j_vgpr, max_ipl):
         assert np.array_equal(x, y)
 
 
-def _fdps_replay(L, functor, n_walk_limit=200):
-    """Replays FDPS's multi-walk call order (send once, then dispatch/retrieve
-    per batch of <= n_walk_limit walks) with FDPS-layout records."""
+def _epj_records(L):
+    """EPJSoft records of a host list set (ids = tree positions)."""
     n = L.epj_pos.shape[0]
     epj = np.zeros(n, sf.EPJ_SOFT)
     epj["id"] = np.arange(n)
@@ -113,14 +112,27 @@ def _fdps_replay(L, functor, n_walk_limit=200):
     epj["pos"] = L.epj_pos
     epj["r_search"] = L.epj_rs
     epj["r_out"] = R_OUT
-    ns = L.spj_mass.size
-    spj = np.zeros(ns, sf.SPJ_SOFT)
+    return epj
+
+
+def _spj_records(L):
+    """SPJQuadrupoleInAndOut records of a host list set."""
+    spj = np.zeros(L.spj_mass.size, sf.SPJ_SOFT)
     spj["mass"] = L.spj_mass
     spj["pos"] = L.spj_pos
     for k, q in enumerate(("xx", "yy", "zz", "xy", "xz", "yz")):
         spj["quad"][q] = L.spj_quad[:, k]
+    return spj
+
+
+def _fdps_replay(L, functor, n_walk_limit=200, send_j=True):
+    """Replays FDPS's multi-walk call order (send once, then dispatch/retrieve
+    per batch of <= n_walk_limit walks) with FDPS-layout records."""
+    epj, spj = _epj_records(L), _spj_records(L)
+    n, ns = epj.size, spj.size
     eoff, soff = L.walk_offsets()
-    functor(0, 0, [], [], [], [], [], [], epj, n, spj, ns, True)
+    if send_j:
+        functor(0, 0, [], [], [], [], [], [], epj, n, spj, ns, True)
     out = []
     for b0 in range(0, L.n_walk, n_walk_limit):
         ws = range(b0, min(b0 + n_walk_limit, L.n_walk))
@@ -276,7 +288,23 @@ def test_edge_cases():
             ctx.close()
 
 
+def test_functors_share_one_context(c1):
+    """Functors with different parameters use the rank's one context (its
+    parameters updated by pth_set_params): j arrays sent through one serve
+    the dispatch of another, and the dispatch uses the dispatching functor's
+    parameters (here G)."""
+    L, a, ref, amb = c1
+    sf.CalcForceWithLinearCutoffMultiWalk.close_contexts()
+    send = sf.make_functor(0, 0.0, R_OUT ** 2, 1.0)
+    send(0, 0, [], [], [], [], [], [], _epj_records(L), L.epj_mass.size, _spj_records(L), L.spj_mass.size, True)
+    res = _fdps_replay(L, sf.make_functor(0, 0.0, R_OUT ** 2, 2.0), send_j=False)
+    assert send.context() is sf.make_functor(0, 0.0, R_OUT ** 2, 2.0).context()
+    _check((res[0] / 2.0, res[1] / 2.0, res[2]), ref, amb, "functor with G = 2 on j arrays sent with G = 1")
+    sf.CalcForceWithLinearCutoffMultiWalk.close_contexts()
+
+
 def test_error_paths():
+    sf.CalcForceWithLinearCutoffMultiWalk.close_contexts()
     ctx = sf.Context(0, rcut2=1e-4)
     fn = sf.CalcForceWithLinearCutoffMultiWalk(0, 0.0, 1e-4, 1.0)
     # retrieve before any dispatch
@@ -346,7 +374,12 @@ def test_theta0_is_direct_summation():
           f"pot med {np.median(ep):.3e}; nnb mismatches {(nnb != dnnb).sum()}", flush=True)
     assert np.median(ea) <= MED and np.percentile(ea, 99) <= P99 and ea.max() <= MAX
     assert np.median(ep) <= MED and np.percentile(ep, 99) <= P99
-    assert np.mean(nnb != dnnb) <= 1e-3
+    # every j is an EPJ: the band of the walk-relative FP32 pair arithmetic
+    # covers every pair the direct sum counts
+    a = O.lists_args(L)
+    amb = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=0.0)
+    bad_exact, bad_band, _ = nnb_check(nnb, dnnb, amb)
+    assert bad_exact == 0 and bad_band == 0
 
 
 @pytest.mark.parametrize("segments,streams", [(1, 1), (4, 2), (8, 4), (32, 8)])
@@ -365,3 +398,64 @@ def test_segments_are_bit_identical(c1, segments, streams):
         ctx.close()
     assert all(np.array_equal(x, y) for x, y in zip(out[0], out[1]))
     _check(out[1], ref, amb, f"segments={segments} streams={streams}")
+
+
+@pytest.mark.parametrize("exec_mode", [sf.EXEC_FUSED, sf.EXEC_PLAIN])
+def test_monopole_superparticles(c1, exec_mode):
+    """spj_monopole = 1 (a PeTar build without USE_QUAD): the superparticles'
+    mass and centre only, the monopole pair of src/soft_force.hpp:125-158,
+    against the FP64 oracle's monopole restatement; the evaluation fed from
+    the device tree equals the one fed from the host lists bit for bit."""
+    L, a, _, amb = c1
+    ref = O.soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2, mono=True)
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, exec_mode=exec_mode, spj_monopole=1)
+    r = sf.Resident(ctx, L.n_epi, L.n_epj, L.n_spj, a["epi_pos"], a["epi_rs"], L.id_epj, L.id_spj, L.epj_pos,
+                    L.epj_mass, L.epj_rs, L.spj_pos, L.spj_mass, np.zeros(0))
+    r.run()
+    res = r.results()
+    r.close()
+    _check(res, ref, amb, f"monopole exec={exec_mode}")
+    if exec_mode == sf.EXEC_FUSED:
+        m = np.zeros(L.epj_mass.size)
+        m[L.order] = L.epj_mass
+        x = np.zeros_like(L.epj_pos)
+        x[L.order] = L.epj_pos
+        g = sf.GpuTree(ctx, x, m, np.full(m.size, 1.5 * R_OUT), theta=0.3)
+        rd = sf.Resident.from_gtree(ctx, g)
+        rd.run()
+        dev = rd.results()
+        rd.close()
+        g.close()
+        assert all(np.array_equal(p, q) for p, q in zip(dev, res))
+    ctx.close()
+
+
+def test_double_float_near_pairs():
+    """Linear-cutoff mode with df_near = 1: near EPJ pairs take their
+    separation from double-float walk-relative coordinates.  On a cluster of
+    tight binaries moved far from the origin the acc error against the FP64
+    oracle stays at FP32 rounding (max <= 1e-5), where the FP32-only
+    separations (df_near = 0) leave an error of 2^-24 |x - c| in each close
+    pair's separation; the neighbour counts stay exact outside the band."""
+    m, x, _ = host.cluster(16384, "plummer", kroupa=True, f_bin=0.5, seed=21)
+    L = host.build_lists(x + np.array([300.0, -20.0, 5.0]), m, 1.5 * R_OUT, theta=0.3)
+    a = O.lists_args(L)
+    ref = O.soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2)
+    amb = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=0.0)
+    err = {}
+    for df in (0, 1):
+        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, df_near=df)
+        r = sf.Resident.from_lists(ctx, L)
+        r.run()
+        acc, pot, nnb = r.results()
+        r.close()
+        ctx.close()
+        ea, ep = rel_err(acc, ref[0]), rel_err(pot, ref[1])
+        err[df] = ea
+        print(f"[df_near={df}] acc med {np.median(ea):.2e} p99 {np.percentile(ea, 99):.2e} max {ea.max():.2e}; "
+              f"pot med {np.median(ep):.2e} max {ep.max():.2e}", flush=True)
+        bad_exact, bad_band, _ = nnb_check(nnb, ref[2], amb)
+        assert bad_exact == 0 and bad_band == 0
+        assert np.median(ea) <= MED and np.median(ep) <= MED
+    assert err[1].max() <= 1e-5 and np.percentile(err[1], 99) <= 1e-6
+    assert err[1].max() < err[0].max()

@@ -7,7 +7,7 @@ import pytest
 
 from oracle import oracle_py as O
 from petar_amd import host, soft_force as sf
-from tests._common import R_OUT, rel_err
+from tests._common import R_OUT, nnb_check, rel_err, walk_centres
 
 pytestmark = pytest.mark.gpu
 
@@ -20,8 +20,10 @@ def clusters():
     for k, n in enumerate(SIZES):
         m, x, _ = host.cluster(n, "plummer", kroupa=True, f_bin=0.3, seed=100 + k)
         L = host.build_lists(x, m, 1.5 * R_OUT, theta=0.3)
-        ref = O.soft_force_walks(**O.lists_args(L), eps2=0.0, rout2=R_OUT ** 2)
-        out.append((L, ref))
+        a = O.lists_args(L)
+        ref = O.soft_force_walks(**a, eps2=0.0, rout2=R_OUT ** 2)
+        amb = O.nnb_ambiguous(**a, centres=walk_centres(a), eps2=0.0)
+        out.append((L, ref + (amb,)))
     return out
 
 
@@ -38,7 +40,8 @@ def _oracle_ok(res, ref):
     ea, ep = rel_err(res[0], ref[0]), rel_err(res[1], ref[1])
     assert np.median(ea) <= 1e-6 and np.percentile(ea, 99) <= 1e-3 and ea.max() <= 1e-1
     assert np.median(ep) <= 1e-6 and np.percentile(ep, 99) <= 1e-5
-    assert np.mean(res[2] != ref[2]) <= 1e-3
+    bad_exact, bad_band, _ = nnb_check(res[2], ref[2], ref[3])
+    assert bad_exact == 0 and bad_band == 0
 
 
 def _same(a, b):

@@ -103,6 +103,56 @@ def test_oracle_vs_reference_kernel_tree_lists(eps):
     assert np.all(d[amb == 0] == 0) and np.all(d <= amb)
 
 
+# ---- monopole superparticles (builds without USE_QUAD) ----------------------------
+
+def test_monopole_spj_is_the_quadrupole_pair_without_moments():
+    """The monopole pair (src/soft_force.hpp:125-158) equals the quadrupole
+    pair of src/soft_force.hpp:160-200 with a zero second moment, term for
+    term, on tree lists with softening."""
+    L = c1_lists(4096, seed=6)
+    a = O.lists_args(L)
+    mono = O.soft_force_walks(**a, eps2=1e-6, rout2=R_OUT ** 2, part="spj", mono=True)
+    a0 = dict(a, spj_quad=np.zeros_like(L.spj_quad))
+    quad0 = O.soft_force_walks(**a0, eps2=1e-6, rout2=R_OUT ** 2, part="spj")
+    assert rel_err(mono[0], quad0[0]).max() < 1e-13 and rel_err(mono[1], quad0[1]).max() < 1e-13
+    # and differs from the quadrupole evaluation by the quadrupole terms
+    quad = O.soft_force_walks(**a, eps2=1e-6, rout2=R_OUT ** 2, part="spj")
+    assert np.median(rel_err(mono[0], quad[0])) > 1e-6
+
+
+def test_monopole_spj_two_body():
+    """One superparticle of mass m at separation d: acc = G m d / (d^2 +
+    eps^2)^(3/2) towards it, pot = -G m / sqrt(d^2 + eps^2), no cutoff."""
+    xi, xs, m, G, eps2 = np.array([0.1, -0.2, 0.3]), np.array([0.1003, -0.2004, 0.3]), 2.5e-3, 1.7, 1e-8
+    w = dict(n_epi=[1], n_epj=[0], n_spj=[1], epi_pos=np.array([xi]), epi_rs=[0.0], id_epj=[], id_spj=[0],
+             epj_pos=[], epj_mass=[], epj_rs=[], spj_pos=np.array([xs]), spj_mass=[m], spj_quad=np.ones((1, 6)))
+    a, p, nb = O.soft_force_walks(**w, eps2=eps2, rout2=R_OUT ** 2, G=G, mono=True)
+    d = xs - xi
+    r2 = d @ d + eps2
+    np.testing.assert_allclose(a[0], G * m * d / r2 ** 1.5, rtol=1e-14)
+    assert p[0] == pytest.approx(-G * m / np.sqrt(r2), rel=1e-14) and nb[0] == 0
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference kernel not built (needs /root/reference)")
+@pytest.mark.parametrize("eps", [0.0, 0.5])
+def test_monopole_oracle_vs_reference_kernel(eps):
+    """The monopole restatement against the reference x86 kernel driven as
+    CalcForceEpSpMonoSimd drives it (src/soft_force.hpp:427-480: run_epj on
+    the superparticles' mass and centre, one Newton step on its rsqrt)."""
+    eps2 = (eps * R_OUT) ** 2
+    L = c1_lists(8192, seed=8)
+    a = O.lists_args(L)
+    acc, pot, _ = O.soft_force_walks(**a, eps2=eps2, rout2=R_OUT ** 2, part="spj", mono=True)
+    racc, rpot, _ = O.ref_soft_force_walks(**a, eps2=eps2, rout2=R_OUT ** 2, part="spj", mono=True)
+    ea, ep = rel_err(racc, acc), rel_err(rpot, pot)
+    assert np.median(ea) < 2e-6 and ea.max() < 5e-5
+    assert np.median(ep) < 2e-6 and ep.max() < 5e-5
+    # a planted sign error (the repulsive form) is caught
+    w = dict(a, spj_mass=-L.spj_mass)
+    bad, _, _ = O.soft_force_walks(**w, eps2=eps2, rout2=R_OUT ** 2, part="spj", mono=True)
+    assert np.median(rel_err(racc, bad)) > 1.0
+
+
 # ---- closed forms ----------------------------------------------------------------
 
 def _pair(xi, xj, mj, rsi=0.0, rsj=0.0, eps2=0.0, rout2=R_OUT ** 2, G=1.0, mode=0):
