@@ -166,42 +166,83 @@ def build_serving(n_clusters=256, seed=4, f_bin=0.3, r_out=0.01, nthreads=0):
     return out, sizes
 
 
-def serving(r_out, n_streams=4, loads=(0.5, 0.8, 0.95), seed=5, accuracy_clusters=6):
-    """Fused server vs plain-form server on the 256-cluster stream."""
+def _sweep(srv, ids, ints, cap_req_s, loads, rng, label):
+    """Poisson arrivals of every cluster twice, at each load (fraction of the
+    saturated request rate cap_req_s): p50/p99 latency over all requests and
+    over those that had to upload their cluster."""
+    out = []
+    for f in loads:
+        seq = rng.permutation(np.repeat(ids, 2))
+        arr = np.cumsum(rng.exponential(1.0 / (f * cap_req_s), seq.size))
+        arr -= arr[0]
+        st0 = srv.stats()
+        lat, wall, adm = srv.replay(seq, arr, admitted=True)
+        st1 = srv.stats()
+        row = dict(load=f, rate_req_s=round(f * cap_req_s, 1), p50_ms=round(float(np.percentile(lat, 50)), 4),
+                   p99_ms=round(float(np.percentile(lat, 99)), 4),
+                   gint_s=round(float(ints[seq].sum() / wall / 1e9), 2), admissions=int(adm.sum()),
+                   evictions=int(st1["evicted"] - st0["evicted"]))
+        if adm.any():
+            row["admitted_p50_ms"] = round(float(np.percentile(lat[adm], 50)), 4)
+            row["admitted_p99_ms"] = round(float(np.percentile(lat[adm], 99)), 4)
+        out.append(row)
+        log(f"[bench] serving {label} load {f}: {row}")
+    return out
+
+
+def serving(r_out, n_streams=4, loads=(0.5, 0.8, 0.95), seed=5, accuracy_clusters=12, budget_frac=0.25):
+    """Fused server vs plain-form server on the 256-cluster stream: capacity
+    and a Poisson load sweep with every cluster resident, then the same sweep
+    under a binding HBM budget (every stream's scratch plus budget_frac of the
+    clusters' resident bytes), where requests evict and re-admit clusters;
+    every cluster's results alone (one request at a time) against its results
+    after the mixes (bit-identical), sampled walks of 12 clusters against the
+    FP64 oracle."""
     from petar_amd import soft_force as sf
     from oracle import oracle_py as O
     lists, sizes = build_serving(r_out=r_out)
     ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0)
     srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=0, n_streams=n_streams)
+    t0 = time.time()
     ids = [srv.add_lists(L) for L in lists]
+    add_s = time.time() - t0
     ints = np.array([srv.interactions(k) for k in ids], dtype=np.int64)
+    mem = srv.memory()
     rng = np.random.default_rng(seed)
-    # first request of each cluster: admission (upload + graph capture)
-    lat_first, _ = srv.replay(ids, np.zeros(len(ids)))
-    alone = {k: srv.results(k) for k in ids[:accuracy_clusters]}
+    # alone: one request at a time (the first of each cluster admits it)
+    lat_alone, alone = [], {}
+    for k in ids:
+        lat_alone.append(srv.wait(srv.submit(k)))
+        alone[k] = srv.results(k)
     # saturated: every cluster once, all arriving at t = 0
     order = rng.permutation(ids)
     _, wall_sat = srv.replay(order, np.zeros(len(order)))
     cap_req_s = len(order) / wall_sat
     sat_gint = ints.sum() / wall_sat / 1e9
-    sweep = []
-    for f in loads:
-        seq = rng.permutation(np.repeat(ids, 2))
-        arr = np.cumsum(rng.exponential(1.0 / (f * cap_req_s), seq.size))
-        arr -= arr[0]
-        lat, wall = srv.replay(seq, arr)
-        sweep.append(dict(load=f, rate_req_s=round(f * cap_req_s, 1), p50_ms=round(float(np.percentile(lat, 50)), 4),
-                          p99_ms=round(float(np.percentile(lat, 99)), 4),
-                          gint_s=round(float(ints[seq].sum() / wall / 1e9), 2)))
-        log(f"[bench] serving load {f}: {sweep[-1]}")
-    same = sum(all(np.array_equal(a, b) for a, b in zip(srv.results(k), alone[k])) for k in alone)
+    sweep = _sweep(srv, ids, ints, cap_req_s, loads, rng, "resident")
+    same = sum(all(np.array_equal(a, b) for a, b in zip(srv.results(k), alone[k])) for k in ids)
     st = srv.stats()
+    mem_after = srv.memory()
     srv.close()
-    # accuracy of a few clusters against the FP64 oracle (same lists)
+    # a binding budget: clusters are evicted and re-admitted throughout
+    budget = int(n_streams * mem["scratch_max"] + budget_frac * mem["resident_total"])
+    srvb = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=budget, n_streams=n_streams)
+    idb = [srvb.add_lists(L) for L in lists]
+    _, wall_b = srvb.replay(rng.permutation(idb), np.zeros(len(idb)))
+    cap_b = len(idb) / wall_b
+    sweep_b = _sweep(srvb, idb, ints, cap_b, (0.5, 0.8), rng, "budget")
+    same_b = sum(all(np.array_equal(a, b) for a, b in zip(srvb.results(kb), alone[k])) for k, kb in zip(ids, idb))
+    stb = srvb.stats()
+    memb = srvb.memory()
+    srvb.close()
+    # accuracy: sampled walks of 12 clusters across the size range
     errs = []
-    for k in list(alone)[:accuracy_clusters]:
-        ref = O.soft_force_walks(**O.lists_args(lists[k]), eps2=0.0, rout2=r_out ** 2)
-        errs.append(np.linalg.norm(alone[k][0] - ref[0], axis=1) / np.linalg.norm(ref[0], axis=1))
+    for k in np.argsort(sizes)[np.linspace(0, len(ids) - 1, accuracy_clusters).astype(int)]:
+        L = lists[k]
+        w = np.sort(rng.choice(L.n_walk, min(L.n_walk, 24), replace=False))
+        a, rows = walk_subset(L, w)
+        ref = O.soft_force_walks(**a, eps2=0.0, rout2=r_out ** 2)
+        errs.append(np.linalg.norm(alone[ids[k]][0][rows] - ref[0], axis=1) / np.linalg.norm(ref[0], axis=1))
     errs = np.concatenate(errs)
     # plain per-walk form, same stream, saturated
     srvp = sf.Server(ctx, sf.SERVE_PLAIN)
@@ -210,15 +251,29 @@ def serving(r_out, n_streams=4, loads=(0.5, 0.8, 0.95), seed=5, accuracy_cluster
     srvp.close()
     ctx.close()
     plain_gint = ints.sum() / wall_p / 1e9
-    return dict(clusters=len(ids), particles=int(sizes.sum()), interactions_per_pass=int(ints.sum()),
+    n_part = int(sizes.sum())
+    return dict(clusters=len(ids), particles=n_part, interactions_per_pass=int(ints.sum()),
                 n_streams=n_streams, saturated_gint_s=round(float(sat_gint), 2),
                 saturated_req_s=round(float(cap_req_s), 1), load_sweep=sweep,
-                first_request_ms_p50=round(float(np.percentile(lat_first, 50)), 3),
+                alone_ms_p50=round(float(np.percentile(lat_alone, 50)), 3),
+                register_s=round(add_s, 2),
                 plain_form_saturated_gint_s=round(float(plain_gint), 2),
                 speedup_vs_plain=round(float(sat_gint / plain_gint), 2),
-                alone_vs_mix_bit_identical=f"{same}/{len(alone)}",
+                alone_vs_mix_bit_identical=f"{same}/{len(ids)}",
                 acc_rel_err_median=float(np.median(errs)), acc_rel_err_p99=float(np.percentile(errs, 99)),
-                pool_peak_bytes=st["pool_peak"], evictions=st["evicted"])
+                acc_rel_err_max=float(errs.max()), accuracy_sample_i=int(errs.size),
+                resident_bytes_per_particle=round(mem["resident_total"] / n_part, 1),
+                scratch_bytes_per_stream=mem["scratch_max"], pool_peak_bytes=st["pool_peak"],
+                evictions=st["evicted"], graph_captures=mem_after["captures"],
+                graph_instantiations=mem_after["instantiations"],
+                binding_budget=dict(budget_bytes=budget, budget_note=f"{n_streams} streams' scratch + "
+                                    f"{budget_frac} of the clusters' resident bytes",
+                                    saturated_req_s=round(float(cap_b), 1),
+                                    saturated_gint_s=round(float(ints.sum() / wall_b / 1e9), 2),
+                                    load_sweep=sweep_b, admissions=stb["admitted"], evictions=stb["evicted"],
+                                    pool_peak_bytes=stb["pool_peak"], graph_captures=memb["captures"],
+                                    graph_instantiations=memb["instantiations"],
+                                    alone_vs_mix_bit_identical=f"{same_b}/{len(ids)}"))
 
 
 def read_traffic(name):

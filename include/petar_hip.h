@@ -121,10 +121,16 @@ typedef struct pth_params {
  * src/soft_ptcl.hpp:97-101,349-353), j != i, the linear-cutoff terms are
  * replaced by G m (1 - W_acc(r)) dx / r^3 and -G m (1 - W_pot(r)) / r (pot)
  * and -G m / r (pot_tot), W taken from the pair member with the larger r_out;
- * the self term -G m_i / r_out is dropped.  Every j is treated as a single
- * (group_data is not read).  Requires EPJ ids and r_in/r_out (epj view),
- * EPI ids (epi view), each i's own EPJ in its walk's list, and at most 1024
- * i per walk.  pot receives pot_soft; pot_tot goes to force view member
+ * the self term -G m_i / r_out is dropped.  With the EPJ group data
+ * (GroupDataDeliver::artificial) the particle types follow
+ * src/hard.hpp:1453-1472 and 1669: a group member j (status < 0, zero tree
+ * mass) adds its backup mass's -G m_b (1 - W_pot)/r and -G m_b/r to the
+ * potentials, an artificial or c.m. particle j (status > 0) leaves no
+ * potential (acc as a single in every case), and only singles and members
+ * without a c.m. address (status -LARGE_FLOAT, taken as below -1e30) lose
+ * their self term.  Requires EPJ ids and r_in/r_out (epj view), EPI ids
+ * (epi view), each i's own EPJ in its walk's list, and at most 1024 i per
+ * walk.  pot receives pot_soft; pot_tot goes to force view member
  * off_pot_tot when that is >= 0. */
 
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
@@ -182,6 +188,10 @@ int pth_device(const pth_ctx *ctx);
 typedef struct pth_epj_view { /* EPJSoft */
     size_t stride, off_pos, off_mass, off_rsearch;
     size_t off_id, off_rin, off_rout; /* changeover mode only */
+    long long off_group; /* changeover mode: EPJSoft::group_data, whose
+                            artificial member holds mass_backup then status
+                            (two doubles, src/artificial_particles.hpp:20-23);
+                            -1: every particle is a single */
 } pth_epj_view;
 typedef struct pth_spj_view { /* SPJQuadrupoleInAndOut */
     size_t stride, off_pos, off_mass, off_qxx, off_qyy, off_qzz, off_qxy, off_qxz, off_qyz;
@@ -226,10 +236,11 @@ int pth_resident_create(pth_ctx *ctx, pth_resident **res, int n_walk, const int 
 int pth_resident_run(pth_resident *res);
 /* Waits for the last evaluation. */
 int pth_resident_sync(pth_resident *res);
-/* Changeover mode: the ids of the EPI (concatenated per walk) and the ids
- * and changeover radii of the EPJ; must be given before the first run. */
+/* Changeover mode: the ids of the EPI (concatenated per walk) and the ids,
+ * changeover radii and group data ([n][2]: mass_backup, status; NULL: every
+ * particle a single) of the EPJ; must be given before the first run. */
 int pth_resident_changeover(pth_resident *res, const long long *epi_id, const long long *epj_id,
-                            const double *epj_rin, const double *epj_rout);
+                            const double *epj_rin, const double *epj_rout, const double *epj_group);
 /* Copies results (waits first): acc [ni][3], pot [ni], nnb [ni]; pot_tot
  * [ni] in changeover mode (else left alone).  Any pointer may be NULL. */
 int pth_resident_results(pth_resident *res, double *acc, double *pot, long long *nnb, double *pot_tot);
@@ -258,14 +269,19 @@ void pth_resident_free(pth_resident *res);
  * evaluation of one cluster.
  *
  * PTH_SERVE_FUSED: clusters are admitted into one HBM pool of hbm_budget
- *   bytes when their footprint fits (idle clusters are evicted, least recently
- *   used first, to make room; results are saved to the host on eviction), and
- *   stay resident; a worker thread starts each admitted request as one
- *   hipGraph launch (stage + force + reduce, captured at admission) on one of
- *   n_streams streams, so requests that arrive while others run overlap on the
- *   device.  A cluster runs one request at a time; requests start in arrival
- *   order as streams free up.  Each cluster's results do not depend on which
- *   other requests run beside it (bit-identical).
+ *   bytes when their footprint (inputs and outputs) fits, copied from pinned
+ *   host images, and stay resident; idle clusters are evicted, least recently
+ *   used first, to make room (their results are copied to pinned host memory
+ *   asynchronously).  What a request needs only while it runs (staged lists,
+ *   partial sums) is a per-stream scratch range from the same pool, sized
+ *   for the largest cluster the stream has run.  A worker thread starts each
+ *   request as one hipGraph launch (stage + force + reduce) on one of
+ *   n_streams streams, so requests that arrive while others run overlap on
+ *   the device; a cluster keeps one graph per stream and updates it in place
+ *   when its range or the scratch moved.  A cluster runs one request at a
+ *   time; requests start in arrival order as streams free up.  Each
+ *   cluster's results do not depend on which other requests run beside it
+ *   (bit-identical).
  * PTH_SERVE_PLAIN: the plain per-walk form as PeTar would drive it per
  *   request: all EPJ/SPJ sent, then batches of at most 200 walks dispatched
  *   and retrieved with synchronous copies (PTH_EXEC_PLAIN), one request at a
@@ -277,25 +293,34 @@ typedef struct pth_server pth_server;
 #define PTH_SERVE_PLAIN 1
 
 int pth_server_create(pth_ctx *ctx, pth_server **srv, int mode, size_t hbm_budget, int n_streams);
-/* Registers a cluster (arguments as pth_resident_create); nothing is sent to
- * the device until its first request is admitted.  Fails with
- * PTH_ERR_CAPACITY if its footprint exceeds the whole budget. */
+/* Registers a cluster (arguments as pth_resident_create); its image goes to
+ * pinned host memory, nothing is sent to the device until its first request
+ * is admitted.  Fails with PTH_ERR_CAPACITY if its footprint and its scratch
+ * exceed the whole budget. */
 int pth_server_add_cluster(pth_server *srv, int *cluster, int n_walk, const int *n_epi, const int *n_epj,
                            const int *n_spj, const double *epi_pos, const double *epi_rs, const int *id_epj,
                            const int *id_spj, int n_epj_tot, const double *epj_pos, const double *epj_mass,
                            const double *epj_rs, int n_spj_tot, const double *spj_pos, const double *spj_mass,
                            const double *spj_quad);
-/* Device bytes a cluster takes while resident. */
+/* Device bytes a cluster takes while resident (inputs and outputs; its
+ * requests also need a stream's scratch, see pth_server_memory). */
 long long pth_server_footprint(const pth_server *srv, int cluster);
+/* Fused server memory: the sum of the registered clusters' footprints, the
+ * largest per-stream scratch a cluster needs, pinned host bytes (images and
+ * saved results), and the request graphs captured and instantiated so far.
+ * Any pointer may be NULL. */
+int pth_server_memory(const pth_server *srv, long long *resident_total, long long *scratch_max,
+                      long long *pinned_host, long long *n_capture, long long *n_instantiate);
 /* Queues one evaluation of a cluster (returns at once). */
 int pth_server_submit(pth_server *srv, int cluster, long long *ticket);
 /* Waits for a request; latency_ms (may be NULL) from submission to done. */
 int pth_server_wait(pth_server *srv, long long ticket, double *latency_ms);
 /* Load generator: submits request k for cluster[k] at arrival_s[k] seconds
- * after the call (non-decreasing), waits for all, and returns each latency
+ * after the call (non-decreasing), waits for all, and returns each latency,
+ * whether the request had to upload its cluster (admitted[k], may be NULL)
  * and the wall time from the first arrival to the last completion. */
 int pth_server_replay(pth_server *srv, int n_req, const int *cluster, const double *arrival_s,
-                      double *latency_ms, double *wall_s);
+                      double *latency_ms, double *wall_s, int *admitted);
 /* Results of the cluster's last completed request (waits for one in
  * flight): acc [ni][3], pot [ni], nnb [ni]; any pointer may be NULL. */
 int pth_server_results(pth_server *srv, int cluster, double *acc, double *pot, long long *nnb);

@@ -190,9 +190,10 @@ inline pth_ctx *context(PS::S32 rank, PS::F64 eps2, PS::F64 rcut2, PS::F64 G)
 
 inline pth_epj_view epj_view()
 {
-    return pth_epj_view{sizeof(EPJSoft), offsetof(EPJSoft, pos), offsetof(EPJSoft, mass),
+    // group_data.artificial: mass_backup then status (src/artificial_particles.hpp:20-23)
+    return pth_epj_view{sizeof(EPJSoft),          offsetof(EPJSoft, pos), offsetof(EPJSoft, mass),
                         offsetof(EPJSoft, r_search), offsetof(EPJSoft, id), offsetof(EPJSoft, r_in),
-                        offsetof(EPJSoft, r_out)};
+                        offsetof(EPJSoft, r_out),    (long long)offsetof(EPJSoft, group_data)};
 }
 
 inline pth_spj_view spj_view()

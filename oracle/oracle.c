@@ -487,16 +487,32 @@ double orc_changeover_wacc_two(double r, double rin1, double rout1, double rin2,
     return (rout1 > rout2) ? orc_changeover_wacc(r, rin1, rout1) : orc_changeover_wacc(r, rin2, rout2);
 }
 
+/* Particle types of PeTar's group data (ArtificialParticleInformation,
+   src/artificial_particles.hpp:20-102: g[0] = mass_backup, g[1] = status):
+   single (status 0, mass_backup 0), member (status < 0), otherwise an
+   artificial or c.m. particle (status > 0).  A member's c.m. address
+   (src/ptcl.hpp:215-222) is unset (getParticleCMAddress() < 0) when its
+   status is -PS::LARGE_FLOAT; FDPS's constant is not available here, and
+   every set address is an array index, so "unset" is taken as status below
+   -1e30. */
+static int grp_single(const double *g) { return !g || (g[1] == 0.0 && g[0] == 0.0); }
+static int grp_member(const double *g) { return g && g[1] < 0.0; }
+static int grp_member_no_cm(const double *g) { return g && g[1] < -1e30; }
+
 void orc_changeover_correct_one(const double *pos_i, double mass_i, double rin_i, double rout_i,
                                 long long id_i, int n_ngb, const double *pos_j,
                                 const double *mass_j, const double *rin_j, const double *rout_j,
                                 const long long *id_j, double eps2, double r_out_glb, double G,
-                                double *acc, double *pot_soft, double *pot_tot)
+                                double *acc, double *pot_soft, double *pot_tot, const double *grp_i,
+                                const double *grp_j)
 {
-    /* self-potential term for singles, src/hard.hpp:1669-1673 */
-    double pot_cor = G * mass_i / r_out_glb;
-    *pot_tot += pot_cor;
-    *pot_soft += pot_cor;
+    /* self-potential term, src/hard.hpp:1669-1673: singles and members
+       without a c.m. address (binaries without artificial particles) */
+    if (grp_single(grp_i) || grp_member_no_cm(grp_i)) {
+        double pot_cor = G * mass_i / r_out_glb;
+        *pot_tot += pot_cor;
+        *pot_soft += pot_cor;
+    }
     for (int k = 0; k < n_ngb; k++) {
         if (id_j[k] == id_i) continue; /* src/hard.hpp:1677 */
         /* FP64 branch of src/hard.hpp:1404-1476 (single j) */
@@ -517,8 +533,22 @@ void orc_changeover_correct_one(const double *pos_i, double mass_i, double rin_i
         double gmor3_max = gmor_max * drinv2_max;
         for (int d = 0; d < 3; d++) acc[d] -= (gmor3 * kacc - gmor3_max) * dr[d];
         double kpot = 1.0 - orc_changeover_wpot_two(dr_eps, rin_i, rout_i, rin_j[k], rout_j[k]);
-        *pot_soft -= gmor * kpot - gmor_max;
-        *pot_tot -= gmor - gmor_max;
+        const double *gj = grp_j ? grp_j + 2 * k : NULL;
+        if (grp_single(gj)) {
+            *pot_soft -= gmor * kpot - gmor_max;
+            *pot_tot -= gmor - gmor_max;
+        } else if (grp_member(gj)) {
+            /* src/hard.hpp:1459-1464: the member's mass is zero in the tree;
+               its potential uses the backup mass */
+            gmor = G * gj[0] * drinv;
+            *pot_soft -= gmor * kpot - gmor_max;
+            *pot_tot -= gmor - gmor_max;
+        } else {
+            /* src/hard.hpp:1466-1470: artificial particles leave no
+               potential (the linear-cutoff term is cancelled) */
+            *pot_soft += gmor_max;
+            *pot_tot += gmor_max;
+        }
     }
 }
 
@@ -541,7 +571,7 @@ int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, cons
                             const double *spj_pos, const double *spj_mass, const double *spj_quad,
                             double eps2, double rout2, double G, double ngb_factor, double amb_rel,
                             double *acc, double *pot_soft, double *pot_tot, long long *nnb,
-                            long long *n_amb)
+                            long long *n_amb, const double *epj_group)
 {
     int rc = orc_soft_force_walks(n_walk, n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos,
                                   epj_mass, epj_rs, spj_pos, spj_mass, spj_quad, eps2, rout2, G, 0, acc,
@@ -561,6 +591,7 @@ int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, cons
         double *ri = malloc(sizeof(double) * (ne > 0 ? ne : 1));
         double *ro = malloc(sizeof(double) * (ne > 0 ? ne : 1));
         long long *idj = malloc(sizeof(long long) * (ne > 0 ? ne : 1));
+        double *gj = malloc(sizeof(double) * 2 * (ne > 0 ? ne : 1));
         const int *lst = id_epj + eo[w];
         for (long long i = io[w]; i < io[w + 1]; i++) {
             const double *xi = epi_pos + 3 * i;
@@ -593,16 +624,20 @@ int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, cons
                 ri[nn] = epj_rin[j];
                 ro[nn] = epj_rout[j];
                 idj[nn] = epj_id[j];
+                gj[2 * nn] = epj_group ? epj_group[2 * j] : 0.0;
+                gj[2 * nn + 1] = epj_group ? epj_group[2 * j + 1] : 0.0;
                 nn++;
             }
             orc_changeover_correct_one(xi, epj_mass[self], epj_rin[self], epj_rout[self], epi_id[i], nn, pj, mj,
-                                       ri, ro, idj, eps2, r_out, G, acc + 3 * i, pot_soft + i, pot_tot + i);
+                                       ri, ro, idj, eps2, r_out, G, acc + 3 * i, pot_soft + i, pot_tot + i,
+                                       epj_group ? epj_group + 2 * (long long)self : NULL, gj);
         }
         free(pj);
         free(mj);
         free(ri);
         free(ro);
         free(idj);
+        free(gj);
     }
     free(io);
     free(eo);

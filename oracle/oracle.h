@@ -90,13 +90,16 @@ double orc_changeover_wacc_two(double r, double rin1, double rout1, double rin2,
 
 /* Changeover correction of one i against its neighbour set (self excluded
  * by id), FP64 branch of src/hard.hpp:1404-1476 plus the self-potential term
- * of src/hard.hpp:1669-1673; all j are singles.  Adds to acc[3] and to
- * pot_soft / pot_tot. */
+ * of src/hard.hpp:1669-1673.  grp_i [2] and grp_j [n_ngb][2] are the
+ * particles' group data (mass_backup, status; NULL: singles), which select
+ * the single / member / artificial rules of src/hard.hpp:1453-1472 and the
+ * self-term condition.  Adds to acc[3] and to pot_soft / pot_tot. */
 void orc_changeover_correct_one(const double *pos_i, double mass_i, double rin_i, double rout_i,
                                 long long id_i, int n_ngb, const double *pos_j,
                                 const double *mass_j, const double *rin_j, const double *rout_j,
                                 const long long *id_j, double eps2, double r_out_glb, double G,
-                                double *acc, double *pot_soft, double *pot_tot);
+                                double *acc, double *pot_soft, double *pot_tot, const double *grp_i,
+                                const double *grp_j);
 
 /* Changeover mode (the composition src/hard.hpp:1404-1476,1655-1691 applies
  * on top of the linear-cutoff force) over walks; see oracle.c. */
@@ -108,7 +111,7 @@ int orc_soft_force_walks_co(int n_walk, const int *n_epi, const int *n_epj, cons
                             const double *spj_pos, const double *spj_mass, const double *spj_quad,
                             double eps2, double rout2, double G, double ngb_factor, double amb_rel,
                             double *acc, double *pot_soft, double *pot_tot, long long *nnb,
-                            long long *n_amb);
+                            long long *n_amb, const double *epj_group);
 
 /* Neighbour lists (FP64, r^2 < max(rs_i, rs_j)^2, self included) over each
  * walk's EPJ list: counts, and with ngb != NULL the entries at offset[i]. */

@@ -46,7 +46,7 @@ def orc():
         _llp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
         L.orc_soft_force_walks_co.argtypes = wl + [_dp, _dp, _llp, _ip, _ip, _dp, _dp, _dp, _llp, _dp, _dp,
                                                    _dp, _dp, C.c_void_p] + [C.c_double] * 5 + [_dp, _dp, _dp, _lp,
-                                                                                               _lp]
+                                                                                               _lp, C.c_void_p]
         L.orc_ngb_walks.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _ip, _dp, _dp, _llp, C.c_void_p, C.c_void_p]
         L.orc_ngb_walks.restype = C.c_longlong
         L.orc_changeover_setr.argtypes = [C.c_double] * 3 + [C.POINTER(C.c_double)] * 2
@@ -60,7 +60,7 @@ def orc():
             getattr(L, f).restype = C.c_double
         L.orc_changeover_correct_one.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_longlong,
                                                  C.c_int, _dp, _dp, _dp, _dp, _lp, C.c_double, C.c_double,
-                                                 C.c_double, _dp, _dp, _dp]
+                                                 C.c_double, _dp, _dp, _dp, C.c_void_p, C.c_void_p]
         _orc = L
     return _orc
 
@@ -183,7 +183,7 @@ def ref_soft_force_walks(n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, e
 
 def soft_force_walks_co(n_epi, n_epj, n_spj, epi_pos, epi_rs, epi_id, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
                         epj_id, epj_rin, epj_rout, spj_pos, spj_mass, spj_quad, eps2, rout2, G=1.0, ngb_factor=0.99,
-                        amb_rel=1e-5, mono=False):
+                        amb_rel=1e-5, mono=False, epj_group=None):
     """FP64 oracle of the changeover mode: acc, pot_soft, pot_tot, nnb and the
     per-i count of neighbour pairs at the search radius within amb_rel."""
     n_epi, n_epj, n_spj = _i32(n_epi), _i32(n_epj), _i32(n_spj)
@@ -198,11 +198,13 @@ def soft_force_walks_co(n_epi, n_epj, n_spj, epi_pos, epi_rs, epi_id, id_epj, id
         a = np.ascontiguousarray(a, dtype=dt).ravel()
         return a if a.size else np.zeros(1, dt)
     q = None if mono else arr(spj_quad)  # kept alive through the call
+    grp = None if epj_group is None else arr(epj_group)
     rc = orc().orc_soft_force_walks_co(
         n_epi.size, n_epi, n_epj, n_spj, arr(epi_pos), arr(epi_rs), arr(epi_id, np.int64),
         arr(id_epj, np.int32), arr(id_spj, np.int32), arr(epj_pos), arr(epj_mass), arr(epj_rs),
         arr(epj_id, np.int64), arr(epj_rin), arr(epj_rout), arr(spj_pos), arr(spj_mass),
-        None if q is None else q.ctypes.data, eps2, rout2, G, ngb_factor, amb_rel, acc.ravel(), ps, pt, nnb, amb)
+        None if q is None else q.ctypes.data, eps2, rout2, G, ngb_factor, amb_rel, acc.ravel(), ps, pt, nnb, amb,
+        None if grp is None else grp.ctypes.data)
     if rc != 0:
         raise RuntimeError("orc_soft_force_walks_co: an i-particle has no own EPJ entry in its walk")
     return acc, ps, pt, nnb, amb

@@ -269,11 +269,10 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n_epj_tot; i++) {
             const char *r = eb + (size_t)i * ev->stride;
-            EpjCoDev d;
-            d.rin = (float)*reinterpret_cast<const double *>(r + ev->off_rin);
-            d.rout = (float)*reinterpret_cast<const double *>(r + ev->off_rout);
-            d.id = *reinterpret_cast<const long long *>(r + ev->off_id);
-            hc[i] = d;
+            hc[i] = co_record(*reinterpret_cast<const double *>(r + ev->off_rin),
+                              *reinterpret_cast<const double *>(r + ev->off_rout),
+                              *reinterpret_cast<const long long *>(r + ev->off_id),
+                              ev->off_group >= 0 ? reinterpret_cast<const double *>(r + ev->off_group) : nullptr);
         }
         if (n_epj_tot > 0)
             HIPCHK(c, hipMemcpyAsync(c->d_epj_co.p, hc, sizeof(EpjCoDev) * n_epj_tot, hipMemcpyHostToDevice,
@@ -648,7 +647,7 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
 }
 
 int pth_resident_changeover(pth_resident *r, const long long *epi_id, const long long *epj_id,
-                            const double *epj_rin, const double *epj_rout)
+                            const double *epj_rin, const double *epj_rout, const double *epj_group)
 {
     if (!r) return PTH_ERR_ARG;
     pth_ctx *c = r->ctx;
@@ -659,11 +658,8 @@ int pth_resident_changeover(pth_resident *r, const long long *epi_id, const long
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     std::vector<EpjCoDev> co(std::max(r->n_epj_tot, 1));
-    for (int i = 0; i < r->n_epj_tot; i++) {
-        co[i].rin = (float)epj_rin[i];
-        co[i].rout = (float)epj_rout[i];
-        co[i].id = epj_id[i];
-    }
+    for (int i = 0; i < r->n_epj_tot; i++)
+        co[i] = co_record(epj_rin[i], epj_rout[i], epj_id[i], epj_group ? epj_group + 2 * (size_t)i : nullptr);
     if (r->n_epj_tot > 0)
         HIPCHK(c, hipMemcpy(r->epj_co.p, co.data(), sizeof(EpjCoDev) * r->n_epj_tot, hipMemcpyHostToDevice),
                "resident upload");

@@ -27,12 +27,36 @@ struct __align__(64) SpjDev {
     float pad[3];
 };
 
-// Changeover data of an EPJ (16 B, changeover mode only): the particle's
-// changeover radii (EPJSoft::r_in, r_out) and id.
-struct __align__(16) EpjCoDev {
+// Changeover data of an EPJ (24 B, changeover mode only): the particle's
+// changeover radii (EPJSoft::r_in, r_out), its group type and backup mass
+// (EPJSoft::group_data.artificial, src/artificial_particles.hpp:20-102) and
+// id.
+struct __align__(8) EpjCoDev {
     float rin, rout;
+    float mback;  // member: the mass its c.m. carries (src/hard.hpp:1459-1464)
+    int type;     // PTH_CO_SINGLE, _MEMBER, _MEMBER_NO_CM or _ARTIFICIAL
     long long id;
 };
+#define PTH_CO_SINGLE 0
+#define PTH_CO_MEMBER 1        // group member with a c.m. address: zero tree mass
+#define PTH_CO_MEMBER_NO_CM 2  // member of a group without artificial particles
+#define PTH_CO_ARTIFICIAL 3    // artificial or c.m. particle (status > 0)
+
+// Potential mass of a neighbour j in the changeover correction
+// (src/hard.hpp:1453-1472): a single's mass, a member's backup mass, and
+// zero for an artificial particle (its linear-cutoff term is cancelled and
+// nothing added).
+__host__ __device__ inline float co_pot_mass(const EpjCoDev &c, float m)
+{
+    return c.type == PTH_CO_SINGLE ? m : (c.type == PTH_CO_ARTIFICIAL ? 0.f : c.mback);
+}
+// Mass of the self term -G m / r_out an i loses in changeover mode: singles
+// and members without a c.m. address (src/hard.hpp:1665-1673); the others
+// keep it.
+__host__ __device__ inline float co_self_mass(const EpjCoDev &c, float m)
+{
+    return (c.type == PTH_CO_SINGLE || c.type == PTH_CO_MEMBER_NO_CM) ? m : 0.f;
+}
 
 // Batch i-particle (32 B): FP64 position, r_search.
 struct __align__(32) EpiDev {
@@ -45,8 +69,9 @@ struct __align__(32) EpiDev {
 // id table lives in LDS).
 #define PTH_CO_MAX_NI 1024
 #define PTH_CO_HASH 2048
-// floats per staged EPJ block of 8 in the changeover extras
-#define PTH_CO_EXTRA 40
+// floats per staged EPJ block of 8 in the changeover extras:
+// r_in[8], r_out[8], x, y, z low parts[8], potential mass[8]
+#define PTH_CO_EXTRA 48
 // floats per staged near EPJ block of 8 in linear-cutoff mode with
 // double-float near pairs: the low-order parts x[8], y[8], z[8]
 #define PTH_DF_EXTRA 24
@@ -108,7 +133,7 @@ struct BatchDev {
     float4 *staged_ic = nullptr;  // per i: {self slot (int bits), r_in, r_out, m}
     float4 *staged_il = nullptr;  // per i: low parts of x-c, y-c, z-c (double-float), 0 (co or df)
     float *staged_el = nullptr;   // df: per staged near EPJ block of 8: {x, y, z low parts[8]}
-    float *staged_ec = nullptr;   // per staged EPJ block of 8: {r_in[8], r_out[8], x, y, z low parts[8]}
+    float *staged_ec = nullptr;   // per staged EPJ block of 8: {r_in[8], r_out[8], x, y, z low parts[8], m_pot[8]}
     int *walk_err = nullptr;      // per walk: number of i without their own EPJ
     double *out_pott = nullptr;
     double *part_pott = nullptr;

@@ -848,24 +848,6 @@ hipError_t cub_temp(pth_gtree *t, size_t bytes) { return t->tmp.ensure(std::max<
 
 namespace {
 
-// true when [p, p + bytes) lies inside one device allocation of this process
-bool device_range(const void *p, size_t bytes)
-{
-    hipPointerAttribute_t at;
-    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    if (at.type != hipMemoryTypeDevice) return false;
-    hipDeviceptr_t base = nullptr;
-    size_t size = 0;
-    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    return (const char *)p + bytes <= (const char *)base + size;
-}
-
 // The largest cells with at most `limit` particles (or leaves), as node
 // indices in Morton order (by first particle): the i-groups for limit =
 // n_group_limit, the neighbour-search tiles for limit = 64.

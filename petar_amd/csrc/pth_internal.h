@@ -153,6 +153,25 @@ struct Work {
     }
 };
 
+// Whether [p, p + bytes) lies inside one device allocation of this process
+// (host pointers and ranges overrunning their allocation are rejected).
+inline bool device_range(const void *p, size_t bytes)
+{
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (at.type != hipMemoryTypeDevice) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return (const char *)p + bytes <= (const char *)base + size;
+}
+
 // Linear-cutoff mode with double-float near pairs (fused form; the
 // changeover mode has its own double-float neighbour pairs).
 inline int df_enabled(const pth_params &par)
@@ -241,6 +260,30 @@ inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const 
         b.part_pott = W.part_pott.as<double>();
         b.out_pott = out_pott;
     }
+}
+
+// Changeover data of one EPJ from its radii, id and group data (mass_backup,
+// status; null: a single); types of src/artificial_particles.hpp:60-102, a
+// member's c.m. address unset (src/ptcl.hpp:215-222) when its status is
+// -PS::LARGE_FLOAT, taken as any status below -1e30 (an address is an
+// array index).
+inline EpjCoDev co_record(double rin, double rout, long long id, const double *group)
+{
+    EpjCoDev d;
+    d.rin = (float)rin;
+    d.rout = (float)rout;
+    d.id = id;
+    d.mback = 0.f;
+    d.type = PTH_CO_SINGLE;
+    if (group) {
+        const double mb = group[0], st = group[1];
+        if (st == 0.0 && mb == 0.0) d.type = PTH_CO_SINGLE;
+        else if (st < 0.0) {
+            d.type = st < -1e30 ? PTH_CO_MEMBER_NO_CM : PTH_CO_MEMBER;
+            d.mback = (float)mb;
+        } else d.type = PTH_CO_ARTIFICIAL;
+    }
+    return d;
 }
 
 // Checks of the changeover-mode preconditions on a plan.

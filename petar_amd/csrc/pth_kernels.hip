@@ -264,6 +264,7 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
                 blk[16 + l] = q.xl;
                 blk[24 + l] = q.yl;
                 blk[32 + l] = q.zl;
+                blk[40 + l] = co_pot_mass(co, q.m);
                 // is this the EPJ of one of the walk's i?
                 unsigned hb = co_hash(co.id);
                 for (;;) {
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
                         co_self[k] = (int)slot;
                         co_rin[k] = co.rin;
                         co_rout[k] = co.rout;
-                        co_m[k] = q.m;
+                        co_m[k] = co_self_mass(co, q.m);
                         break;
                     }
                     hb = (hb + 1) & (PTH_CO_HASH - 1);
@@ -469,11 +470,13 @@ __device__ __forceinline__ void epj_blocks(cfp se, int b_begin, int b_end, const
 
 // Changeover weight of one neighbour pair (src/changeover.hpp:294-334 with
 // the pair rule of :366-375, i being the first changeover): the Newtonian
-// terms weighted by the changeover, f = m (1 - W_acc) / r^3 (acc factor),
-// pg = m (1 - W_pot) / r (pot_soft) and pt = -m W_pot / r (pot_tot -
-// pot_soft), i.e. what src/hard.hpp:1404-1476 leaves for a single j once the
-// linear-cutoff terms are removed.
-__device__ __forceinline__ void co_pair(float r2, float mj, float rin_i, float rout_i, float rin_j,
+// terms weighted by the changeover, f = m (1 - W_acc) / r^3 (acc factor,
+// with the j's tree mass m), pg = mp (1 - W_pot) / r (pot_soft) and pt = -mp
+// W_pot / r (pot_tot - pot_soft) with its potential mass mp, i.e. what
+// src/hard.hpp:1404-1476 leaves for j once the linear-cutoff terms are
+// removed (mp = m for a single, the backup mass for a member, 0 for an
+// artificial particle).
+__device__ __forceinline__ void co_pair(float r2, float mj, float mp, float rin_i, float rout_i, float rin_j,
                                         float rout_j, float &f, float &pg, float &pt)
 {
     const float rinv = __builtin_amdgcn_rsqf(r2);
@@ -492,10 +495,10 @@ __device__ __forceinline__ void co_pair(float r2, float mj, float rin_i, float r
     float wpot = __builtin_fmaf(-coff * x5,
                                 __builtin_fmaf(x, __builtin_fmaf(x, __builtin_fmaf(x, 5.f, -20.f), 28.f), -14.f), 1.f);
     if (xr >= 1.f) wpot = (1.f + coff) * r * __builtin_amdgcn_rcpf(rout);
-    const float mri = mj * rinv;
-    f = mri * rinv * rinv * (1.f - wacc);
-    pg = mri * (1.f - wpot);
-    pt = -mri * wpot;
+    f = mj * rinv * rinv * rinv * (1.f - wacc);
+    const float mpri = mp * rinv;
+    pg = mpri * (1.f - wpot);
+    pt = -mpri * wpot;
 }
 
 // Near EPJ blocks in changeover mode.  Every pair first gets the
@@ -561,6 +564,7 @@ __device__ __forceinline__ void epj_near_co(cfp se, cfp sec, int b_begin, int b_
                     const float xj = B[k], yj = B[8 + k], zj = B[16 + k], mj = B[24 + k], rj = B[32 + k];
                     const float rinj = E[k], routj = E[8 + k];
                     const float xjl = E[16 + k], yjl = E[24 + k], zjl = E[32 + k];
+                    const float mpj = E[40 + k];
                     const int sj = b * 8 + k;
 #pragma unroll
                     for (int q = 0; q < IPL; q++) {
@@ -585,7 +589,7 @@ __device__ __forceinline__ void epj_near_co(cfp se, cfp sec, int b_begin, int b_
                         const float r2p =
                             __builtin_fmaf(dzp, dzp, __builtin_fmaf(dyp, dyp, __builtin_fmaf(dxp, dxp, K.eps2)));
                         float f, pg, ptj;
-                        co_pair(r2p, mj, rin_i[q], rout_i[q], rinj, routj, f, pg, ptj);
+                        co_pair(r2p, mj, mpj, rin_i[q], rout_i[q], rinj, routj, f, pg, ptj);
                         ax[q] += nbr ? __builtin_fmaf(f, dxp, -mri3 * dx) : 0.f;
                         ay[q] += nbr ? __builtin_fmaf(f, dyp, -mri3 * dy) : 0.f;
                         az[q] += nbr ? __builtin_fmaf(f, dzp, -mri3 * dz) : 0.f;

@@ -1,16 +1,24 @@
 // pth_server.cpp -- the serving loop of include/petar_hip.h (pth_server_*):
 // many independent soft-force evaluations ("clusters") behind one queue.
 //
-// Fused mode: one device block of hbm_budget bytes is the pool; a cluster is
-// admitted by giving it one contiguous range of the pool laid out as
-// [upload image: batch arena, EPJ, SPJ | staging and partial buffers |
-// outputs].  At admission the image is copied in and the evaluation
-// (k_stage_walk, the k_force classes, k_reduce) is captured once into a
-// hipGraph against those addresses; each request is then one graph launch on
-// one of n_streams streams, so requests overlap on the device.  Idle clusters
-// are evicted least recently used first when an admission needs room (their
-// results are copied to the host first).  A worker thread does admission,
-// launches and completion (hipEventQuery polling) under one mutex.
+// Fused mode: one device block of hbm_budget bytes is the pool.  A cluster is
+// admitted by giving it one contiguous range of the pool, [upload image:
+// batch arena, EPJ, SPJ | outputs], into which its pinned host image is
+// copied; it stays resident until evicted.  What a request only needs while
+// it runs (the staged lists, the partial sums of split tiles) lives in a
+// scratch range per stream, also from the pool, sized for the largest
+// cluster the stream has run, so a resident cluster holds only its inputs
+// and outputs.  Each request runs as one hipGraph launch on one of
+// n_streams streams, so requests overlap on the device; a cluster keeps one
+// instantiated graph per stream, captured the first time it runs there and
+// updated in place (hipGraphExecUpdate) when its range or the stream's
+// scratch has moved, so re-admission captures a few launches but
+// instantiates nothing.  Idle clusters are evicted least recently used first
+// when an admission or a scratch range needs room: their results are copied
+// to pinned host memory on a transfer stream, and every later upload or
+// launch waits for those copies on the device, so the worker never blocks on
+// them.  A worker thread does admission, launches and completion
+// (hipEventQuery polling) under one mutex.
 //
 // Plain mode: each request replays PeTar's multi-walk call order on a
 // PTH_EXEC_PLAIN context (send all EPJ/SPJ, then <= 200-walk batches
@@ -77,12 +85,20 @@ struct PoolAlloc {
     }
 };
 
-// Byte offsets of one cluster's arrays inside its pool range.
+constexpr int MAX_STREAMS = 16;
+
+// Byte offsets of a cluster's resident range: the upload image (batch
+// arena, EPJ, SPJ) and the outputs.
 struct Layout {
     Arena A{};
-    size_t epj = 0, spj = 0, upload = 0;
+    size_t epj = 0, spj = 0, upload = 0, out_accp = 0, out_nnb = 0, total = 0;
+};
+
+// Byte offsets of what one request of a cluster needs while it runs, inside
+// a stream's scratch range.
+struct Scratch {
     size_t centre = 0, near_blk = 0, staged_i = 0, staged_e = 0, staged_s = 0, part_accp = 0, part_nnb = 0,
-           staged_il = 0, staged_el = 0, out_accp = 0, out_nnb = 0, total = 0;
+           staged_il = 0, staged_el = 0, total = 0;
 };
 
 // Host records of the plain mode (FDPS-like array-of-structs).
@@ -100,23 +116,30 @@ struct PForce {
     long long nnb;
 };
 
+// A cluster's instantiated graph on one stream and the placements it was
+// made for.
+struct SGraph {
+    hipGraphExec_t exec = nullptr;
+    long long cl_gen = -1, sc_gen = -1;
+};
+
 struct SCluster {
     Plan plan;
     Layout L;
+    Scratch S;
     int n_walk = 0, n_epj_tot = 0, n_spj_tot = 0;
     long long ni = 0, n_int = 0;
-    std::vector<char> image;  // fused: host copy of the upload image
-    long long off = -1;       // fused: pool offset while resident
-    BatchDev b;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // fused mode
+    PinBuf image;          // pinned host copy of the upload image
+    long long off = -1;    // pool offset while resident
+    long long gen = 0;     // admissions so far: a new one may place it elsewhere
+    SGraph g[MAX_STREAMS];
+    PinBuf h_accp, h_nnb;  // results copied out at eviction
+    hipEvent_t saved = nullptr;  // that copy (on the transfer stream)
     bool busy = false;
     long long last_use = 0;
-    // results kept on the host (fused: saved at eviction; plain: every request)
-    std::vector<double4> h_accp;
-    std::vector<int> h_nnb;
-    bool host_valid = false;
     bool ran = false;
+    bool saved_valid = false;  // h_accp / h_nnb hold the last results (after `saved`)
     // plain mode
     std::vector<int> n_epi, n_epj, n_spj;
     std::vector<PEpj> p_epj;
@@ -127,12 +150,26 @@ struct SCluster {
     std::vector<const void *> epi_ptr;
     std::vector<const int *> ide_ptr, ids_ptr;
     std::vector<void *> force_ptr;
+    bool host_valid = false;
 };
 
 struct Req {
     int cluster = -1;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+    int admitted = 0;  // the request uploaded its cluster
     double t_submit = 0.0, t_done = 0.0;
+};
+
+// One stream of the fused server: the request running on it and its scratch
+// range.
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;  // completion of the running request
+    long long req = -1;
+    long long sc_off = -1;
+    size_t sc_size = 0;
+    long long sc_gen = -1;       // placement of the scratch range
+    long long evict_seen = 0;    // evictions this stream has waited for
 };
 
 }  // namespace
@@ -148,10 +185,12 @@ struct pth_server {
     std::deque<Req> req;
     std::deque<long long> queue;
     int n_streams = 1;
-    std::vector<hipStream_t> st;
-    std::vector<hipEvent_t> ev;
-    std::vector<long long> slot_req;
-    hipStream_t cap = nullptr;  // capture and eviction copies
+    std::vector<Slot> slot;
+    hipStream_t cap = nullptr;   // graph capture
+    hipStream_t xfer = nullptr;  // result copies (evictions, pth_server_results)
+    hipEvent_t ev_evicted = nullptr;  // the latest eviction copy
+    long long n_evict_copies = 0;
+    long long gen_clock = 0;     // placements of cluster ranges and scratch ranges
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread th;
@@ -159,6 +198,7 @@ struct pth_server {
     bool failed = false;
     std::string err;
     long long n_done = 0, n_admit = 0, n_evict = 0, ints_done = 0, use_clock = 0;
+    long long n_capture = 0, n_instantiate = 0;
 };
 
 namespace {
@@ -168,33 +208,39 @@ void layout(SCluster &c, long long n_ide, long long n_ids, bool df)
     const Plan &P = c.plan;
     Layout &L = c.L;
     L.A.layout(P, n_ide, n_ids, false);
+    const size_t nw = std::max<size_t>(P.walks.size(), 1), ni = (size_t)std::max<long long>(P.ni_tot, 1);
+    const size_t np = (size_t)std::max(P.n_part, 1);
     size_t o = L.A.total;
     L.epj = o; o = align256(o + sizeof(EpjDev) * (size_t)std::max(c.n_epj_tot, 1));
     L.spj = o; o = align256(o + sizeof(SpjDev) * (size_t)std::max(c.n_spj_tot, 1));
     L.upload = o;
-    const size_t nw = std::max<size_t>(P.walks.size(), 1), ni = (size_t)std::max<long long>(P.ni_tot, 1);
-    const size_t np = (size_t)std::max(P.n_part, 1);
-    L.centre = o; o = align256(o + sizeof(double4) * nw);
-    L.near_blk = o; o = align256(o + sizeof(int) * nw);
-    L.staged_i = o; o = align256(o + sizeof(float4) * ni);
-    L.staged_e = o; o = align256(o + sizeof(float) * 5 * (size_t)std::max<long long>(P.n_e_slots, 8));
-    L.staged_s = o; o = align256(o + sizeof(float) * 12 * (size_t)std::max<long long>(P.n_s_rec, 1));
-    L.part_accp = o; o = align256(o + sizeof(double4) * 64 * np);
-    L.part_nnb = o; o = align256(o + sizeof(int) * 64 * np);
-    if (df) {
-        L.staged_il = o; o = align256(o + sizeof(float4) * ni);
-        L.staged_el = o; o = align256(o + sizeof(float) * (PTH_DF_EXTRA / 8) * (size_t)std::max<long long>(P.n_e_slots, 8));
-    }
     L.out_accp = o; o = align256(o + sizeof(double4) * ni);
     L.out_nnb = o; o = align256(o + sizeof(int) * ni);
     L.total = o;
+    Scratch &S = c.S;
+    o = 0;
+    S.centre = o; o = align256(o + sizeof(double4) * nw);
+    S.near_blk = o; o = align256(o + sizeof(int) * nw);
+    S.staged_i = o; o = align256(o + sizeof(float4) * ni);
+    S.staged_e = o; o = align256(o + sizeof(float) * 5 * (size_t)std::max<long long>(P.n_e_slots, 8));
+    S.staged_s = o; o = align256(o + sizeof(float) * 12 * (size_t)std::max<long long>(P.n_s_rec, 1));
+    S.part_accp = o; o = align256(o + sizeof(double4) * 64 * np);
+    S.part_nnb = o; o = align256(o + sizeof(int) * 64 * np);
+    if (df) {
+        S.staged_il = o; o = align256(o + sizeof(float4) * ni);
+        S.staged_el = o;
+        o = align256(o + sizeof(float) * (PTH_DF_EXTRA / 8) * (size_t)std::max<long long>(P.n_e_slots, 8));
+    }
+    S.total = o;
 }
 
-void bind(SCluster &c, char *base, const pth_params &par)
+// Launch descriptor of cluster c with its range at base and a stream's
+// scratch range at sc.
+void bind(SCluster &c, BatchDev &b, char *base, char *sc, const pth_params &par)
 {
     const Plan &P = c.plan;
     const Layout &L = c.L;
-    BatchDev &b = c.b;
+    const Scratch &S = c.S;
     b = BatchDev();
     b.n_walk = (int)P.walks.size();
     b.ni_tot = P.ni_tot;
@@ -218,81 +264,154 @@ void bind(SCluster &c, char *base, const pth_params &par)
     b.ritems = reinterpret_cast<const RItem *>(base + L.A.ritems);
     b.epj = reinterpret_cast<const EpjDev *>(base + L.epj);
     b.spj = reinterpret_cast<const SpjDev *>(base + L.spj);
-    b.centre = reinterpret_cast<double4 *>(base + L.centre);
-    b.near_blk = reinterpret_cast<int *>(base + L.near_blk);
-    b.staged_i = reinterpret_cast<float4 *>(base + L.staged_i);
-    b.staged_e = reinterpret_cast<float *>(base + L.staged_e);
-    b.staged_s = reinterpret_cast<float *>(base + L.staged_s);
-    b.part_accp = reinterpret_cast<double4 *>(base + L.part_accp);
-    b.part_nnb = reinterpret_cast<int *>(base + L.part_nnb);
     b.out_accp = reinterpret_cast<double4 *>(base + L.out_accp);
     b.out_nnb = reinterpret_cast<int *>(base + L.out_nnb);
+    b.centre = reinterpret_cast<double4 *>(sc + S.centre);
+    b.near_blk = reinterpret_cast<int *>(sc + S.near_blk);
+    b.staged_i = reinterpret_cast<float4 *>(sc + S.staged_i);
+    b.staged_e = reinterpret_cast<float *>(sc + S.staged_e);
+    b.staged_s = reinterpret_cast<float *>(sc + S.staged_s);
+    b.part_accp = reinterpret_cast<double4 *>(sc + S.part_accp);
+    b.part_nnb = reinterpret_cast<int *>(sc + S.part_nnb);
     b.df = df_enabled(par);
     if (b.df) {
-        b.staged_il = reinterpret_cast<float4 *>(base + L.staged_il);
-        b.staged_el = reinterpret_cast<float *>(base + L.staged_el);
+        b.staged_il = reinterpret_cast<float4 *>(sc + S.staged_il);
+        b.staged_el = reinterpret_cast<float *>(sc + S.staged_el);
     }
 }
 
-hipError_t save_results(pth_server *s, SCluster &c)
-{
-    c.h_accp.resize((size_t)std::max<long long>(c.ni, 1));
-    c.h_nnb.resize((size_t)std::max<long long>(c.ni, 1));
-    if (c.ni > 0) {
-        hipError_t e;
-        if ((e = hipMemcpyAsync(c.h_accp.data(), c.b.out_accp, sizeof(double4) * c.ni, hipMemcpyDeviceToHost,
-                                s->cap)))
-            return e;
-        if ((e = hipMemcpyAsync(c.h_nnb.data(), c.b.out_nnb, sizeof(int) * c.ni, hipMemcpyDeviceToHost, s->cap)))
-            return e;
-        if ((e = hipStreamSynchronize(s->cap))) return e;
-    }
-    c.host_valid = true;
-    return hipSuccess;
-}
-
+// Evicts an idle resident cluster: its results go to pinned host memory on
+// the transfer stream (asynchronously; later uploads and launches wait for
+// the copy on the device) and its range returns to the pool.
 hipError_t evict(pth_server *s, SCluster &c)
 {
-    hipError_t e = hipSuccess;
-    if (c.ran) e = save_results(s, c);
-    if (c.exec) (void)hipGraphExecDestroy(c.exec);
-    if (c.graph) (void)hipGraphDestroy(c.graph);
-    c.exec = nullptr;
-    c.graph = nullptr;
+    hipError_t e;
+    if (c.ran && c.ni > 0) {
+        char *base = s->pool.as<char>() + c.off;
+        if ((e = hipMemcpyAsync(c.h_accp.p, base + c.L.out_accp, sizeof(double4) * c.ni, hipMemcpyDeviceToHost,
+                                s->xfer)))
+            return e;
+        if ((e = hipMemcpyAsync(c.h_nnb.p, base + c.L.out_nnb, sizeof(int) * c.ni, hipMemcpyDeviceToHost,
+                                s->xfer)))
+            return e;
+        if ((e = hipEventRecord(c.saved, s->xfer))) return e;
+        if ((e = hipEventRecord(s->ev_evicted, s->xfer))) return e;
+        c.saved_valid = true;
+        s->n_evict_copies++;
+    }
     s->pa.release((size_t)c.off, c.L.total);
     c.off = -1;
     s->n_evict++;
-    return e;
+    return hipSuccess;
 }
 
-// Makes cluster k resident (returns false while memory is held by busy
-// clusters); the upload goes on stream `st`, ahead of the first launch.
-bool admit(pth_server *s, int k, hipStream_t st, hipError_t &e)
+// A pool range of n bytes; idle clusters (least recently used first, never
+// cluster `keep`) and then the scratch of idle streams (never stream
+// `keep_slot`) give way.  Returns -1 while the memory is held by running
+// requests (e stays hipSuccess) or on an error.
+long long pool_alloc(pth_server *s, size_t n, int keep, int keep_slot, hipError_t &e)
 {
-    SCluster &c = *s->cl[k];
     e = hipSuccess;
-    if (c.off >= 0) return true;
-    long long off;
-    while ((off = s->pa.alloc(c.L.total)) < 0) {
+    for (;;) {
+        const long long off = s->pa.alloc(n);
+        if (off >= 0) return off;
         int victim = -1;
         for (int j = 0; j < (int)s->cl.size(); j++) {
             const SCluster &v = *s->cl[j];
-            if (j != k && v.off >= 0 && !v.busy && (victim < 0 || v.last_use < s->cl[victim]->last_use)) victim = j;
+            if (j != keep && v.off >= 0 && !v.busy && (victim < 0 || v.last_use < s->cl[victim]->last_use))
+                victim = j;
         }
-        if (victim < 0) return false;
-        if ((e = evict(s, *s->cl[victim]))) return false;
+        if (victim >= 0) {
+            if ((e = evict(s, *s->cl[victim]))) return -1;
+            continue;
+        }
+        int idle = -1;
+        for (int k = 0; k < s->n_streams; k++)
+            if (k != keep_slot && s->slot[k].req < 0 && s->slot[k].sc_off >= 0) idle = k;
+        if (idle < 0) return -1;
+        Slot &q = s->slot[idle];
+        s->pa.release((size_t)q.sc_off, q.sc_size);
+        q.sc_off = -1;
+        q.sc_size = 0;
     }
-    c.off = off;
-    char *base = s->pool.as<char>() + off;
-    bind(c, base, s->ctx->par);
-    if ((e = hipMemcpyAsync(base, c.image.data(), c.L.upload, hipMemcpyHostToDevice, st))) return false;
-    if ((e = hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal))) return false;
-    const hipError_t el = launch_all(c.b, s->cap);
-    e = hipStreamEndCapture(s->cap, &c.graph);
-    if (el != hipSuccess) e = el;
-    if (e) return false;
-    if ((e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0))) return false;
-    s->n_admit++;
+}
+
+// Makes cluster k resident with stream `k_slot`'s scratch large enough for
+// it, and its graph on that stream current; false while memory is held by
+// running requests (e stays hipSuccess) or on an error.  `admitted` tells
+// whether the cluster's image was uploaded.
+bool admit(pth_server *s, int k, int k_slot, hipError_t &e, bool &admitted)
+{
+    SCluster &c = *s->cl[k];
+    Slot &q = s->slot[k_slot];
+    e = hipSuccess;
+    admitted = false;
+    if (q.sc_size < c.S.total) {
+        if (q.sc_off >= 0) s->pa.release((size_t)q.sc_off, q.sc_size);
+        q.sc_off = -1;
+        q.sc_size = 0;
+        const long long off = pool_alloc(s, c.S.total, k, k_slot, e);
+        if (off < 0) return false;
+        q.sc_off = off;
+        q.sc_size = c.S.total;
+        q.sc_gen = ++s->gen_clock;
+    }
+    if (c.off < 0) {
+        const long long off = pool_alloc(s, c.L.total, k, k_slot, e);
+        if (off < 0) return false;
+        c.off = off;
+        c.gen = ++s->gen_clock;
+        admitted = true;
+    }
+    // the ranges just handed out may still be read by an eviction copy
+    if (q.evict_seen != s->n_evict_copies) {
+        if ((e = hipStreamWaitEvent(q.st, s->ev_evicted, 0))) return false;
+        q.evict_seen = s->n_evict_copies;
+    }
+    char *base = s->pool.as<char>() + c.off;
+    if (admitted) {
+        if ((e = hipMemcpyAsync(base, c.image.p, c.L.upload, hipMemcpyHostToDevice, q.st))) return false;
+        s->n_admit++;
+    }
+    SGraph &g = c.g[k_slot];
+    if (!g.exec || g.cl_gen != c.gen || g.sc_gen != q.sc_gen) {
+        BatchDev b;
+        bind(c, b, base, s->pool.as<char>() + q.sc_off, s->ctx->par);
+        hipGraph_t graph = nullptr;
+        if ((e = hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal))) return false;
+        const hipError_t el = launch_all(b, s->cap);
+        e = hipStreamEndCapture(s->cap, &graph);
+        if (el != hipSuccess) e = el;
+        if (e) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return false;
+        }
+        s->n_capture++;
+        bool updated = false;
+        if (g.exec) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(g.exec, graph, &err_node, &res) == hipSuccess &&
+                      res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(g.exec);
+                g.exec = nullptr;
+            }
+        }
+        if (!updated) {
+            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            if (e) {
+                g.exec = nullptr;
+                (void)hipGraphDestroy(graph);
+                return false;
+            }
+            s->n_instantiate++;
+        }
+        (void)hipGraphDestroy(graph);
+        g.cl_gen = c.gen;
+        g.sc_gen = q.sc_gen;
+    }
     return true;
 }
 
@@ -314,6 +433,7 @@ void complete(pth_server *s, long long t)
     r.state = 2;
     c.busy = false;
     c.ran = true;
+    c.saved_valid = false;
     c.host_valid = false;
     c.last_use = ++s->use_clock;
     s->n_done++;
@@ -323,7 +443,7 @@ void complete(pth_server *s, long long t)
 // One plain-mode request: PeTar's call order on the PTH_EXEC_PLAIN context.
 int run_plain(pth_server *s, SCluster &c)
 {
-    static const pth_epj_view ev{sizeof(PEpj), 0, 24, 32, 0, 0, 0};
+    static const pth_epj_view ev{sizeof(PEpj), 0, 24, 32, 0, 0, 0, -1};
     static const pth_spj_view sv{sizeof(PSpj), 0, 24, 32, 40, 48, 56, 64, 72};
     static const pth_epi_view iv{sizeof(PEpi), 0, 24, 0};
     static const pth_force_view fv{sizeof(PForce), 0, 24, 32, 8, -1};
@@ -377,12 +497,14 @@ void worker_fused(pth_server *s)
     std::unique_lock<std::mutex> lk(s->mu);
     hipError_t e = hipSetDevice(s->ctx->device);
     if (e) fail_all(s, e, "hipSetDevice");
+    bool compacted = false;  // everything idle was released for a blocked admission
     while (!s->stop) {
         bool progress = false, running = false;
         // completions
         for (int k = 0; k < s->n_streams && !s->failed; k++) {
-            if (s->slot_req[k] < 0) continue;
-            e = hipEventQuery(s->ev[k]);
+            Slot &q = s->slot[k];
+            if (q.req < 0) continue;
+            e = hipEventQuery(q.ev);
             if (e == hipErrorNotReady) {
                 running = true;
                 continue;
@@ -391,37 +513,71 @@ void worker_fused(pth_server *s)
                 fail_all(s, e, "request execution");
                 break;
             }
-            complete(s, s->slot_req[k]);
-            s->slot_req[k] = -1;
+            complete(s, q.req);
+            q.req = -1;
             progress = true;
             s->cv_done.notify_all();
         }
         // starts: queued requests in arrival order, on free streams
         for (int k = 0; k < s->n_streams && !s->failed && !s->queue.empty(); k++) {
-            if (s->slot_req[k] >= 0) continue;
+            Slot &q = s->slot[k];
+            if (q.req >= 0) continue;
             auto it = s->queue.begin();
+            bool blocked = false, admitted = false;
             for (; it != s->queue.end(); ++it) {
-                SCluster &c = *s->cl[s->req[*it].cluster];
-                if (c.busy) continue;
-                if (!admit(s, s->req[*it].cluster, s->st[k], e)) {
+                const int ck = s->req[*it].cluster;
+                if (s->cl[ck]->busy) continue;
+                if (!admit(s, ck, k, e, admitted)) {
                     if (e) fail_all(s, e, "admission");
-                    it = s->queue.end();  // memory is held by running requests: wait for them
+                    blocked = true;
                 }
                 break;
             }
-            if (s->failed || it == s->queue.end()) break;
+            if (s->failed) break;
+            if (blocked) {
+                bool any = false;
+                for (const Slot &x : s->slot) any |= x.req >= 0;
+                if (!any) {
+                    // nothing runs, yet the request does not fit: free every
+                    // stream's scratch and evict every cluster once (the pool
+                    // is then empty and holds any cluster with its scratch,
+                    // which registration checked); failing after that is an
+                    // error
+                    if (compacted) {
+                        fail_all(s, hipErrorOutOfMemory, "admission: the pool cannot hold the cluster");
+                        break;
+                    }
+                    for (Slot &x : s->slot)
+                        if (x.sc_off >= 0) {
+                            s->pa.release((size_t)x.sc_off, x.sc_size);
+                            x.sc_off = -1;
+                            x.sc_size = 0;
+                        }
+                    for (auto &c : s->cl)
+                        if (c->off >= 0 && (e = evict(s, *c))) {
+                            fail_all(s, e, "eviction");
+                            break;
+                        }
+                    compacted = true;
+                    progress = true;
+                }
+                break;
+            }
+            if (it == s->queue.end()) break;
             const long long t = *it;
             SCluster &c = *s->cl[s->req[t].cluster];
-            if ((e = hipGraphLaunch(c.exec, s->st[k])) || (e = hipEventRecord(s->ev[k], s->st[k]))) {
+            if ((e = hipGraphLaunch(c.g[k].exec, q.st)) || (e = hipEventRecord(q.ev, q.st))) {
                 fail_all(s, e, "graph launch");
                 break;
             }
             s->queue.erase(it);
             s->req[t].state = 1;
+            s->req[t].admitted = admitted ? 1 : 0;
             c.busy = true;
-            s->slot_req[k] = t;
+            q.req = t;
             running = true;
             progress = true;
+            compacted = false;
         }
         if (progress) continue;
         if (running) {
@@ -434,6 +590,16 @@ void worker_fused(pth_server *s)
             s->cv_work.wait(lk);
         }
     }
+}
+
+void release_cluster(SCluster &c)
+{
+    for (SGraph &g : c.g)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (c.saved) (void)hipEventDestroy(c.saved);
+    c.image.release();
+    c.h_accp.release();
+    c.h_nnb.release();
 }
 
 }  // namespace
@@ -450,7 +616,8 @@ int pth_server_create(pth_ctx *ctx, pth_server **out, int mode, size_t hbm_budge
     if (ctx->par.changeover) return ctx->fail(PTH_ERR_ARG, "pth_server_create: the changeover mode is not served");
     if (ctx->par.spj_monopole)
         return ctx->fail(PTH_ERR_ARG, "pth_server_create: monopole superparticles are not served");
-    if (n_streams < 1 || n_streams > 16) return ctx->fail(PTH_ERR_ARG, "pth_server_create: n_streams outside [1, 16]");
+    if (n_streams < 1 || n_streams > MAX_STREAMS)
+        return ctx->fail(PTH_ERR_ARG, "pth_server_create: n_streams outside [1, 16]");
     HIPCHK(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     pth_server *s = new pth_server;
     s->ctx = ctx;
@@ -477,14 +644,14 @@ int pth_server_create(pth_ctx *ctx, pth_server **out, int mode, size_t hbm_budge
         s->budget = want;
         s->pa.init(want);
         s->n_streams = n_streams;
-        s->st.assign(n_streams, nullptr);
-        s->ev.assign(n_streams, nullptr);
-        s->slot_req.assign(n_streams, -1);
-        for (int k = 0; k < n_streams; k++) {
-            if ((e = hipStreamCreateWithFlags(&s->st[k], hipStreamNonBlocking))) return hip_bail(e, "hipStreamCreate");
-            if ((e = hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming))) return hip_bail(e, "hipEventCreate");
+        s->slot.assign(n_streams, Slot());
+        for (Slot &q : s->slot) {
+            if ((e = hipStreamCreateWithFlags(&q.st, hipStreamNonBlocking))) return hip_bail(e, "hipStreamCreate");
+            if ((e = hipEventCreateWithFlags(&q.ev, hipEventDisableTiming))) return hip_bail(e, "hipEventCreate");
         }
         if ((e = hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking))) return hip_bail(e, "hipStreamCreate");
+        if ((e = hipStreamCreateWithFlags(&s->xfer, hipStreamNonBlocking))) return hip_bail(e, "hipStreamCreate");
+        if ((e = hipEventCreateWithFlags(&s->ev_evicted, hipEventDisableTiming))) return hip_bail(e, "hipEventCreate");
         s->th = std::thread(worker_fused, s);
     } else {
         pth_params p = ctx->par;
@@ -535,11 +702,20 @@ int pth_server_add_cluster(pth_server *s, int *cluster, int n_walk, const int *n
         if (!make_plan(n_walk, n_epi, n_epj, n_spj, PTH_EXEC_FUSED, par.chunk_j, par.max_ipl, c->plan))
             return ctx->fail(PTH_ERR_CAPACITY, "pth_server_add_cluster: sizes exceed the int32 index range");
         layout(*c, n_ide, n_ids, df_enabled(par) != 0);
-        if (c->L.total > s->budget)
+        if (c->L.total + c->S.total > s->budget)
             return ctx->fail(PTH_ERR_CAPACITY, "pth_server_add_cluster: the cluster needs " +
-                                                   std::to_string(c->L.total) + " bytes, more than the budget");
-        c->image.assign(c->L.upload, 0);
-        char *h = c->image.data();
+                                                   std::to_string(c->L.total + c->S.total) +
+                                                   " bytes with a stream's scratch, more than the budget");
+        hipError_t e;
+        if ((e = hipSetDevice(ctx->device)) || (e = c->image.ensure(c->L.upload)) ||
+            (e = c->h_accp.ensure(sizeof(double4) * (size_t)std::max<long long>(ni, 1))) ||
+            (e = c->h_nnb.ensure(sizeof(int) * (size_t)std::max<long long>(ni, 1))) ||
+            (e = hipEventCreateWithFlags(&c->saved, hipEventDisableTiming))) {
+            release_cluster(*c);
+            return ctx->hip_fail(e, "pth_server_add_cluster: pinned image");
+        }
+        char *h = c->image.as<char>();
+        std::memset(h, 0, c->L.upload);
         const Layout &L = c->L;
         EpiDev *he = reinterpret_cast<EpiDev *>(h + L.A.epi);
         for (long long i = 0; i < ni; i++) {
@@ -648,7 +824,7 @@ int pth_server_wait(pth_server *s, long long t, double *latency_ms)
 }
 
 int pth_server_replay(pth_server *s, int n_req, const int *cluster, const double *arrival_s, double *latency_ms,
-                      double *wall_s)
+                      double *wall_s, int *admitted)
 {
     if (!s) return PTH_ERR_ARG;
     if (n_req < 0 || (n_req > 0 && (!cluster || !arrival_s)))
@@ -675,6 +851,7 @@ int pth_server_replay(pth_server *s, int n_req, const int *cluster, const double
         if (latency_ms) latency_ms[k] = l;
         std::lock_guard<std::mutex> lk(s->mu);
         last = std::max(last, s->req[t[k]].t_done);
+        if (admitted) admitted[k] = s->req[t[k]].admitted;
     }
     if (wall_s) *wall_s = n_req ? last - (t0 + arrival_s[0]) : 0.0;
     return PTH_OK;
@@ -699,19 +876,33 @@ int pth_server_results(pth_server *s, int k, double *acc, double *pot, long long
         }
         return PTH_OK;
     }
-    if (!c.host_valid) {
-        HIPCHK(s->ctx, hipSetDevice(s->ctx->device), "hipSetDevice");
-        HIPCHK(s->ctx, save_results(s, c), "result download");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device), "hipSetDevice");
+    if (c.off >= 0) {
+        // resident: its last request has completed; copy the outputs now
+        // (the worker is held off by the lock meanwhile)
+        if (n > 0) {
+            const char *base = s->pool.as<char>() + c.off;
+            HIPCHK(s->ctx, hipMemcpyAsync(c.h_accp.p, base + c.L.out_accp, sizeof(double4) * n,
+                                          hipMemcpyDeviceToHost, s->xfer), "result download");
+            HIPCHK(s->ctx, hipMemcpyAsync(c.h_nnb.p, base + c.L.out_nnb, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                          s->xfer), "result download");
+            HIPCHK(s->ctx, hipStreamSynchronize(s->xfer), "result download");
+        }
+    } else {
+        if (!c.saved_valid) return s->ctx->fail(PTH_ERR_STATE, "pth_server_results: results not kept");
+        HIPCHK(s->ctx, hipEventSynchronize(c.saved), "result download");
     }
+    const double4 *a = c.h_accp.as<double4>();
+    const int *nb = c.h_nnb.as<int>();
     for (long long i = 0; i < n; i++) {
-        const double4 v = c.h_accp[i];
+        const double4 v = a[i];
         if (acc) {
             acc[3 * i] = v.x;
             acc[3 * i + 1] = v.y;
             acc[3 * i + 2] = v.z;
         }
         if (pot) pot[i] = v.w;
-        if (nnb) nnb[i] = c.h_nnb[i];
+        if (nnb) nnb[i] = nb[i];
     }
     return PTH_OK;
 }
@@ -729,6 +920,26 @@ int pth_server_stats(const pth_server *s, long long *n_done, long long *n_admit,
     return PTH_OK;
 }
 
+int pth_server_memory(const pth_server *s, long long *resident_total, long long *scratch_max,
+                      long long *pinned_host, long long *n_capture, long long *n_instantiate)
+{
+    if (!s) return PTH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(const_cast<pth_server *>(s)->mu);
+    long long res = 0, sc = 0, pin = 0;
+    for (const auto &c : s->cl) {
+        if (s->mode != PTH_SERVE_FUSED) break;
+        res += (long long)c->L.total;
+        sc = std::max(sc, (long long)c->S.total);
+        pin += (long long)(c->image.cap + c->h_accp.cap + c->h_nnb.cap);
+    }
+    if (resident_total) *resident_total = res;
+    if (scratch_max) *scratch_max = sc;
+    if (pinned_host) *pinned_host = pin;
+    if (n_capture) *n_capture = s->n_capture;
+    if (n_instantiate) *n_instantiate = s->n_instantiate;
+    return PTH_OK;
+}
+
 void pth_server_free(pth_server *s)
 {
     if (!s) return;
@@ -739,17 +950,17 @@ void pth_server_free(pth_server *s)
     }
     if (s->th.joinable()) s->th.join();
     if (s->ctx->device >= 0) (void)hipSetDevice(s->ctx->device);
-    for (hipStream_t x : s->st)
-        if (x) (void)hipStreamSynchronize(x);
-    for (auto &c : s->cl) {
-        if (c->exec) (void)hipGraphExecDestroy(c->exec);
-        if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (Slot &q : s->slot)
+        if (q.st) (void)hipStreamSynchronize(q.st);
+    if (s->xfer) (void)hipStreamSynchronize(s->xfer);
+    for (auto &c : s->cl) release_cluster(*c);
+    for (Slot &q : s->slot) {
+        if (q.ev) (void)hipEventDestroy(q.ev);
+        if (q.st) (void)hipStreamDestroy(q.st);
     }
-    for (hipEvent_t x : s->ev)
-        if (x) (void)hipEventDestroy(x);
-    for (hipStream_t x : s->st)
-        if (x) (void)hipStreamDestroy(x);
+    if (s->ev_evicted) (void)hipEventDestroy(s->ev_evicted);
     if (s->cap) (void)hipStreamDestroy(s->cap);
+    if (s->xfer) (void)hipStreamDestroy(s->xfer);
     s->pool.release();
     if (s->pctx) pth_destroy(s->pctx);
     delete s;

@@ -69,7 +69,7 @@ class Profile(C.Structure):
 
 class EpjView(C.Structure):
     _fields_ = [(n, C.c_size_t) for n in ("stride", "off_pos", "off_mass", "off_rsearch", "off_id", "off_rin",
-                                          "off_rout")]
+                                          "off_rout")] + [("off_group", C.c_longlong)]
 
 
 class SpjView(C.Structure):
@@ -95,7 +95,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_get_param
             "pth_gtree_update", "pth_resident_refill", "pth_gtree_neighbors", "pth_gtree_neighbors_download",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free",
-            "pth_profile_enable", "pth_profile_take")
+            "pth_profile_enable", "pth_profile_take", "pth_server_memory")
 
 
 def lib():
@@ -128,7 +128,7 @@ def lib():
                                       C.c_int, dp, dp, dp]
     L.pth_resident_run.argtypes = [vp]
     L.pth_resident_sync.argtypes = [vp]
-    L.pth_resident_changeover.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp, dp]
+    L.pth_resident_changeover.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp, dp, vp]
     L.pth_resident_results.argtypes = [vp, dp, dp, C.POINTER(C.c_longlong), dp]
     L.pth_resident_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pth_resident_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 4
@@ -140,7 +140,8 @@ def lib():
     L.pth_server_footprint.restype = C.c_longlong
     L.pth_server_submit.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
     L.pth_server_wait.argtypes = [vp, C.c_longlong, dp]
-    L.pth_server_replay.argtypes = [vp, C.c_int, ip, dp, dp, dp]
+    L.pth_server_replay.argtypes = [vp, C.c_int, ip, dp, dp, dp, ip]
+    L.pth_server_memory.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 5
     L.pth_server_results.argtypes = [vp, C.c_int, dp, dp, C.POINTER(C.c_longlong)]
     L.pth_server_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 5
     L.pth_server_free.argtypes = [vp]
@@ -287,8 +288,9 @@ class Context:
 
 def _views(mono=False):
     f = EPJ_SOFT.fields
+    # group_data holds ArtificialParticleInformation: mass_backup, status
     ev = EpjView(EPJ_SOFT.itemsize, f["pos"][1], f["mass"][1], f["r_search"][1], f["id"][1], f["r_in"][1],
-                 f["r_out"][1])
+                 f["r_out"][1], f["group_data"][1])
     sf = SPJ_SOFT.fields
     qf = SPJ_SOFT.fields["quad"][0].fields
     qo = sf["quad"][1]
@@ -449,27 +451,31 @@ class Resident:
                                         _ptr(k[11], C.c_double), _ptr(k[12], C.c_double)), "pth_resident_create")
         self._keep = None
 
-    def changeover(self, epi_id, epj_id, epj_rin, epj_rout):
-        """Changeover mode: ids of the EPI (concatenated per walk), ids and
-        changeover radii of the EPJ."""
+    def changeover(self, epi_id, epj_id, epj_rin, epj_rout, epj_group=None):
+        """Changeover mode: ids of the EPI (concatenated per walk), ids,
+        changeover radii and group data ([n, 2]: mass_backup, status; None:
+        every particle a single) of the EPJ."""
         ei = np.ascontiguousarray(epi_id, dtype=np.int64)
         ej = np.ascontiguousarray(epj_id, dtype=np.int64)
         ri, ro = _f64(epj_rin), _f64(epj_rout)
+        g = None if epj_group is None else _f64(epj_group).reshape(-1, 2)
         self.ctx.check(lib().pth_resident_changeover(self._h, _ptr(ei, C.c_longlong), _ptr(ej, C.c_longlong),
-                                                     _ptr(ri, C.c_double), _ptr(ro, C.c_double)),
+                                                     _ptr(ri, C.c_double), _ptr(ro, C.c_double),
+                                                     None if g is None else g.ctypes.data),
                        "pth_resident_changeover")
 
     @classmethod
-    def from_lists(cls, ctx, lists, rin=None, rout=None):
+    def from_lists(cls, ctx, lists, rin=None, rout=None, group=None):
         """A resident evaluation of host.build_lists output; in changeover
-        mode, rin / rout are the per-particle changeover radii (the EPJ ids are
+        mode, rin / rout are the per-particle changeover radii and group the
+        group data ([n, 2] or None) in the lists' EPJ order (the EPJ ids are
         the particle indices)."""
         idx = np.concatenate([np.arange(b, b + n) for b, n in zip(lists.epi_begin, lists.n_epi)])
         r = cls(ctx, lists.n_epi, lists.n_epj, lists.n_spj, lists.epj_pos[idx], lists.epj_rs[idx], lists.id_epj,
                 lists.id_spj, lists.epj_pos, lists.epj_mass, lists.epj_rs, lists.spj_pos, lists.spj_mass,
                 lists.spj_quad)
         if r.co:
-            r.changeover(idx, np.arange(len(lists.epj_mass)), rin, rout)
+            r.changeover(idx, np.arange(len(lists.epj_mass)), rin, rout, group)
         return r
 
     def refill(self, tree):
@@ -714,18 +720,32 @@ class Server:
         self.ctx.check(lib().pth_server_wait(self._h, ticket, _ptr(lat, C.c_double)), "pth_server_wait")
         return float(lat[0])
 
-    def replay(self, clusters, arrival_s):
+    def replay(self, clusters, arrival_s, admitted=False):
         """Submits clusters[k] at arrival_s[k] seconds; returns the latencies
         (ms) and the wall time (s) from the first arrival to the last
-        completion."""
+        completion (and, with admitted=True, whether each request uploaded
+        its cluster)."""
         n = len(clusters)
         cl = _i32(clusters) if n else np.zeros(1, np.int32)
         ar = _f64(arrival_s) if n else np.zeros(1)
         lat = np.zeros(max(n, 1))
         wall = np.zeros(1)
+        adm = np.zeros(max(n, 1), np.int32)
         self.ctx.check(lib().pth_server_replay(self._h, n, _ptr(cl, C.c_int), _ptr(ar, C.c_double),
-                                               _ptr(lat, C.c_double), _ptr(wall, C.c_double)), "pth_server_replay")
+                                               _ptr(lat, C.c_double), _ptr(wall, C.c_double), _ptr(adm, C.c_int)),
+                       "pth_server_replay")
+        if admitted:
+            return lat[:n], float(wall[0]), adm[:n].astype(bool)
         return lat[:n], float(wall[0])
+
+    def memory(self):
+        """pth_server_memory: resident bytes of all clusters, the largest
+        per-stream scratch, pinned host bytes, graph captures and
+        instantiations."""
+        v = [C.c_longlong() for _ in range(5)]
+        self.ctx.check(lib().pth_server_memory(self._h, *[C.byref(x) for x in v]), "pth_server_memory")
+        return dict(zip(("resident_total", "scratch_max", "pinned_host", "captures", "instantiations"),
+                        (x.value for x in v)))
 
     def results(self, cluster):
         n = self._ni[cluster]

@@ -11,8 +11,12 @@
 //
 // Built with -DPETAR_HIP_CHANGEOVER (fdps_replay_co) it runs the adapter in
 // changeover mode: changeover radii and r_search scale with
-// max((m/<m>)^(1/3), 1) per particle (src/changeover.hpp:44-59) and the
-// retrieved ForceSoft carries pot_tot.
+// max((m/<m>)^(1/3), 1) per particle (src/changeover.hpp:44-59), the
+// retrieved ForceSoft carries pot_tot, and the EPJ group data carries every
+// particle type: of each 50 particles (by input index) 5 are group members
+// whose c.m. holds their mass (tree mass 0, backup mass, status -(address)),
+// 2 members of groups without artificial particles (status -LARGE_FLOAT)
+// and 3 artificial particles (status > 0).
 //
 // With a timed-pass count the FDPS call sequence (send, then every batch
 // dispatched and retrieved) is repeated that many times after one untimed
@@ -98,7 +102,7 @@ int main(int argc, char **argv)
 
     // FDPS-side records (tree order)
     std::vector<EPJSoft> epj(n);
-    std::vector<double> epj_pos(3 * n), epj_mass(n), epj_rs(n), epj_rin(n), epj_rout(n);
+    std::vector<double> epj_pos(3 * n), epj_mass(n), epj_rs(n), epj_rin(n), epj_rout(n), epj_group(2 * n, 0.0);
     std::vector<long long> epj_id(n);
     for (long k = 0; k < n; k++) {
         const long i = order[k];
@@ -110,8 +114,23 @@ int main(int argc, char **argv)
         epj[k].r_out = r_out * fac(mass[i]);
         epj[k].r_search = rs[i];
         epj[k].r_scale_next = 1.0;
+#ifdef PETAR_HIP_CHANGEOVER
+        const long t = i % 50;
+        if (t < 5) {  // member: the c.m. carries its mass
+            epj[k].group_data.d[0] = mass[i];
+            epj[k].group_data.d[1] = -(double)(i + 1);
+            epj[k].mass = 0.0;
+        } else if (t < 7) {  // member of a group without artificial particles
+            epj[k].group_data.d[0] = mass[i];
+            epj[k].group_data.d[1] = -3.4028234663852886e38 / 16;
+        } else if (t < 10) {  // artificial particle
+            epj[k].group_data.d[1] = 2.0 + (double)(t - 7);
+        }
+        epj_group[2 * k] = epj[k].group_data.d[0];
+        epj_group[2 * k + 1] = epj[k].group_data.d[1];
+#endif
         for (int d = 0; d < 3; d++) epj_pos[3 * k + d] = pos[3 * i + d];
-        epj_mass[k] = mass[i];
+        epj_mass[k] = epj[k].mass;
         epj_rs[k] = rs[i];
         epj_rin[k] = epj[k].r_in;
         epj_rout[k] = epj[k].r_out;
@@ -239,6 +258,7 @@ int main(int argc, char **argv)
     put(out, epj_rin);
     put(out, epj_rout);
     put(out, pot_tot);
+    put(out, epj_group);
     std::fclose(out);
     std::printf("fdps_replay: N=%ld walks=%ld batches=%ld written %s\n", n, n_group,
                 (n_group + n_walk_limit - 1) / n_walk_limit, argv[6]);

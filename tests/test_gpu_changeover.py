@@ -224,3 +224,53 @@ def test_co_argument_checks(single):
         r.run()
     r.close()
     ctx.close()
+
+
+def _group_types(L, seed=5):
+    """Group data (mass_backup, status) for the lists' EPJ, in tree order, and
+    the tree masses PeTar's soft force then sees: 10% group members whose c.m.
+    carries their mass (tree mass 0, backup = mass, status -(c.m. address)),
+    4% members of groups without artificial particles (mass kept, status
+    -LARGE_FLOAT), 6% artificial particles (status > 0), the rest singles."""
+    rng = np.random.default_rng(seed)
+    n = L.epj_mass.size
+    u = rng.random(n)
+    grp = np.zeros((n, 2))
+    mass = L.epj_mass.copy()
+    mem = u < 0.10
+    grp[mem, 0] = mass[mem]
+    grp[mem, 1] = -(1.0 + rng.integers(0, n, int(mem.sum())))
+    mass[mem] = 0.0
+    nocm = (u >= 0.10) & (u < 0.14)
+    grp[nocm, 0] = mass[nocm]
+    grp[nocm, 1] = -float(np.finfo(np.float32).max) / 16
+    art = (u >= 0.14) & (u < 0.20)
+    grp[art, 0] = 0.0
+    grp[art, 1] = 2.0 + rng.integers(0, 8, int(art.sum()))
+    return grp, mass
+
+
+def test_co_group_members_and_artificial_particles():
+    """Particle types of PeTar's groups in changeover mode
+    (src/hard.hpp:1453-1472 per neighbour, :1669 for the self term):
+    members with zero tree mass add their backup mass's changeover-weighted
+    potential, artificial particles leave no potential, and only singles and
+    members without a c.m. address lose their self term; against the FP64
+    oracle with the same group data, at the changeover tolerances."""
+    L, rin, rout = co_lists(16384, seed=9, f_bin=0.5)
+    grp, mass = _group_types(L)
+    L.epj_mass[:] = mass
+    a = co_args(L, rin, rout)
+    ref = O.soft_force_walks_co(**a, eps2=0.0, rout2=R_OUT ** 2, amb_rel=AMB_REL, epj_group=grp)
+    single = O.soft_force_walks_co(**a, eps2=0.0, rout2=R_OUT ** 2, amb_rel=AMB_REL)
+    amb = O.nnb_ambiguous(**{k: a[k] for k in ("n_epi", "n_epj", "epi_pos", "epi_rs", "id_epj", "epj_pos",
+                                              "epj_rs")}, centres=walk_centres(a), eps2=0.0)
+    # the types change the potentials of many i (not a vacuous comparison)
+    assert np.mean(rel_err(ref[2], single[2]) > 1e-6) > 0.1 and np.mean(rel_err(ref[1], single[1]) > 1e-6) > 0.05
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, changeover=1)
+    r = sf.Resident.from_lists(ctx, L, rin, rout, grp)
+    r.run()
+    res = r.results()
+    r.close()
+    ctx.close()
+    _check(res, ref, amb, "co group types")

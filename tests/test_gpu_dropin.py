@@ -30,6 +30,7 @@ def _read(path):
         d.update(epj_id=np.fromfile(f, np.int64, n), epj_rin=np.fromfile(f, np.float64, n),
                  epj_rout=np.fromfile(f, np.float64, n))
         pot_tot = np.fromfile(f, np.float64, ni)
+        d["epj_group"] = np.fromfile(f, np.float64, 2 * n).reshape(-1, 2)
     return d, res + (pot_tot,)
 
 
@@ -84,8 +85,10 @@ def test_cpp_dropin_fdps_replay(tmp_path, exe, n, walk_limit):
 
 def test_cpp_dropin_changeover_mode(tmp_path):
     """The adapter compiled with PETAR_HIP_CHANGEOVER (ForceSoft with
-    pot_tot, mass-scaled changeover radii) against the FP64 changeover
-    oracle on the lists the program used."""
+    pot_tot, mass-scaled changeover radii, EPJSoft group data with group
+    members, members of groups without artificial particles and artificial
+    particles) against the FP64 changeover oracle on the lists and group
+    data the program used."""
     exe = EXE + "_co"
     if not os.path.exists(exe):
         pytest.fail(f"{exe} not built (build() compiles it)")
@@ -102,7 +105,9 @@ def test_cpp_dropin_changeover_mode(tmp_path):
              epj_pos=d["epj_pos"], epj_mass=d["epj_mass"], epj_rs=d["epj_rs"], epj_id=d["epj_id"],
              epj_rin=d["epj_rin"], epj_rout=d["epj_rout"], spj_pos=d["spj_pos"], spj_mass=d["spj_mass"],
              spj_quad=d["spj_quad"])
-    racc, rps, rpt, rnnb, amb = O.soft_force_walks_co(**a, eps2=0.0, rout2=r_out ** 2, amb_rel=1e-4)
+    grp = d["epj_group"]
+    assert (grp[:, 1] < 0).sum() > 0 and (grp[:, 1] > 0).sum() > 0
+    racc, rps, rpt, rnnb, amb = O.soft_force_walks_co(**a, eps2=0.0, rout2=r_out ** 2, amb_rel=1e-4, epj_group=grp)
     ea, es, et = rel_err(acc, racc), rel_err(pot[amb == 0], rps[amb == 0]), rel_err(pot_tot, rpt)
     print(f"[dropin co] acc med {np.median(ea):.2e} p99 {np.percentile(ea, 99):.2e}; pot_soft med "
           f"{np.median(es):.2e}; pot_tot med {np.median(et):.2e} p99 {np.percentile(et, 99):.2e}; "

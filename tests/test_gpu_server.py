@@ -90,9 +90,11 @@ def test_fused_server_evicts_under_a_small_budget(clusters):
     probe = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=1 << 30, n_streams=2)
     ids = [probe.add_lists(L) for L, _ in clusters]
     fp = [probe.footprint(k) for k in ids]
+    scratch = probe.memory()["scratch_max"]
     alone = _alone(probe, ids)
     probe.close()
-    budget = int(1.3 * max(fp))  # the largest cluster and a few small ones
+    # two streams' scratch, the largest cluster and a few small ones
+    budget = int(2 * scratch + 1.3 * max(fp))
     srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=budget, n_streams=2)
     ids2 = [srv.add_lists(L) for L, _ in clusters]
     rng = np.random.default_rng(3)
@@ -102,6 +104,33 @@ def test_fused_server_evicts_under_a_small_budget(clusters):
     print(f"[server budget {budget / 2**20:.1f} MiB] admitted {st['admitted']} evicted {st['evicted']} "
           f"pool peak {st['pool_peak'] / 2**20:.1f} MiB", flush=True)
     assert st["evicted"] > 0 and st["pool_peak"] <= budget
+    for k, k2 in zip(ids, ids2):
+        assert _same(srv.results(k2), alone[k])
+    # re-admissions update the clusters' graphs instead of instantiating new ones
+    mem = srv.memory()
+    print(f"[server budget] captures {mem['captures']} instantiations {mem['instantiations']}", flush=True)
+    assert mem["instantiations"] <= 2 * len(ids2) < mem["captures"]
+    srv.close()
+    ctx.close()
+
+
+def test_fused_server_budget_for_one_stream_only(clusters):
+    """A budget that holds the largest cluster with one stream's scratch but
+    not four streams' scratch: requests still complete (idle streams give
+    their scratch up, and a blocked request frees the pool once) with the
+    same results."""
+    ctx = _ctx()
+    probe = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=1 << 30, n_streams=1)
+    ids = [probe.add_lists(L) for L, _ in clusters]
+    fp = max(probe.footprint(k) for k in ids)
+    scratch = probe.memory()["scratch_max"]
+    alone = _alone(probe, ids)
+    probe.close()
+    srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=int(1.05 * (fp + scratch)), n_streams=4)
+    ids2 = [srv.add_lists(L) for L, _ in clusters]
+    seq = np.random.default_rng(5).permutation(np.repeat(ids2, 2))
+    lat, wall = srv.replay(seq, np.zeros(seq.size))
+    assert np.all(lat > 0)
     for k, k2 in zip(ids, ids2):
         assert _same(srv.results(k2), alone[k])
     srv.close()

@@ -2,8 +2,8 @@
 log-uniform in [1e3, 2e5] (Kroupa masses, 30% primordial binaries as in the
 precision workload, theta 0.3, quadrupole SPJ, r_out 0.01, r_search 1.5
 r_out; the stream bench.py serves), replayed with Poisson arrivals under an
-HBM budget of a quarter of the stream's footprint, so that clusters are
-evicted and re-admitted throughout.
+HBM budget of every stream's scratch plus a quarter of the clusters'
+resident bytes, so that clusters are evicted and re-admitted throughout.
 
 Checked: every one of the 256 clusters gives bit-identical results alone
 (one request at a time) and in the mix; sampled walks of 12 clusters
@@ -51,14 +51,17 @@ def _sample(L, walks):
 def test_serving_stream_256_clusters(stream):
     sizes, lists = stream
     ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
-    # footprints from an unconstrained server
+    # footprints from an unconstrained server: the budget holds every
+    # stream's scratch and a quarter of the clusters' resident bytes
     probe = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=0, n_streams=4)
     fp = np.array([probe.footprint(probe.add_lists(L)) for L in lists], dtype=np.int64)
+    scratch = probe.memory()["scratch_max"]
     probe.close()
-    budget = max(int(0.25 * fp.sum()), int(1.2 * fp.max()))
+    budget = int(4 * scratch + max(0.25 * fp.sum(), 1.2 * fp.max()))
     srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=budget, n_streams=4)
     ids = [srv.add_lists(L) for L in lists]
-    print(f"[stream] footprint {fp.sum() / 2**30:.2f} GiB, budget {budget / 2**30:.2f} GiB", flush=True)
+    print(f"[stream] resident {fp.sum() / 2**30:.2f} GiB ({fp.sum() / sizes.sum():.0f} B per particle), scratch "
+          f"{scratch / 2**30:.2f} GiB per stream, budget {budget / 2**30:.2f} GiB", flush=True)
     # alone: one request at a time
     alone = {}
     for k in ids:

@@ -358,7 +358,7 @@ def test_changeover_correction_gives_weighted_newton(eps):
         pt = np.array([p_lin])
         O.orc().orc_changeover_correct_one(
             np.ascontiguousarray(xi), mi, r_in, r_out, 1, 1, np.ascontiguousarray(xj), np.array([mj]),
-            np.array([r_in]), np.array([r_out]), np.array([2], np.int64), eps2, r_out, G, acc, ps, pt)
+            np.array([r_in]), np.array([r_out]), np.array([2], np.int64), eps2, r_out, G, acc, ps, pt, None, None)
         d = xi - xj
         r = np.sqrt(d @ d + eps2)
         self_term = G * mi / r_out
@@ -370,9 +370,10 @@ def test_changeover_correction_gives_weighted_newton(eps):
 # ---- changeover mode over walks (src/hard.hpp:1404-1476,1655-1691) --------------
 
 def _co_two_body(r, rin=(0.001, 0.001), rout=(0.01, 0.01), m=(2e-4, 3e-4), rs=(0.015, 0.015), G=1.3,
-                 r_out_glb=0.01):
+                 r_out_glb=0.01, group=None):
     """Particles A (id 10) and B (id 20) at distance r along a random axis;
-    walk 0 holds A as i, walk 1 holds B, both lists hold A and B."""
+    walk 0 holds A as i, walk 1 holds B, both lists hold A and B; group
+    [2][2] their group data (mass_backup, status) or None (singles)."""
     rng = np.random.default_rng(int(r * 1e7) % 1000)
     u = rng.normal(size=3)
     u /= np.linalg.norm(u)
@@ -380,8 +381,63 @@ def _co_two_body(r, rin=(0.001, 0.001), rout=(0.01, 0.01), m=(2e-4, 3e-4), rs=(0
     pos = np.array([xa, xa + r * u])
     acc, ps, pt, nnb, amb = O.soft_force_walks_co(
         [1, 1], [2, 2], [0, 0], pos, np.array(rs), np.array([10, 20]), [0, 1, 0, 1], [], pos, np.array(m),
-        np.array(rs), np.array([10, 20]), np.array(rin), np.array(rout), [], [], [], 0.0, r_out_glb ** 2, G)
+        np.array(rs), np.array([10, 20]), np.array(rin), np.array(rout), [], [], [], 0.0, r_out_glb ** 2, G,
+        epj_group=group)
     return u, acc, ps, pt, nnb
+
+
+LARGE = float(np.finfo(np.float32).max) / 16  # an unset c.m. address: status = -LARGE (src/ptcl.hpp:219-220)
+
+
+@pytest.mark.parametrize("r", [0.0004, 0.004, 0.012])
+def test_co_member_neighbour(r):
+    """B is a group member whose c.m. holds its mass: its tree mass is zero
+    and its backup mass m_b is used for the potential
+    (src/hard.hpp:1459-1464): A gets no acc from B, pot_soft = -G m_b (1 -
+    W_pot)/r, pot_tot = -G m_b/r; B (with a c.m. address) gets no self
+    term, which its zero mass makes zero anyway."""
+    wpot, wacc, _ = _w()
+    G, mb, ma = 1.3, 3e-4, 2e-4
+    u, acc, ps, pt, _ = _co_two_body(r, m=(ma, 0.0), G=G, group=[[0.0, 0.0], [mb, -7.0]])
+    wa, wp = wacc(r, 0.001, 0.01), wpot(r, 0.001, 0.01)
+    assert np.all(acc[0] == 0.0)
+    assert ps[0] == pytest.approx(-G * mb * (1 - wp) / r, rel=1e-12, abs=1e-300)
+    assert pt[0] == pytest.approx(-G * mb / r, rel=1e-12)
+    np.testing.assert_allclose(acc[1], -G * ma * (1 - wa) * u / r ** 2, rtol=1e-12, atol=1e-300)
+    assert ps[1] == pytest.approx(-G * ma * (1 - wp) / r, rel=1e-12, abs=1e-300)
+    assert pt[1] == pytest.approx(-G * ma / r, rel=1e-12)
+
+
+def test_co_member_without_cm_keeps_its_mass_and_loses_the_self_term():
+    """A binary without artificial particles: members keep their tree mass,
+    have no c.m. address (status -LARGE_FLOAT), and their self term is
+    removed as for singles (src/hard.hpp:1665-1673); with backup mass equal
+    to the tree mass the pair gives the single-pair closed forms."""
+    G, r = 1.3, 0.004
+    m = (2e-4, 3e-4)
+    single = _co_two_body(r, m=m, G=G)
+    grp = _co_two_body(r, m=m, G=G, group=[[m[0], -LARGE], [m[1], -LARGE]])
+    for x, y in zip(single[1:], grp[1:]):
+        np.testing.assert_allclose(x, y, rtol=1e-13, atol=1e-300)
+
+
+@pytest.mark.parametrize("r", [0.0004, 0.004, 0.012])
+def test_co_artificial_neighbour(r):
+    """B is an artificial particle (status > 0): A's acc gets the changeover
+    weight with B's mass, but B leaves no potential on A (the tree's
+    linear-cutoff term is cancelled, src/hard.hpp:1466-1470); B keeps its
+    own self term -G m_B / r_out (only singles and members without a c.m.
+    lose it, src/hard.hpp:1669)."""
+    wpot, wacc, _ = _w()
+    G = 1.3
+    m = (2e-4, 3e-4)
+    u, acc, ps, pt, _ = _co_two_body(r, m=m, G=G, group=[[0.0, 0.0], [0.0, 3.0]])
+    wa, wp = wacc(r, 0.001, 0.01), wpot(r, 0.001, 0.01)
+    np.testing.assert_allclose(acc[0], G * m[1] * (1 - wa) * u / r ** 2, rtol=1e-12, atol=1e-300)
+    assert ps[0] == 0.0 and pt[0] == 0.0
+    self_b = -G * m[1] / 0.01
+    assert ps[1] == pytest.approx(self_b - G * m[0] * (1 - wp) / r, rel=1e-12)
+    assert pt[1] == pytest.approx(self_b - G * m[0] / r, rel=1e-12)
 
 
 @pytest.mark.parametrize("r", [0.0004, 0.004, 0.009, 0.012])
