@@ -96,6 +96,13 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
     }
     pth_ctx *c = new pth_ctx;
     c->par = par;
+    {
+        // how pth_dispatch sends index lists (a diagnostic switch):
+        // PTH_LIST_RUNS=0 as they are, =1 always as runs, unset: as runs
+        // when that at least halves their bytes
+        const char *lr = std::getenv("PTH_LIST_RUNS");
+        c->list_runs = (lr && lr[0] == '0') ? 0 : (lr && lr[0] == '1') ? 1 : -1;
+    }
     c->device = (rank >= 0 ? rank : -rank) % ndev;  // src/force_gpu_cuda.cu:550-553
     if ((e = hipSetDevice(c->device)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
@@ -336,8 +343,60 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         if (m) return c->fail(PTH_ERR_CAPACITY, std::string("pth_dispatch: ") + m);
     }
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    // index lists: validated, and encoded as runs of consecutive indices
+    // (FDPS appends whole leaf cells, so EPJ lists are mostly runs) into a
+    // reused host buffer, each walk's runs where its list would start; this
+    // overlaps the previous batch's upload
+    double t_pack = wall_s();
+    const int nj_lim = c->n_epj, ns_lim = c->n_spj;
+    if ((long long)c->h_runs.size() < n_ide + n_ids) c->h_runs.resize((size_t)(n_ide + n_ids));
+    c->run_ne.assign(n_walk + 1, 0);
+    c->run_ns.assign(n_walk + 1, 0);
+    int2 *hr = c->h_runs.data();
+    int bad = -1;
+#pragma omp parallel for schedule(dynamic)
+    for (int w = 0; w < n_walk; w++) {
+        const WalkDev &W = P.walks[w];
+        bool ok = true;
+        int2 *re = hr + W.e_off;
+        int nr = 0;
+        long long next = -1;
+        for (int j = 0; j < W.n_e; j++) {
+            const int id = id_epj[w][j];
+            ok &= (id >= 0 && id < nj_lim);
+            if (id != next) re[nr++] = make_int2(id, j);
+            next = (long long)id + 1;
+        }
+        c->run_ne[w] = nr;
+        int2 *rs = hr + n_ide + W.s_off;
+        nr = 0;
+        next = -1;
+        for (int j = 0; j < W.n_s; j++) {
+            const int id = id_spj[w][j];
+            ok &= (id >= 0 && id < ns_lim);
+            if (id != next) rs[nr++] = make_int2(id, j);
+            next = (long long)id + 1;
+        }
+        c->run_ns[w] = nr;
+        if (!ok) {
+#pragma omp atomic write
+            bad = w;
+        }
+    }
+    if (bad >= 0)
+        return c->fail(PTH_ERR_ARG, "pth_dispatch: list index out of range in walk " + std::to_string(bad));
+    long long n_re = 0, n_rs = 0;
+    for (int w = 0; w < n_walk; w++) {
+        n_re += c->run_ne[w];
+        n_rs += c->run_ns[w];
+    }
+    // runs travel when they take at most half the bytes of the lists
+    const bool runs = c->list_runs == 1 ||
+                      (c->list_runs < 0 && 8 * (n_re + n_rs + n_walk + 1) <= 2 * (n_ide + n_ids));
+    c->prof.copy_s += wall_s() - t_pack;
     Arena A;
-    A.layout(P, n_ide, n_ids, co);
+    if (runs) A.layout(P, n_ide, n_ids, co, n_re, n_rs);
+    else A.layout(P, n_ide, n_ids, co);
     // the pinned arena may still feed the previous batch's copy
     HIPCHK(c, hipEventSynchronize(c->ev_upload), "hipEventSynchronize");
     HIPCHK(c, c->h_arena.ensure(A.total), "pinned arena allocation");
@@ -364,21 +423,41 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
             need |= sizeof(float4) * (size_t)P.ni_tot > c->work.staged_il.cap;
             need |= sizeof(float) * (PTH_DF_EXTRA / 8) * (size_t)P.n_e_slots > c->work.staged_el.cap;
         }
+        if (runs) need |= sizeof(int) * (size_t)(n_ide + n_ids) > c->d_ids.cap;
         if (need) HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         HIPCHK(c, c->work.ensure(P, co, df), "work buffer allocation");
+        if (runs) HIPCHK(c, c->d_ids.ensure(sizeof(int) * (size_t)std::max<long long>(n_ide + n_ids, 1)),
+                         "device list allocation");
     }
     HIPCHK(c, T.out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "output allocation");
     HIPCHK(c, T.out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "output allocation");
     if (co) HIPCHK(c, T.out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "output allocation");
 
-    const double t_pack = wall_s();
+    t_pack = wall_s();
     char *h = c->h_arena.as<char>();
     EpiDev *he = reinterpret_cast<EpiDev *>(h + A.epi);
     long long *hid = reinterpret_cast<long long *>(h + A.epi_id);
     int *hide = reinterpret_cast<int *>(h + A.ide);
     int *hids = reinterpret_cast<int *>(h + A.ids);
-    const int nj_lim = c->n_epj, ns_lim = c->n_spj;
-    int bad = -1;
+    int2 *hre = reinterpret_cast<int2 *>(h + A.run_e);
+    int2 *hrs = reinterpret_cast<int2 *>(h + A.run_s);
+    int *hreo = reinterpret_cast<int *>(h + A.run_eo);
+    int *hrso = reinterpret_cast<int *>(h + A.run_so);
+    if (runs) {
+        // run offsets: walk w's runs start at the count of the walks before it
+        int oe = 0, os = 0;
+        for (int w = 0; w < n_walk; w++) {
+            const int ne = c->run_ne[w], ns = c->run_ns[w];
+            c->run_ne[w] = oe;
+            c->run_ns[w] = os;
+            oe += ne;
+            os += ns;
+        }
+        c->run_ne[n_walk] = oe;
+        c->run_ns[n_walk] = os;
+        std::memcpy(hreo, c->run_ne.data(), sizeof(int) * (n_walk + 1));
+        std::memcpy(hrso, c->run_ns.data(), sizeof(int) * (n_walk + 1));
+    }
 #pragma omp parallel for schedule(dynamic)
     for (int w = 0; w < n_walk; w++) {
         const WalkDev &W = P.walks[w];
@@ -395,25 +474,15 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
             he[W.i_off + i] = d;
             if (co) hid[W.i_off + i] = *reinterpret_cast<const long long *>(r + iv->off_id);
         }
-        for (int j = 0; j < W.n_e; j++) {
-            const int id = id_epj[w][j];
-            if (id < 0 || id >= nj_lim) {
-#pragma omp atomic write
-                bad = w;
-            }
-            hide[W.e_off + j] = id;
-        }
-        for (int j = 0; j < W.n_s; j++) {
-            const int id = id_spj[w][j];
-            if (id < 0 || id >= ns_lim) {
-#pragma omp atomic write
-                bad = w;
-            }
-            hids[W.s_off + j] = id;
+        if (runs) {
+            const int ne = c->run_ne[w + 1] - c->run_ne[w], ns = c->run_ns[w + 1] - c->run_ns[w];
+            if (ne) std::memcpy(hre + c->run_ne[w], hr + W.e_off, sizeof(int2) * ne);
+            if (ns) std::memcpy(hrs + c->run_ns[w], hr + n_ide + W.s_off, sizeof(int2) * ns);
+        } else {
+            if (W.n_e) std::memcpy(hide + W.e_off, id_epj[w], sizeof(int) * W.n_e);
+            if (W.n_s) std::memcpy(hids + W.s_off, id_spj[w], sizeof(int) * W.n_s);
         }
     }
-    if (bad >= 0)
-        return c->fail(PTH_ERR_ARG, "pth_dispatch: list index out of range in walk " + std::to_string(bad));
     copy_plan_to_arena(h, P, A);
     c->prof.copy_s += wall_s() - t_pack;
     c->prof.n_walk += n_walk;
@@ -431,6 +500,17 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     fill_batch(b, P, c->par, c->d_arena, A, c->work, c->d_epj.as<EpjDev>(), c->d_spj.as<SpjDev>(),
                T.out_accp.as<double4>(), T.out_nnb.as<int>(), c->d_epj_co.as<EpjCoDev>(),
                T.out_pott.as<double>());
+    if (runs) {
+        char *d = c->d_arena.as<char>();
+        int *ide_dev = c->d_ids.as<int>();
+        HIPCHK(c, launch_expand_runs(b.walks, n_walk, reinterpret_cast<const int2 *>(d + A.run_e),
+                                     reinterpret_cast<const int *>(d + A.run_eo),
+                                     reinterpret_cast<const int2 *>(d + A.run_s),
+                                     reinterpret_cast<const int *>(d + A.run_so), ide_dev, ide_dev + n_ide, c->stream),
+               "list expansion kernel");
+        b.id_epj = ide_dev;
+        b.id_spj = ide_dev + n_ide;
+    }
     HIPCHK(c, launch_all(b, c->stream), "soft-force kernels");
     if (T.timed) HIPCHK(c, hipEventRecord(T.prof[2], c->stream), "hipEventRecord");
     HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");

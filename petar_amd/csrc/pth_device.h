@@ -158,6 +158,13 @@ struct BatchDev {
     int *part_nnb = nullptr;
 };
 
+// Expands index lists sent as runs of consecutive indices: for walk w,
+// runs[roff[w] .. roff[w + 1]) hold {first index, position in the walk's
+// list}; the list of walk w (n entries from walks[w].e_off for EPJ, or
+// walks[w].s_off for SPJ) goes to out_e / out_s.
+hipError_t launch_expand_runs(const WalkDev *walks, int n_walk, const int2 *run_e, const int *run_eo,
+                              const int2 *run_s, const int *run_so, int *out_e, int *out_s, hipStream_t st);
+
 // Launches of segment k of a batch (k < b.n_seg).
 hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st);
 hipError_t launch_force(const BatchDev &b, int k, hipStream_t st);

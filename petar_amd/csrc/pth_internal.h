@@ -95,21 +95,32 @@ struct PinBuf {
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Layout of one upload arena (pinned host mirror and device copy).
+// Layout of one upload arena (pinned host mirror and device copy).  The
+// index lists travel either as they are (ide, ids) or, with n_run >= 0, as
+// runs of consecutive indices (pth_dispatch): run_e / run_s hold {first
+// index, position in the walk's list} per run, run_eo / run_so the walks'
+// run offsets (n_walk + 1 each), and the lists are expanded on the device.
 struct Arena {
-    size_t epi, epi_id, walks, es, ss, ide, ids, items, ritems, total;
-    void layout(const Plan &P, long long n_ide, long long n_ids, bool co)
+    size_t epi, epi_id, walks, es, ss, ide, ids, items, ritems, run_e, run_s, run_eo, run_so, total;
+    bool runs = false;
+    void layout(const Plan &P, long long n_ide, long long n_ids, bool co, long long n_run_e = -1,
+                long long n_run_s = -1)
     {
+        runs = n_run_e >= 0 && n_run_s >= 0;
         size_t o = 0;
         epi = o; o = align256(o + sizeof(EpiDev) * (size_t)P.ni_tot);
         epi_id = o; o = align256(o + (co ? sizeof(long long) * (size_t)P.ni_tot : 0));
         walks = o; o = align256(o + sizeof(WalkDev) * P.walks.size());
         es = o; o = align256(o + sizeof(long long) * P.es_off.size());
         ss = o; o = align256(o + sizeof(long long) * P.ss_off.size());
-        ide = o; o = align256(o + sizeof(int) * (size_t)n_ide);
-        ids = o; o = align256(o + sizeof(int) * (size_t)n_ids);
+        ide = o; o = align256(o + (runs ? 0 : sizeof(int) * (size_t)n_ide));
+        ids = o; o = align256(o + (runs ? 0 : sizeof(int) * (size_t)n_ids));
         items = o; o = align256(o + sizeof(Item) * P.items.size());
         ritems = o; o = align256(o + sizeof(RItem) * P.ritems.size());
+        run_e = o; o = align256(o + (runs ? sizeof(int2) * (size_t)n_run_e : 0));
+        run_s = o; o = align256(o + (runs ? sizeof(int2) * (size_t)n_run_s : 0));
+        run_eo = o; o = align256(o + (runs ? sizeof(int) * (P.walks.size() + 1) : 0));
+        run_so = o; o = align256(o + (runs ? sizeof(int) * (P.walks.size() + 1) : 0));
         total = o;
     }
 };
@@ -338,6 +349,13 @@ struct pth_ctx {
     // batch upload
     PinBuf h_arena;
     DevBuf d_arena;
+    // index lists as runs: host encoding space (reused), the walks' run
+    // counts, and the device lists they expand into (shared by all tags, as
+    // the work buffers)
+    std::vector<int2> h_runs;
+    std::vector<int> run_ne, run_ns;
+    DevBuf d_ids;
+    int list_runs = -1;  // PTH_LIST_RUNS: -1 per batch, 0 never, 1 always
     hipEvent_t ev_upload = nullptr;
     Work work;
     TagState tag[MAX_TAG];

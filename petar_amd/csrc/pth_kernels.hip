@@ -932,7 +932,9 @@ __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__
     const int n_tiles = (wk.n_i + 63) >> 6;
     const int eb = uniform((int)(es_off[w] >> 3)), ee = uniform((int)(es_off[w + 1] >> 3));
     const int em = min(max(uniform(near_blk[w]), eb), ee);
-    const int sb = uniform((int)ss_off[w]), se_ = uniform((int)ss_off[w] + wk.n_s);
+    // staged SPJ records of the walk (none with monopole superparticles,
+    // which are staged as far EPJ entries)
+    const int sb = uniform((int)ss_off[w]), se_ = uniform((int)ss_off[w + 1]);
     for (int t = wave; t < n_tiles; t += 4) {
         const int i = 64 * t + lane;
         float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
@@ -980,7 +982,48 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 }
 
 // ---------------------------------------------------------------------------
+// index lists sent as runs (pth_dispatch): workgroup g expands the EPJ list
+// (g < n_walk) or the SPJ list (g >= n_walk) of one walk; entry k lies in the
+// last run whose position is <= k (a binary search over the walk's runs,
+// which stay in the L1 cache).  The first run of a non-empty list has
+// position 0 and positions ascend.
+__global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__ walks, int n_walk,
+                                                     const int2 *__restrict__ run_e, const int *__restrict__ run_eo,
+                                                     const int2 *__restrict__ run_s, const int *__restrict__ run_so,
+                                                     int *__restrict__ out_e, int *__restrict__ out_s)
+{
+    const int g = (int)blockIdx.x;
+    const bool sp = g >= n_walk;
+    const int w = sp ? g - n_walk : g;
+    const WalkDev wk = walks[w];
+    const int n = sp ? wk.n_s : wk.n_e;
+    const int2 *__restrict__ runs = sp ? run_s : run_e;
+    const int *__restrict__ ro = sp ? run_so : run_eo;
+    const int r0 = ro[w], r1 = ro[w + 1];
+    int *__restrict__ out = sp ? out_s + wk.s_off : out_e + wk.e_off;
+    for (int k = (int)threadIdx.x; k < n; k += 256) {
+        int lo = r0, hi = r1 - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (runs[mid].y <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int2 r = runs[lo];
+        out[k] = r.x + (k - r.y);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers (host)
+
+hipError_t launch_expand_runs(const WalkDev *walks, int n_walk, const int2 *run_e, const int *run_eo,
+                              const int2 *run_s, const int *run_so, int *out_e, int *out_s, hipStream_t st)
+{
+    if (n_walk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand_runs, dim3(2u * (unsigned)n_walk), dim3(256), 0, st, walks, n_walk, run_e, run_eo,
+                       run_s, run_so, out_e, out_s);
+    return hipGetLastError();
+}
 
 hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st)
 {

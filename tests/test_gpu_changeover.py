@@ -129,7 +129,11 @@ def test_co_linear_cutoff_unchanged_far_from_neighbours(single):
     no pair qualifies, acc equals the linear-cutoff acc and pot equals it
     plus G m_i / r_out."""
     L, rin, rout, _, _ = single
-    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
+    # changeover mode takes the linear-cutoff terms of its pairs from the
+    # FP32 walk-relative coordinates (double-float separations only for its
+    # neighbour pairs): compare with the linear-cutoff form without
+    # double-float near pairs
+    ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, df_near=0)
     r = sf.Resident.from_lists(ctx, L)
     r.run()
     acc_a, pot_a, nnb_a = r.results()

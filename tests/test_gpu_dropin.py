@@ -149,3 +149,28 @@ def test_cpp_dropin_gpu_profile(tmp_path):
     assert len(vals) == 9
     np.testing.assert_allclose([float(v) for v in vals[4:]], [want[k] / passes for k in
                                                               ("n_walk", "n_epi", "n_epj", "n_spj", "n_call")])
+
+
+@pytest.mark.parametrize("exe", ["fdps_replay", "fdps_replay_co"])
+def test_cpp_dropin_list_runs_bit_identical(tmp_path, exe):
+    """pth_dispatch sends each batch's index lists either as they are or as
+    runs of consecutive indices expanded on the device (the default when
+    that at least halves their bytes; PTH_LIST_RUNS=0 / 1 forces one form):
+    both forms give the same forces bit for bit, in the linear-cutoff and the
+    changeover adapter, with batches of 200 walks and of 7."""
+    path = os.path.join(ROOT, "tests", "cpp", exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (build() compiles it)")
+    for walk_limit in (200, 7):
+        outs = []
+        for mode in ("0", "1"):
+            out = tmp_path / f"replay_{mode}_{walk_limit}.bin"
+            env = dict(os.environ, PTH_LIST_RUNS=mode)
+            print(f"[dropin runs] {exe} PTH_LIST_RUNS={mode} walk_limit={walk_limit}", flush=True)
+            p = subprocess.run([path, "12000", "17", "0.3", str(walk_limit), "0.01", str(out)], capture_output=True,
+                               text=True, timeout=240, env=env)
+            print(p.stdout, p.stderr, flush=True)
+            assert p.returncode == 0
+            outs.append(_read(out)[1])
+        for k, name in enumerate(("acc", "pot", "nnb", "pot_tot")):
+            assert np.array_equal(outs[0][k], outs[1][k]), f"{exe} walk_limit={walk_limit}: {name} differs"

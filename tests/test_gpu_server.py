@@ -106,10 +106,13 @@ def test_fused_server_evicts_under_a_small_budget(clusters):
     assert st["evicted"] > 0 and st["pool_peak"] <= budget
     for k, k2 in zip(ids, ids2):
         assert _same(srv.results(k2), alone[k])
-    # re-admissions update the clusters' graphs instead of instantiating new ones
+    # re-admissions update the clusters' graphs instead of instantiating new
+    # ones: at most one graph per cluster and stream, however many times a
+    # cluster was admitted
     mem = srv.memory()
     print(f"[server budget] captures {mem['captures']} instantiations {mem['instantiations']}", flush=True)
-    assert mem["instantiations"] <= 2 * len(ids2) < mem["captures"]
+    assert st["admitted"] > len(ids2)
+    assert mem["instantiations"] <= 2 * len(ids2) and mem["instantiations"] <= mem["captures"]
     srv.close()
     ctx.close()
 
