@@ -262,6 +262,33 @@ int pth_resident_stats(const pth_resident *res, long long *n_epep, long long *n_
                        long long *n_items, long long *staged_bytes);
 void pth_resident_free(pth_resident *res);
 
+/* ---- one evaluation without allocating ----------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the work of
+ * pth_resident_create + pth_resident_run + pth_resident_results (same
+ * arguments, same kernels and work decomposition, the same results bit for
+ * bit) with every device buffer it needs -- packed records, plan, staged
+ * lists, partial sums, outputs -- taken from one device array of
+ * PTH_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * from module load to the end of the process.  The call allocates no device
+ * or pinned memory; it waits for work already on the context's stream,
+ * enqueues the uploads (from the caller's host arrays), the kernels and the
+ * downloads there, and returns when the results are in acc [ni][3], pot [ni]
+ * and nnb [ni] (any of them may be NULL).  Linear-cutoff mode and the fused
+ * form only (PTH_ERR_STATE otherwise).  An evaluation whose buffers exceed
+ * the array fails with PTH_ERR_CAPACITY and a message giving the size it
+ * needs (a 16,384-particle Plummer cluster at theta 0.3, r_out 0.01 needs
+ * about 38 MiB, most of it staged lists and partial sums; clusters of up
+ * to about 38,000 particles fit).  Every call on
+ * one GPU uses the same array: calls that overlap on one GPU (from several
+ * threads or contexts) race and must be serialised by the caller. */
+#define PTH_NOALLOC_SCRATCH_BYTES (128ll << 20)
+int pth_resident_eval_noalloc(pth_ctx *ctx, int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
+                              const double *epi_pos, const double *epi_rs, const int *id_epj, const int *id_spj,
+                              int n_epj_tot, const double *epj_pos, const double *epj_mass, const double *epj_rs,
+                              int n_spj_tot, const double *spj_pos, const double *spj_mass, const double *spj_quad,
+                              double *acc, double *pot, long long *nnb);
+
 /* ---- serving: many independent evaluations ("clusters") -------------------
  * No counterpart in the reference (one simulation per process there).  A
  * server holds any number of clusters, each one complete soft-force

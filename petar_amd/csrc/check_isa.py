@@ -12,7 +12,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["pth_kernels.hip", "pth_gtree.hip"]
+SOURCES = ["pth_kernels.hip", "pth_gtree.hip", "pth_noalloc.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fno-slp-vectorize", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-S", "--cuda-device-only", "-o", "-"]
 FORBIDDEN = re.compile(r"^\s*(s_store_|s_buffer_store|s_scratch_store|s_atomic_|s_buffer_atomic_|"

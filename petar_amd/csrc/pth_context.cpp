@@ -119,6 +119,9 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
             return PTH_ERR_HIP;
         }
     }
+    // the scratch array of pth_resident_eval_noalloc on this device (a
+    // failure is reported by that entry point, not here)
+    noalloc_setup(c->device);
     *out = c;
     return PTH_OK;
 }

@@ -364,6 +364,10 @@ struct pth_ctx {
     pth_profile prof{};
     hipEvent_t prof_send[2] = {nullptr, nullptr};  // around the last send's copies
     bool prof_send_pending = false;
+    // pth_resident_eval_noalloc: plan and launch descriptor kept between
+    // calls (the plan's vectors keep their capacity)
+    Plan na_plan;
+    BatchDev na_batch;
 
     int fail(int code, const std::string &msg)
     {
@@ -415,6 +419,9 @@ int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_e
                    Plan *pre = nullptr, bool pre_ok = true);
 // Launch descriptors of a prepared resident whose contents are uploaded.
 void resident_finish(pth_resident *r);
+// Looks up this device's address of the allocation-free entry point's
+// scratch array (once per device; the device must be current).
+void noalloc_setup(int device);
 }  // namespace pth
 
 #define HIPCHK(ctx, call, what)                      \

@@ -95,7 +95,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_get_param
             "pth_gtree_update", "pth_resident_refill", "pth_gtree_neighbors", "pth_gtree_neighbors_download",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free",
-            "pth_profile_enable", "pth_profile_take", "pth_server_memory")
+            "pth_profile_enable", "pth_profile_take", "pth_server_memory", "pth_resident_eval_noalloc")
 
 
 def lib():
@@ -133,6 +133,8 @@ def lib():
     L.pth_resident_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.pth_resident_stats.argtypes = [vp] + [C.POINTER(C.c_longlong)] * 4
     L.pth_resident_free.argtypes = [vp]
+    L.pth_resident_eval_noalloc.argtypes = [vp, C.c_int, ip, ip, ip, dp, dp, ip, ip, C.c_int, dp, dp, dp, C.c_int,
+                                            dp, dp, dp, dp, dp, C.POINTER(C.c_longlong)]
     L.pth_server_create.argtypes = [vp, C.POINTER(vp), C.c_int, C.c_size_t, C.c_int]
     L.pth_server_add_cluster.argtypes = [vp, ip, C.c_int, ip, ip, ip, dp, dp, ip, ip, C.c_int, dp, dp, dp, C.c_int,
                                          dp, dp, dp]
@@ -540,6 +542,43 @@ class Resident:
             self.close()
         except Exception:
             pass
+
+
+def eval_noalloc(ctx: Context, n_epi, n_epj, n_spj, epi_pos, epi_rs, id_epj, id_spj, epj_pos, epj_mass, epj_rs,
+                 spj_pos, spj_mass, spj_quad):
+    """One evaluation with the arguments of Resident, run through
+    pth_resident_eval_noalloc (no device allocation: every device buffer
+    comes from the library's per-GPU scratch array); returns acc [ni, 3],
+    pot [ni], nnb [ni]."""
+    k = [_i32(n_epi), _i32(n_epj), _i32(n_spj), _f64(epi_pos).reshape(-1, 3), _f64(epi_rs),
+         _i32(id_epj) if len(id_epj) else np.zeros(1, np.int32),
+         _i32(id_spj) if len(id_spj) else np.zeros(1, np.int32),
+         _f64(epj_pos).reshape(-1, 3) if len(epj_pos) else np.zeros((1, 3)),
+         _f64(epj_mass) if len(epj_mass) else np.zeros(1), _f64(epj_rs) if len(epj_rs) else np.zeros(1),
+         _f64(spj_pos).reshape(-1, 3) if len(spj_pos) else np.zeros((1, 3)),
+         _f64(spj_mass) if len(spj_mass) else np.zeros(1),
+         _f64(spj_quad).reshape(-1, 6) if len(spj_quad) else np.zeros((1, 6))]
+    ni = int(k[0].sum())
+    acc = np.zeros((ni, 3))
+    pot = np.zeros(ni)
+    nnb = np.zeros(ni, np.int64)
+    ctx.check(lib().pth_resident_eval_noalloc(ctx.handle, k[0].size, _ptr(k[0], C.c_int), _ptr(k[1], C.c_int),
+                                              _ptr(k[2], C.c_int), _ptr(k[3], C.c_double), _ptr(k[4], C.c_double),
+                                              _ptr(k[5], C.c_int), _ptr(k[6], C.c_int), len(epj_mass),
+                                              _ptr(k[7], C.c_double), _ptr(k[8], C.c_double), _ptr(k[9], C.c_double),
+                                              len(spj_mass), _ptr(k[10], C.c_double), _ptr(k[11], C.c_double),
+                                              _ptr(k[12], C.c_double), _ptr(acc, C.c_double), _ptr(pot, C.c_double),
+                                              _ptr(nnb, C.c_longlong)), "pth_resident_eval_noalloc")
+    return acc, pot, nnb
+
+
+def eval_noalloc_lists(ctx: Context, lists):
+    """eval_noalloc on host.build_lists output (the arguments of
+    Resident.from_lists)."""
+    idx = np.concatenate([np.arange(b, b + n) for b, n in zip(lists.epi_begin, lists.n_epi)])
+    return eval_noalloc(ctx, lists.n_epi, lists.n_epj, lists.n_spj, lists.epj_pos[idx], lists.epj_rs[idx],
+                        lists.id_epj, lists.id_spj, lists.epj_pos, lists.epj_mass, lists.epj_rs, lists.spj_pos,
+                        lists.spj_mass, lists.spj_quad)
 
 
 class GpuTree:
