@@ -537,6 +537,7 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--no-device-tree", action="store_true")
     ap.add_argument("--no-changeover", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay comparison")
     ap.add_argument("--df-near", type=int, default=1, choices=[0, 1],
                     help="near EPJ pairs in double-float walk-relative coordinates (linear-cutoff mode)")
     a = ap.parse_args()
@@ -637,6 +638,31 @@ def main():
         "autotuned": tuned,
     }
     acc, pot, nnb = res.results()
+    if a.exec == "fused" and not a.no_graph:
+        # the same evaluation replayed as one captured hipGraph (same tiles,
+        # segments and streams, same warm-up and step count)
+        ctxg = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=cj_used,
+                          max_ipl=ipl_used, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams,
+                          df_near=a.df_near, graph=1)
+        rg = sf.Resident.from_lists(ctxg, L)
+        for _ in range(a.warmup):
+            rg.run()
+        rg.sync()
+        torch.cuda.synchronize()
+        tg0 = time.perf_counter()
+        for _ in range(a.steps):
+            rg.run()
+        rg.sync()
+        torch.cuda.synchronize()
+        ms_graph = (time.perf_counter() - tg0) * 1e3 / a.steps
+        ga, gp, gn = rg.results()
+        rg.close()
+        ctxg.close()
+        out["graph_replay"] = dict(ms_per_step=round(ms_graph, 4), gint_s=round(n_int / (ms_graph * 1e-3) / 1e9, 2),
+                                   launched_ms_per_step=round(ms_step, 4),
+                                   bit_identical=bool(np.array_equal(ga, acc) and np.array_equal(gp, pot)
+                                                      and np.array_equal(gn, nnb)))
+        log(f"[bench] graph replay: {out['graph_replay']}")
     out["config"]["flop_convention_note"] = (
         "EPJ pair 26 (src/force_gpu_cuda.cu:57-74): dx 3 sub, r2 3 FMA (6), search radius max + square + "
         "compare (3), cutoff max (1), rsqrt (1), m rinv (1), G rinv rinv pij (3), acc 3 FMA (6), pot G pij + sub "

@@ -111,6 +111,10 @@ typedef struct pth_params {
                       only, the pair of src/soft_force.hpp:125-158 (no cutoff,
                       no neighbour count); the quadrupole members of the SPJ
                       view and arrays are not read */
+    int graph;     /* resident evaluations: 1 captures an evaluation's launches
+                      (all segments and streams) once as a hipGraph and
+                      replays it on later runs; 0 launches them one by one.
+                      Results do not depend on it. */
 } pth_params;
 
 /* Changeover mode folds PeTar's changeover correction of the soft force
@@ -135,7 +139,7 @@ typedef struct pth_params {
 
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
  * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99,
- * segments 8, streams 3, df_near 1, spj_monopole 0. */
+ * segments 8, streams 3, df_near 1, spj_monopole 0, graph 0. */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */
@@ -246,8 +250,9 @@ int pth_resident_changeover(pth_resident *res, const long long *epi_id, const lo
 int pth_resident_results(pth_resident *res, double *acc, double *pot, long long *nnb, double *pot_tot);
 /* Per-kernel GPU times of the last evaluation (ms), from HIP events on the
  * context's stream: stage (centres + gather), force, reduce.  With more than
- * one segment the phases overlap: ms_force is then the whole evaluation and
- * ms_stage = ms_reduce = 0. */
+ * one segment the phases overlap, and a graph replay (pth_params.graph) is
+ * timed as a whole: ms_force is then the whole evaluation and ms_stage =
+ * ms_reduce = 0. */
 int pth_resident_timing(pth_resident *res, float *ms_stage, float *ms_force, float *ms_reduce);
 /* Re-plans the work decomposition (fused form) with another i-per-lane
  * bound and chunk size; the uploaded inputs stay. */

@@ -34,6 +34,7 @@ void pth_params_default(pth_params *p)
     p->streams = 3;
     p->df_near = 1;
     p->spj_monopole = 0;
+    p->graph = 0;
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -85,6 +86,7 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
     if (par.max_walk <= 0) par.max_walk = 1000;
     par.df_near = par.df_near ? 1 : 0;
     par.spj_monopole = par.spj_monopole ? 1 : 0;
+    par.graph = par.graph ? 1 : 0;
     par.segments = std::min(std::max(par.segments, 1), PTH_MAX_SEGMENTS);
     par.streams = std::min(std::max(par.streams, 1), 8);
     int ndev = 0;
@@ -157,6 +159,7 @@ int pth_set_params(pth_ctx *c, const pth_params *p)
     c->par.streams = std::min(std::max(p->streams, 1), 8);
     c->par.df_near = p->df_near ? 1 : 0;
     c->par.spj_monopole = p->spj_monopole ? 1 : 0;
+    c->par.graph = p->graph ? 1 : 0;
     return PTH_OK;
 }
 
@@ -753,6 +756,83 @@ int pth_resident_changeover(pth_resident *r, const long long *epi_id, const long
     return PTH_OK;
 }
 
+// The launches of one evaluation on stream st: on st alone, or pipelined
+// over the side streams (forked from st by fork_ev, joined back into st).
+// With timed (st the context's stream), events bracket the stage, force and
+// reduce phases of the one-stream form.
+static int resident_enqueue(pth_resident *r, hipStream_t st, hipEvent_t fork_ev, bool timed)
+{
+    pth_ctx *c = r->ctx;
+    if (r->side.empty() || r->b.n_seg < 2 || (int)r->staged.size() < r->b.n_seg) {
+        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_stage(r->b, k, st), "stage kernels");
+        if (timed) HIPCHK(c, hipEventRecord(r->ev[1], st), "hipEventRecord");
+        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_force(r->b, k, st), "force kernel");
+        if (timed) HIPCHK(c, hipEventRecord(r->ev[2], st), "hipEventRecord");
+        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_reduce(r->b, k, st), "reduce kernel");
+        r->pipelined = false;
+        return PTH_OK;
+    }
+    // a pipeline: the first side stream stages the segments one after
+    // another; segment k's force and reduce run on side stream
+    // 1 + k mod (n - 1) once its staging is done, so the staging of the
+    // segments after k overlaps k's force kernels
+    const int ns = (int)r->side.size();
+    if (!timed) HIPCHK(c, hipEventRecord(fork_ev, st), "hipEventRecord");
+    for (int s = 0; s < ns; s++) HIPCHK(c, hipStreamWaitEvent(r->side[s], fork_ev, 0), "hipStreamWaitEvent");
+    for (int k = 0; k < r->b.n_seg; k++) {
+        HIPCHK(c, launch_stage(r->b, k, r->side[0]), "stage kernels");
+        HIPCHK(c, hipEventRecord(r->staged[k], r->side[0]), "hipEventRecord");
+    }
+    for (int k = 0; k < r->b.n_seg; k++) {
+        hipStream_t fs = r->side[ns > 1 ? 1 + k % (ns - 1) : 0];
+        HIPCHK(c, hipStreamWaitEvent(fs, r->staged[k], 0), "hipStreamWaitEvent");
+        HIPCHK(c, launch_force(r->b, k, fs), "force kernel");
+        HIPCHK(c, launch_reduce(r->b, k, fs), "reduce kernel");
+    }
+    for (int s = 0; s < ns; s++) {
+        HIPCHK(c, hipEventRecord(r->side_done[s], r->side[s]), "hipEventRecord");
+        HIPCHK(c, hipStreamWaitEvent(st, r->side_done[s], 0), "hipStreamWaitEvent");
+    }
+    r->pipelined = true;
+    return PTH_OK;
+}
+
+// Graph mode: records the evaluation's launches on the capture stream and
+// instantiates them (again after any rebuild of the launch descriptor).
+static int resident_capture(pth_resident *r)
+{
+    pth_ctx *c = r->ctx;
+    if (r->gexec && r->g_gen == r->b_gen) return PTH_OK;
+    if (r->gexec) {
+        HIPCHK(c, hipGraphExecDestroy(r->gexec), "hipGraphExecDestroy");
+        r->gexec = nullptr;
+    }
+    if (!r->cap) HIPCHK(c, hipStreamCreateWithFlags(&r->cap, hipStreamNonBlocking), "hipStreamCreate");
+    if (!r->fork) HIPCHK(c, hipEventCreateWithFlags(&r->fork, hipEventDisableTiming), "hipEventCreate");
+    // the capture stream starts after the context's stream's work
+    HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    HIPCHK(c, hipStreamBeginCapture(r->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    const int rc = resident_enqueue(r, r->cap, r->fork, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(r->cap, &graph);
+    if (rc != PTH_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return c->hip_fail(e, "hipStreamEndCapture");
+    }
+    const hipError_t ei = hipGraphInstantiate(&r->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        r->gexec = nullptr;
+        return c->hip_fail(ei, "hipGraphInstantiate");
+    }
+    r->g_gen = r->b_gen;
+    return PTH_OK;
+}
+
 int pth_resident_run(pth_resident *r)
 {
     if (!r) return PTH_ERR_ARG;
@@ -760,39 +840,21 @@ int pth_resident_run(pth_resident *r)
     if (r->co && !r->co_ready)
         return c->fail(PTH_ERR_STATE, "pth_resident_run: changeover mode needs pth_resident_changeover first");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
-    HIPCHK(c, hipEventRecord(r->ev[0], c->stream), "hipEventRecord");
-    if (r->side.empty() || r->b.n_seg < 2 || (int)r->staged.size() < r->b.n_seg) {
-        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_stage(r->b, k, c->stream), "stage kernels");
-        HIPCHK(c, hipEventRecord(r->ev[1], c->stream), "hipEventRecord");
-        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_force(r->b, k, c->stream), "force kernel");
-        HIPCHK(c, hipEventRecord(r->ev[2], c->stream), "hipEventRecord");
-        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_reduce(r->b, k, c->stream), "reduce kernel");
+    if (c->par.graph) {
+        const int rc = resident_capture(r);
+        if (rc != PTH_OK) return rc;
+        HIPCHK(c, hipEventRecord(r->ev[0], c->stream), "hipEventRecord");
+        HIPCHK(c, hipGraphLaunch(r->gexec, c->stream), "hipGraphLaunch");
         HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
-        r->pipelined = false;
-    } else {
-        // a pipeline: the first side stream stages the segments one after
-        // another; segment k's force and reduce run on side stream
-        // 1 + k mod (n - 1) once its staging is done, so the staging of the
-        // segments after k overlaps k's force kernels
-        const int ns = (int)r->side.size();
-        for (int s = 0; s < ns; s++) HIPCHK(c, hipStreamWaitEvent(r->side[s], r->ev[0], 0), "hipStreamWaitEvent");
-        for (int k = 0; k < r->b.n_seg; k++) {
-            HIPCHK(c, launch_stage(r->b, k, r->side[0]), "stage kernels");
-            HIPCHK(c, hipEventRecord(r->staged[k], r->side[0]), "hipEventRecord");
-        }
-        for (int k = 0; k < r->b.n_seg; k++) {
-            hipStream_t st = r->side[ns > 1 ? 1 + k % (ns - 1) : 0];
-            HIPCHK(c, hipStreamWaitEvent(st, r->staged[k], 0), "hipStreamWaitEvent");
-            HIPCHK(c, launch_force(r->b, k, st), "force kernel");
-            HIPCHK(c, launch_reduce(r->b, k, st), "reduce kernel");
-        }
-        for (int s = 0; s < ns; s++) {
-            HIPCHK(c, hipEventRecord(r->side_done[s], r->side[s]), "hipEventRecord");
-            HIPCHK(c, hipStreamWaitEvent(c->stream, r->side_done[s], 0), "hipStreamWaitEvent");
-        }
-        HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
-        r->pipelined = true;
+        r->graph_run = true;
+        r->ran = true;
+        return PTH_OK;
     }
+    HIPCHK(c, hipEventRecord(r->ev[0], c->stream), "hipEventRecord");
+    const int rc = resident_enqueue(r, c->stream, r->ev[0], true);
+    if (rc != PTH_OK) return rc;
+    HIPCHK(c, hipEventRecord(r->ev[3], c->stream), "hipEventRecord");
+    r->graph_run = false;
     r->ran = true;
     return PTH_OK;
 }
@@ -854,7 +916,7 @@ int pth_resident_timing(pth_resident *r, float *ms_stage, float *ms_force, float
     if (!r->ran) return c->fail(PTH_ERR_STATE, "pth_resident_timing: no evaluation run");
     HIPCHK(c, hipEventSynchronize(r->ev[3]), "hipEventSynchronize");
     float t = 0;
-    if (r->pipelined) {
+    if (r->pipelined || r->graph_run) {
         if (ms_stage) *ms_stage = 0.f;
         if (ms_reduce) *ms_reduce = 0.f;
         if (ms_force) { HIPCHK(c, hipEventElapsedTime(&t, r->ev[0], r->ev[3]), "hipEventElapsedTime"); *ms_force = t; }
@@ -898,6 +960,7 @@ int pth_resident_replan(pth_resident *r, int max_ipl, int chunk_j)
                r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(), r->out_pott.as<double>());
     r->b.items = r->plan_dev.as<const Item>();
     r->b.ritems = reinterpret_cast<const RItem *>(r->plan_dev.as<char>() + off_r);
+    r->b_gen++;
     r->plan = std::move(P);
     r->max_ipl = max_ipl;
     r->chunk_j = chunk_j;
@@ -977,6 +1040,9 @@ void pth_resident_free(pth_resident *r)
         if (x) (void)hipEventDestroy(x);
     for (hipStream_t x : r->side)
         if (x) (void)hipStreamDestroy(x);
+    if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+    if (r->fork) (void)hipEventDestroy(r->fork);
+    if (r->cap) (void)hipStreamDestroy(r->cap);
     delete r;
 }
 
@@ -1101,6 +1167,7 @@ void resident_finish(pth_resident *r)
     fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
                r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(),
                r->out_pott.as<double>());
+    r->b_gen++;
     r->max_ipl = c->par.max_ipl;
     r->chunk_j = c->par.chunk_j;
     r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 48 * P.n_s_rec);

@@ -396,6 +396,14 @@ struct pth_resident {
     std::vector<hipEvent_t> side_done;   // one per side stream
     std::vector<hipEvent_t> staged;      // one per segment: its staging is done
     bool pipelined = false;              // the last run used the side streams
+    // graph mode: the evaluation captured on its own stream (fork and join
+    // events over the side streams) and replayed; b_gen counts rebuilds of
+    // the launch descriptor, g_gen the one the graph holds
+    hipStream_t cap = nullptr;
+    hipEvent_t fork = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    long long b_gen = 0, g_gen = -1;
+    bool graph_run = false;              // the last run replayed the graph
     Work work;
     BatchDev b;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -459,3 +459,71 @@ def test_double_float_near_pairs():
         assert np.median(ea) <= MED and np.median(ep) <= MED
     assert err[1].max() <= 1e-5 and np.percentile(err[1], 99) <= 1e-6
     assert err[1].max() < err[0].max()
+
+
+@pytest.mark.parametrize("segments,streams", [(8, 3), (1, 1)])
+def test_graph_replay_equals_launches(c1, segments, streams):
+    """pth_params.graph = 1: the evaluation's launches (all segments and
+    streams) captured once as a hipGraph and replayed.  Every replay, with
+    eager evaluations of another context and synchronisations between the
+    replays, equals the launched evaluation bit for bit; a replan (new work
+    items) is captured again and still equals the launched form."""
+    L, a, ref, amb = c1
+    eager = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=segments, streams=streams)
+    re = sf.Resident.from_lists(eager, L)
+    re.run()
+    want = re.results()
+    gctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=segments, streams=streams, graph=1)
+    rg = sf.Resident.from_lists(gctx, L)
+    for k in range(4):
+        rg.run()
+        got = rg.results()
+        print(f"[graph seg={segments}] replay {k}: identical {all(np.array_equal(x, y) for x, y in zip(got, want))}",
+              flush=True)
+        assert all(np.array_equal(x, y) for x, y in zip(got, want))
+        re.run()
+        re.sync()
+    _check(got, ref, amb, f"graph segments={segments}")
+    for ipl, cj in ((1, 256), (4, 2048)):
+        rg.replan(ipl, cj)
+        re.replan(ipl, cj)
+        re.run()
+        want2 = re.results()
+        for k in range(2):
+            rg.run()
+            assert all(np.array_equal(x, y) for x, y in zip(rg.results(), want2)), f"replan {ipl},{cj} replay {k}"
+    ms = rg.timing()
+    assert ms[1] > 0 and ms[0] == 0 and ms[2] == 0
+    rg.close()
+    re.close()
+    gctx.close()
+    eager.close()
+
+
+def test_graph_replay_after_refill():
+    """Graph mode on an evaluation fed from the device tree: a refill from a
+    rebuilt tree (new particles) is captured again and equals the launched
+    evaluation of the same tree."""
+    outs = []
+    for graph in (0, 1):
+        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, graph=graph)
+        res = []
+        g = None
+        r = None
+        for seed in (5, 6):
+            m, x, _ = host.cluster(12000, "plummer", kroupa=True, seed=seed)
+            if g is None:
+                g = sf.GpuTree(ctx, x, m, np.full(m.size, 1.5 * R_OUT), theta=0.3)
+                r = sf.Resident.from_gtree(ctx, g)
+            else:
+                g.update(x, m, np.full(m.size, 1.5 * R_OUT))
+                r.refill(g)
+            r.run()
+            r.run()
+            res.append(r.results())
+        r.close()
+        g.close()
+        ctx.close()
+        outs.append(res)
+    for k in range(2):
+        assert all(np.array_equal(x, y) for x, y in zip(outs[0][k], outs[1][k])), f"tree {k}"
