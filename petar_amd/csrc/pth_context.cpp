@@ -310,6 +310,7 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
     c->n_epj = n_epj_tot;
     c->n_spj = n_spj_tot;
     c->j_sent = true;
+    c->tab_valid = false;
     return PTH_OK;
 }
 
@@ -415,7 +416,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         need |= sizeof(double4) * P.walks.size() > c->work.centre.cap;
         need |= sizeof(float4) * (size_t)P.ni_tot > c->work.staged_i.cap;
         need |= sizeof(float) * 5 * (size_t)P.n_e_slots > c->work.staged_e.cap;
-        need |= sizeof(float) * 12 * (size_t)P.n_s_rec > c->work.staged_s.cap;
+        need |= sizeof(float) * PTH_SPJ_STAGED * (size_t)P.n_s_rec > c->work.staged_s.cap;
         need |= sizeof(double4) * 64 * (size_t)P.n_part > c->work.part_accp.cap;
         if (co) {
             need |= sizeof(float4) * (size_t)P.ni_tot > c->work.staged_ic.cap;
@@ -503,9 +504,20 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     if (T.timed) HIPCHK(c, hipEventRecord(T.prof[1], c->stream), "hipEventRecord");
     if (c->par.exec_mode == PTH_EXEC_PLAIN) HIPCHK(c, hipStreamSynchronize(c->stream), "upload sync");
     BatchDev b;
+    // the SPJ table of the sent superparticles (quadrupole form), for this
+    // batch's eps2
+    if (!c->mono_sent && c->n_spj > 0 && (!c->tab_valid || c->tab_eps2 != (float)c->par.eps2)) {
+        if (sizeof(SpjTab) * (size_t)c->n_spj > c->d_spj_tab.cap)
+            HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        HIPCHK(c, c->d_spj_tab.ensure(sizeof(SpjTab) * (size_t)c->n_spj), "SPJ table allocation");
+        HIPCHK(c, launch_spj_table(c->d_spj.as<SpjDev>(), c->n_spj, (float)c->par.eps2, c->d_spj_tab.as<SpjTab>(),
+                                   c->stream), "SPJ table kernel");
+        c->tab_valid = true;
+        c->tab_eps2 = (float)c->par.eps2;
+    }
     fill_batch(b, P, c->par, c->d_arena, A, c->work, c->d_epj.as<EpjDev>(), c->d_spj.as<SpjDev>(),
                T.out_accp.as<double4>(), T.out_nnb.as<int>(), c->d_epj_co.as<EpjCoDev>(),
-               T.out_pott.as<double>());
+               T.out_pott.as<double>(), c->d_spj_tab.as<SpjTab>());
     if (runs) {
         char *d = c->d_arena.as<char>();
         int *ide_dev = c->d_ids.as<int>();
@@ -727,7 +739,7 @@ int pth_resident_create(pth_ctx *c, pth_resident **out, int n_walk, const int *n
     if (n_spj_tot && (e = P.mono ? hipMemcpy(r->spj.p, sm.data(), sizeof(EpjDev) * n_spj_tot, hipMemcpyHostToDevice)
                                  : hipMemcpy(r->spj.p, sj.data(), sizeof(SpjDev) * n_spj_tot, hipMemcpyHostToDevice)))
         return bail(e, "resident upload");
-    resident_finish(r);
+    if ((e = resident_finish(r))) return bail(e, "resident SPJ table");
     *out = r;
     return PTH_OK;
 }
@@ -957,7 +969,8 @@ int pth_resident_replan(pth_resident *r, int max_ipl, int chunk_j)
     if (br) HIPCHK(c, hipMemcpy(r->plan_dev.as<char>() + off_r, P.ritems.data(), br, hipMemcpyHostToDevice),
                    "plan upload");
     fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
-               r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(), r->out_pott.as<double>());
+               r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(), r->out_pott.as<double>(),
+               r->spj_tab.as<SpjTab>());
     r->b.items = r->plan_dev.as<const Item>();
     r->b.ritems = reinterpret_cast<const RItem *>(r->plan_dev.as<char>() + off_r);
     r->b_gen++;
@@ -1141,6 +1154,9 @@ int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_e
         }
     }
     r->n_epj_tot = n_epj_tot;
+    r->n_spj_tot = n_spj_tot;
+    if (!P.mono && (e = r->spj_tab.ensure(sizeof(SpjTab) * (size_t)std::max(n_spj_tot, 1))))
+        return bail(e, "resident SPJ table");
     return PTH_OK;
 }
 
@@ -1160,17 +1176,23 @@ int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_ep
     return PTH_OK;
 }
 
-void resident_finish(pth_resident *r)
+hipError_t resident_finish(pth_resident *r)
 {
     const Plan &P = r->plan;
     pth_ctx *c = r->ctx;
+    if (!P.mono) {
+        const hipError_t e = launch_spj_table(r->spj.as<SpjDev>(), r->n_spj_tot, (float)c->par.eps2,
+                                              r->spj_tab.as<SpjTab>(), c->stream);
+        if (e != hipSuccess) return e;
+    }
     fill_batch(r->b, P, c->par, r->arena, r->A, r->work, r->epj.as<EpjDev>(), r->spj.as<SpjDev>(),
                r->out_accp.as<double4>(), r->out_nnb.as<int>(), r->epj_co.as<EpjCoDev>(),
-               r->out_pott.as<double>());
+               r->out_pott.as<double>(), P.mono ? nullptr : r->spj_tab.as<SpjTab>());
     r->b_gen++;
     r->max_ipl = c->par.max_ipl;
     r->chunk_j = c->par.chunk_j;
-    r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 48 * P.n_s_rec);
+    r->staged_bytes = (long long)(16 * P.ni_tot + 20 * P.n_e_slots + 4 * PTH_SPJ_STAGED * P.n_s_rec);
+    return hipSuccess;
 }
 
 }  // namespace pth

@@ -27,6 +27,15 @@ struct __align__(64) SpjDev {
     float pad[3];
 };
 
+// Walk-independent part of an SPJ term (32 B), one per superparticle of the
+// force call: the mass, the traceless quadrupole q' = 3Q - tr I (xx, yy, zz,
+// xy, yz, zx; the rearrangement of src/phantomquad_for_p3t_x86.hpp:144-163)
+// and -eps^2 tr.  A staged SPJ record holds only the walk-relative position
+// and the index of its entry here.
+struct __align__(32) SpjTab {
+    float m, qxx, qyy, qzz, qxy, qyz, qzx, mtr;
+};
+
 // Changeover data of an EPJ (24 B, changeover mode only): the particle's
 // changeover radii (EPJSoft::r_in, r_out), its group type and backup mass
 // (EPJSoft::group_data.artificial, src/artificial_particles.hpp:20-102) and
@@ -145,18 +154,25 @@ struct BatchDev {
     const int *id_spj = nullptr;
     const EpjDev *epj = nullptr;
     const SpjDev *spj = nullptr;
+    const SpjTab *spj_tab = nullptr;  // quadrupole superparticles: the table of spj
     const Item *items = nullptr;
     const RItem *ritems = nullptr;
     double4 *centre = nullptr;
     int *near_blk = nullptr;  // per walk: first block of the far region (absolute)
     float4 *staged_i = nullptr;
     float *staged_e = nullptr;
-    float *staged_s = nullptr;
+    float *staged_s = nullptr;  // per SPJ record {x-c, y-c, z-c, table index (int bits)}
     double4 *out_accp = nullptr;
     int *out_nnb = nullptr;
     double4 *part_accp = nullptr;
     int *part_nnb = nullptr;
 };
+
+// The SPJ table of n quadrupole superparticles for softening eps2 (the
+// batch's FP32 eps2, as the staging uses it).
+hipError_t launch_spj_table(const SpjDev *spj, long long n, float eps2, SpjTab *tab, hipStream_t st);
+// Floats per staged SPJ record.
+#define PTH_SPJ_STAGED 4
 
 // Expands index lists sent as runs of consecutive indices: for walk w,
 // runs[roff[w] .. roff[w + 1]) hold {first index, position in the walk's

@@ -1286,7 +1286,7 @@ int resident_fill_gtree(pth_resident *r, const pth_gtree *t, const char *who)
         hipLaunchKernelGGL(k_res_nodes, dim3(blocks(t->n_node)), dim3(256), 0, st, t->n_node, t->nmass.as<double>(),
                            t->ncom.as<double>(), t->nquad.as<double>(), r->spj.as<SpjDev>());
     if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return bail(e, "resident records");
-    resident_finish(r);
+    if ((e = resident_finish(r))) return bail(e, "resident SPJ table");
     return PTH_OK;
 }
 

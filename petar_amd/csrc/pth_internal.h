@@ -150,7 +150,7 @@ struct Work {
         if ((e = near_blk.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)))) return e;
         if ((e = staged_i.ensure(sizeof(float4) * std::max<long long>(P.ni_tot, 1)))) return e;
         if ((e = staged_e.ensure(sizeof(float) * 5 * std::max<long long>(P.n_e_slots, 8)))) return e;
-        if ((e = staged_s.ensure(sizeof(float) * 12 * std::max<long long>(P.n_s_rec, 1)))) return e;
+        if ((e = staged_s.ensure(sizeof(float) * PTH_SPJ_STAGED * std::max<long long>(P.n_s_rec, 1)))) return e;
         if ((e = part_accp.ensure(sizeof(double4) * 64 * std::max(P.n_part, 1)))) return e;
         if ((e = part_nnb.ensure(sizeof(int) * 64 * std::max(P.n_part, 1)))) return e;
         return hipSuccess;
@@ -215,7 +215,7 @@ inline void set_segments(BatchDev &b, const Plan &P)
 
 inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const DevBuf &arena,
                 const Arena &A, Work &W, const EpjDev *epj, const SpjDev *spj, double4 *out_accp,
-                int *out_nnb, const EpjCoDev *epj_co, double *out_pott)
+                int *out_nnb, const EpjCoDev *epj_co, double *out_pott, const SpjTab *spj_tab)
 {
     b.n_walk = (int)P.walks.size();
     b.ni_tot = P.ni_tot;
@@ -248,6 +248,7 @@ inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const 
     b.ritems = reinterpret_cast<const RItem *>(base + A.ritems);
     b.epj = epj;
     b.spj = spj;
+    b.spj_tab = spj_tab;
     b.centre = W.centre.as<double4>();
     b.near_blk = W.near_blk.as<int>();
     b.staged_i = W.staged_i.as<float4>();
@@ -355,6 +356,11 @@ struct pth_ctx {
     std::vector<int2> h_runs;
     std::vector<int> run_ne, run_ns;
     DevBuf d_ids;
+    // SPJ table of the sent superparticles, built at the first dispatch
+    // after a send (again when eps2 changes)
+    DevBuf d_spj_tab;
+    bool tab_valid = false;
+    float tab_eps2 = 0.f;
     int list_runs = -1;  // PTH_LIST_RUNS: -1 per batch, 0 never, 1 always
     hipEvent_t ev_upload = nullptr;
     Work work;
@@ -388,7 +394,8 @@ struct pth_resident {
     DevBuf arena, epj, spj, out_accp, out_nnb;
     DevBuf epj_co, out_pott;  // changeover mode
     bool co = false, co_ready = false;
-    int n_epj_tot = 0;
+    int n_epj_tot = 0, n_spj_tot = 0;
+    DevBuf spj_tab;  // SPJ table (quadrupole superparticles)
     DevBuf plan_dev;  // work items and reduction items after a re-plan
     int max_ipl = 0, chunk_j = 0;
     int n_seg = 1;                       // walk segments of the plan
@@ -425,8 +432,9 @@ int resident_prepare(pth_ctx *c, pth_resident *&out, int n_walk, const int *n_ep
 int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
                    long long n_ide, long long n_ids, int n_epj_tot, int n_spj_tot, const char *who,
                    Plan *pre = nullptr, bool pre_ok = true);
-// Launch descriptors of a prepared resident whose contents are uploaded.
-void resident_finish(pth_resident *r);
+// Builds the SPJ table and the launch descriptors of a prepared resident
+// whose contents are uploaded (or being uploaded on the context's stream).
+hipError_t resident_finish(pth_resident *r);
 // Looks up this device's address of the allocation-free entry point's
 // scratch array (once per device; the device must be current).
 void noalloc_setup(int device);

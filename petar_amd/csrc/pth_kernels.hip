@@ -75,17 +75,34 @@ __device__ __forceinline__ void put_epj(float *se, long long slot, float x, floa
     blk[32 + l] = r2;
 }
 
-// SPJ record {x-c, y-c, z-c, m, q'xx, q'yy, q'zz, q'xy, q'yz, q'zx, -eps^2 tr, 0},
-// q' = 3Q - tr I (the traceless rearrangement of
-// src/phantomquad_for_p3t_x86.hpp:144-163)
-__device__ __forceinline__ void put_spj(float4 *o, const SpjDev &p, const double (&c)[3], double eps2)
+// Staged SPJ record {x-c, y-c, z-c, index of the superparticle's SpjTab
+// entry}: only the position depends on the walk.
+__device__ __forceinline__ void put_spj(float4 *o, double x, double y, double z, const double (&c)[3], int id)
 {
+    *o = make_float4((float)(x - c[0]), (float)(y - c[1]), (float)(z - c[2]), __int_as_float(id));
+}
+
+// The SPJ table: m, q' = 3Q - tr I (the traceless rearrangement of
+// src/phantomquad_for_p3t_x86.hpp:144-163) and -eps^2 tr, from the FP32
+// moments in FP64 and rounded once.
+__global__ __launch_bounds__(256) void k_spj_table(const SpjDev *__restrict__ spj, long long n, double eps2,
+                                                   SpjTab *__restrict__ tab)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SpjDev p = spj[i];
     const double qxx = p.qxx, qyy = p.qyy, qzz = p.qzz;
     const double tr = qxx + qyy + qzz;
-    o[0] = make_float4((float)(p.x - c[0]), (float)(p.y - c[1]), (float)(p.z - c[2]), p.m);
-    o[1] = make_float4((float)(3.0 * qxx - tr), (float)(3.0 * qyy - tr), (float)(3.0 * qzz - tr),
-                       (float)(3.0 * (double)p.qxy));
-    o[2] = make_float4((float)(3.0 * (double)p.qyz), (float)(3.0 * (double)p.qxz), (float)(-eps2 * tr), 0.f);
+    SpjTab t;
+    t.m = p.m;
+    t.qxx = (float)(3.0 * qxx - tr);
+    t.qyy = (float)(3.0 * qyy - tr);
+    t.qzz = (float)(3.0 * qzz - tr);
+    t.qxy = (float)(3.0 * (double)p.qxy);
+    t.qyz = (float)(3.0 * (double)p.qyz);
+    t.qzx = (float)(3.0 * (double)p.qxz);
+    t.mtr = (float)(-eps2 * tr);
+    tab[i] = t;
 }
 
 // bucket of a particle id in the per-walk LDS table (PTH_CO_HASH buckets)
@@ -355,9 +372,9 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         const int n_miss = __syncthreads_count(miss);
         if (tid == 0) a.walk_err[w] = n_miss;
     }
-    // -- SPJ list: 12-float records relative to the centre, U per thread
-    //    and pass so that U gathers are in flight
-    float4 *out_s = reinterpret_cast<float4 *>(a.staged_s) + a.ss_off[w] * 3;
+    // -- SPJ list: positions relative to the centre and table indices, U
+    //    per thread and pass so that U gathers are in flight
+    float4 *out_s = reinterpret_cast<float4 *>(a.staged_s) + a.ss_off[w];
     const int *ids = a.id_spj + wk.s_off;
     const SpjDev *__restrict__ spj = a.spj;
     // (two in changeover mode: fewer registers, so the staging of the next
@@ -372,13 +389,17 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
             const int k = k0 + 256 * u + tid;
             id[u] = k < n_srec ? ids[k] : -1;
         }
-        SpjDev p[U];
+        double px[U], py[U], pz[U];
 #pragma unroll
         for (int u = 0; u < U; u++)
-            if (id[u] >= 0) p[u] = spj[id[u]];
+            if (id[u] >= 0) {
+                px[u] = spj[id[u]].x;
+                py[u] = spj[id[u]].y;
+                pz[u] = spj[id[u]].z;
+            }
 #pragma unroll
         for (int u = 0; u < U; u++)
-            if (id[u] >= 0) put_spj(out_s + 3 * (k0 + 256 * u + tid), p[u], c, a.eps2);
+            if (id[u] >= 0) put_spj(out_s + (k0 + 256 * u + tid), px[u], py[u], pz[u], c, id[u]);
     }
 }
 
@@ -616,12 +637,25 @@ struct SpjRec {
     float x, y, z, m, qxx, qyy, qzz, qxy, qyz, qzx, mtr;
 };
 
-// One staged SPJ record from a wave-uniform address: scalar loads, used as
-// SGPR operands, or with VC copied into VGPRs first.
-template <bool VC> __device__ __forceinline__ SpjRec spj_load(cfp R)
+// A staged SPJ position and its table index (scalar loads).
+struct SpjPos {
+    float x, y, z;
+    int id;
+};
+__device__ __forceinline__ SpjPos spj_pos(cfp ss, int s)
 {
-    return SpjRec{jv<VC>(R[0]), jv<VC>(R[1]), jv<VC>(R[2]), jv<VC>(R[3]), jv<VC>(R[4]), jv<VC>(R[5]),
-                  jv<VC>(R[6]), jv<VC>(R[7]), jv<VC>(R[8]), jv<VC>(R[9]), jv<VC>(R[10])};
+    const cfp R = ss + (size_t)s * PTH_SPJ_STAGED;
+    return SpjPos{R[0], R[1], R[2], __float_as_int(R[3])};
+}
+
+// One SPJ record from wave-uniform addresses: its staged position and its
+// table entry, scalar loads used as SGPR operands, or with VC copied into
+// VGPRs first.
+template <bool VC> __device__ __forceinline__ SpjRec spj_rec(const SpjPos &q, cfp tab)
+{
+    const cfp T = tab + (size_t)q.id * 8;
+    return SpjRec{jv<VC>(q.x), jv<VC>(q.y), jv<VC>(q.z), jv<VC>(T[0]), jv<VC>(T[1]), jv<VC>(T[2]),
+                  jv<VC>(T[3]),  jv<VC>(T[4]), jv<VC>(T[5]), jv<VC>(T[6]), jv<VC>(T[7])};
 }
 
 // One SPJ record against the lane's i (src/force_gpu_cuda.cu:276-323 in the
@@ -669,26 +703,32 @@ __device__ __forceinline__ void spj_flush(const float (&ax)[IPL], const float (&
     }
 }
 
-// SPJ records [s_begin, s_end): the record after the current one is loaded
-// before the current one is used (scalar loads in flight during the pair
-// arithmetic instead of a wait at the top of every record).
+// SPJ records [s_begin, s_end): the next record's table entry and the
+// position after it are loaded before the current record is used (scalar
+// loads in flight during the pair arithmetic instead of a wait at the top of
+// every record; the table index comes from the position loaded one record
+// earlier).
 template <int IPL, bool VC>
-__device__ __forceinline__ void spj_records(cfp ss, int s_begin, int s_end, const float (&xi)[IPL],
+__device__ __forceinline__ void spj_records(cfp ss, cfp tab, int s_begin, int s_end, const float (&xi)[IPL],
                                             const float (&yi)[IPL], const float (&zi)[IPL], const Consts &K,
                                             double (&A)[IPL][4])
 {
     if (s_begin >= s_end) return;
-    SpjRec c = spj_load<VC>(ss + (size_t)s_begin * 12);
+    SpjRec c = spj_rec<VC>(spj_pos(ss, s_begin), tab);
+    SpjPos pn = spj_pos(ss, min(s_begin + 1, s_end - 1));
     for (int s0 = s_begin; s0 < s_end; s0 += PTH_FLUSH_SPJ) {
         const int s1 = min(s0 + PTH_FLUSH_SPJ, s_end);
         float ax[IPL], ay[IPL], az[IPL], p[IPL];
 #pragma unroll
         for (int q = 0; q < IPL; q++) ax[q] = ay[q] = az[q] = p[q] = 0.f;
         for (int s = s0; s < s1; s++) {
-            // next record (the last one is re-read: stays in bounds)
-            const SpjRec n = spj_load<VC>(ss + (size_t)min(s + 1, s_end - 1) * 12);
+            // the next record and the position after it (the last ones are
+            // re-read: stays in bounds)
+            const SpjPos pnn = spj_pos(ss, min(s + 2, s_end - 1));
+            const SpjRec n = spj_rec<VC>(pn, tab);
             spj_pair<IPL>(c, xi, yi, zi, K, ax, ay, az, p);
             c = n;
+            pn = pnn;
         }
         spj_flush<IPL>(ax, ay, az, p, A);
     }
@@ -701,6 +741,7 @@ struct ForceArgs {
     const int *near_blk;
     const float4 *staged_i;
     const float *staged_e, *staged_s;
+    const SpjTab *spj_tab;
     const float4 *staged_ic;  // changeover mode
     const float4 *staged_il;  // changeover mode
     const float *staged_ec;   // changeover mode
@@ -778,7 +819,8 @@ __device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int
     } else
         epj_blocks<IPL, NNB_LE, true, VC>(se, it.e_blk_begin, e_mid, xi, yi, zi, rsi2, K, A, nnb);
     epj_blocks<IPL, NNB_LE, false, VC>(se, e_mid, it.e_blk_end, xi, yi, zi, rsi2, K, A, nnb);
-    spj_records<IPL, VC>(as_const(a.staged_s), it.s_begin, it.s_end, xi, yi, zi, K, A);
+    spj_records<IPL, VC>(as_const(a.staged_s), as_const(reinterpret_cast<const float *>(a.spj_tab)), it.s_begin,
+                         it.s_end, xi, yi, zi, K, A);
     const double G = a.G;
 #pragma unroll
     for (int q = 0; q < IPL; q++) {
@@ -882,23 +924,25 @@ __device__ __forceinline__ void plain_epj(cfp se, int b_begin, int b_end, float 
     }
 }
 
-__device__ __forceinline__ void plain_spj(cfp ss, int s_begin, int s_end, float xi, float yi, float zi,
+__device__ __forceinline__ void plain_spj(cfp ss, cfp tab, int s_begin, int s_end, float xi, float yi, float zi,
                                           float eps2, double (&A)[4])
 {
     for (int s0 = s_begin; s0 < s_end; s0 += PTH_FLUSH_SPJ) {
         const int s1 = min(s0 + PTH_FLUSH_SPJ, s_end);
         float ax = 0.f, ay = 0.f, az = 0.f, p = 0.f;
         for (int s = s0; s < s1; s++) {
-            const cfp R = ss + (size_t)s * 12;
+            const cfp R = ss + (size_t)s * PTH_SPJ_STAGED;
+            // table entry: m, q'xx, q'yy, q'zz, q'xy, q'yz, q'zx, -eps^2 tr
+            const cfp T = tab + (size_t)__float_as_int(R[3]) * 8;
             const float dx = xi - R[0], dy = yi - R[1], dz = zi - R[2];
-            const float m = R[3];
+            const float m = T[0];
             const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
             const float ri = __builtin_amdgcn_rsqf(r2);
             const float ri2 = ri * ri, ri3 = ri * ri2, ri4 = ri2 * ri2, ri5 = ri2 * ri3;
-            const float qrx = __builtin_fmaf(R[9], dz, __builtin_fmaf(R[7], dy, R[4] * dx));
-            const float qry = __builtin_fmaf(R[8], dz, __builtin_fmaf(R[7], dx, R[5] * dy));
-            const float qrz = __builtin_fmaf(R[8], dy, __builtin_fmaf(R[9], dx, R[6] * dz));
-            const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, R[10])));
+            const float qrx = __builtin_fmaf(T[6], dz, __builtin_fmaf(T[4], dy, T[1] * dx));
+            const float qry = __builtin_fmaf(T[5], dz, __builtin_fmaf(T[4], dx, T[2] * dy));
+            const float qrz = __builtin_fmaf(T[5], dy, __builtin_fmaf(T[6], dx, T[3] * dz));
+            const float rqr = __builtin_fmaf(qrz, dz, __builtin_fmaf(qry, dy, __builtin_fmaf(qrx, dx, T[7])));
             const float rqr_ri4 = rqr * ri4;
             const float meff = __builtin_fmaf(rqr_ri4, 0.5f, m);
             const float meff3 = __builtin_fmaf(rqr_ri4, 2.5f, m) * ri3;
@@ -920,6 +964,7 @@ __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__
                                                      const float4 *__restrict__ staged_i,
                                                      const float *__restrict__ staged_e,
                                                      const float *__restrict__ staged_s,
+                                                     const SpjTab *__restrict__ spj_tab,
                                                      const long long *__restrict__ es_off,
                                                      const long long *__restrict__ ss_off,
                                                      float eps2, float rout2, double G,
@@ -943,7 +988,8 @@ __global__ __launch_bounds__(256) void k_force_plain(const WalkDev *__restrict__
         int nnb = 0;
         plain_epj<NNB_LE, true>(as_const(staged_e), eb, em, v.x, v.y, v.z, v.w, eps2, rout2, A, nnb);
         plain_epj<NNB_LE, false>(as_const(staged_e), em, ee, v.x, v.y, v.z, v.w, eps2, rout2, A, nnb);
-        plain_spj(as_const(staged_s), sb, se_, v.x, v.y, v.z, eps2, A);
+        plain_spj(as_const(staged_s), as_const(reinterpret_cast<const float *>(spj_tab)), sb, se_, v.x, v.y, v.z,
+                  eps2, A);
         if (i < wk.n_i) {
             out_accp[wk.i_off + i] = make_double4(G * A[0], G * A[1], G * A[2], G * A[3]);
             out_nnb[wk.i_off + i] = nnb;
@@ -1016,6 +1062,13 @@ __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__
 // ---------------------------------------------------------------------------
 // launchers (host)
 
+hipError_t launch_spj_table(const SpjDev *spj, long long n, float eps2, SpjTab *tab, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spj_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, spj, n, (double)eps2, tab);
+    return hipGetLastError();
+}
+
 hipError_t launch_expand_runs(const WalkDev *walks, int n_walk, const int2 *run_e, const int *run_eo,
                               const int2 *run_s, const int *run_so, int *out_e, int *out_s, hipStream_t st)
 {
@@ -1084,11 +1137,11 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
         if (b.n_walk <= 0) return hipSuccess;
         if (b.nnb_le)
             hipLaunchKernelGGL(k_force_plain<1>, dim3(b.n_walk), dim3(256), 0, st, b.walks, b.near_blk, b.staged_i,
-                               b.staged_e, b.staged_s, b.es_off, b.ss_off, b.eps2, b.rout2, b.G,
+                               b.staged_e, b.staged_s, b.spj_tab, b.es_off, b.ss_off, b.eps2, b.rout2, b.G,
                                b.out_accp, b.out_nnb);
         else
             hipLaunchKernelGGL(k_force_plain<0>, dim3(b.n_walk), dim3(256), 0, st, b.walks, b.near_blk, b.staged_i,
-                               b.staged_e, b.staged_s, b.es_off, b.ss_off, b.eps2, b.rout2, b.G,
+                               b.staged_e, b.staged_s, b.spj_tab, b.es_off, b.ss_off, b.eps2, b.rout2, b.G,
                                b.out_accp, b.out_nnb);
         return hipGetLastError();
     }
@@ -1098,6 +1151,7 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
     a.staged_i = b.staged_i;
     a.staged_e = b.staged_e;
     a.staged_s = b.staged_s;
+    a.spj_tab = b.spj_tab;
     a.staged_ic = b.staged_ic;
     a.staged_il = b.staged_il;
     a.staged_ec = b.staged_ec;

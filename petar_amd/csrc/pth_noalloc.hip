@@ -197,6 +197,7 @@ extern "C" int pth_resident_eval_noalloc(pth_ctx *c, int n_walk, const int *n_ep
     const size_t o_arena = cv.take(A.total);
     const size_t o_epj = cv.take(sizeof(EpjDev) * nj1);
     const size_t o_spj = cv.take((mono ? sizeof(EpjDev) : sizeof(SpjDev)) * ns1);
+    const size_t o_tab = cv.take(mono ? 0 : sizeof(SpjTab) * ns1);
     const size_t o_accp = cv.take(sizeof(double4) * ni1);
     const size_t o_nnb = cv.take(sizeof(int) * ni1);
     const size_t persist = cv.off;
@@ -214,7 +215,7 @@ extern "C" int pth_resident_eval_noalloc(pth_ctx *c, int n_walk, const int *n_ep
     const size_t o_nearb = r2.take(sizeof(int) * std::max<long long>(nw, 1));
     const size_t o_si = r2.take(sizeof(float4) * ni1);
     const size_t o_se = r2.take(sizeof(float) * 5 * slots);
-    const size_t o_ss = r2.take(sizeof(float) * 12 * recs);
+    const size_t o_ss = r2.take(sizeof(float) * PTH_SPJ_STAGED * recs);
     const size_t o_pa = r2.take(sizeof(double4) * 64 * parts);
     const size_t o_pn = r2.take(sizeof(int) * 64 * parts);
     const size_t o_il = r2.take(df ? sizeof(float4) * ni1 : 0);
@@ -265,6 +266,8 @@ extern "C" int pth_resident_eval_noalloc(pth_ctx *c, int n_walk, const int *n_ep
                                (const double *)(base + o_sq), (SpjDev *)(base + o_spj));
     }
     HIPCHK(c, hipGetLastError(), "packing kernels");
+    if (!mono) HIPCHK(c, launch_spj_table((const SpjDev *)(base + o_spj), n_spj_tot, (float)c->par.eps2,
+                                          (SpjTab *)(base + o_tab), st), "SPJ table kernel");
 
     // the launch descriptor over scratch ranges (DevBuf views: never grown
     // or released)
@@ -285,7 +288,8 @@ extern "C" int pth_resident_eval_noalloc(pth_ctx *c, int n_walk, const int *n_ep
     }
     BatchDev &b = c->na_batch;
     fill_batch(b, P, c->par, arena, A, Wk, (const EpjDev *)(base + o_epj), (const SpjDev *)(base + o_spj),
-               (double4 *)(base + o_accp), (int *)(base + o_nnb), nullptr, nullptr);
+               (double4 *)(base + o_accp), (int *)(base + o_nnb), nullptr, nullptr,
+               mono ? nullptr : (const SpjTab *)(base + o_tab));
     HIPCHK(c, launch_all(b, st), "evaluation kernels");
     if (ni > 0) {
         hipLaunchKernelGGL(k_unpack_out, dim3(grid(ni)), dim3(256), 0, st, ni, (const double4 *)(base + o_accp),

@@ -98,7 +98,7 @@ struct Layout {
 // a stream's scratch range.
 struct Scratch {
     size_t centre = 0, near_blk = 0, staged_i = 0, staged_e = 0, staged_s = 0, part_accp = 0, part_nnb = 0,
-           staged_il = 0, staged_el = 0, total = 0;
+           staged_il = 0, staged_el = 0, spj_tab = 0, total = 0;
 };
 
 // Host records of the plain mode (FDPS-like array-of-structs).
@@ -223,7 +223,8 @@ void layout(SCluster &c, long long n_ide, long long n_ids, bool df)
     S.near_blk = o; o = align256(o + sizeof(int) * nw);
     S.staged_i = o; o = align256(o + sizeof(float4) * ni);
     S.staged_e = o; o = align256(o + sizeof(float) * 5 * (size_t)std::max<long long>(P.n_e_slots, 8));
-    S.staged_s = o; o = align256(o + sizeof(float) * 12 * (size_t)std::max<long long>(P.n_s_rec, 1));
+    S.staged_s = o; o = align256(o + sizeof(float) * PTH_SPJ_STAGED * (size_t)std::max<long long>(P.n_s_rec, 1));
+    S.spj_tab = o; o = align256(o + sizeof(SpjTab) * (size_t)std::max(c.n_spj_tot, 1));
     S.part_accp = o; o = align256(o + sizeof(double4) * 64 * np);
     S.part_nnb = o; o = align256(o + sizeof(int) * 64 * np);
     if (df) {
@@ -264,6 +265,7 @@ void bind(SCluster &c, BatchDev &b, char *base, char *sc, const pth_params &par)
     b.ritems = reinterpret_cast<const RItem *>(base + L.A.ritems);
     b.epj = reinterpret_cast<const EpjDev *>(base + L.epj);
     b.spj = reinterpret_cast<const SpjDev *>(base + L.spj);
+    b.spj_tab = reinterpret_cast<const SpjTab *>(sc + S.spj_tab);
     b.out_accp = reinterpret_cast<double4 *>(base + L.out_accp);
     b.out_nnb = reinterpret_cast<int *>(base + L.out_nnb);
     b.centre = reinterpret_cast<double4 *>(sc + S.centre);
@@ -379,7 +381,9 @@ bool admit(pth_server *s, int k, int k_slot, hipError_t &e, bool &admitted)
         bind(c, b, base, s->pool.as<char>() + q.sc_off, s->ctx->par);
         hipGraph_t graph = nullptr;
         if ((e = hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal))) return false;
-        const hipError_t el = launch_all(b, s->cap);
+        // the SPJ table in the stream's scratch, then the evaluation
+        hipError_t el = launch_spj_table(b.spj, c.n_spj_tot, b.eps2, const_cast<SpjTab *>(b.spj_tab), s->cap);
+        if (el == hipSuccess) el = launch_all(b, s->cap);
         e = hipStreamEndCapture(s->cap, &graph);
         if (el != hipSuccess) e = el;
         if (e) {
