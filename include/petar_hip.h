@@ -18,6 +18,10 @@
  *       the same operator() with send_flag == false
  *       (src/force_gpu_cuda.cu:621-699): one batch of walks with their
  *       index lists; returns before the kernels finish.
+ *   pth_dispatch_copies
+ *       CalcForceWithLinearCutoffCUDA::operator() of the build without
+ *       PARTICLE_SIMULATOR_GPU_MULIT_WALK_INDEX (src/force_gpu_cuda.cu:704-816):
+ *       a batch of walks with per-walk copies of their EPJ and SPJ records.
  *   pth_retrieve
  *       RetrieveForceCUDA (src/force_gpu_cuda.cu:831-880; declared at
  *       src/force_gpu_cuda.hpp:165-168): waits, then ASSIGNS acc, pot and
@@ -222,6 +226,17 @@ int pth_dispatch(pth_ctx *ctx, int tag, int n_walk, const void *const *epi, cons
 
 int pth_retrieve(pth_ctx *ctx, int tag, int n_walk, const int *ni, void *const *force,
                  const pth_force_view *fv);
+
+/* The non-index multi-walk functor CalcForceWithLinearCutoffCUDA::operator()
+ * (src/force_gpu_cuda.cu:704-816; declared at src/force_gpu_cuda.hpp:137-162;
+ * used by src/petar.hpp:901 when PARTICLE_SIMULATOR_GPU_MULIT_WALK_INDEX is
+ * not defined): one batch of walks, each with its own copies of its EPJ and
+ * SPJ records (epj[w][0 .. n_epj[w]), spj[w][0 .. n_spj[w])).  The records
+ * become the context's j arrays for this batch (no pth_send_j), then the
+ * batch runs as pth_dispatch runs it; pth_retrieve returns its results. */
+int pth_dispatch_copies(pth_ctx *ctx, int tag, int n_walk, const void *const *epi, const int *n_epi,
+                        const pth_epi_view *iv, const void *const *epj, const int *n_epj, const pth_epj_view *ev,
+                        const void *const *spj, const int *n_spj, const pth_spj_view *sv);
 
 /* ---- device-resident evaluation (inputs kept in HBM across calls) --------- */
 typedef struct pth_resident pth_resident;

@@ -1,8 +1,8 @@
 // petar_hip_fdps.hpp -- header-only drop-in for PeTar's GPU soft-force hooks.
 //
-// Replaces src/force_gpu_cuda.hpp (declarations at :101-133 and :165-168)
-// for the multi-walk-index build (PARTICLE_SIMULATOR_GPU_MULIT_WALK_INDEX,
-// always set by Makefile.in:175).  Include it where PeTar includes
+// Replaces src/force_gpu_cuda.hpp (declarations at :101-133, :137-162 and
+// :165-168): the multi-walk-index functor (PARTICLE_SIMULATOR_GPU_MULIT_WALK_INDEX,
+// always set by Makefile.in:175) and the functor of a build without it.  Include it where PeTar includes
 // force_gpu_cuda.hpp, after the FDPS header and soft_ptcl.hpp (it needs
 // PS::S32/F64, EPISoft, EPJSoft, ForceSoft and PS::SPJQuadrupoleInAndOut),
 // and link libpetar_hip.so.  The call at src/petar.hpp:894-899,
@@ -295,6 +295,43 @@ struct CalcForceWithLinearCutoffHIPMultiWalk {
     }
 };
 
+// The dispatch functor of a build without PARTICLE_SIMULATOR_GPU_MULIT_WALK_INDEX
+// (src/force_gpu_cuda.hpp:137-162, called by FDPS's
+// calcForceAllAndWriteBackMultiWalk, src/petar.hpp:901): each walk comes
+// with its own copies of its EPJ and SPJ records.
+struct CalcForceWithLinearCutoffHIP {
+    PS::S32 my_rank;
+    PS::F64 eps2;
+    PS::F64 rcut2;
+    PS::F64 G;
+
+    CalcForceWithLinearCutoffHIP() {}
+    CalcForceWithLinearCutoffHIP(PS::S32 _rank, PS::F64 _eps2, PS::F64 _rcut2, PS::F64 _G)
+        : my_rank(_rank), eps2(_eps2), rcut2(_rcut2), G(_G) {}
+
+    void initialize(PS::S32 _rank, PS::F64 _eps2, PS::F64 _rcut2, PS::F64 _G)
+    {
+        my_rank = _rank;
+        eps2 = _eps2;
+        rcut2 = _rcut2;
+        G = _G;
+    }
+
+    PS::S32 operator()(const PS::S32 tag, const PS::S32 n_walk, const EPISoft *epi[], const PS::S32 n_epi[],
+                       const EPJSoft *epj[], const PS::S32 n_epj[], const SPJSoft *spj[], const PS::S32 n_spj[])
+    {
+        pth_ctx *c = petar_hip::context(my_rank, eps2, rcut2, G);
+        const pth_epi_view iv = petar_hip::epi_view();
+        const pth_epj_view ev = petar_hip::epj_view();
+        const pth_spj_view sv = petar_hip::spj_view();
+        if (pth_dispatch_copies(c, tag, n_walk, reinterpret_cast<const void *const *>(epi), n_epi, &iv,
+                                reinterpret_cast<const void *const *>(epj), n_epj, &ev,
+                                reinterpret_cast<const void *const *>(spj), n_spj, &sv) != PTH_OK)
+            petar_hip::fail("pth_dispatch_copies", c);
+        return 0;
+    }
+};
+
 // The retrieve function (src/force_gpu_cuda.hpp:165-168).
 inline PS::S32 RetrieveForceHIP(const PS::S32 tag, const PS::S32 n_walk, const PS::S32 *ni,
                                 ForceSoft **force)
@@ -311,8 +348,9 @@ inline PS::S32 RetrieveForceHIP(const PS::S32 tag, const PS::S32 n_walk, const P
 }
 
 #ifndef PETAR_HIP_NO_REFERENCE_NAMES
-// The names src/petar.hpp:894-899 uses.
+// The names src/petar.hpp:894-904 uses.
 typedef CalcForceWithLinearCutoffHIPMultiWalk CalcForceWithLinearCutoffCUDAMultiWalk;
+typedef CalcForceWithLinearCutoffHIP CalcForceWithLinearCutoffCUDA;
 inline PS::S32 RetrieveForceCUDA(const PS::S32 tag, const PS::S32 n_walk, const PS::S32 *ni,
                                  ForceSoft **force)
 {

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -198,25 +199,81 @@ void pth_destroy(pth_ctx *c)
     }
     for (hipEvent_t &x : c->prof_send)
         if (x) (void)hipEventDestroy(x);
+    c->d_ids.release();
+    c->d_spj_tab.release();
     if (c->ev_send) (void)hipEventDestroy(c->ev_send);
     if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
-// ---- send_flag == true: src/force_gpu_cuda.cu:577-620 -----------------------
-int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *ev, const void *spj,
-               int n_spj_tot, const pth_spj_view *sv)
+}  // extern "C"
+
+namespace {
+
+// One EPJ, monopole-superparticle or quadrupole-superparticle record from
+// the caller's struct through its view.
+inline EpjDev epj_record(const char *r, const pth_epj_view *ev)
 {
-    if (!c) return PTH_ERR_ARG;
-    if (n_epj_tot < 0 || n_spj_tot < 0) return c->fail(PTH_ERR_ARG, "pth_send_j: negative count");
-    if ((n_epj_tot > 0 && (!epj || !ev)) || (n_spj_tot > 0 && (!spj || !sv)))
-        return c->fail(PTH_ERR_ARG, "pth_send_j: null records or view");
+    const double *p = reinterpret_cast<const double *>(r + ev->off_pos);
+    EpjDev d;
+    d.x = p[0];
+    d.y = p[1];
+    d.z = p[2];
+    d.m = (float)*reinterpret_cast<const double *>(r + ev->off_mass);
+    d.rs = (float)*reinterpret_cast<const double *>(r + ev->off_rsearch);
+    return d;
+}
+inline EpjDev spj_mono_record(const char *r, const pth_spj_view *sv)
+{
+    const double *p = reinterpret_cast<const double *>(r + sv->off_pos);
+    EpjDev d;
+    d.x = p[0];
+    d.y = p[1];
+    d.z = p[2];
+    d.m = (float)*reinterpret_cast<const double *>(r + sv->off_mass);
+    d.rs = 0.f;
+    return d;
+}
+inline SpjDev spj_record(const char *r, const pth_spj_view *sv)
+{
+    const double *p = reinterpret_cast<const double *>(r + sv->off_pos);
+    SpjDev d;
+    d.x = p[0];
+    d.y = p[1];
+    d.z = p[2];
+    d.m = (float)*reinterpret_cast<const double *>(r + sv->off_mass);
+    d.qxx = (float)*reinterpret_cast<const double *>(r + sv->off_qxx);
+    d.qyy = (float)*reinterpret_cast<const double *>(r + sv->off_qyy);
+    d.qzz = (float)*reinterpret_cast<const double *>(r + sv->off_qzz);
+    d.qxy = (float)*reinterpret_cast<const double *>(r + sv->off_qxy);
+    d.qxz = (float)*reinterpret_cast<const double *>(r + sv->off_qxz);
+    d.qyz = (float)*reinterpret_cast<const double *>(r + sv->off_qyz);
+    d.pad[0] = d.pad[1] = d.pad[2] = 0.f;
+    return d;
+}
+inline EpjCoDev epj_co_record(const char *r, const pth_epj_view *ev)
+{
+    return co_record(*reinterpret_cast<const double *>(r + ev->off_rin),
+                     *reinterpret_cast<const double *>(r + ev->off_rout),
+                     *reinterpret_cast<const long long *>(r + ev->off_id),
+                     ev->off_group >= 0 ? reinterpret_cast<const double *>(r + ev->off_group) : nullptr);
+}
+
+// The j arrays of a force call: waits for the previous call's use of the
+// buffers, sizes them, packs record k (k < n_epj_tot, s < n_spj_tot) with
+// rec_e(k) / rec_s(s) giving the caller's record, uploads them and marks
+// them as sent.
+template <class RE, class RS>
+int send_records(pth_ctx *c, int n_epj_tot, const pth_epj_view *ev, int n_spj_tot, const pth_spj_view *sv,
+                 int n_walk, RE rec_e, RS rec_s)
+{
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     // the pinned staging and the resident arrays may still be in use by the
     // previous force call's copies and kernels
     HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     const bool mono = c->par.spj_monopole != 0;
+    const bool co = c->par.changeover != 0;
     const size_t be = sizeof(EpjDev) * (size_t)std::max(n_epj_tot, 1);
     // monopole superparticles are kept as EPJ-like records (centre, mass)
     const size_t bs = (mono ? sizeof(EpjDev) : sizeof(SpjDev)) * (size_t)std::max(n_spj_tot, 1);
@@ -224,74 +281,22 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
     HIPCHK(c, c->h_spj.ensure(bs), "pinned SPJ allocation");
     HIPCHK(c, c->d_epj.ensure(be), "device EPJ allocation");
     HIPCHK(c, c->d_spj.ensure(bs), "device SPJ allocation");
-    const double t_pack = wall_s();
-    EpjDev *he = c->h_epj.as<EpjDev>();
-    SpjDev *hs = c->h_spj.as<SpjDev>();
-    const char *eb = static_cast<const char *>(epj);
-    const char *sb = static_cast<const char *>(spj);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n_epj_tot; i++) {
-        const char *r = eb + (size_t)i * ev->stride;
-        const double *p = reinterpret_cast<const double *>(r + ev->off_pos);
-        EpjDev d;
-        d.x = p[0];
-        d.y = p[1];
-        d.z = p[2];
-        d.m = (float)*reinterpret_cast<const double *>(r + ev->off_mass);
-        d.rs = (float)*reinterpret_cast<const double *>(r + ev->off_rsearch);
-        he[i] = d;
-    }
-    if (mono) {
-        EpjDev *hm = c->h_spj.as<EpjDev>();
-#pragma omp parallel for schedule(static)
-        for (int i = 0; i < n_spj_tot; i++) {
-            const char *r = sb + (size_t)i * sv->stride;
-            const double *p = reinterpret_cast<const double *>(r + sv->off_pos);
-            EpjDev d;
-            d.x = p[0];
-            d.y = p[1];
-            d.z = p[2];
-            d.m = (float)*reinterpret_cast<const double *>(r + sv->off_mass);
-            d.rs = 0.f;
-            hm[i] = d;
-        }
-    }
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < (mono ? 0 : n_spj_tot); i++) {
-        const char *r = sb + (size_t)i * sv->stride;
-        const double *p = reinterpret_cast<const double *>(r + sv->off_pos);
-        SpjDev d;
-        d.x = p[0];
-        d.y = p[1];
-        d.z = p[2];
-        d.m = (float)*reinterpret_cast<const double *>(r + sv->off_mass);
-        d.qxx = (float)*reinterpret_cast<const double *>(r + sv->off_qxx);
-        d.qyy = (float)*reinterpret_cast<const double *>(r + sv->off_qyy);
-        d.qzz = (float)*reinterpret_cast<const double *>(r + sv->off_qzz);
-        d.qxy = (float)*reinterpret_cast<const double *>(r + sv->off_qxy);
-        d.qxz = (float)*reinterpret_cast<const double *>(r + sv->off_qxz);
-        d.qyz = (float)*reinterpret_cast<const double *>(r + sv->off_qyz);
-        d.pad[0] = d.pad[1] = d.pad[2] = 0.f;
-        hs[i] = d;
-    }
-    if (c->par.changeover) {
+    if (co) {
         const size_t bc = sizeof(EpjCoDev) * (size_t)std::max(n_epj_tot, 1);
         HIPCHK(c, c->h_epj_co.ensure(bc), "pinned EPJ allocation");
         HIPCHK(c, c->d_epj_co.ensure(bc), "device EPJ allocation");
-        EpjCoDev *hc = c->h_epj_co.as<EpjCoDev>();
-#pragma omp parallel for schedule(static)
-        for (int i = 0; i < n_epj_tot; i++) {
-            const char *r = eb + (size_t)i * ev->stride;
-            hc[i] = co_record(*reinterpret_cast<const double *>(r + ev->off_rin),
-                              *reinterpret_cast<const double *>(r + ev->off_rout),
-                              *reinterpret_cast<const long long *>(r + ev->off_id),
-                              ev->off_group >= 0 ? reinterpret_cast<const double *>(r + ev->off_group) : nullptr);
-        }
-        if (n_epj_tot > 0)
-            HIPCHK(c, hipMemcpyAsync(c->d_epj_co.p, hc, sizeof(EpjCoDev) * n_epj_tot, hipMemcpyHostToDevice,
-                                     c->stream), "EPJ upload");
     }
-    c->co_sent = c->par.changeover != 0;
+    const double t_pack = wall_s();
+    EpjDev *he = c->h_epj.as<EpjDev>();
+    EpjCoDev *hc = c->h_epj_co.as<EpjCoDev>();
+    rec_e(he, co ? hc : nullptr, ev);
+    if (mono) rec_s(c->h_spj.as<EpjDev>(), (SpjDev *)nullptr, sv);
+    else rec_s((EpjDev *)nullptr, c->h_spj.as<SpjDev>(), sv);
+    (void)n_walk;
+    if (co && n_epj_tot > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_epj_co.p, hc, sizeof(EpjCoDev) * n_epj_tot, hipMemcpyHostToDevice, c->stream),
+               "EPJ upload");
+    c->co_sent = co;
     c->mono_sent = mono;
     c->prof.copy_s += wall_s() - t_pack;
     if (c->prof_on) HIPCHK(c, hipEventRecord(c->prof_send[0], c->stream), "hipEventRecord");
@@ -312,6 +317,40 @@ int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *e
     c->j_sent = true;
     c->tab_valid = false;
     return PTH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- send_flag == true: src/force_gpu_cuda.cu:577-620 -----------------------
+int pth_send_j(pth_ctx *c, const void *epj, int n_epj_tot, const pth_epj_view *ev, const void *spj,
+               int n_spj_tot, const pth_spj_view *sv)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (n_epj_tot < 0 || n_spj_tot < 0) return c->fail(PTH_ERR_ARG, "pth_send_j: negative count");
+    if ((n_epj_tot > 0 && (!epj || !ev)) || (n_spj_tot > 0 && (!spj || !sv)))
+        return c->fail(PTH_ERR_ARG, "pth_send_j: null records or view");
+    const char *eb = static_cast<const char *>(epj);
+    const char *sb = static_cast<const char *>(spj);
+    return send_records(
+        c, n_epj_tot, ev, n_spj_tot, sv, 0,
+        [&](EpjDev *he, EpjCoDev *hc, const pth_epj_view *v) {
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < n_epj_tot; i++) {
+                const char *r = eb + (size_t)i * v->stride;
+                he[i] = epj_record(r, v);
+                if (hc) hc[i] = epj_co_record(r, v);
+            }
+        },
+        [&](EpjDev *hm, SpjDev *hs, const pth_spj_view *v) {
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < n_spj_tot; i++) {
+                const char *r = sb + (size_t)i * v->stride;
+                if (hm) hm[i] = spj_mono_record(r, v);
+                else hs[i] = spj_record(r, v);
+            }
+        });
 }
 
 // ---- send_flag == false: src/force_gpu_cuda.cu:621-699 -----------------------
@@ -363,28 +402,41 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     int bad = -1;
 #pragma omp parallel for schedule(dynamic)
     for (int w = 0; w < n_walk; w++) {
+        // runs without a branch on run starts: every entry writes the slot
+        // after the last run (at most one run per entry, so in bounds), and
+        // only a run start advances the count
         const WalkDev &W = P.walks[w];
-        bool ok = true;
+        int mn = 0, mx = -1;
         int2 *re = hr + W.e_off;
+        const int *le = id_epj[w];
         int nr = 0;
         long long next = -1;
         for (int j = 0; j < W.n_e; j++) {
-            const int id = id_epj[w][j];
-            ok &= (id >= 0 && id < nj_lim);
-            if (id != next) re[nr++] = make_int2(id, j);
+            const int id = le[j];
+            mn = std::min(mn, id);
+            mx = std::max(mx, id);
+            re[nr] = make_int2(id, j);
+            nr += (id != next);
             next = (long long)id + 1;
         }
         c->run_ne[w] = nr;
+        bool ok = mn >= 0 && mx < nj_lim;
+        mn = 0;
+        mx = -1;
         int2 *rs = hr + n_ide + W.s_off;
+        const int *ls = id_spj[w];
         nr = 0;
         next = -1;
         for (int j = 0; j < W.n_s; j++) {
-            const int id = id_spj[w][j];
-            ok &= (id >= 0 && id < ns_lim);
-            if (id != next) rs[nr++] = make_int2(id, j);
+            const int id = ls[j];
+            mn = std::min(mn, id);
+            mx = std::max(mx, id);
+            rs[nr] = make_int2(id, j);
+            nr += (id != next);
             next = (long long)id + 1;
         }
         c->run_ns[w] = nr;
+        ok = ok && mn >= 0 && mx < ns_lim;
         if (!ok) {
 #pragma omp atomic write
             bad = w;
@@ -521,7 +573,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     if (runs) {
         char *d = c->d_arena.as<char>();
         int *ide_dev = c->d_ids.as<int>();
-        HIPCHK(c, launch_expand_runs(b.walks, n_walk, reinterpret_cast<const int2 *>(d + A.run_e),
+        HIPCHK(c, launch_expand_runs(b.walks, 0, n_walk, reinterpret_cast<const int2 *>(d + A.run_e),
                                      reinterpret_cast<const int *>(d + A.run_eo),
                                      reinterpret_cast<const int2 *>(d + A.run_s),
                                      reinterpret_cast<const int *>(d + A.run_so), ide_dev, ide_dev + n_ide, c->stream),
@@ -556,6 +608,71 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     T.ni_tot = P.ni_tot;
     T.pending = true;
     return PTH_OK;
+}
+
+// ---- the non-index multi-walk functor: src/force_gpu_cuda.cu:704-816 ----------
+// The batch's own EPJ / SPJ records per walk become the j arrays of this
+// batch (packed and sent as pth_send_j sends a force call's), and the batch
+// is dispatched with each walk's list naming its own records in order (one
+// run of consecutive indices per walk).
+int pth_dispatch_copies(pth_ctx *c, int tag, int n_walk, const void *const *epi, const int *n_epi,
+                        const pth_epi_view *iv, const void *const *epj, const int *n_epj, const pth_epj_view *ev,
+                        const void *const *spj, const int *n_spj, const pth_spj_view *sv)
+{
+    if (!c) return PTH_ERR_ARG;
+    if (tag < 0 || tag >= MAX_TAG) return c->fail(PTH_ERR_ARG, "pth_dispatch_copies: tag out of range [0,8)");
+    if (n_walk < 0 || n_walk > c->par.max_walk)
+        return c->fail(PTH_ERR_CAPACITY, "pth_dispatch_copies: n_walk " + std::to_string(n_walk) +
+                                             " outside [0, " + std::to_string(c->par.max_walk) + "]");
+    if (n_walk > 0 && (!n_epi || !n_epj || !n_spj || !epj || !spj || !ev || !sv))
+        return c->fail(PTH_ERR_ARG, "pth_dispatch_copies: null argument");
+    if (c->tag[tag].pending)
+        return c->fail(PTH_ERR_STATE, "pth_dispatch_copies: previous batch of this tag not retrieved");
+    std::vector<long long> oe(n_walk + 1, 0), os(n_walk + 1, 0);
+    for (int w = 0; w < n_walk; w++) {
+        if (n_epj[w] < 0 || n_spj[w] < 0)
+            return c->fail(PTH_ERR_ARG, "pth_dispatch_copies: negative size in walk " + std::to_string(w));
+        if ((n_epj[w] > 0 && !epj[w]) || (n_spj[w] > 0 && !spj[w]))
+            return c->fail(PTH_ERR_ARG, "pth_dispatch_copies: null records in walk " + std::to_string(w));
+        oe[w + 1] = oe[w] + n_epj[w];
+        os[w + 1] = os[w] + n_spj[w];
+    }
+    if (oe[n_walk] > INT_MAX || os[n_walk] > INT_MAX)
+        return c->fail(PTH_ERR_CAPACITY, "pth_dispatch_copies: batch exceeds the int32 index range");
+    const int ne = (int)oe[n_walk], ns = (int)os[n_walk];
+    const int rc = send_records(
+        c, ne, ev, ns, sv, n_walk,
+        [&](EpjDev *he, EpjCoDev *hc, const pth_epj_view *v) {
+#pragma omp parallel for schedule(dynamic)
+            for (int w = 0; w < n_walk; w++) {
+                const char *b = static_cast<const char *>(epj[w]);
+                for (int j = 0; j < n_epj[w]; j++) {
+                    const char *r = b + (size_t)j * v->stride;
+                    he[oe[w] + j] = epj_record(r, v);
+                    if (hc) hc[oe[w] + j] = epj_co_record(r, v);
+                }
+            }
+        },
+        [&](EpjDev *hm, SpjDev *hs, const pth_spj_view *v) {
+#pragma omp parallel for schedule(dynamic)
+            for (int w = 0; w < n_walk; w++) {
+                const char *b = static_cast<const char *>(spj[w]);
+                for (int j = 0; j < n_spj[w]; j++) {
+                    const char *r = b + (size_t)j * v->stride;
+                    if (hm) hm[os[w] + j] = spj_mono_record(r, v);
+                    else hs[os[w] + j] = spj_record(r, v);
+                }
+            }
+        });
+    if (rc != PTH_OK) return rc;
+    const size_t n_id = (size_t)std::max(ne, ns);
+    for (size_t k = c->ident.size(); k < n_id; k++) c->ident.push_back((int)k);
+    std::vector<const int *> le(n_walk), ls(n_walk);
+    for (int w = 0; w < n_walk; w++) {
+        le[w] = c->ident.data() + oe[w];
+        ls[w] = c->ident.data() + os[w];
+    }
+    return pth_dispatch(c, tag, n_walk, epi, n_epi, iv, le.data(), n_epj, ls.data(), n_spj);
 }
 
 // ---- RetrieveForceCUDA: src/force_gpu_cuda.cu:831-880 -------------------------

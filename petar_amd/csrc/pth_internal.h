@@ -355,6 +355,7 @@ struct pth_ctx {
     // the work buffers)
     std::vector<int2> h_runs;
     std::vector<int> run_ne, run_ns;
+    std::vector<int> ident;  // 0, 1, 2, ...: the lists of pth_dispatch_copies
     DevBuf d_ids;
     // SPJ table of the sent superparticles, built at the first dispatch
     // after a send (again when eps2 changes)

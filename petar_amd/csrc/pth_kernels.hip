@@ -1033,14 +1033,14 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 // last run whose position is <= k (a binary search over the walk's runs,
 // which stay in the L1 cache).  The first run of a non-empty list has
 // position 0 and positions ascend.
-__global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__ walks, int n_walk,
+__global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__ walks, int w_begin, int n_walk,
                                                      const int2 *__restrict__ run_e, const int *__restrict__ run_eo,
                                                      const int2 *__restrict__ run_s, const int *__restrict__ run_so,
                                                      int *__restrict__ out_e, int *__restrict__ out_s)
 {
     const int g = (int)blockIdx.x;
     const bool sp = g >= n_walk;
-    const int w = sp ? g - n_walk : g;
+    const int w = w_begin + (sp ? g - n_walk : g);
     const WalkDev wk = walks[w];
     const int n = sp ? wk.n_s : wk.n_e;
     const int2 *__restrict__ runs = sp ? run_s : run_e;
@@ -1069,12 +1069,12 @@ hipError_t launch_spj_table(const SpjDev *spj, long long n, float eps2, SpjTab *
     return hipGetLastError();
 }
 
-hipError_t launch_expand_runs(const WalkDev *walks, int n_walk, const int2 *run_e, const int *run_eo,
+hipError_t launch_expand_runs(const WalkDev *walks, int w_begin, int n_walk, const int2 *run_e, const int *run_eo,
                               const int2 *run_s, const int *run_so, int *out_e, int *out_s, hipStream_t st)
 {
     if (n_walk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_expand_runs, dim3(2u * (unsigned)n_walk), dim3(256), 0, st, walks, n_walk, run_e, run_eo,
-                       run_s, run_so, out_e, out_s);
+    hipLaunchKernelGGL(k_expand_runs, dim3(2u * (unsigned)n_walk), dim3(256), 0, st, walks, w_begin, n_walk, run_e,
+                       run_eo, run_s, run_so, out_e, out_s);
     return hipGetLastError();
 }
 

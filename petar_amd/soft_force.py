@@ -95,7 +95,7 @@ EXPORTED = ("pth_params_default", "pth_create", "pth_set_params", "pth_get_param
             "pth_gtree_update", "pth_resident_refill", "pth_gtree_neighbors", "pth_gtree_neighbors_download",
             "pth_server_create", "pth_server_add_cluster", "pth_server_footprint", "pth_server_submit",
             "pth_server_wait", "pth_server_replay", "pth_server_results", "pth_server_stats", "pth_server_free",
-            "pth_profile_enable", "pth_profile_take", "pth_server_memory", "pth_resident_eval_noalloc")
+            "pth_profile_enable", "pth_profile_take", "pth_server_memory", "pth_resident_eval_noalloc", "pth_dispatch_copies")
 
 
 def lib():
@@ -124,6 +124,8 @@ def lib():
     L.pth_dispatch.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), ip, C.POINTER(EpiView), C.POINTER(vp), ip,
                                C.POINTER(vp), ip]
     L.pth_retrieve.argtypes = [vp, C.c_int, C.c_int, ip, C.POINTER(vp), C.POINTER(ForceView)]
+    L.pth_dispatch_copies.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp), ip, C.POINTER(EpiView), C.POINTER(vp), ip,
+                                      C.POINTER(EpjView), C.POINTER(vp), ip, C.POINTER(SpjView)]
     L.pth_resident_create.argtypes = [vp, C.POINTER(vp), C.c_int, ip, ip, ip, dp, dp, ip, ip, C.c_int, dp, dp, dp,
                                       C.c_int, dp, dp, dp]
     L.pth_resident_run.argtypes = [vp]
@@ -396,6 +398,29 @@ class CalcForceWithLinearCutoffMultiWalk:
         pf = (C.c_void_p * max(n_walk, 1))(*[f.ctypes.data for f in force[:n_walk]])
         n = _i32(ni)
         c.check(lib().pth_retrieve(c.handle, tag, n_walk, _ptr(n, C.c_int), pf, C.byref(fv)), "pth_retrieve")
+        return 0
+
+
+class CalcForceWithLinearCutoff(CalcForceWithLinearCutoffMultiWalk):
+    """The dispatch functor of a build without the multi-walk index
+    (src/force_gpu_cuda.hpp:137-162): each walk comes with its own EPJ and
+    SPJ record arrays, f(tag, n_walk, epi, n_epi, epj, n_epj, spj, n_spj),
+    over pth_dispatch_copies; retrieve as the multi-walk functor's.  Shares
+    the per-rank contexts of CalcForceWithLinearCutoffMultiWalk."""
+
+    def __call__(self, tag, n_walk, epi, n_epi, epj, n_epj, spj, n_spj):
+        c = self.context()
+        ev, sv, iv, _ = _views(bool(self.spj_monopole))
+        epi = [np.ascontiguousarray(e, dtype=EPI_SOFT) for e in epi[:n_walk]]
+        epj = [np.ascontiguousarray(e, dtype=EPJ_SOFT) for e in epj[:n_walk]]
+        spj = [np.ascontiguousarray(e, dtype=SPJ_SOFT_MONO if self.spj_monopole else SPJ_SOFT) for e in spj[:n_walk]]
+        pe = (C.c_void_p * max(n_walk, 1))(*[e.ctypes.data for e in epi])
+        pj = (C.c_void_p * max(n_walk, 1))(*[e.ctypes.data for e in epj])
+        ps = (C.c_void_p * max(n_walk, 1))(*[e.ctypes.data for e in spj])
+        ne, nj, ns = _i32(n_epi), _i32(n_epj), _i32(n_spj)
+        c.check(lib().pth_dispatch_copies(c.handle, tag, n_walk, pe, _ptr(ne, C.c_int), C.byref(iv), pj,
+                                          _ptr(nj, C.c_int), C.byref(ev), ps, _ptr(ns, C.c_int), C.byref(sv)),
+                "pth_dispatch_copies")
         return 0
 
 

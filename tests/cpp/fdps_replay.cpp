@@ -169,6 +169,34 @@ int main(int argc, char **argv)
 
     // FDPS multi-walk call sequence
     const PS::F64 eps2 = 0.0, rout2 = r_out * r_out, G = 1.0;
+#ifdef PETAR_HIP_REPLAY_COPIES
+    // a build without PARTICLE_SIMULATOR_GPU_MULIT_WALK_INDEX: FDPS hands each
+    // walk copies of its EPJ and SPJ records (calcForceAllAndWriteBackMultiWalk)
+    std::vector<std::vector<EPJSoft>> wepj(n_group);
+    std::vector<std::vector<SPJSoft>> wspj(n_group);
+    for (long g = 0; g < n_group; g++) {
+        for (int j = 0; j < n_epj[g]; j++) wepj[g].push_back(epj[id_epj[eoff[g] + j]]);
+        for (int j = 0; j < n_spj[g]; j++) wspj[g].push_back(spj[id_spj[soff[g] + j]]);
+    }
+    auto sequence = [&]() {
+        for (long g0 = 0; g0 < n_group; g0 += n_walk_limit) {
+            const int nw = (int)std::min<long>(n_walk_limit, n_group - g0);
+            std::vector<const EPISoft *> pe(nw);
+            std::vector<const EPJSoft *> pj(nw);
+            std::vector<const SPJSoft *> ps(nw);
+            std::vector<ForceSoft *> pf(nw);
+            for (int w = 0; w < nw; w++) {
+                pe[w] = epi[g0 + w].data();
+                pj[w] = wepj[g0 + w].data();
+                ps[w] = wspj[g0 + w].data();
+                pf[w] = force[g0 + w].data();
+            }
+            CalcForceWithLinearCutoffCUDA fw(0, eps2, rout2, G);
+            fw(0, nw, pe.data(), n_epi.data() + g0, pj.data(), n_epj.data() + g0, ps.data(), n_spj.data() + g0);
+            RetrieveForceCUDA(0, nw, n_epi.data() + g0, pf.data());
+        }
+    };
+#else
     auto sequence = [&]() {
         CalcForceWithLinearCutoffCUDAMultiWalk f(0, eps2, rout2, G);
         f(0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, epj.data(), (PS::S32)n, spj.data(),
@@ -190,6 +218,7 @@ int main(int argc, char **argv)
             RetrieveForceCUDA(0, nw, n_epi.data() + g0, pf.data());
         }
     };
+#endif
     sequence();
     double inter = 0.0;
     for (long g = 0; g < n_group; g++) inter += (double)n_epi[g] * ((double)n_epj[g] + (double)n_spj[g]);

@@ -35,13 +35,17 @@ def _read(path):
 
 
 @pytest.mark.parametrize("exe,n,walk_limit", [("fdps_replay", 20000, 200), ("fdps_replay", 3000, 7),
-                                              ("fdps_replay_mono", 20000, 200), ("fdps_replay_chunk", 20000, 200)])
+                                              ("fdps_replay_mono", 20000, 200), ("fdps_replay_chunk", 20000, 200),
+                                              ("fdps_replay_copies", 20000, 200), ("fdps_replay_copies", 3000, 7)])
 def test_cpp_dropin_fdps_replay(tmp_path, exe, n, walk_limit):
     """The adapter as a default PeTar GPU build compiles it (USE_QUAD), as a
     build configured with --disable-quad compiles it (monopole
     superparticles: the oracle's monopole pair, src/soft_force.hpp:125-158),
-    and with -DPETAR_HIP_CHUNK_J=256 (work items of at most 256 list
-    entries: long lists split over several items and summed by k_reduce)."""
+    with -DPETAR_HIP_CHUNK_J=256 (work items of at most 256 list entries:
+    long lists split over several items and summed by k_reduce), and as a
+    build without PARTICLE_SIMULATOR_GPU_MULIT_WALK_INDEX, whose functor
+    (CalcForceWithLinearCutoffCUDA, src/force_gpu_cuda.hpp:137-162) receives
+    per-walk copies of the j records (fdps_replay_copies)."""
     path = os.path.join(ROOT, "tests", "cpp", exe)
     if not os.path.exists(path):
         pytest.fail(f"{path} not built (build() compiles it)")
@@ -83,13 +87,15 @@ def test_cpp_dropin_fdps_replay(tmp_path, exe, n, walk_limit):
     assert bad_exact == 0 and bad_band == 0
 
 
-def test_cpp_dropin_changeover_mode(tmp_path):
+@pytest.mark.parametrize("exe", ["fdps_replay_co", "fdps_replay_copies_co"])
+def test_cpp_dropin_changeover_mode(tmp_path, exe):
     """The adapter compiled with PETAR_HIP_CHANGEOVER (ForceSoft with
     pot_tot, mass-scaled changeover radii, EPJSoft group data with group
     members, members of groups without artificial particles and artificial
     particles) against the FP64 changeover oracle on the lists and group
-    data the program used."""
-    exe = EXE + "_co"
+    data the program used; with the multi-walk-index functor and with the
+    functor that receives per-walk copies of the j records."""
+    exe = os.path.join(ROOT, "tests", "cpp", exe)
     if not os.path.exists(exe):
         pytest.fail(f"{exe} not built (build() compiles it)")
     out = tmp_path / "replay_co.bin"

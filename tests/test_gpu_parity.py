@@ -527,3 +527,40 @@ def test_graph_replay_after_refill():
         outs.append(res)
     for k in range(2):
         assert all(np.array_equal(x, y) for x, y in zip(outs[0][k], outs[1][k])), f"tree {k}"
+
+
+@pytest.mark.parametrize("exec_mode", [sf.EXEC_FUSED, sf.EXEC_PLAIN])
+def test_fdps_copies_functor(c1, exec_mode):
+    """The functor of a build without the multi-walk index
+    (CalcForceWithLinearCutoffCUDA, src/force_gpu_cuda.hpp:137-162;
+    pth_dispatch_copies): every batch hands each walk copies of its EPJ and
+    SPJ records.  Against the FP64 oracle, and bit-identical to the
+    multi-walk-index path and the resident path (the same lists in the same
+    order)."""
+    L, a, ref, amb = c1
+    epj_all, spj_all = _epj_records(L), _spj_records(L)
+    eoff, soff = L.walk_offsets()
+    fn = sf.CalcForceWithLinearCutoff(0, 0.0, R_OUT ** 2, 1.0, exec_mode=exec_mode)
+    out = []
+    for b0 in range(0, L.n_walk, 200):
+        ws = list(range(b0, min(b0 + 200, L.n_walk)))
+        epi, force, epj, spj = [], [], [], []
+        for w in ws:
+            e = np.zeros(L.n_epi[w], sf.EPI_SOFT)
+            sl = slice(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w])
+            e["pos"] = L.epj_pos[sl]
+            e["r_search"] = L.epj_rs[sl]
+            epi.append(e)
+            f = np.zeros(L.n_epi[w], sf.FORCE_SOFT)
+            f["pot"] = 123.0
+            force.append(f)
+            epj.append(epj_all[L.id_epj[eoff[w]:eoff[w + 1]]])
+            spj.append(spj_all[L.id_spj[soff[w]:soff[w + 1]]])
+        fn(0, len(ws), epi, L.n_epi[ws], epj, L.n_epj[ws], spj, L.n_spj[ws])
+        sf.RetrieveForce(0, len(ws), L.n_epi[ws], force, fn)
+        out.extend(force)
+    f = np.concatenate(out)
+    res = (f["acc"], f["pot"], f["n_ngb"])
+    _check(res, ref, amb, f"copies functor exec={exec_mode}")
+    idx_res = _fdps_replay(L, sf.make_functor(0, 0.0, R_OUT ** 2, 1.0, exec_mode=exec_mode))
+    assert all(np.array_equal(x, y) for x, y in zip(res, idx_res))
