@@ -119,6 +119,11 @@ typedef struct pth_params {
                       (all segments and streams) once as a hipGraph and
                       replays it on later runs; 0 launches them one by one.
                       Results do not depend on it. */
+    double min_seg_pairs; /* resident evaluations: fewest pair interactions a
+                      pipelined segment gets (0 = 1e9), so a small evaluation
+                      runs in fewer segments, down to one on one stream
+                      (launches and cross-stream events cost more there than
+                      the overlap gains).  Results do not depend on it. */
 } pth_params;
 
 /* Changeover mode folds PeTar's changeover correction of the soft force
@@ -143,7 +148,8 @@ typedef struct pth_params {
 
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
  * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99,
- * segments 8, streams 3, df_near 1, spj_monopole 0, graph 0. */
+ * segments 8, streams 3, df_near 1, spj_monopole 0, graph 0, min_seg_pairs 0
+ * (1e9). */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <immintrin.h>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -36,6 +37,7 @@ void pth_params_default(pth_params *p)
     p->df_near = 1;
     p->spj_monopole = 0;
     p->graph = 0;
+    p->min_seg_pairs = 0.0;
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -88,6 +90,7 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
     par.df_near = par.df_near ? 1 : 0;
     par.spj_monopole = par.spj_monopole ? 1 : 0;
     par.graph = par.graph ? 1 : 0;
+    par.min_seg_pairs = par.min_seg_pairs > 0 ? par.min_seg_pairs : 0.0;
     par.segments = std::min(std::max(par.segments, 1), PTH_MAX_SEGMENTS);
     par.streams = std::min(std::max(par.streams, 1), 8);
     int ndev = 0;
@@ -161,6 +164,7 @@ int pth_set_params(pth_ctx *c, const pth_params *p)
     c->par.df_near = p->df_near ? 1 : 0;
     c->par.spj_monopole = p->spj_monopole ? 1 : 0;
     c->par.graph = p->graph ? 1 : 0;
+    c->par.min_seg_pairs = p->min_seg_pairs > 0 ? p->min_seg_pairs : 0.0;
     return PTH_OK;
 }
 
@@ -210,6 +214,76 @@ void pth_destroy(pth_ctx *c)
 }  // extern "C"
 
 namespace {
+
+// Runs of consecutive indices of one list p[0 .. n): run k starts at
+// position rpos[k] with index rid[k]; returns the number of runs and the
+// smallest and largest index (for the range check).  rid / rpos have room
+// for n entries: every entry writes the slot after the last run (at most one
+// run per entry, so the writes stay inside), and only a run start advances
+// the count, so no branch depends on the data.
+int encode_runs_scalar(const int *p, int n, int *rid, int *rpos, int &mn, int &mx)
+{
+    int nr = 0;
+    mn = 0;
+    mx = -1;
+    long long next = -1;
+    for (int j = 0; j < n; j++) {
+        const int id = p[j];
+        mn = std::min(mn, id);
+        mx = std::max(mx, id);
+        rid[nr] = id;
+        rpos[nr] = j;
+        nr += (id != next);
+        next = (long long)id + 1;
+    }
+    return nr;
+}
+
+// The same with AVX-512: 16 entries at a time, run starts as a lane mask,
+// the starts compressed in registers and stored 16 lanes wide (nr <= j, so
+// the 16 lanes at nr stay inside the n entries while j + 16 <= n).
+__attribute__((target("avx512f"))) int encode_runs_avx512(const int *p, int n, int *rid, int *rpos, int &mn,
+                                                              int &mx)
+{
+    if (n < 16) return encode_runs_scalar(p, n, rid, rpos, mn, mx);
+    const __m512i one = _mm512_set1_epi32(1), step = _mm512_set1_epi32(16);
+    __m512i pos = _mm512_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15);
+    __m512i vmn = _mm512_set1_epi32(INT_MAX), vmx = _mm512_set1_epi32(INT_MIN);
+    int nr = 0;
+    // entry j - 1 of the first vector's lane 0: p[0] - 2, so lane 0 starts a run
+    __m512i prev = _mm512_alignr_epi32(_mm512_loadu_si512(p), _mm512_set1_epi32(p[0] - 2), 15);
+    int j = 0;
+    for (; j + 16 <= n; j += 16) {
+        const __m512i v = _mm512_loadu_si512(p + j);
+        if (j > 0) prev = _mm512_loadu_si512(p + j - 1);
+        const __mmask16 m = _mm512_cmpneq_epi32_mask(v, _mm512_add_epi32(prev, one));
+        _mm512_storeu_si512(rid + nr, _mm512_maskz_compress_epi32(m, v));
+        _mm512_storeu_si512(rpos + nr, _mm512_maskz_compress_epi32(m, pos));
+        nr += __builtin_popcount((unsigned)m);
+        vmn = _mm512_min_epi32(vmn, v);
+        vmx = _mm512_max_epi32(vmx, v);
+        pos = _mm512_add_epi32(pos, step);
+    }
+    mn = std::min(0, _mm512_reduce_min_epi32(vmn));
+    mx = _mm512_reduce_max_epi32(vmx);
+    long long next = (long long)p[j - 1] + 1;
+    for (; j < n; j++) {
+        const int id = p[j];
+        mn = std::min(mn, id);
+        mx = std::max(mx, id);
+        rid[nr] = id;
+        rpos[nr] = j;
+        nr += (id != next);
+        next = (long long)id + 1;
+    }
+    return nr;
+}
+
+int encode_runs(const int *p, int n, int *rid, int *rpos, int &mn, int &mx)
+{
+    static const bool avx512 = __builtin_cpu_supports("avx512f");
+    return avx512 ? encode_runs_avx512(p, n, rid, rpos, mn, mx) : encode_runs_scalar(p, n, rid, rpos, mn, mx);
+}
 
 // One EPJ, monopole-superparticle or quadrupole-superparticle record from
 // the caller's struct through its view.
@@ -395,47 +469,21 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     // overlaps the previous batch's upload
     double t_pack = wall_s();
     const int nj_lim = c->n_epj, ns_lim = c->n_spj;
-    if ((long long)c->h_runs.size() < n_ide + n_ids) c->h_runs.resize((size_t)(n_ide + n_ids));
+    if ((long long)c->h_rid.size() < n_ide + n_ids) {
+        c->h_rid.resize((size_t)(n_ide + n_ids));
+        c->h_rpos.resize((size_t)(n_ide + n_ids));
+    }
     c->run_ne.assign(n_walk + 1, 0);
     c->run_ns.assign(n_walk + 1, 0);
-    int2 *hr = c->h_runs.data();
+    int *hrid = c->h_rid.data(), *hrpos = c->h_rpos.data();
     int bad = -1;
 #pragma omp parallel for schedule(dynamic)
     for (int w = 0; w < n_walk; w++) {
-        // runs without a branch on run starts: every entry writes the slot
-        // after the last run (at most one run per entry, so in bounds), and
-        // only a run start advances the count
         const WalkDev &W = P.walks[w];
-        int mn = 0, mx = -1;
-        int2 *re = hr + W.e_off;
-        const int *le = id_epj[w];
-        int nr = 0;
-        long long next = -1;
-        for (int j = 0; j < W.n_e; j++) {
-            const int id = le[j];
-            mn = std::min(mn, id);
-            mx = std::max(mx, id);
-            re[nr] = make_int2(id, j);
-            nr += (id != next);
-            next = (long long)id + 1;
-        }
-        c->run_ne[w] = nr;
+        int mn, mx;
+        c->run_ne[w] = encode_runs(id_epj[w], W.n_e, hrid + W.e_off, hrpos + W.e_off, mn, mx);
         bool ok = mn >= 0 && mx < nj_lim;
-        mn = 0;
-        mx = -1;
-        int2 *rs = hr + n_ide + W.s_off;
-        const int *ls = id_spj[w];
-        nr = 0;
-        next = -1;
-        for (int j = 0; j < W.n_s; j++) {
-            const int id = ls[j];
-            mn = std::min(mn, id);
-            mx = std::max(mx, id);
-            rs[nr] = make_int2(id, j);
-            nr += (id != next);
-            next = (long long)id + 1;
-        }
-        c->run_ns[w] = nr;
+        c->run_ns[w] = encode_runs(id_spj[w], W.n_s, hrid + n_ide + W.s_off, hrpos + n_ide + W.s_off, mn, mx);
         ok = ok && mn >= 0 && mx < ns_lim;
         if (!ok) {
 #pragma omp atomic write
@@ -498,8 +546,9 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     long long *hid = reinterpret_cast<long long *>(h + A.epi_id);
     int *hide = reinterpret_cast<int *>(h + A.ide);
     int *hids = reinterpret_cast<int *>(h + A.ids);
-    int2 *hre = reinterpret_cast<int2 *>(h + A.run_e);
-    int2 *hrs = reinterpret_cast<int2 *>(h + A.run_s);
+    // runs: first indices of all of the batch's runs, then their positions
+    int *hre = reinterpret_cast<int *>(h + A.run_e);
+    int *hrs = reinterpret_cast<int *>(h + A.run_s);
     int *hreo = reinterpret_cast<int *>(h + A.run_eo);
     int *hrso = reinterpret_cast<int *>(h + A.run_so);
     if (runs) {
@@ -535,8 +584,14 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         }
         if (runs) {
             const int ne = c->run_ne[w + 1] - c->run_ne[w], ns = c->run_ns[w + 1] - c->run_ns[w];
-            if (ne) std::memcpy(hre + c->run_ne[w], hr + W.e_off, sizeof(int2) * ne);
-            if (ns) std::memcpy(hrs + c->run_ns[w], hr + n_ide + W.s_off, sizeof(int2) * ns);
+            if (ne) {
+                std::memcpy(hre + c->run_ne[w], hrid + W.e_off, sizeof(int) * ne);
+                std::memcpy(hre + n_re + c->run_ne[w], hrpos + W.e_off, sizeof(int) * ne);
+            }
+            if (ns) {
+                std::memcpy(hrs + c->run_ns[w], hrid + n_ide + W.s_off, sizeof(int) * ns);
+                std::memcpy(hrs + n_rs + c->run_ns[w], hrpos + n_ide + W.s_off, sizeof(int) * ns);
+            }
         } else {
             if (W.n_e) std::memcpy(hide + W.e_off, id_epj[w], sizeof(int) * W.n_e);
             if (W.n_s) std::memcpy(hids + W.s_off, id_spj[w], sizeof(int) * W.n_s);
@@ -573,10 +628,11 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     if (runs) {
         char *d = c->d_arena.as<char>();
         int *ide_dev = c->d_ids.as<int>();
-        HIPCHK(c, launch_expand_runs(b.walks, 0, n_walk, reinterpret_cast<const int2 *>(d + A.run_e),
-                                     reinterpret_cast<const int *>(d + A.run_eo),
-                                     reinterpret_cast<const int2 *>(d + A.run_s),
-                                     reinterpret_cast<const int *>(d + A.run_so), ide_dev, ide_dev + n_ide, c->stream),
+        const int *re = reinterpret_cast<const int *>(d + A.run_e);
+        const int *rs = reinterpret_cast<const int *>(d + A.run_s);
+        HIPCHK(c, launch_expand_runs(b.walks, 0, n_walk, re, re + n_re, reinterpret_cast<const int *>(d + A.run_eo),
+                                     rs, rs + n_rs, reinterpret_cast<const int *>(d + A.run_so), ide_dev,
+                                     ide_dev + n_ide, c->stream),
                "list expansion kernel");
         b.id_epj = ide_dev;
         b.id_spj = ide_dev + n_ide;
@@ -1071,7 +1127,8 @@ int pth_resident_replan(pth_resident *r, int max_ipl, int chunk_j)
         ns[w] = P0.walks[w].n_s;
     }
     Plan P;
-    if (!make_plan(nw, ne.data(), nj.data(), ns.data(), PTH_EXEC_FUSED, chunk_j, max_ipl, P, r->n_seg, P0.mono))
+    if (!make_plan(nw, ne.data(), nj.data(), ns.data(), PTH_EXEC_FUSED, chunk_j, max_ipl, P, r->n_seg, P0.mono,
+                   seg_pairs(c->par)))
         return c->fail(PTH_ERR_CAPACITY, "pth_resident_replan: sizes exceed the int32 index range");
     // the staged layout depends on the walk sizes only
     if (P.es_off != P0.es_off || P.ss_off != P0.ss_off || P.ni_tot != P0.ni_tot)
@@ -1230,7 +1287,7 @@ int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_e
         if (!pre_ok) return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
         std::swap(r->plan, *pre);
     } else if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, r->plan,
-                          r->n_seg, c->par.spj_monopole != 0))
+                          r->n_seg, c->par.spj_monopole != 0, seg_pairs(c->par)))
         return c->fail(PTH_ERR_CAPACITY, std::string(who) + ": sizes exceed the int32 index range");
     const Plan &P = r->plan;
     r->co = c->par.changeover != 0;

@@ -175,12 +175,13 @@ hipError_t launch_spj_table(const SpjDev *spj, long long n, float eps2, SpjTab *
 #define PTH_SPJ_STAGED 4
 
 // Expands index lists sent as runs of consecutive indices for walks
-// [w_begin, w_begin + n_walk): for walk w, runs[roff[w] .. roff[w + 1]) hold
-// {first index, position in the walk's list}; the list of walk w (n entries
-// from walks[w].e_off for EPJ, or walks[w].s_off for SPJ) goes to out_e /
-// out_s.
-hipError_t launch_expand_runs(const WalkDev *walks, int w_begin, int n_walk, const int2 *run_e, const int *run_eo,
-                              const int2 *run_s, const int *run_so, int *out_e, int *out_s, hipStream_t st);
+// [w_begin, w_begin + n_walk): for walk w, runs roff[w] .. roff[w + 1] - 1
+// start at position rpos[k] of the walk's list with index rid[k]; the list
+// of walk w (n entries from walks[w].e_off for EPJ, or walks[w].s_off for
+// SPJ) goes to out_e / out_s.
+hipError_t launch_expand_runs(const WalkDev *walks, int w_begin, int n_walk, const int *rid_e, const int *rpos_e,
+                              const int *run_eo, const int *rid_s, const int *rpos_s, const int *run_so, int *out_e,
+                              int *out_s, hipStream_t st);
 
 // Launches of segment k of a batch (k < b.n_seg).
 hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st);

@@ -1214,9 +1214,11 @@ int gtree_run(pth_gtree *t, long long n, const double *pos, const double *mass, 
         }
         const pth_params &par = ctx->par;
         t->pre_key = PlanKey{par.exec_mode, par.chunk_j, par.max_ipl,
-                             par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1, par.spj_monopole};
+                             par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1, par.spj_monopole,
+                             seg_pairs(par)};
         t->pre_ok = make_plan((int)ng, ni.data(), nje.data(), njs.data(), t->pre_key.exec_mode, t->pre_key.chunk_j,
-                              t->pre_key.max_ipl, t->pre_plan, t->pre_key.n_seg, t->pre_key.mono != 0);
+                              t->pre_key.max_ipl, t->pre_plan, t->pre_key.n_seg, t->pre_key.mono != 0,
+                              t->pre_key.min_seg_pairs);
         t->pre_valid = true;
     }
     if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return ctx->hip_fail(e, "tree walk");
@@ -1250,7 +1252,7 @@ int resident_fill_gtree(pth_resident *r, const pth_gtree *t, const char *who)
     // parameters (then taken over: a second refill plans again)
     const pth_params &par = c->par;
     const PlanKey key{par.exec_mode, par.chunk_j, par.max_ipl, par.exec_mode == PTH_EXEC_FUSED ? par.segments : 1,
-                      par.spj_monopole};
+                      par.spj_monopole, seg_pairs(par)};
     const bool take = t->pre_valid && t->pre_key == key;
     const int rc = resident_setup(r, (int)ng, ni.data(), ne.data(), ns.data(), t->n_le, t->n_ls, (int)t->n,
                                   (int)t->n_node, who, take ? &t->pre_plan : nullptr, t->pre_ok);

@@ -97,8 +97,9 @@ inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Layout of one upload arena (pinned host mirror and device copy).  The
 // index lists travel either as they are (ide, ids) or, with n_run >= 0, as
-// runs of consecutive indices (pth_dispatch): run_e / run_s hold {first
-// index, position in the walk's list} per run, run_eo / run_so the walks'
+// runs of consecutive indices (pth_dispatch): run_e / run_s hold the first
+// index of every run, then every run's position in its walk's list (two
+// int32 per run), run_eo / run_so the walks'
 // run offsets (n_walk + 1 each), and the lists are expanded on the device.
 struct Arena {
     size_t epi, epi_id, walks, es, ss, ide, ids, items, ritems, run_e, run_s, run_eo, run_so, total;
@@ -117,8 +118,8 @@ struct Arena {
         ids = o; o = align256(o + (runs ? 0 : sizeof(int) * (size_t)n_ids));
         items = o; o = align256(o + sizeof(Item) * P.items.size());
         ritems = o; o = align256(o + sizeof(RItem) * P.ritems.size());
-        run_e = o; o = align256(o + (runs ? sizeof(int2) * (size_t)n_run_e : 0));
-        run_s = o; o = align256(o + (runs ? sizeof(int2) * (size_t)n_run_s : 0));
+        run_e = o; o = align256(o + (runs ? 2 * sizeof(int) * (size_t)n_run_e : 0));
+        run_s = o; o = align256(o + (runs ? 2 * sizeof(int) * (size_t)n_run_s : 0));
         run_eo = o; o = align256(o + (runs ? sizeof(int) * (P.walks.size() + 1) : 0));
         run_so = o; o = align256(o + (runs ? sizeof(int) * (P.walks.size() + 1) : 0));
         total = o;
@@ -181,6 +182,12 @@ inline bool device_range(const void *p, size_t bytes)
         return false;
     }
     return (const char *)p + bytes <= (const char *)base + size;
+}
+
+// Fewest interactions per pipelined segment of a resident evaluation.
+inline long long seg_pairs(const pth_params &par)
+{
+    return par.min_seg_pairs > 0 ? (long long)par.min_seg_pairs : MIN_SEGMENT_PAIRS;
 }
 
 // Linear-cutoff mode with double-float near pairs (fused form; the
@@ -353,7 +360,7 @@ struct pth_ctx {
     // index lists as runs: host encoding space (reused), the walks' run
     // counts, and the device lists they expand into (shared by all tags, as
     // the work buffers)
-    std::vector<int2> h_runs;
+    std::vector<int> h_rid, h_rpos;
     std::vector<int> run_ne, run_ns;
     std::vector<int> ident;  // 0, 1, 2, ...: the lists of pth_dispatch_copies
     DevBuf d_ids;

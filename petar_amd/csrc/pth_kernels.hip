@@ -1034,8 +1034,9 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 // which stay in the L1 cache).  The first run of a non-empty list has
 // position 0 and positions ascend.
 __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__ walks, int w_begin, int n_walk,
-                                                     const int2 *__restrict__ run_e, const int *__restrict__ run_eo,
-                                                     const int2 *__restrict__ run_s, const int *__restrict__ run_so,
+                                                     const int *__restrict__ rid_e, const int *__restrict__ rpos_e,
+                                                     const int *__restrict__ run_eo, const int *__restrict__ rid_s,
+                                                     const int *__restrict__ rpos_s, const int *__restrict__ run_so,
                                                      int *__restrict__ out_e, int *__restrict__ out_s)
 {
     const int g = (int)blockIdx.x;
@@ -1043,7 +1044,8 @@ __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__
     const int w = w_begin + (sp ? g - n_walk : g);
     const WalkDev wk = walks[w];
     const int n = sp ? wk.n_s : wk.n_e;
-    const int2 *__restrict__ runs = sp ? run_s : run_e;
+    const int *__restrict__ rid = sp ? rid_s : rid_e;
+    const int *__restrict__ rpos = sp ? rpos_s : rpos_e;
     const int *__restrict__ ro = sp ? run_so : run_eo;
     const int r0 = ro[w], r1 = ro[w + 1];
     int *__restrict__ out = sp ? out_s + wk.s_off : out_e + wk.e_off;
@@ -1051,11 +1053,10 @@ __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__
         int lo = r0, hi = r1 - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (runs[mid].y <= k) lo = mid;
+            if (rpos[mid] <= k) lo = mid;
             else hi = mid - 1;
         }
-        const int2 r = runs[lo];
-        out[k] = r.x + (k - r.y);
+        out[k] = rid[lo] + (k - rpos[lo]);
     }
 }
 
@@ -1069,12 +1070,13 @@ hipError_t launch_spj_table(const SpjDev *spj, long long n, float eps2, SpjTab *
     return hipGetLastError();
 }
 
-hipError_t launch_expand_runs(const WalkDev *walks, int w_begin, int n_walk, const int2 *run_e, const int *run_eo,
-                              const int2 *run_s, const int *run_so, int *out_e, int *out_s, hipStream_t st)
+hipError_t launch_expand_runs(const WalkDev *walks, int w_begin, int n_walk, const int *rid_e, const int *rpos_e,
+                              const int *run_eo, const int *rid_s, const int *rpos_s, const int *run_so, int *out_e,
+                              int *out_s, hipStream_t st)
 {
     if (n_walk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_expand_runs, dim3(2u * (unsigned)n_walk), dim3(256), 0, st, walks, w_begin, n_walk, run_e,
-                       run_eo, run_s, run_so, out_e, out_s);
+    hipLaunchKernelGGL(k_expand_runs, dim3(2u * (unsigned)n_walk), dim3(256), 0, st, walks, w_begin, n_walk, rid_e,
+                       rpos_e, run_eo, rid_s, rpos_s, run_so, out_e, out_s);
     return hipGetLastError();
 }
 

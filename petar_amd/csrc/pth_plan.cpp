@@ -124,7 +124,7 @@ void tile_walk(int n, long long n_e, long long n_s, int max_ipl, std::vector<Til
 }  // namespace
 
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, int max_ipl, Plan &P, int n_seg, bool mono)
+               int chunk_j, int max_ipl, Plan &P, int n_seg, bool mono, long long min_seg_pairs)
 {
     // the item array keeps its size from the previous plan (every slot is
     // written below, so growing it is the only initialisation it needs)
@@ -278,6 +278,10 @@ bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj,
     if (n_seg < 1) n_seg = 1;
     if (n_seg > MAX_SEGMENTS) n_seg = MAX_SEGMENTS;
     if (n_seg > n_walk) n_seg = std::max(n_walk, 1);
+    // a pipeline pays for its launches and cross-stream events, so each
+    // segment gets at least min_seg_pairs interactions
+    if (min_seg_pairs > 0)
+        n_seg = (int)std::min<long long>(n_seg, std::max<long long>(1, (P.n_epep + P.n_epsp) / min_seg_pairs));
     for (int w = 0; w < n_walk; w++) wcost[w + 1] += wcost[w];
     // the first segment gets half the cost of the others: its staging is the
     // one part of a pipelined evaluation that nothing overlaps

@@ -34,6 +34,12 @@ struct PlanSeg {
     int ritem_begin = 0, ritem_count = 0;
 };
 constexpr int MAX_SEGMENTS = PTH_MAX_SEGMENTS;
+// Fewest pair interactions a pipelined segment of a resident evaluation is
+// given by default (smaller evaluations get fewer segments, down to one).
+// Measured: the 16,384-particle cluster (1.1e8 interactions) 0.18 ms per
+// evaluation on one stream against 0.58 ms over 8 segments on 3 streams; the
+// 1,048,576-particle one (1.6e10) 10.9 ms over 8 against 11.6 ms on one.
+constexpr long long MIN_SEGMENT_PAIRS = 1000000000ll;
 
 struct Plan {
     std::vector<WalkDev> walks;
@@ -65,13 +71,14 @@ struct Plan {
 };
 
 // The parameters a plan was made with (exec_mode, chunk_j, max_ipl, n_seg,
-// monopole superparticles).
+// monopole superparticles, fewest interactions per segment).
 struct PlanKey {
     int exec_mode = 0, chunk_j = 0, max_ipl = 0, n_seg = 1, mono = 0;
+    long long min_seg_pairs = 0;
     bool operator==(const PlanKey &o) const
     {
         return exec_mode == o.exec_mode && chunk_j == o.chunk_j && max_ipl == o.max_ipl && n_seg == o.n_seg &&
-               mono == o.mono;
+               mono == o.mono && min_seg_pairs == o.min_seg_pairs;
     }
 };
 
@@ -83,12 +90,13 @@ inline long long epj_region_slots(long long n_e) { return n_e > 0 ? (n_e + 7) / 
 // largest list length (EPJ entries + 2 x SPJ entries) of one item (<= 0:
 // default), max_ipl the largest number of i per lane (1 to 4), n_seg the
 // number of walk segments of about equal cost (fused form; the items of a
-// walk do not depend on it).  With mono (monopole superparticles) a walk's
+// walk do not depend on it; with min_seg_pairs > 0, at most one segment per
+// min_seg_pairs interactions).  With mono (monopole superparticles) a walk's
 // SPJ entries are staged as far EPJ entries after its EPJ list (a monopole
 // pair is a far EPJ pair: no cutoff, no neighbour count), so its staged EPJ
 // region holds n_epj + n_spj entries and it has no staged SPJ records.
 // Returns false if a size overflows the int32 ranges of the device records.
 bool make_plan(int n_walk, const int *n_epi, const int *n_epj, const int *n_spj, int exec_mode,
-               int chunk_j, int max_ipl, Plan &P, int n_seg = 1, bool mono = false);
+               int chunk_j, int max_ipl, Plan &P, int n_seg = 1, bool mono = false, long long min_seg_pairs = 0);
 
 }  // namespace pth

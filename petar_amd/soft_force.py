@@ -57,7 +57,8 @@ class Params(C.Structure):
     _fields_ = [("eps2", C.c_double), ("rcut2", C.c_double), ("G", C.c_double), ("nnb_mode", C.c_int),
                 ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int), ("max_ipl", C.c_int),
                 ("j_vgpr", C.c_int), ("changeover", C.c_int), ("ngb_factor", C.c_double), ("segments", C.c_int),
-                ("streams", C.c_int), ("df_near", C.c_int), ("spj_monopole", C.c_int), ("graph", C.c_int)]
+                ("streams", C.c_int), ("df_near", C.c_int), ("spj_monopole", C.c_int), ("graph", C.c_int),
+                ("min_seg_pairs", C.c_double)]
 
 
 class Profile(C.Structure):
@@ -189,7 +190,7 @@ class PthError(RuntimeError):
 
 def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=1000,
                    max_ipl=None, j_vgpr=None, changeover=0, ngb_factor=None, segments=None, streams=None,
-                   df_near=None, spj_monopole=0, graph=None):
+                   df_near=None, spj_monopole=0, graph=None, min_seg_pairs=None):
     p = Params()
     lib().pth_params_default(C.byref(p))
     p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk = (
@@ -210,6 +211,8 @@ def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED,
     p.spj_monopole = spj_monopole
     if graph is not None:
         p.graph = graph
+    if min_seg_pairs is not None:
+        p.min_seg_pairs = min_seg_pairs
     return p
 
 

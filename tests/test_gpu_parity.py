@@ -385,11 +385,13 @@ def test_theta0_is_direct_summation():
 @pytest.mark.parametrize("segments,streams", [(1, 1), (4, 2), (8, 4), (32, 8)])
 def test_segments_are_bit_identical(c1, segments, streams):
     """Walk segments run concurrently on several streams; every walk keeps
-    its items, so the results equal the one-stream evaluation bit for bit."""
+    its items, so the results equal the one-stream evaluation bit for bit
+    (min_seg_pairs = 1: this small evaluation is cut into every segment asked
+    for)."""
     L, a, ref, amb = c1
     out = []
     for seg, st in ((1, 1), (segments, streams)):
-        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=seg, streams=st)
+        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=seg, streams=st, min_seg_pairs=1)
         r = sf.Resident.from_lists(ctx, L)
         r.run()
         r.run()
@@ -469,11 +471,12 @@ def test_graph_replay_equals_launches(c1, segments, streams):
     replays, equals the launched evaluation bit for bit; a replan (new work
     items) is captured again and still equals the launched form."""
     L, a, ref, amb = c1
-    eager = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=segments, streams=streams)
+    eager = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=segments, streams=streams, min_seg_pairs=1)
     re = sf.Resident.from_lists(eager, L)
     re.run()
     want = re.results()
-    gctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=segments, streams=streams, graph=1)
+    gctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=segments, streams=streams, graph=1,
+                      min_seg_pairs=1)
     rg = sf.Resident.from_lists(gctx, L)
     for k in range(4):
         rg.run()

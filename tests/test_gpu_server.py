@@ -93,9 +93,13 @@ def test_fused_server_evicts_under_a_small_budget(clusters):
     scratch = probe.memory()["scratch_max"]
     alone = _alone(probe, ids)
     probe.close()
-    # two streams' scratch, the largest cluster and a few small ones
-    budget = int(2 * scratch + 1.3 * max(fp))
-    srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=budget, n_streams=2)
+    # one stream: once it has run the largest cluster its scratch is the
+    # largest, and the rest of the pool holds the largest cluster and little
+    # more, so every later admission evicts (evictions with several streams
+    # overlapping: the 256-cluster stream test)
+    assert sum(fp) > 1.05 * max(fp) + 4096
+    budget = int(scratch + 1.05 * max(fp) + 4096)
+    srv = sf.Server(ctx, sf.SERVE_FUSED, hbm_budget=budget, n_streams=1)
     ids2 = [srv.add_lists(L) for L, _ in clusters]
     rng = np.random.default_rng(3)
     seq = rng.permutation(np.repeat(ids2, 3))
@@ -112,7 +116,7 @@ def test_fused_server_evicts_under_a_small_budget(clusters):
     mem = srv.memory()
     print(f"[server budget] captures {mem['captures']} instantiations {mem['instantiations']}", flush=True)
     assert st["admitted"] > len(ids2)
-    assert mem["instantiations"] <= 2 * len(ids2) and mem["instantiations"] <= mem["captures"]
+    assert mem["instantiations"] <= len(ids2) and mem["instantiations"] <= mem["captures"]
     srv.close()
     ctx.close()
 

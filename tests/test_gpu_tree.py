@@ -467,7 +467,7 @@ def test_resident_from_a_tree_of_another_context():
     rs = np.full(m.size, 1.5 * R_OUT)
     L = host.build_lists(x, m, rs, theta=0.3)
     kw = dict(eps2=0.0, rcut2=R_OUT ** 2, G=1.0)
-    ca = sf.Context(0, **kw)
+    ca = sf.Context(0, min_seg_pairs=1, **kw)
     cb = sf.Context(0, segments=1, max_ipl=4, chunk_j=512, **kw)
     g = sf.GpuTree(ca, x, m, rs)
     outs = []
