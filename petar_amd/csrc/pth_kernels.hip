@@ -1029,16 +1029,19 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 
 // ---------------------------------------------------------------------------
 // index lists sent as runs (pth_dispatch): workgroup g expands the EPJ list
-// (g < n_walk) or the SPJ list (g >= n_walk) of one walk; entry k lies in the
-// last run whose position is <= k (a binary search over the walk's runs,
-// which stay in the L1 cache).  The first run of a non-empty list has
+// (g < n_walk) or the SPJ list (g >= n_walk) of one walk.  The walk's runs
+// are taken EXPAND_RUNS at a time into LDS (first index, position); each
+// entry of the positions those runs cover finds its run by a binary search
+// in LDS and is written coalesced.  The first run of a non-empty list has
 // position 0 and positions ascend.
+#define EXPAND_RUNS 1024
 __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__ walks, int w_begin, int n_walk,
                                                      const int *__restrict__ rid_e, const int *__restrict__ rpos_e,
                                                      const int *__restrict__ run_eo, const int *__restrict__ rid_s,
                                                      const int *__restrict__ rpos_s, const int *__restrict__ run_so,
                                                      int *__restrict__ out_e, int *__restrict__ out_s)
 {
+    __shared__ int l_id[EXPAND_RUNS], l_pos[EXPAND_RUNS];
     const int g = (int)blockIdx.x;
     const bool sp = g >= n_walk;
     const int w = w_begin + (sp ? g - n_walk : g);
@@ -1049,14 +1052,26 @@ __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__
     const int *__restrict__ ro = sp ? run_so : run_eo;
     const int r0 = ro[w], r1 = ro[w + 1];
     int *__restrict__ out = sp ? out_s + wk.s_off : out_e + wk.e_off;
-    for (int k = (int)threadIdx.x; k < n; k += 256) {
-        int lo = r0, hi = r1 - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (rpos[mid] <= k) lo = mid;
-            else hi = mid - 1;
+    const int tid = (int)threadIdx.x;
+    for (int c0 = r0; c0 < r1; c0 += EXPAND_RUNS) {
+        const int nc = min(EXPAND_RUNS, r1 - c0);
+        for (int k = tid; k < nc; k += 256) {
+            l_id[k] = rid[c0 + k];
+            l_pos[k] = rpos[c0 + k];
         }
-        out[k] = rid[lo] + (k - rpos[lo]);
+        __syncthreads();
+        // the entries these runs cover: up to the next chunk's first run
+        const int p0 = l_pos[0], p1 = (c0 + nc < r1) ? rpos[c0 + nc] : n;
+        for (int k = p0 + tid; k < p1; k += 256) {
+            int lo = 0, hi = nc - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (l_pos[mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
+            out[k] = l_id[lo] + (k - l_pos[lo]);
+        }
+        __syncthreads();
     }
 }
 
