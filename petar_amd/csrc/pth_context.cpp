@@ -108,6 +108,8 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
         // when that at least halves their bytes
         const char *lr = std::getenv("PTH_LIST_RUNS");
         c->list_runs = (lr && lr[0] == '0') ? 0 : (lr && lr[0] == '1') ? 1 : -1;
+        const char *cs = std::getenv("PTH_CLASS_STREAMS");
+        c->class_streams = !(cs && cs[0] == '0');
     }
     c->device = (rank >= 0 ? rank : -rank) % ndev;  // src/force_gpu_cuda.cu:550-553
     if ((e = hipSetDevice(c->device)) != hipSuccess ||
@@ -207,6 +209,12 @@ void pth_destroy(pth_ctx *c)
     c->d_spj_tab.release();
     if (c->ev_send) (void)hipEventDestroy(c->ev_send);
     if (c->ev_upload) (void)hipEventDestroy(c->ev_upload);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -637,7 +645,27 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         b.id_epj = ide_dev;
         b.id_spj = ide_dev + n_ide;
     }
-    HIPCHK(c, launch_all(b, c->stream), "soft-force kernels");
+    // a batch of up to a few hundred walks holds fewer waves of each
+    // i-per-lane class than the chip has slots: each class launch is one
+    // latency-bound round, so the 1-i items run on a second stream beside
+    // the longer classes instead of after them
+    const bool split = c->class_streams && !b.plain && P.seg_count[0] > 0 &&
+                       P.seg_count[1] + P.seg_count[2] + P.seg_count[3] > 0;
+    if (split) {
+        if (!c->side) HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "hipStreamCreate");
+        if (!c->ev_fork) HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
+        if (!c->ev_join) HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
+        HIPCHK(c, launch_stage(b, 0, c->stream), "staging kernel");
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream), "hipEventRecord");
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "hipStreamWaitEvent");
+        HIPCHK(c, launch_force(b, 0, c->stream, 0xEu), "soft-force kernels");
+        HIPCHK(c, launch_force(b, 0, c->side, 0x1u), "soft-force kernels");
+        HIPCHK(c, hipEventRecord(c->ev_join, c->side), "hipEventRecord");
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0), "hipStreamWaitEvent");
+        HIPCHK(c, launch_reduce(b, 0, c->stream), "reduction kernel");
+    } else {
+        HIPCHK(c, launch_all(b, c->stream), "soft-force kernels");
+    }
     if (T.timed) HIPCHK(c, hipEventRecord(T.prof[2], c->stream), "hipEventRecord");
     HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");

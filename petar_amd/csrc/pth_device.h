@@ -185,7 +185,8 @@ hipError_t launch_expand_runs(const WalkDev *walks, int w_begin, int n_walk, con
 
 // Launches of segment k of a batch (k < b.n_seg).
 hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st);
-hipError_t launch_force(const BatchDev &b, int k, hipStream_t st);
+// classes: bit c set launches the items of i-per-lane class c + 1
+hipError_t launch_force(const BatchDev &b, int k, hipStream_t st, unsigned classes = 0xFu);
 hipError_t launch_reduce(const BatchDev &b, int k, hipStream_t st);
 // All segments of a batch in order on one stream.
 hipError_t launch_all(const BatchDev &b, hipStream_t st);

@@ -370,6 +370,11 @@ struct pth_ctx {
     bool tab_valid = false;
     float tab_eps2 = 0.f;
     int list_runs = -1;  // PTH_LIST_RUNS: -1 per batch, 0 never, 1 always
+    // pth_dispatch runs the 1-i items on a second stream beside the longer
+    // classes (PTH_CLASS_STREAMS=0: all on one stream)
+    bool class_streams = true;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_upload = nullptr;
     Work work;
     TagState tag[MAX_TAG];

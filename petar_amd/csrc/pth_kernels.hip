@@ -1147,7 +1147,7 @@ static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t
     }
 }
 
-hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
+hipError_t launch_force(const BatchDev &b, int k, hipStream_t st, unsigned classes)
 {
     const SegDev &g = b.seg[k];
     if (b.plain) {
@@ -1188,7 +1188,7 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st)
     // instead of holding back a longer launch behind it on this stream
     for (int c = PLAN_CLASSES - 1; c >= 0; c--) {
         const int n = g.seg_count[c];
-        if (n <= 0) continue;
+        if (n <= 0 || !((classes >> c) & 1u)) continue;
         a.items = b.items + g.seg_begin[c];
         a.n_items = n;
         const unsigned nblk = (unsigned)((n + 3) / 4);
