@@ -450,53 +450,91 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         return c->fail(PTH_ERR_ARG, "pth_dispatch: null argument");
     TagState &T = c->tag[tag];
     if (T.pending) return c->fail(PTH_ERR_STATE, "pth_dispatch: previous batch of this tag not retrieved");
-    long long n_ide = 0, n_ids = 0;
+    // where each walk's i records, EPJ and SPJ list entries start
+    c->w_off.resize(3 * (size_t)(n_walk + 1));
+    long long *oi = c->w_off.data(), *oe = oi + (n_walk + 1), *os = oe + (n_walk + 1);
+    oi[0] = oe[0] = os[0] = 0;
     for (int w = 0; w < n_walk; w++) {
         if (n_epi[w] < 0 || n_epj[w] < 0 || n_spj[w] < 0)
             return c->fail(PTH_ERR_ARG, "pth_dispatch: negative size in walk " + std::to_string(w));
-        n_ide += n_epj[w];
-        n_ids += n_spj[w];
+        oi[w + 1] = oi[w] + n_epi[w];
+        oe[w + 1] = oe[w] + n_epj[w];
+        os[w + 1] = os[w] + n_spj[w];
     }
+    const long long ni_tot = oi[n_walk], n_ide = oe[n_walk], n_ids = os[n_walk];
+    if (ni_tot > INT_MAX || n_ide > INT_MAX || n_ids > INT_MAX)
+        return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: batch exceeds the int32 index range");
     const bool co = c->par.changeover != 0;
     if (co && !c->co_sent)
         return c->fail(PTH_ERR_STATE, "pth_dispatch: changeover mode, but the j arrays were sent without it");
     if ((c->par.spj_monopole != 0) != c->mono_sent)
         return c->fail(PTH_ERR_STATE, "pth_dispatch: spj_monopole differs from the last pth_send_j");
-    Plan P;
-    if (!make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, P, 1,
-                   c->mono_sent))
-        return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: batch exceeds the int32 index range");
-    if (co) {
-        const char *m = co_plan_error(P);
-        if (m) return c->fail(PTH_ERR_CAPACITY, std::string("pth_dispatch: ") + m);
-    }
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
-    // index lists: validated, and encoded as runs of consecutive indices
-    // (FDPS appends whole leaf cells, so EPJ lists are mostly runs) into a
-    // reused host buffer, each walk's runs where its list would start; this
-    // overlaps the previous batch's upload
+    // the pinned arena may still feed the previous batch's copy
+    HIPCHK(c, hipEventSynchronize(c->ev_upload), "hipEventSynchronize");
+    // the i records lead the arena: packed before its full size is known
+    HIPCHK(c, c->h_arena.ensure(Arena::i_part(ni_tot, co)), "pinned arena allocation");
     double t_pack = wall_s();
+    const bool encode = c->list_runs != 0;
     const int nj_lim = c->n_epj, ns_lim = c->n_spj;
-    if ((long long)c->h_rid.size() < n_ide + n_ids) {
+    if (encode && (long long)c->h_rid.size() < n_ide + n_ids) {
         c->h_rid.resize((size_t)(n_ide + n_ids));
         c->h_rpos.resize((size_t)(n_ide + n_ids));
     }
     c->run_ne.assign(n_walk + 1, 0);
     c->run_ns.assign(n_walk + 1, 0);
     int *hrid = c->h_rid.data(), *hrpos = c->h_rpos.data();
+    EpiDev *he = c->h_arena.as<EpiDev>();
+    long long *hid = reinterpret_cast<long long *>(c->h_arena.as<char>() + Arena::i_id_offset(ni_tot));
+    Plan &P = c->d_plan;
+    bool plan_ok = true;
     int bad = -1;
-#pragma omp parallel for schedule(dynamic)
-    for (int w = 0; w < n_walk; w++) {
-        const WalkDev &W = P.walks[w];
-        int mn, mx;
-        c->run_ne[w] = encode_runs(id_epj[w], W.n_e, hrid + W.e_off, hrpos + W.e_off, mn, mx);
-        bool ok = mn >= 0 && mx < nj_lim;
-        c->run_ns[w] = encode_runs(id_spj[w], W.n_s, hrid + n_ide + W.s_off, hrpos + n_ide + W.s_off, mn, mx);
-        ok = ok && mn >= 0 && mx < ns_lim;
-        if (!ok) {
+    // one thread makes the plan while the others validate the index lists,
+    // encode them as runs of consecutive indices (FDPS appends whole leaf
+    // cells, so EPJ lists are mostly runs; each walk's runs go where its
+    // list would start) and pack the i records
+#pragma omp parallel
+    {
+#pragma omp single nowait
+        plan_ok = make_plan(n_walk, n_epi, n_epj, n_spj, c->par.exec_mode, c->par.chunk_j, c->par.max_ipl, P, 1,
+                            c->mono_sent);
+#pragma omp for schedule(dynamic) nowait
+        for (int w = 0; w < n_walk; w++) {
+            const int ne = n_epj[w], ns = n_spj[w];
+            bool ok = true;
+            if (encode) {
+                int mn, mx;
+                c->run_ne[w] = encode_runs(id_epj[w], ne, hrid + oe[w], hrpos + oe[w], mn, mx);
+                ok = mn >= 0 && mx < nj_lim;
+                c->run_ns[w] = encode_runs(id_spj[w], ns, hrid + n_ide + os[w], hrpos + n_ide + os[w], mn, mx);
+                ok = ok && mn >= 0 && mx < ns_lim;
+            } else {
+                for (int k = 0; k < ne; k++) ok = ok && id_epj[w][k] >= 0 && id_epj[w][k] < nj_lim;
+                for (int k = 0; k < ns; k++) ok = ok && id_spj[w][k] >= 0 && id_spj[w][k] < ns_lim;
+            }
+            if (!ok) {
 #pragma omp atomic write
-            bad = w;
+                bad = w;
+            }
+            const char *base = static_cast<const char *>(epi[w]);
+            for (int i = 0; i < n_epi[w]; i++) {
+                const char *r = base + (size_t)i * iv->stride;
+                const double *q = reinterpret_cast<const double *>(r + iv->off_pos);
+                EpiDev d;
+                d.x = q[0];
+                d.y = q[1];
+                d.z = q[2];
+                d.rs = (float)*reinterpret_cast<const double *>(r + iv->off_rsearch);
+                d.pad = 0;
+                he[oi[w] + i] = d;
+                if (co) hid[oi[w] + i] = *reinterpret_cast<const long long *>(r + iv->off_id);
+            }
         }
+    }
+    if (!plan_ok) return c->fail(PTH_ERR_CAPACITY, "pth_dispatch: batch exceeds the int32 index range");
+    if (co) {
+        const char *m = co_plan_error(P);
+        if (m) return c->fail(PTH_ERR_CAPACITY, std::string("pth_dispatch: ") + m);
     }
     if (bad >= 0)
         return c->fail(PTH_ERR_ARG, "pth_dispatch: list index out of range in walk " + std::to_string(bad));
@@ -512,9 +550,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     Arena A;
     if (runs) A.layout(P, n_ide, n_ids, co, n_re, n_rs);
     else A.layout(P, n_ide, n_ids, co);
-    // the pinned arena may still feed the previous batch's copy
-    HIPCHK(c, hipEventSynchronize(c->ev_upload), "hipEventSynchronize");
-    HIPCHK(c, c->h_arena.ensure(A.total), "pinned arena allocation");
+    HIPCHK(c, c->h_arena.ensure_keep(A.total, A.walks), "pinned arena allocation");
     const bool grow = A.total > c->d_arena.cap;
     if (grow) HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     HIPCHK(c, c->d_arena.ensure(A.total), "device arena allocation");
@@ -550,8 +586,6 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
 
     t_pack = wall_s();
     char *h = c->h_arena.as<char>();
-    EpiDev *he = reinterpret_cast<EpiDev *>(h + A.epi);
-    long long *hid = reinterpret_cast<long long *>(h + A.epi_id);
     int *hide = reinterpret_cast<int *>(h + A.ide);
     int *hids = reinterpret_cast<int *>(h + A.ids);
     // runs: first indices of all of the batch's runs, then their positions
@@ -577,19 +611,6 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
 #pragma omp parallel for schedule(dynamic)
     for (int w = 0; w < n_walk; w++) {
         const WalkDev &W = P.walks[w];
-        const char *base = static_cast<const char *>(epi[w]);
-        for (int i = 0; i < W.n_i; i++) {
-            const char *r = base + (size_t)i * iv->stride;
-            const double *p = reinterpret_cast<const double *>(r + iv->off_pos);
-            EpiDev d;
-            d.x = p[0];
-            d.y = p[1];
-            d.z = p[2];
-            d.rs = (float)*reinterpret_cast<const double *>(r + iv->off_rsearch);
-            d.pad = 0;
-            he[W.i_off + i] = d;
-            if (co) hid[W.i_off + i] = *reinterpret_cast<const long long *>(r + iv->off_id);
-        }
         if (runs) {
             const int ne = c->run_ne[w + 1] - c->run_ne[w], ns = c->run_ns[w + 1] - c->run_ns[w];
             if (ne) {

@@ -84,6 +84,23 @@ struct PinBuf {
         cap = want;
         return hipSuccess;
     }
+    // grows like ensure, keeping the first `keep` bytes
+    hipError_t ensure_keep(size_t bytes, size_t keep)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (inject_alloc_failure()) return hipErrorOutOfMemory;
+        void *q = nullptr;
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+        hipError_t e = hipHostMalloc(&q, want, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        if (p) {
+            std::memcpy(q, p, std::min(keep, cap));
+            (void)hipHostFree(p);
+        }
+        p = q;
+        cap = want;
+        return hipSuccess;
+    }
     void release()
     {
         if (p) (void)hipHostFree(p);
@@ -104,13 +121,20 @@ inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 struct Arena {
     size_t epi, epi_id, walks, es, ss, ide, ids, items, ritems, run_e, run_s, run_eo, run_so, total;
     bool runs = false;
+    // the i records (and ids) lead: [0, i_part(ni_tot, co)), i ids at
+    // i_id_offset(ni_tot)
+    static size_t i_id_offset(long long ni_tot) { return align256(sizeof(EpiDev) * (size_t)ni_tot); }
+    static size_t i_part(long long ni_tot, bool co)
+    {
+        return align256(i_id_offset(ni_tot) + (co ? sizeof(long long) * (size_t)ni_tot : 0));
+    }
     void layout(const Plan &P, long long n_ide, long long n_ids, bool co, long long n_run_e = -1,
                 long long n_run_s = -1)
     {
         runs = n_run_e >= 0 && n_run_s >= 0;
-        size_t o = 0;
-        epi = o; o = align256(o + sizeof(EpiDev) * (size_t)P.ni_tot);
-        epi_id = o; o = align256(o + (co ? sizeof(long long) * (size_t)P.ni_tot : 0));
+        epi = 0;
+        epi_id = i_id_offset(P.ni_tot);
+        size_t o = i_part(P.ni_tot, co);
         walks = o; o = align256(o + sizeof(WalkDev) * P.walks.size());
         es = o; o = align256(o + sizeof(long long) * P.es_off.size());
         ss = o; o = align256(o + sizeof(long long) * P.ss_off.size());
@@ -362,6 +386,7 @@ struct pth_ctx {
     // the work buffers)
     std::vector<int> h_rid, h_rpos;
     std::vector<int> run_ne, run_ns;
+    std::vector<long long> w_off;  // pth_dispatch: prefix sums of the walks' i, EPJ, SPJ counts
     std::vector<int> ident;  // 0, 1, 2, ...: the lists of pth_dispatch_copies
     DevBuf d_ids;
     // SPJ table of the sent superparticles, built at the first dispatch
@@ -387,6 +412,9 @@ struct pth_ctx {
     // calls (the plan's vectors keep their capacity)
     Plan na_plan;
     BatchDev na_batch;
+    // pth_dispatch's plan, kept between batches: fresh vectors of a few
+    // hundred kB would be fresh pages, whose faults cost more than the planning
+    Plan d_plan;
 
     int fail(int code, const std::string &msg)
     {

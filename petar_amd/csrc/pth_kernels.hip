@@ -1,6 +1,8 @@
 // pth_kernels.hip -- gfx950 kernels of the soft-force path.
 //
 // Pipeline for one batch of walks (i-groups with their interaction lists):
+//   k_spj_table   per superparticle, once per send: {m, q'xx q'yy q'zz q'xy
+//                 q'yz q'zx, -eps^2 tr}, q' = 3Q - tr I
 //   k_stage_walk  one workgroup per walk: FP64 bounding box of the walk's i
 //                 (centre c, half extents, largest r_search), every i as FP32
 //                 (x-c, y-c, z-c, rs^2), and the walk's EPJ list gathered
@@ -8,10 +10,9 @@
 //                 region (entries that may lie within max(rs_i, rs_j, r_out)
 //                 of some i of the walk) at the front and a "far" region at
 //                 the back; records are SoA blocks of 8 {x[8] y[8] z[8] m[8]
-//                 rs2[8]}, gaps padded with far massless entries; then the
-//                 walk's SPJ list gathered into 12-float records relative to
-//                 the centre {x y z m q'xx q'yy q'zz q'xy q'yz q'zx
-//                 -eps^2 tr pad}, q' = 3Q - tr I
+//                 rs2[8]}, gaps padded with far massless entries; beside it,
+//                 one or more workgroups per walk gather the walk's SPJ list
+//                 into 4-float records {x-c y-c z-c table index}
 //   k_force       one wave per work item (walk, tile of 64/128/256 i with
 //                 1/2/4 i per lane, j-range), one launch per i-per-lane
 //                 class: j records are read through the constant address
@@ -113,7 +114,10 @@ __device__ __forceinline__ unsigned co_hash(long long id)
 }
 
 struct StageArgs {
-    int walk_begin;  // block b stages walk walk_begin + b
+    int walk_begin;  // block b < n_walk stages walk walk_begin + b
+    // blocks n_walk + p n_walk + v (p < spj_parts) stage part p of the SPJ
+    // list of walk walk_begin + v
+    int n_walk, spj_parts;
     const EpiDev *epi;
     const WalkDev *walks;
     const long long *es_off, *ss_off;
@@ -140,6 +144,38 @@ struct StageArgs {
     const EpjDev *spj_mono;
 };
 
+// SPJ list entries [k_begin, k_end) of walk w: positions relative to the
+// centre and table indices, U per thread and pass so that U gathers are in
+// flight
+template <int U>
+__device__ __forceinline__ void stage_spj(const StageArgs &a, int w, const WalkDev &wk, const double (&c)[3],
+                                          int k_begin, int k_end)
+{
+    float4 *out_s = reinterpret_cast<float4 *>(a.staged_s) + a.ss_off[w];
+    const int *ids = a.id_spj + wk.s_off;
+    const SpjDev *__restrict__ spj = a.spj;
+    const int tid = (int)threadIdx.x;
+    for (int k0 = k_begin; k0 < k_end; k0 += 256 * U) {
+        int id[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + 256 * u + tid;
+            id[u] = k < k_end ? ids[k] : -1;
+        }
+        double px[U], py[U], pz[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (id[u] >= 0) {
+                px[u] = spj[id[u]].x;
+                py[u] = spj[id[u]].y;
+                pz[u] = spj[id[u]].z;
+            }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (id[u] >= 0) put_spj(out_s + (k0 + 256 * u + tid), px[u], py[u], pz[u], c, id[u]);
+    }
+}
+
 // CO (changeover mode): every near EPJ also gets its changeover radii and the
 // low-order parts of its FP32 relative coordinates staged (per block of 8:
 // r_in[8], r_out[8], xlo[8], ylo[8], zlo[8]; x - c = x_hi + x_lo as a
@@ -150,11 +186,20 @@ struct StageArgs {
 // in place, in global memory); the i's staged slot of that entry and its radii go to
 // staged_ic (the force kernel drops the self pair by slot and takes the i's
 // changeover radii from there).  walk_err counts i with no such entry.
-template <bool CO>
+// EU: EPJ entries per thread and pass of the near/far split, all gathers in
+// flight at once: 4 where the launch has few walks (its latency is exposed),
+// 2 where it has many (fewer registers, so that staging waves fit beside the
+// force kernels of other segments) and in changeover mode
+template <bool CO, int EU>
 __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
 {
-    const int w = a.walk_begin + (int)blockIdx.x;
+    // the SPJ blocks need only the walk's centre: they run beside the EPJ
+    // split, which is sequential within a walk
+    const int nw = a.n_walk;
+    const bool spj_wg = (int)blockIdx.x >= nw;
+    const int w = a.walk_begin + (spj_wg ? ((int)blockIdx.x - nw) % nw : (int)blockIdx.x);
     const WalkDev wk = a.walks[w];
+    if (spj_wg && (a.mono || wk.n_s == 0)) return;
     const EpiDev *__restrict__ epi = a.epi;
     const EpjDev *__restrict__ epj = a.epj;
     const int *__restrict__ id_epj = a.id_epj;
@@ -163,25 +208,27 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
     float *__restrict__ staged_e = a.staged_e;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     __shared__ double red[4][7];
-    __shared__ int cnt[4][4];
+    __shared__ int cnt[2 * EU][4];
     __shared__ int co_idx[CO ? PTH_CO_HASH : 1];
     __shared__ int co_self[CO ? PTH_CO_MAX_NI : 1];
     __shared__ float co_rin[CO ? PTH_CO_MAX_NI : 1], co_rout[CO ? PTH_CO_MAX_NI : 1], co_m[CO ? PTH_CO_MAX_NI : 1];
-    const int n_co = CO ? min(wk.n_i, PTH_CO_MAX_NI) : 0;
+    const int n_co = (CO && !spj_wg) ? min(wk.n_i, PTH_CO_MAX_NI) : 0;
     if constexpr (CO) {
-        for (int k = tid; k < PTH_CO_HASH; k += 256) co_idx[k] = -1;
-        __syncthreads();
-        for (int k = tid; k < n_co; k += 256) {
-            const long long id = a.epi_id[wk.i_off + k];
-            unsigned h = co_hash(id);
-            // at most PTH_CO_MAX_NI keys in PTH_CO_HASH buckets: a free one exists
-            while (atomicCAS(&co_idx[h], -1, k) != -1) h = (h + 1) & (PTH_CO_HASH - 1);
-            co_self[k] = -1;
-            co_rin[k] = 0.f;
-            co_rout[k] = 0.f;
-            co_m[k] = 0.f;
+        if (!spj_wg) {
+            for (int k = tid; k < PTH_CO_HASH; k += 256) co_idx[k] = -1;
+            __syncthreads();
+            for (int k = tid; k < n_co; k += 256) {
+                const long long id = a.epi_id[wk.i_off + k];
+                unsigned h = co_hash(id);
+                // at most PTH_CO_MAX_NI keys in PTH_CO_HASH buckets: a free one exists
+                while (atomicCAS(&co_idx[h], -1, k) != -1) h = (h + 1) & (PTH_CO_HASH - 1);
+                co_self[k] = -1;
+                co_rin[k] = 0.f;
+                co_rout[k] = 0.f;
+                co_m[k] = 0.f;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     // -- bounding box and largest r_search of the walk's i
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rsm = 0.0;
@@ -212,6 +259,16 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         h[d] = (wk.n_i > 0) ? 0.5 * (u - l) : 0.0;
     }
     rsm = fmax(fmax(red[0][6], red[1][6]), fmax(red[2][6], red[3][6]));
+    if (spj_wg) {
+        const int part = ((int)blockIdx.x - nw) / nw;
+        const int k_begin = (int)((long long)wk.n_s * part / a.spj_parts);
+        const int k_end = (int)((long long)wk.n_s * (part + 1) / a.spj_parts);
+        // (two in changeover mode: fewer registers, so the staging of the
+        // next segment fits beside the changeover force kernels, which are
+        // held to 7 waves per SIMD)
+        stage_spj<CO ? 2 : 4>(a, w, wk, c, k_begin, k_end);
+        return;
+    }
     if (tid == 0) a.centre[w] = make_double4(c[0], c[1], c[2], 0.0);
     for (int k = tid; k < wk.n_i; k += 256) {
         const EpiDev e = epi[wk.i_off + k];
@@ -224,8 +281,8 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
                 make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.f);
     }
     // -- EPJ list: stable split into near (front) and far (back, reversed);
-    //    two entries per thread and pass (entries t0 + tid and t0 + 256 + tid)
-    //    so that two gathers are in flight per thread between the barriers
+    //    EU entries per thread and pass (entries t0 + 256 u + tid) so that EU
+    //    gathers are in flight per thread between the barriers
     const long long base = a.es_off[w];
     const long long cap = a.es_off[w + 1] - base;
     const double hmax = fmax(h[0], fmax(h[1], h[2]));
@@ -301,39 +358,45 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         }
     };
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int t0 = 0; t0 < wk.n_e; t0 += 512) {
-        Ent qa = load(t0 + tid), qb = load(t0 + 256 + tid);
-        classify(qa);
-        classify(qb);
-        const unsigned long long an = __ballot(qa.valid && qa.nearj), af = __ballot(qa.valid && !qa.nearj);
-        const unsigned long long bn = __ballot(qb.valid && qb.nearj), bf = __ballot(qb.valid && !qb.nearj);
+    for (int t0 = 0; t0 < wk.n_e; t0 += 256 * EU) {
+        Ent q[EU];
+#pragma unroll
+        for (int u = 0; u < EU; u++) q[u] = load(t0 + 256 * u + tid);
+#pragma unroll
+        for (int u = 0; u < EU; u++) classify(q[u]);
+        unsigned long long bn[EU], bf[EU];
+#pragma unroll
+        for (int u = 0; u < EU; u++) {
+            bn[u] = __ballot(q[u].valid && q[u].nearj);
+            bf[u] = __ballot(q[u].valid && !q[u].nearj);
+        }
         if (lane == 0) {
-            cnt[0][wave] = __popcll(an);
-            cnt[1][wave] = __popcll(af);
-            cnt[2][wave] = __popcll(bn);
-            cnt[3][wave] = __popcll(bf);
+#pragma unroll
+            for (int u = 0; u < EU; u++) {
+                cnt[2 * u][wave] = __popcll(bn[u]);
+                cnt[2 * u + 1][wave] = __popcll(bf[u]);
+            }
         }
         __syncthreads();
-        int oan = n_near, oaf = n_far, obn = 0, obf = 0;
-        for (int q = 0; q < wave; q++) {
-            oan += cnt[0][q];
-            oaf += cnt[1][q];
-            obn += cnt[2][q];
-            obf += cnt[3][q];
+        // the entries of part u (t0 + 256 u + tid) follow all of part u - 1's
+#pragma unroll
+        for (int u = 0; u < EU; u++) {
+            int pn = 0, pf = 0, tn = 0, tf = 0;
+            for (int v = 0; v < 4; v++) {
+                const int cn = cnt[2 * u][v], cf = cnt[2 * u + 1][v];
+                if (v < wave) {
+                    pn += cn;
+                    pf += cf;
+                }
+                tn += cn;
+                tf += cf;
+            }
+            if (q[u].valid)
+                place(q[u], q[u].nearj ? base + n_near + pn + __popcll(bn[u] & below)
+                                       : base + cap - 1 - (n_far + pf + __popcll(bf[u] & below)));
+            n_near += tn;
+            n_far += tf;
         }
-        const int tan = cnt[0][0] + cnt[0][1] + cnt[0][2] + cnt[0][3];
-        const int taf = cnt[1][0] + cnt[1][1] + cnt[1][2] + cnt[1][3];
-        const int tbn = cnt[2][0] + cnt[2][1] + cnt[2][2] + cnt[2][3];
-        const int tbf = cnt[3][0] + cnt[3][1] + cnt[3][2] + cnt[3][3];
-        // the second half's entries follow all of the first half's
-        obn += n_near + tan;
-        obf += n_far + taf;
-        if (qa.valid)
-            place(qa, qa.nearj ? base + oan + __popcll(an & below) : base + cap - 1 - (oaf + __popcll(af & below)));
-        if (qb.valid)
-            place(qb, qb.nearj ? base + obn + __popcll(bn & below) : base + cap - 1 - (obf + __popcll(bf & below)));
-        n_near += tan + tbn;
-        n_far += taf + tbf;
         __syncthreads();
     }
     // monopole superparticles: far entries after the EPJ list's far ones (a
@@ -371,35 +434,6 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
         }
         const int n_miss = __syncthreads_count(miss);
         if (tid == 0) a.walk_err[w] = n_miss;
-    }
-    // -- SPJ list: positions relative to the centre and table indices, U
-    //    per thread and pass so that U gathers are in flight
-    float4 *out_s = reinterpret_cast<float4 *>(a.staged_s) + a.ss_off[w];
-    const int *ids = a.id_spj + wk.s_off;
-    const SpjDev *__restrict__ spj = a.spj;
-    // (two in changeover mode: fewer registers, so the staging of the next
-    // segment fits beside the changeover force kernels, which are held to 7
-    // waves per SIMD; 4.5% off the pipelined changeover step)
-    constexpr int U = CO ? 2 : 4;
-    const int n_srec = a.mono ? 0 : wk.n_s;
-    for (int k0 = 0; k0 < n_srec; k0 += 256 * U) {
-        int id[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = k0 + 256 * u + tid;
-            id[u] = k < n_srec ? ids[k] : -1;
-        }
-        double px[U], py[U], pz[U];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (id[u] >= 0) {
-                px[u] = spj[id[u]].x;
-                py[u] = spj[id[u]].y;
-                pz[u] = spj[id[u]].z;
-            }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (id[u] >= 0) put_spj(out_s + (k0 + 256 * u + tid), px[u], py[u], pz[u], c, id[u]);
     }
 }
 
@@ -1028,21 +1062,24 @@ __global__ __launch_bounds__(256) void k_reduce(const RItem *__restrict__ ritems
 }
 
 // ---------------------------------------------------------------------------
-// index lists sent as runs (pth_dispatch): workgroup g expands the EPJ list
-// (g < n_walk) or the SPJ list (g >= n_walk) of one walk.  The walk's runs
-// are taken EXPAND_RUNS at a time into LDS (first index, position); each
-// entry of the positions those runs cover finds its run by a binary search
-// in LDS and is written coalesced.  The first run of a non-empty list has
-// position 0 and positions ascend.
+// index lists sent as runs (pth_dispatch): workgroup g expands part
+// g / (2 n_walk) of n_part of the EPJ list (g % (2 n_walk) < n_walk) or the
+// SPJ list of one walk.  The part's first run is found by a binary search
+// over the walk's run positions; the runs are then taken EXPAND_RUNS at a
+// time into LDS (first index, position); each entry of the part that those
+// runs cover finds its run by a binary search in LDS and is written
+// coalesced.  The first run of a non-empty list has position 0 and
+// positions ascend.
 #define EXPAND_RUNS 1024
 __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__ walks, int w_begin, int n_walk,
-                                                     const int *__restrict__ rid_e, const int *__restrict__ rpos_e,
-                                                     const int *__restrict__ run_eo, const int *__restrict__ rid_s,
-                                                     const int *__restrict__ rpos_s, const int *__restrict__ run_so,
-                                                     int *__restrict__ out_e, int *__restrict__ out_s)
+                                                     int n_part, const int *__restrict__ rid_e,
+                                                     const int *__restrict__ rpos_e, const int *__restrict__ run_eo,
+                                                     const int *__restrict__ rid_s, const int *__restrict__ rpos_s,
+                                                     const int *__restrict__ run_so, int *__restrict__ out_e,
+                                                     int *__restrict__ out_s)
 {
     __shared__ int l_id[EXPAND_RUNS], l_pos[EXPAND_RUNS];
-    const int g = (int)blockIdx.x;
+    const int part = (int)blockIdx.x / (2 * n_walk), g = (int)blockIdx.x % (2 * n_walk);
     const bool sp = g >= n_walk;
     const int w = w_begin + (sp ? g - n_walk : g);
     const WalkDev wk = walks[w];
@@ -1051,17 +1088,29 @@ __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__
     const int *__restrict__ rpos = sp ? rpos_s : rpos_e;
     const int *__restrict__ ro = sp ? run_so : run_eo;
     const int r0 = ro[w], r1 = ro[w + 1];
+    // this part's entries [q0, q1)
+    const int q0 = (int)((long long)n * part / n_part), q1 = (int)((long long)n * (part + 1) / n_part);
+    if (q0 >= q1) return;
+    // its first run: the last one whose position is <= q0 (the first run of a
+    // non-empty list has position 0)
+    int f = r0;
+    for (int hi = r1 - 1; f < hi;) {
+        const int mid = (f + hi + 1) >> 1;
+        if (rpos[mid] <= q0) f = mid;
+        else hi = mid - 1;
+    }
     int *__restrict__ out = sp ? out_s + wk.s_off : out_e + wk.e_off;
     const int tid = (int)threadIdx.x;
-    for (int c0 = r0; c0 < r1; c0 += EXPAND_RUNS) {
+    for (int c0 = f; c0 < r1; c0 += EXPAND_RUNS) {
         const int nc = min(EXPAND_RUNS, r1 - c0);
         for (int k = tid; k < nc; k += 256) {
             l_id[k] = rid[c0 + k];
             l_pos[k] = rpos[c0 + k];
         }
         __syncthreads();
-        // the entries these runs cover: up to the next chunk's first run
-        const int p0 = l_pos[0], p1 = (c0 + nc < r1) ? rpos[c0 + nc] : n;
+        // the part's entries these runs cover: up to the next chunk's first run
+        const int next = (c0 + nc < r1) ? rpos[c0 + nc] : n;
+        const int p0 = max(l_pos[0], q0), p1 = min(next, q1);
         for (int k = p0 + tid; k < p1; k += 256) {
             int lo = 0, hi = nc - 1;
             while (lo < hi) {
@@ -1072,6 +1121,7 @@ __global__ __launch_bounds__(256) void k_expand_runs(const WalkDev *__restrict__
             out[k] = l_id[lo] + (k - l_pos[lo]);
         }
         __syncthreads();
+        if (next >= q1) break;
     }
 }
 
@@ -1090,8 +1140,10 @@ hipError_t launch_expand_runs(const WalkDev *walks, int w_begin, int n_walk, con
                               int *out_s, hipStream_t st)
 {
     if (n_walk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_expand_runs, dim3(2u * (unsigned)n_walk), dim3(256), 0, st, walks, w_begin, n_walk, rid_e,
-                       rpos_e, run_eo, rid_s, rpos_s, run_so, out_e, out_s);
+    // parts per list: a few blocks per CU for a batch of a few hundred walks
+    const int n_part = std::max(1, std::min(8, 4096 / (2 * n_walk)));
+    hipLaunchKernelGGL(k_expand_runs, dim3(2u * (unsigned)n_walk * (unsigned)n_part), dim3(256), 0, st, walks,
+                       w_begin, n_walk, n_part, rid_e, rpos_e, run_eo, rid_s, rpos_s, run_so, out_e, out_s);
     return hipGetLastError();
 }
 
@@ -1126,8 +1178,14 @@ hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st)
     a.staged_el = b.staged_el;
     a.mono = b.mono;
     a.spj_mono = reinterpret_cast<const EpjDev *>(b.spj);
-    if (b.co) hipLaunchKernelGGL(k_stage_walk<true>, dim3(g.n_walk), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_stage_walk<false>, dim3(g.n_walk), dim3(256), 0, st, a);
+    // SPJ parts per walk: enough blocks for a few per CU when the launch
+    // has few walks (a dispatched batch), one beside each EPJ block otherwise
+    a.n_walk = g.n_walk;
+    a.spj_parts = b.mono ? 0 : std::max(1, std::min(8, 2048 / g.n_walk));
+    const unsigned nblk = (unsigned)g.n_walk * (1u + (unsigned)a.spj_parts);
+    if (b.co) hipLaunchKernelGGL((k_stage_walk<true, 2>), dim3(nblk), dim3(256), 0, st, a);
+    else if (g.n_walk <= 2048) hipLaunchKernelGGL((k_stage_walk<false, 4>), dim3(nblk), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_stage_walk<false, 2>), dim3(nblk), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
