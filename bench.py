@@ -538,6 +538,8 @@ def main():
     ap.add_argument("--no-device-tree", action="store_true")
     ap.add_argument("--no-changeover", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay comparison")
+    ap.add_argument("--persistent", type=int, default=0, choices=[0, 1],
+                    help="force launches as one wave per free wave slot taking items from per-XCD counters")
     ap.add_argument("--df-near", type=int, default=1, choices=[0, 1],
                     help="near EPJ pairs in double-float walk-relative coordinates (linear-cutoff mode)")
     a = ap.parse_args()
@@ -555,7 +557,8 @@ def main():
     import torch  # device-wide synchronisation around the timed region
     exec_mode = sf.EXEC_FUSED if a.exec == "fused" else sf.EXEC_PLAIN
     ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=a.chunk_j,
-                     max_ipl=a.max_ipl, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams, df_near=a.df_near)
+                     max_ipl=a.max_ipl, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams, df_near=a.df_near,
+                     persistent=a.persistent)
     res = sf.Resident.from_lists(ctx, L)
     tuned = None
     if a.exec == "fused" and not a.no_autotune:
@@ -590,7 +593,7 @@ def main():
     ipl_used = tuned["max_ipl"] if tuned else a.max_ipl
     cj_used = tuned["chunk_j"] if tuned else a.chunk_j
     ctx1 = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, exec_mode=exec_mode, chunk_j=cj_used, max_ipl=ipl_used,
-                      j_vgpr=a.j_vgpr, segments=1, df_near=a.df_near)
+                      j_vgpr=a.j_vgpr, segments=1, df_near=a.df_near, persistent=a.persistent)
     r1 = sf.Resident.from_lists(ctx1, L)
     r1.run()
     k_force, k_stage, k_red = [], [], []
@@ -643,7 +646,7 @@ def main():
         # segments and streams, same warm-up and step count)
         ctxg = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=cj_used,
                           max_ipl=ipl_used, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams,
-                          df_near=a.df_near, graph=1)
+                          df_near=a.df_near, graph=1, persistent=a.persistent)
         rg = sf.Resident.from_lists(ctxg, L)
         for _ in range(a.warmup):
             rg.run()
@@ -680,7 +683,7 @@ def main():
         # be compared bit for bit
         ctxd = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, nnb_mode=0, exec_mode=exec_mode, chunk_j=cj_used,
                           max_ipl=ipl_used, j_vgpr=a.j_vgpr, segments=a.segments, streams=a.streams,
-                          df_near=a.df_near)
+                          df_near=a.df_near, persistent=a.persistent)
         out["device_tree_step"] = device_tree_step(ctxd, particles, r_out, L, (acc, pot, nnb))
         ctxd.close()
         out["device_tree_step"]["host_tree_walk_s"] = prep["tree_walk_s"]

@@ -124,6 +124,12 @@ typedef struct pth_params {
                       runs in fewer segments, down to one on one stream
                       (launches and cross-stream events cost more there than
                       the overlap gains).  Results do not depend on it. */
+    int persistent; /* fused form (pth_dispatch and resident evaluations): 1
+                      launches each i-per-lane class as one wave per free
+                      wave slot, the waves of each XCD taking the plan's
+                      items of that XCD from a counter in plan order; 0
+                      launches one wave per item.  Results do not depend on
+                      it. */
 } pth_params;
 
 /* Changeover mode folds PeTar's changeover correction of the soft force
@@ -149,7 +155,7 @@ typedef struct pth_params {
 /* Fills the defaults: eps2 0, rcut2 0, G 1, nnb_mode 0, fused, chunk 0,
  * max_walk 1000, max_ipl 2, j_vgpr 0, changeover 0, ngb_factor 0.99,
  * segments 8, streams 3, df_near 1, spj_monopole 0, graph 0, min_seg_pairs 0
- * (1e9). */
+ * (1e9), persistent 0. */
 void pth_params_default(pth_params *p);
 
 /* Creates a context on device (rank % number of visible devices). */

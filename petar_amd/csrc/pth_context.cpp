@@ -38,6 +38,7 @@ void pth_params_default(pth_params *p)
     p->spj_monopole = 0;
     p->graph = 0;
     p->min_seg_pairs = 0.0;
+    p->persistent = 0;
 }
 
 const char *pth_last_error(const pth_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -167,6 +168,7 @@ int pth_set_params(pth_ctx *c, const pth_params *p)
     c->par.spj_monopole = p->spj_monopole ? 1 : 0;
     c->par.graph = p->graph ? 1 : 0;
     c->par.min_seg_pairs = p->min_seg_pairs > 0 ? p->min_seg_pairs : 0.0;
+    c->par.persistent = p->persistent ? 1 : 0;
     return PTH_OK;
 }
 

@@ -166,6 +166,11 @@ struct BatchDev {
     int *out_nnb = nullptr;
     double4 *part_accp = nullptr;
     int *part_nnb = nullptr;
+    // persistent force launches: PTH_MAX_SEGMENTS x PLAN_CLASSES x 8 item
+    // counters (per segment, i-per-lane class and XCD), zeroed by the
+    // segment's staging launch
+    int persistent = 0;
+    int *counters = nullptr;
 };
 
 // The SPJ table of n quadrupole superparticles for softening eps2 (the

@@ -155,6 +155,7 @@ struct Work {
     DevBuf centre, near_blk, staged_i, staged_e, staged_s, part_accp, part_nnb;
     DevBuf staged_ic, staged_il, staged_ec, part_pott, walk_err;  // changeover mode
     DevBuf staged_el;                                             // linear mode, double-float near pairs
+    DevBuf counters;                                              // persistent force launches
     hipError_t ensure(const Plan &P, bool co, bool df)
     {
         hipError_t e;
@@ -178,6 +179,7 @@ struct Work {
         if ((e = staged_s.ensure(sizeof(float) * PTH_SPJ_STAGED * std::max<long long>(P.n_s_rec, 1)))) return e;
         if ((e = part_accp.ensure(sizeof(double4) * 64 * std::max(P.n_part, 1)))) return e;
         if ((e = part_nnb.ensure(sizeof(int) * 64 * std::max(P.n_part, 1)))) return e;
+        if ((e = counters.ensure(sizeof(int) * PTH_MAX_SEGMENTS * PLAN_CLASSES * 8))) return e;
         return hipSuccess;
     }
     void release()
@@ -186,6 +188,7 @@ struct Work {
         part_accp.release(); part_nnb.release();
         staged_ic.release(); staged_il.release(); staged_ec.release(); part_pott.release(); walk_err.release();
         staged_el.release();
+        counters.release();
     }
 };
 
@@ -289,6 +292,8 @@ inline void fill_batch(BatchDev &b, const Plan &P, const pth_params &par, const 
     b.part_nnb = W.part_nnb.as<int>();
     b.out_accp = out_accp;
     b.out_nnb = out_nnb;
+    b.counters = W.counters.as<int>();
+    b.persistent = (par.persistent && b.counters) ? 1 : 0;
     if (b.df) {
         b.staged_il = W.staged_il.as<float4>();
         b.staged_el = W.staged_el.as<float>();

@@ -39,6 +39,9 @@
 // G multiplies the FP64 sums once at the end.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <unordered_map>
+
 #include "pth_device.h"
 
 namespace pth {
@@ -142,6 +145,9 @@ struct StageArgs {
     // monopole superparticles (centre and mass), staged as far EPJ entries
     int mono;
     const EpjDev *spj_mono;
+    // persistent force launches: the segment's item counters (zeroed here,
+    // before the segment's force launches), or null
+    int *counters;
 };
 
 // SPJ list entries [k_begin, k_end) of walk w: positions relative to the
@@ -199,6 +205,7 @@ __global__ __launch_bounds__(256) void k_stage_walk(const StageArgs a)
     const bool spj_wg = (int)blockIdx.x >= nw;
     const int w = a.walk_begin + (spj_wg ? ((int)blockIdx.x - nw) % nw : (int)blockIdx.x);
     const WalkDev wk = a.walks[w];
+    if (blockIdx.x == 0 && a.counters && threadIdx.x < PLAN_CLASSES * 8) a.counters[threadIdx.x] = 0;
     if (spj_wg && (a.mono || wk.n_s == 0)) return;
     const EpiDev *__restrict__ epi = a.epi;
     const EpjDev *__restrict__ epj = a.epj;
@@ -772,6 +779,10 @@ __device__ __forceinline__ void spj_records(cfp ss, cfp tab, int s_begin, int s_
 struct ForceArgs {
     const Item *items;
     int n_items;
+    // persistent launches: the class's 8 item counters (one per XCD) and the
+    // slots of each XCD's run of items
+    int *counter;
+    int n_run;
     const int *near_blk;
     const float4 *staged_i;
     const float *staged_e, *staged_s;
@@ -873,12 +884,10 @@ __device__ __forceinline__ void run_tile(const ForceArgs &a, const Item &it, int
     }
 }
 
-// One launch per i-per-lane class: items [0, n_items) of this class.
+// Work item idx of this class's launch (empty slots have walk < 0).
 template <int NNB_LE, int IPL, bool VC, bool CO, bool DF>
-__device__ __forceinline__ void force_items(const ForceArgs &a)
+__device__ __forceinline__ void force_item(const ForceArgs &a, int idx)
 {
-    const int idx = uniform((int)blockIdx.x * 4 + ((int)threadIdx.x >> 6));
-    if (idx >= a.n_items) return;
     const cip ci = (cip)(a.items) + (size_t)idx * 8;
     Item it;
     it.walk = ci[0];
@@ -895,20 +904,45 @@ __device__ __forceinline__ void force_items(const ForceArgs &a)
     run_tile<IPL, NNB_LE, VC, CO, DF>(a, it, near_end, threadIdx.x & 63, K);
 }
 
-template <int NNB_LE, int IPL, bool VC, bool CO, bool DF>
+// One launch per i-per-lane class: items [0, n_items) of this class, one
+// per wave; or, persistent (P), one wave per free wave slot: the waves of
+// block b (on XCD b % 8, workgroups being handed to the XCDs round robin)
+// take the items of that XCD's run from its counter, in plan order (most
+// expensive first), until the run is done.
+template <int NNB_LE, int IPL, bool VC, bool CO, bool DF, bool P>
+__device__ __forceinline__ void force_items(const ForceArgs &a)
+{
+    if constexpr (P) {
+        const int x = (int)blockIdx.x & 7;
+        int *cnt = a.counter + x;
+        for (;;) {
+            int k = 0;
+            if ((threadIdx.x & 63) == 0) k = atomicAdd(cnt, 1);
+            k = uniform(k);
+            if (k >= a.n_run) break;
+            force_item<NNB_LE, IPL, VC, CO, DF>(a, ((k >> 2) * 8 + x) * 4 + (k & 3));
+        }
+    } else {
+        const int idx = uniform((int)blockIdx.x * 4 + ((int)threadIdx.x >> 6));
+        if (idx >= a.n_items) return;
+        force_item<NNB_LE, IPL, VC, CO, DF>(a, idx);
+    }
+}
+
+template <int NNB_LE, int IPL, bool VC, bool CO, bool DF, bool P = false>
 __global__ __launch_bounds__(256) void k_force(const ForceArgs a)
 {
-    force_items<NNB_LE, IPL, VC, CO, DF>(a);
+    force_items<NNB_LE, IPL, VC, CO, DF, P>(a);
 }
 
 // Changeover mode: its extra per-i state and the revisit of neighbour pairs
 // leave room for 5 waves per SIMD at 2 i per lane; held to 7 (a few spilled
 // registers, on the revisit path) the force phase runs 10% faster.  With 3
 // or 4 i per lane the spills would be heavy: no limit there.
-template <int NNB_LE, int IPL, bool VC>
+template <int NNB_LE, int IPL, bool VC, bool P = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(IPL <= 2 ? 7 : 1))) void k_force_co(const ForceArgs a)
 {
-    force_items<NNB_LE, IPL, VC, true, false>(a);
+    force_items<NNB_LE, IPL, VC, true, false, P>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1178,6 +1212,7 @@ hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st)
     a.staged_el = b.staged_el;
     a.mono = b.mono;
     a.spj_mono = reinterpret_cast<const EpjDev *>(b.spj);
+    a.counters = b.persistent ? b.counters + k * PLAN_CLASSES * 8 : nullptr;
     // SPJ parts per walk: enough blocks for a few per CU when the launch
     // has few walks (a dispatched batch), one beside each EPJ block otherwise
     a.n_walk = g.n_walk;
@@ -1189,20 +1224,55 @@ hipError_t launch_stage(const BatchDev &b, int k, hipStream_t st)
     return hipGetLastError();
 }
 
-template <int NNB, bool VC, bool CO, bool DF>
+// 256-thread blocks of kernel f that the current device holds at once
+static unsigned fit_blocks(const void *f)
+{
+    static std::mutex m;
+    static std::unordered_map<const void *, int> per_cu;
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    std::lock_guard<std::mutex> lk(m);
+    if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    auto it = per_cu.find(f);
+    if (it == per_cu.end()) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess) return 0;
+        it = per_cu.emplace(f, nb).first;
+    }
+    return (unsigned)(it->second * cus[dev]);
+}
+
+// nblk blocks of one item per wave; persistent (P): as many blocks as the
+// device holds, at most nblk, a multiple of 8 so that every XCD's run of
+// items has blocks that drain it
+template <int NNB, bool VC, bool CO, bool DF, bool P>
 static void launch_class(int ipl, unsigned nblk, const ForceArgs &a, hipStream_t st)
 {
+    auto go = [&](void (*kern)(ForceArgs)) {
+        unsigned n = nblk;
+        if constexpr (P) n = std::max(8u, std::min(nblk, fit_blocks((const void *)kern)) & ~7u);
+        hipLaunchKernelGGL(kern, dim3(n), dim3(256), 0, st, a);
+    };
     if constexpr (CO) {
-        if (ipl == 1) hipLaunchKernelGGL((k_force_co<NNB, 1, VC>), dim3(nblk), dim3(256), 0, st, a);
-        else if (ipl == 2) hipLaunchKernelGGL((k_force_co<NNB, 2, VC>), dim3(nblk), dim3(256), 0, st, a);
-        else if (ipl == 3) hipLaunchKernelGGL((k_force_co<NNB, 3, VC>), dim3(nblk), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_force_co<NNB, 4, VC>), dim3(nblk), dim3(256), 0, st, a);
+        if (ipl == 1) go(k_force_co<NNB, 1, VC, P>);
+        else if (ipl == 2) go(k_force_co<NNB, 2, VC, P>);
+        else if (ipl == 3) go(k_force_co<NNB, 3, VC, P>);
+        else go(k_force_co<NNB, 4, VC, P>);
     } else {
-        if (ipl == 1) hipLaunchKernelGGL((k_force<NNB, 1, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
-        else if (ipl == 2) hipLaunchKernelGGL((k_force<NNB, 2, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
-        else if (ipl == 3) hipLaunchKernelGGL((k_force<NNB, 3, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_force<NNB, 4, VC, false, DF>), dim3(nblk), dim3(256), 0, st, a);
+        if (ipl == 1) go(k_force<NNB, 1, VC, false, DF, P>);
+        else if (ipl == 2) go(k_force<NNB, 2, VC, false, DF, P>);
+        else if (ipl == 3) go(k_force<NNB, 3, VC, false, DF, P>);
+        else go(k_force<NNB, 4, VC, false, DF, P>);
     }
+}
+
+template <int NNB, bool VC, bool CO, bool DF>
+static void launch_class(bool persistent, int ipl, unsigned nblk, const ForceArgs &a, hipStream_t st)
+{
+    if (persistent) launch_class<NNB, VC, CO, DF, true>(ipl, nblk, a, st);
+    else launch_class<NNB, VC, CO, DF, false>(ipl, nblk, a, st);
 }
 
 hipError_t launch_force(const BatchDev &b, int k, hipStream_t st, unsigned classes)
@@ -1249,20 +1319,24 @@ hipError_t launch_force(const BatchDev &b, int k, hipStream_t st, unsigned class
         if (n <= 0 || !((classes >> c) & 1u)) continue;
         a.items = b.items + g.seg_begin[c];
         a.n_items = n;
+        // (the plan gives every class whole rows of 8 x 4 items, XCD-major)
+        a.counter = b.persistent ? b.counters + (k * PLAN_CLASSES + c) * 8 : nullptr;
+        a.n_run = n / 8;
+        const bool pers = b.persistent != 0;
         const unsigned nblk = (unsigned)((n + 3) / 4);
         // the changeover and double-float near modes always use SGPR operands
         if (b.co) {
-            if (b.nnb_le) launch_class<1, false, true, false>(ipls[c], nblk, a, st);
-            else launch_class<0, false, true, false>(ipls[c], nblk, a, st);
+            if (b.nnb_le) launch_class<1, false, true, false>(pers, ipls[c], nblk, a, st);
+            else launch_class<0, false, true, false>(pers, ipls[c], nblk, a, st);
         } else if (b.df) {
-            if (b.nnb_le) launch_class<1, false, false, true>(ipls[c], nblk, a, st);
-            else launch_class<0, false, false, true>(ipls[c], nblk, a, st);
+            if (b.nnb_le) launch_class<1, false, false, true>(pers, ipls[c], nblk, a, st);
+            else launch_class<0, false, false, true>(pers, ipls[c], nblk, a, st);
         } else if (b.j_vgpr) {
-            if (b.nnb_le) launch_class<1, true, false, false>(ipls[c], nblk, a, st);
-            else launch_class<0, true, false, false>(ipls[c], nblk, a, st);
+            if (b.nnb_le) launch_class<1, true, false, false, false>(ipls[c], nblk, a, st);
+            else launch_class<0, true, false, false, false>(ipls[c], nblk, a, st);
         } else {
-            if (b.nnb_le) launch_class<1, false, false, false>(ipls[c], nblk, a, st);
-            else launch_class<0, false, false, false>(ipls[c], nblk, a, st);
+            if (b.nnb_le) launch_class<1, false, false, false>(pers, ipls[c], nblk, a, st);
+            else launch_class<0, false, false, false>(pers, ipls[c], nblk, a, st);
         }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

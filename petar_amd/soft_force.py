@@ -58,7 +58,7 @@ class Params(C.Structure):
                 ("exec_mode", C.c_int), ("chunk_j", C.c_int), ("max_walk", C.c_int), ("max_ipl", C.c_int),
                 ("j_vgpr", C.c_int), ("changeover", C.c_int), ("ngb_factor", C.c_double), ("segments", C.c_int),
                 ("streams", C.c_int), ("df_near", C.c_int), ("spj_monopole", C.c_int), ("graph", C.c_int),
-                ("min_seg_pairs", C.c_double)]
+                ("min_seg_pairs", C.c_double), ("persistent", C.c_int)]
 
 
 class Profile(C.Structure):
@@ -190,7 +190,7 @@ class PthError(RuntimeError):
 
 def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED, chunk_j=0, max_walk=1000,
                    max_ipl=None, j_vgpr=None, changeover=0, ngb_factor=None, segments=None, streams=None,
-                   df_near=None, spj_monopole=0, graph=None, min_seg_pairs=None):
+                   df_near=None, spj_monopole=0, graph=None, min_seg_pairs=None, persistent=None):
     p = Params()
     lib().pth_params_default(C.byref(p))
     p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk = (
@@ -213,6 +213,8 @@ def default_params(eps2=0.0, rcut2=0.0, G=1.0, nnb_mode=0, exec_mode=EXEC_FUSED,
         p.graph = graph
     if min_seg_pairs is not None:
         p.min_seg_pairs = min_seg_pairs
+    if persistent is not None:
+        p.persistent = persistent
     return p
 
 

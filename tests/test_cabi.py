@@ -38,7 +38,7 @@ def test_params_default():
     sf.lib().pth_params_default(C.byref(p))
     assert (p.eps2, p.rcut2, p.G, p.nnb_mode, p.exec_mode, p.chunk_j, p.max_walk, p.max_ipl, p.j_vgpr,
             p.changeover, p.ngb_factor, p.segments, p.streams) == (0.0, 0.0, 1.0, 0, 0, 0, 1000, 2, 0, 0, 0.99, 8, 3)
-    assert (p.df_near, p.spj_monopole, p.graph, p.min_seg_pairs) == (1, 0, 0, 0.0)
+    assert (p.df_near, p.spj_monopole, p.graph, p.min_seg_pairs, p.persistent) == (1, 0, 0, 0.0, 0)
 
 
 def test_create_without_device_fails_cleanly():

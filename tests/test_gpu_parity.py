@@ -567,3 +567,72 @@ def test_fdps_copies_functor(c1, exec_mode):
     _check(res, ref, amb, f"copies functor exec={exec_mode}")
     idx_res = _fdps_replay(L, sf.make_functor(0, 0.0, R_OUT ** 2, 1.0, exec_mode=exec_mode))
     assert all(np.array_equal(x, y) for x, y in zip(res, idx_res))
+
+
+def test_persistent_launches_are_bit_identical(c1):
+    """pth_params.persistent = 1: each i-per-lane class runs as one wave per
+    free wave slot, the waves of each XCD taking that XCD's items from a
+    counter.  Every item is computed as in the launch of one wave per item,
+    so the results are the same bit for bit: resident evaluations over 8
+    segments on 3 streams at several tilings (double-float near pairs, plain
+    FP32 near pairs, changeover mode), a captured graph replayed, and the
+    FDPS multi-walk replay through pth_dispatch (1-i items on the second
+    stream)."""
+    L, a, ref, amb = c1
+    rin, rout = np.full(L.epj_mass.size, 0.1 * R_OUT), np.full(L.epj_mass.size, R_OUT)
+    tiles = ((2, 1024), (1, 256), (3, 2048), (4, 512))
+    for mode in ("df", "fp32", "changeover"):
+        outs = []
+        for pers in (0, 1):
+            kw = dict(segments=8, streams=3, min_seg_pairs=1, persistent=pers)
+            if mode == "changeover":
+                ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, changeover=1, **kw)
+                r = sf.Resident.from_lists(ctx, L, rin, rout)
+            else:
+                ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, df_near=int(mode == "df"), **kw)
+                r = sf.Resident.from_lists(ctx, L)
+            got = []
+            for ipl, cj in tiles:
+                r.replan(ipl, cj)
+                r.run()
+                r.run()
+                got.append(r.results())
+            outs.append(got)
+            r.close()
+            ctx.close()
+        for (ipl, cj), x, y in zip(tiles, outs[0], outs[1]):
+            same = all(np.array_equal(u, v) for u, v in zip(x, y))
+            print(f"[persistent {mode}] ipl {ipl} chunk {cj}: identical {same}", flush=True)
+            assert same, f"{mode} ipl {ipl} chunk {cj}"
+        if mode != "changeover":
+            _check(outs[1][0], ref, amb, f"persistent {mode}")
+    # a captured graph of persistent launches, replayed
+    eager = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=8, streams=3, min_seg_pairs=1)
+    re = sf.Resident.from_lists(eager, L)
+    re.run()
+    want = re.results()
+    gctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=8, streams=3, min_seg_pairs=1, graph=1,
+                      persistent=1)
+    rg = sf.Resident.from_lists(gctx, L)
+    for k in range(3):
+        rg.run()
+        assert all(np.array_equal(x, y) for x, y in zip(rg.results(), want)), f"graph replay {k}"
+        re.run()
+        re.sync()
+    rg.close()
+    re.close()
+    gctx.close()
+    eager.close()
+    # the drop-in path
+    fn = sf.make_functor(0, 0.0, R_OUT ** 2, 1.0)
+    base = _fdps_replay(L, fn)
+    c = fn.context()
+    p = c.get_params()
+    p.persistent = 1
+    c.set_params(p)
+    try:
+        got = _fdps_replay(L, fn)
+    finally:
+        p.persistent = 0
+        c.set_params(p)
+    assert all(np.array_equal(x, y) for x, y in zip(base, got))
