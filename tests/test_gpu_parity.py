@@ -329,6 +329,45 @@ def test_error_paths():
     ctx.close()
 
 
+@pytest.mark.parametrize("list_runs", ["0", "1", None])
+def test_dispatch_rejects_bad_lists_in_every_list_form(list_runs, monkeypatch):
+    """pth_dispatch validates every index of a batch's lists whether it
+    sends them as they are (PTH_LIST_RUNS=0), always as runs (=1) or as
+    runs when that halves their bytes (unset): an index outside the sent
+    arrays, negative, or in the SPJ list, fails the call before any kernel
+    runs, and the context serves the next batch."""
+    if list_runs is None:
+        monkeypatch.delenv("PTH_LIST_RUNS", raising=False)
+    else:
+        monkeypatch.setenv("PTH_LIST_RUNS", list_runs)
+    sf.CalcForceWithLinearCutoffMultiWalk.close_contexts()
+    try:
+        fn = sf.CalcForceWithLinearCutoffMultiWalk(0, 0.0, 1e-4, 1.0)
+        epj = np.zeros(64, sf.EPJ_SOFT)
+        epj["mass"] = 1.0
+        epj["pos"][:, 0] = np.arange(64)
+        spj = np.zeros(3, sf.SPJ_SOFT)
+        spj["mass"] = 1.0
+        spj["pos"][:, 1] = 100.0 + np.arange(3)
+        fn(0, 0, [], [], [], [], [], [], epj, 64, spj, 3, True)
+        epi = [np.zeros(2, sf.EPI_SOFT), np.zeros(3, sf.EPI_SOFT)]
+        good_e = [np.arange(0, 40, dtype=np.int32), np.arange(10, 64, dtype=np.int32)]
+        good_s = [np.array([0, 2], np.int32), np.array([1], np.int32)]
+        ne, nj, ns = [2, 3], [40, 54], [2, 1]
+        for bad_e, bad_s in (([good_e[0], np.r_[good_e[1][:-1], 64].astype(np.int32)], good_s),
+                             ([np.r_[-1, good_e[0][1:]].astype(np.int32), good_e[1]], good_s),
+                             (good_e, [good_s[0], np.array([3], np.int32)])):
+            with pytest.raises(sf.PthError, match="out of range"):
+                fn(0, 2, epi, ne, bad_e, nj, bad_s, ns, epj, 64, spj, 3, False)
+        fn(0, 2, epi, ne, good_e, nj, good_s, ns, epj, 64, spj, 3, False)
+        force = [np.zeros(2, sf.FORCE_SOFT), np.zeros(3, sf.FORCE_SOFT)]
+        sf.RetrieveForce(0, 2, ne, force, fn)
+        pot = np.concatenate([f["pot"] for f in force])
+        assert np.all(np.isfinite(pot)) and np.all(pot < 0)
+    finally:
+        sf.CalcForceWithLinearCutoffMultiWalk.close_contexts()
+
+
 def test_replan_and_autotune(c1):
     """A re-planned resident evaluation equals one created with the same
     tiling parameters; the autotuner keeps one of the candidates and the
