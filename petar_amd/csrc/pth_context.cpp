@@ -1000,9 +1000,25 @@ static int resident_enqueue(pth_resident *r, hipStream_t st, hipEvent_t fork_ev,
 {
     pth_ctx *c = r->ctx;
     if (r->side.empty() || r->b.n_seg < 2 || (int)r->staged.size() < r->b.n_seg) {
+        // one segment: its 1-i items run on a side stream beside the longer
+        // classes (the 1-i launch is about one round of waves, whose run-down
+        // tail the longer launch then fills)
+        const SegDev &g0 = r->b.seg[0];
+        const bool split = r->b.n_seg == 1 && c->class_streams && !r->b.plain && !r->side.empty() &&
+                           !r->staged.empty() && g0.seg_count[0] > 0 &&
+                           g0.seg_count[1] + g0.seg_count[2] + g0.seg_count[3] > 0;
         for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_stage(r->b, k, st), "stage kernels");
         if (timed) HIPCHK(c, hipEventRecord(r->ev[1], st), "hipEventRecord");
-        for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_force(r->b, k, st), "force kernel");
+        if (split) {
+            HIPCHK(c, hipEventRecord(r->staged[0], st), "hipEventRecord");
+            HIPCHK(c, hipStreamWaitEvent(r->side[0], r->staged[0], 0), "hipStreamWaitEvent");
+            HIPCHK(c, launch_force(r->b, 0, st, 0xEu), "force kernel");
+            HIPCHK(c, launch_force(r->b, 0, r->side[0], 0x1u), "force kernel");
+            HIPCHK(c, hipEventRecord(r->side_done[0], r->side[0]), "hipEventRecord");
+            HIPCHK(c, hipStreamWaitEvent(st, r->side_done[0], 0), "hipStreamWaitEvent");
+        } else {
+            for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_force(r->b, k, st), "force kernel");
+        }
         if (timed) HIPCHK(c, hipEventRecord(r->ev[2], st), "hipEventRecord");
         for (int k = 0; k < r->b.n_seg; k++) HIPCHK(c, launch_reduce(r->b, k, st), "reduce kernel");
         r->pipelined = false;
@@ -1363,8 +1379,9 @@ int resident_setup(pth_resident *r, int n_walk, const int *n_epi, const int *n_e
     }
     for (int k = 0; k < 4; k++)
         if (!r->ev[k] && (e = hipEventCreate(&r->ev[k]))) return bail(e, "hipEventCreate");
-    if ((int)P.segs.size() > 1) {
-        const int ns = std::min(c->par.streams, (int)P.segs.size());
+    {
+        // (one side stream also for a single segment: its 1-i class)
+        const int ns = std::max(1, std::min(c->par.streams, (int)P.segs.size()));
         // side streams are only ever added (a later plan may use fewer)
         while ((int)r->side.size() < ns) {
             r->side.push_back(nullptr);

@@ -675,3 +675,32 @@ def test_persistent_launches_are_bit_identical(c1):
         p.persistent = 0
         c.set_params(p)
     assert all(np.array_equal(x, y) for x, y in zip(base, got))
+
+
+def test_single_segment_class_streams_are_bit_identical(c1, monkeypatch):
+    """A resident evaluation of one segment runs its 1-i items on a side
+    stream beside the longer classes (PTH_CLASS_STREAMS=0: one stream), in
+    launched and in graph mode: the same results bit for bit, at several
+    tilings."""
+    L, a, ref, amb = c1
+    outs = []
+    for env, graph in (("0", 0), ("1", 0), ("1", 1)):
+        monkeypatch.setenv("PTH_CLASS_STREAMS", env)
+        ctx = sf.Context(0, eps2=0.0, rcut2=R_OUT ** 2, G=1.0, segments=1, graph=graph)
+        r = sf.Resident.from_lists(ctx, L)
+        got = []
+        for ipl, cj in ((2, 1024), (1, 256), (3, 2048)):
+            r.replan(ipl, cj)
+            r.run()
+            r.run()
+            got.append(r.results())
+        ms = r.timing()
+        print(f"[class streams {env} graph {graph}] force ms {ms[1]:.3f}", flush=True)
+        outs.append(got)
+        r.close()
+        ctx.close()
+    monkeypatch.delenv("PTH_CLASS_STREAMS", raising=False)
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert all(np.array_equal(u, v) for u, v in zip(x, y))
+    _check(outs[1][0], ref, amb, "single segment, class streams")
