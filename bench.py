@@ -292,6 +292,22 @@ def read_traffic(name):
     return None
 
 
+def read_list_gather(name):
+    """HBM bytes per launch of the list-gather (staging) kernel, from the
+    same committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", f"pmc_force_{name}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("list_gather_bytes_per_launch:"):
+                try:
+                    return float(line.split(":", 1)[1].split()[0])
+                except ValueError:
+                    return None
+    return None
+
+
 def host_cpu():
     """Model, logical CPUs and SMT of the host (from /proc/cpuinfo)."""
     model, siblings, cores = "unknown", None, None
@@ -609,6 +625,18 @@ def main():
     unseg_ms = float(np.mean(np.array(k_stage) + np.array(k_force) + np.array(k_red)))
     achieved = flops / (force_ms * 1e-3) / 1e12
     traffic = read_traffic(a.workload)
+    # the list-gather phase (k_stage_walk: every walk's EPJ/SPJ lists gathered
+    # into walk-relative records) against HBM: its bytes per launch from the
+    # committed pmc summary over its time in the unsegmented evaluation above
+    stage_ms = float(np.mean(k_stage))
+    lg_bytes = read_list_gather(a.workload)
+    list_gather = None
+    if lg_bytes is not None and stage_ms > 0:
+        gbs = lg_bytes / (stage_ms * 1e-3) / 1e9
+        list_gather = {"kernel": "k_stage_walk", "ms": round(stage_ms, 4), "hbm_bytes": lg_bytes,
+                       "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(gbs / HBM_PEAK_GBS, 4),
+                       "source": f"profiles/pmc_force_{a.workload}.txt (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"}
 
     out = {
         "metric": "pair interactions/s (Gint/s, % FP32 peak) + p50/p99 latency at fixed force error",
@@ -631,6 +659,7 @@ def main():
                      "kernel": "k_force (FP32 vector ALU, no MFMA); all class launches of one evaluation, "
                                "unsegmented, HIP events on their stream",
                      "kernel_ms": round(force_ms, 4)},
+        "list_gather": list_gather,
         "pct_fp32_peak_step": round(flops / (ms_step * 1e-3) / 1e12 / P32_PEAK_TFLOPS * 100, 2),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "stage_ms": round(float(np.mean(k_stage)), 4), "reduce_ms": round(float(np.mean(k_red)), 4),
