@@ -470,17 +470,25 @@ def accuracy(res_acc, res_pot, res_nnb, L, r_out, frac=0.05, seed=0):
                 nnb_pairs_in_fp32_band=int(amb.sum()), nnb_i_with_band_pairs=int((amb > 0).sum()))
 
 
-def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0):
+def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0, autotune=True):
     """The same lists in changeover mode (the (1 - K) weight and the
     pot_soft / pot_tot rules fused into the EPJ loop): ms per evaluation,
     Gint/s, and the error against the FP64 changeover oracle on a sample of
     walks.  Radii r_in = 0.1 r_out, r_out as the linear cutoff (the lists'
-    r_search is 1.5 r_out); ids are the tree positions."""
+    r_search is 1.5 r_out); ids are the tree positions.  Tiles autotuned for
+    this mode (its kernels hold more state per i than the linear-cutoff
+    ones), starting from the linear-cutoff evaluation's."""
     from oracle import oracle_py as O
     from petar_amd import soft_force as sf
     rin, rout = np.full(L.epj_mass.size, 0.1 * r_out), np.full(L.epj_mass.size, r_out)
     ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, changeover=1, max_ipl=tiles[0], chunk_j=tiles[1])
     res = sf.Resident.from_lists(ctx, L, rin, rout)
+    tuned = None
+    if autotune:
+        t0 = time.time()
+        ipl, cj, tms = res.autotune(ipl=(1, 2, 3, 4), chunk=(512, 1024, 2048), reps=3)
+        tiles = (ipl, cj)
+        tuned = dict(max_ipl=ipl, chunk_j=cj, ms=round(tms, 4), tuning_s=round(time.time() - t0, 2))
     for _ in range(max(warmup, 1)):
         res.run()
     res.sync()
@@ -520,6 +528,7 @@ def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0):
     et = np.abs(pt[rows] - rpt) / np.abs(rpt)
     ne, ns = L.interactions()
     out = dict(ms_per_step=round(ms, 4), gint_s=round((ne + ns) / (ms * 1e-3) / 1e9, 2),
+               tiles=dict(max_ipl=tiles[0], chunk_j=tiles[1]), autotuned=tuned,
                unsegmented_ms=dict(stage=round(float(ph[0]), 4), force=round(float(ph[1]), 4),
                                    reduce=round(float(ph[2]), 4), step=round(float(ph.sum()), 4)),
                sample_i=int(rows.size),
@@ -706,7 +715,8 @@ def main():
     res.close()
     if not a.no_changeover and a.exec == "fused":
         # same warm-up and step count as the timed evaluation above
-        out["changeover_mode"] = changeover_mode(L, r_out, (ipl_used, cj_used), steps=a.steps, warmup=a.warmup)
+        out["changeover_mode"] = changeover_mode(L, r_out, (ipl_used, cj_used), steps=a.steps, warmup=a.warmup,
+                                                 autotune=a.exec == "fused" and not a.no_autotune)
     if not a.no_device_tree and a.exec == "fused":
         # same tiles and segments as the timed evaluation, so the forces can
         # be compared bit for bit
