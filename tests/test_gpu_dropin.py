@@ -184,3 +184,30 @@ def test_cpp_dropin_list_runs_bit_identical(tmp_path, exe):
         for k, name in enumerate(("acc", "pot", "nnb", "pot_tot")):
             for o in outs[1:]:
                 assert np.array_equal(outs[0][k], o[k]), f"{exe} walk_limit={walk_limit}: {name} differs"
+
+
+def test_cpp_dropin_long_run_lists_bit_identical(tmp_path):
+    """At 400,000 particles some walks' SPJ lists hold more than 1,024 runs
+    of consecutive indices, so their expansion on the device takes several
+    chunks of runs through LDS: lists sent as runs (PTH_LIST_RUNS=1) give
+    the same forces bit for bit as lists sent as they are."""
+    path = os.path.join(ROOT, "tests", "cpp", "fdps_replay")
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (build() compiles it)")
+    outs = []
+    for mode in ("0", "1"):
+        out = tmp_path / f"replay_long_{mode}.bin"
+        env = dict(os.environ, PTH_LIST_RUNS=mode)
+        print(f"[dropin long runs] PTH_LIST_RUNS={mode}", flush=True)
+        p = subprocess.run([path, "400000", "11", "0.3", "200", "0.01", str(out)], capture_output=True, text=True,
+                           timeout=240, env=env)
+        print(p.stdout, p.stderr, flush=True)
+        assert p.returncode == 0
+        d, res = _read(out)
+        outs.append(res)
+    eoff = np.concatenate([[0], np.cumsum(d["n_spj"])])
+    runs = [1 + int(np.count_nonzero(np.diff(d["id_spj"][eoff[w]:eoff[w + 1]]) != 1))
+            for w in range(d["n_spj"].size) if d["n_spj"][w] > 0]
+    assert max(runs) > 1024
+    for k, name in enumerate(("acc", "pot", "nnb")):
+        assert np.array_equal(outs[0][k], outs[1][k]), f"{name} differs"
