@@ -665,8 +665,9 @@ def main():
                    "flops_per_step": flops, "flop_convention": "26/EPJ pair + 65/SPJ pair"},
         "roofline": {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": P32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / P32_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "k_force (FP32 vector ALU, no MFMA); all class launches of one evaluation, "
-                               "unsegmented, HIP events on their stream",
+                     "kernel": "k_force (FP32 vector ALU, no MFMA); all class launches of one evaluation "
+                               "(the 1-i class on a second stream beside the others), unsegmented; HIP events "
+                               "before the first and after the last launch, on the evaluation's stream",
                      "kernel_ms": round(force_ms, 4)},
         "list_gather": list_gather,
         "pct_fp32_peak_step": round(flops / (ms_step * 1e-3) / 1e12 / P32_PEAK_TFLOPS * 100, 2),
