@@ -1038,7 +1038,19 @@ static int resident_enqueue(pth_resident *r, hipStream_t st, hipEvent_t fork_ev,
     for (int k = 0; k < r->b.n_seg; k++) {
         hipStream_t fs = r->side[ns > 1 ? 1 + k % (ns - 1) : 0];
         HIPCHK(c, hipStreamWaitEvent(fs, r->staged[k], 0), "hipStreamWaitEvent");
-        HIPCHK(c, launch_force(r->b, k, fs), "force kernel");
+        // the last segment's 1-i launch on the staging stream (idle by then)
+        // beside its longer classes: nothing else is left to fill its tail
+        const SegDev &g = r->b.seg[k];
+        const bool split = k == r->b.n_seg - 1 && ns > 1 && c->class_streams && !r->b.plain &&
+                           g.seg_count[0] > 0 && g.seg_count[1] + g.seg_count[2] + g.seg_count[3] > 0;
+        if (split) {
+            HIPCHK(c, launch_force(r->b, k, fs, 0xEu), "force kernel");
+            HIPCHK(c, launch_force(r->b, k, r->side[0], 0x1u), "force kernel");
+            HIPCHK(c, hipEventRecord(r->staged[k], r->side[0]), "hipEventRecord");
+            HIPCHK(c, hipStreamWaitEvent(fs, r->staged[k], 0), "hipStreamWaitEvent");
+        } else {
+            HIPCHK(c, launch_force(r->b, k, fs), "force kernel");
+        }
         HIPCHK(c, launch_reduce(r->b, k, fs), "reduce kernel");
     }
     for (int s = 0; s < ns; s++) {
