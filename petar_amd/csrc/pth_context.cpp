@@ -111,6 +111,8 @@ int pth_create(pth_ctx **out, int rank, const pth_params *params)
         c->list_runs = (lr && lr[0] == '0') ? 0 : (lr && lr[0] == '1') ? 1 : -1;
         const char *cs = std::getenv("PTH_CLASS_STREAMS");
         c->class_streams = !(cs && cs[0] == '0');
+        const char *zc = std::getenv("PTH_ZERO_COPY");
+        c->zero_copy = !(zc && zc[0] == '0');
     }
     c->device = (rank >= 0 ? rank : -rank) % ndev;  // src/force_gpu_cuda.cu:550-553
     if ((e = hipSetDevice(c->device)) != hipSuccess ||
@@ -582,9 +584,21 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         if (runs) HIPCHK(c, c->d_ids.ensure(sizeof(int) * (size_t)std::max<long long>(n_ide + n_ids, 1)),
                          "device list allocation");
     }
-    HIPCHK(c, T.out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "output allocation");
-    HIPCHK(c, T.out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "output allocation");
-    if (co) HIPCHK(c, T.out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "output allocation");
+    // the tag's pinned result buffers (nothing of this tag is in flight);
+    // with zero_copy the kernels write the results straight into them (host
+    // memory the device addresses), else into device buffers then copied
+    HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
+    HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");
+    if (co) {
+        HIPCHK(c, T.h_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "pinned output");
+        HIPCHK(c, T.h_err.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)), "pinned output");
+    }
+    const bool zc = c->zero_copy;
+    if (!zc) {
+        HIPCHK(c, T.out_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "output allocation");
+        HIPCHK(c, T.out_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "output allocation");
+        if (co) HIPCHK(c, T.out_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "output allocation");
+    }
 
     t_pack = wall_s();
     char *h = c->h_arena.as<char>();
@@ -654,8 +668,9 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         c->tab_eps2 = (float)c->par.eps2;
     }
     fill_batch(b, P, c->par, c->d_arena, A, c->work, c->d_epj.as<EpjDev>(), c->d_spj.as<SpjDev>(),
-               T.out_accp.as<double4>(), T.out_nnb.as<int>(), c->d_epj_co.as<EpjCoDev>(),
-               T.out_pott.as<double>(), c->d_spj_tab.as<SpjTab>());
+               zc ? T.h_accp.as<double4>() : T.out_accp.as<double4>(), zc ? T.h_nnb.as<int>() : T.out_nnb.as<int>(),
+               c->d_epj_co.as<EpjCoDev>(), zc ? T.h_pott.as<double>() : T.out_pott.as<double>(),
+               c->d_spj_tab.as<SpjTab>());
     if (runs) {
         char *d = c->d_arena.as<char>();
         int *ide_dev = c->d_ids.as<int>();
@@ -690,13 +705,7 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
         HIPCHK(c, launch_all(b, c->stream), "soft-force kernels");
     }
     if (T.timed) HIPCHK(c, hipEventRecord(T.prof[2], c->stream), "hipEventRecord");
-    HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
-    HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");
-    if (co) {
-        HIPCHK(c, T.h_pott.ensure(sizeof(double) * std::max<long long>(P.ni_tot, 1)), "pinned output");
-        HIPCHK(c, T.h_err.ensure(sizeof(int) * std::max<size_t>(P.walks.size(), 1)), "pinned output");
-    }
-    if (P.ni_tot > 0) {
+    if (!zc && P.ni_tot > 0) {
         HIPCHK(c, hipMemcpyAsync(T.h_accp.p, T.out_accp.p, sizeof(double4) * P.ni_tot, hipMemcpyDeviceToHost,
                                  c->stream), "result download");
         HIPCHK(c, hipMemcpyAsync(T.h_nnb.p, T.out_nnb.p, sizeof(int) * P.ni_tot, hipMemcpyDeviceToHost,

@@ -403,6 +403,9 @@ struct pth_ctx {
     // pth_dispatch runs the 1-i items on a second stream beside the longer
     // classes (PTH_CLASS_STREAMS=0: all on one stream)
     bool class_streams = true;
+    // pth_dispatch's kernels write the results straight into the tag's pinned
+    // host buffers (PTH_ZERO_COPY=0: device buffers and a copy)
+    bool zero_copy = true;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_upload = nullptr;

@@ -162,20 +162,22 @@ def test_cpp_dropin_list_runs_bit_identical(tmp_path, exe):
     """pth_dispatch sends each batch's index lists either as they are or as
     runs of consecutive indices expanded on the device (the default when
     that at least halves their bytes; PTH_LIST_RUNS=0 / 1 forces one form),
-    and runs the 1-i items on a second stream beside the longer classes
-    (PTH_CLASS_STREAMS=0: one stream): every combination gives the same
-    forces bit for bit, in the linear-cutoff and the changeover adapter,
-    with batches of 200 walks and of 7."""
+    runs the 1-i items on a second stream beside the longer classes
+    (PTH_CLASS_STREAMS=0: one stream), and has the kernels write the
+    results straight into pinned host memory (PTH_ZERO_COPY=0: device
+    buffers and a copy): every combination gives the same forces bit for
+    bit, in the linear-cutoff and the changeover adapter, with batches of
+    200 walks and of 7."""
     path = os.path.join(ROOT, "tests", "cpp", exe)
     if not os.path.exists(path):
         pytest.fail(f"{path} not built (build() compiles it)")
     for walk_limit in (200, 7):
         outs = []
-        for mode, streams in (("0", "1"), ("1", "1"), ("1", "0")):
-            out = tmp_path / f"replay_{mode}_{streams}_{walk_limit}.bin"
-            env = dict(os.environ, PTH_LIST_RUNS=mode, PTH_CLASS_STREAMS=streams)
-            print(f"[dropin runs] {exe} PTH_LIST_RUNS={mode} PTH_CLASS_STREAMS={streams} walk_limit={walk_limit}",
-                  flush=True)
+        for mode, streams, zc in (("0", "1", "1"), ("1", "1", "1"), ("1", "0", "0")):
+            out = tmp_path / f"replay_{mode}_{streams}_{zc}_{walk_limit}.bin"
+            env = dict(os.environ, PTH_LIST_RUNS=mode, PTH_CLASS_STREAMS=streams, PTH_ZERO_COPY=zc)
+            print(f"[dropin runs] {exe} PTH_LIST_RUNS={mode} PTH_CLASS_STREAMS={streams} PTH_ZERO_COPY={zc} "
+                  f"walk_limit={walk_limit}", flush=True)
             p = subprocess.run([path, "12000", "17", "0.3", str(walk_limit), "0.01", str(out)], capture_output=True,
                                text=True, timeout=240, env=env)
             print(p.stdout, p.stderr, flush=True)
