@@ -586,7 +586,9 @@ int pth_dispatch(pth_ctx *c, int tag, int n_walk, const void *const *epi, const 
     }
     // the tag's pinned result buffers (nothing of this tag is in flight);
     // with zero_copy the kernels write the results straight into them (host
-    // memory the device addresses), else into device buffers then copied
+    // memory the device addresses; the writes are visible to the host once
+    // T.done, recorded after the kernels without hipEventDisableSystemFence,
+    // has completed), else into device buffers then copied
     HIPCHK(c, T.h_accp.ensure(sizeof(double4) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     HIPCHK(c, T.h_nnb.ensure(sizeof(int) * std::max<long long>(P.ni_tot, 1)), "pinned output");
     if (co) {
