@@ -704,3 +704,51 @@ def test_single_segment_class_streams_are_bit_identical(c1, monkeypatch):
         for x, y in zip(outs[0], o):
             assert all(np.array_equal(u, v) for u, v in zip(x, y))
     _check(outs[1][0], ref, amb, "single segment, class streams")
+
+
+def test_two_tags_in_flight_are_bit_identical(c1):
+    """Batches dispatched on tags 0 and 1 before either is retrieved, then
+    retrieved in the other order (each tag has its own result buffers, which
+    the kernels write directly, and the batches share the context's stream
+    and its second stream): the same forces bit for bit as the one-tag
+    replay of the same batches."""
+    L, a, ref, amb = c1
+    fn = sf.make_functor(0, 0.0, R_OUT ** 2, 1.0)
+    want = _fdps_replay(L, fn)
+    epj, spj = _epj_records(L), _spj_records(L)
+    n, ns = epj.size, spj.size
+    eoff, soff = L.walk_offsets()
+    fn(0, 0, [], [], [], [], [], [], epj, n, spj, ns, True)
+
+    def batch(ws):
+        epi, force = [], []
+        for w in ws:
+            e = np.zeros(L.n_epi[w], sf.EPI_SOFT)
+            sl = slice(L.epi_begin[w], L.epi_begin[w] + L.n_epi[w])
+            e["pos"] = L.epj_pos[sl]
+            e["r_search"] = L.epj_rs[sl]
+            e["type"] = 1
+            epi.append(e)
+            force.append(np.zeros(L.n_epi[w], sf.FORCE_SOFT))
+        ide = [L.id_epj[eoff[w]:eoff[w + 1]] for w in ws]
+        ids = [L.id_spj[soff[w]:soff[w + 1]] for w in ws]
+        return epi, force, ide, ids
+
+    out = [None] * L.n_walk
+    for b0 in range(0, L.n_walk, 400):
+        wa = list(range(b0, min(b0 + 200, L.n_walk)))
+        wb = list(range(b0 + 200, min(b0 + 400, L.n_walk)))
+        ea, fa, ia, sa = batch(wa)
+        fn(0, len(wa), ea, L.n_epi[wa], ia, L.n_epj[wa], sa, L.n_spj[wa], epj, n, spj, ns, False)
+        if wb:
+            eb, fb, ib, sb = batch(wb)
+            fn(1, len(wb), eb, L.n_epi[wb], ib, L.n_epj[wb], sb, L.n_spj[wb], epj, n, spj, ns, False)
+            sf.RetrieveForce(1, len(wb), L.n_epi[wb], fb, fn)
+            for w, f in zip(wb, fb):
+                out[w] = f
+        sf.RetrieveForce(0, len(wa), L.n_epi[wa], fa, fn)
+        for w, f in zip(wa, fa):
+            out[w] = f
+    f = np.concatenate(out)
+    got = (f["acc"], f["pot"], f["n_ngb"])
+    assert all(np.array_equal(x, y) for x, y in zip(want, got))
