@@ -495,8 +495,14 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs a)
                     ms = 1;
                 }
             }
-            const long long pe = wave_scan(me, lane), pc = wave_scan(mc, lane), ps = wave_scan(ms, lane);
-            const long long te = __shfl(pe, 63, 64), tc = __shfl(pc, 63, 64), ts = __shfl(ps, 63, 64);
+            // the three counts scanned as one word: EPJ entries in bits 0-31
+            // (a step's popped leaves are distinct, so their total is at most
+            // n < 2^31), children in 32-47 (at most 8 per lane), SPJ entries
+            // in 48-63 (at most 1 per lane)
+            const long long pk = wave_scan(me | (mc << 32) | (ms << 48), lane);
+            const long long tk = __shfl(pk, 63, 64);
+            const long long pe = pk & 0xffffffffLL, pc = (pk >> 32) & 0xffff, ps = (pk >> 48) & 0xffff;
+            const long long te = tk & 0xffffffffLL, tc = (tk >> 32) & 0xffff, ts = (tk >> 48) & 0xffff;
             if (WRITE) {
                 for (long long t = 0; t < me; t++) {
                     const long long k = ne + pe - me + t;
