@@ -32,8 +32,8 @@ void pth_params_default(pth_params *p)
     p->j_vgpr = 0;   // SGPR operands beat VGPR copies at every max_ipl
     p->changeover = 0;
     p->ngb_factor = 0.99;
-    p->segments = 8;  // measured on the precision workload: 8 segments, 1 staging + 2 force streams
-    p->streams = 3;
+    p->segments = 8;  // 1 staging + 2 force streams; 16 on 4 streams is faster with GPU_MAX_HW_QUEUES=8, but
+    p->streams = 3;   // with HIP's default 4 queues a process's first such pipeline ran 4% slower
     p->df_near = 1;
     p->spj_monopole = 0;
     p->graph = 0;

@@ -421,7 +421,7 @@ def test_theta0_is_direct_summation():
     assert bad_exact == 0 and bad_band == 0
 
 
-@pytest.mark.parametrize("segments,streams", [(1, 1), (4, 2), (8, 4), (32, 8)])
+@pytest.mark.parametrize("segments,streams", [(1, 1), (4, 2), (8, 4), (16, 4), (32, 8)])
 def test_segments_are_bit_identical(c1, segments, streams):
     """Walk segments run concurrently on several streams; every walk keeps
     its items, so the results equal the one-stream evaluation bit for bit
@@ -502,7 +502,7 @@ def test_double_float_near_pairs():
     assert err[1].max() < err[0].max()
 
 
-@pytest.mark.parametrize("segments,streams", [(8, 3), (1, 1)])
+@pytest.mark.parametrize("segments,streams", [(8, 3), (16, 4), (1, 1)])
 def test_graph_replay_equals_launches(c1, segments, streams):
     """pth_params.graph = 1: the evaluation's launches (all segments and
     streams) captured once as a hipGraph and replayed.  Every replay, with
