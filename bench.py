@@ -32,6 +32,13 @@ P32_PEAK_TFLOPS = 157.3   # MI355X FP32 peak (vector = matrix)
 HBM_PEAK_GBS = 8000.0
 FLOP_EPJ, FLOP_SPJ = 26, 65
 
+# walk segments, pipeline streams and GPU_MAX_HW_QUEUES per workload (0:
+# HIP's default): the large evaluations gain from a third force stream given
+# queues enough for it (8.4M: 89.0 vs 89.9 ms per step, 4.2M: 37.9 vs 38.9);
+# the 1M one runs best on the library's defaults (with 8 queues 10.86-11.18
+# ms, with 4 10.84-10.90 ms, either setting)
+PIPELINE = {"c3": (16, 4, 8), "c5": (16, 4, 8)}
+
 WORKLOADS = {
     # name: (description, generator kwargs, r_out)
     "c1": ("Plummer N=16384, Kroupa IMF, no binaries, theta=0.3, quad SPJ, r_out=0.01",
@@ -555,10 +562,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--segments", type=int, default=16)
-    ap.add_argument("--streams", type=int, default=4)
-    ap.add_argument("--hw-queues", type=int, default=8,
-                    help="GPU_MAX_HW_QUEUES for this process (0: as the environment has it)")
+    ap.add_argument("--segments", type=int, default=0, help="walk segments (0: the workload's setting)")
+    ap.add_argument("--streams", type=int, default=0, help="pipeline streams (0: the workload's setting)")
+    ap.add_argument("--hw-queues", type=int, default=-1,
+                    help="GPU_MAX_HW_QUEUES for this process (0: as the environment has it; -1: the workload's "
+                         "setting)")
     ap.add_argument("--direct-sample", type=int, default=65536,
                     help="i sampled for the error against the FP64 all-pairs direct sum (0: skip)")
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -574,12 +582,17 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the soft-force path is "
               f"single-GPU here and multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
-    # hardware queues of the process, set before the first HIP call: the
-    # pipeline's 1 staging + 3 force streams, the context's stream and the
+    # hardware queues of the process, set before the first HIP call: a
+    # 4-stream pipeline (1 staging + 3 force), the context's stream and the
     # null stream are 6 streams, and beyond HIP's default of 4 queues streams
     # share a queue and run in submission order with its other streams
     # (measured: a process's first 16-segment, 4-stream pipeline at 93.8 ms
-    # per step with 4 queues, 89.0 ms with 8)
+    # per 8.4M step with 4 queues, 89.0 ms with 8)
+    pipe = PIPELINE.get(a.workload, (8, 3, 0))
+    a.segments = a.segments or pipe[0]
+    a.streams = a.streams or pipe[1]
+    if a.hw_queues < 0:
+        a.hw_queues = pipe[2]
     if a.hw_queues > 0:
         os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
 
