@@ -477,18 +477,21 @@ def accuracy(res_acc, res_pot, res_nnb, L, r_out, frac=0.05, seed=0):
                 nnb_pairs_in_fp32_band=int(amb.sum()), nnb_i_with_band_pairs=int((amb > 0).sum()))
 
 
-def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0, autotune=True):
+def changeover_mode(L, r_out, tiles, steps=50, warmup=5, frac=0.02, seed=0, autotune=True, pipeline=(8, 3)):
     """The same lists in changeover mode (the (1 - K) weight and the
     pot_soft / pot_tot rules fused into the EPJ loop): ms per evaluation,
     Gint/s, and the error against the FP64 changeover oracle on a sample of
     walks.  Radii r_in = 0.1 r_out, r_out as the linear cutoff (the lists'
     r_search is 1.5 r_out); ids are the tree positions.  Tiles autotuned for
     this mode (its kernels hold more state per i than the linear-cutoff
-    ones), starting from the linear-cutoff evaluation's."""
+    ones), starting from the linear-cutoff evaluation's; walk segments and
+    streams as the linear-cutoff evaluation's (8.4M with 8 queues: 94.7-94.9
+    ms on 16 segments and 4 streams, 96.3 ms on 8 and 3)."""
     from oracle import oracle_py as O
     from petar_amd import soft_force as sf
     rin, rout = np.full(L.epj_mass.size, 0.1 * r_out), np.full(L.epj_mass.size, r_out)
-    ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, changeover=1, max_ipl=tiles[0], chunk_j=tiles[1])
+    ctx = sf.Context(0, eps2=0.0, rcut2=r_out ** 2, G=1.0, changeover=1, max_ipl=tiles[0], chunk_j=tiles[1],
+                     segments=pipeline[0], streams=pipeline[1])
     res = sf.Resident.from_lists(ctx, L, rin, rout)
     tuned = None
     if autotune:
@@ -741,7 +744,8 @@ def main():
     if not a.no_changeover and a.exec == "fused":
         # same warm-up and step count as the timed evaluation above
         out["changeover_mode"] = changeover_mode(L, r_out, (ipl_used, cj_used), steps=a.steps, warmup=a.warmup,
-                                                 autotune=a.exec == "fused" and not a.no_autotune)
+                                                 autotune=a.exec == "fused" and not a.no_autotune,
+                                                 pipeline=(a.segments, a.streams))
     if not a.no_device_tree and a.exec == "fused":
         # same tiles and segments as the timed evaluation, so the forces can
         # be compared bit for bit
